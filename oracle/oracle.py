@@ -1,0 +1,352 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker of the HIP path. It restates the
+reference's shaders (src/renderer/shaders/*.wgsl) and scene code (src/scene)
+on the CPU; nothing in the product library calls it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+def build() -> None:
+    """Compiles liboracle.so with its Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+class CameraUniform(C.Structure):
+    _fields_ = [("view_inverse", C.c_float * 16), ("proj_inverse", C.c_float * 16),
+                ("view", C.c_float * 16), ("proj", C.c_float * 16),
+                ("frame_count", C.c_uint32), ("pad", C.c_uint32 * 3)]
+
+
+class Config(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("max_depth", C.c_uint32),
+                ("photon_max_depth", C.c_uint32), ("photons_per_pass", C.c_uint32),
+                ("hash_size", C.c_uint32), ("cell_size", C.c_float),
+                ("gather_radius", C.c_float), ("gather_radius_exponent", C.c_float),
+                ("spectral_radius", C.c_float), ("gather_cap", C.c_uint32),
+                ("rr_after", C.c_uint32), ("enable_photons", C.c_uint32),
+                ("photon_norm", C.c_uint32), ("guiding", C.c_uint32),
+                ("guide_mix", C.c_float), ("guide_lr", C.c_float),
+                ("guide_train_samples", C.c_uint32), ("env_map", C.c_uint32),
+                ("reserved", C.c_uint32 * 6)]
+
+
+def default_config(**kw) -> Config:
+    """The reference's compile-time constants (renderer/mod.rs:18-19,
+    compute_pass.rs:26, build_grid.wgsl:23, shader.wgsl:274-342,696)."""
+    c = Config(C.sizeof(Config), 12, 5, 1 << 20, 1 << 22, 0.02, 0.02, -0.15, 20.0, 64, 3, 1,
+               1 << 20, 0, 0.5, 1e-3, 1 << 18, 0)
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+PHOTON_DTYPE = np.dtype([("position", np.float32, 3), ("wavelength", np.float32),
+                         ("direction", np.float32, 3), ("energy", np.float32)])
+VERTEX_DTYPE = np.dtype([("pos", np.float32, 4), ("normal", np.float32, 4)])
+MATERIAL_DTYPE = np.dtype([("color", np.float32, 4), ("extra", np.float32, 4)])
+LIGHT_DTYPE = np.dtype([("v0", np.float32, 4), ("v1", np.float32, 4), ("v2", np.float32, 4),
+                        ("params", np.float32, 4)])
+
+
+def _load():
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = C.CDLL(_LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "or_pcg_hash": (C.c_uint32, [C.c_uint32]),
+        "or_rand_stream": (None, [C.c_uint32, C.c_uint32, f32p]),
+        "or_cie_xyz_f": (None, [C.c_float, f32p]),
+        "or_rgb_to_spectrum_f": (C.c_float, [f32p, C.c_float]),
+        "or_blackbody_f": (C.c_float, [C.c_float, C.c_float]),
+        "or_thin_film_f": (C.c_float, [C.c_float] * 6),
+        "or_schlick_f": (C.c_float, [C.c_float, C.c_float]),
+        "or_cauchy_constants": (None, [C.POINTER(C.c_float)] * 3),
+        "or_cauchy_sample_f": (C.c_float, [C.c_float]),
+        "or_grid_jitter_f": (None, [C.c_uint32, C.c_float, f32p]),
+        "or_hash_position_f": (C.c_uint32, [f32p, C.c_uint32, C.c_float, C.c_uint32]),
+        "or_hero_wavelengths": (None, [C.c_float, C.c_float, f32p]),
+        "or_geometry_plane": (C.c_int, [vp, vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "or_geometry_cube": (C.c_int, [vp, vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "or_geometry_sphere": (C.c_int, [C.c_uint32, vp, vp, C.POINTER(C.c_uint32),
+                                         C.POINTER(C.c_uint32)]),
+        "or_mat4_identity": (None, [f32p]),
+        "or_mat4_mul": (None, [f32p, f32p, f32p]),
+        "or_mat4_translation": (None, [C.c_float] * 3 + [f32p]),
+        "or_mat4_scale": (None, [C.c_float] * 3 + [f32p]),
+        "or_mat4_rotation_x": (None, [C.c_float, f32p]),
+        "or_mat4_rotation_y": (None, [C.c_float, f32p]),
+        "or_mat4_rotation_z": (None, [C.c_float, f32p]),
+        "or_camera_uniform": (None, [f32p, C.c_float, C.c_float, C.c_float, C.c_uint32,
+                                     C.POINTER(CameraUniform)]),
+        "or_scene_create": (vp, []),
+        "or_scene_destroy": (None, [vp]),
+        "or_scene_add_material": (C.c_uint32, [vp, vp]),
+        "or_scene_add_mesh": (C.c_uint32, [vp, vp, C.c_uint32, u32p, C.c_uint32]),
+        "or_scene_add_instance": (None, [vp, C.c_uint32, C.c_uint32, f32p]),
+        "or_scene_add_light_instance": (None, [vp, C.c_uint32, C.c_uint32, f32p, f32p]),
+        "or_scene_add_point_light": (None, [vp, f32p, f32p]),
+        "or_scene_build": (C.c_int, [vp]),
+        "or_scene_num_triangles": (C.c_uint32, [vp]),
+        "or_scene_num_lights": (C.c_uint32, [vp]),
+        "or_scene_get_lights": (None, [vp, vp]),
+        "or_scene_get_world": (None, [vp, vp, vp]),
+        "or_scene_cornell_box": (None, [vp]),
+        "or_closest_hit": (None, [vp, f32p, C.c_uint32, C.c_int, u32p, u32p]),
+        "or_any_hit": (None, [vp, f32p, C.c_uint32, C.c_int, u32p]),
+        "or_bvh8_closest": (None, [vp, C.c_uint32, vp, f32p, C.c_uint32, vp, vp, vp, vp]),
+        "or_morton_codes": (None, [vp, u32p]),
+        "or_photon_trace": (C.c_uint32, [vp, C.POINTER(Config), C.c_uint32, C.c_uint32,
+                                         C.c_uint32, vp]),
+        "or_grid_counting": (C.c_uint32, [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32,
+                                          u32p, u32p, u32p, u32p]),
+        "or_grid_linked": (None, [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, u32p,
+                                  u32p]),
+        "or_frame_create": (vp, [vp, C.POINTER(Config), C.POINTER(CameraUniform), C.c_uint32,
+                                 C.c_uint32]),
+        "or_frame_create_with_photons": (vp, [vp, C.POINTER(Config), C.POINTER(CameraUniform),
+                                              C.c_uint32, C.c_uint32, vp, C.c_uint32]),
+        "or_frame_destroy": (None, [vp]),
+        "or_frame_photons_stored": (C.c_uint32, [vp]),
+        "or_frame_trace_pixels": (None, [vp, u32p, C.c_uint32, vp, vp, vp]),
+        "or_frame_render": (None, [vp, f32p, vp]),
+        "or_frame_gather": (None, [vp, f32p, f32p, f32p, f32p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def pcg_hash(x: int) -> int:
+    return lib.or_pcg_hash(x)
+
+
+def rand_stream(seed_input: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.float32)
+    lib.or_rand_stream(seed_input, n, out)
+    return out
+
+
+def geometry(kind: str, subdiv: int = 0):
+    nv, ni = C.c_uint32(), C.c_uint32()
+    fn = {"plane": lib.or_geometry_plane, "cube": lib.or_geometry_cube}.get(kind)
+    if kind == "sphere":
+        lib.or_geometry_sphere(subdiv, None, None, C.byref(nv), C.byref(ni))
+    else:
+        fn(None, None, C.byref(nv), C.byref(ni))
+    v = np.zeros(nv.value, VERTEX_DTYPE)
+    i = np.zeros(ni.value, np.uint32)
+    if kind == "sphere":
+        rc = lib.or_geometry_sphere(subdiv, _ptr(v), _ptr(i), C.byref(nv), C.byref(ni))
+    else:
+        rc = fn(_ptr(v), _ptr(i), C.byref(nv), C.byref(ni))
+    assert rc == 0, rc
+    return v, i
+
+
+def mat4(kind: str, *args) -> np.ndarray:
+    m = np.zeros(16, np.float32)
+    getattr(lib, "or_mat4_" + kind)(*args, m)
+    return m
+
+
+def mat4_mul(a, b) -> np.ndarray:
+    out = np.zeros(16, np.float32)
+    lib.or_mat4_mul(np.ascontiguousarray(a, np.float32), np.ascontiguousarray(b, np.float32), out)
+    return out
+
+
+def camera_uniform(pos=(0.0, 0.0, 3.0), yaw=None, pitch=0.0, aspect=1.0, frame=0):
+    """camera.rs:144-194 defaults: position (0,0,3), yaw -90 deg."""
+    if yaw is None:
+        yaw = float(np.float32(-90.0) * np.float32(np.pi / 180.0))
+    cam = CameraUniform()
+    lib.or_camera_uniform(np.array(pos, np.float32), yaw, pitch, aspect, frame, C.byref(cam))
+    return cam
+
+
+class Scene:
+    """Oracle scene mirroring SceneBuilder (builder.rs:47-266)."""
+
+    def __init__(self):
+        self.h = lib.or_scene_create()
+        self.built = False
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.or_scene_destroy(self.h)
+            self.h = None
+
+    def add_material(self, color, extra) -> int:
+        m = np.zeros(1, MATERIAL_DTYPE)
+        m["color"] = color
+        m["extra"] = extra
+        return lib.or_scene_add_material(self.h, _ptr(m))
+
+    def add_mesh(self, verts, idx) -> int:
+        verts = np.ascontiguousarray(verts)
+        idx = np.ascontiguousarray(idx, np.uint32)
+        return lib.or_scene_add_mesh(self.h, _ptr(verts), len(verts), idx, len(idx))
+
+    def add_instance(self, mesh, mat, m16):
+        lib.or_scene_add_instance(self.h, mesh, mat, np.ascontiguousarray(m16, np.float32))
+
+    def add_light_instance(self, mesh, mat, m16, params):
+        lib.or_scene_add_light_instance(self.h, mesh, mat, np.ascontiguousarray(m16, np.float32),
+                                        np.asarray(params, np.float32))
+
+    def add_point_light(self, pos, params):
+        lib.or_scene_add_point_light(self.h, np.asarray(pos, np.float32),
+                                     np.asarray(params, np.float32))
+
+    def build(self):
+        assert lib.or_scene_build(self.h) == 0
+        self.built = True
+        return self
+
+    @property
+    def n_triangles(self) -> int:
+        return lib.or_scene_num_triangles(self.h)
+
+    def lights(self) -> np.ndarray:
+        out = np.zeros(lib.or_scene_num_lights(self.h), LIGHT_DTYPE)
+        lib.or_scene_get_lights(self.h, _ptr(out))
+        return out
+
+    def world(self):
+        n = self.n_triangles
+        tris = np.zeros((n, 9), np.float32)
+        ids = np.zeros((n, 3), np.uint32)
+        lib.or_scene_get_world(self.h, _ptr(tris), _ptr(ids))
+        return tris, ids
+
+    def closest_hit(self, rays: np.ndarray, brute: bool = False):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = len(rays)
+        hits = np.zeros((n, 4), np.uint32)
+        flags = np.zeros(n, np.uint32)
+        lib.or_closest_hit(self.h, rays, n, int(brute), hits, flags)
+        return hits, flags
+
+    def any_hit(self, rays: np.ndarray, brute: bool = False):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        occ = np.zeros(len(rays), np.uint32)
+        lib.or_any_hit(self.h, rays, len(rays), int(brute), occ)
+        return occ
+
+    def morton(self) -> np.ndarray:
+        out = np.zeros(self.n_triangles, np.uint32)
+        lib.or_morton_codes(self.h, out)
+        return out
+
+    def photon_trace(self, cfg: Config, frame: int, first: int, n: int) -> np.ndarray:
+        out = np.zeros(n, PHOTON_DTYPE)
+        lib.or_photon_trace(self.h, C.byref(cfg), frame, first, n, _ptr(out))
+        return out
+
+
+def cornell_box() -> Scene:
+    s = Scene()
+    lib.or_scene_cornell_box(s.h)
+    return s.build()
+
+
+def bvh8_closest(nodes: np.ndarray, tris: np.ndarray, rays: np.ndarray):
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = len(rays)
+    hits = np.zeros((n, 4), np.uint32)
+    flags = np.zeros(n, np.uint32)
+    nv = np.zeros(n, np.uint32)
+    nt = np.zeros(n, np.uint32)
+    n_nodes = len(nodes) // 80
+    lib.or_bvh8_closest(_ptr(nodes), n_nodes, _ptr(tris), rays, n, _ptr(hits), _ptr(flags),
+                        _ptr(nv), _ptr(nt))
+    return hits, flags, nv, nt
+
+
+def grid_counting(photons: np.ndarray, frame: int, cell: float, hash_size: int):
+    n = len(photons)
+    cell_id = np.zeros(n, np.uint32)
+    count = np.zeros(hash_size, np.uint32)
+    offset = np.zeros(hash_size, np.uint32)
+    sorted_slot = np.zeros(max(n, 1), np.uint32)
+    valid = lib.or_grid_counting(_ptr(photons), n, frame, cell, hash_size, cell_id, count,
+                                 offset, sorted_slot)
+    return cell_id, count, offset, sorted_slot[:valid]
+
+
+def grid_linked(photons: np.ndarray, frame: int, cell: float, hash_size: int):
+    n = len(photons)
+    head = np.zeros(hash_size, np.uint32)
+    nxt = np.zeros(max(n, 1), np.uint32)
+    lib.or_grid_linked(_ptr(photons), n, frame, cell, hash_size, head, nxt)
+    return head, nxt[:n]
+
+
+class Frame:
+    """One frame of the render loop on the CPU: photon pass, grid, camera paths."""
+
+    def __init__(self, scene: Scene, cfg: Config, cam: CameraUniform, width: int, height: int,
+                 photons: np.ndarray | None = None):
+        self.scene, self.W, self.H = scene, width, height
+        if photons is None:
+            self.h = lib.or_frame_create(scene.h, C.byref(cfg), C.byref(cam), width, height)
+        else:
+            photons = np.ascontiguousarray(photons, PHOTON_DTYPE)
+            self.h = lib.or_frame_create_with_photons(scene.h, C.byref(cfg), C.byref(cam), width,
+                                                      height, _ptr(photons), len(photons))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.or_frame_destroy(self.h)
+            self.h = None
+
+    @property
+    def photons_stored(self) -> int:
+        return lib.or_frame_photons_stored(self.h)
+
+    def trace_pixels(self, pixels):
+        px = np.ascontiguousarray(pixels, np.uint32)
+        n = len(px)
+        wl = np.zeros((n, 4), np.float32)
+        rad = np.zeros((n, 4), np.float32)
+        rgb = np.zeros((n, 3), np.float32)
+        lib.or_frame_trace_pixels(self.h, px, n, _ptr(wl), _ptr(rad), _ptr(rgb))
+        return wl, rad, rgb
+
+    def render(self, accum: np.ndarray | None = None, want_rgba8: bool = False):
+        if accum is None:
+            accum = np.zeros((self.H * self.W, 4), np.float32)
+        rgba = np.zeros((self.H * self.W, 4), np.uint8) if want_rgba8 else None
+        lib.or_frame_render(self.h, accum, _ptr(rgba))
+        return accum, rgba
+
+    def gather(self, pos, n, wl):
+        out = np.zeros(4, np.float32)
+        lib.or_frame_gather(self.h, np.asarray(pos, np.float32), np.asarray(n, np.float32),
+                            np.asarray(wl, np.float32), out)
+        return out

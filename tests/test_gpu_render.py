@@ -1,0 +1,169 @@
+"""The render loop on the GPU against the oracle (photon pass, grid, camera
+paths, accumulation), on the reference's Cornell box and on the caustic scene.
+
+Tolerances (f32 on both sides; the GPU's exp/pow/tan/cos/sin differ from the
+host libm by a few ulp, which can flip a threshold test such as rand() < F
+and send a path down another branch):
+  * photons: the stored/not-stored decision agrees for >= 99% of photon
+    paths; positions of photons stored by both agree to 1e-3 for >= 99%;
+  * grid: bit-exact on the GPU's own photon buffer;
+  * camera paths (oracle fed the GPU's photon buffer): hero wavelengths agree
+    to 1e-4 relative and the spectral radiance of a path to 1e-4 relative
+    (absolute floor 1e-6 of the frame's mean) for >= 99% of paths;
+  * accumulated image: relMSE = mean (I - R)^2 / (R^2 + 1e-2) <= 1e-4.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+import scenes
+import wsrt
+
+pytestmark = pytest.mark.gpu
+
+
+def to_oracle_cam(cam):
+    return O.CameraUniform.from_buffer_copy(bytes(cam))
+
+
+def to_oracle_cfg(cfg):
+    return O.Config.from_buffer_copy(bytes(cfg))
+
+
+def setup(device, name, W, H, photons_per_pass):
+    if name == "cornell":
+        g = wsrt.create_cornell_box(device)
+        o = O.cornell_box()
+        cfg = wsrt.default_config(photons_per_pass=photons_per_pass)
+        view = scenes.View((0.0, 0.0, 3.0), scenes.HALF_PI * -1, 0.0, W, H)
+    else:
+        g = scenes.build_c2_caustic(wsrt.SceneBuilder()).build(device)
+        o = scenes.build_c2_caustic(O.Scene()).build()
+        cfg = scenes.c2_config(photons_per_pass=photons_per_pass)
+        view = scenes.View(scenes.C2_VIEW.position, scenes.C2_VIEW.yaw, scenes.C2_VIEW.pitch, W, H)
+    return g, o, cfg, view
+
+
+def rel_close(a, b, rel, floor):
+    return np.all(np.abs(a - b) <= rel * np.maximum(np.abs(a), np.abs(b)) + floor, axis=-1)
+
+
+@pytest.mark.parametrize("name,frame", [("cornell", 0), ("cornell", 5), ("c2", 2)])
+def test_photon_pass_and_grid(device, name, frame):
+    print(f"photons {name} frame {frame}", flush=True)
+    g, o, cfg, view = setup(device, name, 64, 64, 1 << 16)
+    r = wsrt.Renderer(device, g, view.width, view.height, cfg)
+    r.set_camera(scenes.camera(view, frame))
+    r.render()
+    ph, stored = r.read_photons()
+    ref = o.photon_trace(to_oracle_cfg(cfg), frame, 0, cfg.photons_per_pass)
+    gs, os_ = ph["energy"] != 0, ref["energy"] != 0
+    assert stored == gs.sum() and stored > 100
+    assert (gs == os_).mean() >= 0.99
+    both = gs & os_
+    ok = np.all(np.abs(ph["position"][both] - ref["position"][both]) <= 1e-3, axis=1)
+    assert ok.mean() >= 0.99
+    wl_ok = np.abs(ph["wavelength"][both] - ref["wavelength"][both]) <= 1e-3
+    assert wl_ok.mean() >= 0.99
+    # grid of the frame: bit-exact with the oracle's counting sort of the GPU photons
+    cell, count, offset, slot = r.read_grid()
+    oc, ocount, ooff, oslot = O.grid_counting(ph, frame, cfg.cell_size, cfg.hash_size)
+    np.testing.assert_array_equal(cell, oc)
+    np.testing.assert_array_equal(count, ocount)
+    np.testing.assert_array_equal(offset, ooff)
+    np.testing.assert_array_equal(slot[: len(oslot)], oslot)
+
+
+@pytest.mark.parametrize("name,frame,photons", [("cornell", 0, 1 << 16), ("cornell", 9, 1 << 16),
+                                                ("c2", 1, 1 << 18)])
+def test_camera_paths_match_oracle(device, name, frame, photons):
+    print(f"paths {name} frame {frame}", flush=True)
+    W = H = 96
+    g, o, cfg, view = setup(device, name, W, H, photons)
+    r = wsrt.Renderer(device, g, W, H, cfg)
+    cam = scenes.camera(view, frame)
+    r.set_camera(cam)
+    r.render()
+    wl, L = r.read_paths()
+    ph, _ = r.read_photons()
+    f = O.Frame(o, to_oracle_cfg(cfg), to_oracle_cam(cam), W, H, photons=ph)
+    owl, oL, orgb = f.trace_pixels(np.arange(W * H, dtype=np.uint32))
+    assert rel_close(wl, owl, 1e-4, 0.0).mean() >= 0.999
+    floor = 1e-6 * max(float(np.abs(oL).mean()), 1e-12)
+    same = rel_close(L, oL, 1e-4, floor)
+    print(f"  paths within 1e-4: {same.mean():.5f}; nonzero {np.mean(np.abs(oL).sum(1) > 0):.3f}")
+    assert same.mean() >= 0.99
+    assert np.abs(oL).sum() > 0
+
+
+@pytest.mark.parametrize("name", ["cornell", "c2"])
+def test_accumulated_image_matches_oracle(device, name):
+    print(f"image {name}", flush=True)
+    W = H = 64
+    frames = 8
+    g, o, cfg, view = setup(device, name, W, H, 1 << 16)
+    r = wsrt.Renderer(device, g, W, H, cfg)
+    r.set_camera(scenes.camera(view, 0))
+    r.render_frames(frames)
+    acc = r.read_accum()
+    assert r.frame_count == frames
+    ref = np.zeros((W * H, 4), np.float32)
+    ocfg = to_oracle_cfg(cfg)
+    for k in range(frames):
+        cam = to_oracle_cam(scenes.camera(view, k))
+        O.Frame(o, ocfg, cam, W, H).render(ref)
+    np.testing.assert_array_equal(acc[:, 3], frames)
+    img, want = acc[:, :3] / frames, ref[:, :3] / frames
+    relmse = float(np.mean((img - want) ** 2 / (want**2 + 1e-2)))
+    print(f"  relMSE {relmse:.3e}")
+    assert relmse <= 1e-4
+    rgba = r.read_rgba8()
+    assert rgba[:, 3].min() == 255 and rgba[:, :3].max() > 0
+
+
+def test_frame_count_semantics(device, cornell_pair):
+    g, _ = cornell_pair
+    r = wsrt.Renderer(device, g, 32, 32, wsrt.default_config(photons_per_pass=1 << 12))
+    assert r.frame_count == 0
+    r.render_frames(3)
+    assert r.frame_count == 3
+    np.testing.assert_array_equal(r.read_accum()[:, 3], 3)
+    r.frame_count = 0  # camera moved (engine.rs:154-156): accumulation restarts
+    r.render_frames(2)
+    np.testing.assert_array_equal(r.read_accum()[:, 3], 2)
+    # resume: exported accumulation written back is continued
+    acc = r.read_accum()
+    r2 = wsrt.Renderer(device, g, 32, 32, wsrt.default_config(photons_per_pass=1 << 12))
+    r2.write_accum(acc)
+    r2.frame_count = 2
+    r2.render_frames(1)
+    r.render_frames(1)
+    np.testing.assert_array_equal(r2.read_accum(), r.read_accum())
+    r.resize(1920, 1080)  # refits presentation only
+    assert r.frame_count == 3
+
+
+def test_zero_light_scene_renders_black(device):
+    b = wsrt.SceneBuilder()
+    m = b.add_material([0.5, 0.5, 0.5, 1], [0, 0, 0, 0])
+    b.add_instance(b.add_mesh(*wsrt.create_cube()), m, wsrt.mat4_scale(1, 1, 1))
+    s = b.build(device)
+    r = wsrt.Renderer(device, s, 16, 16, wsrt.default_config(photons_per_pass=1 << 10))
+    r.render()
+    assert np.all(r.read_accum()[:, :3] == 0)
+    _, stored = r.read_photons()
+    assert stored == 0
+
+
+def test_empty_scene(device):
+    b = wsrt.SceneBuilder()
+    b.add_material([0.5, 0.5, 0.5, 1], [0, 0, 0, 0])
+    s = b.build(device)
+    assert s.info.n_triangles == 0 and s.info.n_nodes == 0
+    r = wsrt.Renderer(device, s, 8, 8, wsrt.default_config(photons_per_pass=256))
+    r.render()
+    assert np.all(r.read_accum()[:, :3] == 0)
+    hits, flags = s.closest_hit(np.array([[0, 0, 0, 1e-4, 0, 0, -1, 100]], np.float32))
+    assert flags[0] == 0

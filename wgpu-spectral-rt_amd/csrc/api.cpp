@@ -1,0 +1,1308 @@
+// api.cpp - the C ABI of libwsrt (include/wsrt.h): devices, the scene
+// builder (src/scene/builder.rs), the renderer (src/renderer/mod.rs) and the
+// debug taps. Host code; kernels are reached through kernels.hpp.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <array>
+#include <map>
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "shading.hpp"
+
+using namespace wsrt;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_OR_FAIL(expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            if (e_ == hipErrorOutOfMemory)                                                  \
+                return fail(WSRT_E_OOM, "%s: %s", #expr, hipGetErrorString(e_));            \
+            return fail(WSRT_E_HIP, "%s: %s", #expr, hipGetErrorString(e_));                \
+        }                                                                                   \
+    } while (0)
+
+#define CHECK_ARG(cond)                                                                     \
+    do {                                                                                    \
+        if (!(cond)) return fail(WSRT_E_INVALID, "invalid argument: %s", #cond);            \
+    } while (0)
+
+extern "C" const char* wsrt_last_error(void) { return g_last_error.c_str(); }
+extern "C" int wsrt_abi_version(void) { return WSRT_ABI_VERSION; }
+
+// ---------------------------------------------------------------------------
+// device memory helper
+// ---------------------------------------------------------------------------
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n) {
+        release();
+        bytes = n;
+        return hipMalloc(&p, n < 256 ? 256 : n);
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+void mat4_mul_vec4(const float* a, const float* v, float* out) {
+    float r[4];
+    for (int i = 0; i < 4; ++i) {
+        float s = a[0 + i] * v[0];
+        s = s + a[4 + i] * v[1];
+        s = s + a[8 + i] * v[2];
+        s = s + a[12 + i] * v[3];
+        r[i] = s;
+    }
+    memcpy(out, r, sizeof r);
+}
+void mat4_mul(const float* a, const float* b, float* out) {
+    float r[16];
+    for (int c = 0; c < 4; ++c) mat4_mul_vec4(a, b + 4 * c, r + 4 * c);
+    memcpy(out, r, sizeof r);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// device
+// ---------------------------------------------------------------------------
+struct wsrt_device {
+    int ordinal = 0;
+    hipStream_t stream = nullptr;  // builds and debug queries
+    char name[256] = {0};
+};
+
+extern "C" int wsrt_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" int wsrt_device_create(int ordinal, wsrt_device** out) {
+    CHECK_ARG(out);
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        return fail(WSRT_E_UNSUPPORTED, "no HIP device available");
+    if (ordinal < 0 || ordinal >= n) return fail(WSRT_E_INVALID, "device ordinal %d of %d", ordinal, n);
+    HIP_OR_FAIL(hipSetDevice(ordinal));
+    hipDeviceProp_t prop;
+    HIP_OR_FAIL(hipGetDeviceProperties(&prop, ordinal));
+    auto d = std::make_unique<wsrt_device>();
+    d->ordinal = ordinal;
+    snprintf(d->name, sizeof d->name, "%s", prop.gcnArchName);
+    HIP_OR_FAIL(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    *out = d.release();
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_device_destroy(wsrt_device* dev) {
+    if (!dev) return WSRT_OK;
+    (void)hipSetDevice(dev->ordinal);
+    if (dev->stream) (void)hipStreamDestroy(dev->stream);
+    delete dev;
+    return WSRT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// geometry generators (src/scene/geometry.rs)
+// ---------------------------------------------------------------------------
+static wsrt_vertex vtx(float px, float py, float pz, float nx, float ny, float nz) {
+    wsrt_vertex v;
+    v.pos[0] = px; v.pos[1] = py; v.pos[2] = pz; v.pos[3] = 1.0f;
+    v.normal[0] = nx; v.normal[1] = ny; v.normal[2] = nz; v.normal[3] = 0.0f;
+    return v;
+}
+
+static void gen_plane(std::vector<wsrt_vertex>& v, std::vector<uint32_t>& i) {
+    // 1 x 1 quad on the XZ plane facing +Y (geometry.rs:12-35)
+    v = {vtx(-0.5f, 0, 0.5f, 0, 1, 0), vtx(0.5f, 0, 0.5f, 0, 1, 0), vtx(-0.5f, 0, -0.5f, 0, 1, 0),
+         vtx(0.5f, 0, -0.5f, 0, 1, 0)};
+    i = {0, 1, 2, 2, 1, 3};
+}
+
+static void gen_cube(std::vector<wsrt_vertex>& v, std::vector<uint32_t>& idx) {
+    // six quads, outward normals, two triangles each (geometry.rs:38-118)
+    struct Face {
+        float n[3];
+        float c[4][3];
+    };
+    static const Face faces[6] = {
+        {{0, 0, 1}, {{-0.5f, -0.5f, 0.5f}, {0.5f, -0.5f, 0.5f}, {0.5f, 0.5f, 0.5f}, {-0.5f, 0.5f, 0.5f}}},
+        {{0, 0, -1}, {{0.5f, -0.5f, -0.5f}, {-0.5f, -0.5f, -0.5f}, {-0.5f, 0.5f, -0.5f}, {0.5f, 0.5f, -0.5f}}},
+        {{0, 1, 0}, {{-0.5f, 0.5f, 0.5f}, {0.5f, 0.5f, 0.5f}, {0.5f, 0.5f, -0.5f}, {-0.5f, 0.5f, -0.5f}}},
+        {{0, -1, 0}, {{-0.5f, -0.5f, -0.5f}, {0.5f, -0.5f, -0.5f}, {0.5f, -0.5f, 0.5f}, {-0.5f, -0.5f, 0.5f}}},
+        {{1, 0, 0}, {{0.5f, -0.5f, 0.5f}, {0.5f, -0.5f, -0.5f}, {0.5f, 0.5f, -0.5f}, {0.5f, 0.5f, 0.5f}}},
+        {{-1, 0, 0}, {{-0.5f, -0.5f, -0.5f}, {-0.5f, -0.5f, 0.5f}, {-0.5f, 0.5f, 0.5f}, {-0.5f, 0.5f, -0.5f}}},
+    };
+    v.clear();
+    idx.clear();
+    for (const Face& f : faces) {
+        uint32_t b = (uint32_t)v.size();
+        for (const auto& c : f.c) v.push_back(vtx(c[0], c[1], c[2], f.n[0], f.n[1], f.n[2]));
+        const uint32_t q[6] = {b, b + 1, b + 2, b, b + 2, b + 3};
+        idx.insert(idx.end(), q, q + 6);
+    }
+}
+
+static void gen_sphere(uint32_t subdiv, std::vector<wsrt_vertex>& v, std::vector<uint32_t>& idx) {
+    // icosphere of radius 0.5 (geometry.rs:121-232): 12 icosahedron vertices,
+    // each level splits every face into four at cached edge midpoints that are
+    // pushed back onto the sphere
+    v.clear();
+    auto push = [&v](float x, float y, float z) {
+        float len = sqrtf(x * x + y * y + z * z);
+        float nx = x / len, ny = y / len, nz = z / len;
+        v.push_back(vtx(nx * 0.5f, ny * 0.5f, nz * 0.5f, nx, ny, nz));
+        return (uint32_t)v.size() - 1;
+    };
+    const float t = (1.0f + sqrtf(5.0f)) / 2.0f;
+    const float ico[12][3] = {{-1, t, 0}, {1, t, 0}, {-1, -t, 0}, {1, -t, 0}, {0, -1, t}, {0, 1, t},
+                              {0, -1, -t}, {0, 1, -t}, {t, 0, -1}, {t, 0, 1}, {-t, 0, -1}, {-t, 0, 1}};
+    for (const auto& p : ico) push(p[0], p[1], p[2]);
+    std::vector<std::array<uint32_t, 3>> faces = {
+        {0, 11, 5}, {0, 5, 1}, {0, 1, 7}, {0, 7, 10}, {0, 10, 11}, {1, 5, 9}, {5, 11, 4},
+        {11, 10, 2}, {10, 7, 6}, {7, 1, 8}, {3, 9, 4}, {3, 4, 2}, {3, 2, 6}, {3, 6, 8},
+        {3, 8, 9}, {4, 9, 5}, {2, 4, 11}, {6, 2, 10}, {8, 6, 7}, {9, 8, 1}};
+    std::map<std::pair<uint32_t, uint32_t>, uint32_t> mid;
+    auto midpoint = [&](uint32_t a, uint32_t b) {
+        auto key = a < b ? std::make_pair(a, b) : std::make_pair(b, a);
+        auto it = mid.find(key);
+        if (it != mid.end()) return it->second;
+        const float* pa = v[a].pos;
+        const float* pb = v[b].pos;
+        uint32_t id = push((pa[0] + pb[0]) * 0.5f, (pa[1] + pb[1]) * 0.5f, (pa[2] + pb[2]) * 0.5f);
+        mid.emplace(key, id);
+        return id;
+    };
+    for (uint32_t s = 0; s < subdiv; ++s) {
+        std::vector<std::array<uint32_t, 3>> next;
+        next.reserve(faces.size() * 4);
+        for (const auto& f : faces) {
+            uint32_t a = midpoint(f[0], f[1]);
+            uint32_t b = midpoint(f[1], f[2]);
+            uint32_t c = midpoint(f[2], f[0]);
+            next.push_back({f[0], a, c});
+            next.push_back({f[1], b, a});
+            next.push_back({f[2], c, b});
+            next.push_back({a, b, c});
+        }
+        faces.swap(next);
+    }
+    idx.clear();
+    for (const auto& f : faces) idx.insert(idx.end(), f.begin(), f.end());
+}
+
+static int copy_geometry(const std::vector<wsrt_vertex>& v, const std::vector<uint32_t>& i,
+                         wsrt_vertex* verts, uint32_t* idx, uint32_t* nv, uint32_t* ni) {
+    CHECK_ARG(nv && ni);
+    if (verts && idx) {
+        if (*nv < v.size() || *ni < i.size())
+            return fail(WSRT_E_INVALID, "geometry buffers too small (%u/%zu vertices, %u/%zu indices)",
+                        *nv, v.size(), *ni, i.size());
+        memcpy(verts, v.data(), v.size() * sizeof(wsrt_vertex));
+        memcpy(idx, i.data(), i.size() * sizeof(uint32_t));
+    }
+    *nv = (uint32_t)v.size();
+    *ni = (uint32_t)i.size();
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_geometry_plane(wsrt_vertex* verts, uint32_t* idx, uint32_t* nv, uint32_t* ni) {
+    std::vector<wsrt_vertex> v;
+    std::vector<uint32_t> i;
+    gen_plane(v, i);
+    return copy_geometry(v, i, verts, idx, nv, ni);
+}
+extern "C" int wsrt_geometry_cube(wsrt_vertex* verts, uint32_t* idx, uint32_t* nv, uint32_t* ni) {
+    std::vector<wsrt_vertex> v;
+    std::vector<uint32_t> i;
+    gen_cube(v, i);
+    return copy_geometry(v, i, verts, idx, nv, ni);
+}
+extern "C" int wsrt_geometry_sphere(uint32_t subdiv, wsrt_vertex* verts, uint32_t* idx, uint32_t* nv,
+                                    uint32_t* ni) {
+    if (subdiv > 8) return fail(WSRT_E_INVALID, "sphere subdivisions %u > 8", subdiv);
+    std::vector<wsrt_vertex> v;
+    std::vector<uint32_t> i;
+    gen_sphere(subdiv, v, i);
+    return copy_geometry(v, i, verts, idx, nv, ni);
+}
+
+// ---------------------------------------------------------------------------
+// scene builder (src/scene/builder.rs:47-266)
+// ---------------------------------------------------------------------------
+struct MeshRec {
+    uint32_t vertex_offset, index_offset, n_verts, n_idx;
+};
+struct InstRec {
+    uint32_t mesh, mat;
+    float m[16];
+};
+
+struct wsrt_scene_builder {
+    std::vector<wsrt_vertex> verts;
+    std::vector<uint32_t> idx;
+    std::vector<MeshRec> meshes;
+    std::vector<wsrt_material> materials;
+    std::vector<InstRec> instances;
+    std::vector<wsrt_light_info> lights;
+    std::vector<float> env;
+    uint32_t env_w = 0, env_h = 0;
+};
+
+extern "C" int wsrt_scene_builder_create(wsrt_scene_builder** out) {
+    CHECK_ARG(out);
+    *out = new (std::nothrow) wsrt_scene_builder();
+    return *out ? WSRT_OK : fail(WSRT_E_OOM, "scene builder allocation failed");
+}
+extern "C" int wsrt_scene_builder_destroy(wsrt_scene_builder* b) {
+    delete b;
+    return WSRT_OK;
+}
+extern "C" int wsrt_scene_add_material(wsrt_scene_builder* b, const wsrt_material* m, uint32_t* id) {
+    CHECK_ARG(b && m);
+    uint32_t t = (uint32_t)m->extra[0];
+    if (!(m->extra[0] >= 0.0f) || t > 5u)
+        return fail(WSRT_E_INVALID, "material type %g not in 0..5", (double)m->extra[0]);
+    if (id) *id = (uint32_t)b->materials.size();
+    b->materials.push_back(*m);
+    return WSRT_OK;
+}
+extern "C" int wsrt_scene_add_mesh(wsrt_scene_builder* b, const wsrt_vertex* verts, size_t n_verts,
+                                   const uint32_t* idx, size_t n_idx, uint32_t* id) {
+    CHECK_ARG(b && (verts || n_verts == 0) && (idx || n_idx == 0));
+    if (n_idx % 3) return fail(WSRT_E_INVALID, "index count %zu is not a multiple of 3", n_idx);
+    for (size_t k = 0; k < n_idx; ++k)
+        if (idx[k] >= n_verts)
+            return fail(WSRT_E_INVALID, "index %u at %zu out of range (%zu vertices)", idx[k], k, n_verts);
+    if (b->verts.size() + n_verts > 0xFFFFFFFFull || b->idx.size() + n_idx > 0xFFFFFFFFull)
+        return fail(WSRT_E_INVALID, "scene exceeds 2^32 vertices or indices");
+    MeshRec r{(uint32_t)b->verts.size(), (uint32_t)b->idx.size(), (uint32_t)n_verts, (uint32_t)n_idx};
+    if (id) *id = (uint32_t)b->meshes.size();
+    b->meshes.push_back(r);
+    b->verts.insert(b->verts.end(), verts, verts + n_verts);
+    b->idx.insert(b->idx.end(), idx, idx + n_idx);
+    return WSRT_OK;
+}
+extern "C" int wsrt_scene_add_instance(wsrt_scene_builder* b, uint32_t mesh, uint32_t mat,
+                                       const float transform[16]) {
+    CHECK_ARG(b && transform);
+    if (mesh >= b->meshes.size()) return fail(WSRT_E_INVALID, "mesh %u of %zu", mesh, b->meshes.size());
+    if (mat >= b->materials.size())
+        return fail(WSRT_E_INVALID, "material %u of %zu", mat, b->materials.size());
+    InstRec r;
+    r.mesh = mesh;
+    r.mat = mat;
+    memcpy(r.m, transform, sizeof r.m);
+    b->instances.push_back(r);
+    return WSRT_OK;
+}
+extern "C" int wsrt_scene_add_light_instance(wsrt_scene_builder* b, uint32_t mesh, uint32_t mat,
+                                             const float transform[16], const float params[4]) {
+    CHECK_ARG(params);
+    int rc = wsrt_scene_add_instance(b, mesh, mat, transform);
+    if (rc) return rc;
+    // one world-space light per triangle (builder.rs:110-128)
+    const MeshRec& mr = b->meshes[mesh];
+    for (uint32_t i = 0; i + 2 < mr.n_idx; i += 3) {
+        wsrt_light_info L;
+        float* dst[3] = {L.v0, L.v1, L.v2};
+        for (int k = 0; k < 3; ++k) {
+            const float* p = b->verts[mr.vertex_offset + b->idx[mr.index_offset + i + k]].pos;
+            float v[4] = {p[0], p[1], p[2], 1.0f}, r[4];
+            mat4_mul_vec4(transform, v, r);
+            dst[k][0] = r[0]; dst[k][1] = r[1]; dst[k][2] = r[2]; dst[k][3] = 1.0f;
+        }
+        memcpy(L.params, params, sizeof L.params);
+        b->lights.push_back(L);
+    }
+    return WSRT_OK;
+}
+extern "C" int wsrt_scene_add_point_light(wsrt_scene_builder* b, const float position[3],
+                                          const float params[4]) {
+    CHECK_ARG(b && position && params);
+    wsrt_light_info L;
+    memset(&L, 0, sizeof L);
+    L.v0[0] = position[0]; L.v0[1] = position[1]; L.v0[2] = position[2]; L.v0[3] = 1.0f;
+    memcpy(L.params, params, sizeof L.params);
+    L.params[0] = 1.0f;
+    b->lights.push_back(L);
+    return WSRT_OK;
+}
+extern "C" int wsrt_scene_set_environment(wsrt_scene_builder* b, const float* rgb, uint32_t w,
+                                          uint32_t h) {
+    CHECK_ARG(b && rgb && w > 0 && h > 0 && (uint64_t)w * h <= (1ull << 26));
+    b->env.assign(rgb, rgb + (size_t)w * h * 3);
+    b->env_w = w;
+    b->env_h = h;
+    return WSRT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// scene
+// ---------------------------------------------------------------------------
+struct wsrt_scene {
+    wsrt_device* dev = nullptr;
+    DevBuf verts, idx, meshes, materials, lights, instances, env;
+    DevBuf nodes, tris, wv, tri_inst, codes, sorted;
+    SceneView view{};
+    wsrt_scene_info info{};
+};
+
+static void linear_inverse(const float* m, float* w) {
+    // inverse of the 3x3 linear part in f64, rounded to f32 (column-major)
+    double a = m[0], b = m[4], c = m[8], d = m[1], e = m[5], f = m[9], g = m[2], h = m[6], i = m[10];
+    double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+    double D = -(b * i - c * h), E = a * i - c * g, F = -(a * h - b * g);
+    double G = b * f - c * e, H = -(a * f - c * d), I = a * e - b * d;
+    double det = a * A + b * B + c * C;
+    double r[3][3] = {{A / det, D / det, G / det}, {B / det, E / det, H / det}, {C / det, F / det, I / det}};
+    for (int col = 0; col < 3; ++col)
+        for (int row = 0; row < 3; ++row) w[col * 3 + row] = (float)r[row][col];
+}
+
+template <class T>
+static hipError_t upload(DevBuf& buf, const std::vector<T>& v, hipStream_t s) {
+    hipError_t e = buf.alloc(v.size() * sizeof(T));
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+extern "C" int wsrt_scene_build(wsrt_scene_builder* b, wsrt_device* dev, wsrt_scene** out) {
+    CHECK_ARG(b && dev && out);
+    *out = nullptr;
+    HIP_OR_FAIL(hipSetDevice(dev->ordinal));
+    auto sc = std::make_unique<wsrt_scene>();
+    sc->dev = dev;
+    hipStream_t s = dev->stream;
+    // instance table with world-triangle ranges and world-to-object matrices
+    std::vector<Instance> inst(b->instances.size());
+    uint64_t n_tris = 0;
+    for (size_t k = 0; k < b->instances.size(); ++k) {
+        const InstRec& r = b->instances[k];
+        Instance& I = inst[k];
+        memset(&I, 0, sizeof I);
+        I.mesh = r.mesh;
+        I.material = r.mat;
+        I.first_tri = (uint32_t)n_tris;
+        I.n_tris = b->meshes[r.mesh].n_idx / 3;
+        memcpy(I.m, r.m, sizeof I.m);
+        linear_inverse(r.m, I.w2o);
+        n_tris += I.n_tris;
+    }
+    if (n_tris >= 0x7FFFFFFFull) return fail(WSRT_E_INVALID, "scene has %llu triangles (max 2^31-1)",
+                                             (unsigned long long)n_tris);
+    std::vector<wsrt_mesh_info> mi(b->meshes.size());
+    for (size_t k = 0; k < b->meshes.size(); ++k) {
+        mi[k].vertex_offset = b->meshes[k].vertex_offset;
+        mi[k].index_offset = b->meshes[k].index_offset;
+        mi[k].pad[0] = mi[k].pad[1] = 0;
+    }
+    HIP_OR_FAIL(upload(sc->verts, b->verts, s));
+    HIP_OR_FAIL(upload(sc->idx, b->idx, s));
+    HIP_OR_FAIL(upload(sc->meshes, mi, s));
+    HIP_OR_FAIL(upload(sc->materials, b->materials, s));
+    HIP_OR_FAIL(upload(sc->lights, b->lights, s));
+    HIP_OR_FAIL(upload(sc->instances, inst, s));
+    HIP_OR_FAIL(upload(sc->env, b->env, s));
+    hipEvent_t e0, e1;
+    HIP_OR_FAIL(hipEventCreate(&e0));
+    HIP_OR_FAIL(hipEventCreate(&e1));
+    HIP_OR_FAIL(hipEventRecord(e0, s));
+    BvhInputs bi;
+    bi.instances = sc->instances.as<Instance>();
+    bi.n_instances = (uint32_t)inst.size();
+    bi.verts = sc->verts.as<wsrt_vertex>();
+    bi.indices = sc->idx.as<uint32_t>();
+    bi.meshes = sc->meshes.as<wsrt_mesh_info>();
+    bi.n_tris = (uint32_t)n_tris;
+    BvhOutputs bo;
+    int rc = bvh_build(bi, &bo, s);
+    HIP_OR_FAIL(hipEventRecord(e1, s));
+    HIP_OR_FAIL(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == -2) return fail(WSRT_E_OOM, "BVH build: device allocation failed");
+    if (rc != 0) return fail(WSRT_E_HIP, "BVH build failed (%d): %s", rc, hipGetErrorString(hipGetLastError()));
+    if (bo.placed_tris != n_tris)
+        return fail(WSRT_E_HIP, "BVH build placed %u of %llu triangles", bo.placed_tris,
+                    (unsigned long long)n_tris);
+    sc->nodes.p = bo.nodes;
+    sc->nodes.bytes = (size_t)bo.n_nodes * sizeof(wsrt_bvh8_node);
+    sc->tris.p = bo.tris;
+    sc->tris.bytes = (size_t)n_tris * sizeof(wsrt_bvh_tri);
+    sc->wv.p = bo.wv;
+    sc->wv.bytes = (size_t)n_tris * 9 * sizeof(float);
+    sc->tri_inst.p = bo.tri_inst;
+    sc->tri_inst.bytes = (size_t)n_tris * sizeof(uint32_t);
+    sc->codes.p = bo.codes;
+    sc->codes.bytes = (size_t)n_tris * sizeof(uint32_t);
+    sc->sorted.p = bo.sorted;
+    sc->sorted.bytes = (size_t)n_tris * sizeof(uint32_t);
+    SceneView& v = sc->view;
+    v.nodes = sc->nodes.as<wsrt_bvh8_node>();
+    v.tris = sc->tris.as<float4>();
+    v.verts = sc->verts.as<wsrt_vertex>();
+    v.indices = sc->idx.as<uint32_t>();
+    v.meshes = sc->meshes.as<wsrt_mesh_info>();
+    v.materials = sc->materials.as<wsrt_material>();
+    v.lights = sc->lights.as<wsrt_light_info>();
+    v.instances = sc->instances.as<Instance>();
+    v.tri_instance = sc->tri_inst.as<uint32_t>();
+    v.env = b->env.empty() ? nullptr : sc->env.as<float>();
+    v.n_nodes = bo.n_nodes;
+    v.n_tris = (uint32_t)n_tris;
+    v.n_lights = (uint32_t)b->lights.size();
+    v.n_instances = (uint32_t)inst.size();
+    v.env_w = b->env_w;
+    v.env_h = b->env_h;
+    v.stack_size = 1u + 7u * bo.max_depth;
+    wsrt_scene_info& I = sc->info;
+    I.n_triangles = (uint32_t)n_tris;
+    I.n_instances = (uint32_t)inst.size();
+    I.n_lights = (uint32_t)b->lights.size();
+    I.n_materials = (uint32_t)b->materials.size();
+    I.n_nodes = bo.n_nodes;
+    I.max_depth = bo.max_depth;
+    I.stack_size = v.stack_size;
+    I.node_bytes = sizeof(wsrt_bvh8_node);
+    I.tri_bytes = sizeof(wsrt_bvh_tri);
+    I.build_ms = ms;
+    if (bo.n_nodes) {
+        wsrt_bvh8_node root;
+        HIP_OR_FAIL(hipMemcpy(&root, bo.nodes, sizeof root, hipMemcpyDeviceToHost));
+        for (int a = 0; a < 3; ++a) {
+            I.bounds[a] = root.origin[a];
+            float s = ldexpf(1.0f, root.exp[a]);
+            float hi = root.origin[a];
+            for (int c = 0; c < 8; ++c)
+                if (root.meta[c]) hi = fmaxf(hi, root.origin[a] + (float)root.qhi[a][c] * s);
+            I.bounds[3 + a] = hi;
+        }
+    }
+    *out = sc.release();
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_scene_destroy(wsrt_scene* s) {
+    if (!s) return WSRT_OK;
+    (void)hipSetDevice(s->dev->ordinal);
+    (void)hipStreamSynchronize(s->dev->stream);
+    delete s;
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_scene_get_info(const wsrt_scene* s, wsrt_scene_info* out) {
+    CHECK_ARG(s && out);
+    *out = s->info;
+    return WSRT_OK;
+}
+
+// cornell_box.rs:6-110
+static void trs(float tx, float ty, float tz, int axis, float ang, float sx, float sy, float sz,
+                float* out) {
+    float T[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, tx, ty, tz, 1};
+    float R[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    float S[16] = {sx, 0, 0, 0, 0, sy, 0, 0, 0, 0, sz, 0, 0, 0, 0, 1};
+    float sn = sinf(ang), cs = cosf(ang);
+    if (axis == 0) { R[5] = cs; R[6] = sn; R[9] = -sn; R[10] = cs; }
+    if (axis == 1) { R[0] = cs; R[2] = -sn; R[8] = sn; R[10] = cs; }
+    if (axis == 2) { R[0] = cs; R[1] = sn; R[4] = -sn; R[5] = cs; }
+    float TR[16];
+    if (axis < 0) {
+        mat4_mul(T, S, out);
+        return;
+    }
+    mat4_mul(T, R, TR);
+    mat4_mul(TR, S, out);
+}
+
+extern "C" int wsrt_scene_builder_cornell_box(wsrt_scene_builder** out) {
+    CHECK_ARG(out);
+    int rc = wsrt_scene_builder_create(out);
+    if (rc) return rc;
+    wsrt_scene_builder* b = *out;
+    const float PI_F = 3.14159265358979323846f, HALF_PI = 1.57079632679489661923f;
+    const wsrt_material mats[6] = {
+        {{6500.0f, 1.0f, 0.0f, 1.0f}, {3, 0, 0, 0}},           // light
+        {{0.65f, 0.05f, 0.05f, 1}, {0, 0, 0, 0}},              // red
+        {{0.12f, 0.45f, 0.15f, 1}, {0, 0, 0, 0}},              // green
+        {{0.73f, 0.73f, 0.73f, 1}, {0, 0, 0, 0}},              // white
+        {{1, 1, 1, 1}, {2, 0, 1.5f, 0.01f}},                   // glass
+        {{1, 1, 1, 1}, {4, 500.0f, 1.33f, 1.0f}},              // bubble
+    };
+    uint32_t id[6];
+    for (int k = 0; k < 6; ++k) wsrt_scene_add_material(b, &mats[k], &id[k]);
+    std::vector<wsrt_vertex> v;
+    std::vector<uint32_t> i;
+    uint32_t plane, cube, sphere;
+    gen_plane(v, i);
+    wsrt_scene_add_mesh(b, v.data(), v.size(), i.data(), i.size(), &plane);
+    gen_cube(v, i);
+    wsrt_scene_add_mesh(b, v.data(), v.size(), i.data(), i.size(), &cube);
+    gen_sphere(4, v, i);
+    wsrt_scene_add_mesh(b, v.data(), v.size(), i.data(), i.size(), &sphere);
+    float M[16];
+    trs(0, -1, 0, -1, 0, 2, 2, 2, M);
+    wsrt_scene_add_instance(b, plane, id[3], M);
+    trs(0, 1, 0, 0, PI_F, 2, 2, 2, M);
+    wsrt_scene_add_instance(b, plane, id[3], M);
+    trs(0, 0, -1, 0, HALF_PI, 2, 2, 2, M);
+    wsrt_scene_add_instance(b, plane, id[3], M);
+    trs(-1, 0, 0, 2, -HALF_PI, 2, 2, 2, M);
+    wsrt_scene_add_instance(b, plane, id[1], M);
+    trs(1, 0, 0, 2, HALF_PI, 2, 2, 2, M);
+    wsrt_scene_add_instance(b, plane, id[2], M);
+    trs(0, 0.99f, 0, 0, PI_F, 0.5f, 0.5f, 0.5f, M);
+    const float lp[4] = {0.0f, 6500.0f, 1.0f, 0.0f};
+    wsrt_scene_add_light_instance(b, plane, id[0], M, lp);
+    trs(-0.35f, -0.4f + 0.002f, -0.3f, 1, 0.4f, 0.6f, 1.2f, 0.6f, M);
+    wsrt_scene_add_instance(b, cube, id[4], M);
+    trs(0.4f, -0.64f, 0.3f, -1, 0, 0.75f, 0.75f, 0.75f, M);
+    wsrt_scene_add_instance(b, sphere, id[5], M);
+    return WSRT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// camera (src/scene/camera.rs:178-194)
+// ---------------------------------------------------------------------------
+static void inverse4(const float* mf, float* out) {
+    // cofactor expansion in f64, rounded to f32
+    double m[16], inv[16];
+    for (int i = 0; i < 16; ++i) m[i] = mf[i];
+    auto cof = [&](int r, int c) {
+        double a[9];
+        int k = 0;
+        for (int cc = 0; cc < 4; ++cc) {
+            if (cc == c) continue;
+            for (int rr = 0; rr < 4; ++rr) {
+                if (rr == r) continue;
+                a[k++] = m[cc * 4 + rr];
+            }
+        }
+        double det3 = a[0] * (a[4] * a[8] - a[7] * a[5]) - a[3] * (a[1] * a[8] - a[7] * a[2]) +
+                      a[6] * (a[1] * a[5] - a[4] * a[2]);
+        return ((r + c) & 1) ? -det3 : det3;
+    };
+    double det = 0.0;
+    for (int c = 0; c < 4; ++c) det += m[c * 4 + 0] * cof(0, c);
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) inv[r * 4 + c] = cof(r, c) / det;  // transpose of cofactors
+    for (int i = 0; i < 16; ++i) out[i] = (float)inv[i];
+}
+
+extern "C" int wsrt_camera_build_uniform(const float position[3], float yaw, float pitch,
+                                         float aspect, uint32_t frame_count,
+                                         wsrt_camera_uniform* out) {
+    CHECK_ARG(position && out && aspect > 0.0f);
+    float sy = sinf(yaw), cy = cosf(yaw), sp = sinf(pitch), cp = cosf(pitch);
+    float fw[3] = {cp * cy, sp, cp * sy};
+    float il = 1.0f / sqrtf(fw[0] * fw[0] + fw[1] * fw[1] + fw[2] * fw[2]);
+    for (float& x : fw) x = x * il;
+    const float* eye = position;
+    float f[3] = {(eye[0] + fw[0]) - eye[0], (eye[1] + fw[1]) - eye[1], (eye[2] + fw[2]) - eye[2]};
+    il = 1.0f / sqrtf(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    for (float& x : f) x = x * il;
+    // s = f x up, u = s x f (up = +Y)
+    float s[3] = {f[1] * 0.0f - f[2] * 1.0f, f[2] * 0.0f - f[0] * 0.0f, f[0] * 1.0f - f[1] * 0.0f};
+    il = 1.0f / sqrtf(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    for (float& x : s) x = x * il;
+    float u[3] = {s[1] * f[2] - s[2] * f[1], s[2] * f[0] - s[0] * f[2], s[0] * f[1] - s[1] * f[0]};
+    auto dot3 = [](const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    float view[16] = {s[0], u[0], -f[0], 0, s[1], u[1], -f[1], 0, s[2], u[2], -f[2], 0,
+                      -dot3(eye, s), -dot3(eye, u), dot3(eye, f), 1};
+    float fov = 45.0f * (3.14159265359f / 180.0f);
+    float sf = sinf(0.5f * fov), cf = cosf(0.5f * fov);
+    float h = cf / sf, w = h / aspect, r = 100.0f / (0.1f - 100.0f);
+    float proj[16] = {w, 0, 0, 0, 0, h, 0, 0, 0, 0, r, -1, 0, 0, r * 0.1f, 0};
+    memcpy(out->view, view, sizeof view);
+    memcpy(out->proj, proj, sizeof proj);
+    inverse4(view, out->view_inverse);
+    inverse4(proj, out->proj_inverse);
+    out->frame_count = frame_count;
+    out->pad[0] = out->pad[1] = out->pad[2] = 0;
+    return WSRT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// config
+// ---------------------------------------------------------------------------
+extern "C" int wsrt_config_default(wsrt_config* c) {
+    CHECK_ARG(c);
+    memset(c, 0, sizeof *c);
+    c->struct_size = sizeof(wsrt_config);
+    c->max_depth = 12;
+    c->photon_max_depth = 5;
+    c->photons_per_pass = 1u << 20;
+    c->hash_size = 1u << 22;
+    c->cell_size = 0.02f;
+    c->gather_radius = 0.02f;
+    c->gather_radius_exponent = -0.15f;
+    c->spectral_radius = 20.0f;
+    c->gather_cap = 64;
+    c->rr_after = 3;
+    c->enable_photons = 1;
+    c->photon_norm = 1u << 20;
+    c->guiding = 0;
+    c->guide_mix = 0.5f;
+    c->guide_lr = 1e-3f;
+    c->guide_train_samples = 1u << 18;
+    c->env_map = 0;
+    return WSRT_OK;
+}
+
+static int validate_config(const wsrt_config* c) {
+    if (c->struct_size != sizeof(wsrt_config))
+        return fail(WSRT_E_INVALID, "wsrt_config.struct_size %u != %zu", c->struct_size, sizeof(wsrt_config));
+    if (c->max_depth == 0 || c->max_depth > 64) return fail(WSRT_E_INVALID, "max_depth %u", c->max_depth);
+    if (c->photon_max_depth > 64) return fail(WSRT_E_INVALID, "photon_max_depth %u", c->photon_max_depth);
+    if (c->photons_per_pass > (1u << 24))
+        return fail(WSRT_E_INVALID, "photons_per_pass %u > 2^24", c->photons_per_pass);
+    if (c->hash_size == 0 || (c->hash_size & (c->hash_size - 1)) || c->hash_size > (1u << 26))
+        return fail(WSRT_E_INVALID, "hash_size %u must be a power of two <= 2^26", c->hash_size);
+    if (!(c->cell_size > 0.0f) || !(c->gather_radius > 0.0f) || !(c->spectral_radius > 0.0f))
+        return fail(WSRT_E_INVALID, "cell_size, gather_radius and spectral_radius must be > 0");
+    if (c->photon_norm == 0) return fail(WSRT_E_INVALID, "photon_norm must be > 0");
+    if (c->guiding) return fail(WSRT_E_UNSUPPORTED, "guiding is not available in this build");
+    return WSRT_OK;
+}
+
+// Host-side constants of one frame (hero-wavelength CDF, gather radius and
+// density, grid jitter, camera unprojection matrix).
+static FrameConst make_frame_const(const wsrt_config& c, const wsrt_camera_uniform& cam, uint32_t W,
+                                   uint32_t H) {
+    FrameConst fc;
+    memset(&fc, 0, sizeof fc);
+    const float pi = 3.14159265359f;
+    float mn = (1.0f / pi) * atanf((380.0f - 555.0f) / 40.0f) + 0.5f;
+    float mx = (1.0f / pi) * atanf((780.0f - 555.0f) / 40.0f) + 0.5f;
+    fc.cdf_min = mn;
+    fc.cdf_range = mx - mn;
+    const uint32_t frame = cam.frame_count;
+    fc.radius = c.gather_radius * powf((float)(frame + 1u), c.gather_radius_exponent);
+    fc.radius_sq = fc.radius * fc.radius;
+    fc.density = (3.0f / (pi * fc.radius_sq)) * (1.0f / c.spectral_radius) * (1.0f / (float)c.photon_norm);
+    uint32_t state = (frame * 114514u) * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    uint32_t seed = (word >> 22u) ^ word;
+    float rx = (float)(seed & 0xFFu) / 255.0f, ry = (float)((seed >> 8u) & 0xFFu) / 255.0f,
+          rz = (float)((seed >> 16u) & 0xFFu) / 255.0f;
+    fc.jitter[0] = (rx - 0.5f) * c.cell_size;
+    fc.jitter[1] = (ry - 0.5f) * c.cell_size;
+    fc.jitter[2] = (rz - 0.5f) * c.cell_size;
+    fc.spectral_radius = c.spectral_radius;
+    fc.cell_size = c.cell_size;
+    fc.hash_size = c.hash_size;
+    fc.gather_cap = c.gather_cap;
+    fc.frame = frame;
+    fc.width = W;
+    fc.height = H;
+    fc.max_depth = c.max_depth;
+    fc.photon_max_depth = c.photon_max_depth;
+    fc.rr_after = c.rr_after;
+    fc.enable_photons = c.enable_photons;
+    fc.n_photons = c.enable_photons ? c.photons_per_pass : 0u;
+    mat4_mul(cam.view_inverse, cam.proj_inverse, fc.M);
+    fc.origin[0] = cam.view_inverse[12];
+    fc.origin[1] = cam.view_inverse[13];
+    fc.origin[2] = cam.view_inverse[14];
+    return fc;
+}
+
+// ---------------------------------------------------------------------------
+// renderer
+// ---------------------------------------------------------------------------
+namespace {
+enum KernelId {
+    K_FRAME_PREP, K_PHOTON_TRACE, K_GRID_BUILD, K_GENERATE, K_EXTEND, K_SHADE, K_SHADOW,
+    K_BOUNCE_PREP, K_ACCUMULATE, K_COUNT
+};
+const char* kKernelNames[K_COUNT] = {"frame_prep", "photon_trace", "grid_build", "generate", "extend",
+                                     "shade", "shadow", "bounce_prep", "accumulate"};
+struct Pending {
+    int kid;
+    hipEvent_t a, b;
+};
+}  // namespace
+
+struct wsrt_renderer {
+    wsrt_device* dev = nullptr;
+    const wsrt_scene* scene = nullptr;
+    wsrt_config cfg{};
+    uint32_t W = 0, H = 0;
+    hipStream_t stream = nullptr;
+    wsrt_camera_uniform cam{};
+    bool have_cam = false;
+    uint32_t frame_count = 0;
+    // buffers
+    DevBuf photons, grid_count, grid_offset, grid_cell, grid_slot, grid_sorted, grid_nvalid, grid_temp;
+    DevBuf path_o, path_d, path_tp, path_L, path_wl, path_rng, path_flags, path_hit, q_a, q_b, shade_q,
+        shadow;
+    DevBuf accum, rgba, counters, gstack;
+    GridBuffers gb{};
+    PathLaunch pl{};
+    // profiling
+    bool profiling = false;
+    bool count_traversal = false;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<Pending> pending;
+    double k_ms[K_COUNT] = {0};
+    uint64_t k_launches[K_COUNT] = {0};
+    uint64_t frames = 0, samples = 0, photon_paths = 0;
+};
+
+namespace {
+
+hipEvent_t take_event(wsrt_renderer* r) {
+    if (!r->event_pool.empty()) {
+        hipEvent_t e = r->event_pool.back();
+        r->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+struct ProfScope {
+    wsrt_renderer* r;
+    int kid;
+    hipEvent_t a = nullptr;
+    ProfScope(wsrt_renderer* r_, int k) : r(r_), kid(k) {
+        if (r->profiling) {
+            a = take_event(r);
+            (void)hipEventRecord(a, r->stream);
+        }
+    }
+    ~ProfScope() {
+        if (r->profiling && a) {
+            hipEvent_t b = take_event(r);
+            (void)hipEventRecord(b, r->stream);
+            r->pending.push_back({kid, a, b});
+        }
+    }
+};
+
+void drain_profile(wsrt_renderer* r) {
+    for (const Pending& p : r->pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            r->k_ms[p.kid] += ms;
+            r->k_launches[p.kid]++;
+        }
+        r->event_pool.push_back(p.a);
+        r->event_pool.push_back(p.b);
+    }
+    r->pending.clear();
+}
+
+}  // namespace
+
+extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, uint32_t width,
+                                    uint32_t height, const wsrt_config* cfg, wsrt_renderer** out) {
+    CHECK_ARG(dev && scene && out && width > 0 && height > 0);
+    *out = nullptr;
+    if ((uint64_t)width * height > (1ull << 26)) return fail(WSRT_E_INVALID, "resolution %ux%u too large", width, height);
+    if (scene->dev != dev) return fail(WSRT_E_INVALID, "scene was built on another device");
+    wsrt_config c;
+    if (cfg) c = *cfg;
+    else wsrt_config_default(&c);
+    int rc = validate_config(&c);
+    if (rc) return rc;
+    HIP_OR_FAIL(hipSetDevice(dev->ordinal));
+    auto r = std::make_unique<wsrt_renderer>();
+    r->dev = dev;
+    r->scene = scene;
+    r->cfg = c;
+    r->W = width;
+    r->H = height;
+    HIP_OR_FAIL(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+    const uint64_t npx = (uint64_t)width * height;
+    const uint32_t np = c.enable_photons ? c.photons_per_pass : 0u;
+    const uint32_t np1 = np ? np : 1u;
+    HIP_OR_FAIL(r->photons.alloc(sizeof(wsrt_photon) * np1));
+    HIP_OR_FAIL(r->grid_count.alloc(sizeof(uint32_t) * c.hash_size));
+    HIP_OR_FAIL(r->grid_offset.alloc(sizeof(uint32_t) * c.hash_size));
+    HIP_OR_FAIL(r->grid_cell.alloc(sizeof(uint32_t) * np1));
+    HIP_OR_FAIL(r->grid_slot.alloc(sizeof(uint32_t) * np1));
+    HIP_OR_FAIL(r->grid_sorted.alloc(sizeof(wsrt_photon) * np1));
+    HIP_OR_FAIL(r->grid_nvalid.alloc(sizeof(uint32_t)));
+    HIP_OR_FAIL(r->grid_temp.alloc(sizeof(uint32_t) * grid_temp_words(np1, c.hash_size)));
+    HIP_OR_FAIL(r->path_o.alloc(16 * npx));
+    HIP_OR_FAIL(r->path_d.alloc(16 * npx));
+    HIP_OR_FAIL(r->path_tp.alloc(16 * npx));
+    HIP_OR_FAIL(r->path_L.alloc(16 * npx));
+    HIP_OR_FAIL(r->path_wl.alloc(16 * npx));
+    HIP_OR_FAIL(r->path_rng.alloc(4 * npx));
+    HIP_OR_FAIL(r->path_flags.alloc(4 * npx));
+    HIP_OR_FAIL(r->path_hit.alloc(16 * npx));
+    HIP_OR_FAIL(r->q_a.alloc(4 * npx));
+    HIP_OR_FAIL(r->q_b.alloc(4 * npx));
+    HIP_OR_FAIL(r->shade_q.alloc(4 * npx * kMatBins));
+    HIP_OR_FAIL(r->shadow.alloc(48 * npx));
+    HIP_OR_FAIL(r->accum.alloc(16 * npx));
+    HIP_OR_FAIL(r->rgba.alloc(4 * npx));
+    HIP_OR_FAIL(r->counters.alloc(sizeof(Counters)));
+    const uint32_t spill = scene->view.stack_size > (uint32_t)kShortStack ? scene->view.stack_size - kShortStack : 0u;
+    HIP_OR_FAIL(r->gstack.alloc(sizeof(uint2) * kTraceLanes * (spill ? spill : 1u)));
+    HIP_OR_FAIL(hipMemsetAsync(r->counters.p, 0, sizeof(Counters), r->stream));
+    HIP_OR_FAIL(hipMemsetAsync(r->accum.p, 0, 16 * npx, r->stream));
+    HIP_OR_FAIL(hipMemsetAsync(r->photons.p, 0, sizeof(wsrt_photon) * np1, r->stream));
+    HIP_OR_FAIL(hipMemsetAsync(r->grid_count.p, 0, sizeof(uint32_t) * c.hash_size, r->stream));
+    HIP_OR_FAIL(hipMemsetAsync(r->grid_offset.p, 0, sizeof(uint32_t) * c.hash_size, r->stream));
+    HIP_OR_FAIL(hipMemsetAsync(r->grid_nvalid.p, 0, sizeof(uint32_t), r->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(r->stream));
+    r->gb.count = r->grid_count.as<uint32_t>();
+    r->gb.offset = r->grid_offset.as<uint32_t>();
+    r->gb.cell_id = r->grid_cell.as<uint32_t>();
+    r->gb.sorted_slot = r->grid_slot.as<uint32_t>();
+    r->gb.sorted = r->grid_sorted.as<wsrt_photon>();
+    r->gb.n_valid = r->grid_nvalid.as<uint32_t>();
+    r->gb.temp = r->grid_temp.as<uint32_t>();
+    PathLaunch& pl = r->pl;
+    pl.sv = scene->view;
+    pl.ps.o = r->path_o.as<float4>();
+    pl.ps.d = r->path_d.as<float4>();
+    pl.ps.tp = r->path_tp.as<float4>();
+    pl.ps.L = r->path_L.as<float4>();
+    pl.ps.wl = r->path_wl.as<float4>();
+    pl.ps.rng = r->path_rng.as<uint32_t>();
+    pl.ps.flags = r->path_flags.as<uint32_t>();
+    pl.ps.hit = r->path_hit.as<uint4>();
+    pl.ps.queue_a = r->q_a.as<uint32_t>();
+    pl.ps.queue_b = r->q_b.as<uint32_t>();
+    pl.ps.shade_q = r->shade_q.as<uint32_t>();
+    pl.ps.shadow = r->shadow.as<float4>();
+    pl.ps.capacity = (uint32_t)npx;
+    pl.grid.count = r->gb.count;
+    pl.grid.offset = r->gb.offset;
+    pl.grid.sorted = r->gb.sorted;
+    pl.grid.hash_size = c.hash_size;
+    pl.grid.cell_size = c.cell_size;
+    pl.ctr = r->counters.as<Counters>();
+    pl.gstack = r->gstack.as<uint2>();
+    pl.accum = r->accum.as<float4>();
+    pl.rgba = r->rgba.as<uint8_t>();
+    // default camera of CameraController::new (camera.rs:144-162)
+    const float pos[3] = {0.0f, 0.0f, 3.0f};
+    wsrt_camera_build_uniform(pos, -90.0f * (3.14159265359f / 180.0f), 0.0f, (float)width / (float)height,
+                              0, &r->cam);
+    *out = r.release();
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_renderer_destroy(wsrt_renderer* r) {
+    if (!r) return WSRT_OK;
+    (void)hipSetDevice(r->dev->ordinal);
+    (void)hipStreamSynchronize(r->stream);
+    for (const Pending& p : r->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (hipEvent_t e : r->event_pool) (void)hipEventDestroy(e);
+    hipStream_t s = r->stream;
+    delete r;
+    (void)hipStreamDestroy(s);
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_renderer_set_camera(wsrt_renderer* r, const wsrt_camera_uniform* cam) {
+    CHECK_ARG(r && cam);
+    r->cam = *cam;
+    r->have_cam = true;
+    return WSRT_OK;
+}
+
+static int render_one(wsrt_renderer* r, const wsrt_camera_uniform& cam) {
+    hipStream_t s = r->stream;
+    FrameConst fc = make_frame_const(r->cfg, cam, r->W, r->H);
+    fc.count_traversal = r->count_traversal ? 1u : 0u;
+    const PathLaunch& pl = r->pl;
+    const uint32_t npx = r->W * r->H;
+    {
+        ProfScope p(r, K_FRAME_PREP);
+        launch_frame_prep(pl.ctr, npx, s);
+    }
+    if (fc.n_photons) {
+        {
+            ProfScope p(r, K_PHOTON_TRACE);
+            launch_photon_trace(pl.sv, fc, r->photons.as<wsrt_photon>(), pl.ctr, pl.gstack, s);
+        }
+        ProfScope p(r, K_GRID_BUILD);
+        if (grid_build(r->photons.as<wsrt_photon>(), fc.n_photons, fc, r->gb, s))
+            return fail(WSRT_E_HIP, "grid build launch failed");
+    }
+    {
+        ProfScope p(r, K_GENERATE);
+        launch_generate(pl, fc, s);
+    }
+    for (uint32_t b = 0; b < fc.max_depth; ++b) {
+        uint32_t* q = (b & 1u) ? pl.ps.queue_b : pl.ps.queue_a;
+        uint32_t* nq = (b & 1u) ? pl.ps.queue_a : pl.ps.queue_b;
+        {
+            ProfScope p(r, K_EXTEND);
+            launch_extend(pl, fc, q, s);
+        }
+        {
+            ProfScope p(r, K_SHADE);
+            launch_shade(pl, fc, b, nq, s);
+        }
+        {
+            ProfScope p(r, K_SHADOW);
+            launch_shadow(pl, fc, s);
+        }
+        ProfScope p(r, K_BOUNCE_PREP);
+        launch_bounce_prep(pl.ctr, s);
+    }
+    {
+        ProfScope p(r, K_ACCUMULATE);
+        launch_accumulate(pl, fc, s);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(WSRT_E_HIP, "frame launch: %s", hipGetErrorString(e));
+    r->frames++;
+    r->samples += npx;
+    r->photon_paths += fc.n_photons;
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_renderer_render(wsrt_renderer* r) {
+    CHECK_ARG(r);
+    HIP_OR_FAIL(hipSetDevice(r->dev->ordinal));
+    int rc = render_one(r, r->cam);
+    if (rc) return rc;
+    r->frame_count++;  // mod.rs:204
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_renderer_render_frames(wsrt_renderer* r, uint32_t n) {
+    CHECK_ARG(r);
+    HIP_OR_FAIL(hipSetDevice(r->dev->ordinal));
+    for (uint32_t k = 0; k < n; ++k) {
+        wsrt_camera_uniform cam = r->cam;
+        cam.frame_count = r->frame_count;  // engine.rs:158-166
+        int rc = render_one(r, cam);
+        if (rc) return rc;
+        r->frame_count++;
+    }
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_renderer_get_frame_count(const wsrt_renderer* r, uint32_t* out) {
+    CHECK_ARG(r && out);
+    *out = r->frame_count;
+    return WSRT_OK;
+}
+extern "C" int wsrt_renderer_set_frame_count(wsrt_renderer* r, uint32_t fc) {
+    CHECK_ARG(r);
+    r->frame_count = fc;
+    return WSRT_OK;
+}
+extern "C" int wsrt_renderer_resize(wsrt_renderer* r, uint32_t ww, uint32_t wh) {
+    CHECK_ARG(r && ww > 0 && wh > 0);
+    return WSRT_OK;  // the internal resolution is fixed; frame_count is kept (mod.rs:149-166)
+}
+extern "C" int wsrt_renderer_sync(wsrt_renderer* r) {
+    CHECK_ARG(r);
+    HIP_OR_FAIL(hipSetDevice(r->dev->ordinal));
+    HIP_OR_FAIL(hipStreamSynchronize(r->stream));
+    return WSRT_OK;
+}
+
+static int read_back(wsrt_renderer* r, void* dst, const void* src, size_t bytes) {
+    HIP_OR_FAIL(hipSetDevice(r->dev->ordinal));
+    HIP_OR_FAIL(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, r->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(r->stream));
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_renderer_read_accum(wsrt_renderer* r, float* out) {
+    CHECK_ARG(r && out);
+    return read_back(r, out, r->accum.p, 16ull * r->W * r->H);
+}
+extern "C" int wsrt_renderer_read_rgba8(wsrt_renderer* r, uint8_t* out) {
+    CHECK_ARG(r && out);
+    return read_back(r, out, r->rgba.p, 4ull * r->W * r->H);
+}
+extern "C" int wsrt_renderer_write_accum(wsrt_renderer* r, const float* in) {
+    CHECK_ARG(r && in);
+    HIP_OR_FAIL(hipSetDevice(r->dev->ordinal));
+    HIP_OR_FAIL(hipMemcpyAsync(r->accum.p, in, 16ull * r->W * r->H, hipMemcpyHostToDevice, r->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(r->stream));
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_renderer_get_stats(wsrt_renderer* r, wsrt_frame_stats* out) {
+    CHECK_ARG(r && out);
+    Counters c;
+    int rc = read_back(r, &c, r->counters.p, sizeof c);
+    if (rc) return rc;
+    out->frames = r->frames;
+    out->camera_paths = r->samples;
+    out->extend_rays = c.extend_rays;
+    out->shadow_rays = c.shadow_rays;
+    out->photon_paths = r->photon_paths;
+    out->photon_rays = c.photon_rays;
+    out->photons_stored = c.photons_stored;
+    out->guide_samples = c.guide_samples;
+    for (int k = 0; k < 3; ++k) {
+        out->trav_nodes[k] = c.trav_nodes[k];
+        out->trav_tris[k] = c.trav_tris[k];
+        out->trav_rays[k] = c.trav_rays[k];
+    }
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_renderer_set_traversal_counting(wsrt_renderer* r, int enable) {
+    CHECK_ARG(r);
+    r->count_traversal = enable != 0;
+    return WSRT_OK;
+}
+extern "C" int wsrt_renderer_reset_stats(wsrt_renderer* r) {
+    CHECK_ARG(r);
+    HIP_OR_FAIL(hipSetDevice(r->dev->ordinal));
+    HIP_OR_FAIL(hipStreamSynchronize(r->stream));
+    Counters c;
+    HIP_OR_FAIL(hipMemcpy(&c, r->counters.p, sizeof c, hipMemcpyDeviceToHost));
+    c.extend_rays = c.shadow_rays = c.photon_rays = c.photons_stored = c.guide_samples = 0;
+    for (int k = 0; k < 3; ++k) c.trav_nodes[k] = c.trav_tris[k] = c.trav_rays[k] = 0;
+    HIP_OR_FAIL(hipMemcpy(r->counters.p, &c, sizeof c, hipMemcpyHostToDevice));
+    r->frames = r->samples = r->photon_paths = 0;
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_renderer_set_profiling(wsrt_renderer* r, int enable) {
+    CHECK_ARG(r);
+    r->profiling = enable != 0;
+    return WSRT_OK;
+}
+extern "C" int wsrt_renderer_get_profile(wsrt_renderer* r, wsrt_kernel_stat* out, uint32_t cap,
+                                         uint32_t* n) {
+    CHECK_ARG(r && n);
+    HIP_OR_FAIL(hipSetDevice(r->dev->ordinal));
+    HIP_OR_FAIL(hipStreamSynchronize(r->stream));
+    drain_profile(r);
+    *n = K_COUNT;
+    if (!out) return WSRT_OK;
+    for (uint32_t k = 0; k < K_COUNT && k < cap; ++k) {
+        memset(&out[k], 0, sizeof out[k]);
+        snprintf(out[k].name, sizeof out[k].name, "%s", kKernelNames[k]);
+        out[k].launches = r->k_launches[k];
+        out[k].total_ms = r->k_ms[k];
+    }
+    return WSRT_OK;
+}
+extern "C" int wsrt_renderer_reset_profile(wsrt_renderer* r) {
+    CHECK_ARG(r);
+    HIP_OR_FAIL(hipStreamSynchronize(r->stream));
+    drain_profile(r);
+    for (int k = 0; k < K_COUNT; ++k) {
+        r->k_ms[k] = 0;
+        r->k_launches[k] = 0;
+    }
+    return WSRT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// debug / parity taps
+// ---------------------------------------------------------------------------
+extern "C" int wsrt_debug_read_photons(wsrt_renderer* r, wsrt_photon* out, uint32_t cap,
+                                       uint32_t* n_stored) {
+    CHECK_ARG(r && out);
+    uint32_t np = r->cfg.enable_photons ? r->cfg.photons_per_pass : 0u;
+    if (cap < np) return fail(WSRT_E_INVALID, "photon buffer holds %u < %u", cap, np);
+    int rc = read_back(r, out, r->photons.p, sizeof(wsrt_photon) * np);
+    if (rc) return rc;
+    if (n_stored) {
+        uint32_t k = 0;
+        for (uint32_t i = 0; i < np; ++i) k += out[i].energy != 0.0f;
+        *n_stored = k;
+    }
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_debug_read_grid(wsrt_renderer* r, uint32_t* cell_id, uint32_t* count,
+                                    uint32_t* offset, uint32_t* sorted_slot) {
+    CHECK_ARG(r);
+    uint32_t np = r->cfg.enable_photons ? r->cfg.photons_per_pass : 0u;
+    int rc = WSRT_OK;
+    if (cell_id && !rc) rc = read_back(r, cell_id, r->grid_cell.p, 4ull * np);
+    if (count && !rc) rc = read_back(r, count, r->grid_count.p, 4ull * r->cfg.hash_size);
+    if (offset && !rc) rc = read_back(r, offset, r->grid_offset.p, 4ull * r->cfg.hash_size);
+    if (sorted_slot && !rc) rc = read_back(r, sorted_slot, r->grid_slot.p, 4ull * np);
+    return rc;
+}
+
+extern "C" int wsrt_debug_read_paths(wsrt_renderer* r, float* wl, float* L) {
+    CHECK_ARG(r);
+    int rc = WSRT_OK;
+    if (wl) rc = read_back(r, wl, r->path_wl.p, 16ull * r->W * r->H);
+    if (L && !rc) rc = read_back(r, L, r->path_L.p, 16ull * r->W * r->H);
+    return rc;
+}
+
+extern "C" int wsrt_photon_grid_build(wsrt_device* dev, const wsrt_photon* photons, uint32_t n,
+                                      uint32_t frame_count, float cell_size, uint32_t hash_size,
+                                      uint32_t* cell_id, uint32_t* count, uint32_t* offset,
+                                      uint32_t* sorted_slot) {
+    CHECK_ARG(dev && (photons || n == 0) && count && offset);
+    CHECK_ARG((cell_id && sorted_slot) || n == 0);
+    if (hash_size == 0 || (hash_size & (hash_size - 1)) || hash_size > (1u << 26))
+        return fail(WSRT_E_INVALID, "hash_size %u must be a power of two <= 2^26", hash_size);
+    if (!(cell_size > 0.0f)) return fail(WSRT_E_INVALID, "cell_size must be > 0");
+    if (n > (1u << 26)) return fail(WSRT_E_INVALID, "%u photons > 2^26", n);
+    HIP_OR_FAIL(hipSetDevice(dev->ordinal));
+    hipStream_t s = dev->stream;
+    wsrt_config c;
+    wsrt_config_default(&c);
+    c.cell_size = cell_size;
+    c.hash_size = hash_size;
+    wsrt_camera_uniform cam;
+    memset(&cam, 0, sizeof cam);
+    cam.frame_count = frame_count;
+    FrameConst fc = make_frame_const(c, cam, 1, 1);
+    const uint32_t n1 = n ? n : 1u;
+    DevBuf ph, cnt, off, cell, slot, sorted, nvalid, temp;
+    HIP_OR_FAIL(ph.alloc(sizeof(wsrt_photon) * n1));
+    HIP_OR_FAIL(cnt.alloc(4ull * hash_size));
+    HIP_OR_FAIL(off.alloc(4ull * hash_size));
+    HIP_OR_FAIL(cell.alloc(4ull * n1));
+    HIP_OR_FAIL(slot.alloc(4ull * n1));
+    HIP_OR_FAIL(sorted.alloc(sizeof(wsrt_photon) * n1));
+    HIP_OR_FAIL(nvalid.alloc(4));
+    HIP_OR_FAIL(temp.alloc(4 * grid_temp_words(n1, hash_size)));
+    if (n) HIP_OR_FAIL(hipMemcpyAsync(ph.p, photons, sizeof(wsrt_photon) * n, hipMemcpyHostToDevice, s));
+    GridBuffers g;
+    g.count = cnt.as<uint32_t>();
+    g.offset = off.as<uint32_t>();
+    g.cell_id = cell.as<uint32_t>();
+    g.sorted_slot = slot.as<uint32_t>();
+    g.sorted = sorted.as<wsrt_photon>();
+    g.n_valid = nvalid.as<uint32_t>();
+    g.temp = temp.as<uint32_t>();
+    HIP_OR_FAIL(hipMemsetAsync(g.n_valid, 0, 4, s));
+    if (grid_build(ph.as<wsrt_photon>(), n, fc, g, s)) return fail(WSRT_E_HIP, "grid build failed");
+    uint32_t valid = 0;
+    HIP_OR_FAIL(hipMemcpyAsync(count, g.count, 4ull * hash_size, hipMemcpyDeviceToHost, s));
+    HIP_OR_FAIL(hipMemcpyAsync(offset, g.offset, 4ull * hash_size, hipMemcpyDeviceToHost, s));
+    if (n) {
+        HIP_OR_FAIL(hipMemcpyAsync(cell_id, g.cell_id, 4ull * n, hipMemcpyDeviceToHost, s));
+        HIP_OR_FAIL(hipMemcpyAsync(sorted_slot, g.sorted_slot, 4ull * n, hipMemcpyDeviceToHost, s));
+        HIP_OR_FAIL(hipMemcpyAsync(&valid, g.n_valid, 4, hipMemcpyDeviceToHost, s));
+    }
+    HIP_OR_FAIL(hipStreamSynchronize(s));
+    // slots past the valid ones are not part of the grid
+    for (uint32_t k = valid; k < n; ++k) sorted_slot[k] = kInvalid;
+    return WSRT_OK;
+}
+
+static int scene_query(const wsrt_scene* s, const float* rays, uint32_t n, int mode, uint32_t* out0,
+                       size_t out0_words, uint32_t* out1) {
+    CHECK_ARG(s && (rays || n == 0) && (out0 || n == 0));
+    if (n == 0) return WSRT_OK;
+    if (n > (1u << 26)) return fail(WSRT_E_INVALID, "%u rays > 2^26", n);
+    HIP_OR_FAIL(hipSetDevice(s->dev->ordinal));
+    hipStream_t st = s->dev->stream;
+    DevBuf dr, d0, d1, gs;
+    HIP_OR_FAIL(dr.alloc(32ull * n));
+    HIP_OR_FAIL(d0.alloc(4ull * out0_words * n));
+    HIP_OR_FAIL(d1.alloc(4ull * n));
+    const uint32_t spill = s->view.stack_size > (uint32_t)kShortStack ? s->view.stack_size - kShortStack : 0u;
+    HIP_OR_FAIL(gs.alloc(sizeof(uint2) * kTraceLanes * (spill ? spill : 1u)));
+    HIP_OR_FAIL(hipMemcpyAsync(dr.p, rays, 32ull * n, hipMemcpyHostToDevice, st));
+    launch_query(s->view, dr.as<float>(), n, mode, d0.as<uint32_t>(), d1.as<uint32_t>(), gs.as<uint2>(), st);
+    HIP_OR_FAIL(hipGetLastError());
+    HIP_OR_FAIL(hipMemcpyAsync(out0, d0.p, 4ull * out0_words * n, hipMemcpyDeviceToHost, st));
+    if (out1) HIP_OR_FAIL(hipMemcpyAsync(out1, d1.p, 4ull * n, hipMemcpyDeviceToHost, st));
+    HIP_OR_FAIL(hipStreamSynchronize(st));
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_debug_closest_hit(const wsrt_scene* s, const float* rays, uint32_t n,
+                                      uint32_t* hits, uint32_t* flags) {
+    CHECK_ARG(flags || n == 0);
+    return scene_query(s, rays, n, 0, hits, 4, flags);
+}
+extern "C" int wsrt_debug_any_hit(const wsrt_scene* s, const float* rays, uint32_t n, uint32_t* occ) {
+    return scene_query(s, rays, n, 1, occ, 1, nullptr);
+}
+extern "C" int wsrt_debug_traversal_counts(const wsrt_scene* s, const float* rays, uint32_t n,
+                                           uint32_t* nodes, uint32_t* tris) {
+    CHECK_ARG(tris || n == 0);
+    return scene_query(s, rays, n, 2, nodes, 1, tris);
+}
+
+extern "C" int wsrt_debug_read_morton(const wsrt_scene* s, uint32_t* codes, uint32_t* sorted_tris) {
+    CHECK_ARG(s);
+    HIP_OR_FAIL(hipSetDevice(s->dev->ordinal));
+    size_t n = s->info.n_triangles;
+    if (codes && n) HIP_OR_FAIL(hipMemcpy(codes, s->codes.p, 4 * n, hipMemcpyDeviceToHost));
+    if (sorted_tris && n) HIP_OR_FAIL(hipMemcpy(sorted_tris, s->sorted.p, 4 * n, hipMemcpyDeviceToHost));
+    return WSRT_OK;
+}
+extern "C" int wsrt_debug_read_bvh(const wsrt_scene* s, void* nodes, void* tris) {
+    CHECK_ARG(s);
+    HIP_OR_FAIL(hipSetDevice(s->dev->ordinal));
+    if (nodes && s->nodes.bytes) HIP_OR_FAIL(hipMemcpy(nodes, s->nodes.p, s->nodes.bytes, hipMemcpyDeviceToHost));
+    if (tris && s->tris.bytes) HIP_OR_FAIL(hipMemcpy(tris, s->tris.p, s->tris.bytes, hipMemcpyDeviceToHost));
+    return WSRT_OK;
+}
+extern "C" int wsrt_debug_read_world(const wsrt_scene* s, float* tris, uint32_t* ids) {
+    CHECK_ARG(s);
+    HIP_OR_FAIL(hipSetDevice(s->dev->ordinal));
+    size_t n = s->info.n_triangles;
+    if (tris && n) HIP_OR_FAIL(hipMemcpy(tris, s->wv.p, 36 * n, hipMemcpyDeviceToHost));
+    if (ids && n) {
+        std::vector<uint32_t> ti(n);
+        std::vector<Instance> inst(s->info.n_instances);
+        HIP_OR_FAIL(hipMemcpy(ti.data(), s->tri_inst.p, 4 * n, hipMemcpyDeviceToHost));
+        HIP_OR_FAIL(hipMemcpy(inst.data(), s->instances.p, sizeof(Instance) * inst.size(), hipMemcpyDeviceToHost));
+        for (size_t t = 0; t < n; ++t) {
+            ids[3 * t] = ti[t];
+            ids[3 * t + 1] = inst[ti[t]].material;
+            ids[3 * t + 2] = (uint32_t)t - inst[ti[t]].first_tri;
+        }
+    }
+    return WSRT_OK;
+}

@@ -1,0 +1,93 @@
+// kernels.hpp - host-callable launchers of the HIP kernels of libwsrt.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "dev_math.hpp"
+
+namespace wsrt {
+
+struct FrameConst;
+
+// Traversal launches use a fixed grid of kTraceBlocks x kTraceThreads lanes
+// that loop over their work (so the per-lane HBM stack spill area has a fixed
+// size): 2048 x 128 lanes = 16 waves on each of the 256 CUs.
+constexpr int kTraceThreads = 128;
+constexpr int kTraceBlocks = 2048;
+constexpr uint64_t kTraceLanes = (uint64_t)kTraceThreads * kTraceBlocks;
+
+// ---- primitives (sort.hip) -------------------------------------------------
+size_t scan_temp_words(uint32_t n);
+int scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* temp,
+                   uint32_t* total_out, hipStream_t s);
+size_t radix_temp_words(uint32_t n);
+int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, uint32_t* temp,
+                     hipStream_t s, bool* result_in_temp);
+void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+void iota_u32(uint32_t* p, uint32_t n, hipStream_t s);
+
+// ---- BVH build (bvh.hip) ------------------------------------------------------
+struct BvhInputs {
+    const Instance* instances;
+    uint32_t n_instances;
+    const wsrt_vertex* verts;
+    const uint32_t* indices;
+    const wsrt_mesh_info* meshes;
+    uint32_t n_tris;
+};
+struct BvhOutputs {
+    wsrt_bvh8_node* nodes;
+    float4* tris;
+    float* wv;           // world vertices, 9 floats per triangle
+    uint32_t* tri_inst;  // world triangle -> instance
+    uint32_t* codes;     // Morton code per triangle
+    uint32_t* sorted;    // triangles in Morton order
+    uint32_t n_nodes;
+    uint32_t max_depth;
+    uint32_t placed_tris;
+};
+int bvh_build(const BvhInputs& in, BvhOutputs* out, hipStream_t s);
+
+// ---- ray queries (query.hip) ------------------------------------------------
+// rays: n x 8 floats (o, tmin, d, tmax). mode 0 closest, 1 any, 2 counts.
+void launch_query(const SceneView& sv, const float* rays, uint32_t n, int mode, uint32_t* out0,
+                  uint32_t* out1, uint2* gstack, hipStream_t s);
+
+// ---- photon pass + grid (photon.hip) ---------------------------------------
+struct GridBuffers {
+    uint32_t* count;        // hash_size
+    uint32_t* offset;       // hash_size
+    uint32_t* cell_id;      // n
+    uint32_t* sorted_slot;  // n
+    wsrt_photon* sorted;    // n
+    uint32_t* n_valid;      // 1
+    uint32_t* temp;         // grid_temp_words(n, hash_size)
+};
+void launch_photon_trace(const SceneView& sv, const FrameConst& fc, wsrt_photon* photons,
+                         Counters* ctr, uint2* gstack, hipStream_t s);
+size_t grid_temp_words(uint32_t n, uint32_t hash_size);
+int grid_build(const wsrt_photon* photons, uint32_t n, const FrameConst& fc, GridBuffers& g,
+               hipStream_t s);
+
+// ---- wavefront camera paths (path.hip) ----------------------------------------
+struct PathLaunch {
+    SceneView sv;
+    PathState ps;
+    GridView grid;
+    Counters* ctr;
+    uint2* gstack;
+    float4* accum;
+    uint8_t* rgba;
+};
+void launch_generate(const PathLaunch& pl, const FrameConst& fc, hipStream_t s);
+void launch_extend(const PathLaunch& pl, const FrameConst& fc, const uint32_t* queue,
+                   hipStream_t s);
+void launch_shade(const PathLaunch& pl, const FrameConst& fc, uint32_t bounce, uint32_t* next_queue,
+                  hipStream_t s);
+void launch_shadow(const PathLaunch& pl, const FrameConst& fc, hipStream_t s);
+void launch_bounce_prep(Counters* ctr, hipStream_t s);
+void launch_accumulate(const PathLaunch& pl, const FrameConst& fc, hipStream_t s);
+void launch_frame_prep(Counters* ctr, uint32_t n_paths, hipStream_t s);
+
+}  // namespace wsrt

@@ -1,0 +1,358 @@
+// path.hip - the camera pass of shader.wgsl:547-823 as wavefront kernels.
+//
+// The reference runs one megakernel thread per pixel. Here a frame is
+//   generate  -> [extend -> shade -> shadow] x max_depth -> accumulate
+// with the path state in HBM (structure of arrays indexed by pixel):
+//   extend  closest hit of every queued path; hits are binned by material
+//           type (sort-by-material) with wave-aggregated appends;
+//   shade   emission, photon gather, light sample, BSDF sample, roulette of
+//           each bin, so a wave mostly runs one material's code; surviving
+//           paths are compacted into the next queue;
+//   shadow  any-hit test of the deferred light samples, adding their
+//           contribution when unoccluded.
+// The per-path random stream is consumed in the reference's order (the
+// shadow test draws no numbers), and contributions are added to a path's
+// radiance in the reference's order (gather, light sample, then the next
+// bounce), so each path's result matches the per-pixel reference.
+#include "kernels.hpp"
+#include "shading.hpp"
+
+namespace wsrt {
+
+namespace {
+
+constexpr uint32_t kFlagDiffuse = 1u, kFlagCaustic = 2u;
+constexpr int kShadeThreads = 256;
+constexpr int kShadeBlocks = 1024;
+
+__global__ __launch_bounds__(256) void k_frame_prep(Counters* ctr, uint32_t n_paths) {
+    if (threadIdx.x == 0) {
+        ctr->active = n_paths;
+        ctr->next = 0;
+        ctr->shadow = 0;
+        ctr->stored = 0;
+        ctr->valid = 0;
+    }
+    if (threadIdx.x < (unsigned)kMatBins) ctr->shade[threadIdx.x] = 0;
+}
+
+__global__ void k_bounce_prep(Counters* ctr) {
+    if (threadIdx.x == 0) {
+        ctr->active = ctr->next;
+        ctr->next = 0;
+        ctr->shadow = 0;
+    }
+    if (threadIdx.x < (unsigned)kMatBins) ctr->shade[threadIdx.x] = 0;
+}
+
+// shader.wgsl:709-767
+__global__ __launch_bounds__(256) void k_generate(PathState ps, FrameConst fc) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = fc.width * fc.height;
+    if (p >= n) return;
+    const uint32_t x = p % fc.width, y = p / fc.width;
+    uint32_t rng = pcg_hash(x + y * fc.width + fc.frame * 927163u);
+    float jx = rand01(rng), jy = rand01(rng);
+    float ux = ((float)x + jx) / (float)fc.width, uy = ((float)y + jy) / (float)fc.height;
+    float v[4] = {ux * 2.0f - 1.0f, uy * 2.0f - 1.0f, 1.0f, 1.0f};
+    float tw[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float s = fc.M[0 + i] * v[0];
+        s = s + fc.M[4 + i] * v[1];
+        s = s + fc.M[8 + i] * v[2];
+        s = s + fc.M[12 + i] * v[3];
+        tw[i] = s;
+    }
+    f3 origin = mk3(fc.origin[0], fc.origin[1], fc.origin[2]);
+    f3 dir = normalize(mk3(tw[0] / tw[3], tw[1] / tw[3], tw[2] / tw[3]) - origin);
+    float u = rand01(rng);
+    float rot = rand01(rng);
+    const CauchyConst cc{fc.cdf_min, fc.cdf_range};
+    float hero = cauchy_sample(u, cc);
+    const float q = kLambdaRange / 4.0f;
+    float4 w = make_float4(hero, hero + q * 1.0f, hero + q * 2.0f, hero + q * 3.0f);
+    w.x = kLambdaMin + fmod_wgsl(w.x - kLambdaMin, kLambdaRange);
+    w.y = kLambdaMin + fmod_wgsl(w.y - kLambdaMin, kLambdaRange);
+    w.z = kLambdaMin + fmod_wgsl(w.z - kLambdaMin, kLambdaRange);
+    w.w = kLambdaMin + fmod_wgsl(w.w - kLambdaMin, kLambdaRange);
+    if (rot < 0.25f) {
+    } else if (rot < 0.5f) {
+        w = make_float4(w.y, w.z, w.w, w.x);
+    } else if (rot < 0.75f) {
+        w = make_float4(w.z, w.w, w.x, w.y);
+    } else {
+        w = make_float4(w.w, w.x, w.y, w.z);
+    }
+    ps.o[p] = f4(origin, 0.0f);
+    ps.d[p] = f4(dir, 0.0f);
+    ps.tp[p] = splat4(1.0f);
+    ps.L[p] = splat4(0.0f);
+    ps.wl[p] = w;
+    ps.rng[p] = rng;
+    ps.flags[p] = 0u;
+    ps.queue_a[p] = p;
+}
+
+__global__ __launch_bounds__(kTraceThreads) void k_extend(SceneView sv, PathState ps,
+                                                          uint32_t count_traversal,
+                                                          Counters* __restrict__ ctr,
+                                                          const uint32_t* __restrict__ queue,
+                                                          uint2* __restrict__ gstack) {
+    __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack st;
+    st.lds = lds_stack + threadIdx.x;
+    st.lds_stride = kTraceThreads;
+    st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
+    const uint32_t n = ctr->active;
+    if (gtid == 0) atomicAdd(&ctr->extend_rays, (unsigned long long)n);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // every lane of a wave runs the same number of iterations (wave-uniform
+    // trip count) so the binning ballots below see the whole wave
+    const uint32_t wave_base = gtid & ~63u;
+    const bool count = count_traversal != 0u;
+    uint32_t my_rays = 0, my_nodes = 0, my_tris = 0;
+    for (uint32_t base = wave_base; base < n; base += stride) {
+        const uint32_t i = base + __lane_id();
+        bool have = i < n;
+        uint32_t p = 0, bin = 0;
+        if (have) {
+            p = queue[i];
+            const float4 o = ps.o[p], d = ps.d[p];
+            Hit h = trace_ray<false>(sv, xyz(o), xyz(d), 0.0001f, 100.0f, st, count, my_nodes, my_tris);
+            my_rays++;
+            if (h.tri == kInvalid) {
+                have = false;  // miss: the path ends (shader.wgsl:566)
+            } else {
+                ps.hit[p] = make_uint4(__float_as_uint(h.t), h.tri | (h.front ? 0x80000000u : 0u),
+                                       __float_as_uint(h.u), __float_as_uint(h.v));
+                uint32_t mat = sv.instances[sv.tri_instance[h.tri]].material;
+                uint32_t type = (uint32_t)sv.materials[mat].extra[0];
+                bin = type < (uint32_t)kMatBins ? type : 0u;
+            }
+        }
+        // sort-by-material: wave-aggregated append into one queue per type
+        uint64_t pending = __ballot(have);
+        while (pending) {
+            const uint32_t leader = __ffsll((unsigned long long)pending) - 1;
+            const uint32_t b = __shfl(bin, leader);
+            const bool mine = have && bin == b;
+            const uint64_t m = __ballot(mine);
+            uint32_t slot = 0;
+            if (__lane_id() == leader) slot = atomicAdd(&ctr->shade[b], (uint32_t)__popcll(m));
+            slot = __shfl(slot, leader);
+            if (mine) {
+                ps.shade_q[(uint64_t)b * ps.capacity + slot + __popcll(m & lanemask_lt())] = p;
+                have = false;
+            }
+            pending &= ~m;
+        }
+    }
+    if (count) flush_traversal_counts(ctr, 1, my_rays, my_nodes, my_tris);
+}
+
+__global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState ps, GridView grid,
+                                                         FrameConst fc, Counters* __restrict__ ctr,
+                                                         uint32_t bounce,
+                                                         uint32_t* __restrict__ next_queue) {
+    uint32_t start[kMatBins + 1];
+    start[0] = 0;
+#pragma unroll
+    for (int b = 0; b < kMatBins; ++b) start[b + 1] = start[b] + ctr->shade[b];
+    const uint32_t total = start[kMatBins];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t wave_base = gtid & ~63u;
+    for (uint32_t base = wave_base; base < total; base += stride) {
+        const uint32_t i = base + __lane_id();
+        bool alive = false;
+        bool have = i < total;
+        uint32_t p = 0;
+        float4 o4 = splat4(0.0f), d4 = splat4(0.0f), tp = splat4(0.0f), L = splat4(0.0f);
+        uint32_t rng = 0, flags = 0;
+        bool shadow = false;
+        f3 hp = mk3(0, 0, 0), sdir = mk3(0, 0, 0);
+        float stmax = 0.0f;
+        float4 contrib = splat4(0.0f);
+        if (have) {
+            uint32_t b = 0;
+#pragma unroll
+            for (int k = 1; k < kMatBins; ++k) b += (i >= start[k]) ? 1u : 0u;
+            p = ps.shade_q[(uint64_t)b * ps.capacity + (i - start[b])];
+            const uint4 hr = ps.hit[p];
+            const float t = __uint_as_float(hr.x);
+            const uint32_t tri = hr.y & 0x7FFFFFFFu;
+            const bool front = (hr.y >> 31) != 0u;
+            o4 = ps.o[p];
+            d4 = ps.d[p];
+            tp = ps.tp[p];
+            L = ps.L[p];
+            const float4 wl = ps.wl[p];
+            rng = ps.rng[p];
+            flags = ps.flags[p];
+            uint32_t mat_id;
+            f3 wn = hit_world_normal(sv, tri, __uint_as_float(hr.z), __uint_as_float(hr.w), &mat_id);
+            f3 ff = front ? wn : -wn;
+            const wsrt_material& m = sv.materials[mat_id];
+            const float4 col = *reinterpret_cast<const float4*>(m.color);
+            const float4 ex = *reinterpret_cast<const float4*>(m.extra);
+            const uint32_t type = (uint32_t)ex.x;
+            const f3 o = xyz(o4), d = xyz(d4);
+            if (type == 3u) {  // emitter (shader.wgsl:586-631); the MIS weight is not applied
+                if (!(flags & kFlagCaustic)) {
+                    float4 em = mul4s(blackbody4(wl, col.x), col.y);
+                    if (front || ex.y > 0.0f) L = add4(L, mul4(em, tp));
+                }
+            } else {
+                if (type == 0u) {
+                    flags = (flags | kFlagDiffuse) & ~kFlagCaustic;
+                } else if (type == 2u) {
+                    if (flags & kFlagDiffuse) flags |= kFlagCaustic;
+                }
+                hp = o + d * t;
+                if (type == 0u) {
+                    float4 spec = rgb_to_spectrum4(mk3(col.x, col.y, col.z), wl);
+                    if (fc.enable_photons) {
+                        float4 g = gather_photons(grid, fc, hp, ff, wl);
+                        L = add4(L, mul4(mul4(tp, div4s(spec, kPi)), g));
+                    }
+                    LightSample ls = sample_light(sv, hp, wl, rng);
+                    if (ls.pdf > 0.0f) {
+                        float ct = fmaxf(dot(ff, ls.dir), 0.0f);
+                        float4 be = div4s(spec, kPi);
+                        float bp = ct / kPi;
+                        float w = ls.delta ? 1.0f / ls.pdf : (ls.pdf / (ls.pdf + bp)) / ls.pdf;
+                        contrib = mul4s(mul4s(mul4(mul4(tp, be), ls.radiance), ct), w);
+                        shadow = true;
+                        sdir = ls.dir;
+                        stmax = ls.dist - 0.001f;
+                    }
+                }
+                Scatter sc = sample_bsdf(d, ff, m, front, wl, tp, rng);
+                if (!sc.absorbed) {
+                    tp = mul4(tp, sc.att);
+                    o4 = f4(hp, 0.0f);
+                    d4 = f4(sc.dir, 0.0f);
+                    alive = true;
+                    if (bounce > fc.rr_after) {
+                        float pr = fmaxf(fmaxf(tp.x, tp.y), fmaxf(tp.z, tp.w));
+                        if (rand01(rng) > pr) alive = false;
+                        else tp = div4s(tp, pr);
+                    }
+                }
+            }
+            ps.L[p] = L;
+            if (alive) {
+                ps.o[p] = o4;
+                ps.d[p] = d4;
+                ps.tp[p] = tp;
+                ps.rng[p] = rng;
+                ps.flags[p] = flags;
+            }
+        }
+        uint32_t ss = wave_append(&ctr->shadow, shadow);
+        if (shadow) {
+            float4* r = ps.shadow + 3ull * ss;
+            r[0] = f4(hp, stmax);
+            r[1] = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(p));
+            r[2] = contrib;
+        }
+        uint32_t ns = wave_append(&ctr->next, alive);
+        if (alive) next_queue[ns] = p;
+    }
+}
+
+__global__ __launch_bounds__(kTraceThreads) void k_shadow(SceneView sv, PathState ps,
+                                                          uint32_t count_traversal,
+                                                          Counters* __restrict__ ctr,
+                                                          uint2* __restrict__ gstack) {
+    __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack st;
+    st.lds = lds_stack + threadIdx.x;
+    st.lds_stride = kTraceThreads;
+    st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
+    const uint32_t n = ctr->shadow;
+    if (gtid == 0) atomicAdd(&ctr->shadow_rays, (unsigned long long)n);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const bool count = count_traversal != 0u;
+    uint32_t my_rays = 0, my_nodes = 0, my_tris = 0;
+    for (uint32_t i = gtid; i < n; i += stride) {
+        const float4* r = ps.shadow + 3ull * i;
+        const float4 a = r[0], b = r[1];
+        Hit h = trace_ray<true>(sv, xyz(a), xyz(b), 0.001f, a.w, st, count, my_nodes, my_tris);
+        my_rays++;
+        if (h.tri == kInvalid) {
+            const uint32_t p = __float_as_uint(b.w);
+            ps.L[p] = add4(ps.L[p], r[2]);
+        }
+    }
+    if (count) flush_traversal_counts(ctr, 2, my_rays, my_nodes, my_tris);
+}
+
+// shader.wgsl:771-822
+__global__ __launch_bounds__(256) void k_accumulate(PathState ps, FrameConst fc,
+                                                    float4* __restrict__ accum,
+                                                    uint8_t* __restrict__ rgba) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= fc.width * fc.height) return;
+    const float4 wl = ps.wl[p], L = ps.L[p];
+    f3 xz = mk3(0, 0, 0);
+    xz = xz + cie_xyz(wl.x) * L.x;
+    xz = xz + cie_xyz(wl.y) * L.y;
+    xz = xz + cie_xyz(wl.z) * L.z;
+    xz = xz + cie_xyz(wl.w) * L.w;
+    const CauchyConst cc{fc.cdf_min, fc.cdf_range};
+    float sp = cauchy_pdf(wl.x, cc) + cauchy_pdf(wl.y, cc) + cauchy_pdf(wl.z, cc) + cauchy_pdf(wl.w, cc);
+    xz = xz * (1.0f / sp);
+    f3 c = xyz_to_srgb(xz);
+    float4 cur = fc.frame > 0u ? accum[p] : splat4(0.0f);
+    float4 na = add4(cur, make_float4(c.x, c.y, c.z, 1.0f));
+    accum[p] = na;
+    f3 fcol = mk3(fmaxf(na.x / na.w * 1.0f, 0.0f), fmaxf(na.y / na.w * 1.0f, 0.0f),
+                  fmaxf(na.z / na.w * 1.0f, 0.0f));
+    float luma = dot(fcol, mk3(0.2126f, 0.7152f, 0.0722f));
+    float ml = luma / (luma + 1.0f);
+    if (luma > 0.00001f) fcol = fcol * (ml / luma);
+    uchar4 o;
+    o.x = (uint8_t)__float2int_rn(fminf(fmaxf(fcol.x, 0.0f), 1.0f) * 255.0f);
+    o.y = (uint8_t)__float2int_rn(fminf(fmaxf(fcol.y, 0.0f), 1.0f) * 255.0f);
+    o.z = (uint8_t)__float2int_rn(fminf(fmaxf(fcol.z, 0.0f), 1.0f) * 255.0f);
+    o.w = 255;
+    reinterpret_cast<uchar4*>(rgba)[p] = o;
+}
+
+}  // namespace
+
+void launch_frame_prep(Counters* ctr, uint32_t n_paths, hipStream_t s) {
+    hipLaunchKernelGGL(k_frame_prep, dim3(1), dim3(64), 0, s, ctr, n_paths);
+}
+void launch_bounce_prep(Counters* ctr, hipStream_t s) {
+    hipLaunchKernelGGL(k_bounce_prep, dim3(1), dim3(64), 0, s, ctr);
+}
+void launch_generate(const PathLaunch& pl, const FrameConst& fc, hipStream_t s) {
+    uint32_t n = fc.width * fc.height;
+    hipLaunchKernelGGL(k_generate, dim3((n + 255) / 256), dim3(256), 0, s, pl.ps, fc);
+}
+void launch_extend(const PathLaunch& pl, const FrameConst& fc, const uint32_t* queue,
+                   hipStream_t s) {
+    hipLaunchKernelGGL(k_extend, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, pl.sv, pl.ps,
+                       fc.count_traversal, pl.ctr, queue, pl.gstack);
+}
+void launch_shade(const PathLaunch& pl, const FrameConst& fc, uint32_t bounce, uint32_t* next_queue,
+                  hipStream_t s) {
+    hipLaunchKernelGGL(k_shade, dim3(kShadeBlocks), dim3(kShadeThreads), 0, s, pl.sv, pl.ps, pl.grid,
+                       fc, pl.ctr, bounce, next_queue);
+}
+void launch_shadow(const PathLaunch& pl, const FrameConst& fc, hipStream_t s) {
+    hipLaunchKernelGGL(k_shadow, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, pl.sv, pl.ps,
+                       fc.count_traversal, pl.ctr, pl.gstack);
+}
+void launch_accumulate(const PathLaunch& pl, const FrameConst& fc, hipStream_t s) {
+    uint32_t n = fc.width * fc.height;
+    hipLaunchKernelGGL(k_accumulate, dim3((n + 255) / 256), dim3(256), 0, s, pl.ps, fc, pl.accum,
+                       pl.rgba);
+}
+
+}  // namespace wsrt

@@ -1,0 +1,281 @@
+// sort.hip - device-wide primitives for 64-wide CDNA4 wavefronts:
+//  * exclusive prefix scan of uint32 arrays (reduce -> scan of block sums ->
+//    downsweep), with wave-level inclusive scans inside each block;
+//  * stable LSD radix sort of (key, value) uint32 pairs, 8 bits per pass:
+//    per-tile digit histograms, a scan of the digit-major histogram matrix,
+//    and a scatter that ranks equal digits inside a wave with ballots, so the
+//    order of equal keys is the input order.
+#include "kernels.hpp"
+
+namespace wsrt {
+
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;  // 4096
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t n = __shfl_up(v, off, 64);
+        if (lane >= off) v += n;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_reduce(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the block total.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& total,
+                                                         uint32_t* smem /* >= 16 */) {
+    const int lane = __lane_id();
+    const int wave = threadIdx.x >> 6;
+    const int nwaves = blockDim.x >> 6;
+    uint32_t inc = wave_inclusive_scan(v);
+    if (lane == 63) smem[wave] = inc;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        uint32_t w = threadIdx.x < (unsigned)nwaves ? smem[threadIdx.x] : 0u;
+        uint32_t wi = wave_inclusive_scan(w);
+        if (threadIdx.x < (unsigned)nwaves) smem[16 + threadIdx.x] = wi - w;
+        if (threadIdx.x == (unsigned)nwaves - 1) smem[32] = wi;
+    }
+    __syncthreads();
+    uint32_t res = smem[16 + wave] + inc - v;
+    total = smem[32];
+    __syncthreads();
+    return res;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ in,
+                                                              uint32_t n,
+                                                              uint32_t* __restrict__ block_sums) {
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        uint64_t i = base + (uint64_t)k * kScanThreads + threadIdx.x;
+        s += i < n ? in[i] : 0u;
+    }
+    s = wave_reduce(s);
+    __shared__ uint32_t part[kScanThreads / 64];
+    if (__lane_id() == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kScanThreads / 64; ++w) t += part[w];
+        block_sums[blockIdx.x] = t;
+    }
+}
+
+// Single-block exclusive scan of up to 1024*16 block sums, in place.
+__global__ __launch_bounds__(1024) void k_scan_block_sums(uint32_t* __restrict__ sums, uint32_t n,
+                                                          uint32_t* __restrict__ total_out) {
+    __shared__ uint32_t smem[64];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += blockDim.x) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t v = i < n ? sums[i] : 0u;
+        uint32_t total;
+        uint32_t ex = block_exclusive_scan(v, total, smem);
+        uint32_t c = carry;
+        if (i < n) sums[i] = ex + c;
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c + total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_downsweep(const uint32_t* __restrict__ in,
+                                                                 uint32_t* __restrict__ out,
+                                                                 uint32_t n,
+                                                                 const uint32_t* __restrict__ block_sums) {
+    __shared__ uint32_t smem[64];
+    // each thread owns kScanItems consecutive elements
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        uint64_t i = base + k;
+        v[k] = i < n ? in[i] : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t ex = block_exclusive_scan(s, total, smem) + block_sums[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        uint64_t i = base + k;
+        if (i < n) out[i] = ex;
+        ex += v[k];
+    }
+}
+
+// ---- radix sort ---------------------------------------------------------------
+constexpr int kSortThreads = 256;
+constexpr int kSortRounds = 16;
+constexpr int kSortTile = kSortThreads * kSortRounds;  // 4096 keys per tile
+constexpr int kRadix = 256;
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_histogram(const uint32_t* __restrict__ keys,
+                                                                  uint32_t n, int shift,
+                                                                  uint32_t num_tiles,
+                                                                  uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[kRadix];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+#pragma unroll 4
+    for (int r = 0; r < kSortRounds; ++r) {
+        uint64_t i = base + (uint64_t)r * kSortThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    // digit-major layout: hist[digit * num_tiles + tile]
+    hist[(uint64_t)threadIdx.x * num_tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter of one tile. Keys are taken in input order 256 at a time;
+// equal digits inside a wave are ranked with 8 ballots (match-any), waves of
+// a round are ordered by per-digit counts in LDS, rounds by a running count.
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
+    const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, int shift,
+    uint32_t num_tiles, const uint32_t* __restrict__ hist_scanned) {
+    constexpr int kWaves = kSortThreads / 64;
+    __shared__ uint32_t run[kRadix];
+    __shared__ uint32_t wave_cnt[kWaves][kRadix];
+    run[threadIdx.x] = hist_scanned[(uint64_t)threadIdx.x * num_tiles + blockIdx.x];
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) wave_cnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    const int wave = threadIdx.x >> 6;
+    const uint64_t lt = lanemask_lt();
+    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortRounds; ++r) {
+        uint64_t i = base + (uint64_t)r * kSortThreads + threadIdx.x;
+        bool valid = i < n;
+        uint32_t key = valid ? keys_in[i] : 0u;
+        uint32_t val = valid ? vals_in[i] : 0u;
+        uint32_t dig = (key >> shift) & 0xFFu;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            uint64_t m = __ballot((dig >> b) & 1u);
+            peers &= ((dig >> b) & 1u) ? m : ~m;
+        }
+        uint32_t rank = (uint32_t)__popcll(peers & lt);
+        bool leader = valid && (peers & lt) == 0;
+        if (leader) wave_cnt[wave][dig] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[dig] + rank;
+            for (int w = 0; w < wave; ++w) pos += wave_cnt[w][dig];
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+        }
+        __syncthreads();
+        {
+            uint32_t add = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                add += wave_cnt[w][threadIdx.x];
+                wave_cnt[w][threadIdx.x] = 0;
+            }
+            run[threadIdx.x] += add;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint4 v4 = make_uint4(v, v, v, v);
+    uint64_t n4 = n / 4;
+    uint4* p4 = reinterpret_cast<uint4*>(p);
+    for (uint64_t k = i; k < n4; k += stride) p4[k] = v4;
+    for (uint64_t k = n4 * 4 + i; k < n; k += stride) p[k] = v;
+}
+
+__global__ void k_iota(uint32_t* __restrict__ p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+}  // namespace
+
+size_t scan_temp_words(uint32_t n) {
+    uint32_t blocks = (n + kScanTile - 1) / kScanTile;
+    return (size_t)blocks + 1;
+}
+
+int scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* temp,
+                   uint32_t* total_out, hipStream_t s) {
+    if (n == 0) return 0;
+    uint32_t blocks = (n + kScanTile - 1) / kScanTile;
+    if (blocks > 1024u * 64u) return -1;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(blocks), dim3(kScanThreads), 0, s, in, n, temp);
+    hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(1024), 0, s, temp, blocks, total_out);
+    hipLaunchKernelGGL(k_scan_downsweep, dim3(blocks), dim3(kScanThreads), 0, s, in, out, n, temp);
+    return 0;
+}
+
+size_t radix_temp_words(uint32_t n) {
+    uint32_t tiles = (n + kSortTile - 1) / kSortTile;
+    size_t hist = (size_t)tiles * kRadix;
+    return 2 * (size_t)n + 2 * hist + scan_temp_words((uint32_t)hist);
+}
+
+int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, uint32_t* temp,
+                     hipStream_t s, bool* result_in_temp) {
+    // temp layout: keys_alt[n], vals_alt[n], hist[tiles*256], hist_scan[tiles*256], scan temp
+    *result_in_temp = false;
+    if (n <= 1) return 0;
+    uint32_t tiles = (n + kSortTile - 1) / kSortTile;
+    uint32_t hist_n = tiles * kRadix;
+    uint32_t* kalt = temp;
+    uint32_t* valt = temp + n;
+    uint32_t* hist = temp + 2 * (size_t)n;
+    uint32_t* hscan = hist + hist_n;
+    uint32_t* stemp = hscan + hist_n;
+    uint32_t *ki = keys, *vi = vals, *ko = kalt, *vo = valt;
+    int passes = (bits + 7) / 8;
+    for (int p = 0; p < passes; ++p) {
+        int shift = 8 * p;
+        hipLaunchKernelGGL(k_radix_histogram, dim3(tiles), dim3(kSortThreads), 0, s, ki, n, shift,
+                           tiles, hist);
+        if (scan_exclusive(hist, hscan, hist_n, stemp, nullptr, s)) return -1;
+        hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kSortThreads), 0, s, ki, vi, ko, vo,
+                           n, shift, tiles, hscan);
+        uint32_t* t;
+        t = ki; ki = ko; ko = t;
+        t = vi; vi = vo; vo = t;
+    }
+    *result_in_temp = (ki != keys);
+    return 0;
+}
+
+void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
+    if (n == 0) return;
+    uint64_t blocks = (n / 4 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, s, p, n, v);
+}
+
+void iota_u32(uint32_t* p, uint32_t n, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+}
+
+}  // namespace wsrt

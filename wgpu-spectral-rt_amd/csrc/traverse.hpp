@@ -1,0 +1,218 @@
+// traverse.hpp - software ray queries over the compressed BVH8 (CDNA4 has no
+// ray-tracing units). Replaces rayQueryProceed of shader.wgsl:561-566,654-659
+// and photon_emit.wgsl:222-227.
+//
+// Semantics (kept identical in the CPU oracle):
+//  * closest hit on (tmin, tmax), ties broken by the lower world triangle id;
+//  * triangle test Moller-Trumbore, no culling; front face <=> det > 0, i.e.
+//    dot(dir, cross(v1 - v0, v2 - v0)) < 0 (the DXR/Vulkan default winding);
+//  * any hit terminates on the first accepted triangle.
+// Node boxes are tested with fused slab values plus a bound of their f32
+// error, so culling never rejects a box the exact box would accept; the
+// visiting order (leaf children first, internal children nearest-first by
+// entry distance, ties by slot) is what the oracle's BVH8 traversal counts.
+#pragma once
+
+#include "common.hpp"
+#include "dev_math.hpp"
+
+namespace wsrt {
+
+constexpr float kBoxPad = 1.00001f;
+constexpr int kShortStack = 16;  // entries per lane in LDS; deeper ones spill to HBM
+
+struct Hit {
+    float t;
+    uint32_t tri;  // kInvalid on miss
+    float u, v;
+    bool front;
+};
+
+__device__ __forceinline__ float safe_inv(float d) {
+    if (fabsf(d) < 1e-20f) d = d < 0.0f ? -1e-20f : 1e-20f;
+    return 1.0f / d;
+}
+
+__device__ __forceinline__ bool tri_intersect(f3 o, f3 d, f3 v0, f3 e1, f3 e2, float& t, float& u,
+                                              float& v, bool& front) {
+    f3 pvec = cross(d, e2);
+    float det = dot(e1, pvec);
+    if (det == 0.0f) return false;
+    float inv = 1.0f / det;
+    f3 tvec = o - v0;
+    float uu = dot(tvec, pvec) * inv;
+    if (uu < 0.0f || uu > 1.0f) return false;
+    f3 qvec = cross(tvec, e1);
+    float vv = dot(d, qvec) * inv;
+    if (vv < 0.0f || uu + vv > 1.0f) return false;
+    t = dot(e2, qvec) * inv;
+    u = uu;
+    v = vv;
+    front = det > 0.0f;
+    return true;
+}
+
+// Traversal stack: entries (node, tnear bits). The first kShortStack entries
+// of a lane live in LDS at lds[k * lds_stride]; deeper entries spill to the
+// lane's private HBM region gstack[0 .. stack_size - kShortStack).
+struct Stack {
+    uint2* lds;
+    uint32_t lds_stride;
+    uint2* glob;
+    __device__ __forceinline__ void put(int k, uint2 e) {
+        if (k < kShortStack) lds[k * lds_stride] = e;
+        else glob[k - kShortStack] = e;
+    }
+    __device__ __forceinline__ uint2 get(int k) {
+        return k < kShortStack ? lds[k * lds_stride] : glob[k - kShortStack];
+    }
+};
+
+__device__ __forceinline__ uint32_t byte_of(uint32_t lo, uint32_t hi, int c) {
+    return ((c < 4 ? lo : hi) >> (8 * (c & 3))) & 0xFFu;
+}
+
+template <bool kAny, bool kCount>
+__device__ Hit traverse(const SceneView& sv, f3 o, f3 d, float tmin, float tmax, Stack st,
+                        uint32_t* n_nodes_visited = nullptr, uint32_t* n_tris_tested = nullptr) {
+    Hit best;
+    best.t = tmax;
+    best.tri = kInvalid;
+    best.u = best.v = 0.0f;
+    best.front = false;
+    if (sv.n_nodes == 0) return best;
+    const float inv[3] = {safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
+    const float oo[3] = {o.x, o.y, o.z};
+    uint32_t nvis = 0, ntest = 0;
+    int sp = 0;
+    st.put(sp++, make_uint2(0u, __float_as_uint(tmin)));
+    const uint4* nodes = reinterpret_cast<const uint4*>(sv.nodes);
+    while (sp > 0) {
+        uint2 e = st.get(--sp);
+        if (!kAny && __uint_as_float(e.y) > best.t * kBoxPad) continue;
+        const uint4* np = nodes + 5ull * e.x;
+        uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
+        if (kCount) nvis++;
+        float as[3], ao[3], sl[3];
+        const float org[3] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            int ex = (int)(int8_t)((w0.w >> (8 * a)) & 0xFFu);
+            float s = __uint_as_float((uint32_t)(ex + 127) << 23);
+            as[a] = s * inv[a];
+            ao[a] = (org[a] - oo[a]) * inv[a];
+            sl[a] = (fabsf(as[a]) * 255.0f + fabsf(ao[a])) * 4.0e-7f;
+        }
+        const uint32_t child_base = w1.x, tri_base = w1.y;
+        // quantized bytes: qlo x/y/z = (w2.xy, w2.zw, w3.xy), qhi x/y/z = (w3.zw, w4.xy, w4.zw)
+        const uint32_t qlo_lo[3] = {w2.x, w2.z, w3.x}, qlo_hi[3] = {w2.y, w2.w, w3.y};
+        const uint32_t qhi_lo[3] = {w3.z, w4.x, w4.z}, qhi_hi[3] = {w3.w, w4.y, w4.w};
+        float cnear[8];
+        uint32_t hitmask = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            uint32_t meta = byte_of(w1.z, w1.w, c);
+            float tn = tmin, tf = best.t;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                float t0 = __fmaf_rn((float)byte_of(qlo_lo[a], qlo_hi[a], c), as[a], ao[a]);
+                float t1 = __fmaf_rn((float)byte_of(qhi_lo[a], qhi_hi[a], c), as[a], ao[a]);
+                tn = fmaxf(tn, fminf(t0, t1) - sl[a]);
+                tf = fminf(tf, fmaxf(t0, t1) + sl[a]);
+            }
+            cnear[c] = tn;
+            if (meta != 0u && tn <= tf * kBoxPad) hitmask |= 1u << c;
+        }
+        // leaf children, in slot order
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            uint32_t meta = byte_of(w1.z, w1.w, c);
+            if (!((hitmask >> c) & 1u) || (meta & 0x80u)) continue;
+            uint32_t cnt = meta >> 5, off = meta & 0x1Fu;
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const float4* tr = sv.tris + 3ull * (tri_base + off + k);
+                float4 a = tr[0], b = tr[1], cc = tr[2];
+                if (kCount) ntest++;
+                float tt, u, v;
+                bool fr;
+                if (!tri_intersect(o, d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z),
+                                   tt, u, v, fr))
+                    continue;
+                if (!(tt > tmin)) continue;
+                uint32_t id = __float_as_uint(a.w);
+                if (tt < best.t || (tt == best.t && best.tri != kInvalid && id < best.tri)) {
+                    best.t = tt;
+                    best.tri = id;
+                    best.u = u;
+                    best.v = v;
+                    best.front = fr;
+                    if (kAny) {
+                        if (kCount) { *n_nodes_visited = nvis; *n_tris_tested = ntest; }
+                        return best;
+                    }
+                }
+            }
+        }
+        // internal children: nearest on top of the stack
+        uint32_t imask = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            uint32_t meta = byte_of(w1.z, w1.w, c);
+            if (((hitmask >> c) & 1u) && (meta & 0x80u) && !(cnear[c] > best.t * kBoxPad))
+                imask |= 1u << c;
+        }
+        int m = __popc(imask);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if (!((imask >> c) & 1u)) continue;
+            int rank = 0;
+#pragma unroll
+            for (int c2 = 0; c2 < 8; ++c2) {
+                bool before = ((imask >> c2) & 1u) &&
+                              (cnear[c2] < cnear[c] || (cnear[c2] == cnear[c] && c2 < c));
+                rank += before ? 1 : 0;
+            }
+            uint32_t meta = byte_of(w1.z, w1.w, c);
+            st.put(sp + (m - 1 - rank), make_uint2(child_base + (meta & 0x7Fu), __float_as_uint(cnear[c])));
+        }
+        sp += m;
+    }
+    if (kCount) {
+        *n_nodes_visited = nvis;
+        *n_tris_tested = ntest;
+    }
+    return best;
+}
+
+// Closest or any hit, counting node visits and triangle tests into the
+// caller's per-lane totals when `count` is set (a wave-uniform flag).
+template <bool kAny>
+__device__ __forceinline__ Hit trace_ray(const SceneView& sv, f3 o, f3 d, float tmin, float tmax,
+                                         Stack st, bool count, uint32_t& nodes, uint32_t& tris) {
+    if (count) {
+        uint32_t nv = 0, nt = 0;
+        Hit h = traverse<kAny, true>(sv, o, d, tmin, tmax, st, &nv, &nt);
+        nodes += nv;
+        tris += nt;
+        return h;
+    }
+    return traverse<kAny, false>(sv, o, d, tmin, tmax, st);
+}
+
+// Adds per-lane traversal totals of a wave to the frame counters.
+__device__ __forceinline__ void flush_traversal_counts(Counters* ctr, int kind, uint32_t rays,
+                                                       uint32_t nodes, uint32_t tris) {
+    unsigned long long r = rays, n = nodes, t = tris;
+    for (int off = 32; off > 0; off >>= 1) {
+        r += __shfl_xor(r, off, 64);
+        n += __shfl_xor(n, off, 64);
+        t += __shfl_xor(t, off, 64);
+    }
+    if (__lane_id() == 0 && r) {
+        atomicAdd(&ctr->trav_rays[kind], r);
+        atomicAdd(&ctr->trav_nodes[kind], n);
+        atomicAdd(&ctr->trav_tris[kind], t);
+    }
+}
+
+}  // namespace wsrt
