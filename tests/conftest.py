@@ -20,9 +20,7 @@ def device():
 
     if wsrt.device_count() == 0:
         pytest.fail("GPU test run without a HIP device")
-    dev = wsrt.Device(0)
-    yield dev
-    dev.close()
+    return wsrt.Device(0)
 
 
 @pytest.fixture(scope="session")

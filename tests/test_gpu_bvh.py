@@ -59,7 +59,7 @@ def test_closest_and_any_hit_match_brute_force(device, cornell_pair, scene):
     oh, of = o.closest_hit(rays, brute=True)
     np.testing.assert_array_equal(gf, of)
     np.testing.assert_array_equal(gh, oh)
-    assert (gf & 1).mean() > 0.3
+    assert (gf & 1).mean() > 0.2
     rays[:, 3], rays[:, 7] = 1e-3, np.random.default_rng(2).uniform(0.01, 2.0, len(rays))
     np.testing.assert_array_equal(g.any_hit(rays), o.any_hit(rays, brute=True))
 

@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <array>
 #include <map>
 #include <memory>
@@ -108,6 +109,7 @@ struct wsrt_device {
     int ordinal = 0;
     hipStream_t stream = nullptr;  // builds and debug queries
     char name[256] = {0};
+    std::atomic<int> live{0};      // scenes and renderers created on this device
 };
 
 extern "C" int wsrt_device_count(void) {
@@ -136,6 +138,9 @@ extern "C" int wsrt_device_create(int ordinal, wsrt_device** out) {
 
 extern "C" int wsrt_device_destroy(wsrt_device* dev) {
     if (!dev) return WSRT_OK;
+    if (dev->live.load() > 0)
+        return fail(WSRT_E_STATE, "device still has %d live scenes/renderers; destroy them first",
+                    dev->live.load());
     (void)hipSetDevice(dev->ordinal);
     if (dev->stream) (void)hipStreamDestroy(dev->stream);
     delete dev;
@@ -382,6 +387,7 @@ extern "C" int wsrt_scene_set_environment(wsrt_scene_builder* b, const float* rg
 // ---------------------------------------------------------------------------
 struct wsrt_scene {
     wsrt_device* dev = nullptr;
+    std::atomic<int> renderers{0};
     DevBuf verts, idx, meshes, materials, lights, instances, env;
     DevBuf nodes, tris, wv, tri_inst, codes, sorted;
     SceneView view{};
@@ -521,12 +527,17 @@ extern "C" int wsrt_scene_build(wsrt_scene_builder* b, wsrt_device* dev, wsrt_sc
             I.bounds[3 + a] = hi;
         }
     }
+    dev->live++;
     *out = sc.release();
     return WSRT_OK;
 }
 
 extern "C" int wsrt_scene_destroy(wsrt_scene* s) {
     if (!s) return WSRT_OK;
+    if (s->renderers.load() > 0)
+        return fail(WSRT_E_STATE, "scene still has %d live renderers; destroy them first",
+                    s->renderers.load());
+    s->dev->live--;
     (void)hipSetDevice(s->dev->ordinal);
     (void)hipStreamSynchronize(s->dev->stream);
     delete s;
@@ -928,12 +939,16 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     const float pos[3] = {0.0f, 0.0f, 3.0f};
     wsrt_camera_build_uniform(pos, -90.0f * (3.14159265359f / 180.0f), 0.0f, (float)width / (float)height,
                               0, &r->cam);
+    dev->live++;
+    const_cast<wsrt_scene*>(scene)->renderers++;
     *out = r.release();
     return WSRT_OK;
 }
 
 extern "C" int wsrt_renderer_destroy(wsrt_renderer* r) {
     if (!r) return WSRT_OK;
+    r->dev->live--;
+    const_cast<wsrt_scene*>(r->scene)->renderers--;
     (void)hipSetDevice(r->dev->ordinal);
     (void)hipStreamSynchronize(r->stream);
     for (const Pending& p : r->pending) {

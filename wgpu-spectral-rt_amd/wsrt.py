@@ -236,8 +236,9 @@ class Device:
         self.h = h
 
     def close(self):
-        if getattr(self, "h", None):
-            load().wsrt_device_destroy(self.h)
+        """Destroys the device; a no-op while scenes or renderers created on it
+        are alive (the library refuses, and they keep this object alive)."""
+        if getattr(self, "h", None) and load().wsrt_device_destroy(self.h) == 0:
             self.h = None
 
     def __del__(self):
@@ -309,8 +310,7 @@ class Scene:
         self.device = device
 
     def close(self):
-        if getattr(self, "h", None):
-            load().wsrt_scene_destroy(self.h)
+        if getattr(self, "h", None) and load().wsrt_scene_destroy(self.h) == 0:
             self.h = None
 
     def __del__(self):
