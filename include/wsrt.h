@@ -160,7 +160,9 @@ typedef struct wsrt_config {
     float guide_lr;               /* Adam learning rate                        */
     uint32_t guide_train_samples; /* training samples per frame                */
     uint32_t env_map;             /* 1: scene environment map lights misses    */
-    uint32_t reserved[6];
+    uint32_t frame_graphs;        /* 1: submit each batch as one HIP graph     */
+    uint32_t frames_in_flight;    /* consecutive frames traced in one wavefront */
+    uint32_t reserved[4];
 } wsrt_config;
 
 /* Fills *cfg with the reference's defaults. */

@@ -696,6 +696,7 @@ void or_bvh8_closest(const wsrt_bvh8_node* nodes, uint32_t n_nodes, const wsrt_b
         uint32_t nv = 0, nt = 0;
         float inv[3] = {safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
         float oo[3] = {o.x, o.y, o.z};
+        int oct = (d.x < 0.0f ? 1 : 0) | (d.y < 0.0f ? 2 : 0) | (d.z < 0.0f ? 4 : 0);
         uint32_t st_node[256];
         float st_t[256];
         int sp = 0;
@@ -748,17 +749,12 @@ void or_bvh8_closest(const wsrt_bvh8_node* nodes, uint32_t n_nodes, const wsrt_b
                     }
                 }
             }
-            /* internal children: push farthest first (insertion order by tnear, ties by slot) */
-            int ord[8], m = 0;
-            for (int c = 0; c < 8; ++c) {
+            /* internal children in the ray-octant order of the slots: slot
+             * (k ^ oct) pushed for k = 7 .. 0, so slot oct is popped first */
+            for (int k = 7; k >= 0; --k) {
+                int c = k ^ oct;
                 if (!chit[c] || !(nd->meta[c] & 0x80)) continue;
                 if (cnear[c] > best_t * BOX_PAD) continue;
-                int k = m++;
-                while (k > 0 && cnear[ord[k - 1]] > cnear[c]) { ord[k] = ord[k - 1]; --k; }
-                ord[k] = c;
-            }
-            for (int k = m - 1; k >= 0; --k) {
-                int c = ord[k];
                 st_node[sp] = nd->child_base + (nd->meta[c] & 0x7F);
                 st_t[sp] = cnear[c];
                 sp++;

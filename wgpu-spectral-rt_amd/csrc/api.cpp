@@ -700,6 +700,8 @@ extern "C" int wsrt_config_default(wsrt_config* c) {
     c->guide_lr = 1e-3f;
     c->guide_train_samples = 1u << 18;
     c->env_map = 0;
+    c->frame_graphs = 1;
+    c->frames_in_flight = 8;
     return WSRT_OK;
 }
 
@@ -715,6 +717,11 @@ static int validate_config(const wsrt_config* c) {
     if (!(c->cell_size > 0.0f) || !(c->gather_radius > 0.0f) || !(c->spectral_radius > 0.0f))
         return fail(WSRT_E_INVALID, "cell_size, gather_radius and spectral_radius must be > 0");
     if (c->photon_norm == 0) return fail(WSRT_E_INVALID, "photon_norm must be > 0");
+    if (c->frames_in_flight == 0 || c->frames_in_flight > 64)
+        return fail(WSRT_E_INVALID, "frames_in_flight %u not in 1..64", c->frames_in_flight);
+    if ((uint64_t)c->photons_per_pass * c->frames_in_flight > (1ull << 28) ||
+        (uint64_t)c->hash_size * c->frames_in_flight > (1ull << 30))
+        return fail(WSRT_E_INVALID, "photons or cells times frames_in_flight too large");
     if (c->guiding) return fail(WSRT_E_UNSUPPORTED, "guiding is not available in this build");
     return WSRT_OK;
 }
@@ -767,10 +774,11 @@ static FrameConst make_frame_const(const wsrt_config& c, const wsrt_camera_unifo
 namespace {
 enum KernelId {
     K_FRAME_PREP, K_PHOTON_TRACE, K_GRID_BUILD, K_GENERATE, K_EXTEND, K_SHADE, K_SHADOW,
-    K_BOUNCE_PREP, K_ACCUMULATE, K_COUNT
+    K_ACCUMULATE, K_COUNT
 };
 const char* kKernelNames[K_COUNT] = {"frame_prep", "photon_trace", "grid_build", "generate", "extend",
-                                     "shade", "shadow", "bounce_prep", "accumulate"};
+                                     "shade", "shadow", "accumulate"};
+constexpr uint32_t kUploadHalf = kFrameTable / 2;  // frames per upload (staging half)
 struct Pending {
     int kid;
     hipEvent_t a, b;
@@ -791,8 +799,17 @@ struct wsrt_renderer {
     DevBuf path_o, path_d, path_tp, path_L, path_wl, path_rng, path_flags, path_hit, q_a, q_b, shade_q,
         shadow;
     DevBuf accum, rgba, counters, gstack;
+    DevBuf ftable, dframe;           // per-frame constants, device frame counter
+    FrameConst* staging = nullptr;   // pinned, 2 halves of kUploadHalf entries
+    hipEvent_t upload_ev[2] = {nullptr, nullptr};
+    bool upload_pending[2] = {false, false};
+    uint32_t upload_half = 0;
     GridBuffers gb{};
     PathLaunch pl{};
+    std::map<uint32_t, std::pair<hipGraph_t, hipGraphExec_t>> graphs;  // by batch size
+    bool graph_failed = false;
+    uint32_t S = 1;                  // frames in flight
+    uint32_t last_slot = 0;          // slot of the last frame of the last batch
     // profiling
     bool profiling = false;
     bool count_traversal = false;
@@ -869,39 +886,49 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     r->W = width;
     r->H = height;
     HIP_OR_FAIL(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
-    const uint64_t npx = (uint64_t)width * height;
+    r->S = c.frames_in_flight;
+    const uint64_t npx = (uint64_t)width * height;  // pixels of one frame
+    const uint64_t npaths = npx * r->S;             // path slots of a batch
+    if (npaths > (1ull << 30)) return fail(WSRT_E_INVALID, "resolution x frames_in_flight too large");
     const uint32_t np = c.enable_photons ? c.photons_per_pass : 0u;
-    const uint32_t np1 = np ? np : 1u;
-    HIP_OR_FAIL(r->photons.alloc(sizeof(wsrt_photon) * np1));
-    HIP_OR_FAIL(r->grid_count.alloc(sizeof(uint32_t) * c.hash_size));
-    HIP_OR_FAIL(r->grid_offset.alloc(sizeof(uint32_t) * c.hash_size));
-    HIP_OR_FAIL(r->grid_cell.alloc(sizeof(uint32_t) * np1));
-    HIP_OR_FAIL(r->grid_slot.alloc(sizeof(uint32_t) * np1));
-    HIP_OR_FAIL(r->grid_sorted.alloc(sizeof(wsrt_photon) * np1));
+    const uint64_t nph = (uint64_t)(np ? np : 1u) * r->S;
+    const uint64_t cells = (uint64_t)c.hash_size * r->S;
+    HIP_OR_FAIL(r->photons.alloc(sizeof(wsrt_photon) * nph));
+    HIP_OR_FAIL(r->grid_count.alloc(sizeof(uint32_t) * cells));
+    HIP_OR_FAIL(r->grid_offset.alloc(sizeof(uint32_t) * cells));
+    HIP_OR_FAIL(r->grid_cell.alloc(sizeof(uint32_t) * nph));
+    HIP_OR_FAIL(r->grid_slot.alloc(sizeof(uint32_t) * nph));
+    HIP_OR_FAIL(r->grid_sorted.alloc(sizeof(wsrt_photon) * nph));
     HIP_OR_FAIL(r->grid_nvalid.alloc(sizeof(uint32_t)));
-    HIP_OR_FAIL(r->grid_temp.alloc(sizeof(uint32_t) * grid_temp_words(np1, c.hash_size)));
-    HIP_OR_FAIL(r->path_o.alloc(16 * npx));
-    HIP_OR_FAIL(r->path_d.alloc(16 * npx));
-    HIP_OR_FAIL(r->path_tp.alloc(16 * npx));
-    HIP_OR_FAIL(r->path_L.alloc(16 * npx));
-    HIP_OR_FAIL(r->path_wl.alloc(16 * npx));
-    HIP_OR_FAIL(r->path_rng.alloc(4 * npx));
-    HIP_OR_FAIL(r->path_flags.alloc(4 * npx));
-    HIP_OR_FAIL(r->path_hit.alloc(16 * npx));
-    HIP_OR_FAIL(r->q_a.alloc(4 * npx));
-    HIP_OR_FAIL(r->q_b.alloc(4 * npx));
-    HIP_OR_FAIL(r->shade_q.alloc(4 * npx * kMatBins));
-    HIP_OR_FAIL(r->shadow.alloc(48 * npx));
+    HIP_OR_FAIL(r->grid_temp.alloc(sizeof(uint32_t) * grid_temp_words((uint32_t)nph, (uint32_t)cells)));
+    HIP_OR_FAIL(r->path_o.alloc(16 * npaths));
+    HIP_OR_FAIL(r->path_d.alloc(16 * npaths));
+    HIP_OR_FAIL(r->path_tp.alloc(16 * npaths));
+    HIP_OR_FAIL(r->path_L.alloc(16 * npaths));
+    HIP_OR_FAIL(r->path_wl.alloc(16 * npaths));
+    HIP_OR_FAIL(r->path_rng.alloc(4 * npaths));
+    HIP_OR_FAIL(r->path_flags.alloc(4 * npaths));
+    HIP_OR_FAIL(r->path_hit.alloc(16 * npaths));
+    HIP_OR_FAIL(r->q_a.alloc(4 * npaths));
+    HIP_OR_FAIL(r->q_b.alloc(4 * npaths));
+    HIP_OR_FAIL(r->shade_q.alloc(4 * npaths * kMatBins));
+    HIP_OR_FAIL(r->shadow.alloc(48 * npaths));
     HIP_OR_FAIL(r->accum.alloc(16 * npx));
     HIP_OR_FAIL(r->rgba.alloc(4 * npx));
     HIP_OR_FAIL(r->counters.alloc(sizeof(Counters)));
     const uint32_t spill = scene->view.stack_size > (uint32_t)kShortStack ? scene->view.stack_size - kShortStack : 0u;
     HIP_OR_FAIL(r->gstack.alloc(sizeof(uint2) * kTraceLanes * (spill ? spill : 1u)));
+    HIP_OR_FAIL(r->ftable.alloc(sizeof(FrameConst) * kFrameTable));
+    HIP_OR_FAIL(r->dframe.alloc(sizeof(uint32_t)));
+    HIP_OR_FAIL(hipHostMalloc((void**)&r->staging, sizeof(FrameConst) * kFrameTable, hipHostMallocDefault));
+    HIP_OR_FAIL(hipEventCreateWithFlags(&r->upload_ev[0], hipEventDisableTiming));
+    HIP_OR_FAIL(hipEventCreateWithFlags(&r->upload_ev[1], hipEventDisableTiming));
+    HIP_OR_FAIL(hipMemsetAsync(r->dframe.p, 0, sizeof(uint32_t), r->stream));
     HIP_OR_FAIL(hipMemsetAsync(r->counters.p, 0, sizeof(Counters), r->stream));
     HIP_OR_FAIL(hipMemsetAsync(r->accum.p, 0, 16 * npx, r->stream));
-    HIP_OR_FAIL(hipMemsetAsync(r->photons.p, 0, sizeof(wsrt_photon) * np1, r->stream));
-    HIP_OR_FAIL(hipMemsetAsync(r->grid_count.p, 0, sizeof(uint32_t) * c.hash_size, r->stream));
-    HIP_OR_FAIL(hipMemsetAsync(r->grid_offset.p, 0, sizeof(uint32_t) * c.hash_size, r->stream));
+    HIP_OR_FAIL(hipMemsetAsync(r->photons.p, 0, sizeof(wsrt_photon) * nph, r->stream));
+    HIP_OR_FAIL(hipMemsetAsync(r->grid_count.p, 0, sizeof(uint32_t) * cells, r->stream));
+    HIP_OR_FAIL(hipMemsetAsync(r->grid_offset.p, 0, sizeof(uint32_t) * cells, r->stream));
     HIP_OR_FAIL(hipMemsetAsync(r->grid_nvalid.p, 0, sizeof(uint32_t), r->stream));
     HIP_OR_FAIL(hipStreamSynchronize(r->stream));
     r->gb.count = r->grid_count.as<uint32_t>();
@@ -925,7 +952,7 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     pl.ps.queue_b = r->q_b.as<uint32_t>();
     pl.ps.shade_q = r->shade_q.as<uint32_t>();
     pl.ps.shadow = r->shadow.as<float4>();
-    pl.ps.capacity = (uint32_t)npx;
+    pl.ps.capacity = (uint32_t)npaths;
     pl.grid.count = r->gb.count;
     pl.grid.offset = r->gb.offset;
     pl.grid.sorted = r->gb.sorted;
@@ -935,6 +962,10 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     pl.gstack = r->gstack.as<uint2>();
     pl.accum = r->accum.as<float4>();
     pl.rgba = r->rgba.as<uint8_t>();
+    pl.fa.table = r->ftable.as<FrameConst>();
+    pl.fa.frame = r->dframe.as<uint32_t>();
+    pl.fa.mask = kFrameTable - 1;
+    pl.count_traversal = 0;
     // default camera of CameraController::new (camera.rs:144-162)
     const float pos[3] = {0.0f, 0.0f, 3.0f};
     wsrt_camera_build_uniform(pos, -90.0f * (3.14159265359f / 180.0f), 0.0f, (float)width / (float)height,
@@ -956,6 +987,13 @@ extern "C" int wsrt_renderer_destroy(wsrt_renderer* r) {
         (void)hipEventDestroy(p.b);
     }
     for (hipEvent_t e : r->event_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : r->upload_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& kv : r->graphs) {
+        (void)hipGraphExecDestroy(kv.second.second);
+        (void)hipGraphDestroy(kv.second.first);
+    }
+    if (r->staging) (void)hipHostFree(r->staging);
     hipStream_t s = r->stream;
     delete r;
     (void)hipStreamDestroy(s);
@@ -969,63 +1007,136 @@ extern "C" int wsrt_renderer_set_camera(wsrt_renderer* r, const wsrt_camera_unif
     return WSRT_OK;
 }
 
-static int render_one(wsrt_renderer* r, const wsrt_camera_uniform& cam) {
+// Enqueues a batch of nf consecutive frames (mod.rs:168-207 each) traced in
+// one wavefront; kernels read each frame's constants through the device
+// frame counter, which the batch advances by nf at its end. Used directly
+// when profiling or counting, and captured into one graph per batch size.
+static int enqueue_batch(wsrt_renderer* r, uint32_t nf) {
     hipStream_t s = r->stream;
-    FrameConst fc = make_frame_const(r->cfg, cam, r->W, r->H);
-    fc.count_traversal = r->count_traversal ? 1u : 0u;
     const PathLaunch& pl = r->pl;
     const uint32_t npx = r->W * r->H;
+    const uint32_t np = r->cfg.enable_photons ? r->cfg.photons_per_pass : 0u;
     {
         ProfScope p(r, K_FRAME_PREP);
-        launch_frame_prep(pl.ctr, npx, s);
+        launch_frame_prep(pl.ctr, npx * nf, s);
     }
-    if (fc.n_photons) {
+    if (np) {
         {
             ProfScope p(r, K_PHOTON_TRACE);
-            launch_photon_trace(pl.sv, fc, r->photons.as<wsrt_photon>(), pl.ctr, pl.gstack, s);
+            launch_photon_trace(pl.sv, pl.fa, np, nf, pl.count_traversal, r->photons.as<wsrt_photon>(),
+                                pl.ctr, pl.gstack, s);
         }
         ProfScope p(r, K_GRID_BUILD);
-        if (grid_build(r->photons.as<wsrt_photon>(), fc.n_photons, fc, r->gb, s))
+        if (grid_build(r->photons.as<wsrt_photon>(), np, nf, r->cfg.hash_size, pl.fa, r->gb, s))
             return fail(WSRT_E_HIP, "grid build launch failed");
     }
     {
         ProfScope p(r, K_GENERATE);
-        launch_generate(pl, fc, s);
+        launch_generate(pl, npx, nf, s);
     }
-    for (uint32_t b = 0; b < fc.max_depth; ++b) {
+    for (uint32_t b = 0; b < r->cfg.max_depth; ++b) {
         uint32_t* q = (b & 1u) ? pl.ps.queue_b : pl.ps.queue_a;
         uint32_t* nq = (b & 1u) ? pl.ps.queue_a : pl.ps.queue_b;
         {
             ProfScope p(r, K_EXTEND);
-            launch_extend(pl, fc, q, s);
+            launch_extend(pl, b, q, s);
         }
         {
             ProfScope p(r, K_SHADE);
-            launch_shade(pl, fc, b, nq, s);
+            launch_shade(pl, npx, b, nq, s);
         }
-        {
-            ProfScope p(r, K_SHADOW);
-            launch_shadow(pl, fc, s);
-        }
-        ProfScope p(r, K_BOUNCE_PREP);
-        launch_bounce_prep(pl.ctr, s);
+        ProfScope p(r, K_SHADOW);
+        launch_shadow(pl, b, s);
     }
     {
         ProfScope p(r, K_ACCUMULATE);
-        launch_accumulate(pl, fc, s);
+        launch_accumulate(pl, npx, nf, s);
     }
+    launch_advance_frame(r->dframe.as<uint32_t>(), nf, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(WSRT_E_HIP, "frame launch: %s", hipGetErrorString(e));
-    r->frames++;
-    r->samples += npx;
-    r->photon_paths += fc.n_photons;
+    return WSRT_OK;
+}
+
+static hipGraphExec_t batch_graph(wsrt_renderer* r, uint32_t nf) {
+    auto it = r->graphs.find(nf);
+    if (it != r->graphs.end()) return it->second.second;
+    if (r->graph_failed) return nullptr;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    bool ok = hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    bool prof = r->profiling;
+    r->profiling = false;
+    int rc = ok ? enqueue_batch(r, nf) : WSRT_E_HIP;
+    r->profiling = prof;
+    if (ok) ok = hipStreamEndCapture(r->stream, &g) == hipSuccess && rc == WSRT_OK;
+    if (ok) ok = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) == hipSuccess;
+    if (!ok) {
+        if (g) (void)hipGraphDestroy(g);
+        r->graph_failed = true;
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    r->graphs[nf] = {g, ge};
+    return ge;
+}
+
+// Frames [first, first + n) with camera `cam`: the host computes each
+// frame's constants, uploads them in stream order into the device table
+// (so no frame in flight has its entry overwritten), sets the device frame
+// counter and submits batches of up to frames_in_flight frames, each one
+// graph (or eager launches when profiling or counting).
+static int submit_frames(wsrt_renderer* r, uint32_t first, uint32_t n, const wsrt_camera_uniform& cam) {
+    HIP_OR_FAIL(hipSetDevice(r->dev->ordinal));
+    hipStream_t s = r->stream;
+    const bool eager = r->profiling || r->count_traversal || !r->cfg.frame_graphs;
+    r->pl.count_traversal = r->count_traversal ? 1u : 0u;
+    for (uint32_t done = 0; done < n;) {
+        const uint32_t m = std::min(n - done, kUploadHalf);
+        const uint32_t half = r->upload_half;
+        r->upload_half ^= 1u;
+        if (r->upload_pending[half]) HIP_OR_FAIL(hipEventSynchronize(r->upload_ev[half]));
+        FrameConst* st = r->staging + (size_t)half * kUploadHalf;
+        const uint32_t f0 = first + done;
+        for (uint32_t i = 0; i < m; ++i) {
+            wsrt_camera_uniform c = cam;
+            c.frame_count = f0 + i;
+            st[i] = make_frame_const(r->cfg, c, r->W, r->H);
+        }
+        // device slots f & mask: at most two contiguous ranges
+        uint32_t slot = f0 & (kFrameTable - 1);
+        uint32_t m1 = std::min(m, kFrameTable - slot);
+        FrameConst* dt = r->ftable.as<FrameConst>();
+        HIP_OR_FAIL(hipMemcpyAsync(dt + slot, st, sizeof(FrameConst) * m1, hipMemcpyHostToDevice, s));
+        if (m1 < m)
+            HIP_OR_FAIL(hipMemcpyAsync(dt, st + m1, sizeof(FrameConst) * (m - m1), hipMemcpyHostToDevice, s));
+        HIP_OR_FAIL(hipEventRecord(r->upload_ev[half], s));
+        r->upload_pending[half] = true;
+        launch_set_frame(r->dframe.as<uint32_t>(), f0, s);
+        for (uint32_t i = 0; i < m;) {
+            const uint32_t nf = std::min(m - i, r->S);
+            hipGraphExec_t ge = eager ? nullptr : batch_graph(r, nf);
+            if (ge) {
+                HIP_OR_FAIL(hipGraphLaunch(ge, s));
+            } else {
+                int rc = enqueue_batch(r, nf);
+                if (rc) return rc;
+            }
+            r->last_slot = nf - 1;
+            i += nf;
+        }
+        done += m;
+    }
+    const uint32_t npx = r->W * r->H;
+    r->frames += n;
+    r->samples += (uint64_t)npx * n;
+    r->photon_paths += (uint64_t)(r->cfg.enable_photons ? r->cfg.photons_per_pass : 0u) * n;
     return WSRT_OK;
 }
 
 extern "C" int wsrt_renderer_render(wsrt_renderer* r) {
     CHECK_ARG(r);
-    HIP_OR_FAIL(hipSetDevice(r->dev->ordinal));
-    int rc = render_one(r, r->cam);
+    int rc = submit_frames(r, r->cam.frame_count, 1, r->cam);
     if (rc) return rc;
     r->frame_count++;  // mod.rs:204
     return WSRT_OK;
@@ -1033,14 +1144,10 @@ extern "C" int wsrt_renderer_render(wsrt_renderer* r) {
 
 extern "C" int wsrt_renderer_render_frames(wsrt_renderer* r, uint32_t n) {
     CHECK_ARG(r);
-    HIP_OR_FAIL(hipSetDevice(r->dev->ordinal));
-    for (uint32_t k = 0; k < n; ++k) {
-        wsrt_camera_uniform cam = r->cam;
-        cam.frame_count = r->frame_count;  // engine.rs:158-166
-        int rc = render_one(r, cam);
-        if (rc) return rc;
-        r->frame_count++;
-    }
+    // frame k is shaded with frame_count = the renderer's counter (engine.rs:158-166)
+    int rc = submit_frames(r, r->frame_count, n, r->cam);
+    if (rc) return rc;
+    r->frame_count += n;
     return WSRT_OK;
 }
 
@@ -1162,12 +1269,15 @@ extern "C" int wsrt_renderer_reset_profile(wsrt_renderer* r) {
 // ---------------------------------------------------------------------------
 // debug / parity taps
 // ---------------------------------------------------------------------------
+// The debug taps read the last frame of the last batch (frame slot
+// last_slot of the batch's buffers).
 extern "C" int wsrt_debug_read_photons(wsrt_renderer* r, wsrt_photon* out, uint32_t cap,
                                        uint32_t* n_stored) {
     CHECK_ARG(r && out);
     uint32_t np = r->cfg.enable_photons ? r->cfg.photons_per_pass : 0u;
     if (cap < np) return fail(WSRT_E_INVALID, "photon buffer holds %u < %u", cap, np);
-    int rc = read_back(r, out, r->photons.p, sizeof(wsrt_photon) * np);
+    const wsrt_photon* src = r->photons.as<wsrt_photon>() + (size_t)r->last_slot * np;
+    int rc = read_back(r, out, src, sizeof(wsrt_photon) * np);
     if (rc) return rc;
     if (n_stored) {
         uint32_t k = 0;
@@ -1180,20 +1290,38 @@ extern "C" int wsrt_debug_read_photons(wsrt_renderer* r, wsrt_photon* out, uint3
 extern "C" int wsrt_debug_read_grid(wsrt_renderer* r, uint32_t* cell_id, uint32_t* count,
                                     uint32_t* offset, uint32_t* sorted_slot) {
     CHECK_ARG(r);
-    uint32_t np = r->cfg.enable_photons ? r->cfg.photons_per_pass : 0u;
+    const uint32_t np = r->cfg.enable_photons ? r->cfg.photons_per_pass : 0u;
+    const uint32_t hs = r->cfg.hash_size, ls = r->last_slot;
     int rc = WSRT_OK;
-    if (cell_id && !rc) rc = read_back(r, cell_id, r->grid_cell.p, 4ull * np);
-    if (count && !rc) rc = read_back(r, count, r->grid_count.p, 4ull * r->cfg.hash_size);
-    if (offset && !rc) rc = read_back(r, offset, r->grid_offset.p, 4ull * r->cfg.hash_size);
-    if (sorted_slot && !rc) rc = read_back(r, sorted_slot, r->grid_slot.p, 4ull * np);
-    return rc;
+    if (cell_id && !rc) rc = read_back(r, cell_id, r->grid_cell.as<uint32_t>() + (size_t)ls * np, 4ull * np);
+    std::vector<uint32_t> cnt(hs), off(hs);
+    if (!rc) rc = read_back(r, cnt.data(), r->grid_count.as<uint32_t>() + (size_t)ls * hs, 4ull * hs);
+    if (!rc) rc = read_back(r, off.data(), r->grid_offset.as<uint32_t>() + (size_t)ls * hs, 4ull * hs);
+    if (rc) return rc;
+    const uint32_t base = hs ? off[0] : 0u;
+    uint64_t valid = 0;
+    for (uint32_t c = 0; c < hs; ++c) {
+        valid += cnt[c];
+        off[c] -= base;  // offsets relative to this frame's first cell
+    }
+    if (count) memcpy(count, cnt.data(), 4ull * hs);
+    if (offset) memcpy(offset, off.data(), 4ull * hs);
+    if (sorted_slot && np) {
+        std::vector<uint32_t> sl(valid);
+        if (valid) rc = read_back(r, sl.data(), r->grid_slot.as<uint32_t>() + base, 4ull * valid);
+        if (rc) return rc;
+        for (uint64_t k = 0; k < np; ++k)
+            sorted_slot[k] = k < valid ? sl[k] - ls * np : kInvalid;  // photon slot within the frame
+    }
+    return WSRT_OK;
 }
 
 extern "C" int wsrt_debug_read_paths(wsrt_renderer* r, float* wl, float* L) {
     CHECK_ARG(r);
+    const size_t npx = (size_t)r->W * r->H, off = npx * r->last_slot;
     int rc = WSRT_OK;
-    if (wl) rc = read_back(r, wl, r->path_wl.p, 16ull * r->W * r->H);
-    if (L && !rc) rc = read_back(r, L, r->path_L.p, 16ull * r->W * r->H);
+    if (wl) rc = read_back(r, wl, r->path_wl.as<float4>() + off, 16 * npx);
+    if (L && !rc) rc = read_back(r, L, r->path_L.as<float4>() + off, 16 * npx);
     return rc;
 }
 
@@ -1218,7 +1346,12 @@ extern "C" int wsrt_photon_grid_build(wsrt_device* dev, const wsrt_photon* photo
     cam.frame_count = frame_count;
     FrameConst fc = make_frame_const(c, cam, 1, 1);
     const uint32_t n1 = n ? n : 1u;
-    DevBuf ph, cnt, off, cell, slot, sorted, nvalid, temp;
+    DevBuf ph, cnt, off, cell, slot, sorted, nvalid, temp, tab, fr;
+    HIP_OR_FAIL(tab.alloc(sizeof fc));
+    HIP_OR_FAIL(fr.alloc(sizeof(uint32_t)));
+    HIP_OR_FAIL(hipMemcpyAsync(tab.p, &fc, sizeof fc, hipMemcpyHostToDevice, s));
+    HIP_OR_FAIL(hipMemsetAsync(fr.p, 0, sizeof(uint32_t), s));
+    FrameArgs fa{tab.as<FrameConst>(), fr.as<uint32_t>(), 0u};
     HIP_OR_FAIL(ph.alloc(sizeof(wsrt_photon) * n1));
     HIP_OR_FAIL(cnt.alloc(4ull * hash_size));
     HIP_OR_FAIL(off.alloc(4ull * hash_size));
@@ -1237,7 +1370,7 @@ extern "C" int wsrt_photon_grid_build(wsrt_device* dev, const wsrt_photon* photo
     g.n_valid = nvalid.as<uint32_t>();
     g.temp = temp.as<uint32_t>();
     HIP_OR_FAIL(hipMemsetAsync(g.n_valid, 0, 4, s));
-    if (grid_build(ph.as<wsrt_photon>(), n, fc, g, s)) return fail(WSRT_E_HIP, "grid build failed");
+    if (grid_build(ph.as<wsrt_photon>(), n, 1, hash_size, fa, g, s)) return fail(WSRT_E_HIP, "grid build failed");
     uint32_t valid = 0;
     HIP_OR_FAIL(hipMemcpyAsync(count, g.count, 4ull * hash_size, hipMemcpyDeviceToHost, s));
     HIP_OR_FAIL(hipMemcpyAsync(offset, g.offset, 4ull * hash_size, hipMemcpyDeviceToHost, s));

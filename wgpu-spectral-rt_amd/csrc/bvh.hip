@@ -316,29 +316,58 @@ __global__ void k_collapse(CollapseArgs A, const uint4* __restrict__ qin, uint32
     }
     out.child_base = child_base;
     out.tri_base = tri_base;
+    // Slot of each child = the octant of its box centre around the node's
+    // centre (bit a set: upper half along axis a), resolved greedily, largest
+    // child first, to the free slot differing in the fewest axes. Traversal
+    // then visits slots in the order (k ^ ray octant), near to far, without
+    // sorting distances.
+    int slot_of[8];
+    {
+        uint32_t used = 0;
+        bool done[8] = {false, false, false, false, false, false, false, false};
+        for (int it = 0; it < m; ++it) {
+            int pick = -1;
+            float best_a = -1.0f;
+            for (int k = 0; k < m; ++k)
+                if (!done[k] && half_area(eb[k]) > best_a) { best_a = half_area(eb[k]); pick = k; }
+            done[pick] = true;
+            uint32_t want = 0;
+            for (int a = 0; a < 3; ++a)
+                if ((eb[pick][a] + eb[pick][3 + a]) > (nb[a] + nb[3 + a])) want |= 1u << a;
+            int sbest = -1, cbest = 99;
+            for (int sl = 0; sl < 8; ++sl) {
+                if (used & (1u << sl)) continue;
+                int cost = __popc((uint32_t)sl ^ want);
+                if (cost < cbest) { cbest = cost; sbest = sl; }
+            }
+            slot_of[pick] = sbest;
+            used |= 1u << sbest;
+        }
+        for (int sl = 0; sl < 8; ++sl) {
+            if (used & (1u << sl)) continue;
+            out.meta[sl] = 0;
+            for (int a = 0; a < 3; ++a) { out.qlo[a][sl] = 0; out.qhi[a][sl] = 0; }
+        }
+    }
     uint8_t imask = 0;
     uint32_t ri = 0, off = 0;
-    for (int k = 0; k < 8; ++k) {
-        if (k >= m) {
-            out.meta[k] = 0;
-            for (int a = 0; a < 3; ++a) { out.qlo[a][k] = 0; out.qhi[a][k] = 0; }
-            continue;
-        }
+    for (int k = 0; k < m; ++k) {
+        const int sl = slot_of[k];
         for (int a = 0; a < 3; ++a) {
             double lo = floor(((double)eb[k][a] - (double)nb[a]) / scale[a]);
             double hi = ceil(((double)eb[k][3 + a] - (double)nb[a]) / scale[a]);
             lo = lo < 0.0 ? 0.0 : (lo > 255.0 ? 255.0 : lo);
             hi = hi < 0.0 ? 0.0 : (hi > 255.0 ? 255.0 : hi);
-            out.qlo[a][k] = (uint8_t)lo;
-            out.qhi[a][k] = (uint8_t)hi;
+            out.qlo[a][sl] = (uint8_t)lo;
+            out.qhi[a][sl] = (uint8_t)hi;
         }
         if (ec[k] > 3u) {
-            out.meta[k] = (uint8_t)(0x80u | ri);
-            imask |= (uint8_t)(1u << k);
+            out.meta[sl] = (uint8_t)(0x80u | ri);
+            imask |= (uint8_t)(1u << sl);
             qout[qbase + ri] = make_uint4(ent[k], child_base + ri, depth + 1, 0);
             ri++;
         } else {
-            out.meta[k] = (uint8_t)((ec[k] << 5) | off);
+            out.meta[sl] = (uint8_t)((ec[k] << 5) | off);
             // gather the <= 3 triangles of this subtree in left-to-right order
             uint32_t stk[4];
             int sp = 0;

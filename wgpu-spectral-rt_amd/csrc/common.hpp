@@ -53,15 +53,19 @@ struct FrameParams {
     uint32_t pad[2];
 };
 
-// Device-side counters of one frame in flight.
+constexpr int kMatBins = 8;
+constexpr int kMaxBounces = 64;
+
+// Device-side counters of the frame in flight. Queue sizes are kept per
+// bounce, so no kernel has to reset them between bounces: frame_prep clears
+// them once per frame.
 struct Counters {
-    uint32_t active;        // paths in the current extend queue
-    uint32_t next;          // paths queued for the next bounce
-    uint32_t shade[8];      // per material type (sort-by-material bins)
-    uint32_t shadow;        // shadow rays queued
-    uint32_t stored;        // photons stored this frame
-    uint32_t valid;         // photons in the grid
-    uint32_t pad[3];
+    uint32_t active[kMaxBounces + 1];        // paths entering bounce b
+    uint32_t shade[kMaxBounces][kMatBins];  // bounce b's hits per material type
+    uint32_t shadow[kMaxBounces];           // shadow rays queued at bounce b
+    uint32_t stored;                        // photons stored this frame
+    uint32_t valid;                         // photons in the grid
+    uint32_t pad[2];
     unsigned long long extend_rays;
     unsigned long long shadow_rays;
     unsigned long long photon_rays;
@@ -71,8 +75,6 @@ struct Counters {
     unsigned long long trav_tris[3];
     unsigned long long trav_rays[3];
 };
-
-constexpr int kMatBins = 8;
 
 // Wavefront path state, structure of arrays indexed by path id (= pixel).
 struct PathState {
@@ -98,6 +100,34 @@ struct GridView {
     const wsrt_photon* sorted;  // photons in cell order
     uint32_t hash_size;
     float cell_size;
+};
+
+// Constants of one frame computed on the host (so the transcendental values
+// are those of the host libm, identical frame to frame).
+struct FrameConst {
+    float cdf_min, cdf_range;  // hero-wavelength Cauchy CDF bounds
+    float radius, radius_sq;   // gather radius r0 * (frame + 1)^exponent
+    float density;             // 3/(pi r^2) * 1/spectral_radius * 1/photon_norm
+    float jitter[3];           // grid jitter of this frame
+    float spectral_radius;
+    float cell_size;
+    uint32_t hash_size;
+    uint32_t gather_cap;
+    uint32_t frame;
+    uint32_t width, height;
+    uint32_t max_depth, photon_max_depth, rr_after;
+    uint32_t enable_photons;
+    uint32_t n_photons;
+    float M[16];               // view_inverse * proj_inverse
+    float origin[3];           // camera position (view_inverse column 3)
+    uint32_t count_traversal;  // 1: count node visits / triangle tests
+};
+
+constexpr uint32_t kFrameTable = 4096;  // entries; frames wrap modulo this
+struct FrameArgs {
+    const FrameConst* table;
+    const uint32_t* frame;  // device frame counter
+    uint32_t mask;          // table entries - 1
 };
 
 }  // namespace wsrt

@@ -8,7 +8,6 @@
 
 namespace wsrt {
 
-struct FrameConst;
 
 // Traversal launches use a fixed grid of kTraceBlocks x kTraceThreads lanes
 // that loop over their work (so the per-lane HBM stack spill area has a fixed
@@ -64,11 +63,16 @@ struct GridBuffers {
     uint32_t* n_valid;      // 1
     uint32_t* temp;         // grid_temp_words(n, hash_size)
 };
-void launch_photon_trace(const SceneView& sv, const FrameConst& fc, wsrt_photon* photons,
-                         Counters* ctr, uint2* gstack, hipStream_t s);
-size_t grid_temp_words(uint32_t n, uint32_t hash_size);
-int grid_build(const wsrt_photon* photons, uint32_t n, const FrameConst& fc, GridBuffers& g,
-               hipStream_t s);
+// Photon paths of nf consecutive frames, np per frame: path i is photon
+// i % np of frame slot i / np, stored at slot i.
+void launch_photon_trace(const SceneView& sv, const FrameArgs& fa, uint32_t np, uint32_t nf,
+                         uint32_t count_traversal, wsrt_photon* photons, Counters* ctr, uint2* gstack,
+                         hipStream_t s);
+size_t grid_temp_words(uint32_t n, uint32_t cells);
+// One counting sort over the grids of nf frames: cell (frame slot, hash)
+// = slot * hash_size + hash; count/offset hold nf * hash_size cells.
+int grid_build(const wsrt_photon* photons, uint32_t np, uint32_t nf, uint32_t hash_size,
+               const FrameArgs& fa, GridBuffers& g, hipStream_t s);
 
 // ---- wavefront camera paths (path.hip) ----------------------------------------
 struct PathLaunch {
@@ -79,15 +83,17 @@ struct PathLaunch {
     uint2* gstack;
     float4* accum;
     uint8_t* rgba;
+    uint32_t count_traversal;
+    FrameArgs fa;
 };
-void launch_generate(const PathLaunch& pl, const FrameConst& fc, hipStream_t s);
-void launch_extend(const PathLaunch& pl, const FrameConst& fc, const uint32_t* queue,
-                   hipStream_t s);
-void launch_shade(const PathLaunch& pl, const FrameConst& fc, uint32_t bounce, uint32_t* next_queue,
-                  hipStream_t s);
-void launch_shadow(const PathLaunch& pl, const FrameConst& fc, hipStream_t s);
-void launch_bounce_prep(Counters* ctr, hipStream_t s);
-void launch_accumulate(const PathLaunch& pl, const FrameConst& fc, hipStream_t s);
 void launch_frame_prep(Counters* ctr, uint32_t n_paths, hipStream_t s);
+void launch_advance_frame(uint32_t* frame, uint32_t nf, hipStream_t s);
+void launch_set_frame(uint32_t* frame, uint32_t value, hipStream_t s);
+// A batch of nf consecutive frames of npx pixels shares one wavefront.
+void launch_generate(const PathLaunch& pl, uint32_t npx, uint32_t nf, hipStream_t s);
+void launch_extend(const PathLaunch& pl, uint32_t bounce, const uint32_t* queue, hipStream_t s);
+void launch_shade(const PathLaunch& pl, uint32_t npx, uint32_t bounce, uint32_t* next_queue, hipStream_t s);
+void launch_shadow(const PathLaunch& pl, uint32_t bounce, hipStream_t s);
+void launch_accumulate(const PathLaunch& pl, uint32_t npx, uint32_t nf, hipStream_t s);
 
 }  // namespace wsrt

@@ -25,32 +25,37 @@ constexpr uint32_t kFlagDiffuse = 1u, kFlagCaustic = 2u;
 constexpr int kShadeThreads = 256;
 constexpr int kShadeBlocks = 1024;
 
+// Clears the per-bounce queue sizes of a frame (a fill kernel, so the
+// frame can be captured in a graph).
 __global__ __launch_bounds__(256) void k_frame_prep(Counters* ctr, uint32_t n_paths) {
+    uint32_t* a = ctr->active;
+    for (uint32_t i = threadIdx.x; i <= (uint32_t)kMaxBounces; i += blockDim.x) a[i] = i == 0 ? n_paths : 0u;
+    uint32_t* s = &ctr->shade[0][0];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(kMaxBounces * kMatBins); i += blockDim.x) s[i] = 0u;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kMaxBounces; i += blockDim.x) ctr->shadow[i] = 0u;
     if (threadIdx.x == 0) {
-        ctr->active = n_paths;
-        ctr->next = 0;
-        ctr->shadow = 0;
         ctr->stored = 0;
         ctr->valid = 0;
     }
-    if (threadIdx.x < (unsigned)kMatBins) ctr->shade[threadIdx.x] = 0;
 }
 
-__global__ void k_bounce_prep(Counters* ctr) {
-    if (threadIdx.x == 0) {
-        ctr->active = ctr->next;
-        ctr->next = 0;
-        ctr->shadow = 0;
-    }
-    if (threadIdx.x < (unsigned)kMatBins) ctr->shade[threadIdx.x] = 0;
+// Last node of a frame: the next frame is shaded with frame_count + 1.
+__global__ void k_advance_frame(uint32_t* frame, uint32_t nf) {
+    if (threadIdx.x == 0) *frame = *frame + nf;
+}
+
+__global__ void k_set_frame(uint32_t* frame, uint32_t value) {
+    if (threadIdx.x == 0) *frame = value;
 }
 
 // shader.wgsl:709-767
-__global__ __launch_bounds__(256) void k_generate(PathState ps, FrameConst fc) {
+// Path p of a batch of frames belongs to frame slot p / npx, pixel p % npx.
+__global__ __launch_bounds__(256) void k_generate(PathState ps, FrameArgs fa, uint32_t npx, uint32_t nf) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = fc.width * fc.height;
-    if (p >= n) return;
-    const uint32_t x = p % fc.width, y = p / fc.width;
+    if (p >= npx * nf) return;
+    const uint32_t slot = p / npx, pix = p - slot * npx;
+    const FrameConst& fc = fa.table[(*fa.frame + slot) & fa.mask];
+    const uint32_t x = pix % fc.width, y = pix / fc.width;
     uint32_t rng = pcg_hash(x + y * fc.width + fc.frame * 927163u);
     float jx = rand01(rng), jy = rand01(rng);
     float ux = ((float)x + jx) / (float)fc.width, uy = ((float)y + jy) / (float)fc.height;
@@ -96,7 +101,7 @@ __global__ __launch_bounds__(256) void k_generate(PathState ps, FrameConst fc) {
 
 __global__ __launch_bounds__(kTraceThreads) void k_extend(SceneView sv, PathState ps,
                                                           uint32_t count_traversal,
-                                                          Counters* __restrict__ ctr,
+                                                          Counters* __restrict__ ctr, uint32_t bounce,
                                                           const uint32_t* __restrict__ queue,
                                                           uint2* __restrict__ gstack) {
     __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
@@ -105,8 +110,9 @@ __global__ __launch_bounds__(kTraceThreads) void k_extend(SceneView sv, PathStat
     st.lds = lds_stack + threadIdx.x;
     st.lds_stride = kTraceThreads;
     st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
-    const uint32_t n = ctr->active;
+    const uint32_t n = ctr->active[bounce];
     if (gtid == 0) atomicAdd(&ctr->extend_rays, (unsigned long long)n);
+    uint32_t* bins = ctr->shade[bounce];
     const uint32_t stride = gridDim.x * blockDim.x;
     // every lane of a wave runs the same number of iterations (wave-uniform
     // trip count) so the binning ballots below see the whole wave
@@ -140,7 +146,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_extend(SceneView sv, PathStat
             const bool mine = have && bin == b;
             const uint64_t m = __ballot(mine);
             uint32_t slot = 0;
-            if (__lane_id() == leader) slot = atomicAdd(&ctr->shade[b], (uint32_t)__popcll(m));
+            if (__lane_id() == leader) slot = atomicAdd(&bins[b], (uint32_t)__popcll(m));
             slot = __shfl(slot, leader);
             if (mine) {
                 ps.shade_q[(uint64_t)b * ps.capacity + slot + __popcll(m & lanemask_lt())] = p;
@@ -153,13 +159,16 @@ __global__ __launch_bounds__(kTraceThreads) void k_extend(SceneView sv, PathStat
 }
 
 __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState ps, GridView grid,
-                                                         FrameConst fc, Counters* __restrict__ ctr,
-                                                         uint32_t bounce,
+                                                         FrameArgs fa, uint32_t npx,
+                                                         Counters* __restrict__ ctr, uint32_t bounce,
                                                          uint32_t* __restrict__ next_queue) {
+    const uint32_t base_frame = *fa.frame;
+    const uint32_t rr_after = fa.table[base_frame & fa.mask].rr_after;
+    const uint32_t photons_on = fa.table[base_frame & fa.mask].enable_photons;
     uint32_t start[kMatBins + 1];
     start[0] = 0;
 #pragma unroll
-    for (int b = 0; b < kMatBins; ++b) start[b + 1] = start[b] + ctr->shade[b];
+    for (int b = 0; b < kMatBins; ++b) start[b + 1] = start[b] + ctr->shade[bounce][b];
     const uint32_t total = start[kMatBins];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -213,8 +222,10 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
                 hp = o + d * t;
                 if (type == 0u) {
                     float4 spec = rgb_to_spectrum4(mk3(col.x, col.y, col.z), wl);
-                    if (fc.enable_photons) {
-                        float4 g = gather_photons(grid, fc, hp, ff, wl);
+                    if (photons_on) {
+                        const uint32_t slot = p / npx;
+                        const FrameConst& fs = fa.table[(base_frame + slot) & fa.mask];
+                        float4 g = gather_photons(grid, fs, slot * fs.hash_size, hp, ff, wl);
                         L = add4(L, mul4(mul4(tp, div4s(spec, kPi)), g));
                     }
                     LightSample ls = sample_light(sv, hp, wl, rng);
@@ -235,7 +246,7 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
                     o4 = f4(hp, 0.0f);
                     d4 = f4(sc.dir, 0.0f);
                     alive = true;
-                    if (bounce > fc.rr_after) {
+                    if (bounce > rr_after) {
                         float pr = fmaxf(fmaxf(tp.x, tp.y), fmaxf(tp.z, tp.w));
                         if (rand01(rng) > pr) alive = false;
                         else tp = div4s(tp, pr);
@@ -251,21 +262,21 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
                 ps.flags[p] = flags;
             }
         }
-        uint32_t ss = wave_append(&ctr->shadow, shadow);
+        uint32_t ss = wave_append(&ctr->shadow[bounce], shadow);
         if (shadow) {
             float4* r = ps.shadow + 3ull * ss;
             r[0] = f4(hp, stmax);
             r[1] = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(p));
             r[2] = contrib;
         }
-        uint32_t ns = wave_append(&ctr->next, alive);
+        uint32_t ns = wave_append(&ctr->active[bounce + 1], alive);
         if (alive) next_queue[ns] = p;
     }
 }
 
 __global__ __launch_bounds__(kTraceThreads) void k_shadow(SceneView sv, PathState ps,
                                                           uint32_t count_traversal,
-                                                          Counters* __restrict__ ctr,
+                                                          Counters* __restrict__ ctr, uint32_t bounce,
                                                           uint2* __restrict__ gstack) {
     __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -273,7 +284,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_shadow(SceneView sv, PathStat
     st.lds = lds_stack + threadIdx.x;
     st.lds_stride = kTraceThreads;
     st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
-    const uint32_t n = ctr->shadow;
+    const uint32_t n = ctr->shadow[bounce];
     if (gtid == 0) atomicAdd(&ctr->shadow_rays, (unsigned long long)n);
     const uint32_t stride = gridDim.x * blockDim.x;
     const bool count = count_traversal != 0u;
@@ -291,25 +302,32 @@ __global__ __launch_bounds__(kTraceThreads) void k_shadow(SceneView sv, PathStat
     if (count) flush_traversal_counts(ctr, 2, my_rays, my_nodes, my_tris);
 }
 
-// shader.wgsl:771-822
-__global__ __launch_bounds__(256) void k_accumulate(PathState ps, FrameConst fc,
-                                                    float4* __restrict__ accum,
+// shader.wgsl:771-822, for the frames of the batch in frame order, so the
+// sum per pixel is the one frame-by-frame accumulation produces.
+__global__ __launch_bounds__(256) void k_accumulate(PathState ps, FrameArgs fa, uint32_t npx,
+                                                    uint32_t nf, float4* __restrict__ accum,
                                                     uint8_t* __restrict__ rgba) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= fc.width * fc.height) return;
-    const float4 wl = ps.wl[p], L = ps.L[p];
-    f3 xz = mk3(0, 0, 0);
-    xz = xz + cie_xyz(wl.x) * L.x;
-    xz = xz + cie_xyz(wl.y) * L.y;
-    xz = xz + cie_xyz(wl.z) * L.z;
-    xz = xz + cie_xyz(wl.w) * L.w;
-    const CauchyConst cc{fc.cdf_min, fc.cdf_range};
-    float sp = cauchy_pdf(wl.x, cc) + cauchy_pdf(wl.y, cc) + cauchy_pdf(wl.z, cc) + cauchy_pdf(wl.w, cc);
-    xz = xz * (1.0f / sp);
-    f3 c = xyz_to_srgb(xz);
-    float4 cur = fc.frame > 0u ? accum[p] : splat4(0.0f);
-    float4 na = add4(cur, make_float4(c.x, c.y, c.z, 1.0f));
-    accum[p] = na;
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= npx) return;
+    const uint32_t base_frame = *fa.frame;
+    float4 na = accum[pix];
+    for (uint32_t s = 0; s < nf; ++s) {
+        const FrameConst& fc = fa.table[(base_frame + s) & fa.mask];
+        const uint64_t p = (uint64_t)s * npx + pix;
+        const float4 wl = ps.wl[p], L = ps.L[p];
+        f3 xz = mk3(0, 0, 0);
+        xz = xz + cie_xyz(wl.x) * L.x;
+        xz = xz + cie_xyz(wl.y) * L.y;
+        xz = xz + cie_xyz(wl.z) * L.z;
+        xz = xz + cie_xyz(wl.w) * L.w;
+        const CauchyConst cc{fc.cdf_min, fc.cdf_range};
+        float sp = cauchy_pdf(wl.x, cc) + cauchy_pdf(wl.y, cc) + cauchy_pdf(wl.z, cc) + cauchy_pdf(wl.w, cc);
+        xz = xz * (1.0f / sp);
+        f3 c = xyz_to_srgb(xz);
+        float4 cur = fc.frame > 0u ? na : splat4(0.0f);
+        na = add4(cur, make_float4(c.x, c.y, c.z, 1.0f));
+    }
+    accum[pix] = na;
     f3 fcol = mk3(fmaxf(na.x / na.w * 1.0f, 0.0f), fmaxf(na.y / na.w * 1.0f, 0.0f),
                   fmaxf(na.z / na.w * 1.0f, 0.0f));
     float luma = dot(fcol, mk3(0.2126f, 0.7152f, 0.0722f));
@@ -320,39 +338,39 @@ __global__ __launch_bounds__(256) void k_accumulate(PathState ps, FrameConst fc,
     o.y = (uint8_t)__float2int_rn(fminf(fmaxf(fcol.y, 0.0f), 1.0f) * 255.0f);
     o.z = (uint8_t)__float2int_rn(fminf(fmaxf(fcol.z, 0.0f), 1.0f) * 255.0f);
     o.w = 255;
-    reinterpret_cast<uchar4*>(rgba)[p] = o;
+    reinterpret_cast<uchar4*>(rgba)[pix] = o;
 }
 
 }  // namespace
 
 void launch_frame_prep(Counters* ctr, uint32_t n_paths, hipStream_t s) {
-    hipLaunchKernelGGL(k_frame_prep, dim3(1), dim3(64), 0, s, ctr, n_paths);
+    hipLaunchKernelGGL(k_frame_prep, dim3(1), dim3(256), 0, s, ctr, n_paths);
 }
-void launch_bounce_prep(Counters* ctr, hipStream_t s) {
-    hipLaunchKernelGGL(k_bounce_prep, dim3(1), dim3(64), 0, s, ctr);
+void launch_advance_frame(uint32_t* frame, uint32_t nf, hipStream_t s) {
+    hipLaunchKernelGGL(k_advance_frame, dim3(1), dim3(64), 0, s, frame, nf);
 }
-void launch_generate(const PathLaunch& pl, const FrameConst& fc, hipStream_t s) {
-    uint32_t n = fc.width * fc.height;
-    hipLaunchKernelGGL(k_generate, dim3((n + 255) / 256), dim3(256), 0, s, pl.ps, fc);
+void launch_set_frame(uint32_t* frame, uint32_t value, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_frame, dim3(1), dim3(64), 0, s, frame, value);
 }
-void launch_extend(const PathLaunch& pl, const FrameConst& fc, const uint32_t* queue,
-                   hipStream_t s) {
+void launch_generate(const PathLaunch& pl, uint32_t npx, uint32_t nf, hipStream_t s) {
+    uint32_t n = npx * nf;
+    hipLaunchKernelGGL(k_generate, dim3((n + 255) / 256), dim3(256), 0, s, pl.ps, pl.fa, npx, nf);
+}
+void launch_extend(const PathLaunch& pl, uint32_t bounce, const uint32_t* queue, hipStream_t s) {
     hipLaunchKernelGGL(k_extend, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, pl.sv, pl.ps,
-                       fc.count_traversal, pl.ctr, queue, pl.gstack);
+                       pl.count_traversal, pl.ctr, bounce, queue, pl.gstack);
 }
-void launch_shade(const PathLaunch& pl, const FrameConst& fc, uint32_t bounce, uint32_t* next_queue,
-                  hipStream_t s) {
+void launch_shade(const PathLaunch& pl, uint32_t npx, uint32_t bounce, uint32_t* next_queue, hipStream_t s) {
     hipLaunchKernelGGL(k_shade, dim3(kShadeBlocks), dim3(kShadeThreads), 0, s, pl.sv, pl.ps, pl.grid,
-                       fc, pl.ctr, bounce, next_queue);
+                       pl.fa, npx, pl.ctr, bounce, next_queue);
 }
-void launch_shadow(const PathLaunch& pl, const FrameConst& fc, hipStream_t s) {
+void launch_shadow(const PathLaunch& pl, uint32_t bounce, hipStream_t s) {
     hipLaunchKernelGGL(k_shadow, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, pl.sv, pl.ps,
-                       fc.count_traversal, pl.ctr, pl.gstack);
+                       pl.count_traversal, pl.ctr, bounce, pl.gstack);
 }
-void launch_accumulate(const PathLaunch& pl, const FrameConst& fc, hipStream_t s) {
-    uint32_t n = fc.width * fc.height;
-    hipLaunchKernelGGL(k_accumulate, dim3((n + 255) / 256), dim3(256), 0, s, pl.ps, fc, pl.accum,
-                       pl.rgba);
+void launch_accumulate(const PathLaunch& pl, uint32_t npx, uint32_t nf, hipStream_t s) {
+    hipLaunchKernelGGL(k_accumulate, dim3((npx + 255) / 256), dim3(256), 0, s, pl.ps, pl.fa, npx, nf,
+                       pl.accum, pl.rgba);
 }
 
 }  // namespace wsrt

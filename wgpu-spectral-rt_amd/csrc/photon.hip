@@ -15,10 +15,15 @@ namespace wsrt {
 
 namespace {
 
-__global__ __launch_bounds__(kTraceThreads) void k_photon_trace(SceneView sv, FrameConst fc,
+__global__ __launch_bounds__(kTraceThreads) void k_photon_trace(SceneView sv, FrameArgs fa,
+                                                                uint32_t np, uint32_t nf,
+                                                                uint32_t count_traversal,
                                                                 wsrt_photon* __restrict__ photons,
                                                                 Counters* __restrict__ ctr,
                                                                 uint2* __restrict__ gstack) {
+    const uint32_t base_frame = *fa.frame;
+    const FrameConst& f0 = fa.table[base_frame & fa.mask];
+    const uint32_t photon_max_depth = f0.photon_max_depth;
     __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;
@@ -26,12 +31,14 @@ __global__ __launch_bounds__(kTraceThreads) void k_photon_trace(SceneView sv, Fr
     st.lds_stride = kTraceThreads;
     st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
     const uint32_t stride = gridDim.x * blockDim.x;
-    const CauchyConst cc{fc.cdf_min, fc.cdf_range};
+    const CauchyConst cc{f0.cdf_min, f0.cdf_range};
+    const uint32_t n_photons = np * nf;
     uint32_t my_rays = 0, my_stored = 0, my_nodes = 0, my_tris = 0;
-    const bool count = fc.count_traversal != 0u;
-    for (uint32_t id = gtid; id < fc.n_photons; id += stride) {
+    const bool count = count_traversal != 0u;
+    for (uint32_t id = gtid; id < n_photons; id += stride) {
         float4 out0 = splat4(0.0f), out1 = splat4(0.0f);
-        uint32_t rng = pcg_hash(id + fc.frame * 927163u);
+        const uint32_t slot = id / np, local = id - slot * np;
+        uint32_t rng = pcg_hash(local + (base_frame + slot) * 927163u);
         const uint32_t nl = sv.n_lights;
         if (nl != 0u) {
             uint32_t li = (uint32_t)(rand01(rng) * (float)nl);
@@ -65,7 +72,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_photon_trace(SceneView sv, Fr
             f3 pos = emit_pos + emit_dir * 0.001f;
             f3 dir = emit_dir;
             bool caustic = false;
-            for (uint32_t depth = 0; depth < fc.photon_max_depth; ++depth) {
+            for (uint32_t depth = 0; depth < photon_max_depth; ++depth) {
                 Hit h = trace_ray<false>(sv, pos, dir, 0.0001f, 100.0f, st, count, my_nodes, my_tris);
                 my_rays++;
                 if (h.tri == kInvalid) break;
@@ -145,7 +152,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_photon_trace(SceneView sv, Fr
 // cell id per slot (hash_size marks skipped photons so they sort last) and
 // the atomic histogram.
 __global__ __launch_bounds__(256) void k_grid_hash(const wsrt_photon* __restrict__ photons,
-                                                   uint32_t n, FrameConst fc,
+                                                   uint32_t n, uint32_t np, uint32_t nf, FrameArgs fa,
                                                    uint32_t* __restrict__ cell_id,
                                                    uint32_t* __restrict__ keys,
                                                    uint32_t* __restrict__ vals,
@@ -154,17 +161,20 @@ __global__ __launch_bounds__(256) void k_grid_hash(const wsrt_photon* __restrict
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool valid = false;
     if (i < n) {
+        const uint32_t slot = i / np;
+        const FrameConst& fc = fa.table[(*fa.frame + slot) & fa.mask];
         const float4 a = reinterpret_cast<const float4*>(photons + i)[0];
         const float e = reinterpret_cast<const float4*>(photons + i)[1].w;
-        uint32_t c = kInvalid;
+        uint32_t c = kInvalid, key = nf * fc.hash_size;
         if (e != 0.0f) {
             c = hash_cell(xyz(a), mk3(fc.jitter[0], fc.jitter[1], fc.jitter[2]), fc.cell_size,
                           fc.hash_size);
-            atomicAdd(&count[c], 1u);
+            key = slot * fc.hash_size + c;
+            atomicAdd(&count[key], 1u);
             valid = true;
         }
         cell_id[i] = c;
-        keys[i] = valid ? c : fc.hash_size;
+        keys[i] = key;
         vals[i] = i;
     }
     uint64_t m = __ballot(valid);
@@ -185,36 +195,38 @@ __global__ __launch_bounds__(256) void k_grid_copy(const wsrt_photon* __restrict
 
 }  // namespace
 
-void launch_photon_trace(const SceneView& sv, const FrameConst& fc, wsrt_photon* photons,
-                         Counters* ctr, uint2* gstack, hipStream_t s) {
-    hipLaunchKernelGGL(k_photon_trace, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, sv, fc,
-                       photons, ctr, gstack);
+void launch_photon_trace(const SceneView& sv, const FrameArgs& fa, uint32_t np, uint32_t nf,
+                         uint32_t count_traversal, wsrt_photon* photons, Counters* ctr, uint2* gstack,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_photon_trace, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, sv, fa, np, nf,
+                       count_traversal, photons, ctr, gstack);
 }
 
-size_t grid_temp_words(uint32_t n, uint32_t hash_size) {
+size_t grid_temp_words(uint32_t n, uint32_t cells) {
     size_t sort = radix_temp_words(n);
-    size_t scan = scan_temp_words(hash_size);
+    size_t scan = scan_temp_words(cells);
     return 2 * (size_t)n + (sort > scan ? sort : scan) + 4;
 }
 
-int grid_build(const wsrt_photon* photons, uint32_t n, const FrameConst& fc, GridBuffers& g,
-               hipStream_t s) {
+int grid_build(const wsrt_photon* photons, uint32_t np, uint32_t nf, uint32_t hash_size,
+               const FrameArgs& fa, GridBuffers& g, hipStream_t s) {
     // g.temp layout: keys[n], vals[n], sort/scan temp, n_valid
+    const uint32_t n = np * nf, cells = hash_size * nf;
     uint32_t* keys = g.temp;
     uint32_t* vals = g.temp + n;
     uint32_t* work = g.temp + 2 * (size_t)n;
-    uint32_t* n_valid = g.temp + grid_temp_words(n, fc.hash_size) - 1;
-    fill_u32(g.count, fc.hash_size, 0u, s);
+    uint32_t* n_valid = g.temp + grid_temp_words(n, cells) - 1;
+    fill_u32(g.count, cells, 0u, s);
     fill_u32(n_valid, 1, 0u, s);
     if (n == 0) {
-        if (scan_exclusive(g.count, g.offset, fc.hash_size, work, nullptr, s)) return -1;
+        if (scan_exclusive(g.count, g.offset, cells, work, nullptr, s)) return -1;
         return 0;
     }
-    hipLaunchKernelGGL(k_grid_hash, dim3((n + 255) / 256), dim3(256), 0, s, photons, n, fc,
+    hipLaunchKernelGGL(k_grid_hash, dim3((n + 255) / 256), dim3(256), 0, s, photons, n, np, nf, fa,
                        g.cell_id, keys, vals, g.count, n_valid);
-    if (scan_exclusive(g.count, g.offset, fc.hash_size, work, nullptr, s)) return -1;
+    if (scan_exclusive(g.count, g.offset, cells, work, nullptr, s)) return -1;
     int bits = 1;
-    while ((1u << bits) <= fc.hash_size) bits++;  // keys in [0, hash_size]
+    while ((1ull << bits) <= (uint64_t)cells) bits++;  // keys in [0, cells]
     bool in_tmp = false;
     if (radix_sort_pairs(keys, vals, n, bits, work, s, &in_tmp)) return -1;
     const uint32_t* sorted_slot = in_tmp ? work + n : vals;

@@ -9,26 +9,11 @@
 
 namespace wsrt {
 
-// Constants of one frame computed on the host (so the transcendental values
-// are those of the host libm, identical frame to frame).
-struct FrameConst {
-    float cdf_min, cdf_range;  // hero-wavelength Cauchy CDF bounds
-    float radius, radius_sq;   // gather radius r0 * (frame + 1)^exponent
-    float density;             // 3/(pi r^2) * 1/spectral_radius * 1/photon_norm
-    float jitter[3];           // grid jitter of this frame
-    float spectral_radius;
-    float cell_size;
-    uint32_t hash_size;
-    uint32_t gather_cap;
-    uint32_t frame;
-    uint32_t width, height;
-    uint32_t max_depth, photon_max_depth, rr_after;
-    uint32_t enable_photons;
-    uint32_t n_photons;
-    float M[16];               // view_inverse * proj_inverse
-    float origin[3];           // camera position (view_inverse column 3)
-    uint32_t count_traversal;  // 1: count node visits / triangle tests
-};
+// Frame constants are read from a device table indexed by a device-side
+// frame counter, so a captured frame graph reads the current frame's values.
+__device__ __forceinline__ FrameConst frame_const(const FrameArgs& fa) {
+    return fa.table[*fa.frame & fa.mask];
+}
 
 // shader.wgsl:530-545 and :573-579: interpolated object normal, world normal
 // through the transposed world-to-object matrix.
@@ -230,11 +215,11 @@ __device__ __forceinline__ Scatter sample_bsdf(f3 rdir, f3 ff, const wsrt_materi
 
 // shader.wgsl:297-342 over the counting-sort grid: the first gather_cap
 // photons of the cell in slot order, read from one contiguous range.
-__device__ __forceinline__ float4 gather_photons(const GridView& g, const FrameConst& fc, f3 hp, f3 n,
-                                                 float4 wl) {
+__device__ __forceinline__ float4 gather_photons(const GridView& g, const FrameConst& fc,
+                                                 uint32_t cell_base, f3 hp, f3 n, float4 wl) {
     float4 total = splat4(0.0f);
     f3 j = mk3(fc.jitter[0], fc.jitter[1], fc.jitter[2]);
-    uint32_t h = hash_cell(hp, j, fc.cell_size, fc.hash_size);
+    uint32_t h = cell_base + hash_cell(hp, j, fc.cell_size, fc.hash_size);
     uint32_t cnt = g.count[h];
     if (cnt > fc.gather_cap) cnt = fc.gather_cap;
     const uint32_t off = g.offset[h];

@@ -9,8 +9,9 @@
 //  * any hit terminates on the first accepted triangle.
 // Node boxes are tested with fused slab values plus a bound of their f32
 // error, so culling never rejects a box the exact box would accept; the
-// visiting order (leaf children first, internal children nearest-first by
-// entry distance, ties by slot) is what the oracle's BVH8 traversal counts.
+// visiting order (leaf children first, then internal children in the order
+// of the ray's direction octant over the build-time octant slots) is what
+// the oracle's BVH8 traversal counts.
 #pragma once
 
 #include "common.hpp"
@@ -83,6 +84,7 @@ __device__ Hit traverse(const SceneView& sv, f3 o, f3 d, float tmin, float tmax,
     if (sv.n_nodes == 0) return best;
     const float inv[3] = {safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
     const float oo[3] = {o.x, o.y, o.z};
+    const int oct = (d.x < 0.0f ? 1 : 0) | (d.y < 0.0f ? 2 : 0) | (d.z < 0.0f ? 4 : 0);
     uint32_t nvis = 0, ntest = 0;
     int sp = 0;
     st.put(sp++, make_uint2(0u, __float_as_uint(tmin)));
@@ -153,29 +155,15 @@ __device__ Hit traverse(const SceneView& sv, f3 o, f3 d, float tmin, float tmax,
                 }
             }
         }
-        // internal children: nearest on top of the stack
-        uint32_t imask = 0;
+        // internal children in octant order: slot (k ^ oct) for k = 7 .. 0, so
+        // the child on the ray's near side is on top of the stack
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
+        for (int k = 7; k >= 0; --k) {
+            const int c = k ^ oct;
             uint32_t meta = byte_of(w1.z, w1.w, c);
             if (((hitmask >> c) & 1u) && (meta & 0x80u) && !(cnear[c] > best.t * kBoxPad))
-                imask |= 1u << c;
+                st.put(sp++, make_uint2(child_base + (meta & 0x7Fu), __float_as_uint(cnear[c])));
         }
-        int m = __popc(imask);
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            if (!((imask >> c) & 1u)) continue;
-            int rank = 0;
-#pragma unroll
-            for (int c2 = 0; c2 < 8; ++c2) {
-                bool before = ((imask >> c2) & 1u) &&
-                              (cnear[c2] < cnear[c] || (cnear[c2] == cnear[c] && c2 < c));
-                rank += before ? 1 : 0;
-            }
-            uint32_t meta = byte_of(w1.z, w1.w, c);
-            st.put(sp + (m - 1 - rank), make_uint2(child_base + (meta & 0x7Fu), __float_as_uint(cnear[c])));
-        }
-        sp += m;
     }
     if (kCount) {
         *n_nodes_visited = nvis;

@@ -42,7 +42,8 @@ class Config(C.Structure):
                 ("photon_norm", C.c_uint32), ("guiding", C.c_uint32),
                 ("guide_mix", C.c_float), ("guide_lr", C.c_float),
                 ("guide_train_samples", C.c_uint32), ("env_map", C.c_uint32),
-                ("reserved", C.c_uint32 * 6)]
+                ("frame_graphs", C.c_uint32), ("frames_in_flight", C.c_uint32),
+                ("reserved", C.c_uint32 * 4)]
 
 
 class SceneInfo(C.Structure):
