@@ -800,6 +800,8 @@ struct wsrt_renderer {
         shadow;
     DevBuf accum, rgba, counters, gstack;
     DevBuf ftable, dframe;           // per-frame constants, device frame counter
+    DevBuf ph_pos, ph_dir, ph_rng, ph_flags, ph_qa, ph_qb;
+    PhotonState phs{};
     FrameConst* staging = nullptr;   // pinned, 2 halves of kUploadHalf entries
     hipEvent_t upload_ev[2] = {nullptr, nullptr};
     bool upload_pending[2] = {false, false};
@@ -901,6 +903,14 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     HIP_OR_FAIL(r->grid_sorted.alloc(sizeof(wsrt_photon) * nph));
     HIP_OR_FAIL(r->grid_nvalid.alloc(sizeof(uint32_t)));
     HIP_OR_FAIL(r->grid_temp.alloc(sizeof(uint32_t) * grid_temp_words((uint32_t)nph, (uint32_t)cells)));
+    HIP_OR_FAIL(r->ph_pos.alloc(16 * nph));
+    HIP_OR_FAIL(r->ph_dir.alloc(16 * nph));
+    HIP_OR_FAIL(r->ph_rng.alloc(4 * nph));
+    HIP_OR_FAIL(r->ph_flags.alloc(4 * nph));
+    HIP_OR_FAIL(r->ph_qa.alloc(4 * nph));
+    HIP_OR_FAIL(r->ph_qb.alloc(4 * nph));
+    r->phs = PhotonState{r->ph_pos.as<float4>(), r->ph_dir.as<float4>(), r->ph_rng.as<uint32_t>(),
+                         r->ph_flags.as<uint32_t>(), r->ph_qa.as<uint32_t>(), r->ph_qb.as<uint32_t>()};
     HIP_OR_FAIL(r->path_o.alloc(16 * npaths));
     HIP_OR_FAIL(r->path_d.alloc(16 * npaths));
     HIP_OR_FAIL(r->path_tp.alloc(16 * npaths));
@@ -1023,8 +1033,8 @@ static int enqueue_batch(wsrt_renderer* r, uint32_t nf) {
     if (np) {
         {
             ProfScope p(r, K_PHOTON_TRACE);
-            launch_photon_trace(pl.sv, pl.fa, np, nf, pl.count_traversal, r->photons.as<wsrt_photon>(),
-                                pl.ctr, pl.gstack, s);
+            launch_photon_trace(pl.sv, pl.fa, np, nf, r->cfg.photon_max_depth, pl.count_traversal, r->phs,
+                                r->photons.as<wsrt_photon>(), pl.ctr, pl.gstack, s);
         }
         ProfScope p(r, K_GRID_BUILD);
         if (grid_build(r->photons.as<wsrt_photon>(), np, nf, r->cfg.hash_size, pl.fa, r->gb, s))

@@ -63,6 +63,7 @@ struct Counters {
     uint32_t active[kMaxBounces + 1];        // paths entering bounce b
     uint32_t shade[kMaxBounces][kMatBins];  // bounce b's hits per material type
     uint32_t shadow[kMaxBounces];           // shadow rays queued at bounce b
+    uint32_t p_active[kMaxBounces + 1];     // photon paths entering bounce b
     uint32_t stored;                        // photons stored this frame
     uint32_t valid;                         // photons in the grid
     uint32_t pad[2];
@@ -91,6 +92,16 @@ struct PathState {
     uint32_t* shade_q;  // kMatBins * capacity
     float4* shadow;     // 3 float4 per shadow ray: o+tmax, d+path, contribution
     uint32_t capacity;
+};
+
+// Photon paths between bounces (structure of arrays indexed by photon slot).
+struct PhotonState {
+    float4* pos;        // origin xyz, wavelength
+    float4* dir;        // direction xyz, energy
+    uint32_t* rng;
+    uint32_t* flags;    // 1: caustic path
+    uint32_t* queue_a;
+    uint32_t* queue_b;
 };
 
 // Photon grid (counting sort view of build_grid.wgsl).

@@ -21,8 +21,10 @@ size_t scan_temp_words(uint32_t n);
 int scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* temp,
                    uint32_t* total_out, hipStream_t s);
 size_t radix_temp_words(uint32_t n);
+// Stable LSD sort of (key, value) pairs by the low `bits` bits of the keys.
+// n_dev (optional): device-resident element count <= n.
 int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, uint32_t* temp,
-                     hipStream_t s, bool* result_in_temp);
+                     hipStream_t s, bool* result_in_temp, const uint32_t* n_dev = nullptr);
 void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 void iota_u32(uint32_t* p, uint32_t n, hipStream_t s);
 
@@ -66,8 +68,8 @@ struct GridBuffers {
 // Photon paths of nf consecutive frames, np per frame: path i is photon
 // i % np of frame slot i / np, stored at slot i.
 void launch_photon_trace(const SceneView& sv, const FrameArgs& fa, uint32_t np, uint32_t nf,
-                         uint32_t count_traversal, wsrt_photon* photons, Counters* ctr, uint2* gstack,
-                         hipStream_t s);
+                         uint32_t max_depth, uint32_t count_traversal, const PhotonState& ps,
+                         wsrt_photon* photons, Counters* ctr, uint2* gstack, hipStream_t s);
 size_t grid_temp_words(uint32_t n, uint32_t cells);
 // One counting sort over the grids of nf frames: cell (frame slot, hash)
 // = slot * hash_size + hash; count/offset hold nf * hash_size cells.

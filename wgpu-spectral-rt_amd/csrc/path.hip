@@ -33,6 +33,7 @@ __global__ __launch_bounds__(256) void k_frame_prep(Counters* ctr, uint32_t n_pa
     uint32_t* s = &ctr->shade[0][0];
     for (uint32_t i = threadIdx.x; i < (uint32_t)(kMaxBounces * kMatBins); i += blockDim.x) s[i] = 0u;
     for (uint32_t i = threadIdx.x; i < (uint32_t)kMaxBounces; i += blockDim.x) ctr->shadow[i] = 0u;
+    for (uint32_t i = threadIdx.x; i <= (uint32_t)kMaxBounces; i += blockDim.x) ctr->p_active[i] = 0u;
     if (threadIdx.x == 0) {
         ctr->stored = 0;
         ctr->valid = 0;

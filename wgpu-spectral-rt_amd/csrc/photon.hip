@@ -15,30 +15,24 @@ namespace wsrt {
 
 namespace {
 
-__global__ __launch_bounds__(kTraceThreads) void k_photon_trace(SceneView sv, FrameArgs fa,
-                                                                uint32_t np, uint32_t nf,
-                                                                uint32_t count_traversal,
-                                                                wsrt_photon* __restrict__ photons,
-                                                                Counters* __restrict__ ctr,
-                                                                uint2* __restrict__ gstack) {
-    const uint32_t base_frame = *fa.frame;
-    const FrameConst& f0 = fa.table[base_frame & fa.mask];
-    const uint32_t photon_max_depth = f0.photon_max_depth;
-    __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack st;
-    st.lds = lds_stack + threadIdx.x;
-    st.lds_stride = kTraceThreads;
-    st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const CauchyConst cc{f0.cdf_min, f0.cdf_range};
-    const uint32_t n_photons = np * nf;
-    uint32_t my_rays = 0, my_stored = 0, my_nodes = 0, my_tris = 0;
-    const bool count = count_traversal != 0u;
-    for (uint32_t id = gtid; id < n_photons; id += stride) {
-        float4 out0 = splat4(0.0f), out1 = splat4(0.0f);
+// Emission of photon path i (photon_emit.wgsl:159-216): light choice, point
+// on the light, cosine direction, hero wavelength and flux. Every slot is
+// cleared; emitted paths enter the first bounce queue in slot order.
+__global__ __launch_bounds__(256) void k_photon_emit(SceneView sv, FrameArgs fa, uint32_t np,
+                                                     uint32_t nf, PhotonState ps,
+                                                     wsrt_photon* __restrict__ photons,
+                                                     Counters* __restrict__ ctr) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = np * nf;
+    bool queued = false;
+    if (id < n) {
+        const uint32_t base_frame = *fa.frame;
+        const FrameConst& f0 = fa.table[base_frame & fa.mask];
         const uint32_t slot = id / np, local = id - slot * np;
         uint32_t rng = pcg_hash(local + (base_frame + slot) * 927163u);
+        float4* dst = reinterpret_cast<float4*>(photons + id);
+        dst[0] = splat4(0.0f);
+        dst[1] = splat4(0.0f);
         const uint32_t nl = sv.n_lights;
         if (nl != 0u) {
             uint32_t li = (uint32_t)(rand01(rng) * (float)nl);
@@ -64,18 +58,60 @@ __global__ __launch_bounds__(kTraceThreads) void k_photon_trace(SceneView sv, Fr
                 emit_pos = a * u + b * v + c * w;
                 emit_dir = normalize(nrm + random_unit_vector(rng));
             }
+            const CauchyConst cc{f0.cdf_min, f0.cdf_range};
             float wl = cauchy_sample(rand01(rng), cc);
             float pdf = cauchy_pdf(wl, cc);
             float radiance = blackbody(wl, prm.y);
             float flux = (radiance / pdf) * area * kPi;
             float energy = flux * prm.z;
             f3 pos = emit_pos + emit_dir * 0.001f;
-            f3 dir = emit_dir;
-            bool caustic = false;
-            for (uint32_t depth = 0; depth < photon_max_depth; ++depth) {
-                Hit h = trace_ray<false>(sv, pos, dir, 0.0001f, 100.0f, st, count, my_nodes, my_tris);
-                my_rays++;
-                if (h.tri == kInvalid) break;
+            ps.pos[id] = f4(pos, wl);
+            ps.dir[id] = f4(emit_dir, energy);
+            ps.rng[id] = rng;
+            ps.flags[id] = 0u;
+            queued = f0.photon_max_depth > 0u;
+        }
+    }
+    const uint32_t q = wave_append(&ctr->p_active[0], queued);
+    if (queued) ps.queue_a[q] = id;
+}
+
+// One bounce of every queued photon path (photon_emit.wgsl:219-330): closest
+// hit, then store at a diffuse surface if the path is caustic, or continue
+// through metal, glass and thin film. Survivors are compacted into the next
+// queue, so the few paths refracting through glass run in full waves.
+__global__ __launch_bounds__(kTraceThreads) void k_photon_bounce(SceneView sv, uint32_t depth,
+                                                                 uint32_t count_traversal,
+                                                                 PhotonState ps,
+                                                                 const uint32_t* __restrict__ queue,
+                                                                 uint32_t* __restrict__ next_queue,
+                                                                 wsrt_photon* __restrict__ photons,
+                                                                 Counters* __restrict__ ctr,
+                                                                 uint2* __restrict__ gstack) {
+    __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack st;
+    st.lds = lds_stack + threadIdx.x;
+    st.lds_stride = kTraceThreads;
+    st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
+    const uint32_t n = ctr->p_active[depth];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const bool count = count_traversal != 0u;
+    uint32_t my_rays = 0, my_stored = 0, my_nodes = 0, my_tris = 0;
+    for (uint32_t base = gtid & ~63u; base < n; base += stride) {
+        const uint32_t i = base + __lane_id();
+        bool alive = false;
+        uint32_t id = 0;
+        if (i < n) {
+            id = queue[i];
+            const float4 pw = ps.pos[id], de = ps.dir[id];
+            f3 pos = xyz(pw), dir = xyz(de);
+            const float wl = pw.w, energy = de.w;
+            uint32_t rng = ps.rng[id];
+            bool caustic = ps.flags[id] != 0u;
+            Hit h = trace_ray<false>(sv, pos, dir, 0.0001f, 100.0f, st, count, my_nodes, my_tris);
+            my_rays++;
+            if (h.tri != kInvalid) {
                 uint32_t mat_id;
                 f3 wn = hit_world_normal(sv, h.tri, h.u, h.v, &mat_id);
                 f3 ff = h.front ? wn : -wn;
@@ -85,14 +121,15 @@ __global__ __launch_bounds__(kTraceThreads) void k_photon_trace(SceneView sv, Fr
                 f3 hp = pos + dir * h.t;
                 if (type == 0u) {
                     if (caustic) {
-                        out0 = make_float4(hp.x, hp.y, hp.z, wl);
-                        out1 = make_float4(-dir.x, -dir.y, -dir.z, energy);
+                        float4* dst = reinterpret_cast<float4*>(photons + id);
+                        dst[0] = make_float4(hp.x, hp.y, hp.z, wl);
+                        dst[1] = make_float4(-dir.x, -dir.y, -dir.z, energy);
                         my_stored++;
                     }
-                    break;
                 } else if (type == 1u) {
                     dir = reflect(normalize(dir), ff);
                     pos = hp + ff * 0.001f;
+                    alive = true;
                 } else if (type == 2u) {
                     float lum = wl / 1000.0f;
                     float ir = ex.z + (ex.w / (lum * lum));
@@ -112,6 +149,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_photon_trace(SceneView sv, Fr
                     }
                     f3 off = dot(dir, ff) > 0.0f ? ff : -ff;
                     pos = hp + off * 0.001f;
+                    alive = true;
                 } else if (type == 4u) {
                     float n2 = ex.z, n3 = ex.w, d = fmaxf(ex.y, 50.0f);
                     f3 ud = normalize(dir);
@@ -126,17 +164,20 @@ __global__ __launch_bounds__(kTraceThreads) void k_photon_trace(SceneView sv, Fr
                         else dir = refract(ud, ff, ratio);
                     }
                     caustic = true;
+                    alive = true;
                     // the origin is not advanced in this branch (photon_emit.wgsl:291-326)
-                } else {
-                    break;
+                }
+                if (alive) {
+                    ps.pos[id] = f4(pos, wl);
+                    ps.dir[id] = f4(dir, energy);
+                    ps.rng[id] = rng;
+                    ps.flags[id] = caustic ? 1u : 0u;
                 }
             }
         }
-        float4* dst = reinterpret_cast<float4*>(photons + id);
-        dst[0] = out0;
-        dst[1] = out1;
+        const uint32_t q = wave_append(&ctr->p_active[depth + 1], alive);
+        if (alive) next_queue[q] = id;
     }
-    // wave-aggregated statistics
     if (count) flush_traversal_counts(ctr, 0, my_rays, my_nodes, my_tris);
     for (int off = 32; off > 0; off >>= 1) {
         my_rays += __shfl_xor(my_rays, off, 64);
@@ -149,45 +190,59 @@ __global__ __launch_bounds__(kTraceThreads) void k_photon_trace(SceneView sv, Fr
     }
 }
 
-// cell id per slot (hash_size marks skipped photons so they sort last) and
-// the atomic histogram.
+// Cell of every photon slot and the atomic histogram. Grid cell = frame
+// slot * hash_size + hash; skipped photons (energy 0) get no cell.
 __global__ __launch_bounds__(256) void k_grid_hash(const wsrt_photon* __restrict__ photons,
-                                                   uint32_t n, uint32_t np, uint32_t nf, FrameArgs fa,
+                                                   uint32_t n, uint32_t np, FrameArgs fa,
                                                    uint32_t* __restrict__ cell_id,
                                                    uint32_t* __restrict__ keys,
-                                                   uint32_t* __restrict__ vals,
-                                                   uint32_t* __restrict__ count,
-                                                   uint32_t* __restrict__ n_valid) {
+                                                   uint32_t* __restrict__ valid_flag,
+                                                   uint32_t* __restrict__ count) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = false;
-    if (i < n) {
-        const uint32_t slot = i / np;
-        const FrameConst& fc = fa.table[(*fa.frame + slot) & fa.mask];
-        const float4 a = reinterpret_cast<const float4*>(photons + i)[0];
-        const float e = reinterpret_cast<const float4*>(photons + i)[1].w;
-        uint32_t c = kInvalid, key = nf * fc.hash_size;
-        if (e != 0.0f) {
-            c = hash_cell(xyz(a), mk3(fc.jitter[0], fc.jitter[1], fc.jitter[2]), fc.cell_size,
-                          fc.hash_size);
-            key = slot * fc.hash_size + c;
-            atomicAdd(&count[key], 1u);
-            valid = true;
-        }
-        cell_id[i] = c;
-        keys[i] = key;
-        vals[i] = i;
+    if (i >= n) return;
+    const uint32_t slot = i / np;
+    const FrameConst& fc = fa.table[(*fa.frame + slot) & fa.mask];
+    const float4 a = reinterpret_cast<const float4*>(photons + i)[0];
+    const float e = reinterpret_cast<const float4*>(photons + i)[1].w;
+    uint32_t c = kInvalid, key = 0, valid = 0;
+    if (e != 0.0f) {
+        c = hash_cell(xyz(a), mk3(fc.jitter[0], fc.jitter[1], fc.jitter[2]), fc.cell_size, fc.hash_size);
+        key = slot * fc.hash_size + c;
+        atomicAdd(&count[key], 1u);
+        valid = 1;
     }
-    uint64_t m = __ballot(valid);
-    if (__lane_id() == 0 && m) atomicAdd(n_valid, (uint32_t)__popcll(m));
+    cell_id[i] = c;
+    keys[i] = key;
+    valid_flag[i] = valid;
+}
+
+// Stream compaction of the valid photons in slot order (positions from an
+// exclusive scan of the flags), so the sort sees only stored photons.
+__global__ __launch_bounds__(256) void k_grid_compact(uint32_t n, const uint32_t* __restrict__ keys,
+                                                      const uint32_t* __restrict__ valid_flag,
+                                                      const uint32_t* __restrict__ pos,
+                                                      uint32_t* __restrict__ ckeys,
+                                                      uint32_t* __restrict__ cvals) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !valid_flag[i]) return;
+    ckeys[pos[i]] = keys[i];
+    cvals[pos[i]] = i;
+}
+
+__global__ void k_grid_copy_count(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
+    if (threadIdx.x == 0) *dst = *src;
 }
 
 __global__ __launch_bounds__(256) void k_grid_copy(const wsrt_photon* __restrict__ photons,
-                                                   const uint32_t* __restrict__ sorted_slot,
+                                                   const uint32_t* __restrict__ sorted_vals,
                                                    const uint32_t* __restrict__ n_valid,
+                                                   uint32_t* __restrict__ sorted_slot,
                                                    wsrt_photon* __restrict__ sorted) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= *n_valid) return;
-    const float4* src = reinterpret_cast<const float4*>(photons + sorted_slot[k]);
+    const uint32_t slot = sorted_vals[k];
+    sorted_slot[k] = slot;
+    const float4* src = reinterpret_cast<const float4*>(photons + slot);
     float4* dst = reinterpret_cast<float4*>(sorted + k);
     dst[0] = src[0];
     dst[1] = src[1];
@@ -196,25 +251,34 @@ __global__ __launch_bounds__(256) void k_grid_copy(const wsrt_photon* __restrict
 }  // namespace
 
 void launch_photon_trace(const SceneView& sv, const FrameArgs& fa, uint32_t np, uint32_t nf,
-                         uint32_t count_traversal, wsrt_photon* photons, Counters* ctr, uint2* gstack,
-                         hipStream_t s) {
-    hipLaunchKernelGGL(k_photon_trace, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, sv, fa, np, nf,
-                       count_traversal, photons, ctr, gstack);
+                         uint32_t max_depth, uint32_t count_traversal, const PhotonState& ps,
+                         wsrt_photon* photons, Counters* ctr, uint2* gstack, hipStream_t s) {
+    const uint32_t n = np * nf;
+    hipLaunchKernelGGL(k_photon_emit, dim3((n + 255) / 256), dim3(256), 0, s, sv, fa, np, nf, ps, photons, ctr);
+    for (uint32_t d = 0; d < max_depth; ++d) {
+        const uint32_t* q = (d & 1u) ? ps.queue_b : ps.queue_a;
+        uint32_t* nq = (d & 1u) ? ps.queue_a : ps.queue_b;
+        hipLaunchKernelGGL(k_photon_bounce, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, sv, d,
+                           count_traversal, ps, q, nq, photons, ctr, gstack);
+    }
 }
 
 size_t grid_temp_words(uint32_t n, uint32_t cells) {
     size_t sort = radix_temp_words(n);
-    size_t scan = scan_temp_words(cells);
-    return 2 * (size_t)n + (sort > scan ? sort : scan) + 4;
+    size_t scan = scan_temp_words(cells > n ? cells : n);
+    return 5 * (size_t)n + (sort > scan ? sort : scan) + 4;
 }
 
 int grid_build(const wsrt_photon* photons, uint32_t np, uint32_t nf, uint32_t hash_size,
                const FrameArgs& fa, GridBuffers& g, hipStream_t s) {
-    // g.temp layout: keys[n], vals[n], sort/scan temp, n_valid
+    // g.temp: keys[n] flags[n] pos[n] ckeys[n] cvals[n] | sort/scan work | n_valid
     const uint32_t n = np * nf, cells = hash_size * nf;
     uint32_t* keys = g.temp;
-    uint32_t* vals = g.temp + n;
-    uint32_t* work = g.temp + 2 * (size_t)n;
+    uint32_t* flags = keys + n;
+    uint32_t* pos = flags + n;
+    uint32_t* ckeys = pos + n;
+    uint32_t* cvals = ckeys + n;
+    uint32_t* work = cvals + n;
     uint32_t* n_valid = g.temp + grid_temp_words(n, cells) - 1;
     fill_u32(g.count, cells, 0u, s);
     fill_u32(n_valid, 1, 0u, s);
@@ -222,21 +286,20 @@ int grid_build(const wsrt_photon* photons, uint32_t np, uint32_t nf, uint32_t ha
         if (scan_exclusive(g.count, g.offset, cells, work, nullptr, s)) return -1;
         return 0;
     }
-    hipLaunchKernelGGL(k_grid_hash, dim3((n + 255) / 256), dim3(256), 0, s, photons, n, np, nf, fa,
-                       g.cell_id, keys, vals, g.count, n_valid);
+    hipLaunchKernelGGL(k_grid_hash, dim3((n + 255) / 256), dim3(256), 0, s, photons, n, np, fa, g.cell_id,
+                       keys, flags, g.count);
+    if (scan_exclusive(flags, pos, n, work, n_valid, s)) return -1;
+    hipLaunchKernelGGL(k_grid_compact, dim3((n + 255) / 256), dim3(256), 0, s, n, keys, flags, pos, ckeys,
+                       cvals);
     if (scan_exclusive(g.count, g.offset, cells, work, nullptr, s)) return -1;
     int bits = 1;
-    while ((1ull << bits) <= (uint64_t)cells) bits++;  // keys in [0, cells]
+    while ((1ull << bits) < (uint64_t)cells) bits++;  // keys in [0, cells)
     bool in_tmp = false;
-    if (radix_sort_pairs(keys, vals, n, bits, work, s, &in_tmp)) return -1;
-    const uint32_t* sorted_slot = in_tmp ? work + n : vals;
-    if (hipMemcpyAsync(g.sorted_slot, sorted_slot, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s) !=
-        hipSuccess)
-        return -1;
-    hipLaunchKernelGGL(k_grid_copy, dim3((n + 255) / 256), dim3(256), 0, s, photons, g.sorted_slot,
-                       n_valid, g.sorted);
-    if (hipMemcpyAsync(g.n_valid, n_valid, sizeof(uint32_t), hipMemcpyDeviceToDevice, s) != hipSuccess)
-        return -1;
+    if (radix_sort_pairs(ckeys, cvals, n, bits, work, s, &in_tmp, n_valid)) return -1;
+    const uint32_t* sorted_vals = in_tmp ? work + n : cvals;
+    hipLaunchKernelGGL(k_grid_copy, dim3((n + 255) / 256), dim3(256), 0, s, photons, sorted_vals, n_valid,
+                       g.sorted_slot, g.sorted);
+    hipLaunchKernelGGL(k_grid_copy_count, dim3(1), dim3(64), 0, s, n_valid, g.n_valid);
     return 0;
 }
 

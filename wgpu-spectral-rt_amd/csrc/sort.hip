@@ -126,10 +126,14 @@ constexpr int kSortRounds = 16;
 constexpr int kSortTile = kSortThreads * kSortRounds;  // 4096 keys per tile
 constexpr int kRadix = 256;
 
+// n_dev (optional): the element count in device memory (at most n), so a
+// sort whose size is produced on the device can be launched for its maximum.
 __global__ __launch_bounds__(kSortThreads) void k_radix_histogram(const uint32_t* __restrict__ keys,
-                                                                  uint32_t n, int shift,
-                                                                  uint32_t num_tiles,
+                                                                  uint32_t n,
+                                                                  const uint32_t* __restrict__ n_dev,
+                                                                  int shift, uint32_t num_tiles,
                                                                   uint32_t* __restrict__ hist) {
+    if (n_dev) n = min(n, *n_dev);
     __shared__ uint32_t h[kRadix];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -149,8 +153,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_histogram(const uint32_t
 // a round are ordered by per-digit counts in LDS, rounds by a running count.
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
-    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, int shift,
-    uint32_t num_tiles, const uint32_t* __restrict__ hist_scanned) {
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n,
+    const uint32_t* __restrict__ n_dev, int shift, uint32_t num_tiles,
+    const uint32_t* __restrict__ hist_scanned) {
+    if (n_dev) n = min(n, *n_dev);
     constexpr int kWaves = kSortThreads / 64;
     __shared__ uint32_t run[kRadix];
     __shared__ uint32_t wave_cnt[kWaves][kRadix];
@@ -237,7 +243,7 @@ size_t radix_temp_words(uint32_t n) {
 }
 
 int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, uint32_t* temp,
-                     hipStream_t s, bool* result_in_temp) {
+                     hipStream_t s, bool* result_in_temp, const uint32_t* n_dev) {
     // temp layout: keys_alt[n], vals_alt[n], hist[tiles*256], hist_scan[tiles*256], scan temp
     *result_in_temp = false;
     if (n <= 1) return 0;
@@ -252,11 +258,11 @@ int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, uint3
     int passes = (bits + 7) / 8;
     for (int p = 0; p < passes; ++p) {
         int shift = 8 * p;
-        hipLaunchKernelGGL(k_radix_histogram, dim3(tiles), dim3(kSortThreads), 0, s, ki, n, shift,
+        hipLaunchKernelGGL(k_radix_histogram, dim3(tiles), dim3(kSortThreads), 0, s, ki, n, n_dev, shift,
                            tiles, hist);
         if (scan_exclusive(hist, hscan, hist_n, stemp, nullptr, s)) return -1;
         hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kSortThreads), 0, s, ki, vi, ko, vo,
-                           n, shift, tiles, hscan);
+                           n, n_dev, shift, tiles, hscan);
         uint32_t* t;
         t = ki; ki = ko; ko = t;
         t = vi; vi = vo; vo = t;
