@@ -707,27 +707,28 @@ void or_bvh8_closest(const wsrt_bvh8_node* nodes, uint32_t n_nodes, const wsrt_b
             if (st_t[sp] > best_t * BOX_PAD) continue;
             const wsrt_bvh8_node* nd = &nodes[ni];
             nv++;
-            float as[3], ao[3], sl[3];
+            float as[3], ao[3], sl = 0.0f;
             for (int a = 0; a < 3; ++a) {
                 as[a] = exp2i(nd->exp[a]) * inv[a];
                 ao[a] = (nd->origin[a] - oo[a]) * inv[a];
-                /* bound of the f32 error of the slab values below */
-                sl[a] = (fabsf(as[a]) * 255.0f + fabsf(ao[a])) * 4.0e-7f;
+                sl = fmaxf(sl, fabsf(as[a]) * 255.0f + fabsf(ao[a]));
             }
+            /* bound of the f32 error of the slab values, applied to both ends */
+            sl = sl * 4.0e-7f;
+            const float sl2 = sl * (1.0f + BOX_PAD);
             float cnear[8];
             int chit[8];
             for (int c = 0; c < 8; ++c) {
-                chit[c] = 0;
-                if (!nd->meta[c]) continue;
                 float tn = tmin, tf = best_t;
                 for (int a = 0; a < 3; ++a) {
-                    float t0 = fmaf((float)nd->qlo[a][c], as[a], ao[a]);
-                    float t1 = fmaf((float)nd->qhi[a][c], as[a], ao[a]);
-                    tn = fmaxf(tn, fminf(t0, t1) - sl[a]);
-                    tf = fminf(tf, fmaxf(t0, t1) + sl[a]);
+                    int ng = inv[a] < 0.0f;
+                    float qn = (float)(ng ? nd->qhi[a][c] : nd->qlo[a][c]);
+                    float qf = (float)(ng ? nd->qlo[a][c] : nd->qhi[a][c]);
+                    tn = fmaxf(tn, fmaf(qn, as[a], ao[a]));
+                    tf = fminf(tf, fmaf(qf, as[a], ao[a]));
                 }
-                cnear[c] = tn;
-                chit[c] = tn <= tf * BOX_PAD;
+                cnear[c] = tn - sl;
+                chit[c] = nd->meta[c] != 0 && tn <= fmaf(tf, BOX_PAD, sl2);
             }
             /* leaf children first, in slot order */
             for (int c = 0; c < 8; ++c) {

@@ -798,7 +798,7 @@ struct wsrt_renderer {
     DevBuf photons, grid_count, grid_offset, grid_cell, grid_slot, grid_sorted, grid_nvalid, grid_temp;
     DevBuf path_o, path_d, path_tp, path_L, path_wl, path_rng, path_flags, path_hit, q_a, q_b, shade_q,
         shadow;
-    DevBuf accum, rgba, counters, gstack;
+    DevBuf accum, rgba, counters, gstack, qcounts;
     DevBuf ftable, dframe;           // per-frame constants, device frame counter
     DevBuf ph_pos, ph_dir, ph_rng, ph_flags, ph_qa, ph_qb;
     PhotonState phs{};
@@ -907,8 +907,9 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     HIP_OR_FAIL(r->ph_dir.alloc(16 * nph));
     HIP_OR_FAIL(r->ph_rng.alloc(4 * nph));
     HIP_OR_FAIL(r->ph_flags.alloc(4 * nph));
-    HIP_OR_FAIL(r->ph_qa.alloc(4 * nph));
-    HIP_OR_FAIL(r->ph_qb.alloc(4 * nph));
+    const uint32_t pseg = seg_capacity(nph);
+    HIP_OR_FAIL(r->ph_qa.alloc(4ull * kSegs * pseg));
+    HIP_OR_FAIL(r->ph_qb.alloc(4ull * kSegs * pseg));
     r->phs = PhotonState{r->ph_pos.as<float4>(), r->ph_dir.as<float4>(), r->ph_rng.as<uint32_t>(),
                          r->ph_flags.as<uint32_t>(), r->ph_qa.as<uint32_t>(), r->ph_qb.as<uint32_t>()};
     HIP_OR_FAIL(r->path_o.alloc(16 * npaths));
@@ -919,10 +920,14 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     HIP_OR_FAIL(r->path_rng.alloc(4 * npaths));
     HIP_OR_FAIL(r->path_flags.alloc(4 * npaths));
     HIP_OR_FAIL(r->path_hit.alloc(16 * npaths));
-    HIP_OR_FAIL(r->q_a.alloc(4 * npaths));
-    HIP_OR_FAIL(r->q_b.alloc(4 * npaths));
-    HIP_OR_FAIL(r->shade_q.alloc(4 * npaths * kMatBins));
-    HIP_OR_FAIL(r->shadow.alloc(48 * npaths));
+    const uint32_t seg = seg_capacity(npaths);
+    HIP_OR_FAIL(r->q_a.alloc(4ull * kSegs * seg));
+    HIP_OR_FAIL(r->q_b.alloc(4ull * kSegs * seg));
+    HIP_OR_FAIL(r->shade_q.alloc(4ull * kSegs * seg * kMatBins));
+    HIP_OR_FAIL(r->shadow.alloc(48ull * kSegs * seg));
+    const uint32_t depths = std::max(c.max_depth, c.photon_max_depth) + 1;
+    HIP_OR_FAIL(r->qcounts.alloc(4 * queue_count_words(depths)));
+    HIP_OR_FAIL(hipMemsetAsync(r->qcounts.p, 0, 4 * queue_count_words(depths), r->stream));
     HIP_OR_FAIL(r->accum.alloc(16 * npx));
     HIP_OR_FAIL(r->rgba.alloc(4 * npx));
     HIP_OR_FAIL(r->counters.alloc(sizeof(Counters)));
@@ -976,6 +981,9 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     pl.fa.frame = r->dframe.as<uint32_t>();
     pl.fa.mask = kFrameTable - 1;
     pl.count_traversal = 0;
+    pl.qs.counts = r->qcounts.as<uint32_t>();
+    pl.qs.seg_cap = seg_capacity(npaths);
+    pl.qs.pseg_cap = seg_capacity(nph);
     // default camera of CameraController::new (camera.rs:144-162)
     const float pos[3] = {0.0f, 0.0f, 3.0f};
     wsrt_camera_build_uniform(pos, -90.0f * (3.14159265359f / 180.0f), 0.0f, (float)width / (float)height,
@@ -1028,13 +1036,13 @@ static int enqueue_batch(wsrt_renderer* r, uint32_t nf) {
     const uint32_t np = r->cfg.enable_photons ? r->cfg.photons_per_pass : 0u;
     {
         ProfScope p(r, K_FRAME_PREP);
-        launch_frame_prep(pl.ctr, npx * nf, s);
+        launch_clear_queues(pl.qs, std::max(r->cfg.max_depth, r->cfg.photon_max_depth) + 1, s);
     }
     if (np) {
         {
             ProfScope p(r, K_PHOTON_TRACE);
             launch_photon_trace(pl.sv, pl.fa, np, nf, r->cfg.photon_max_depth, pl.count_traversal, r->phs,
-                                r->photons.as<wsrt_photon>(), pl.ctr, pl.gstack, s);
+                                pl.qs, r->photons.as<wsrt_photon>(), pl.ctr, pl.gstack, s);
         }
         ProfScope p(r, K_GRID_BUILD);
         if (grid_build(r->photons.as<wsrt_photon>(), np, nf, r->cfg.hash_size, pl.fa, r->gb, s))
@@ -1049,7 +1057,7 @@ static int enqueue_batch(wsrt_renderer* r, uint32_t nf) {
         uint32_t* nq = (b & 1u) ? pl.ps.queue_a : pl.ps.queue_b;
         {
             ProfScope p(r, K_EXTEND);
-            launch_extend(pl, b, q, s);
+            launch_extend(pl, b, npx * nf, q, s);
         }
         {
             ProfScope p(r, K_SHADE);
@@ -1207,7 +1215,8 @@ extern "C" int wsrt_renderer_write_accum(wsrt_renderer* r, const float* in) {
 
 extern "C" int wsrt_renderer_get_stats(wsrt_renderer* r, wsrt_frame_stats* out) {
     CHECK_ARG(r && out);
-    Counters c;
+    std::vector<Counters> cv(1);
+    Counters& c = cv[0];
     int rc = read_back(r, &c, r->counters.p, sizeof c);
     if (rc) return rc;
     out->frames = r->frames;
@@ -1215,8 +1224,12 @@ extern "C" int wsrt_renderer_get_stats(wsrt_renderer* r, wsrt_frame_stats* out) 
     out->extend_rays = c.extend_rays;
     out->shadow_rays = c.shadow_rays;
     out->photon_paths = r->photon_paths;
-    out->photon_rays = c.photon_rays;
-    out->photons_stored = c.photons_stored;
+    out->photon_rays = 0;
+    out->photons_stored = 0;
+    for (int b = 0; b < kStatBlocks; ++b) {
+        out->photon_rays += c.photon_rays_part[b];
+        out->photons_stored += c.photons_stored_part[b];
+    }
     out->guide_samples = c.guide_samples;
     for (int k = 0; k < 3; ++k) {
         out->trav_nodes[k] = c.trav_nodes[k];
@@ -1234,12 +1247,8 @@ extern "C" int wsrt_renderer_set_traversal_counting(wsrt_renderer* r, int enable
 extern "C" int wsrt_renderer_reset_stats(wsrt_renderer* r) {
     CHECK_ARG(r);
     HIP_OR_FAIL(hipSetDevice(r->dev->ordinal));
+    HIP_OR_FAIL(hipMemsetAsync(r->counters.p, 0, sizeof(Counters), r->stream));
     HIP_OR_FAIL(hipStreamSynchronize(r->stream));
-    Counters c;
-    HIP_OR_FAIL(hipMemcpy(&c, r->counters.p, sizeof c, hipMemcpyDeviceToHost));
-    c.extend_rays = c.shadow_rays = c.photon_rays = c.photons_stored = c.guide_samples = 0;
-    for (int k = 0; k < 3; ++k) c.trav_nodes[k] = c.trav_tris[k] = c.trav_rays[k] = 0;
-    HIP_OR_FAIL(hipMemcpy(r->counters.p, &c, sizeof c, hipMemcpyHostToDevice));
     r->frames = r->samples = r->photon_paths = 0;
     return WSRT_OK;
 }

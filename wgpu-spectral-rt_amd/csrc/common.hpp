@@ -55,26 +55,21 @@ struct FrameParams {
 
 constexpr int kMatBins = 8;
 constexpr int kMaxBounces = 64;
+constexpr int kStatBlocks = 2048;  // blocks of the photon bounce launch
 
-// Device-side counters of the frame in flight. Queue sizes are kept per
-// bounce, so no kernel has to reset them between bounces: frame_prep clears
-// them once per frame.
+// Running statistics of a renderer. Totals over a whole launch are added
+// once per launch; per-path totals are kept per producer block (plain
+// read-modify-write by that block only) and summed on the host, so no
+// kernel issues one atomic per wave on a shared address.
 struct Counters {
-    uint32_t active[kMaxBounces + 1];        // paths entering bounce b
-    uint32_t shade[kMaxBounces][kMatBins];  // bounce b's hits per material type
-    uint32_t shadow[kMaxBounces];           // shadow rays queued at bounce b
-    uint32_t p_active[kMaxBounces + 1];     // photon paths entering bounce b
-    uint32_t stored;                        // photons stored this frame
-    uint32_t valid;                         // photons in the grid
-    uint32_t pad[2];
     unsigned long long extend_rays;
     unsigned long long shadow_rays;
-    unsigned long long photon_rays;
-    unsigned long long photons_stored;
     unsigned long long guide_samples;
-    unsigned long long trav_nodes[3];  // photon, extend, shadow
+    unsigned long long trav_nodes[3];  // photon, extend, shadow (while counting)
     unsigned long long trav_tris[3];
     unsigned long long trav_rays[3];
+    unsigned long long photon_rays_part[kStatBlocks];
+    unsigned long long photons_stored_part[kStatBlocks];
 };
 
 // Wavefront path state, structure of arrays indexed by path id (= pixel).
@@ -87,11 +82,11 @@ struct PathState {
     uint32_t* rng;      // pcg state
     uint32_t* flags;    // bit0 has_hit_diffuse, bit1 caustic path
     uint4* hit;         // t bits, tri | front<<31, u bits, v bits
-    uint32_t* queue_a;  // active path ids
-    uint32_t* queue_b;  // next bounce
-    uint32_t* shade_q;  // kMatBins * capacity
+    uint32_t* queue_a;  // path ids entering a bounce (segmented, 64 * seg_cap)
+    uint32_t* queue_b;  // and the next bounce
+    uint32_t* shade_q;  // hits per material type (kMatBins * 64 * seg_cap)
     float4* shadow;     // 3 float4 per shadow ray: o+tmax, d+path, contribution
-    uint32_t capacity;
+    uint32_t capacity;  // path slots (frames in flight x pixels)
 };
 
 // Photon paths between bounces (structure of arrays indexed by photon slot).

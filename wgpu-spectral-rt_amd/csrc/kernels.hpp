@@ -5,6 +5,7 @@
 
 #include "common.hpp"
 #include "dev_math.hpp"
+#include "queues.hpp"
 
 namespace wsrt {
 
@@ -69,7 +70,8 @@ struct GridBuffers {
 // i % np of frame slot i / np, stored at slot i.
 void launch_photon_trace(const SceneView& sv, const FrameArgs& fa, uint32_t np, uint32_t nf,
                          uint32_t max_depth, uint32_t count_traversal, const PhotonState& ps,
-                         wsrt_photon* photons, Counters* ctr, uint2* gstack, hipStream_t s);
+                         const Queues& qs, wsrt_photon* photons, Counters* ctr, uint2* gstack,
+                         hipStream_t s);
 size_t grid_temp_words(uint32_t n, uint32_t cells);
 // One counting sort over the grids of nf frames: cell (frame slot, hash)
 // = slot * hash_size + hash; count/offset hold nf * hash_size cells.
@@ -87,13 +89,15 @@ struct PathLaunch {
     uint8_t* rgba;
     uint32_t count_traversal;
     FrameArgs fa;
+    Queues qs;
 };
-void launch_frame_prep(Counters* ctr, uint32_t n_paths, hipStream_t s);
+void launch_clear_queues(const Queues& qs, uint32_t depths, hipStream_t s);
 void launch_advance_frame(uint32_t* frame, uint32_t nf, hipStream_t s);
 void launch_set_frame(uint32_t* frame, uint32_t value, hipStream_t s);
 // A batch of nf consecutive frames of npx pixels shares one wavefront.
 void launch_generate(const PathLaunch& pl, uint32_t npx, uint32_t nf, hipStream_t s);
-void launch_extend(const PathLaunch& pl, uint32_t bounce, const uint32_t* queue, hipStream_t s);
+// bounce 0 reads path ids 0 .. n0-1 directly
+void launch_extend(const PathLaunch& pl, uint32_t bounce, uint32_t n0, const uint32_t* queue, hipStream_t s);
 void launch_shade(const PathLaunch& pl, uint32_t npx, uint32_t bounce, uint32_t* next_queue, hipStream_t s);
 void launch_shadow(const PathLaunch& pl, uint32_t bounce, hipStream_t s);
 void launch_accumulate(const PathLaunch& pl, uint32_t npx, uint32_t nf, hipStream_t s);

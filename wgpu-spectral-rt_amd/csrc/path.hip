@@ -4,7 +4,7 @@
 //   generate  -> [extend -> shade -> shadow] x max_depth -> accumulate
 // with the path state in HBM (structure of arrays indexed by pixel):
 //   extend  closest hit of every queued path; hits are binned by material
-//           type (sort-by-material) with wave-aggregated appends;
+//           type (sort-by-material) into segmented queues (queues.hpp);
 //   shade   emission, photon gather, light sample, BSDF sample, roulette of
 //           each bin, so a wave mostly runs one material's code; surviving
 //           paths are compacted into the next queue;
@@ -15,6 +15,7 @@
 // radiance in the reference's order (gather, light sample, then the next
 // bounce), so each path's result matches the per-pixel reference.
 #include "kernels.hpp"
+#include "queues.hpp"
 #include "shading.hpp"
 
 namespace wsrt {
@@ -23,24 +24,17 @@ namespace {
 
 constexpr uint32_t kFlagDiffuse = 1u, kFlagCaustic = 2u;
 constexpr int kShadeThreads = 256;
-constexpr int kShadeBlocks = 1024;
+constexpr int kShadeBlocks = (int)(kGridLanes / kShadeThreads);
+static_assert((uint64_t)kTraceBlocks * kTraceThreads == kGridLanes, "queue producers use kGridLanes");
 
-// Clears the per-bounce queue sizes of a frame (a fill kernel, so the
-// frame can be captured in a graph).
-__global__ __launch_bounds__(256) void k_frame_prep(Counters* ctr, uint32_t n_paths) {
-    uint32_t* a = ctr->active;
-    for (uint32_t i = threadIdx.x; i <= (uint32_t)kMaxBounces; i += blockDim.x) a[i] = i == 0 ? n_paths : 0u;
-    uint32_t* s = &ctr->shade[0][0];
-    for (uint32_t i = threadIdx.x; i < (uint32_t)(kMaxBounces * kMatBins); i += blockDim.x) s[i] = 0u;
-    for (uint32_t i = threadIdx.x; i < (uint32_t)kMaxBounces; i += blockDim.x) ctr->shadow[i] = 0u;
-    for (uint32_t i = threadIdx.x; i <= (uint32_t)kMaxBounces; i += blockDim.x) ctr->p_active[i] = 0u;
-    if (threadIdx.x == 0) {
-        ctr->stored = 0;
-        ctr->valid = 0;
-    }
+// Clears the queue counters of a batch (a fill kernel, so it can be
+// captured in a graph).
+__global__ void k_clear_queues(uint32_t* counts, uint32_t words) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x)
+        counts[i] = 0u;
 }
 
-// Last node of a frame: the next frame is shaded with frame_count + 1.
+// Last node of a batch: the next batch starts at frame_count + nf.
 __global__ void k_advance_frame(uint32_t* frame, uint32_t nf) {
     if (threadIdx.x == 0) *frame = *frame + nf;
 }
@@ -49,7 +43,6 @@ __global__ void k_set_frame(uint32_t* frame, uint32_t value) {
     if (threadIdx.x == 0) *frame = value;
 }
 
-// shader.wgsl:709-767
 // Path p of a batch of frames belongs to frame slot p / npx, pixel p % npx.
 __global__ __launch_bounds__(256) void k_generate(PathState ps, FrameArgs fa, uint32_t npx, uint32_t nf) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -97,35 +90,34 @@ __global__ __launch_bounds__(256) void k_generate(PathState ps, FrameArgs fa, ui
     ps.wl[p] = w;
     ps.rng[p] = rng;
     ps.flags[p] = 0u;
-    ps.queue_a[p] = p;
 }
 
-__global__ __launch_bounds__(kTraceThreads) void k_extend(SceneView sv, PathState ps,
+// Bounce 0 reads path ids 0 .. n0-1 directly (every path starts); later
+// bounces read the compacted queue of the previous shade.
+__global__ __launch_bounds__(kTraceThreads) void k_extend(SceneView sv, PathState ps, Queues qs,
                                                           uint32_t count_traversal,
                                                           Counters* __restrict__ ctr, uint32_t bounce,
-                                                          const uint32_t* __restrict__ queue,
+                                                          uint32_t n0, const uint32_t* __restrict__ queue,
                                                           uint2* __restrict__ gstack) {
     __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
+    __shared__ uint32_t pre[kSegs + 1];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;
     st.lds = lds_stack + threadIdx.x;
     st.lds_stride = kTraceThreads;
     st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
-    const uint32_t n = ctr->active[bounce];
+    const uint32_t n = bounce == 0 ? n0 : seg_prefix(qs, bounce, kQActive, 1, pre);
     if (gtid == 0) atomicAdd(&ctr->extend_rays, (unsigned long long)n);
-    uint32_t* bins = ctr->shade[bounce];
     const uint32_t stride = gridDim.x * blockDim.x;
-    // every lane of a wave runs the same number of iterations (wave-uniform
-    // trip count) so the binning ballots below see the whole wave
-    const uint32_t wave_base = gtid & ~63u;
     const bool count = count_traversal != 0u;
     uint32_t my_rays = 0, my_nodes = 0, my_tris = 0;
-    for (uint32_t base = wave_base; base < n; base += stride) {
+    // wave-uniform trip count, so the binning ballots see the whole wave
+    for (uint32_t base = gtid & ~63u; base < n; base += stride) {
         const uint32_t i = base + __lane_id();
         bool have = i < n;
         uint32_t p = 0, bin = 0;
         if (have) {
-            p = queue[i];
+            p = bounce == 0 ? i : queue[seg_entry(pre, kSegs, i, qs.seg_cap)];
             const float4 o = ps.o[p], d = ps.d[p];
             Hit h = trace_ray<false>(sv, xyz(o), xyz(d), 0.0001f, 100.0f, st, count, my_nodes, my_tris);
             my_rays++;
@@ -139,42 +131,35 @@ __global__ __launch_bounds__(kTraceThreads) void k_extend(SceneView sv, PathStat
                 bin = type < (uint32_t)kMatBins ? type : 0u;
             }
         }
-        // sort-by-material: wave-aggregated append into one queue per type
+        // sort-by-material: one segmented queue per material type
         uint64_t pending = __ballot(have);
         while (pending) {
             const uint32_t leader = __ffsll((unsigned long long)pending) - 1;
             const uint32_t b = __shfl(bin, leader);
             const bool mine = have && bin == b;
-            const uint64_t m = __ballot(mine);
-            uint32_t slot = 0;
-            if (__lane_id() == leader) slot = atomicAdd(&bins[b], (uint32_t)__popcll(m));
-            slot = __shfl(slot, leader);
+            const uint32_t e = seg_append(qs, bounce, kQShade0 + b, qs.seg_cap, mine);
             if (mine) {
-                ps.shade_q[(uint64_t)b * ps.capacity + slot + __popcll(m & lanemask_lt())] = p;
+                ps.shade_q[(size_t)b * kSegs * qs.seg_cap + e] = p;
                 have = false;
             }
-            pending &= ~m;
+            pending &= ~__ballot(mine);
         }
     }
     if (count) flush_traversal_counts(ctr, 1, my_rays, my_nodes, my_tris);
 }
 
 __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState ps, GridView grid,
-                                                         FrameArgs fa, uint32_t npx,
-                                                         Counters* __restrict__ ctr, uint32_t bounce,
+                                                         FrameArgs fa, uint32_t npx, Queues qs,
+                                                         uint32_t bounce,
                                                          uint32_t* __restrict__ next_queue) {
+    __shared__ uint32_t pre[kMatBins * kSegs + 1];
     const uint32_t base_frame = *fa.frame;
     const uint32_t rr_after = fa.table[base_frame & fa.mask].rr_after;
     const uint32_t photons_on = fa.table[base_frame & fa.mask].enable_photons;
-    uint32_t start[kMatBins + 1];
-    start[0] = 0;
-#pragma unroll
-    for (int b = 0; b < kMatBins; ++b) start[b + 1] = start[b] + ctr->shade[bounce][b];
-    const uint32_t total = start[kMatBins];
+    const uint32_t total = seg_prefix(qs, bounce, kQShade0, kMatBins, pre);
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t wave_base = gtid & ~63u;
-    for (uint32_t base = wave_base; base < total; base += stride) {
+    for (uint32_t base = gtid & ~63u; base < total; base += stride) {
         const uint32_t i = base + __lane_id();
         bool alive = false;
         bool have = i < total;
@@ -186,10 +171,7 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
         float stmax = 0.0f;
         float4 contrib = splat4(0.0f);
         if (have) {
-            uint32_t b = 0;
-#pragma unroll
-            for (int k = 1; k < kMatBins; ++k) b += (i >= start[k]) ? 1u : 0u;
-            p = ps.shade_q[(uint64_t)b * ps.capacity + (i - start[b])];
+            p = ps.shade_q[seg_entry(pre, kMatBins * kSegs, i, qs.seg_cap)];
             const uint4 hr = ps.hit[p];
             const float t = __uint_as_float(hr.x);
             const uint32_t tri = hr.y & 0x7FFFFFFFu;
@@ -263,35 +245,36 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
                 ps.flags[p] = flags;
             }
         }
-        uint32_t ss = wave_append(&ctr->shadow[bounce], shadow);
+        const uint32_t se = seg_append(qs, bounce, kQShadow, qs.seg_cap, shadow);
         if (shadow) {
-            float4* r = ps.shadow + 3ull * ss;
+            float4* r = ps.shadow + 3ull * se;
             r[0] = f4(hp, stmax);
             r[1] = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(p));
             r[2] = contrib;
         }
-        uint32_t ns = wave_append(&ctr->active[bounce + 1], alive);
-        if (alive) next_queue[ns] = p;
+        const uint32_t ne = seg_append(qs, bounce + 1, kQActive, qs.seg_cap, alive);
+        if (alive) next_queue[ne] = p;
     }
 }
 
-__global__ __launch_bounds__(kTraceThreads) void k_shadow(SceneView sv, PathState ps,
+__global__ __launch_bounds__(kTraceThreads) void k_shadow(SceneView sv, PathState ps, Queues qs,
                                                           uint32_t count_traversal,
                                                           Counters* __restrict__ ctr, uint32_t bounce,
                                                           uint2* __restrict__ gstack) {
     __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
+    __shared__ uint32_t pre[kSegs + 1];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;
     st.lds = lds_stack + threadIdx.x;
     st.lds_stride = kTraceThreads;
     st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
-    const uint32_t n = ctr->shadow[bounce];
+    const uint32_t n = seg_prefix(qs, bounce, kQShadow, 1, pre);
     if (gtid == 0) atomicAdd(&ctr->shadow_rays, (unsigned long long)n);
     const uint32_t stride = gridDim.x * blockDim.x;
     const bool count = count_traversal != 0u;
     uint32_t my_rays = 0, my_nodes = 0, my_tris = 0;
     for (uint32_t i = gtid; i < n; i += stride) {
-        const float4* r = ps.shadow + 3ull * i;
+        const float4* r = ps.shadow + 3ull * seg_entry(pre, kSegs, i, qs.seg_cap);
         const float4 a = r[0], b = r[1];
         Hit h = trace_ray<true>(sv, xyz(a), xyz(b), 0.001f, a.w, st, count, my_nodes, my_tris);
         my_rays++;
@@ -344,8 +327,10 @@ __global__ __launch_bounds__(256) void k_accumulate(PathState ps, FrameArgs fa, 
 
 }  // namespace
 
-void launch_frame_prep(Counters* ctr, uint32_t n_paths, hipStream_t s) {
-    hipLaunchKernelGGL(k_frame_prep, dim3(1), dim3(256), 0, s, ctr, n_paths);
+void launch_clear_queues(const Queues& qs, uint32_t depths, hipStream_t s) {
+    const uint32_t words = (uint32_t)queue_count_words(depths);
+    hipLaunchKernelGGL(k_clear_queues, dim3((words + 255) / 256 < 1024 ? (words + 255) / 256 : 1024), dim3(256),
+                       0, s, qs.counts, words);
 }
 void launch_advance_frame(uint32_t* frame, uint32_t nf, hipStream_t s) {
     hipLaunchKernelGGL(k_advance_frame, dim3(1), dim3(64), 0, s, frame, nf);
@@ -357,16 +342,16 @@ void launch_generate(const PathLaunch& pl, uint32_t npx, uint32_t nf, hipStream_
     uint32_t n = npx * nf;
     hipLaunchKernelGGL(k_generate, dim3((n + 255) / 256), dim3(256), 0, s, pl.ps, pl.fa, npx, nf);
 }
-void launch_extend(const PathLaunch& pl, uint32_t bounce, const uint32_t* queue, hipStream_t s) {
-    hipLaunchKernelGGL(k_extend, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, pl.sv, pl.ps,
-                       pl.count_traversal, pl.ctr, bounce, queue, pl.gstack);
+void launch_extend(const PathLaunch& pl, uint32_t bounce, uint32_t n0, const uint32_t* queue, hipStream_t s) {
+    hipLaunchKernelGGL(k_extend, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, pl.sv, pl.ps, pl.qs,
+                       pl.count_traversal, pl.ctr, bounce, n0, queue, pl.gstack);
 }
 void launch_shade(const PathLaunch& pl, uint32_t npx, uint32_t bounce, uint32_t* next_queue, hipStream_t s) {
     hipLaunchKernelGGL(k_shade, dim3(kShadeBlocks), dim3(kShadeThreads), 0, s, pl.sv, pl.ps, pl.grid,
-                       pl.fa, npx, pl.ctr, bounce, next_queue);
+                       pl.fa, npx, pl.qs, bounce, next_queue);
 }
 void launch_shadow(const PathLaunch& pl, uint32_t bounce, hipStream_t s) {
-    hipLaunchKernelGGL(k_shadow, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, pl.sv, pl.ps,
+    hipLaunchKernelGGL(k_shadow, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, pl.sv, pl.ps, pl.qs,
                        pl.count_traversal, pl.ctr, bounce, pl.gstack);
 }
 void launch_accumulate(const PathLaunch& pl, uint32_t npx, uint32_t nf, hipStream_t s) {

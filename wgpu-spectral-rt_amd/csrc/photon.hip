@@ -9,6 +9,7 @@
 //   sort    stable radix sort of (cell, slot), so cells list slots ascending
 //   copy    photons in cell order, so a gather reads one contiguous range.
 #include "kernels.hpp"
+#include "queues.hpp"
 #include "shading.hpp"
 
 namespace wsrt {
@@ -17,14 +18,13 @@ namespace {
 
 // Emission of photon path i (photon_emit.wgsl:159-216): light choice, point
 // on the light, cosine direction, hero wavelength and flux. Every slot is
-// cleared; emitted paths enter the first bounce queue in slot order.
+// cleared; with lights present every path takes part in the first bounce,
+// which reads photon slots directly (no queue).
 __global__ __launch_bounds__(256) void k_photon_emit(SceneView sv, FrameArgs fa, uint32_t np,
                                                      uint32_t nf, PhotonState ps,
-                                                     wsrt_photon* __restrict__ photons,
-                                                     Counters* __restrict__ ctr) {
+                                                     wsrt_photon* __restrict__ photons) {
     const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = np * nf;
-    bool queued = false;
     if (id < n) {
         const uint32_t base_frame = *fa.frame;
         const FrameConst& f0 = fa.table[base_frame & fa.mask];
@@ -69,11 +69,8 @@ __global__ __launch_bounds__(256) void k_photon_emit(SceneView sv, FrameArgs fa,
             ps.dir[id] = f4(emit_dir, energy);
             ps.rng[id] = rng;
             ps.flags[id] = 0u;
-            queued = f0.photon_max_depth > 0u;
         }
     }
-    const uint32_t q = wave_append(&ctr->p_active[0], queued);
-    if (queued) ps.queue_a[q] = id;
 }
 
 // One bounce of every queued photon path (photon_emit.wgsl:219-330): closest
@@ -81,6 +78,7 @@ __global__ __launch_bounds__(256) void k_photon_emit(SceneView sv, FrameArgs fa,
 // through metal, glass and thin film. Survivors are compacted into the next
 // queue, so the few paths refracting through glass run in full waves.
 __global__ __launch_bounds__(kTraceThreads) void k_photon_bounce(SceneView sv, uint32_t depth,
+                                                                 uint32_t n0, Queues qs,
                                                                  uint32_t count_traversal,
                                                                  PhotonState ps,
                                                                  const uint32_t* __restrict__ queue,
@@ -89,12 +87,14 @@ __global__ __launch_bounds__(kTraceThreads) void k_photon_bounce(SceneView sv, u
                                                                  Counters* __restrict__ ctr,
                                                                  uint2* __restrict__ gstack) {
     __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
+    __shared__ uint32_t pre[kSegs + 1];
+    __shared__ unsigned long long red[kTraceThreads / 64];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;
     st.lds = lds_stack + threadIdx.x;
     st.lds_stride = kTraceThreads;
     st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
-    const uint32_t n = ctr->p_active[depth];
+    const uint32_t n = depth == 0 ? n0 : seg_prefix(qs, depth, kQPhoton, 1, pre);
     const uint32_t stride = gridDim.x * blockDim.x;
     const bool count = count_traversal != 0u;
     uint32_t my_rays = 0, my_stored = 0, my_nodes = 0, my_tris = 0;
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_photon_bounce(SceneView sv, u
         bool alive = false;
         uint32_t id = 0;
         if (i < n) {
-            id = queue[i];
+            id = depth == 0 ? i : queue[seg_entry(pre, kSegs, i, qs.pseg_cap)];
             const float4 pw = ps.pos[id], de = ps.dir[id];
             f3 pos = xyz(pw), dir = xyz(de);
             const float wl = pw.w, energy = de.w;
@@ -175,18 +175,16 @@ __global__ __launch_bounds__(kTraceThreads) void k_photon_bounce(SceneView sv, u
                 }
             }
         }
-        const uint32_t q = wave_append(&ctr->p_active[depth + 1], alive);
-        if (alive) next_queue[q] = id;
+        const uint32_t e = seg_append(qs, depth + 1, kQPhoton, qs.pseg_cap, alive);
+        if (alive) next_queue[e] = id;
     }
     if (count) flush_traversal_counts(ctr, 0, my_rays, my_nodes, my_tris);
-    for (int off = 32; off > 0; off >>= 1) {
-        my_rays += __shfl_xor(my_rays, off, 64);
-        my_stored += __shfl_xor(my_stored, off, 64);
-    }
-    if (__lane_id() == 0) {
-        atomicAdd(&ctr->photon_rays, (unsigned long long)my_rays);
-        atomicAdd(&ctr->photons_stored, (unsigned long long)my_stored);
-        atomicAdd(&ctr->stored, my_stored);
+    // per-block statistics, summed on the host
+    const unsigned long long br = block_sum(my_rays, red);
+    const unsigned long long bs = block_sum(my_stored, red);
+    if (threadIdx.x == 0) {
+        ctr->photon_rays_part[blockIdx.x] += br;
+        ctr->photons_stored_part[blockIdx.x] += bs;
     }
 }
 
@@ -252,13 +250,15 @@ __global__ __launch_bounds__(256) void k_grid_copy(const wsrt_photon* __restrict
 
 void launch_photon_trace(const SceneView& sv, const FrameArgs& fa, uint32_t np, uint32_t nf,
                          uint32_t max_depth, uint32_t count_traversal, const PhotonState& ps,
-                         wsrt_photon* photons, Counters* ctr, uint2* gstack, hipStream_t s) {
+                         const Queues& qs, wsrt_photon* photons, Counters* ctr, uint2* gstack,
+                         hipStream_t s) {
     const uint32_t n = np * nf;
-    hipLaunchKernelGGL(k_photon_emit, dim3((n + 255) / 256), dim3(256), 0, s, sv, fa, np, nf, ps, photons, ctr);
+    hipLaunchKernelGGL(k_photon_emit, dim3((n + 255) / 256), dim3(256), 0, s, sv, fa, np, nf, ps, photons);
+    const uint32_t n0 = sv.n_lights ? n : 0u;
     for (uint32_t d = 0; d < max_depth; ++d) {
         const uint32_t* q = (d & 1u) ? ps.queue_b : ps.queue_a;
         uint32_t* nq = (d & 1u) ? ps.queue_a : ps.queue_b;
-        hipLaunchKernelGGL(k_photon_bounce, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, sv, d,
+        hipLaunchKernelGGL(k_photon_bounce, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, sv, d, n0, qs,
                            count_traversal, ps, q, nq, photons, ctr, gstack);
     }
 }

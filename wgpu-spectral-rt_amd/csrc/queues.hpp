@@ -1,0 +1,129 @@
+// queues.hpp - work queues of the wavefront kernels without hot global
+// atomics.
+//
+// A device-scope atomic on one address from every wave of a 2M-item launch
+// serializes in the memory system (~10 ns each on MI355X, where the eight
+// XCDs do not share an L2), which made queue appends the dominant cost. A
+// queue is therefore split into kSegs segments: producer block b appends to
+// segment b % kSegs (one wave-aggregated atomic on a counter that sits on
+// its own 128-byte line), and a consumer turns its item index into (segment,
+// offset) through a prefix sum of the segment counts held in LDS. Producers
+// are grid-stride kernels of exactly kGridLanes lanes, so segment s receives
+// at most kGridLanes / kSegs items per pass over the input and a segment of
+// seg_cap = (kGridLanes / kSegs) * ceil(capacity / kGridLanes) entries never
+// overflows. Items enter a queue in an order that depends on scheduling;
+// every consumer treats items independently, so results do not depend on it.
+#pragma once
+
+#include "common.hpp"
+#include "dev_math.hpp"
+
+namespace wsrt {
+
+constexpr int kSegs = 64;
+constexpr int kSegStride = 32;             // uint32 counters 128 bytes apart
+constexpr uint32_t kGridLanes = 262144;     // lanes of every queue-producing launch
+
+enum QueueKind : uint32_t {
+    kQActive = 0,                      // camera paths entering a bounce
+    kQShade0 = 1,                      // hits of a bounce, one queue per material type
+    kQShadow = kQShade0 + kMatBins,    // shadow rays of a bounce
+    kQPhoton = kQShadow + 1,           // photon paths entering a bounce
+    kQKinds
+};
+
+struct Queues {
+    uint32_t* counts;  // [depth][kind][segment] counters, kSegStride apart
+    uint32_t seg_cap;  // camera queues: entries per segment
+    uint32_t pseg_cap; // photon queues: entries per segment
+};
+
+__host__ __device__ inline uint32_t seg_capacity(uint64_t capacity) {
+    return (uint32_t)((kGridLanes / kSegs) * ((capacity + kGridLanes - 1) / kGridLanes));
+}
+__host__ __device__ inline size_t queue_count_words(uint32_t depths) {
+    return (size_t)depths * kQKinds * kSegs * kSegStride;
+}
+
+__device__ __forceinline__ uint32_t* qcount(const Queues& q, uint32_t depth, uint32_t kind,
+                                            uint32_t seg) {
+    return q.counts + (((size_t)depth * kQKinds + kind) * kSegs + seg) * kSegStride;
+}
+
+// Appends from every lane with pred set; returns the lane's entry index in
+// the queue's array (segment * cap + slot) or kInvalid. All lanes of the
+// wave must call it.
+__device__ __forceinline__ uint32_t seg_append(const Queues& q, uint32_t depth, uint32_t kind,
+                                               uint32_t cap, bool pred) {
+    const uint32_t seg = blockIdx.x % kSegs;
+    const uint32_t slot = wave_append(qcount(q, depth, kind, seg), pred);
+    return pred ? seg * cap + slot : kInvalid;
+}
+
+// Exclusive prefix of the segment counts of kinds [k0, k0 + nk) into pre
+// (LDS, nk * kSegs + 1 entries, nk <= 8); returns the total. Block-wide:
+// all threads must call it.
+__device__ __forceinline__ uint32_t seg_prefix(const Queues& q, uint32_t depth, uint32_t k0,
+                                               uint32_t nk, uint32_t* pre) {
+    const uint32_t ne = nk * kSegs;
+    if (threadIdx.x < 64) {
+        const uint32_t lane = threadIdx.x, per = ne / 64;
+        uint32_t v[8];
+        uint32_t s = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) {
+            v[k] = 0;
+            if (k < per) {
+                const uint32_t e = lane * per + k;
+                v[k] = *qcount(q, depth, k0 + e / kSegs, e % kSegs);
+                s += v[k];
+            }
+        }
+        uint32_t inc = s;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t t = __shfl_up(inc, off, 64);
+            if ((int)lane >= off) inc += t;
+        }
+        uint32_t ex = inc - s;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k)
+            if (k < per) {
+                pre[lane * per + k] = ex;
+                ex += v[k];
+            }
+        if (lane == 63) pre[ne] = inc;
+    }
+    __syncthreads();
+    return pre[ne];
+}
+
+// Entry of item i (< total): segment j with pre[j] <= i < pre[j + 1].
+__device__ __forceinline__ uint32_t seg_entry(const uint32_t* pre, uint32_t ne, uint32_t i,
+                                              uint32_t cap, uint32_t* seg_out = nullptr) {
+    uint32_t lo = 0, hi = ne;  // invariant: pre[lo] <= i < pre[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    if (seg_out) *seg_out = lo;
+    return lo * cap + (i - pre[lo]);
+}
+
+// Block sum of one value per thread (blockDim a multiple of 64, <= 1024),
+// valid in thread 0.
+__device__ __forceinline__ unsigned long long block_sum(unsigned long long v, unsigned long long* lds) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const uint32_t w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[w] = v;
+    __syncthreads();
+    unsigned long long t = 0;
+    if (threadIdx.x == 0)
+        for (uint32_t k = 0; k < nw; ++k) t += lds[k];
+    __syncthreads();
+    return t;
+}
+
+}  // namespace wsrt

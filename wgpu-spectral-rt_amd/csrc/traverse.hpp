@@ -84,6 +84,7 @@ __device__ Hit traverse(const SceneView& sv, f3 o, f3 d, float tmin, float tmax,
     if (sv.n_nodes == 0) return best;
     const float inv[3] = {safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
     const float oo[3] = {o.x, o.y, o.z};
+    const bool neg[3] = {inv[0] < 0.0f, inv[1] < 0.0f, inv[2] < 0.0f};
     const int oct = (d.x < 0.0f ? 1 : 0) | (d.y < 0.0f ? 2 : 0) | (d.z < 0.0f ? 4 : 0);
     uint32_t nvis = 0, ntest = 0;
     int sp = 0;
@@ -95,42 +96,54 @@ __device__ Hit traverse(const SceneView& sv, f3 o, f3 d, float tmin, float tmax,
         const uint4* np = nodes + 5ull * e.x;
         uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
         if (kCount) nvis++;
-        float as[3], ao[3], sl[3];
+        // per axis: slab scale/offset, and the byte plane of each child's near
+        // and far bound (qlo or qhi by the sign of the direction)
+        float as[3], ao[3], sl = 0.0f;
         const float org[3] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)};
+        // quantized bytes: qlo x/y/z = (w2.xy, w2.zw, w3.xy), qhi x/y/z = (w3.zw, w4.xy, w4.zw)
+        const uint32_t lo_l[3] = {w2.x, w2.z, w3.x}, lo_h[3] = {w2.y, w2.w, w3.y};
+        const uint32_t hi_l[3] = {w3.z, w4.x, w4.z}, hi_h[3] = {w3.w, w4.y, w4.w};
+        uint32_t nl[3], nh[3], fl[3], fh[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             int ex = (int)(int8_t)((w0.w >> (8 * a)) & 0xFFu);
-            float s = __uint_as_float((uint32_t)(ex + 127) << 23);
-            as[a] = s * inv[a];
+            float scale = __uint_as_float((uint32_t)(ex + 127) << 23);
+            as[a] = scale * inv[a];
             ao[a] = (org[a] - oo[a]) * inv[a];
-            sl[a] = (fabsf(as[a]) * 255.0f + fabsf(ao[a])) * 4.0e-7f;
+            sl = fmaxf(sl, fabsf(as[a]) * 255.0f + fabsf(ao[a]));
+            nl[a] = neg[a] ? hi_l[a] : lo_l[a];
+            nh[a] = neg[a] ? hi_h[a] : lo_h[a];
+            fl[a] = neg[a] ? lo_l[a] : hi_l[a];
+            fh[a] = neg[a] ? lo_h[a] : hi_h[a];
         }
+        // bound of the f32 error of the slab values, applied to both ends
+        sl = sl * 4.0e-7f;
+        const float sl2 = sl * (1.0f + kBoxPad);
         const uint32_t child_base = w1.x, tri_base = w1.y;
-        // quantized bytes: qlo x/y/z = (w2.xy, w2.zw, w3.xy), qhi x/y/z = (w3.zw, w4.xy, w4.zw)
-        const uint32_t qlo_lo[3] = {w2.x, w2.z, w3.x}, qlo_hi[3] = {w2.y, w2.w, w3.y};
-        const uint32_t qhi_lo[3] = {w3.z, w4.x, w4.z}, qhi_hi[3] = {w3.w, w4.y, w4.w};
         float cnear[8];
         uint32_t hitmask = 0;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            uint32_t meta = byte_of(w1.z, w1.w, c);
             float tn = tmin, tf = best.t;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                float t0 = __fmaf_rn((float)byte_of(qlo_lo[a], qlo_hi[a], c), as[a], ao[a]);
-                float t1 = __fmaf_rn((float)byte_of(qhi_lo[a], qhi_hi[a], c), as[a], ao[a]);
-                tn = fmaxf(tn, fminf(t0, t1) - sl[a]);
-                tf = fminf(tf, fmaxf(t0, t1) + sl[a]);
+                tn = fmaxf(tn, __fmaf_rn((float)byte_of(nl[a], nh[a], c), as[a], ao[a]));
+                tf = fminf(tf, __fmaf_rn((float)byte_of(fl[a], fh[a], c), as[a], ao[a]));
             }
-            cnear[c] = tn;
-            if (meta != 0u && tn <= tf * kBoxPad) hitmask |= 1u << c;
+            cnear[c] = tn - sl;
+            if (byte_of(w1.z, w1.w, c) != 0u && tn <= __fmaf_rn(tf, kBoxPad, sl2)) hitmask |= 1u << c;
         }
-        // leaf children, in slot order
+        // slot classes: internal children have meta bit 7 set
+        uint32_t imask = 0;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            uint32_t meta = byte_of(w1.z, w1.w, c);
-            if (!((hitmask >> c) & 1u) || (meta & 0x80u)) continue;
-            uint32_t cnt = meta >> 5, off = meta & 0x1Fu;
+        for (int c = 0; c < 8; ++c) imask |= ((byte_of(w1.z, w1.w, c) >> 7) & 1u) << c;
+        // leaf children, in slot order
+        uint32_t leaves = hitmask & ~imask;
+        while (leaves) {
+            const int c = __ffs(leaves) - 1;
+            leaves &= leaves - 1;
+            const uint32_t meta = ((c < 4 ? w1.z : w1.w) >> (8 * (c & 3))) & 0xFFu;
+            const uint32_t cnt = meta >> 5, off = meta & 0x1Fu;
             for (uint32_t k = 0; k < cnt; ++k) {
                 const float4* tr = sv.tris + 3ull * (tri_base + off + k);
                 float4 a = tr[0], b = tr[1], cc = tr[2];
@@ -155,15 +168,24 @@ __device__ Hit traverse(const SceneView& sv, f3 o, f3 d, float tmin, float tmax,
                 }
             }
         }
-        // internal children in octant order: slot (k ^ oct) for k = 7 .. 0, so
-        // the child on the ray's near side is on top of the stack
+        // internal children in the order of the ray's octant over the slots:
+        // slot c is pushed at rank (c ^ oct) from the top, so slot oct is
+        // popped first; positions come from a mask permuted by oct
+        uint32_t push = 0, pm = 0;
 #pragma unroll
-        for (int k = 7; k >= 0; --k) {
-            const int c = k ^ oct;
-            uint32_t meta = byte_of(w1.z, w1.w, c);
-            if (((hitmask >> c) & 1u) && (meta & 0x80u) && !(cnear[c] > best.t * kBoxPad))
-                st.put(sp++, make_uint2(child_base + (meta & 0x7Fu), __float_as_uint(cnear[c])));
+        for (int c = 0; c < 8; ++c) {
+            const bool pc = ((hitmask & imask) >> c) & 1u && !(cnear[c] > best.t * kBoxPad);
+            push |= (pc ? 1u : 0u) << c;
+            pm |= (pc ? 1u : 0u) << (c ^ oct);
         }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if (!((push >> c) & 1u)) continue;
+            const int pos = sp + __popc(pm >> ((c ^ oct) + 1));
+            const uint32_t meta = byte_of(w1.z, w1.w, c);
+            st.put(pos, make_uint2(child_base + (meta & 0x7Fu), __float_as_uint(cnear[c])));
+        }
+        sp += __popc(pm);
     }
     if (kCount) {
         *n_nodes_visited = nvis;
