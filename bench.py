@@ -58,6 +58,91 @@ def cpu_baseline(seconds: float = 12.0) -> dict:
             "sample": f"{frames} full frames (512x512, 2^20 photons each) in {dt:.1f} s"}
 
 
+KERNEL_SYMBOLS = {"extend": "k_extend", "shadow": "k_shadow", "photon_trace": "k_photon_bounce"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_{}_caustic512.txt")
+
+
+def _short(name: str) -> str:
+    for p in ("wsrt::(anonymous namespace)::", "wsrt::"):
+        name = name.replace(p, "")
+    return name.split("(")[0]
+
+
+def _per_dispatch(csv_path: str, counter: str) -> dict:
+    """Mean of a counter per dispatch, by kernel, from a rocprofv3
+    --pmc counter_collection.csv."""
+    import csv
+    import collections
+
+    tot, disp = collections.defaultdict(float), collections.defaultdict(set)
+    with open(csv_path) as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] != counter:
+                continue
+            k = _short(row["Kernel_Name"])
+            tot[k] += float(row["Counter_Value"])
+            disp[k].add(row["Dispatch_Id"])
+    return {k: tot[k] / len(disp[k]) for k in tot}
+
+
+def summarize_traffic(pmc_dir: str, kernel: str, command: str) -> int:
+    """Writes the HBM traffic of one kernel per launch from four rocprofv3
+    --pmc runs in pmc_dir: calib_fetch / calib_write (FETCH_SIZE, WRITE_SIZE
+    of the plain copy of --copy-calibration 256) and fetch / write (the same
+    counters over the bench run). Each counter is scaled so that the copy
+    reads its known size."""
+    copy_kib = 256 * 1024
+    cf = _per_dispatch(os.path.join(pmc_dir, "calib_fetch_counter_collection.csv"), "FETCH_SIZE")
+    cw = _per_dispatch(os.path.join(pmc_dir, "calib_write_counter_collection.csv"), "WRITE_SIZE")
+    copy_name = max(cf, key=lambda k: cf[k])
+    f_scale, w_scale = copy_kib / cf[copy_name], copy_kib / cw[copy_name]
+    sym = KERNEL_SYMBOLS[kernel]
+    fk = _per_dispatch(os.path.join(pmc_dir, "fetch_counter_collection.csv"), "FETCH_SIZE")[sym]
+    wk = _per_dispatch(os.path.join(pmc_dir, "write_counter_collection.csv"), "WRITE_SIZE")[sym]
+    fb, wb = fk * f_scale * 1024, wk * w_scale * 1024
+    with open(TRAFFIC_FILE.format(kernel), "w") as f:
+        f.write(f"# HBM traffic per launch of {sym} (caustic photon mapping, 512x512)\n")
+        f.write(f"# command: {command}\n")
+        f.write(f"# calibration: {copy_name} of {copy_kib} KiB reads FETCH_SIZE {cf[copy_name]:.1f} KiB, "
+                f"WRITE_SIZE {cw[copy_name]:.1f} KiB per dispatch\n")
+        f.write(f"fetch_scale = {f_scale:.5f}\nwrite_scale = {w_scale:.5f}\n")
+        f.write(f"fetch_kib_per_dispatch_raw = {fk:.1f}\nwrite_kib_per_dispatch_raw = {wk:.1f}\n")
+        f.write(f"fetch_bytes_per_launch = {fb:.0f}\nwrite_bytes_per_launch = {wb:.0f}\n")
+        f.write(f"traffic_bytes_per_launch = {fb + wb:.0f}\n")
+    print(open(TRAFFIC_FILE.format(kernel)).read())
+    return 0
+
+
+def hbm_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from its committed counter summary,
+    or None when there is none."""
+    path = TRAFFIC_FILE.format(kernel)
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if "=" in line and not line.startswith("#"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = float(v)
+    return vals.get("traffic_bytes_per_launch")
+
+
+def copy_calibration(mib: int) -> int:
+    """A plain device-to-device copy of `mib` MiB, launched 4 times: the
+    hardware counters of one such dispatch read `mib` MiB fetched and `mib`
+    MiB written when they count bytes one to one."""
+    import torch
+
+    n = mib << 20
+    a = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(4):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n, "launches": 4}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,7 +150,17 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (the reference point that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiled run) and exit")
+    ap.add_argument("--summarize-traffic", nargs=3, metavar=("PMC_DIR", "KERNEL", "COMMAND"),
+                    help="write the per-launch HBM traffic summary of KERNEL from rocprofv3 --pmc "
+                         "output and exit")
     args = ap.parse_args()
+    if args.summarize_traffic:
+        return summarize_traffic(*args.summarize_traffic)
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the render loop runs "
               "on a single MI355X; multi-GPU sharding is a design note only)", file=sys.stderr)
@@ -83,22 +178,26 @@ def main() -> int:
     r.set_camera(scenes.camera(v, 0))
     r.render_frames(args.warmup)
     r.sync()
-    # timed region: exactly `steps` frames submitted as captured batch graphs
+    # timed region: exactly `steps` frames, every kernel (group) bracketed by
+    # HIP events recorded on the renderer's stream, which is the stream the
+    # kernels run on (profiling mode launches the batches eagerly)
     r.reset_stats()
+    r.reset_profile()
+    r.set_profiling(True)
     r.sync()
     t0 = time.perf_counter()
     r.render_frames(args.steps)
     r.sync()
     dt = time.perf_counter() - t0
+    r.set_profiling(False)
     st = r.stats()
-    # the same frames again with HIP events around every kernel on the
-    # renderer's stream (launched one by one), for per-kernel device times
-    r.reset_profile()
-    r.set_profiling(True)
+    prof = r.profile()
+    # the same number of frames replayed as captured batch graphs (no events)
+    r.sync()
+    t1 = time.perf_counter()
     r.render_frames(args.steps)
     r.sync()
-    r.set_profiling(False)
-    prof = r.profile()
+    dt_graph = time.perf_counter() - t1
     # traversal work per ray from the in-kernel counters (pinned against the
     # oracle's traversal of the same BVH8), measured on one extra frame
     r.reset_stats()
@@ -156,12 +255,15 @@ def main() -> int:
             "width": v.width, "height": v.height, "photons_per_pass": cfg.photons_per_pass,
             "hash_size": cfg.hash_size, "max_depth": cfg.max_depth,
         },
+        "graph_replay_ms_per_step": round(1000 * dt_graph / args.steps, 4),
         "rays_per_s": round(rays / dt, 1),
         "photons_per_s": round(st["photon_paths"] / dt, 1),
         "photons_stored_per_frame": round(st["photons_stored"] / args.steps, 1),
         "roofline": {
             "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": hbm_traffic(dom),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "algorithmic_bytes_per_launch": round(d["bytes_per_ray"] * ray_counts[dom] / d["launches"]),
             "algorithmic_bytes_per_ray": d["bytes_per_ray"],
             "avg_launch_us": d["avg_us"],
         },
