@@ -349,6 +349,38 @@ int wsrt_debug_read_bvh(const wsrt_scene* s, void* nodes, void* tris);
  * (instance, material, mesh-local primitive) triples. */
 int wsrt_debug_read_world(const wsrt_scene* s, float* tris, uint32_t* ids);
 
+/* ---- learned path guiding (no reference counterpart: the reference has no
+ *      guiding; this is the online-trained proposal of wsrt_config.guiding) --
+ * A network of 7 input scalars in [0,1] (position over the scene box, normal
+ * mapped from [-1,1], hero wavelength over 380..780 nm), 12-band sin/cos
+ * encoding, 4 hidden layers of 64 ReLU units and 40 outputs: 8 von
+ * Mises-Fisher lobes of (mu xyz, log kappa, logit). Parameters are
+ * WSRT_GUIDE_PARAMS floats: W1[64][192] b1[64] W2[64][64] b2 W3 b3 W4 b4
+ * W5[48][64] b5[48], row-major [out][in]; W1 columns >= 168 and output rows
+ * >= 40 are padding and stay zero. Forward, backward and Adam run as BF16
+ * MFMA kernels with FP32 accumulation. */
+#define WSRT_GUIDE_PARAMS 27952
+#define WSRT_GUIDE_OUTPUTS 40
+typedef struct wsrt_guide wsrt_guide;
+/* A standalone network for up to max_samples samples per call, initialised
+ * from seed. */
+int wsrt_guide_create(wsrt_device* dev, uint32_t max_samples, uint32_t seed, wsrt_guide** out);
+int wsrt_guide_destroy(wsrt_guide* g);
+int wsrt_guide_get_params(wsrt_guide* g, float* out);
+/* Sets the parameters (padding forced to zero) and resets the Adam state. */
+int wsrt_guide_set_params(wsrt_guide* g, const float* in);
+/* Outputs (n x 40) for input scalars (n x 7). */
+int wsrt_guide_forward(wsrt_guide* g, const float* scalars, uint32_t n, float* out);
+/* Gradient of (1/n) sum_i -weights[i] log p_guide(dirs[i] | scalars[i]) with
+ * respect to the parameters (WSRT_GUIDE_PARAMS floats). */
+int wsrt_guide_gradients(wsrt_guide* g, const float* scalars, const float* dirs, const float* weights,
+                         uint32_t n, float* grads);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8, learning rate lr) on that loss. */
+int wsrt_guide_train_step(wsrt_guide* g, const float* scalars, const float* dirs, const float* weights,
+                          uint32_t n, float lr);
+/* The network a guiding renderer trains (owned by the renderer). */
+int wsrt_renderer_get_guide(wsrt_renderer* r, wsrt_guide** out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -722,7 +722,14 @@ static int validate_config(const wsrt_config* c) {
     if ((uint64_t)c->photons_per_pass * c->frames_in_flight > (1ull << 28) ||
         (uint64_t)c->hash_size * c->frames_in_flight > (1ull << 30))
         return fail(WSRT_E_INVALID, "photons or cells times frames_in_flight too large");
-    if (c->guiding) return fail(WSRT_E_UNSUPPORTED, "guiding is not available in this build");
+    if (c->guiding > 1) return fail(WSRT_E_INVALID, "guiding %u not 0 or 1", c->guiding);
+    if (c->guiding) {
+        if (!(c->guide_mix >= 0.0f && c->guide_mix <= 1.0f))
+            return fail(WSRT_E_INVALID, "guide_mix %g not in [0, 1]", (double)c->guide_mix);
+        if (!(c->guide_lr > 0.0f)) return fail(WSRT_E_INVALID, "guide_lr must be > 0");
+        if (c->guide_train_samples < 64 || c->guide_train_samples > (1u << 22))
+            return fail(WSRT_E_INVALID, "guide_train_samples %u not in 64..2^22", c->guide_train_samples);
+    }
     return WSRT_OK;
 }
 
@@ -774,15 +781,130 @@ static FrameConst make_frame_const(const wsrt_config& c, const wsrt_camera_unifo
 namespace {
 enum KernelId {
     K_FRAME_PREP, K_PHOTON_TRACE, K_GRID_BUILD, K_GENERATE, K_EXTEND, K_SHADE, K_SHADOW,
-    K_ACCUMULATE, K_COUNT
+    K_ACCUMULATE, K_GUIDE_SCATTER, K_GUIDE_TRAIN, K_COUNT
 };
 const char* kKernelNames[K_COUNT] = {"frame_prep", "photon_trace", "grid_build", "generate", "extend",
-                                     "shade", "shadow", "accumulate"};
+                                     "shade", "shadow", "accumulate", "guide_scatter", "guide_train"};
 constexpr uint32_t kUploadHalf = kFrameTable / 2;  // frames per upload (staging half)
 struct Pending {
     int kid;
     hipEvent_t a, b;
 };
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// path-guiding network (guide.hip): parameters, Adam state, packed operands
+// and the buffers of one training step
+// ---------------------------------------------------------------------------
+struct wsrt_guide {
+    wsrt_device* dev = nullptr;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool owned_by_renderer = false;
+    DevBuf wf, wt, params, m1, m2, step, act, partial, tin0, tin1, tdir, out;
+    uint32_t ns_max = 0;  // training samples per step the buffers hold
+    GuideView gv{};
+    GuideTrain gt{};
+};
+
+namespace {
+
+constexpr uint32_t kGuideChunk = 2048;
+
+// Training-step size for n samples: a multiple of 64, and of the partial-sum
+// chunk (which divides it).
+void guide_step_shape(uint32_t n, uint32_t* ns, uint32_t* chunk) {
+    if (n <= kGuideChunk) {
+        *ns = (n + 63u) & ~63u;
+        *chunk = *ns;
+    } else {
+        *ns = (n + kGuideChunk - 1) / kGuideChunk * kGuideChunk;
+        *chunk = kGuideChunk;
+    }
+}
+
+// Initial parameters: uniform Xavier weights from a PCG stream; output
+// biases spread the 8 lobes over the cube diagonals with kappa = 2 and equal
+// weights.
+std::vector<float> guide_initial_params(uint32_t seed) {
+    using namespace guide;
+    std::vector<float> w(kParams, 0.0f);
+    uint32_t st = seed * 747796405u + 2891336453u;
+    auto next = [&st]() {
+        st = st * 747796405u + 2891336453u;
+        uint32_t x = ((st >> ((st >> 28u) + 4u)) ^ st) * 277803737u;
+        x = (x >> 22u) ^ x;
+        return (float)x * 2.3283064365386963e-10f;
+    };
+    auto fill = [&](int off, int rows, int cols, int used_rows, int used_cols, float scale) {
+        const float a = scale * sqrtf(6.0f / (float)(used_rows + used_cols));
+        for (int r = 0; r < rows; ++r)
+            for (int c = 0; c < cols; ++c)
+                w[off + r * cols + c] = (r < used_rows && c < used_cols) ? a * (2.0f * next() - 1.0f) : 0.0f;
+    };
+    fill(kW1, kHid, kIn, kHid, kFeat, 1.0f);
+    fill(kW2, kHid, kHid, kHid, kHid, 1.0f);
+    fill(kW3, kHid, kHid, kHid, kHid, 1.0f);
+    fill(kW4, kHid, kHid, kHid, kHid, 1.0f);
+    fill(kW5, kOutPad, kHid, kOut, kHid, 0.1f);
+    const float d = 0.57735027f;
+    for (int k = 0; k < kLobes; ++k) {
+        w[kB5 + 5 * k + 0] = (k & 1) ? -d : d;
+        w[kB5 + 5 * k + 1] = (k & 2) ? -d : d;
+        w[kB5 + 5 * k + 2] = (k & 4) ? -d : d;
+        w[kB5 + 5 * k + 3] = 0.6931472f;  // log 2
+        w[kB5 + 5 * k + 4] = 0.0f;
+    }
+    return w;
+}
+
+int guide_setup(wsrt_guide* g, uint32_t max_samples, uint32_t seed, float lr, float mix) {
+    using namespace guide;
+    uint32_t ns, chunk;
+    guide_step_shape(max_samples, &ns, &chunk);
+    g->ns_max = ns;
+    hipStream_t s = g->stream;
+    HIP_OR_FAIL(g->wf.alloc(2ull * kFwdPack));
+    HIP_OR_FAIL(g->wt.alloc(2ull * kBwdPack));
+    HIP_OR_FAIL(g->params.alloc(4ull * kParams));
+    HIP_OR_FAIL(g->m1.alloc(4ull * kParams));
+    HIP_OR_FAIL(g->m2.alloc(4ull * kParams));
+    HIP_OR_FAIL(g->step.alloc(4));
+    HIP_OR_FAIL(g->act.alloc(4ull * guide_act_words(ns)));
+    HIP_OR_FAIL(g->partial.alloc(4ull * (ns / chunk) * kParams));
+    HIP_OR_FAIL(g->tin0.alloc(16ull * ns));
+    HIP_OR_FAIL(g->tin1.alloc(16ull * ns));
+    HIP_OR_FAIL(g->tdir.alloc(16ull * ns));
+    HIP_OR_FAIL(hipMemsetAsync(g->wf.p, 0, g->wf.bytes, s));
+    HIP_OR_FAIL(hipMemsetAsync(g->wt.p, 0, g->wt.bytes, s));
+    HIP_OR_FAIL(hipMemsetAsync(g->m1.p, 0, g->m1.bytes, s));
+    HIP_OR_FAIL(hipMemsetAsync(g->m2.p, 0, g->m2.bytes, s));
+    HIP_OR_FAIL(hipMemsetAsync(g->step.p, 0, 4, s));
+    const std::vector<float> w = guide_initial_params(seed);
+    HIP_OR_FAIL(hipMemcpyAsync(g->params.p, w.data(), 4ull * kParams, hipMemcpyHostToDevice, s));
+    g->gv.wf = g->wf.as<uint16_t>();
+    g->gv.wt = g->wt.as<uint16_t>();
+    g->gv.params = g->params.as<float>();
+    g->gv.mix = mix;
+    for (int a = 0; a < 3; ++a) {
+        g->gv.bmin[a] = 0.0f;
+        g->gv.binv[a] = 1.0f;
+    }
+    g->gv.rec = guide::Records{nullptr, nullptr, nullptr, nullptr, nullptr};
+    g->gt.act = g->act.as<uint32_t>();
+    g->gt.partial = g->partial.as<float>();
+    g->gt.params = g->params.as<float>();
+    g->gt.m1 = g->m1.as<float>();
+    g->gt.m2 = g->m2.as<float>();
+    g->gt.step = g->step.as<uint32_t>();
+    g->gt.chunk = chunk;
+    g->gt.lr = lr;
+    launch_guide_pack(g->gv, g->params.as<float>(), s);
+    HIP_OR_FAIL(hipGetLastError());
+    HIP_OR_FAIL(hipStreamSynchronize(s));
+    return WSRT_OK;
+}
+
 }  // namespace
 
 struct wsrt_renderer {
@@ -820,6 +942,10 @@ struct wsrt_renderer {
     double k_ms[K_COUNT] = {0};
     uint64_t k_launches[K_COUNT] = {0};
     uint64_t frames = 0, samples = 0, photon_paths = 0;
+    // path guiding (cfg.guiding)
+    std::unique_ptr<wsrt_guide> guide;
+    DevBuf g_in0, g_in1, g_dir, g_tp, g_Lb, g_n, g_alb, g_q;
+    uint32_t guide_ns = 0, guide_stride = 1;
 };
 
 namespace {
@@ -920,7 +1046,9 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     HIP_OR_FAIL(r->path_rng.alloc(4 * npaths));
     HIP_OR_FAIL(r->path_flags.alloc(4 * npaths));
     HIP_OR_FAIL(r->path_hit.alloc(16 * npaths));
-    const uint32_t seg = seg_capacity(npaths);
+    // with guiding, k_shade and k_guide_scatter both fill a bounce's active
+    // queue, so each segment may receive two producers' worth of entries
+    const uint32_t seg = seg_capacity(npaths) * (c.guiding ? 2u : 1u);
     HIP_OR_FAIL(r->q_a.alloc(4ull * kSegs * seg));
     HIP_OR_FAIL(r->q_b.alloc(4ull * kSegs * seg));
     HIP_OR_FAIL(r->shade_q.alloc(4ull * kSegs * seg * kMatBins));
@@ -982,7 +1110,42 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     pl.fa.mask = kFrameTable - 1;
     pl.count_traversal = 0;
     pl.qs.counts = r->qcounts.as<uint32_t>();
-    pl.qs.seg_cap = seg_capacity(npaths);
+    pl.qs.seg_cap = seg;
+    pl.guided = c.guiding;
+    if (c.guiding) {
+        HIP_OR_FAIL(r->g_in0.alloc(16 * npaths));
+        HIP_OR_FAIL(r->g_in1.alloc(16 * npaths));
+        HIP_OR_FAIL(r->g_dir.alloc(16 * npaths));
+        HIP_OR_FAIL(r->g_tp.alloc(16 * npaths));
+        HIP_OR_FAIL(r->g_Lb.alloc(16 * npaths));
+        HIP_OR_FAIL(r->g_n.alloc(16 * npaths));
+        HIP_OR_FAIL(r->g_alb.alloc(16 * npaths));
+        HIP_OR_FAIL(r->g_q.alloc(4ull * kSegs * seg));
+        r->guide = std::make_unique<wsrt_guide>();
+        wsrt_guide* g = r->guide.get();
+        g->dev = dev;
+        g->stream = r->stream;
+        g->owned_by_renderer = true;
+        const uint32_t want = (uint32_t)std::min<uint64_t>(c.guide_train_samples, npx);
+        // whole chunks of samples per step (at least one step of 64)
+        uint32_t ns = want >= kGuideChunk ? want / kGuideChunk * kGuideChunk : (want + 63u) & ~63u;
+        int grc = guide_setup(g, ns, 0x5eedu, c.guide_lr, c.guide_mix);
+        if (grc) return grc;
+        r->guide_ns = ns;
+        r->guide_stride = (uint32_t)std::max<uint64_t>(1, npx / ns);
+        const wsrt_scene_info& si = scene->info;
+        for (int a = 0; a < 3; ++a) {
+            const float lo = si.bounds[a], ext = si.bounds[3 + a] - si.bounds[a];
+            g->gv.bmin[a] = lo;
+            g->gv.binv[a] = ext > 0.0f ? 1.0f / ext : 1.0f;
+        }
+        g->gv.rec = guide::Records{r->g_in0.as<float4>(), r->g_in1.as<float4>(), r->g_dir.as<float4>(),
+                                   r->g_tp.as<float4>(), r->g_Lb.as<float4>()};
+        pl.gv = g->gv;
+    }
+    pl.ps.guide_q = r->g_q.as<uint32_t>();
+    pl.ps.gn = r->g_n.as<float4>();
+    pl.ps.galb = r->g_alb.as<float4>();
     pl.qs.pseg_cap = seg_capacity(nph);
     // default camera of CameraController::new (camera.rs:144-162)
     const float pos[3] = {0.0f, 0.0f, 3.0f};
@@ -1063,12 +1226,29 @@ static int enqueue_batch(wsrt_renderer* r, uint32_t nf) {
             ProfScope p(r, K_SHADE);
             launch_shade(pl, npx, b, nq, s);
         }
-        ProfScope p(r, K_SHADOW);
-        launch_shadow(pl, b, s);
+        {
+            ProfScope p(r, K_SHADOW);
+            launch_shadow(pl, b, s);
+        }
+        if (pl.guided) {
+            ProfScope p(r, K_GUIDE_SCATTER);
+            launch_guide_scatter(pl.ps, pl.qs, pl.gv, pl.fa, b, nq, s);
+        }
     }
     {
         ProfScope p(r, K_ACCUMULATE);
         launch_accumulate(pl, npx, nf, s);
+    }
+    if (pl.guided) {
+        // one Adam step per frame of the batch, on that frame's records
+        ProfScope p(r, K_GUIDE_TRAIN);
+        wsrt_guide* g = r->guide.get();
+        for (uint32_t slot = 0; slot < nf; ++slot) {
+            launch_guide_prepare(pl.ps, pl.gv, pl.fa, npx, slot, r->guide_ns, r->guide_stride,
+                                 g->tin0.as<float4>(), g->tin1.as<float4>(), g->tdir.as<float4>(), s);
+            launch_guide_train_step(g->gv, g->gt, g->tin0.as<float4>(), g->tin1.as<float4>(),
+                                    g->tdir.as<float4>(), r->guide_ns, 1.0f / (float)r->guide_ns, true, s);
+        }
     }
     launch_advance_frame(r->dframe.as<uint32_t>(), nf, s);
     hipError_t e = hipGetLastError();
@@ -1471,5 +1651,158 @@ extern "C" int wsrt_debug_read_world(const wsrt_scene* s, float* tris, uint32_t*
             ids[3 * t + 2] = (uint32_t)t - inst[ti[t]].first_tri;
         }
     }
+    return WSRT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// path-guiding network
+// ---------------------------------------------------------------------------
+extern "C" int wsrt_guide_create(wsrt_device* dev, uint32_t max_samples, uint32_t seed, wsrt_guide** out) {
+    CHECK_ARG(dev && out && max_samples >= 1 && max_samples <= (1u << 22));
+    HIP_OR_FAIL(hipSetDevice(dev->ordinal));
+    auto g = std::make_unique<wsrt_guide>();
+    g->dev = dev;
+    HIP_OR_FAIL(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    g->own_stream = true;
+    int rc = guide_setup(g.get(), max_samples, seed, 1e-3f, 0.5f);
+    if (rc) {
+        (void)hipStreamDestroy(g->stream);
+        return rc;
+    }
+    dev->live++;
+    *out = g.release();
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_guide_destroy(wsrt_guide* g) {
+    if (!g) return WSRT_OK;
+    if (g->owned_by_renderer) return fail(WSRT_E_STATE, "the guide belongs to a renderer");
+    (void)hipSetDevice(g->dev->ordinal);
+    (void)hipStreamSynchronize(g->stream);
+    (void)hipStreamDestroy(g->stream);
+    g->dev->live--;
+    delete g;
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_guide_get_params(wsrt_guide* g, float* out) {
+    CHECK_ARG(g && out);
+    HIP_OR_FAIL(hipSetDevice(g->dev->ordinal));
+    HIP_OR_FAIL(hipMemcpyAsync(out, g->params.p, 4ull * guide::kParams, hipMemcpyDeviceToHost, g->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(g->stream));
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_guide_set_params(wsrt_guide* g, const float* in) {
+    CHECK_ARG(g && in);
+    HIP_OR_FAIL(hipSetDevice(g->dev->ordinal));
+    std::vector<float> w(in, in + guide::kParams);
+    // padding (W1 columns >= 168, output rows >= 40) stays zero
+    for (int r = 0; r < guide::kHid; ++r)
+        for (int c = guide::kFeat; c < guide::kIn; ++c) w[guide::kW1 + r * guide::kIn + c] = 0.0f;
+    for (int r = guide::kOut; r < guide::kOutPad; ++r) {
+        for (int c = 0; c < guide::kHid; ++c) w[guide::kW5 + r * guide::kHid + c] = 0.0f;
+        w[guide::kB5 + r] = 0.0f;
+    }
+    HIP_OR_FAIL(hipMemcpyAsync(g->params.p, w.data(), 4ull * guide::kParams, hipMemcpyHostToDevice, g->stream));
+    HIP_OR_FAIL(hipMemsetAsync(g->m1.p, 0, g->m1.bytes, g->stream));
+    HIP_OR_FAIL(hipMemsetAsync(g->m2.p, 0, g->m2.bytes, g->stream));
+    HIP_OR_FAIL(hipMemsetAsync(g->step.p, 0, 4, g->stream));
+    launch_guide_pack(g->gv, g->params.as<float>(), g->stream);
+    HIP_OR_FAIL(hipGetLastError());
+    HIP_OR_FAIL(hipStreamSynchronize(g->stream));
+    return WSRT_OK;
+}
+
+namespace {
+// Uploads n samples (scalars n x 7, optional dirs n x 3 and weights n),
+// zero-padded to ns, into the guide's training inputs.
+int guide_upload(wsrt_guide* g, const float* scalars, const float* dirs, const float* weights, uint32_t n,
+                 uint32_t ns) {
+    std::vector<float4> a(ns), b(ns), d(ns);
+    for (uint32_t i = 0; i < ns; ++i) {
+        if (i < n) {
+            const float* x = scalars + 7ull * i;
+            a[i] = make_float4(x[0], x[1], x[2], x[3]);
+            b[i] = make_float4(x[4], x[5], x[6], weights ? weights[i] : 0.0f);
+            d[i] = dirs ? make_float4(dirs[3ull * i], dirs[3ull * i + 1], dirs[3ull * i + 2], 0.0f)
+                        : make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+        } else {
+            a[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            b[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            d[i] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+        }
+    }
+    HIP_OR_FAIL(hipMemcpyAsync(g->tin0.p, a.data(), 16ull * ns, hipMemcpyHostToDevice, g->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(g->tin1.p, b.data(), 16ull * ns, hipMemcpyHostToDevice, g->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(g->tdir.p, d.data(), 16ull * ns, hipMemcpyHostToDevice, g->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(g->stream));
+    return WSRT_OK;
+}
+}  // namespace
+
+extern "C" int wsrt_guide_forward(wsrt_guide* g, const float* scalars, uint32_t n, float* out) {
+    CHECK_ARG(g && (scalars || n == 0) && (out || n == 0));
+    if (n == 0) return WSRT_OK;
+    HIP_OR_FAIL(hipSetDevice(g->dev->ordinal));
+    if (n > g->ns_max) return fail(WSRT_E_INVALID, "n %u > max_samples %u", n, g->ns_max);
+    int rc = guide_upload(g, scalars, nullptr, nullptr, n, n);
+    if (rc) return rc;
+    HIP_OR_FAIL(g->out.alloc(4ull * guide::kOut * n));
+    launch_guide_infer(g->gv, g->tin0.as<float4>(), g->tin1.as<float4>(), n, g->out.as<float>(), g->stream);
+    HIP_OR_FAIL(hipGetLastError());
+    HIP_OR_FAIL(hipMemcpyAsync(out, g->out.p, 4ull * guide::kOut * n, hipMemcpyDeviceToHost, g->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(g->stream));
+    return WSRT_OK;
+}
+
+static int guide_train_call(wsrt_guide* g, const float* scalars, const float* dirs, const float* weights,
+                            uint32_t n, bool update, float lr, float* grads) {
+    CHECK_ARG(g && scalars && dirs && weights && n > 0);
+    HIP_OR_FAIL(hipSetDevice(g->dev->ordinal));
+    uint32_t ns, chunk;
+    guide_step_shape(n, &ns, &chunk);
+    if (ns > g->ns_max) return fail(WSRT_E_INVALID, "n %u > max_samples %u", n, g->ns_max);
+    int rc = guide_upload(g, scalars, dirs, weights, n, ns);
+    if (rc) return rc;
+    GuideTrain gt = g->gt;
+    gt.chunk = chunk;
+    if (update) gt.lr = lr;
+    launch_guide_train_step(g->gv, gt, g->tin0.as<float4>(), g->tin1.as<float4>(), g->tdir.as<float4>(), ns,
+                            1.0f / (float)n, update, g->stream);
+    HIP_OR_FAIL(hipGetLastError());
+    if (grads) {
+        const uint32_t nc = ns / chunk;
+        std::vector<float> part((size_t)nc * guide::kParams);
+        HIP_OR_FAIL(hipMemcpyAsync(part.data(), g->partial.p, 4ull * part.size(), hipMemcpyDeviceToHost,
+                                   g->stream));
+        HIP_OR_FAIL(hipStreamSynchronize(g->stream));
+        for (int i = 0; i < guide::kParams; ++i) {
+            double acc = 0.0;
+            for (uint32_t c = 0; c < nc; ++c) acc += part[(size_t)c * guide::kParams + i];
+            grads[i] = (float)acc;
+        }
+    }
+    HIP_OR_FAIL(hipStreamSynchronize(g->stream));
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_guide_gradients(wsrt_guide* g, const float* scalars, const float* dirs,
+                                    const float* weights, uint32_t n, float* grads) {
+    CHECK_ARG(grads);
+    return guide_train_call(g, scalars, dirs, weights, n, false, 0.0f, grads);
+}
+
+extern "C" int wsrt_guide_train_step(wsrt_guide* g, const float* scalars, const float* dirs,
+                                     const float* weights, uint32_t n, float lr) {
+    CHECK_ARG(lr > 0.0f);
+    return guide_train_call(g, scalars, dirs, weights, n, true, lr, nullptr);
+}
+
+extern "C" int wsrt_renderer_get_guide(wsrt_renderer* r, wsrt_guide** out) {
+    CHECK_ARG(r && out);
+    if (!r->guide) return fail(WSRT_E_STATE, "the renderer was created without guiding");
+    HIP_OR_FAIL(hipStreamSynchronize(r->stream));
+    *out = r->guide.get();
     return WSRT_OK;
 }

@@ -86,6 +86,9 @@ struct PathState {
     uint32_t* queue_b;  // and the next bounce
     uint32_t* shade_q;  // hits per material type (kMatBins * 64 * seg_cap)
     float4* shadow;     // 3 float4 per shadow ray: o+tmax, d+path, contribution
+    uint32_t* guide_q;  // Lambert vertices for guided scattering (segmented, 64 * seg_cap)
+    float4* gn;         // forward-facing shading normal at a guided vertex
+    float4* galb;       // spectral albedo at a guided vertex
     uint32_t capacity;  // path slots (frames in flight x pixels)
 };
 

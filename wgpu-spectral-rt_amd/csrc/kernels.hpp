@@ -5,6 +5,7 @@
 
 #include "common.hpp"
 #include "dev_math.hpp"
+#include "guide.hpp"
 #include "queues.hpp"
 
 namespace wsrt {
@@ -78,6 +79,43 @@ size_t grid_temp_words(uint32_t n, uint32_t cells);
 int grid_build(const wsrt_photon* photons, uint32_t np, uint32_t nf, uint32_t hash_size,
                const FrameArgs& fa, GridBuffers& g, hipStream_t s);
 
+// ---- path guiding (guide.hip) ------------------------------------------------
+// Device view of the guiding network used while tracing.
+struct GuideView {
+    const uint16_t* wf;     // packed forward operands (BF16 bits)
+    const uint16_t* wt;     // packed backward operands
+    const float* params;    // fp32 parameters (biases are read from here)
+    float mix;              // one-sample MIS probability of the guide
+    float bmin[3], binv[3]; // scene box for the position inputs
+    guide::Records rec;     // training records (null: none)
+};
+// Buffers of a training step.
+struct GuideTrain {
+    uint32_t* act;          // guide_act_words(ns) words
+    float* partial;         // (ns / chunk) x kParams
+    float* params;          // fp32 master copy
+    float* m1;              // Adam moments
+    float* m2;
+    uint32_t* step;         // Adam step counter (device)
+    uint32_t chunk;         // samples per partial sum (divides ns, multiple of 32)
+    float lr;
+};
+size_t guide_act_words(uint32_t ns);
+void launch_guide_scatter(const PathState& ps, const Queues& qs, const GuideView& gv, const FrameArgs& fa,
+                          uint32_t bounce, uint32_t* next_queue, hipStream_t s);
+// Writes the packed operands of params (after an upload or an initialisation).
+void launch_guide_pack(const GuideView& gv, float* params, hipStream_t s);
+// One step over ns samples (a multiple of 64 and of gt.chunk), the loss
+// averaged with factor norm: gradients into gt.partial; with update, Adam
+// and repacking.
+void launch_guide_train_step(const GuideView& gv, const GuideTrain& gt, const float4* in0, const float4* in1,
+                             const float4* dir, uint32_t ns, float norm, bool update, hipStream_t s);
+void launch_guide_prepare(const PathState& ps, const GuideView& gv, const FrameArgs& fa, uint32_t npx,
+                          uint32_t slot, uint32_t ns, uint32_t stride, float4* in0, float4* in1, float4* dir,
+                          hipStream_t s);
+void launch_guide_infer(const GuideView& gv, const float4* in0, const float4* in1, uint32_t n, float* out,
+                        hipStream_t s);
+
 // ---- wavefront camera paths (path.hip) ----------------------------------------
 struct PathLaunch {
     SceneView sv;
@@ -90,6 +128,8 @@ struct PathLaunch {
     uint32_t count_traversal;
     FrameArgs fa;
     Queues qs;
+    uint32_t guided;  // 1: Lambert vertices scatter through k_guide_scatter
+    GuideView gv;
 };
 void launch_clear_queues(const Queues& qs, uint32_t depths, hipStream_t s);
 void launch_advance_frame(uint32_t* frame, uint32_t nf, hipStream_t s);

@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_extend(SceneView sv, PathStat
 
 __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState ps, GridView grid,
                                                          FrameArgs fa, uint32_t npx, Queues qs,
-                                                         uint32_t bounce,
+                                                         uint32_t bounce, uint32_t guided,
                                                          uint32_t* __restrict__ next_queue) {
     __shared__ uint32_t pre[kMatBins * kSegs + 1];
     const uint32_t base_frame = *fa.frame;
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
         uint32_t p = 0;
         float4 o4 = splat4(0.0f), d4 = splat4(0.0f), tp = splat4(0.0f), L = splat4(0.0f);
         uint32_t rng = 0, flags = 0;
-        bool shadow = false;
+        bool shadow = false, deferred = false;
         f3 hp = mk3(0, 0, 0), sdir = mk3(0, 0, 0);
         float stmax = 0.0f;
         float4 contrib = splat4(0.0f);
@@ -223,7 +223,17 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
                         stmax = ls.dist - 0.001f;
                     }
                 }
-                Scatter sc = sample_bsdf(d, ff, m, front, wl, tp, rng);
+                if (type == 0u && guided) {
+                    // guided scattering of this Lambert vertex follows in
+                    // k_guide_scatter, after the vertex's shadow ray
+                    deferred = true;
+                    ps.gn[p] = f4(ff, 0.0f);
+                    ps.galb[p] = rgb_to_spectrum4(mk3(col.x, col.y, col.z), wl);
+                    o4 = f4(hp, 0.0f);
+                }
+                Scatter sc;
+                sc.absorbed = true;
+                if (!deferred) sc = sample_bsdf(d, ff, m, front, wl, tp, rng);
                 if (!sc.absorbed) {
                     tp = mul4(tp, sc.att);
                     o4 = f4(hp, 0.0f);
@@ -237,6 +247,11 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
                 }
             }
             ps.L[p] = L;
+            if (deferred) {
+                ps.o[p] = o4;
+                ps.rng[p] = rng;
+                ps.flags[p] = flags;
+            }
             if (alive) {
                 ps.o[p] = o4;
                 ps.d[p] = d4;
@@ -254,6 +269,8 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
         }
         const uint32_t ne = seg_append(qs, bounce + 1, kQActive, qs.seg_cap, alive);
         if (alive) next_queue[ne] = p;
+        const uint32_t ge = seg_append(qs, bounce, kQGuide, qs.seg_cap, deferred);
+        if (deferred) ps.guide_q[ge] = p;
     }
 }
 
@@ -348,7 +365,7 @@ void launch_extend(const PathLaunch& pl, uint32_t bounce, uint32_t n0, const uin
 }
 void launch_shade(const PathLaunch& pl, uint32_t npx, uint32_t bounce, uint32_t* next_queue, hipStream_t s) {
     hipLaunchKernelGGL(k_shade, dim3(kShadeBlocks), dim3(kShadeThreads), 0, s, pl.sv, pl.ps, pl.grid,
-                       pl.fa, npx, pl.qs, bounce, next_queue);
+                       pl.fa, npx, pl.qs, bounce, pl.guided, next_queue);
 }
 void launch_shadow(const PathLaunch& pl, uint32_t bounce, hipStream_t s) {
     hipLaunchKernelGGL(k_shadow, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, pl.sv, pl.ps, pl.qs,

@@ -29,6 +29,7 @@ enum QueueKind : uint32_t {
     kQShade0 = 1,                      // hits of a bounce, one queue per material type
     kQShadow = kQShade0 + kMatBins,    // shadow rays of a bounce
     kQPhoton = kQShadow + 1,           // photon paths entering a bounce
+    kQGuide = kQPhoton + 1,            // Lambert vertices waiting for guided scattering
     kQKinds
 };
 

@@ -10,6 +10,10 @@ drive any builder with that interface.
   c2_caustic    glass icosphere (subdivision 4, radius 0.5, Cauchy A = 1.5,
                 B = 0.0042 um^2) on a 2-triangle diffuse plane, point light
                 2 units above the sphere centre
+  occluded_room two rooms split by a wall with a 0.05-wide full-height door
+                gap; the only light is on the ceiling of the far room, so the
+                near room is lit through the gap; both rooms are cluttered
+                with seeded boxes (174,762 boxes for the 2M-triangle room)
 
 Materials use the reference's layout (builder.rs:5-10): extra.x = type
 (0 Lambert, 1 GGX, 2 dielectric, 3 emitter, 4 thin film).
@@ -122,3 +126,95 @@ def camera(view: View, frame: int = 0, builder=None):
     ctl = wsrt.CameraController()
     ctl.position, ctl.yaw, ctl.pitch = view.position, view.yaw, view.pitch
     return ctl.build_uniform(view.width / view.height, frame)
+
+
+def _clutter_mesh(n_boxes: int, seed: int, region):
+    """n_boxes cubes merged into one mesh: edge lengths uniform in
+    [0.02, 0.15] (height up to 0.3), rotated about Y by a uniform angle,
+    standing on the floor at uniform positions inside region
+    ((x0, x1), (z0, z1)). Seeded numpy PCG64."""
+    cv, ci = wsrt.create_cube()
+    rng = np.random.default_rng(seed)
+    sx = rng.uniform(0.02, 0.15, n_boxes)
+    sy = rng.uniform(0.02, 0.3, n_boxes)
+    sz = rng.uniform(0.02, 0.15, n_boxes)
+    ang = rng.uniform(0.0, 2.0 * math.pi, n_boxes)
+    (x0, x1), (z0, z1) = region
+    tx = rng.uniform(x0, x1, n_boxes)
+    tz = rng.uniform(z0, z1, n_boxes)
+    c, sn = np.cos(ang), np.sin(ang)
+    p = cv["pos"][:, :3].astype(np.float64)
+    nrm = cv["normal"][:, :3].astype(np.float64)
+    # scale, rotate about Y, translate (box bottom on y = 0)
+    px = p[None, :, 0] * sx[:, None]
+    py = (p[None, :, 1] + 0.5) * sy[:, None]
+    pz = p[None, :, 2] * sz[:, None]
+    wx = c[:, None] * px + sn[:, None] * pz + tx[:, None]
+    wz = -sn[:, None] * px + c[:, None] * pz + tz[:, None]
+    nx = c[:, None] * nrm[None, :, 0] + sn[:, None] * nrm[None, :, 2]
+    nz = -sn[:, None] * nrm[None, :, 0] + c[:, None] * nrm[None, :, 2]
+    ny = np.broadcast_to(nrm[None, :, 1], nx.shape)
+    v = np.zeros(n_boxes * len(cv), wsrt.VERTEX_DTYPE)
+    v["pos"][:, 0] = wx.reshape(-1)
+    v["pos"][:, 1] = py.reshape(-1)
+    v["pos"][:, 2] = wz.reshape(-1)
+    v["pos"][:, 3] = 1.0
+    v["normal"][:, 0] = nx.reshape(-1)
+    v["normal"][:, 1] = ny.reshape(-1)
+    v["normal"][:, 2] = nz.reshape(-1)
+    idx = (ci[None, :].astype(np.int64) + (np.arange(n_boxes) * len(cv))[:, None]).astype(np.uint32)
+    return v, idx.reshape(-1)
+
+
+OCCLUDED_BOXES = 174762  # x 12 triangles + 24 room triangles = 2,097,168 triangles
+
+
+def build_occluded_room(b, n_boxes: int = OCCLUDED_BOXES, seed: int = 5):
+    """Near room z in [0, 2], far room z in [-2, 0], x in [-2, 2], y in [0, 2].
+    The partition at z = 0 is two 0.05-thick slabs leaving a door gap
+    x in [-0.025, 0.025] over the full height. A 0.5 x 0.5 blackbody
+    (6500 K, intensity 30) quad on the far room's ceiling faces down."""
+    white = b.add_material([0.75, 0.75, 0.75, 1.0], [0, 0, 0, 0])
+    warm = b.add_material([0.7, 0.55, 0.4, 1.0], [0, 0, 0, 0])
+    cool = b.add_material([0.35, 0.45, 0.6, 1.0], [0, 0, 0, 0])
+    light = b.add_material([6500.0, 30.0, 0.0, 1.0], [3, 0, 0, 0])
+    plane = b.add_mesh(*wsrt.create_plane())
+    cube = b.add_mesh(*wsrt.create_cube())
+    b.add_instance(plane, white, _trs((0, 0, 0), s=(4, 1, 4)))                 # floor
+    b.add_instance(plane, white, _trs((0, 2, 0), "x", PI, (4, 1, 4)))          # ceiling
+    b.add_instance(plane, white, _trs((0, 1, -2), "x", HALF_PI, (4, 1, 2)))    # far wall
+    b.add_instance(plane, white, _trs((0, 1, 2), "x", -HALF_PI, (4, 1, 2)))    # near wall
+    b.add_instance(plane, warm, _trs((-2, 1, 0), "z", -HALF_PI, (2, 1, 4)))    # left
+    b.add_instance(plane, cool, _trs((2, 1, 0), "z", HALF_PI, (2, 1, 4)))      # right
+    half = (2.0 - 0.025) / 2.0
+    b.add_instance(cube, white, _trs((-0.025 - half, 1.0, 0.0), s=(2 * half, 2.0, 0.05)))
+    b.add_instance(cube, white, _trs((0.025 + half, 1.0, 0.0), s=(2 * half, 2.0, 0.05)))
+    b.add_light_instance(plane, light, _trs((0, 1.99, -1.2), "x", PI, (0.5, 1, 0.5)),
+                         [0.0, 6500.0, 30.0, 0.0])
+    if n_boxes:
+        nf = n_boxes // 2
+        va, ia = _clutter_mesh(nf, seed, ((-1.9, 1.9), (-1.9, -0.15)))
+        vb, ib = _clutter_mesh(n_boxes - nf, seed + 1, ((-1.9, 1.9), (0.15, 1.9)))
+        b.add_instance(b.add_mesh(va, ia), warm, _trs((0, 0, 0)))
+        b.add_instance(b.add_mesh(vb, ib), white, _trs((0, 0, 0)))
+    return b
+
+
+OCCLUDED_VIEW = View((0.0, 1.0, 1.9), -HALF_PI, -0.15, 1024, 1024)
+
+
+def occluded_config(**kw):
+    """Plain tracer settings of the occluded-light room (no photon pass, depth 8)."""
+    c = wsrt.default_config(max_depth=8, enable_photons=0)
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def guided_config(**kw):
+    """The guided tracer: 4 x 64 MLP, 8 vMF lobes, Adam lr 1e-3, 2^18 training
+    samples per frame, 50 % one-sample MIS with BSDF sampling."""
+    c = occluded_config(guiding=1, guide_mix=0.5, guide_lr=1e-3, guide_train_samples=1 << 18)
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c

@@ -127,7 +127,18 @@ _SIGS = {
     "wsrt_debug_read_morton": (C.c_int, [vp, vp, vp]),
     "wsrt_debug_read_bvh": (C.c_int, [vp, vp, vp]),
     "wsrt_debug_read_world": (C.c_int, [vp, vp, vp]),
+    "wsrt_guide_create": (C.c_int, [vp, u32, u32, P(vp)]),
+    "wsrt_guide_destroy": (C.c_int, [vp]),
+    "wsrt_guide_get_params": (C.c_int, [vp, vp]),
+    "wsrt_guide_set_params": (C.c_int, [vp, vp]),
+    "wsrt_guide_forward": (C.c_int, [vp, vp, u32, vp]),
+    "wsrt_guide_gradients": (C.c_int, [vp, vp, vp, vp, u32, vp]),
+    "wsrt_guide_train_step": (C.c_int, [vp, vp, vp, vp, u32, C.c_float]),
+    "wsrt_renderer_get_guide": (C.c_int, [vp, P(vp)]),
 }
+
+GUIDE_PARAMS = 27952
+GUIDE_OUTPUTS = 40
 
 _lib = None
 
@@ -397,6 +408,61 @@ class CameraController:
 
 
 # ---- renderer (src/renderer/mod.rs) -----------------------------------------------
+class Guide:
+    """The path-guiding network (wsrt_guide_*): 7 input scalars in [0, 1] ->
+    40 outputs (8 vMF lobes of mu xyz, log kappa, logit)."""
+
+    def __init__(self, device: Device | None = None, max_samples: int = 1 << 16, seed: int = 1,
+                 _handle=None, _owner=None):
+        self._owner = _owner
+        if _handle is not None:
+            self.h = _handle
+            return
+        h = vp()
+        _check(load().wsrt_guide_create(device.h, max_samples, seed, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) and self._owner is None:
+            load().wsrt_guide_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.zeros(GUIDE_PARAMS, np.float32)
+        _check(load().wsrt_guide_get_params(self.h, _p(out)))
+        return out
+
+    @params.setter
+    def params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.shape == (GUIDE_PARAMS,)
+        _check(load().wsrt_guide_set_params(self.h, _p(w)))
+
+    def forward(self, scalars) -> np.ndarray:
+        x = np.ascontiguousarray(scalars, np.float32).reshape(-1, 7)
+        out = np.zeros((len(x), GUIDE_OUTPUTS), np.float32)
+        _check(load().wsrt_guide_forward(self.h, _p(x), len(x), _p(out)))
+        return out
+
+    def gradients(self, scalars, dirs, weights) -> np.ndarray:
+        x = np.ascontiguousarray(scalars, np.float32).reshape(-1, 7)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        w = np.ascontiguousarray(weights, np.float32).reshape(-1)
+        g = np.zeros(GUIDE_PARAMS, np.float32)
+        _check(load().wsrt_guide_gradients(self.h, _p(x), _p(d), _p(w), len(x), _p(g)))
+        return g
+
+    def train_step(self, scalars, dirs, weights, lr: float = 1e-3):
+        x = np.ascontiguousarray(scalars, np.float32).reshape(-1, 7)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        w = np.ascontiguousarray(weights, np.float32).reshape(-1)
+        _check(load().wsrt_guide_train_step(self.h, _p(x), _p(d), _p(w), len(x), lr))
+
+
 class Renderer:
     def __init__(self, device: Device, scene: Scene, target_width: int, target_height: int,
                  config: Config | None = None):
@@ -430,6 +496,12 @@ class Renderer:
 
     def set_camera(self, cam: CameraUniform):
         _check(load().wsrt_renderer_set_camera(self.h, C.byref(cam)))
+
+    def guide(self) -> Guide:
+        """The guiding network this renderer trains (config.guiding = 1)."""
+        h = vp()
+        _check(load().wsrt_renderer_get_guide(self.h, C.byref(h)))
+        return Guide(_handle=h, _owner=self)
 
     def render(self):
         _check(load().wsrt_renderer_render(self.h))
