@@ -21,6 +21,7 @@ import numpy as np
 
 SCALARS, BANDS, FEAT, IN, HID, OUT, OUTPAD, LOBES = 7, 12, 168, 192, 64, 40, 48, 8
 LOG_KAPPA_MIN, LOG_KAPPA_MAX = -3.0, 7.0
+KAPPA_SCALE, LOGIT_SCALE = 8.0, 4.0  # kappa = exp(clamp(8 s)), logit = 4 a
 
 # parameter layout (matches the ABI's WSRT_GUIDE_PARAMS layout)
 W1 = 0
@@ -96,9 +97,9 @@ def mixture(out):
     m = o[:, :, 0:3]
     ln = np.maximum(np.linalg.norm(m, axis=2), 1e-6)
     mu = m / ln[:, :, None]
-    sv = o[:, :, 3]
+    sv = KAPPA_SCALE * o[:, :, 3]
     kap = np.exp(np.clip(sv, LOG_KAPPA_MIN, LOG_KAPPA_MAX))
-    lg = o[:, :, 4]
+    lg = LOGIT_SCALE * o[:, :, 4]
     e = np.exp(lg - lg.max(axis=1, keepdims=True))
     pi = e / e.sum(axis=1, keepdims=True)
     norm = kap / (2 * np.pi * -np.expm1(-2 * kap))
@@ -123,12 +124,12 @@ def loss_grad_out(out, dirs, weights, norm_factor):
     r = np.where(wt != 0, pi * v / np.maximum(p, 1e-30), 0.0)
     dlk = 1 / kap - 2 / np.expm1(2 * kap) + (c - 1)
     in_range = (sv > LOG_KAPPA_MIN) & (sv < LOG_KAPPA_MAX)
-    gs = np.where(in_range, -wt * r * kap * dlk, 0.0)
+    gs = np.where(in_range, -wt * r * kap * dlk * KAPPA_SCALE, 0.0)
     gm = (-wt * r * kap / ln)[:, :, None] * (w - mu * c[:, :, None])
     g = np.zeros((len(out), LOBES, 5))
     g[:, :, 0:3] = gm
     g[:, :, 3] = gs
-    g[:, :, 4] = wt * (pi - r)
+    g[:, :, 4] = LOGIT_SCALE * wt * (pi - r)
     full = np.zeros((len(out), OUTPAD))
     full[:, :OUT] = g.reshape(len(out), OUT)
     return full

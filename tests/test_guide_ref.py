@@ -42,8 +42,8 @@ def torch_loss(p, s, d, w):
     m = o[:, :, :3]
     ln = torch.clamp(torch.linalg.norm(m, dim=2), min=1e-6)
     mu = m / ln[:, :, None]
-    kap = torch.exp(torch.clamp(o[:, :, 3], G.LOG_KAPPA_MIN, G.LOG_KAPPA_MAX))
-    pi = torch.softmax(o[:, :, 4], dim=1)
+    kap = torch.exp(torch.clamp(G.KAPPA_SCALE * o[:, :, 3], G.LOG_KAPPA_MIN, G.LOG_KAPPA_MAX))
+    pi = torch.softmax(G.LOGIT_SCALE * o[:, :, 4], dim=1)
     norm = kap / (2 * np.pi * -torch.expm1(-2 * kap))
     dd = torch.tensor(d).reshape(-1, 1, 3)
     c = (mu * dd).sum(2)
@@ -106,7 +106,7 @@ def fib_sphere(n):
 def test_mixture_pdf_integrates_to_one():
     rng = np.random.default_rng(7)
     out = rng.normal(0, 1, (3, 40))
-    out[:, 3::5] = rng.uniform(-2, 3.5, (3, 8))  # kappa from 0.14 to 33
+    out[:, 3::5] = rng.uniform(-2, 3.5, (3, 8)) / G.KAPPA_SCALE  # kappa from 0.14 to 33
     dirs = fib_sphere(400000)
     for o in out:
         p = G.pdf(np.repeat(o[None], len(dirs), 0), dirs)

@@ -809,7 +809,8 @@ struct wsrt_guide {
 
 namespace {
 
-constexpr uint32_t kGuideChunk = 2048;
+constexpr uint32_t kGuideChunk = 256;    // samples per partial gradient sum
+constexpr uint32_t kGuideBatch = 16384;  // samples per Adam step while rendering
 
 // Training-step size for n samples: a multiple of 64, and of the partial-sum
 // chunk (which divides it).
@@ -852,7 +853,7 @@ std::vector<float> guide_initial_params(uint32_t seed) {
         w[kB5 + 5 * k + 0] = (k & 1) ? -d : d;
         w[kB5 + 5 * k + 1] = (k & 2) ? -d : d;
         w[kB5 + 5 * k + 2] = (k & 4) ? -d : d;
-        w[kB5 + 5 * k + 3] = 0.6931472f;  // log 2
+        w[kB5 + 5 * k + 3] = 0.6931472f / kKappaScale;  // kappa = 2
         w[kB5 + 5 * k + 4] = 0.0f;
     }
     return w;
@@ -890,7 +891,7 @@ int guide_setup(wsrt_guide* g, uint32_t max_samples, uint32_t seed, float lr, fl
         g->gv.bmin[a] = 0.0f;
         g->gv.binv[a] = 1.0f;
     }
-    g->gv.rec = guide::Records{nullptr, nullptr, nullptr, nullptr, nullptr};
+    g->gv.rec = guide::Records{nullptr, nullptr, nullptr, nullptr, nullptr, 0};
     g->gt.act = g->act.as<uint32_t>();
     g->gt.partial = g->partial.as<float>();
     g->gt.params = g->params.as<float>();
@@ -945,7 +946,7 @@ struct wsrt_renderer {
     // path guiding (cfg.guiding)
     std::unique_ptr<wsrt_guide> guide;
     DevBuf g_in0, g_in1, g_dir, g_tp, g_Lb, g_n, g_alb, g_q;
-    uint32_t guide_ns = 0, guide_stride = 1;
+    uint32_t guide_ns = 0, guide_stride = 1, guide_batch = 0;
 };
 
 namespace {
@@ -1113,11 +1114,12 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     pl.qs.seg_cap = seg;
     pl.guided = c.guiding;
     if (c.guiding) {
-        HIP_OR_FAIL(r->g_in0.alloc(16 * npaths));
-        HIP_OR_FAIL(r->g_in1.alloc(16 * npaths));
-        HIP_OR_FAIL(r->g_dir.alloc(16 * npaths));
-        HIP_OR_FAIL(r->g_tp.alloc(16 * npaths));
-        HIP_OR_FAIL(r->g_Lb.alloc(16 * npaths));
+        const uint64_t nrec = npaths * guide::kRecBounces;
+        HIP_OR_FAIL(r->g_in0.alloc(16 * nrec));
+        HIP_OR_FAIL(r->g_in1.alloc(16 * nrec));
+        HIP_OR_FAIL(r->g_dir.alloc(16 * nrec));
+        HIP_OR_FAIL(r->g_tp.alloc(16 * nrec));
+        HIP_OR_FAIL(r->g_Lb.alloc(16 * nrec));
         HIP_OR_FAIL(r->g_n.alloc(16 * npaths));
         HIP_OR_FAIL(r->g_alb.alloc(16 * npaths));
         HIP_OR_FAIL(r->g_q.alloc(4ull * kSegs * seg));
@@ -1126,13 +1128,16 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
         g->dev = dev;
         g->stream = r->stream;
         g->owned_by_renderer = true;
-        const uint32_t want = (uint32_t)std::min<uint64_t>(c.guide_train_samples, npx);
+        const uint32_t want = (uint32_t)std::min<uint64_t>(c.guide_train_samples, npx * guide::kRecBounces);
         // whole chunks of samples per step (at least one step of 64)
-        uint32_t ns = want >= kGuideChunk ? want / kGuideChunk * kGuideChunk : (want + 63u) & ~63u;
+        uint32_t ns = want >= kGuideBatch   ? want / kGuideBatch * kGuideBatch
+                      : want >= kGuideChunk ? want / kGuideChunk * kGuideChunk
+                                            : (want + 63u) & ~63u;
         int grc = guide_setup(g, ns, 0x5eedu, c.guide_lr, c.guide_mix);
         if (grc) return grc;
         r->guide_ns = ns;
-        r->guide_stride = (uint32_t)std::max<uint64_t>(1, npx / ns);
+        r->guide_stride = (uint32_t)std::max<uint64_t>(1, npx * guide::kRecBounces / ns);
+        r->guide_batch = std::min(ns, kGuideBatch);  // divides ns
         const wsrt_scene_info& si = scene->info;
         for (int a = 0; a < 3; ++a) {
             const float lo = si.bounds[a], ext = si.bounds[3 + a] - si.bounds[a];
@@ -1140,7 +1145,7 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
             g->gv.binv[a] = ext > 0.0f ? 1.0f / ext : 1.0f;
         }
         g->gv.rec = guide::Records{r->g_in0.as<float4>(), r->g_in1.as<float4>(), r->g_dir.as<float4>(),
-                                   r->g_tp.as<float4>(), r->g_Lb.as<float4>()};
+                                   r->g_tp.as<float4>(), r->g_Lb.as<float4>(), (uint32_t)npaths};
         pl.gv = g->gv;
     }
     pl.ps.guide_q = r->g_q.as<uint32_t>();
@@ -1240,14 +1245,19 @@ static int enqueue_batch(wsrt_renderer* r, uint32_t nf) {
         launch_accumulate(pl, npx, nf, s);
     }
     if (pl.guided) {
-        // one Adam step per frame of the batch, on that frame's records
+        // each frame's records train the network in Adam steps over
+        // minibatches of guide_batch samples
         ProfScope p(r, K_GUIDE_TRAIN);
         wsrt_guide* g = r->guide.get();
+        const uint32_t mb = r->guide_batch;
+        GuideTrain gt = g->gt;
+        gt.chunk = std::min(mb, kGuideChunk);
         for (uint32_t slot = 0; slot < nf; ++slot) {
             launch_guide_prepare(pl.ps, pl.gv, pl.fa, npx, slot, r->guide_ns, r->guide_stride,
                                  g->tin0.as<float4>(), g->tin1.as<float4>(), g->tdir.as<float4>(), s);
-            launch_guide_train_step(g->gv, g->gt, g->tin0.as<float4>(), g->tin1.as<float4>(),
-                                    g->tdir.as<float4>(), r->guide_ns, 1.0f / (float)r->guide_ns, true, s);
+            for (uint32_t off = 0; off < r->guide_ns; off += mb)
+                launch_guide_train_step(g->gv, gt, g->tin0.as<float4>() + off, g->tin1.as<float4>() + off,
+                                        g->tdir.as<float4>() + off, mb, 1.0f / (float)mb, true, s);
         }
     }
     launch_advance_frame(r->dframe.as<uint32_t>(), nf, s);

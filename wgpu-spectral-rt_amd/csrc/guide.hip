@@ -27,7 +27,8 @@ constexpr int kStage = 41;            // floats per sample row of the output sta
 constexpr int kScatterWaves = 8;      // waves per block of the scattering kernel
 constexpr int kTrainWaves = 4;        // waves per block of the training kernel
 constexpr int kTrainTiles = 2;        // 16-sample tiles per wave while training
-constexpr uint32_t kFlagRecorded = 4u;  // path flag: first Lambert vertex recorded for training
+// path flags 4 << b: the Lambert vertex of bounce b is recorded for training
+constexpr uint32_t kFlagRecorded = 4u;
 
 __device__ __forceinline__ short bf16_bits(float f) {  // round to nearest even
     uint32_t u = __float_as_uint(f);
@@ -178,7 +179,7 @@ struct Mixture {
 __device__ __forceinline__ void mixture_from(const float* o, Mixture& m) {
     float mx = -3.0e38f;
 #pragma unroll
-    for (int k = 0; k < kLobes; ++k) mx = fmaxf(mx, o[5 * k + 4]);
+    for (int k = 0; k < kLobes; ++k) mx = fmaxf(mx, kLogitScale * o[5 * k + 4]);
     float z = 0.0f;
 #pragma unroll
     for (int k = 0; k < kLobes; ++k) {
@@ -186,11 +187,11 @@ __device__ __forceinline__ void mixture_from(const float* o, Mixture& m) {
         const float len = fmaxf(sqrtf(dot(v, v)), 1e-6f);
         m.len[k] = len;
         m.mu[k] = v * (1.0f / len);
-        m.s[k] = o[5 * k + 3];
+        m.s[k] = kKappaScale * o[5 * k + 3];  // log kappa before the clamp
         const float kap = expf(fminf(fmaxf(m.s[k], kLogKappaMin), kLogKappaMax));
         m.kappa[k] = kap;
         m.norm[k] = kap / (2.0f * kPi * -expm1f(-2.0f * kap));
-        m.pi[k] = expf(o[5 * k + 4] - mx);
+        m.pi[k] = expf(kLogitScale * o[5 * k + 4] - mx);
         z += m.pi[k];
     }
     const float iz = 1.0f / z;
@@ -316,17 +317,22 @@ __global__ __launch_bounds__(64 * kScatterWaves) void k_guide_scatter(PathState 
             if (cw > 0.0f) {
                 const float pm = gv.mix * mixture_pdf(m, w) + (1.0f - gv.mix) * (cw / kPi);
                 tp = mul4(mul4s(spec, (cw / kPi) / pm), tp);
-                if (!(flags & kFlagRecorded) && gv.rec.in0) {
-                    gv.rec.in0[p] = make_float4(s[0], s[1], s[2], s[3]);
-                    gv.rec.in1[p] = make_float4(s[4], s[5], s[6], pm);
-                    gv.rec.dir[p] = make_float4(w.x, w.y, w.z, cw);
-                    gv.rec.tp[p] = tp;
-                    gv.rec.Lb[p] = ps.L[p];
-                    flags |= kFlagRecorded;
+                if (bounce < kRecBounces && gv.rec.in0) {
+                    const size_t ri = (size_t)bounce * gv.rec.capacity + p;
+                    gv.rec.in0[ri] = make_float4(s[0], s[1], s[2], s[3]);
+                    gv.rec.in1[ri] = make_float4(s[4], s[5], s[6], pm);
+                    gv.rec.dir[ri] = make_float4(w.x, w.y, w.z, cw);
+                    gv.rec.tp[ri] = tp;
+                    gv.rec.Lb[ri] = ps.L[p];
+                    flags |= kFlagRecorded << bounce;
                 }
                 alive = true;
                 if (bounce > rr_after) {
-                    const float pr = fmaxf(fmaxf(tp.x, tp.y), fmaxf(tp.z, tp.w));
+                    // survival probability clamped to 1: guided throughput can
+                    // exceed 1, and dividing by a probability above 1 would
+                    // bias the estimate (the plain tracer keeps the reference's
+                    // unclamped form, where Lambert throughput stays <= 1)
+                    const float pr = fminf(fmaxf(fmaxf(tp.x, tp.y), fmaxf(tp.z, tp.w)), 1.0f);
                     if (rand01(rng) > pr) alive = false;
                     else tp = div4s(tp, pr);
                 }
@@ -344,10 +350,12 @@ __global__ __launch_bounds__(64 * kScatterWaves) void k_guide_scatter(PathState 
 }
 
 // Training inputs of one step from the records of frame slot `slot`:
-// sample k is pixel (k * stride + phase) mod npx. The weight of a sample is
+// sample k is record j = (k * stride + phase) mod (kRecBounces npx), the
+// vertex of bounce j / npx on pixel j % npx. The weight of a sample is
 // L_in cos / p_mix, L_in the incident radiance along the sampled direction
 // (radiance gathered after the vertex over the throughput after it, mean
-// over the hero wavelengths that carry throughput).
+// over the hero wavelengths that carry throughput); records a path did not
+// make have weight 0.
 __global__ __launch_bounds__(256) void k_guide_prepare(PathState ps, GuideView gv, FrameArgs fa, uint32_t npx,
                                                        uint32_t slot, uint32_t ns, uint32_t stride,
                                                        float4* __restrict__ in0, float4* __restrict__ in1,
@@ -355,14 +363,17 @@ __global__ __launch_bounds__(256) void k_guide_prepare(PathState ps, GuideView g
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ns) return;
     const uint32_t phase = (*fa.frame + slot) % stride;
-    const uint32_t p = slot * npx + (uint32_t)(((uint64_t)k * stride + phase) % npx);
+    const uint64_t j = ((uint64_t)k * stride + phase) % ((uint64_t)kRecBounces * npx);
+    const uint32_t bounce = (uint32_t)(j / npx);
+    const uint32_t p = slot * npx + (uint32_t)(j % npx);
+    const size_t ri = (size_t)bounce * gv.rec.capacity + p;
     float wgt = 0.0f;
     float4 a = splat4(0.0f), b = splat4(0.0f), d = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
-    if (ps.flags[p] & kFlagRecorded) {
-        a = gv.rec.in0[p];
-        b = gv.rec.in1[p];
-        d = gv.rec.dir[p];
-        const float4 T = gv.rec.tp[p], Lb = gv.rec.Lb[p], L = ps.L[p];
+    if (ps.flags[p] & (kFlagRecorded << bounce)) {
+        a = gv.rec.in0[ri];
+        b = gv.rec.in1[ri];
+        d = gv.rec.dir[ri];
+        const float4 T = gv.rec.tp[ri], Lb = gv.rec.Lb[ri], L = ps.L[p];
         float sum = 0.0f;
         int cnt = 0;
         if (T.x > 0.0f) { sum += (L.x - Lb.x) / T.x; cnt++; }
@@ -428,14 +439,14 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_guide_train_fwd(GuideView 
                 const float kap = m.kappa[j];
                 const float dlk = 1.0f / kap - 2.0f / expm1f(2.0f * kap) + (cth[j] - 1.0f);
                 const bool in_range = m.s[j] > kLogKappaMin && m.s[j] < kLogKappaMax;
-                const float gs = in_range ? -wt * r * kap * dlk : 0.0f;
+                const float gs = in_range ? -wt * r * kap * dlk * kKappaScale : 0.0f;
                 const float gm = -wt * r * kap / m.len[j];
                 const f3 perp = w - m.mu[j] * cth[j];
                 act.put(kD5, 5 * j + 0, col, gm * perp.x);
                 act.put(kD5, 5 * j + 1, col, gm * perp.y);
                 act.put(kD5, 5 * j + 2, col, gm * perp.z);
                 act.put(kD5, 5 * j + 3, col, gs);
-                act.put(kD5, 5 * j + 4, col, wt * (m.pi[j] - r));
+                act.put(kD5, 5 * j + 4, col, kLogitScale * wt * (m.pi[j] - r));
             }
 #pragma unroll
             for (int row = kOut; row < kOutPad; ++row) act.put(kD5, row, col, 0.0f);

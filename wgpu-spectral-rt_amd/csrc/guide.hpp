@@ -6,8 +6,9 @@
 // forward-facing shading normal mapped from [-1, 1], the hero wavelength
 // over 380..780 nm), each encoded by 12 frequency bands as sin and cos of
 // 2^b * pi * x (168 features, padded to 192), four hidden layers of 64 ReLU
-// units, and 40 outputs: 8 lobes of (mu_x, mu_y, mu_z, log kappa, logit).
-// The guide density is sum_k softmax(logit)_k * vMF(w; mu_k / |mu_k|, kappa_k).
+// units, and 40 outputs: 8 lobes of (mu_x, mu_y, mu_z, s, a), with
+// kappa = exp(clamp(8 s, -3, 7)) and logit = 4 a. The guide density is
+// sum_k softmax(logit)_k * vMF(w; mu_k / |mu_k|, kappa_k).
 //
 // Feature order (the column order of W1): column c = 8 q + j with slot
 // q = 3 a + bq (scalar a, band quad bq = 0..2) and j = 2 (band % 4) + (cos ? 1 : 0),
@@ -88,17 +89,22 @@ __host__ __device__ inline FeatureSlot feature_slot(int c) {
     return f;
 }
 
-// kappa = exp(clamp(s, kLogKappaMin, kLogKappaMax))
+// Output scaling: kappa = exp(clamp(kKappaScale * s, kLogKappaMin, kLogKappaMax))
+// and lobe logit = kLogitScale * raw, so that Adam steps of size lr move
+// log kappa and the logits kKappaScale and kLogitScale times faster.
 constexpr float kLogKappaMin = -3.0f, kLogKappaMax = 7.0f;
+constexpr float kKappaScale = 8.0f, kLogitScale = 4.0f;
 
-// Training records of a path's first Lambert vertex (structure of arrays,
-// indexed by path id).
+// Training records of the Lambert vertices at bounces 0 .. kRecBounces-1 of
+// each path (structure of arrays, index bounce * capacity + path id).
+constexpr uint32_t kRecBounces = 4;
 struct Records {
     float4* in0;   // scalars 0..3
     float4* in1;   // scalars 4..6, pdf of the mixture the direction came from
     float4* dir;   // sampled direction, cos to the normal
     float4* tp;    // throughput after the scatter (before roulette)
     float4* Lb;    // radiance accumulated before the scatter
+    uint32_t capacity;  // path slots
 };
 
 }  // namespace guide

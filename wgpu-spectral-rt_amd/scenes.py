@@ -169,15 +169,16 @@ def _clutter_mesh(n_boxes: int, seed: int, region):
 OCCLUDED_BOXES = 174762  # x 12 triangles + 24 room triangles = 2,097,168 triangles
 
 
-def build_occluded_room(b, n_boxes: int = OCCLUDED_BOXES, seed: int = 5):
+def build_occluded_room(b, n_boxes: int = OCCLUDED_BOXES, seed: int = 5, light_kind: str = "ceiling"):
     """Near room z in [0, 2], far room z in [-2, 0], x in [-2, 2], y in [0, 2].
     The partition at z = 0 is two 0.05-thick slabs leaving a door gap
-    x in [-0.025, 0.025] over the full height. A 0.5 x 0.5 blackbody
-    (6500 K, intensity 30) quad on the far room's ceiling faces down."""
+    x in [-0.025, 0.025] over the full height. The far room's ceiling is a
+    3.8 x 1.8 blackbody (6500 K, intensity 1) panel facing down, so the near
+    room is lit only through the gap."""
     white = b.add_material([0.75, 0.75, 0.75, 1.0], [0, 0, 0, 0])
     warm = b.add_material([0.7, 0.55, 0.4, 1.0], [0, 0, 0, 0])
     cool = b.add_material([0.35, 0.45, 0.6, 1.0], [0, 0, 0, 0])
-    light = b.add_material([6500.0, 30.0, 0.0, 1.0], [3, 0, 0, 0])
+    light = b.add_material([6500.0, 1.0, 0.0, 1.0], [3, 0, 0, 0])
     plane = b.add_mesh(*wsrt.create_plane())
     cube = b.add_mesh(*wsrt.create_cube())
     b.add_instance(plane, white, _trs((0, 0, 0), s=(4, 1, 4)))                 # floor
@@ -189,8 +190,13 @@ def build_occluded_room(b, n_boxes: int = OCCLUDED_BOXES, seed: int = 5):
     half = (2.0 - 0.025) / 2.0
     b.add_instance(cube, white, _trs((-0.025 - half, 1.0, 0.0), s=(2 * half, 2.0, 0.05)))
     b.add_instance(cube, white, _trs((0.025 + half, 1.0, 0.0), s=(2 * half, 2.0, 0.05)))
-    b.add_light_instance(plane, light, _trs((0, 1.99, -1.2), "x", PI, (0.5, 1, 0.5)),
-                         [0.0, 6500.0, 30.0, 0.0])
+    if light_kind == "ceiling":
+        b.add_light_instance(plane, light, _trs((0, 1.99, -1.0), "x", PI, (3.8, 1, 1.8)),
+                             [0.0, 6500.0, 1.0, 0.0])
+    else:  # a 0.5 x 0.5 panel on the far wall facing the gap
+        lamp = b.add_material([6500.0, 8.0, 0.0, 1.0], [3, 0, 0, 0])
+        b.add_light_instance(plane, lamp, _trs((0, 1.0, -1.98), "x", HALF_PI, (0.5, 1, 0.5)),
+                             [0.0, 6500.0, 8.0, 0.0])
     if n_boxes:
         nf = n_boxes // 2
         va, ia = _clutter_mesh(nf, seed, ((-1.9, 1.9), (-1.9, -0.15)))
