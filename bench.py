@@ -143,12 +143,82 @@ def copy_calibration(mib: int) -> int:
     return 0
 
 
+def time_to_relmse(frames: int = 32, warmup: int = 64, boxes: int | None = None, width: int = 1024) -> dict:
+    """Guided vs plain tracer on the occluded-light room (2,097,182 triangles
+    at the default box count), 1024 x 1024, depth 8.
+
+    relMSE = mean over pixels and rgb of (I - R)^2 / (R^2 + 1e-2). Both
+    estimators are unbiased for the same image, so relMSE after N samples
+    per pixel is C / N with C = N * E[(A - B)^2 / (R^2 + 1e-2)] / 2 for two
+    independent N-spp images A, B of the same tracer (R: a 3N-spp plain image
+    of independent seeds for the denominator). Time-to-relMSE 0.01 is then
+    (C / 0.01) * (seconds per spp). The guided tracer is measured after
+    `warmup` frames of online training, and keeps training while measured."""
+    import numpy as np
+    import scenes
+    import wsrt
+
+    dev = wsrt.Device(0)
+    nb = scenes.OCCLUDED_BOXES if boxes is None else boxes
+    scene = scenes.build_occluded_room(wsrt.SceneBuilder(), n_boxes=nb).build(dev)
+    v = scenes.OCCLUDED_VIEW
+    view = scenes.View(v.position, v.yaw, v.pitch, width, width)
+
+    def frames_of(r, n):
+        """Mean image of the next n frames and the seconds they took."""
+        a0 = r.read_accum()[:, :3].astype(np.float64)
+        r.sync()
+        t = time.perf_counter()
+        r.render_frames(n)
+        r.sync()
+        dt = time.perf_counter() - t
+        a1 = r.read_accum()[:, :3].astype(np.float64)
+        return (a1 - a0) / n, dt
+
+    def renderer(cfg, first):
+        r = wsrt.Renderer(dev, scene, width, width, cfg)
+        r.set_camera(scenes.camera(view, first))
+        r.frame_count = first
+        return r
+
+    out = {"scene": f"occluded-light room, {scene.info.n_triangles} triangles, {width}x{width}, depth 8",
+           "relmse": "mean (I - R)^2 / (R^2 + 1e-2)", "target": 0.01}
+    rp = renderer(scenes.occluded_config(frames_in_flight=4), 1 << 22)
+    den, _ = frames_of(rp, 3 * frames)
+    rp.close()
+    res = {}
+    for name, cfg in (("plain", scenes.occluded_config(frames_in_flight=4)),
+                      ("guided", scenes.guided_config(frames_in_flight=4))):
+        r = renderer(cfg, 0)
+        tw = 0.0
+        if name == "guided":
+            _, tw = frames_of(r, warmup)
+        a, ta = frames_of(r, frames)
+        b, tb = frames_of(r, frames)
+        r.close()
+        R = (den * 3 + a + b) / 5
+        c = frames * float(np.mean((a - b) ** 2 / (R**2 + 1e-2))) / 2
+        spp_s = (ta + tb) / (2 * frames)
+        res[name] = {"C": round(c, 3), "ms_per_spp": round(1000 * spp_s, 3),
+                     "relmse_at_measured_spp": round(c / frames, 4),
+                     "time_to_relmse_s": round(tw + c / 0.01 * spp_s, 2),
+                     "training_frames_before_measuring": warmup if name == "guided" else 0}
+    out.update(res)
+    out["speedup_guided_vs_plain"] = round(res["plain"]["time_to_relmse_s"] / res["guided"]["time_to_relmse_s"], 3)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-guided", action="store_true", help="skip the guided-vs-plain time-to-relMSE leg")
+    ap.add_argument("--guided-only", action="store_true", help="only the guided-vs-plain leg (one JSON line)")
+    ap.add_argument("--guided-frames", type=int, default=32)
+    ap.add_argument("--guided-boxes", type=int, default=None)
+    ap.add_argument("--guided-width", type=int, default=1024)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (the reference point that scales the "
@@ -165,6 +235,10 @@ def main() -> int:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the render loop runs "
               "on a single MI355X; multi-GPU sharding is a design note only)", file=sys.stderr)
         return 2
+
+    if args.guided_only:
+        print(json.dumps(time_to_relmse(args.guided_frames, boxes=args.guided_boxes, width=args.guided_width)))
+        return 0
 
     import scenes
     import wsrt
@@ -270,6 +344,11 @@ def main() -> int:
         "kernels": kern,
         "traversal_per_ray": per_ray,
     }
+    if not args.no_guided:
+        log("guided vs plain time-to-relMSE ...")
+        r.close()
+        result["time_to_relmse"] = time_to_relmse(args.guided_frames, boxes=args.guided_boxes,
+                                                  width=args.guided_width)
     if not args.no_cpu:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

@@ -206,7 +206,7 @@ def build_occluded_room(b, n_boxes: int = OCCLUDED_BOXES, seed: int = 5, light_k
     return b
 
 
-OCCLUDED_VIEW = View((0.0, 1.0, 1.9), -HALF_PI, -0.15, 1024, 1024)
+OCCLUDED_VIEW = View((0.0, 1.0, 1.9), -HALF_PI, -0.6, 1024, 1024)
 
 
 def occluded_config(**kw):
