@@ -269,6 +269,10 @@ int wsrt_renderer_read_accum(wsrt_renderer* r, float* out);
 /* width*height RGBA8, Reinhard-luma tone mapped, no gamma, row 0 = image bottom
  * (shader.wgsl:809-822). */
 int wsrt_renderer_read_rgba8(wsrt_renderer* r, uint8_t* out);
+/* The screenshot image of ScreenshotSaver::process_and_save (screenshot.rs:27-73):
+ * width*height RGBA8, rows top to bottom, sRGB OETF applied to the tone-mapped
+ * RGBA8 channels, alpha 255. Writing the PNG file is left to the caller. */
+int wsrt_renderer_read_screenshot(wsrt_renderer* r, uint8_t* out);
 /* Accumulation state export/import (float4 per pixel) for resume. */
 int wsrt_renderer_write_accum(wsrt_renderer* r, const float* in);
 

@@ -133,3 +133,11 @@ def test_cornell_builder_matches_oracle_scene():
     w = np.stack([((Mm[0] * q[0] + Mm[1] * q[1]) + Mm[2] * q[2]) + Mm[3] * np.float32(1) for q in p])
     np.testing.assert_array_equal(w[:, :3], np.stack([L["v0"][0, :3], L["v1"][0, :3], L["v2"][0, :3]]))
     assert lib is not None
+
+
+def test_png_round_trip(tmp_path):
+    img = np.random.default_rng(0).integers(0, 256, (7, 5, 4), dtype=np.uint8)
+    p = str(tmp_path / "x.png")
+    wsrt.save_png(p, img)
+    np.testing.assert_array_equal(wsrt.load_png(p), img)
+    assert open(p, "rb").read()[:8] == b"\x89PNG\r\n\x1a\n"

@@ -38,6 +38,11 @@ def setup(device, name, W, H, photons_per_pass):
         o = O.cornell_box()
         cfg = wsrt.default_config(photons_per_pass=photons_per_pass)
         view = scenes.View((0.0, 0.0, 3.0), scenes.HALF_PI * -1, 0.0, W, H)
+    elif name == "custom":
+        g = scenes.build_custom(wsrt.SceneBuilder()).build(device)
+        o = scenes.build_custom(O.Scene()).build()
+        cfg = wsrt.default_config(photons_per_pass=photons_per_pass)
+        view = scenes.View(scenes.DEFAULT_VIEW.position, scenes.DEFAULT_VIEW.yaw, scenes.DEFAULT_VIEW.pitch, W, H)
     else:
         g = scenes.build_c2_caustic(wsrt.SceneBuilder()).build(device)
         o = scenes.build_c2_caustic(O.Scene()).build()
@@ -50,7 +55,7 @@ def rel_close(a, b, rel, floor):
     return np.all(np.abs(a - b) <= rel * np.maximum(np.abs(a), np.abs(b)) + floor, axis=-1)
 
 
-@pytest.mark.parametrize("name,frame", [("cornell", 0), ("cornell", 5), ("c2", 2)])
+@pytest.mark.parametrize("name,frame", [("cornell", 0), ("cornell", 5), ("c2", 2), ("custom", 3)])
 def test_photon_pass_and_grid(device, name, frame):
     print(f"photons {name} frame {frame}", flush=True)
     g, o, cfg, view = setup(device, name, 64, 64, 1 << 16)
@@ -77,7 +82,7 @@ def test_photon_pass_and_grid(device, name, frame):
 
 
 @pytest.mark.parametrize("name,frame,photons", [("cornell", 0, 1 << 16), ("cornell", 9, 1 << 16),
-                                                ("c2", 1, 1 << 18)])
+                                                ("c2", 1, 1 << 18), ("custom", 4, 1 << 17)])
 def test_camera_paths_match_oracle(device, name, frame, photons):
     print(f"paths {name} frame {frame}", flush=True)
     W = H = 96
@@ -98,7 +103,7 @@ def test_camera_paths_match_oracle(device, name, frame, photons):
     assert np.abs(oL).sum() > 0
 
 
-@pytest.mark.parametrize("name", ["cornell", "c2"])
+@pytest.mark.parametrize("name", ["cornell", "c2", "custom"])
 def test_accumulated_image_matches_oracle(device, name):
     print(f"image {name}", flush=True)
     W = H = 64
@@ -202,3 +207,25 @@ def test_frame_graph_replays_match_eager(device, name):
     rg.render()
     re.render()
     np.testing.assert_array_equal(rg.read_accum(), re.read_accum())
+
+
+def srgb8(c):
+    """screenshot.rs:52-61 restated: u8 -> sRGB OETF -> u8."""
+    cf = np.asarray(c, np.float32) / np.float32(255.0)
+    v = np.where(cf <= 0.0031308, cf * np.float32(12.92),
+                 np.float32(1.055) * np.power(cf, np.float32(1.0 / 2.4)) - np.float32(0.055))
+    return np.round(np.clip(v, 0, 1) * 255).astype(np.uint8)
+
+
+def test_screenshot_and_png(device, cornell_pair, tmp_path):
+    g, _ = cornell_pair
+    r = wsrt.Renderer(device, g, 40, 24, wsrt.default_config(photons_per_pass=1 << 12))
+    r.render_frames(4)
+    lin = r.read_rgba8().reshape(24, 40, 4)
+    shot = r.screenshot()
+    want = srgb8(lin[::-1])
+    want[..., 3] = 255
+    np.testing.assert_array_equal(shot, want)
+    path = str(tmp_path / "shot.png")
+    r.save_png(path)
+    np.testing.assert_array_equal(wsrt.load_png(path), shot)

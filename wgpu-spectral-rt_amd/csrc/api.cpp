@@ -1395,6 +1395,33 @@ extern "C" int wsrt_renderer_read_rgba8(wsrt_renderer* r, uint8_t* out) {
     CHECK_ARG(r && out);
     return read_back(r, out, r->rgba.p, 4ull * r->W * r->H);
 }
+// screenshot.rs:47-73: rows reversed (row 0 = image top), each channel
+// c / 255 through the sRGB OETF, rounded back to 8 bits, alpha 255.
+extern "C" int wsrt_renderer_read_screenshot(wsrt_renderer* r, uint8_t* out) {
+    CHECK_ARG(r && out);
+    const size_t row = 4ull * r->W;
+    std::vector<uint8_t> lin(row * r->H);
+    int rc = read_back(r, lin.data(), r->rgba.p, lin.size());
+    if (rc) return rc;
+    uint8_t lut[256];
+    for (int c = 0; c < 256; ++c) {
+        const float cf = (float)c / 255.0f;
+        const float v = cf <= 0.0031308f ? cf * 12.92f : 1.055f * powf(cf, 1.0f / 2.4f) - 0.055f;
+        lut[c] = (uint8_t)roundf(std::min(std::max(v, 0.0f), 1.0f) * 255.0f);
+    }
+    for (uint32_t y = 0; y < r->H; ++y) {
+        const uint8_t* src = lin.data() + (size_t)(r->H - 1 - y) * row;
+        uint8_t* dst = out + (size_t)y * row;
+        for (uint32_t x = 0; x < r->W; ++x) {
+            dst[4 * x + 0] = lut[src[4 * x + 0]];
+            dst[4 * x + 1] = lut[src[4 * x + 1]];
+            dst[4 * x + 2] = lut[src[4 * x + 2]];
+            dst[4 * x + 3] = 255;
+        }
+    }
+    return WSRT_OK;
+}
+
 extern "C" int wsrt_renderer_write_accum(wsrt_renderer* r, const float* in) {
     CHECK_ARG(r && in);
     HIP_OR_FAIL(hipSetDevice(r->dev->ordinal));

@@ -10,6 +10,9 @@ drive any builder with that interface.
   c2_caustic    glass icosphere (subdivision 4, radius 0.5, Cauchy A = 1.5,
                 B = 0.0042 um^2) on a 2-triangle diffuse plane, point light
                 2 units above the sphere centre
+  custom        the reference's second scene (src/scene/custom.rs): a dark
+                room, a vertical slit light, a high-dispersion glass prism
+                (Cauchy B = 0.12), a thin-film bubble and a white board
   occluded_room two rooms split by a wall with a 0.05-wide full-height door
                 gap; the only light is on the ceiling of the far room, so the
                 near room is lit through the gap; both rooms are cluttered
@@ -224,3 +227,31 @@ def guided_config(**kw):
     for k, v in kw.items():
         setattr(c, k, v)
     return c
+
+
+def build_custom(b):
+    """create_custom_scene (src/scene/custom.rs:6-116) through the builder API:
+    same materials, meshes and instance transforms (translation * rotation *
+    scale, glam order)."""
+    light = b.add_material([6500.0, 5.0, 0.0, 1.0], [3, 0, 0, 0])
+    wall = b.add_material([0.01, 0.01, 0.01, 1.0], [0, 0, 0, 0])
+    board = b.add_material([0.95, 0.95, 0.95, 1.0], [0, 0, 0, 0])
+    prism = b.add_material([1.0, 1.0, 1.0, 1.0], [2, 0.0, 1.6, 0.12])
+    bubble = b.add_material([1.0, 1.0, 1.0, 1.0], [4, 500.0, 1.33, 1.0])
+    plane = b.add_mesh(*wsrt.create_plane())
+    cube = b.add_mesh(*wsrt.create_cube())
+    sphere = b.add_mesh(*wsrt.create_sphere(4))
+    room = 4.0
+    b.add_instance(plane, wall, _trs((0, -1, 0), s=(room * 2, room * 2, room * 2)))
+    b.add_instance(plane, wall, _trs((0, room - 1.0, 0), "x", PI, (room * 2, room * 2, room * 2)))
+    b.add_instance(plane, wall, _trs((0, room * 0.5 - 1.0, -room), "x", HALF_PI, (room * 2, room * 2, room * 2)))
+    b.add_instance(cube, board, _trs((0.5, -0.55, 0.0), s=(5.0, 0.1, 3.0)))
+    tilt = float(np.float32(-np.float32(math.pi / 2) + np.float32(0.02)))
+    b.add_light_instance(plane, light, _trs((-2.0, -0.2, 0.0), "z", tilt, (0.5, 1.0, 0.02)),
+                         [0.0, 6500.0, 5.0, 0.0])
+    b.add_instance(cube, prism, _trs((-1.1, -0.2, 0.0), "y", 0.7, (0.5, 0.8, 0.5)))
+    b.add_instance(sphere, bubble, _trs((1.0, -0.1, -0.5), s=(0.6, 0.6, 0.6)))
+    return b
+
+
+DEFAULT_VIEW = View((0.0, 0.0, 3.0), -HALF_PI, 0.0, 256, 256)  # CameraController::new (camera.rs:35-52)

@@ -111,6 +111,7 @@ _SIGS = {
     "wsrt_renderer_read_accum": (C.c_int, [vp, vp]),
     "wsrt_renderer_read_rgba8": (C.c_int, [vp, vp]),
     "wsrt_renderer_write_accum": (C.c_int, [vp, vp]),
+    "wsrt_renderer_read_screenshot": (C.c_int, [vp, vp]),
     "wsrt_renderer_get_stats": (C.c_int, [vp, P(FrameStats)]),
     "wsrt_renderer_reset_stats": (C.c_int, [vp]),
     "wsrt_renderer_set_traversal_counting": (C.c_int, [vp, C.c_int]),
@@ -524,6 +525,16 @@ class Renderer:
         acc = np.ascontiguousarray(acc, np.float32)
         _check(load().wsrt_renderer_write_accum(self.h, _p(acc)))
 
+    def screenshot(self) -> np.ndarray:
+        """[H, W, 4] uint8: the screenshot image (screenshot.rs:47-73)."""
+        out = np.zeros((self.target_height, self.target_width, 4), np.uint8)
+        _check(load().wsrt_renderer_read_screenshot(self.h, _p(out)))
+        return out
+
+    def save_png(self, path: str):
+        """Writes the screenshot image as an RGBA8 PNG."""
+        save_png(path, self.screenshot())
+
     def read_rgba8(self) -> np.ndarray:
         out = np.zeros((self.target_height * self.target_width, 4), np.uint8)
         _check(load().wsrt_renderer_read_rgba8(self.h, _p(out)))
@@ -595,3 +606,43 @@ def photon_grid_build(device: Device, photons: np.ndarray, frame_count: int, cel
                                          _p(cell), _p(count), _p(offset), _p(slot)))
     valid = int(count.sum())
     return cell[:n], count, offset, slot[:valid]
+
+
+def save_png(path: str, rgba: np.ndarray):
+    """RGBA8 PNG writer (zlib-compressed scanlines, filter 0), [H, W, 4] uint8."""
+    import struct
+    import zlib
+
+    rgba = np.ascontiguousarray(rgba, np.uint8)
+    h, w = rgba.shape[:2]
+
+    def chunk(tag: bytes, data: bytes) -> bytes:
+        return struct.pack(">I", len(data)) + tag + data + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF)
+
+    raw = b"".join(b"\x00" + rgba[y].tobytes() for y in range(h))
+    png = (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 6, 0, 0, 0)) +
+           chunk(b"IDAT", zlib.compress(raw, 1)) + chunk(b"IEND", b""))
+    with open(path, "wb") as f:
+        f.write(png)
+
+
+def load_png(path: str) -> np.ndarray:
+    """Reads back an 8-bit RGBA PNG with filter type 0 (what save_png writes)."""
+    import struct
+    import zlib
+
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat, w = 8, b"", 0
+    while pos < len(data):
+        n = struct.unpack(">I", data[pos:pos + 4])[0]
+        tag, body = data[pos + 4:pos + 8], data[pos + 8:pos + 8 + n]
+        assert zlib.crc32(tag + body) & 0xFFFFFFFF == struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])[0]
+        if tag == b"IHDR":
+            w, h = struct.unpack(">II", body[:8])
+        elif tag == b"IDAT":
+            idat += body
+        pos += 12 + n
+    raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(h, 1 + 4 * w)
+    assert np.all(raw[:, 0] == 0)
+    return raw[:, 1:].reshape(h, w, 4).copy()
