@@ -217,6 +217,7 @@ def main() -> int:
     ap.add_argument("--no-guided", action="store_true", help="skip the guided-vs-plain time-to-relMSE leg")
     ap.add_argument("--guided-only", action="store_true", help="only the guided-vs-plain leg (one JSON line)")
     ap.add_argument("--guided-frames", type=int, default=32)
+    ap.add_argument("--frames-in-flight", type=int, default=0, help="frames per wavefront (default: config's)")
     ap.add_argument("--guided-boxes", type=int, default=None)
     ap.add_argument("--guided-width", type=int, default=1024)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -248,6 +249,14 @@ def main() -> int:
     info = scene.info
     v = scenes.C2_VIEW
     cfg = scenes.c2_config()
+    tune = None
+    if args.frames_in_flight:
+        cfg.frames_in_flight = args.frames_in_flight
+    else:
+        # frames per wavefront picked by the library's autotuner (results do
+        # not depend on it; the timed frames are the same 64)
+        tune = wsrt.autotune(dev, scene, v.width, v.height, cfg)
+        log(f"autotune: {tune}")
     r = wsrt.Renderer(dev, scene, v.width, v.height, cfg)
     r.set_camera(scenes.camera(v, 0))
     r.render_frames(args.warmup)
@@ -266,7 +275,9 @@ def main() -> int:
     r.set_profiling(False)
     st = r.stats()
     prof = r.profile()
-    # the same number of frames replayed as captured batch graphs (no events)
+    # the same number of frames replayed as captured batch graphs (no events);
+    # one untimed batch first captures the graphs
+    r.render_frames(cfg.frames_in_flight)
     r.sync()
     t1 = time.perf_counter()
     r.render_frames(args.steps)
@@ -328,7 +339,9 @@ def main() -> int:
             "triangles": info.n_triangles, "bvh8_nodes": info.n_nodes,
             "width": v.width, "height": v.height, "photons_per_pass": cfg.photons_per_pass,
             "hash_size": cfg.hash_size, "max_depth": cfg.max_depth,
+            "frames_in_flight": cfg.frames_in_flight,
         },
+        "autotune": tune,
         "graph_replay_ms_per_step": round(1000 * dt_graph / args.steps, 4),
         "rays_per_s": round(rays / dt, 1),
         "photons_per_s": round(st["photon_paths"] / dt, 1),

@@ -229,3 +229,31 @@ def test_screenshot_and_png(device, cornell_pair, tmp_path):
     path = str(tmp_path / "shot.png")
     r.save_png(path)
     np.testing.assert_array_equal(wsrt.load_png(path), shot)
+
+
+@pytest.mark.parametrize("name", ["cornell", "c2"])
+def test_frames_in_flight_do_not_change_results(device, name):
+    """Frames traced together in one wavefront accumulate bit for bit like
+    frames traced one at a time."""
+    print(f"frames in flight {name}", flush=True)
+    W = H = 40
+    g, _, cfg, view = setup(device, name, W, H, 1 << 14)
+    accs = []
+    for S in (1, 3, 4):
+        c = wsrt.Config.from_buffer_copy(bytes(cfg))
+        c.frames_in_flight = S
+        r = wsrt.Renderer(device, g, W, H, c)
+        r.set_camera(scenes.camera(view, 0))
+        r.render_frames(6)
+        accs.append(r.read_accum())
+    np.testing.assert_array_equal(accs[0], accs[1])
+    np.testing.assert_array_equal(accs[0], accs[2])
+
+
+def test_autotune_picks_a_measured_candidate(device, cornell_pair):
+    g, _ = cornell_pair
+    cfg = wsrt.default_config(photons_per_pass=1 << 14)
+    res = wsrt.autotune(device, g, 64, 64, cfg)
+    ms = {c["frames_in_flight"]: c["ms_per_frame"] for c in res["candidates"]}
+    assert cfg.frames_in_flight == res["best"] and ms[res["best"]] > 0
+    assert ms[res["best"]] <= min(v for v in ms.values() if v > 0) * 1.02 + 1e-9

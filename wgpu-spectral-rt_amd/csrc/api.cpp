@@ -923,7 +923,7 @@ struct wsrt_renderer {
         shadow;
     DevBuf accum, rgba, counters, gstack, qcounts;
     DevBuf ftable, dframe;           // per-frame constants, device frame counter
-    DevBuf ph_pos, ph_dir, ph_rng, ph_flags, ph_qa, ph_qb;
+    DevBuf ph_pos, ph_dir, ph_rng, ph_flags, ph_qa, ph_qb, ph_pid;
     PhotonState phs{};
     FrameConst* staging = nullptr;   // pinned, 2 halves of kUploadHalf entries
     hipEvent_t upload_ev[2] = {nullptr, nullptr};
@@ -1034,11 +1034,13 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     HIP_OR_FAIL(r->ph_dir.alloc(16 * nph));
     HIP_OR_FAIL(r->ph_rng.alloc(4 * nph));
     HIP_OR_FAIL(r->ph_flags.alloc(4 * nph));
+    HIP_OR_FAIL(r->ph_pid.alloc(4 * nph));
     const uint32_t pseg = seg_capacity(nph);
     HIP_OR_FAIL(r->ph_qa.alloc(4ull * kSegs * pseg));
     HIP_OR_FAIL(r->ph_qb.alloc(4ull * kSegs * pseg));
     r->phs = PhotonState{r->ph_pos.as<float4>(), r->ph_dir.as<float4>(), r->ph_rng.as<uint32_t>(),
-                         r->ph_flags.as<uint32_t>(), r->ph_qa.as<uint32_t>(), r->ph_qb.as<uint32_t>()};
+                         r->ph_flags.as<uint32_t>(), r->ph_qa.as<uint32_t>(), r->ph_qb.as<uint32_t>(),
+                         r->ph_pid.as<uint32_t>()};
     HIP_OR_FAIL(r->path_o.alloc(16 * npaths));
     HIP_OR_FAIL(r->path_d.alloc(16 * npaths));
     HIP_OR_FAIL(r->path_tp.alloc(16 * npaths));
@@ -1841,5 +1843,66 @@ extern "C" int wsrt_renderer_get_guide(wsrt_renderer* r, wsrt_guide** out) {
     if (!r->guide) return fail(WSRT_E_STATE, "the renderer was created without guiding");
     HIP_OR_FAIL(hipStreamSynchronize(r->stream));
     *out = r->guide.get();
+    return WSRT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: frames in flight
+// ---------------------------------------------------------------------------
+extern "C" int wsrt_autotune(wsrt_device* dev, const wsrt_scene* scene, uint32_t width, uint32_t height,
+                             wsrt_config* cfg, uint64_t mem_budget, wsrt_autotune_result* out) {
+    CHECK_ARG(dev && scene && cfg && out && width > 0 && height > 0);
+    int rc = validate_config(cfg);
+    if (rc) return rc;
+    HIP_OR_FAIL(hipSetDevice(dev->ordinal));
+    size_t free0 = 0, total = 0;
+    HIP_OR_FAIL(hipMemGetInfo(&free0, &total));
+    if (mem_budget == 0) mem_budget = total / 4;
+    memset(out, 0, sizeof *out);
+    const uint32_t cands[7] = {1, 2, 4, 8, 16, 32, 64};
+    float best_ms = 0.0f;
+    uint32_t best = cfg->frames_in_flight, slower = 0;
+    for (uint32_t k = 0; k < 7; ++k) {
+        const uint32_t S = cands[k];
+        out->frames_in_flight[k] = S;
+        out->n_candidates = k + 1;
+        wsrt_config c = *cfg;
+        c.frames_in_flight = S;
+        if (validate_config(&c) != WSRT_OK) break;
+        size_t fa = 0, fb = 0;
+        (void)hipMemGetInfo(&fa, &total);
+        wsrt_renderer* r = nullptr;
+        if (wsrt_renderer_create(dev, scene, width, height, &c, &r) != WSRT_OK) break;
+        (void)hipMemGetInfo(&fb, &total);
+        out->device_bytes[k] = fa > fb ? (uint64_t)(fa - fb) : 0;
+        bool ok = out->device_bytes[k] <= mem_budget;
+        float ms = 0.0f;
+        if (ok) {
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+            ok = ok && wsrt_renderer_render_frames(r, S) == WSRT_OK;  // first batch captures the graph
+            ok = ok && hipEventRecord(e0, r->stream) == hipSuccess;
+            ok = ok && wsrt_renderer_render_frames(r, 2 * S) == WSRT_OK;
+            ok = ok && hipEventRecord(e1, r->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+            if (ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) ms /= (float)(2 * S);
+            else ok = false;
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        }
+        wsrt_renderer_destroy(r);
+        (void)hipGetLastError();
+        if (!ok) break;
+        out->ms_per_frame[k] = ms;
+        if (best_ms == 0.0f || ms < best_ms * 0.98f) {  // a larger batch must win by 2 %
+            best_ms = ms;
+            best = S;
+            slower = 0;
+        } else if (++slower == 2) {
+            break;
+        }
+    }
+    if (best_ms == 0.0f) return fail(WSRT_E_OOM, "no frames_in_flight candidate fits the memory budget");
+    out->best = best;
+    cfg->frames_in_flight = best;
     return WSRT_OK;
 }

@@ -92,7 +92,8 @@ struct PathState {
     uint32_t capacity;  // path slots (frames in flight x pixels)
 };
 
-// Photon paths between bounces (structure of arrays indexed by photon slot).
+// Photon paths between bounces (structure of arrays indexed by state slot;
+// slot s carries photon path pid[s]).
 struct PhotonState {
     float4* pos;        // origin xyz, wavelength
     float4* dir;        // direction xyz, energy
@@ -100,6 +101,7 @@ struct PhotonState {
     uint32_t* flags;    // 1: caustic path
     uint32_t* queue_a;
     uint32_t* queue_b;
+    uint32_t* pid;      // photon path id of a state slot
 };
 
 // Photon grid (counting sort view of build_grid.wgsl).

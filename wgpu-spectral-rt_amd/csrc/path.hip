@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_extend(SceneView sv, PathStat
     __shared__ uint32_t pre[kSegs + 1];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;
-    st.lds = lds_stack + threadIdx.x;
+    st.lds = lds_slot(lds_stack + threadIdx.x);
     st.lds_stride = kTraceThreads;
     st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
     const uint32_t n = bounce == 0 ? n0 : seg_prefix(qs, bounce, kQActive, 1, pre);
@@ -111,8 +111,16 @@ __global__ __launch_bounds__(kTraceThreads) void k_extend(SceneView sv, PathStat
     const uint32_t stride = gridDim.x * blockDim.x;
     const bool count = count_traversal != 0u;
     uint32_t my_rays = 0, my_nodes = 0, my_tris = 0;
-    // wave-uniform trip count, so the binning ballots see the whole wave
-    for (uint32_t base = gtid & ~63u; base < n; base += stride) {
+    // wave-uniform trip count, so the binning ballots see the whole wave.
+    // Bounce 0 takes path ids in pixel order; a plain grid stride would give
+    // a wave the same pixel chunk of every frame in flight, so wave w takes
+    // chunk (w + 977 k) mod W of round k instead (W waves), which spreads
+    // expensive image regions over all waves.
+    const uint32_t nw = stride >> 6, wave = gtid >> 6;
+    for (uint32_t k = 0, base0 = 0; base0 < n; ++k, base0 += stride) {
+        const uint32_t chunk = bounce == 0 ? (wave + 977u * k) % nw : wave;
+        const uint32_t base = base0 + (chunk << 6);
+        if (base >= n) continue;
         const uint32_t i = base + __lane_id();
         bool have = i < n;
         uint32_t p = 0, bin = 0;
@@ -282,7 +290,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_shadow(SceneView sv, PathStat
     __shared__ uint32_t pre[kSegs + 1];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;
-    st.lds = lds_stack + threadIdx.x;
+    st.lds = lds_slot(lds_stack + threadIdx.x);
     st.lds_stride = kTraceThreads;
     st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
     const uint32_t n = seg_prefix(qs, bounce, kQShadow, 1, pre);

@@ -16,15 +16,45 @@ namespace wsrt {
 
 namespace {
 
+// Octahedral bin (8 x 8, +Y pole) of a unit direction.
+__device__ __forceinline__ uint32_t direction_bin(f3 d) {
+    const float l1 = fabsf(d.x) + fabsf(d.y) + fabsf(d.z);
+    float px = d.x / l1, pz = d.z / l1;
+    if (d.y < 0.0f) {
+        const float qx = (1.0f - fabsf(pz)) * (px < 0.0f ? -1.0f : 1.0f);
+        const float qz = (1.0f - fabsf(px)) * (pz < 0.0f ? -1.0f : 1.0f);
+        px = qx;
+        pz = qz;
+    }
+    const uint32_t bx = (uint32_t)fminf(fmaxf((px * 0.5f + 0.5f) * 8.0f, 0.0f), 7.0f);
+    const uint32_t bz = (uint32_t)fminf(fmaxf((pz * 0.5f + 0.5f) * 8.0f, 0.0f), 7.0f);
+    return bx * 8u + bz;
+}
+
+constexpr int kEmitThreads = 1024;
+
 // Emission of photon path i (photon_emit.wgsl:159-216): light choice, point
 // on the light, cosine direction, hero wavelength and flux. Every slot is
-// cleared; with lights present every path takes part in the first bounce,
-// which reads photon slots directly (no queue).
-__global__ __launch_bounds__(256) void k_photon_emit(SceneView sv, FrameArgs fa, uint32_t np,
-                                                     uint32_t nf, PhotonState ps,
-                                                     wsrt_photon* __restrict__ photons) {
+// cleared; with lights present every path takes part in the first bounce.
+// Each block also
+// orders its paths by emission-direction bin (a counting sort in LDS) and
+// stores their state in that order, with the path id beside it, so the waves
+// of the first bounce trace neighbouring directions from contiguous state: a
+// path's result depends only on its id, so the order in which the bounce
+// kernel takes paths up changes nothing but divergence.
+__global__ __launch_bounds__(kEmitThreads) void k_photon_emit(SceneView sv, FrameArgs fa, uint32_t np,
+                                                              uint32_t nf, PhotonState ps,
+                                                              wsrt_photon* __restrict__ photons) {
+    __shared__ uint32_t hist[64], cursor[64];
+    __shared__ float4 s_pos[kEmitThreads], s_dir[kEmitThreads];
+    __shared__ uint32_t s_rng[kEmitThreads], s_id[kEmitThreads];
+    if (threadIdx.x < 64) hist[threadIdx.x] = 0u;
+    __syncthreads();
+    float4 my_pos = splat4(0.0f), my_dir = splat4(0.0f);
+    uint32_t my_rng = 0u;
     const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = np * nf;
+    uint32_t key = 0u;
     if (id < n) {
         const uint32_t base_frame = *fa.frame;
         const FrameConst& f0 = fa.table[base_frame & fa.mask];
@@ -65,11 +95,42 @@ __global__ __launch_bounds__(256) void k_photon_emit(SceneView sv, FrameArgs fa,
             float flux = (radiance / pdf) * area * kPi;
             float energy = flux * prm.z;
             f3 pos = emit_pos + emit_dir * 0.001f;
-            ps.pos[id] = f4(pos, wl);
-            ps.dir[id] = f4(emit_dir, energy);
-            ps.rng[id] = rng;
-            ps.flags[id] = 0u;
+            my_pos = f4(pos, wl);
+            my_dir = f4(emit_dir, energy);
+            my_rng = rng;
+            key = direction_bin(emit_dir);
         }
+        atomicAdd(&hist[key], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const uint32_t v = hist[threadIdx.x];
+        uint32_t inc = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t t = __shfl_up(inc, off, 64);
+            if ((int)threadIdx.x >= off) inc += t;
+        }
+        cursor[threadIdx.x] = inc - v;
+    }
+    __syncthreads();
+    // the block's paths in bin order, staged in LDS and stored contiguously:
+    // state slot base + t holds photon path s_id[t]
+    if (id < n) {
+        const uint32_t rank = atomicAdd(&cursor[key], 1u);
+        s_pos[rank] = my_pos;
+        s_dir[rank] = my_dir;
+        s_rng[rank] = my_rng;
+        s_id[rank] = id;
+    }
+    __syncthreads();
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot < n) {
+        ps.pos[slot] = s_pos[threadIdx.x];
+        ps.dir[slot] = s_dir[threadIdx.x];
+        ps.rng[slot] = s_rng[threadIdx.x];
+        ps.flags[slot] = 0u;
+        ps.pid[slot] = s_id[threadIdx.x];
     }
 }
 
@@ -91,24 +152,35 @@ __global__ __launch_bounds__(kTraceThreads) void k_photon_bounce(SceneView sv, u
     __shared__ unsigned long long red[kTraceThreads / 64];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;
-    st.lds = lds_stack + threadIdx.x;
+    st.lds = lds_slot(lds_stack + threadIdx.x);
     st.lds_stride = kTraceThreads;
     st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
     const uint32_t n = depth == 0 ? n0 : seg_prefix(qs, depth, kQPhoton, 1, pre);
     const uint32_t stride = gridDim.x * blockDim.x;
     const bool count = count_traversal != 0u;
     uint32_t my_rays = 0, my_stored = 0, my_nodes = 0, my_tris = 0;
-    for (uint32_t base = gtid & ~63u; base < n; base += stride) {
+    // Depth 0 reads state slots in bin order within each emit block. A wave
+    // takes a contiguous run of 64-path chunks there (so every wave sees all
+    // bins, and the paths of one direction bin share waves); later depths
+    // read the compacted queue grid-stride.
+    const uint32_t nchunks = (n + 63u) >> 6;
+    const uint32_t per_wave = depth == 0 ? (nchunks + (stride >> 6) - 1u) / (stride >> 6) : 0u;
+    const uint32_t wave = gtid >> 6;
+    uint32_t base = depth == 0 ? wave * per_wave * 64u : (gtid & ~63u);
+    const uint32_t end = depth == 0 ? min(n, (wave + 1u) * per_wave * 64u) : n;
+    const uint32_t step = depth == 0 ? 64u : stride;
+    for (; base < end; base += step) {
         const uint32_t i = base + __lane_id();
         bool alive = false;
-        uint32_t id = 0;
-        if (i < n) {
-            id = depth == 0 ? i : queue[seg_entry(pre, kSegs, i, qs.pseg_cap)];
-            const float4 pw = ps.pos[id], de = ps.dir[id];
+        uint32_t sid = 0;  // state slot; the photon path id is ps.pid[sid]
+        if (i < end) {
+            sid = depth == 0 ? i : queue[seg_entry(pre, kSegs, i, qs.pseg_cap)];
+            const uint32_t id = ps.pid[sid];
+            const float4 pw = ps.pos[sid], de = ps.dir[sid];
             f3 pos = xyz(pw), dir = xyz(de);
             const float wl = pw.w, energy = de.w;
-            uint32_t rng = ps.rng[id];
-            bool caustic = ps.flags[id] != 0u;
+            uint32_t rng = ps.rng[sid];
+            bool caustic = ps.flags[sid] != 0u;
             Hit h = trace_ray<false>(sv, pos, dir, 0.0001f, 100.0f, st, count, my_nodes, my_tris);
             my_rays++;
             if (h.tri != kInvalid) {
@@ -168,15 +240,15 @@ __global__ __launch_bounds__(kTraceThreads) void k_photon_bounce(SceneView sv, u
                     // the origin is not advanced in this branch (photon_emit.wgsl:291-326)
                 }
                 if (alive) {
-                    ps.pos[id] = f4(pos, wl);
-                    ps.dir[id] = f4(dir, energy);
-                    ps.rng[id] = rng;
-                    ps.flags[id] = caustic ? 1u : 0u;
+                    ps.pos[sid] = f4(pos, wl);
+                    ps.dir[sid] = f4(dir, energy);
+                    ps.rng[sid] = rng;
+                    ps.flags[sid] = caustic ? 1u : 0u;
                 }
             }
         }
         const uint32_t e = seg_append(qs, depth + 1, kQPhoton, qs.pseg_cap, alive);
-        if (alive) next_queue[e] = id;
+        if (alive) next_queue[e] = sid;
     }
     if (count) flush_traversal_counts(ctr, 0, my_rays, my_nodes, my_tris);
     // per-block statistics, summed on the host
@@ -253,7 +325,8 @@ void launch_photon_trace(const SceneView& sv, const FrameArgs& fa, uint32_t np, 
                          const Queues& qs, wsrt_photon* photons, Counters* ctr, uint2* gstack,
                          hipStream_t s) {
     const uint32_t n = np * nf;
-    hipLaunchKernelGGL(k_photon_emit, dim3((n + 255) / 256), dim3(256), 0, s, sv, fa, np, nf, ps, photons);
+    hipLaunchKernelGGL(k_photon_emit, dim3((n + kEmitThreads - 1) / kEmitThreads), dim3(kEmitThreads), 0, s, sv,
+                       fa, np, nf, ps, photons);
     const uint32_t n0 = sv.n_lights ? n : 0u;
     for (uint32_t d = 0; d < max_depth; ++d) {
         const uint32_t* q = (d & 1u) ? ps.queue_b : ps.queue_a;

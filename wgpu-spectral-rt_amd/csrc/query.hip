@@ -15,7 +15,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_query(SceneView sv, const flo
     __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;
-    st.lds = lds_stack + threadIdx.x;
+    st.lds = lds_slot(lds_stack + threadIdx.x);
     st.lds_stride = kTraceThreads;
     st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
     const uint32_t stride = gridDim.x * blockDim.x;

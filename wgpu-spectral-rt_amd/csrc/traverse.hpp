@@ -55,19 +55,36 @@ __device__ __forceinline__ bool tri_intersect(f3 o, f3 d, f3 v0, f3 e1, f3 e2, f
 
 // Traversal stack: entries (node, tnear bits). The first kShortStack entries
 // of a lane live in LDS at lds[k * lds_stride]; deeper entries spill to the
-// lane's private HBM region gstack[0 .. stack_size - kShortStack).
+// lane's private HBM region gstack[0 .. stack_size - kShortStack). The LDS
+// part is addressed as LDS (address space 3), so accesses compile to ds
+// instructions rather than flat ones that would wait on both counters.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) u32x2 lds_entry;
 struct Stack {
-    uint2* lds;
+    lds_entry* lds;
     uint32_t lds_stride;
     uint2* glob;
     __device__ __forceinline__ void put(int k, uint2 e) {
-        if (k < kShortStack) lds[k * lds_stride] = e;
-        else glob[k - kShortStack] = e;
+        if (k < kShortStack) {
+            u32x2 v;
+            v.x = e.x;
+            v.y = e.y;
+            lds[k * lds_stride] = v;
+        } else {
+            glob[k - kShortStack] = e;
+        }
     }
     __device__ __forceinline__ uint2 get(int k) {
-        return k < kShortStack ? lds[k * lds_stride] : glob[k - kShortStack];
+        if (k < kShortStack) {
+            const u32x2 v = lds[k * lds_stride];
+            return make_uint2(v.x, v.y);
+        }
+        return glob[k - kShortStack];
     }
 };
+__device__ __forceinline__ lds_entry* lds_slot(uint2* shared_base) {
+    return (lds_entry*)(shared_base);
+}
 
 __device__ __forceinline__ uint32_t byte_of(uint32_t lo, uint32_t hi, int c) {
     return ((c < 4 ? lo : hi) >> (8 * (c & 3))) & 0xFFu;
@@ -88,12 +105,22 @@ __device__ Hit traverse(const SceneView& sv, f3 o, f3 d, float tmin, float tmax,
     const int oct = (d.x < 0.0f ? 1 : 0) | (d.y < 0.0f ? 2 : 0) | (d.z < 0.0f ? 4 : 0);
     uint32_t nvis = 0, ntest = 0;
     int sp = 0;
-    st.put(sp++, make_uint2(0u, __float_as_uint(tmin)));
+    // the node to visit next is kept in a register: the root first, then the
+    // first internal child in visiting order of each node (it would be
+    // pushed and popped at once, and with best.t unchanged in between its
+    // pop-time cull test could not fail)
+    uint32_t next = 0u;
+    bool have_next = true;
     const uint4* nodes = reinterpret_cast<const uint4*>(sv.nodes);
-    while (sp > 0) {
-        uint2 e = st.get(--sp);
-        if (!kAny && __uint_as_float(e.y) > best.t * kBoxPad) continue;
-        const uint4* np = nodes + 5ull * e.x;
+    while (true) {
+        if (!have_next) {
+            if (sp == 0) break;
+            const uint2 e = st.get(--sp);
+            if (!kAny && __uint_as_float(e.y) > best.t * kBoxPad) continue;
+            next = e.x;
+        }
+        have_next = false;
+        const uint4* np = nodes + 5ull * next;
         uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
         if (kCount) nvis++;
         // per axis: slab scale/offset, and the byte plane of each child's near
@@ -178,14 +205,25 @@ __device__ Hit traverse(const SceneView& sv, f3 o, f3 d, float tmin, float tmax,
             push |= (pc ? 1u : 0u) << c;
             pm |= (pc ? 1u : 0u) << (c ^ oct);
         }
+        if (pm) {
+            // the lowest rank goes to the register, the others onto the stack
+            // with lower ranks nearer the top
+            const int first = (__ffs(pm) - 1) ^ oct;
+            const uint32_t rest = pm & (pm - 1u);
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            if (!((push >> c) & 1u)) continue;
-            const int pos = sp + __popc(pm >> ((c ^ oct) + 1));
-            const uint32_t meta = byte_of(w1.z, w1.w, c);
-            st.put(pos, make_uint2(child_base + (meta & 0x7Fu), __float_as_uint(cnear[c])));
+            for (int c = 0; c < 8; ++c) {
+                if (!((push >> c) & 1u)) continue;
+                const uint32_t meta = byte_of(w1.z, w1.w, c);
+                if (c == first) {
+                    next = child_base + (meta & 0x7Fu);
+                    continue;
+                }
+                const int pos = sp + __popc(rest >> ((c ^ oct) + 1));
+                st.put(pos, make_uint2(child_base + (meta & 0x7Fu), __float_as_uint(cnear[c])));
+            }
+            sp += __popc(rest);
+            have_next = true;
         }
-        sp += __popc(pm);
     }
     if (kCount) {
         *n_nodes_visited = nvis;

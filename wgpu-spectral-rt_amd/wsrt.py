@@ -46,6 +46,11 @@ class Config(C.Structure):
                 ("reserved", C.c_uint32 * 4)]
 
 
+class AutotuneResult(C.Structure):
+    _fields_ = [("n_candidates", C.c_uint32), ("frames_in_flight", C.c_uint32 * 8),
+                ("ms_per_frame", C.c_float * 8), ("device_bytes", C.c_uint64 * 8), ("best", C.c_uint32)]
+
+
 class SceneInfo(C.Structure):
     _fields_ = [("n_triangles", C.c_uint32), ("n_instances", C.c_uint32),
                 ("n_lights", C.c_uint32), ("n_materials", C.c_uint32), ("n_nodes", C.c_uint32),
@@ -136,6 +141,7 @@ _SIGS = {
     "wsrt_guide_gradients": (C.c_int, [vp, vp, vp, vp, u32, vp]),
     "wsrt_guide_train_step": (C.c_int, [vp, vp, vp, vp, u32, C.c_float]),
     "wsrt_renderer_get_guide": (C.c_int, [vp, P(vp)]),
+    "wsrt_autotune": (C.c_int, [vp, vp, u32, u32, P(Config), C.c_uint64, P(AutotuneResult)]),
 }
 
 GUIDE_PARAMS = 27952
@@ -646,3 +652,16 @@ def load_png(path: str) -> np.ndarray:
     raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(h, 1 + 4 * w)
     assert np.all(raw[:, 0] == 0)
     return raw[:, 1:].reshape(h, w, 4).copy()
+
+
+def autotune(device: "Device", scene: "Scene", width: int, height: int, config: Config,
+             mem_budget: int = 0) -> dict:
+    """Picks config.frames_in_flight by timing candidates (wsrt_autotune);
+    updates config in place and returns the timings."""
+    res = AutotuneResult()
+    _check(load().wsrt_autotune(device.h, scene.h, width, height, C.byref(config), mem_budget, C.byref(res)))
+    n = res.n_candidates
+    return {"best": res.best,
+            "candidates": [{"frames_in_flight": res.frames_in_flight[i],
+                            "ms_per_frame": round(res.ms_per_frame[i], 4),
+                            "device_mib": round(res.device_bytes[i] / 2**20, 1)} for i in range(n)]}
