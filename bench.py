@@ -255,7 +255,7 @@ def main() -> int:
     else:
         # frames per wavefront picked by the library's autotuner (results do
         # not depend on it; the timed frames are the same 64)
-        tune = wsrt.autotune(dev, scene, v.width, v.height, cfg)
+        tune = wsrt.autotune(dev, scene, v.width, v.height, cfg, scenes.camera(v, 0))
         log(f"autotune: {tune}")
     r = wsrt.Renderer(dev, scene, v.width, v.height, cfg)
     r.set_camera(scenes.camera(v, 0))

@@ -356,7 +356,8 @@ int wsrt_debug_read_world(const wsrt_scene* s, float* tris, uint32_t* ids);
 /* ---- autotuner (no reference counterpart: the reference renders one frame
  *      per submission) -------------------------------------------------------
  * Times candidate frames_in_flight values (1, 2, 4, .. 64: frames traced in
- * one wavefront, one graph per batch) for a scene, resolution and config by
+ * one wavefront, one graph per batch) for a scene, camera (null: the default
+ * camera), resolution and config by
  * rendering 2 warm batches each, within a device-memory budget (bytes; 0 =
  * a quarter of the device), and writes the fastest into cfg->frames_in_flight.
  * Results do not depend on frames_in_flight. */
@@ -368,7 +369,8 @@ typedef struct wsrt_autotune_result {
     uint32_t best;
 } wsrt_autotune_result;
 int wsrt_autotune(wsrt_device* dev, const wsrt_scene* scene, uint32_t width, uint32_t height,
-                  wsrt_config* cfg, uint64_t mem_budget, wsrt_autotune_result* out);
+                  const wsrt_camera_uniform* cam, wsrt_config* cfg, uint64_t mem_budget,
+                  wsrt_autotune_result* out);
 
 /* ---- learned path guiding (no reference counterpart: the reference has no
  *      guiding; this is the online-trained proposal of wsrt_config.guiding) --

@@ -780,11 +780,12 @@ static FrameConst make_frame_const(const wsrt_config& c, const wsrt_camera_unifo
 // ---------------------------------------------------------------------------
 namespace {
 enum KernelId {
-    K_FRAME_PREP, K_PHOTON_TRACE, K_GRID_BUILD, K_GENERATE, K_EXTEND, K_SHADE, K_SHADOW,
+    K_FRAME_PREP, K_PHOTON_EMIT, K_PHOTON_TRACE, K_GRID_BUILD, K_GENERATE, K_EXTEND, K_SHADE, K_SHADOW,
     K_ACCUMULATE, K_GUIDE_SCATTER, K_GUIDE_TRAIN, K_COUNT
 };
-const char* kKernelNames[K_COUNT] = {"frame_prep", "photon_trace", "grid_build", "generate", "extend",
-                                     "shade", "shadow", "accumulate", "guide_scatter", "guide_train"};
+const char* kKernelNames[K_COUNT] = {"frame_prep", "photon_emit", "photon_trace", "grid_build", "generate",
+                                     "extend", "shade", "shadow", "accumulate", "guide_scatter",
+                                     "guide_train"};
 constexpr uint32_t kUploadHalf = kFrameTable / 2;  // frames per upload (staging half)
 struct Pending {
     int kid;
@@ -1210,9 +1211,13 @@ static int enqueue_batch(wsrt_renderer* r, uint32_t nf) {
     }
     if (np) {
         {
+            ProfScope p(r, K_PHOTON_EMIT);
+            launch_photon_emit(pl.sv, pl.fa, np, nf, r->phs, r->photons.as<wsrt_photon>(), s);
+        }
+        for (uint32_t d = 0; d < r->cfg.photon_max_depth; ++d) {
             ProfScope p(r, K_PHOTON_TRACE);
-            launch_photon_trace(pl.sv, pl.fa, np, nf, r->cfg.photon_max_depth, pl.count_traversal, r->phs,
-                                pl.qs, r->photons.as<wsrt_photon>(), pl.ctr, pl.gstack, s);
+            launch_photon_bounce(pl.sv, np, nf, d, pl.count_traversal, r->phs, pl.qs, r->photons.as<wsrt_photon>(),
+                                 pl.ctr, pl.gstack, s);
         }
         ProfScope p(r, K_GRID_BUILD);
         if (grid_build(r->photons.as<wsrt_photon>(), np, nf, r->cfg.hash_size, pl.fa, r->gb, s))
@@ -1850,7 +1855,8 @@ extern "C" int wsrt_renderer_get_guide(wsrt_renderer* r, wsrt_guide** out) {
 // autotuner: frames in flight
 // ---------------------------------------------------------------------------
 extern "C" int wsrt_autotune(wsrt_device* dev, const wsrt_scene* scene, uint32_t width, uint32_t height,
-                             wsrt_config* cfg, uint64_t mem_budget, wsrt_autotune_result* out) {
+                             const wsrt_camera_uniform* cam, wsrt_config* cfg, uint64_t mem_budget,
+                             wsrt_autotune_result* out) {
     CHECK_ARG(dev && scene && cfg && out && width > 0 && height > 0);
     int rc = validate_config(cfg);
     if (rc) return rc;
@@ -1876,6 +1882,7 @@ extern "C" int wsrt_autotune(wsrt_device* dev, const wsrt_scene* scene, uint32_t
         (void)hipMemGetInfo(&fb, &total);
         out->device_bytes[k] = fa > fb ? (uint64_t)(fa - fb) : 0;
         bool ok = out->device_bytes[k] <= mem_budget;
+        if (ok && cam) ok = wsrt_renderer_set_camera(r, cam) == WSRT_OK;
         float ms = 0.0f;
         if (ok) {
             hipEvent_t e0 = nullptr, e1 = nullptr;

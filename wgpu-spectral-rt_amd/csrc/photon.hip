@@ -320,20 +320,21 @@ __global__ __launch_bounds__(256) void k_grid_copy(const wsrt_photon* __restrict
 
 }  // namespace
 
-void launch_photon_trace(const SceneView& sv, const FrameArgs& fa, uint32_t np, uint32_t nf,
-                         uint32_t max_depth, uint32_t count_traversal, const PhotonState& ps,
-                         const Queues& qs, wsrt_photon* photons, Counters* ctr, uint2* gstack,
-                         hipStream_t s) {
+void launch_photon_emit(const SceneView& sv, const FrameArgs& fa, uint32_t np, uint32_t nf, const PhotonState& ps,
+                        wsrt_photon* photons, hipStream_t s) {
     const uint32_t n = np * nf;
     hipLaunchKernelGGL(k_photon_emit, dim3((n + kEmitThreads - 1) / kEmitThreads), dim3(kEmitThreads), 0, s, sv,
                        fa, np, nf, ps, photons);
-    const uint32_t n0 = sv.n_lights ? n : 0u;
-    for (uint32_t d = 0; d < max_depth; ++d) {
-        const uint32_t* q = (d & 1u) ? ps.queue_b : ps.queue_a;
-        uint32_t* nq = (d & 1u) ? ps.queue_a : ps.queue_b;
-        hipLaunchKernelGGL(k_photon_bounce, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, sv, d, n0, qs,
-                           count_traversal, ps, q, nq, photons, ctr, gstack);
-    }
+}
+
+void launch_photon_bounce(const SceneView& sv, uint32_t np, uint32_t nf, uint32_t depth,
+                          uint32_t count_traversal, const PhotonState& ps, const Queues& qs, wsrt_photon* photons,
+                          Counters* ctr, uint2* gstack, hipStream_t s) {
+    const uint32_t n0 = sv.n_lights ? np * nf : 0u;
+    const uint32_t* q = (depth & 1u) ? ps.queue_b : ps.queue_a;
+    uint32_t* nq = (depth & 1u) ? ps.queue_a : ps.queue_b;
+    hipLaunchKernelGGL(k_photon_bounce, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, sv, depth, n0, qs,
+                       count_traversal, ps, q, nq, photons, ctr, gstack);
 }
 
 size_t grid_temp_words(uint32_t n, uint32_t cells) {

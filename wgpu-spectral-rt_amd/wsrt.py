@@ -141,7 +141,7 @@ _SIGS = {
     "wsrt_guide_gradients": (C.c_int, [vp, vp, vp, vp, u32, vp]),
     "wsrt_guide_train_step": (C.c_int, [vp, vp, vp, vp, u32, C.c_float]),
     "wsrt_renderer_get_guide": (C.c_int, [vp, P(vp)]),
-    "wsrt_autotune": (C.c_int, [vp, vp, u32, u32, P(Config), C.c_uint64, P(AutotuneResult)]),
+    "wsrt_autotune": (C.c_int, [vp, vp, u32, u32, vp, P(Config), C.c_uint64, P(AutotuneResult)]),
 }
 
 GUIDE_PARAMS = 27952
@@ -655,11 +655,13 @@ def load_png(path: str) -> np.ndarray:
 
 
 def autotune(device: "Device", scene: "Scene", width: int, height: int, config: Config,
-             mem_budget: int = 0) -> dict:
-    """Picks config.frames_in_flight by timing candidates (wsrt_autotune);
-    updates config in place and returns the timings."""
+             camera: CameraUniform | None = None, mem_budget: int = 0) -> dict:
+    """Picks config.frames_in_flight by timing candidates (wsrt_autotune) for
+    the given camera; updates config in place and returns the timings."""
     res = AutotuneResult()
-    _check(load().wsrt_autotune(device.h, scene.h, width, height, C.byref(config), mem_budget, C.byref(res)))
+    cam = C.cast(C.byref(camera), vp) if camera is not None else None
+    _check(load().wsrt_autotune(device.h, scene.h, width, height, cam, C.byref(config), mem_budget,
+                                C.byref(res)))
     n = res.n_candidates
     return {"best": res.best,
             "candidates": [{"frames_in_flight": res.frames_in_flight[i],
