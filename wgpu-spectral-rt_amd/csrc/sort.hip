@@ -12,8 +12,9 @@ namespace wsrt {
 namespace {
 
 constexpr int kScanThreads = 256;
-constexpr int kScanItems = 16;
-constexpr int kScanTile = kScanThreads * kScanItems;  // 4096
+constexpr int kScanRound = kScanThreads * 4;          // 4 consecutive elements per thread
+constexpr int kScanRounds = 16;
+constexpr int kScanTile = kScanRound * kScanRounds;  // 16384 elements per block
 
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
     const int lane = __lane_id();
@@ -53,15 +54,41 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& t
     return res;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ in,
-                                                              uint32_t n,
+// Four consecutive elements from i (zero past n). kVec: one 16-byte load
+// (n is a multiple of 4 and the array 16-byte aligned).
+template <bool kVec>
+__device__ __forceinline__ uint4 load4(const uint32_t* __restrict__ in, uint64_t i, uint32_t n) {
+    if (kVec) return i < n ? *reinterpret_cast<const uint4*>(in + i) : make_uint4(0u, 0u, 0u, 0u);
+    uint4 v;
+    v.x = i < n ? in[i] : 0u;
+    v.y = i + 1 < n ? in[i + 1] : 0u;
+    v.z = i + 2 < n ? in[i + 2] : 0u;
+    v.w = i + 3 < n ? in[i + 3] : 0u;
+    return v;
+}
+template <bool kVec>
+__device__ __forceinline__ void store4(uint32_t* __restrict__ out, uint64_t i, uint32_t n, uint4 v) {
+    if (kVec) {
+        if (i < n) *reinterpret_cast<uint4*>(out + i) = v;
+        return;
+    }
+    if (i < n) out[i] = v.x;
+    if (i + 1 < n) out[i + 1] = v.y;
+    if (i + 2 < n) out[i + 2] = v.z;
+    if (i + 3 < n) out[i + 3] = v.w;
+}
+
+// Tile sums. A tile is kScanRounds rounds of 1024 elements; in a round,
+// thread t holds elements 4t .. 4t+3, so every load is one contiguous 4 KiB.
+template <bool kVec>
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ in, uint32_t n,
                                                               uint32_t* __restrict__ block_sums) {
-    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + 4u * threadIdx.x;
     uint32_t s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        uint64_t i = base + (uint64_t)k * kScanThreads + threadIdx.x;
-        s += i < n ? in[i] : 0u;
+#pragma unroll 4
+    for (int r = 0; r < kScanRounds; ++r) {
+        const uint4 v = load4<kVec>(in, base + (uint64_t)r * kScanRound, n);
+        s += (v.x + v.y) + (v.z + v.w);
     }
     s = wave_reduce(s);
     __shared__ uint32_t part[kScanThreads / 64];
@@ -95,28 +122,27 @@ __global__ __launch_bounds__(1024) void k_scan_block_sums(uint32_t* __restrict__
     if (threadIdx.x == 0 && total_out) *total_out = carry;
 }
 
+// Exclusive scan of a tile from its offset in block_sums, round by round.
+template <bool kVec>
 __global__ __launch_bounds__(kScanThreads) void k_scan_downsweep(const uint32_t* __restrict__ in,
-                                                                 uint32_t* __restrict__ out,
-                                                                 uint32_t n,
+                                                                 uint32_t* __restrict__ out, uint32_t n,
                                                                  const uint32_t* __restrict__ block_sums) {
     __shared__ uint32_t smem[64];
-    // each thread owns kScanItems consecutive elements
-    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
-    uint32_t v[kScanItems];
-    uint32_t s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        uint64_t i = base + k;
-        v[k] = i < n ? in[i] : 0u;
-        s += v[k];
-    }
-    uint32_t total;
-    uint32_t ex = block_exclusive_scan(s, total, smem) + block_sums[blockIdx.x];
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        uint64_t i = base + k;
-        if (i < n) out[i] = ex;
-        ex += v[k];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + 4u * threadIdx.x;
+    uint32_t carry = block_sums[blockIdx.x];
+    for (int r = 0; r < kScanRounds; ++r) {
+        const uint64_t i = base + (uint64_t)r * kScanRound;
+        if (i - 4u * threadIdx.x >= n) break;  // block-uniform: the round starts past the end
+        const uint4 v = load4<kVec>(in, i, n);
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan((v.x + v.y) + (v.z + v.w), total, smem) + carry;
+        uint4 o;
+        o.x = ex;
+        o.y = o.x + v.x;
+        o.z = o.y + v.y;
+        o.w = o.z + v.z;
+        store4<kVec>(out, i, n, o);
+        carry += total;
     }
 }
 
@@ -134,10 +160,14 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_histogram(const uint32_t
                                                                   int shift, uint32_t num_tiles,
                                                                   uint32_t* __restrict__ hist) {
     if (n_dev) n = min(n, *n_dev);
+    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+    if (base >= n) {  // a tile past the device count: an empty histogram
+        hist[(uint64_t)threadIdx.x * num_tiles + blockIdx.x] = 0u;
+        return;
+    }
     __shared__ uint32_t h[kRadix];
     h[threadIdx.x] = 0;
     __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
 #pragma unroll 4
     for (int r = 0; r < kSortRounds; ++r) {
         uint64_t i = base + (uint64_t)r * kSortThreads + threadIdx.x;
@@ -157,6 +187,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint32_t* __restrict__ n_dev, int shift, uint32_t num_tiles,
     const uint32_t* __restrict__ hist_scanned) {
     if (n_dev) n = min(n, *n_dev);
+    if ((uint64_t)blockIdx.x * kSortTile >= n) return;  // block-uniform: nothing to place
     constexpr int kWaves = kSortThreads / 64;
     __shared__ uint32_t run[kRadix];
     __shared__ uint32_t wave_cnt[kWaves][kRadix];
@@ -230,9 +261,13 @@ int scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* temp
     if (n == 0) return 0;
     uint32_t blocks = (n + kScanTile - 1) / kScanTile;
     if (blocks > 1024u * 64u) return -1;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(blocks), dim3(kScanThreads), 0, s, in, n, temp);
+    const bool vec = (n % 4u) == 0 && (reinterpret_cast<uintptr_t>(in) & 15u) == 0 &&
+                     (reinterpret_cast<uintptr_t>(out) & 15u) == 0;
+    if (vec) hipLaunchKernelGGL(k_scan_reduce<true>, dim3(blocks), dim3(kScanThreads), 0, s, in, n, temp);
+    else hipLaunchKernelGGL(k_scan_reduce<false>, dim3(blocks), dim3(kScanThreads), 0, s, in, n, temp);
     hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(1024), 0, s, temp, blocks, total_out);
-    hipLaunchKernelGGL(k_scan_downsweep, dim3(blocks), dim3(kScanThreads), 0, s, in, out, n, temp);
+    if (vec) hipLaunchKernelGGL(k_scan_downsweep<true>, dim3(blocks), dim3(kScanThreads), 0, s, in, out, n, temp);
+    else hipLaunchKernelGGL(k_scan_downsweep<false>, dim3(blocks), dim3(kScanThreads), 0, s, in, out, n, temp);
     return 0;
 }
 
