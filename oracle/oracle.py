@@ -102,6 +102,7 @@ def _load():
         "or_scene_add_instance": (None, [vp, C.c_uint32, C.c_uint32, f32p]),
         "or_scene_add_light_instance": (None, [vp, C.c_uint32, C.c_uint32, f32p, f32p]),
         "or_scene_add_point_light": (None, [vp, f32p, f32p]),
+        "or_scene_set_environment": (None, [vp, f32p, C.c_uint32, C.c_uint32]),
         "or_scene_build": (C.c_int, [vp]),
         "or_scene_num_triangles": (C.c_uint32, [vp]),
         "or_scene_num_lights": (C.c_uint32, [vp]),
@@ -219,6 +220,11 @@ class Scene:
     def add_light_instance(self, mesh, mat, m16, params):
         lib.or_scene_add_light_instance(self.h, mesh, mat, np.ascontiguousarray(m16, np.float32),
                                         np.asarray(params, np.float32))
+
+    def set_environment(self, rgb):
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        h, w = rgb.shape[:2]
+        lib.or_scene_set_environment(self.h, rgb, w, h)
 
     def add_point_light(self, pos, params):
         lib.or_scene_add_point_light(self.h, np.asarray(pos, np.float32),

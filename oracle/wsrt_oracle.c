@@ -333,6 +333,8 @@ struct or_scene {
     uint32_t* tri_inst; uint32_t* tri_prim;
     or_bnode* nodes; uint32_t nnodes;
     uint32_t* order;
+    /* environment light (extension), equirect RGB, row 0 = +Y */
+    float* env; uint32_t env_w, env_h;
 };
 
 #define GROW(ptr, n, cap, add)                                                   \
@@ -351,6 +353,7 @@ void or_scene_destroy(or_scene* s) {
     free(s->verts); free(s->idx); free(s->meshes); free(s->mesh_nv); free(s->mesh_ni);
     free(s->mats); free(s->inst); free(s->lights); free(s->v0); free(s->e1); free(s->e2);
     free(s->tri_inst); free(s->tri_prim); free(s->nodes); free(s->order); free(s->wv);
+    free(s->env);
     free(s);
 }
 uint32_t or_scene_add_material(or_scene* s, const wsrt_material* m) {
@@ -1117,7 +1120,9 @@ static scatter_t sample_bsdf(v3 rdir, v3 ff, const wsrt_material* m, int front, 
     uint32_t type = (uint32_t)m->extra[0];
     v3 v = normalize3(neg3(rdir));
     v4 spec = or_rgb_to_spectrum4(V3(m->color[0], m->color[1], m->color[2]), wl);
-    if (type == 1u) {
+    if (type == 1u || type == 5u) {
+        /* 5 (extension): GGX with a thin-film Fresnel factor (film color[0] nm,
+         * index extra[2], substrate extra[3]) at the half-vector angle */
         float rough = maxf(m->extra[1], 0.001f);
         float a2 = rough * rough;
         float r1 = or_rand(rng), r2 = or_rand(rng);
@@ -1132,9 +1137,18 @@ static scatter_t sample_bsdf(v3 rdir, v3 ff, const wsrt_material* m, int front, 
         float ndl = dot3(ff, l), ndv = dot3(ff, v);
         if (ndl > 0.0f && ndv > 0.0f) {
             float ndh = maxf(dot3(ff, h), 0.0f), vdh = maxf(dot3(v, h), 0.0f);
-            float p5 = powf(clampf(1.0f - vdh, 0.0f, 1.0f), 5.0f);
-            v4 F = V4(spec.x + (1.0f - spec.x) * p5, spec.y + (1.0f - spec.y) * p5,
-                      spec.z + (1.0f - spec.z) * p5, spec.w + (1.0f - spec.w) * p5);
+            v4 F;
+            if (type == 5u) {
+                float d = maxf(m->color[0], 50.0f);
+                F = V4(or_thin_film(vdh, 1.0f, m->extra[2], m->extra[3], d, wl.x),
+                       or_thin_film(vdh, 1.0f, m->extra[2], m->extra[3], d, wl.y),
+                       or_thin_film(vdh, 1.0f, m->extra[2], m->extra[3], d, wl.z),
+                       or_thin_film(vdh, 1.0f, m->extra[2], m->extra[3], d, wl.w));
+            } else {
+                float p5 = powf(clampf(1.0f - vdh, 0.0f, 1.0f), 5.0f);
+                F = V4(spec.x + (1.0f - spec.x) * p5, spec.y + (1.0f - spec.y) * p5,
+                       spec.z + (1.0f - spec.z) * p5, spec.w + (1.0f - spec.w) * p5);
+            }
             float G = g1_schlick(ndl, rough) * g1_schlick(ndv, rough);
             float den = maxf(ndv * ndh, 0.0001f);
             rec.dir = l;
@@ -1197,6 +1211,30 @@ static scatter_t sample_bsdf(v3 rdir, v3 ff, const wsrt_material* m, int front, 
     return rec;
 }
 
+void or_scene_set_environment(or_scene* s, const float* rgb, uint32_t w, uint32_t h) {
+    free(s->env);
+    s->env = malloc(sizeof(float) * 3 * (size_t)w * h);
+    memcpy(s->env, rgb, sizeof(float) * 3 * (size_t)w * h);
+    s->env_w = w;
+    s->env_h = h;
+}
+
+/* environment radiance of a miss: nearest texel of u = atan2(z, x)/2pi + 1/2,
+ * v = acos(y)/pi; spectrum = max(rgb) * rgb_to_spectrum(rgb / max(rgb)) */
+static v4 env_radiance(const or_scene* s, v3 d, v4 wl) {
+    float u = atan2f(d.z, d.x) * (0.5f / OR_PI) + 0.5f;
+    float v = acosf(clampf(d.y, -1.0f, 1.0f)) * (1.0f / OR_PI);
+    float fx = maxf(u * (float)s->env_w, 0.0f), fy = maxf(v * (float)s->env_h, 0.0f);
+    uint32_t x = (uint32_t)fx, y = (uint32_t)fy;
+    if (x > s->env_w - 1u) x = s->env_w - 1u;
+    if (y > s->env_h - 1u) y = s->env_h - 1u;
+    const float* t = s->env + 3 * ((size_t)y * s->env_w + x);
+    float mx = maxf(maxf(t[0], t[1]), t[2]);
+    if (!(mx > 0.0f)) return V4s(0.0f);
+    float im = 1.0f / mx;
+    return mul4s(or_rgb_to_spectrum4(V3(t[0] * im, t[1] * im, t[2] * im), wl), mx);
+}
+
 /* shader.wgsl:547-703 */
 static v4 ray_color(const or_frame* f, v3 o, v3 d, v4 wl, uint32_t* rng) {
     const or_scene* s = f->s;
@@ -1205,7 +1243,10 @@ static v4 ray_color(const or_frame* f, v3 o, v3 d, v4 wl, uint32_t* rng) {
     int has_diffuse = 0, caustic = 0;
     for (uint32_t i = 0; i < c->max_depth; ++i) {
         hit_t h = scene_closest(s, o, d, 0.0001f, 100.0f, 0);
-        if (h.tri == NO_HIT) break;
+        if (h.tri == NO_HIT) {
+            if (c->env_map && s->env) acc = add4(acc, mul4(tp, env_radiance(s, d, wl)));
+            break;
+        }
         const wsrt_material* m = &s->mats[s->inst[s->tri_inst[h.tri]].mat];
         v3 wn = hit_world_normal(s, h.tri, h.u, h.v);
         v3 ff = h.front ? wn : neg3(wn);

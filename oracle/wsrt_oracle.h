@@ -67,6 +67,7 @@ void or_scene_add_instance(or_scene* s, uint32_t mesh, uint32_t mat, const float
 void or_scene_add_light_instance(or_scene* s, uint32_t mesh, uint32_t mat, const float* m16,
                                  const float* params4);
 void or_scene_add_point_light(or_scene* s, const float* pos3, const float* params4);
+void or_scene_set_environment(or_scene* s, const float* rgb, uint32_t w, uint32_t h);
 /* Transforms instances to world triangles and builds the oracle's own BVH. */
 int or_scene_build(or_scene* s);
 uint32_t or_scene_num_triangles(const or_scene* s);

@@ -38,6 +38,12 @@ def setup(device, name, W, H, photons_per_pass):
         o = O.cornell_box()
         cfg = wsrt.default_config(photons_per_pass=photons_per_pass)
         view = scenes.View((0.0, 0.0, 3.0), scenes.HALF_PI * -1, 0.0, W, H)
+    elif name == "heightfield":
+        g = scenes.build_heightfield(wsrt.SceneBuilder(), n=65, tiles=4).build(device)
+        o = scenes.build_heightfield(O.Scene(), n=65, tiles=4).build()
+        cfg = scenes.heightfield_config(photons_per_pass=photons_per_pass)
+        v = scenes.HEIGHTFIELD_VIEW
+        view = scenes.View(v.position, v.yaw, v.pitch, W, H)
     elif name == "custom":
         g = scenes.build_custom(wsrt.SceneBuilder()).build(device)
         o = scenes.build_custom(O.Scene()).build()
@@ -82,7 +88,8 @@ def test_photon_pass_and_grid(device, name, frame):
 
 
 @pytest.mark.parametrize("name,frame,photons", [("cornell", 0, 1 << 16), ("cornell", 9, 1 << 16),
-                                                ("c2", 1, 1 << 18), ("custom", 4, 1 << 17)])
+                                                ("c2", 1, 1 << 18), ("custom", 4, 1 << 17),
+                                                ("heightfield", 2, 1 << 10)])
 def test_camera_paths_match_oracle(device, name, frame, photons):
     print(f"paths {name} frame {frame}", flush=True)
     W = H = 96
@@ -103,7 +110,7 @@ def test_camera_paths_match_oracle(device, name, frame, photons):
     assert np.abs(oL).sum() > 0
 
 
-@pytest.mark.parametrize("name", ["cornell", "c2", "custom"])
+@pytest.mark.parametrize("name", ["cornell", "c2", "custom", "heightfield"])
 def test_accumulated_image_matches_oracle(device, name):
     print(f"image {name}", flush=True)
     W = H = 64
@@ -257,3 +264,9 @@ def test_autotune_picks_a_measured_candidate(device, cornell_pair):
     ms = {c["frames_in_flight"]: c["ms_per_frame"] for c in res["candidates"]}
     assert cfg.frames_in_flight == res["best"] and ms[res["best"]] > 0
     assert ms[res["best"]] <= min(v for v in ms.values() if v > 0) * 1.02 + 1e-9
+
+
+def test_environment_requires_a_map(device, cornell_pair):
+    g, _ = cornell_pair
+    with pytest.raises(wsrt.WsrtError):
+        wsrt.Renderer(device, g, 8, 8, wsrt.default_config(env_map=1))

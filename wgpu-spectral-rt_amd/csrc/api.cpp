@@ -1008,6 +1008,7 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     else wsrt_config_default(&c);
     int rc = validate_config(&c);
     if (rc) return rc;
+    if (c.env_map && !scene->view.env) return fail(WSRT_E_INVALID, "env_map = 1 but the scene has no environment map");
     HIP_OR_FAIL(hipSetDevice(dev->ordinal));
     auto r = std::make_unique<wsrt_renderer>();
     r->dev = dev;
@@ -1087,6 +1088,7 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     r->gb.temp = r->grid_temp.as<uint32_t>();
     PathLaunch& pl = r->pl;
     pl.sv = scene->view;
+    if (!c.env_map) pl.sv.env = nullptr;  // the environment lights misses only on request
     pl.ps.o = r->path_o.as<float4>();
     pl.ps.d = r->path_d.as<float4>();
     pl.ps.tp = r->path_tp.as<float4>();

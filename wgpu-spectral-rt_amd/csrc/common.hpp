@@ -55,7 +55,7 @@ struct FrameParams {
 
 constexpr int kMatBins = 8;
 constexpr int kMaxBounces = 64;
-constexpr int kStatBlocks = 2048;  // blocks of the photon bounce launch
+constexpr int kStatBlocks = 4096;  // blocks of the photon bounce launch (kTraceBlocks)
 
 // Running statistics of a renderer. Totals over a whole launch are added
 // once per launch; per-path totals are kept per producer block (plain

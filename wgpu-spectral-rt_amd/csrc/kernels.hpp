@@ -13,9 +13,9 @@ namespace wsrt {
 
 // Traversal launches use a fixed grid of kTraceBlocks x kTraceThreads lanes
 // that loop over their work (so the per-lane HBM stack spill area has a fixed
-// size): 2048 x 128 lanes = 16 waves on each of the 256 CUs.
+// size): 4096 x 128 lanes = 32 waves on each of the 256 CUs.
 constexpr int kTraceThreads = 128;
-constexpr int kTraceBlocks = 2048;
+constexpr int kTraceBlocks = 4096;
 constexpr uint64_t kTraceLanes = (uint64_t)kTraceThreads * kTraceBlocks;
 
 // ---- primitives (sort.hip) -------------------------------------------------

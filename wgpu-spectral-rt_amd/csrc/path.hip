@@ -131,6 +131,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_extend(SceneView sv, PathStat
             my_rays++;
             if (h.tri == kInvalid) {
                 have = false;  // miss: the path ends (shader.wgsl:566)
+                if (sv.env) ps.L[p] = add4(ps.L[p], mul4(ps.tp[p], env_radiance(sv, xyz(d), ps.wl[p])));
             } else {
                 ps.hit[p] = make_uint4(__float_as_uint(h.t), h.tri | (h.front ? 0x80000000u : 0u),
                                        __float_as_uint(h.u), __float_as_uint(h.v));

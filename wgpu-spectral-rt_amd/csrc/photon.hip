@@ -14,6 +14,8 @@
 
 namespace wsrt {
 
+static_assert(kTraceBlocks <= kStatBlocks, "per-block statistics of the photon bounce launch");
+
 namespace {
 
 // Octahedral bin (8 x 8, +Y pole) of a unit direction.

@@ -22,7 +22,7 @@ namespace wsrt {
 
 constexpr int kSegs = 64;
 constexpr int kSegStride = 32;             // uint32 counters 128 bytes apart
-constexpr uint32_t kGridLanes = 262144;     // lanes of every queue-producing launch
+constexpr uint32_t kGridLanes = 524288;     // lanes of every queue-producing launch
 
 enum QueueKind : uint32_t {
     kQActive = 0,                      // camera paths entering a bounce

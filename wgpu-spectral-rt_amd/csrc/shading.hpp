@@ -133,7 +133,10 @@ __device__ __forceinline__ Scatter sample_bsdf(f3 rdir, f3 ff, const wsrt_materi
     const uint32_t type = (uint32_t)ex.x;
     f3 v = normalize(-rdir);
     float4 spec = rgb_to_spectrum4(mk3(col.x, col.y, col.z), wl);
-    if (type == 1u) {
+    if (type == 1u || type == 5u) {
+        // 5 (extension): the same GGX lobe with the Fresnel factor of a thin
+        // film (thickness color.r nm, index extra.z) on a substrate of index
+        // extra.w, per hero wavelength, at the half-vector angle
         float rough = fmaxf(ex.y, 0.001f);
         float a2 = rough * rough;
         float r1 = rand01(rng), r2 = rand01(rng);
@@ -148,9 +151,16 @@ __device__ __forceinline__ Scatter sample_bsdf(f3 rdir, f3 ff, const wsrt_materi
         float ndl = dot(ff, l), ndv = dot(ff, v);
         if (ndl > 0.0f && ndv > 0.0f) {
             float ndh = fmaxf(dot(ff, h), 0.0f), vdh = fmaxf(dot(v, h), 0.0f);
-            float p5 = powf(fminf(fmaxf(1.0f - vdh, 0.0f), 1.0f), 5.0f);
-            float4 F = make_float4(spec.x + (1.0f - spec.x) * p5, spec.y + (1.0f - spec.y) * p5,
-                                   spec.z + (1.0f - spec.z) * p5, spec.w + (1.0f - spec.w) * p5);
+            float4 F;
+            if (type == 5u) {
+                const float d = fmaxf(col.x, 50.0f);
+                F = make_float4(thin_film(vdh, 1.0f, ex.z, ex.w, d, wl.x), thin_film(vdh, 1.0f, ex.z, ex.w, d, wl.y),
+                                thin_film(vdh, 1.0f, ex.z, ex.w, d, wl.z), thin_film(vdh, 1.0f, ex.z, ex.w, d, wl.w));
+            } else {
+                float p5 = powf(fminf(fmaxf(1.0f - vdh, 0.0f), 1.0f), 5.0f);
+                F = make_float4(spec.x + (1.0f - spec.x) * p5, spec.y + (1.0f - spec.y) * p5,
+                                spec.z + (1.0f - spec.z) * p5, spec.w + (1.0f - spec.w) * p5);
+            }
             float G = g1_schlick(ndl, rough) * g1_schlick(ndv, rough);
             float den = fmaxf(ndv * ndh, 0.0001f);
             rec.dir = l;
@@ -211,6 +221,23 @@ __device__ __forceinline__ Scatter sample_bsdf(f3 rdir, f3 ff, const wsrt_materi
         rec.att = spec;
     }
     return rec;
+}
+
+// Environment light (extension): equirectangular RGB radiance, row 0 at +Y,
+// nearest texel; u = atan2(z, x) / 2 pi + 1/2, v = acos(y) / pi. The RGB
+// value is turned into a spectrum by the reference's reflectance fit
+// (shader.wgsl:106-126) applied to rgb / max(rgb), scaled back by max(rgb).
+__device__ __forceinline__ float4 env_radiance(const SceneView& sv, f3 d, float4 wl) {
+    const float u = atan2f(d.z, d.x) * (0.5f / kPi) + 0.5f;
+    const float v = acosf(fminf(fmaxf(d.y, -1.0f), 1.0f)) * (1.0f / kPi);
+    const uint32_t x = min((uint32_t)fmaxf(u * (float)sv.env_w, 0.0f), sv.env_w - 1u);
+    const uint32_t y = min((uint32_t)fmaxf(v * (float)sv.env_h, 0.0f), sv.env_h - 1u);
+    const float* t = sv.env + 3ull * ((size_t)y * sv.env_w + x);
+    const float r = t[0], g = t[1], b = t[2];
+    const float m = fmaxf(fmaxf(r, g), b);
+    if (!(m > 0.0f)) return splat4(0.0f);
+    const float im = 1.0f / m;
+    return mul4s(rgb_to_spectrum4(mk3(r * im, g * im, b * im), wl), m);
 }
 
 // shader.wgsl:297-342 over the counting-sort grid: the first gather_cap

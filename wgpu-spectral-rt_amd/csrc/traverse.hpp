@@ -20,7 +20,7 @@
 namespace wsrt {
 
 constexpr float kBoxPad = 1.00001f;
-constexpr int kShortStack = 16;  // entries per lane in LDS; deeper ones spill to HBM
+constexpr int kShortStack = 8;  // entries per lane in LDS; deeper ones spill to HBM
 
 struct Hit {
     float t;

@@ -10,6 +10,11 @@ drive any builder with that interface.
   c2_caustic    glass icosphere (subdivision 4, radius 0.5, Cauchy A = 1.5,
                 B = 0.0042 um^2) on a 2-triangle diffuse plane, point light
                 2 units above the sphere centre
+  heightfield   1024 x 1024-vertex heightfield (2,093,058 triangles) displaced
+                by seeded 6-octave Perlin noise of amplitude 0.1, 16 x 16
+                tiles of thin-film-coated GGX (alpha U[0.05, 0.6], film
+                200..800 nm of index 1.33 on a 1.5 substrate), lit by a
+                seeded 512 x 256 sky-gradient environment
   custom        the reference's second scene (src/scene/custom.rs): a dark
                 room, a vertical slit light, a high-dispersion glass prism
                 (Cauchy B = 0.12), a thin-film bubble and a white board
@@ -255,3 +260,119 @@ def build_custom(b):
 
 
 DEFAULT_VIEW = View((0.0, 0.0, 3.0), -HALF_PI, 0.0, 256, 256)  # CameraController::new (camera.rs:35-52)
+
+
+def _perlin_grads(rng):
+    a = rng.uniform(0.0, 2.0 * math.pi, 256)
+    return np.stack([np.cos(a), np.sin(a)], 1), rng.permutation(256)
+
+
+def perlin_fbm(x, z, seed: int, octaves: int = 6) -> np.ndarray:
+    """Seeded 2D gradient (Perlin) noise, fractal sum of `octaves` octaves with
+    lacunarity 2 and gain 1/2, normalised to [-1, 1] by the sum of gains."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros_like(x, dtype=np.float64)
+    amp, freq, norm = 1.0, 4.0, 0.0
+    for _ in range(octaves):
+        g, perm = _perlin_grads(rng)
+        px, pz = x * freq, z * freq
+        x0, z0 = np.floor(px).astype(np.int64), np.floor(pz).astype(np.int64)
+        fx, fz = px - x0, pz - z0
+
+        def grad(ix, iz, dx, dz):
+            h = perm[(perm[ix & 255] + iz) & 255]
+            return g[h, 0] * dx + g[h, 1] * dz
+
+        def fade(t):
+            return t * t * t * (t * (t * 6 - 15) + 10)
+
+        n00 = grad(x0, z0, fx, fz)
+        n10 = grad(x0 + 1, z0, fx - 1, fz)
+        n01 = grad(x0, z0 + 1, fx, fz - 1)
+        n11 = grad(x0 + 1, z0 + 1, fx - 1, fz - 1)
+        u, v = fade(fx), fade(fz)
+        nx0 = n00 + u * (n10 - n00)
+        nx1 = n01 + u * (n11 - n01)
+        out += amp * (nx0 + v * (nx1 - nx0)) * math.sqrt(2.0)
+        norm += amp
+        amp *= 0.5
+        freq *= 2.0
+    return out / norm
+
+
+def sky_environment(seed: int = 7, w: int = 512, h: int = 256) -> np.ndarray:
+    """Seeded sky gradient, [h, w, 3] float32 RGB radiance, row 0 at +Y: a
+    zenith-to-horizon blue-to-white ramp, a warm sun disc at a seeded
+    azimuth and elevation, and a dim ground below the horizon."""
+    rng = np.random.default_rng(seed)
+    v = (np.arange(h) + 0.5) / h                     # 0 at +Y
+    u = (np.arange(w) + 0.5) / w
+    theta = v * math.pi
+    y = np.cos(theta)[:, None]
+    phi = (u - 0.5) * 2.0 * math.pi
+    zen = np.array(rng.uniform([0.15, 0.3, 0.7], [0.25, 0.45, 0.9]))
+    hor = np.array([0.9, 0.9, 0.95])
+    t = np.clip(y, 0.0, 1.0) ** 0.5
+    sky = hor[None, None] * (1 - t[..., None]) + zen[None, None] * t[..., None]
+    ground = np.array([0.08, 0.07, 0.06])
+    img = np.where((y > 0)[..., None], sky, ground[None, None]) * np.ones((h, w, 1))
+    sun_phi, sun_el = rng.uniform(-math.pi, math.pi), rng.uniform(0.3, 0.8)
+    sd = np.array([math.cos(sun_el) * math.cos(sun_phi), math.sin(sun_el), math.cos(sun_el) * math.sin(sun_phi)])
+    dx = np.sin(theta)[:, None] * np.cos(phi)[None, :]
+    dz = np.sin(theta)[:, None] * np.sin(phi)[None, :]
+    dy = np.cos(theta)[:, None] * np.ones((1, w))
+    cosang = dx * sd[0] + dy * sd[1] + dz * sd[2]
+    img = img + (cosang > math.cos(0.05))[..., None] * np.array([60.0, 50.0, 35.0])[None, None]
+    return img.astype(np.float32)
+
+
+HEIGHTFIELD_N = 1024  # vertices per side
+
+
+def build_heightfield(b, n: int = HEIGHTFIELD_N, tiles: int = 16, seed: int = 3, sky_seed: int = 7):
+    """n x n vertices over [-1, 1]^2 in x, z, y = 0.1 * fbm(x, z); split into
+    tiles x tiles meshes (shared border vertices duplicated), each with its
+    own thin-film-coated GGX material (type 5)."""
+    rng = np.random.default_rng(seed + 1000)
+    xs = np.linspace(-1.0, 1.0, n)
+    X, Z = np.meshgrid(xs, xs, indexing="xy")  # [row = z][col = x]
+    Y = 0.1 * perlin_fbm(X, Z, seed)
+    # normals from central differences of the height
+    dydx = np.gradient(Y, xs, axis=1)
+    dydz = np.gradient(Y, xs, axis=0)
+    N = np.stack([-dydx, np.ones_like(Y), -dydz], -1)
+    N /= np.linalg.norm(N, axis=-1, keepdims=True)
+    q = n - 1  # quads per side
+    edges = np.linspace(0, q, tiles + 1).round().astype(int)
+    for ti in range(tiles):
+        for tj in range(tiles):
+            r0, r1, c0, c1 = edges[ti], edges[ti + 1], edges[tj], edges[tj + 1]
+            vr, vc = r1 - r0 + 1, c1 - c0 + 1
+            v = np.zeros(vr * vc, wsrt.VERTEX_DTYPE)
+            sl = (slice(r0, r1 + 1), slice(c0, c1 + 1))
+            v["pos"][:, 0] = X[sl].reshape(-1)
+            v["pos"][:, 1] = Y[sl].reshape(-1)
+            v["pos"][:, 2] = Z[sl].reshape(-1)
+            v["pos"][:, 3] = 1.0
+            v["normal"][:, :3] = N[sl].reshape(-1, 3)
+            rr, cc = np.meshgrid(np.arange(vr - 1), np.arange(vc - 1), indexing="ij")
+            a = (rr * vc + cc).reshape(-1)
+            # counter-clockwise seen from +Y: (a, a+vc, a+1), (a+1, a+vc, a+vc+1)
+            idx = np.stack([a, a + vc, a + 1, a + 1, a + vc, a + vc + 1], 1).reshape(-1).astype(np.uint32)
+            alpha = rng.uniform(0.05, 0.6)
+            film = rng.uniform(200.0, 800.0)
+            mat = b.add_material([film, 0.0, 0.0, 1.0], [5, alpha, 1.33, 1.5])
+            b.add_instance(b.add_mesh(v, idx), mat, _trs((0, 0, 0)))
+    b.set_environment(sky_environment(sky_seed))
+    return b
+
+
+HEIGHTFIELD_VIEW = View((0.0, 0.9, 1.6), -HALF_PI, -0.55, 1920, 1080)
+
+
+def heightfield_config(**kw):
+    """Depth 6, environment light on, no photon pass (no caustic casters)."""
+    c = wsrt.default_config(max_depth=6, enable_photons=0, env_map=1)
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
