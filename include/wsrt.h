@@ -372,6 +372,40 @@ int wsrt_autotune(wsrt_device* dev, const wsrt_scene* scene, uint32_t width, uin
                   const wsrt_camera_uniform* cam, wsrt_config* cfg, uint64_t mem_budget,
                   wsrt_autotune_result* out);
 
+/* ---- persistent multi-scene batch (no reference counterpart: the reference
+ *      renders one scene per window) ------------------------------------------
+ * Jobs of uneven scene, view, resolution and sample count rendered by one
+ * persistent launch pulling 64-pixel work items from a device queue. Each
+ * pixel's samples are traced in frame order with the renderer's per-path
+ * code, so a job's accumulation equals a wsrt_renderer's with the same
+ * scene, camera, size and frame count (photon pass off: the config must have
+ * enable_photons = 0 and guiding = 0). Every render restarts at frame 0. */
+typedef struct wsrt_batch wsrt_batch;
+typedef struct wsrt_batch_job {
+    const wsrt_scene* scene;
+    wsrt_camera_uniform camera;  /* frame_count is ignored: frames 0 .. spp-1 */
+    uint32_t width, height;
+    uint32_t spp;
+    uint32_t pad;
+} wsrt_batch_job;
+typedef struct wsrt_batch_tuning {
+    uint32_t tile_mode;      /* 0: 64 x 1 pixel work items, 1: 8 x 8 */
+    uint32_t blocks_per_cu;  /* persistent blocks of 256 threads per CU */
+} wsrt_batch_tuning;
+int wsrt_batch_create(wsrt_device* dev, const wsrt_batch_job* jobs, uint32_t n_jobs, const wsrt_config* cfg,
+                      wsrt_batch** out);
+int wsrt_batch_destroy(wsrt_batch* b);
+int wsrt_batch_set_tuning(wsrt_batch* b, const wsrt_batch_tuning* t);
+/* Times every tile_mode x blocks_per_cu in {1, 2, 4, 8} candidate on a
+ * one-sample-per-pixel pass of the batch and keeps the fastest. */
+int wsrt_batch_autotune(wsrt_batch* b, wsrt_batch_tuning* best, float* best_ms);
+/* All jobs, all samples: one persistent launch (captured as a graph after
+ * the first call), asynchronous. */
+int wsrt_batch_render(wsrt_batch* b);
+int wsrt_batch_sync(wsrt_batch* b);
+int wsrt_batch_read_accum(wsrt_batch* b, uint32_t job, float* out);
+int wsrt_batch_read_rgba8(wsrt_batch* b, uint32_t job, uint8_t* out);
+
 /* ---- learned path guiding (no reference counterpart: the reference has no
  *      guiding; this is the online-trained proposal of wsrt_config.guiding) --
  * A network of 7 input scalars in [0,1] (position over the scene box, normal

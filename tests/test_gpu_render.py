@@ -270,3 +270,45 @@ def test_environment_requires_a_map(device, cornell_pair):
     g, _ = cornell_pair
     with pytest.raises(wsrt.WsrtError):
         wsrt.Renderer(device, g, 8, 8, wsrt.default_config(env_map=1))
+
+
+@pytest.mark.parametrize("tile_mode", [0, 1])
+def test_batch_equals_single_renderers(device, tile_mode):
+    """One persistent launch over jobs of different scenes, sizes and sample
+    counts gives each job the image its own renderer gives, bit for bit."""
+    print(f"batch tile mode {tile_mode}", flush=True)
+    specs = [("cornell", 40, 24, 3), ("c2", 32, 32, 2), ("custom", 24, 40, 4), ("cornell", 9, 13, 5)]
+    jobs, refs = [], []
+    for k, (name, W, H, spp) in enumerate(specs):
+        g, _, cfg, view = setup(device, name, W, H, 1 << 10)
+        cam = scenes.camera(view, 0)
+        c = wsrt.Config.from_buffer_copy(bytes(cfg))
+        c.enable_photons = 0
+        c.frames_in_flight = 1
+        r = wsrt.Renderer(device, g, W, H, c)
+        r.set_camera(cam)
+        r.render_frames(spp)
+        refs.append(r.read_accum())
+        jobs.append((g, cam, W, H, spp))
+    bcfg = wsrt.default_config(enable_photons=0)
+    b = wsrt.Batch(device, jobs, bcfg)
+    b.set_tuning(tile_mode, 2)
+    b.render()
+    b.render()  # the graph replay restarts every job at frame 0
+    b.sync()
+    for k in range(len(specs)):
+        np.testing.assert_array_equal(b.read_accum(k), refs[k])
+        assert b.read_rgba8(k)[:, 3].min() == 255
+    tune = b.autotune()
+    assert tune["tile_mode"] in (0, 1) and tune["blocks_per_cu"] in (1, 2, 4, 8)
+    b.render()
+    b.sync()
+    for k in range(len(specs)):
+        np.testing.assert_array_equal(b.read_accum(k), refs[k])
+
+
+def test_batch_rejects_photons_and_guiding(device, cornell_pair):
+    g, _ = cornell_pair
+    cam = scenes.camera(scenes.DEFAULT_VIEW, 0)
+    with pytest.raises(wsrt.WsrtError):
+        wsrt.Batch(device, [(g, cam, 8, 8, 1)], wsrt.default_config(enable_photons=1))

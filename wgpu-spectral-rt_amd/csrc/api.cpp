@@ -66,6 +66,20 @@ struct DevBuf {
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
     ~DevBuf() { release(); }
     void release() {
         if (p) (void)hipFree(p);
@@ -1913,5 +1927,247 @@ extern "C" int wsrt_autotune(wsrt_device* dev, const wsrt_scene* scene, uint32_t
     if (best_ms == 0.0f) return fail(WSRT_E_OOM, "no frames_in_flight candidate fits the memory budget");
     out->best = best;
     cfg->frames_in_flight = best;
+    return WSRT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// persistent multi-scene batch
+// ---------------------------------------------------------------------------
+struct wsrt_batch {
+    wsrt_device* dev = nullptr;
+    hipStream_t stream = nullptr;
+    wsrt_config cfg{};
+    std::vector<wsrt_batch_job> jobs;
+    std::vector<BatchJob> hjobs;  // host copies; first_item follows the tile mode
+    std::vector<DevBuf> accum, rgba;
+    DevBuf djobs, djobs_tune, table, head, gstack;
+    BatchArgs args{};
+    wsrt_batch_tuning tuning{1u, 2u};
+    int cus = 256;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static void batch_drop_graph(wsrt_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+static void batch_release(wsrt_batch* b) {
+    (void)hipSetDevice(b->dev->ordinal);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_drop_graph(b);
+    for (const wsrt_batch_job& J : b->jobs) const_cast<wsrt_scene*>(J.scene)->renderers--;
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    b->dev->live--;
+    delete b;
+}
+
+extern "C" int wsrt_batch_set_tuning(wsrt_batch* b, const wsrt_batch_tuning* t) {
+    CHECK_ARG(b && t && t->tile_mode <= 1u && t->blocks_per_cu >= 1u && t->blocks_per_cu <= 8u);
+    HIP_OR_FAIL(hipSetDevice(b->dev->ordinal));
+    uint32_t items = 0;
+    for (size_t j = 0; j < b->jobs.size(); ++j) {
+        BatchJob& B = b->hjobs[j];
+        B.first_item = items;
+        const uint64_t n = t->tile_mode == 0u ? ((uint64_t)B.width * B.height + 63u) / 64u
+                                              : (uint64_t)B.tiles_x * ((B.height + 7u) / 8u);
+        items += (uint32_t)n;
+    }
+    std::vector<BatchJob> tune = b->hjobs;
+    for (BatchJob& B : tune) B.frames = 1;
+    HIP_OR_FAIL(hipMemcpyAsync(b->djobs.p, b->hjobs.data(), sizeof(BatchJob) * b->hjobs.size(),
+                               hipMemcpyHostToDevice, b->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(b->djobs_tune.p, tune.data(), sizeof(BatchJob) * tune.size(), hipMemcpyHostToDevice,
+                               b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    b->args.n_items = items;
+    b->args.tile_mode = t->tile_mode;
+    b->tuning = *t;
+    batch_drop_graph(b);
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_batch_create(wsrt_device* dev, const wsrt_batch_job* jobs, uint32_t n_jobs,
+                                 const wsrt_config* cfg, wsrt_batch** out) {
+    CHECK_ARG(dev && jobs && out && n_jobs > 0 && n_jobs <= kMaxBatchJobs);
+    wsrt_config c;
+    if (cfg) c = *cfg;
+    else wsrt_config_default(&c);
+    int rc = validate_config(&c);
+    if (rc) return rc;
+    if (c.enable_photons || c.guiding)
+        return fail(WSRT_E_UNSUPPORTED, "batches trace without the photon pass and guiding");
+    uint64_t total_frames = 0;
+    uint32_t max_stack = 1;
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        const wsrt_batch_job& J = jobs[j];
+        CHECK_ARG(J.scene && J.scene->dev == dev && J.width > 0 && J.height > 0 && J.spp > 0);
+        if ((uint64_t)J.width * J.height > (1ull << 26)) return fail(WSRT_E_INVALID, "job %u too large", j);
+        if (c.env_map && !J.scene->view.env) return fail(WSRT_E_INVALID, "job %u: scene without environment", j);
+        total_frames += J.spp;
+        max_stack = std::max(max_stack, J.scene->view.stack_size);
+    }
+    if (total_frames > (1u << 20)) return fail(WSRT_E_INVALID, "too many samples per pixel over the jobs");
+    HIP_OR_FAIL(hipSetDevice(dev->ordinal));
+    wsrt_batch* b = new wsrt_batch();
+    b->dev = dev;
+    b->cfg = c;
+    b->jobs.assign(jobs, jobs + n_jobs);
+    for (const wsrt_batch_job& J : b->jobs) const_cast<wsrt_scene*>(J.scene)->renderers++;
+    dev->live++;
+    auto bail = [b](int code) {
+        batch_release(b);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(WSRT_E_HIP, "stream creation failed"));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev->ordinal) == hipSuccess && prop.multiProcessorCount > 0)
+        b->cus = prop.multiProcessorCount;
+    std::vector<FrameConst> table;
+    table.reserve(total_frames);
+    b->accum.resize(n_jobs);
+    b->rgba.resize(n_jobs);
+    b->hjobs.resize(n_jobs);
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        const wsrt_batch_job& J = jobs[j];
+        const size_t npx = (size_t)J.width * J.height;
+        if (b->accum[j].alloc(16 * npx) != hipSuccess || b->rgba[j].alloc(4 * npx) != hipSuccess ||
+            hipMemsetAsync(b->accum[j].p, 0, 16 * npx, b->stream) != hipSuccess)
+            return bail(fail(WSRT_E_OOM, "batch images"));
+        BatchJob& B = b->hjobs[j];
+        B.sv = J.scene->view;
+        if (!c.env_map) B.sv.env = nullptr;
+        B.accum = b->accum[j].as<float4>();
+        B.rgba = b->rgba[j].as<uint8_t>();
+        B.width = J.width;
+        B.height = J.height;
+        B.frames = J.spp;
+        B.table_base = (uint32_t)table.size();
+        B.tiles_x = (J.width + 7u) / 8u;
+        B.first_item = 0;
+        for (uint32_t f = 0; f < J.spp; ++f) {
+            wsrt_camera_uniform cam = J.camera;
+            cam.frame_count = f;
+            table.push_back(make_frame_const(c, cam, J.width, J.height));
+        }
+    }
+    const uint32_t spill = max_stack > (uint32_t)kShortStack ? max_stack - kShortStack : 1u;
+    const uint64_t lanes = (uint64_t)b->cus * 8u * 256u;  // the most blocks_per_cu allows
+    if (b->djobs.alloc(sizeof(BatchJob) * n_jobs) != hipSuccess ||
+        b->djobs_tune.alloc(sizeof(BatchJob) * n_jobs) != hipSuccess ||
+        b->table.alloc(sizeof(FrameConst) * table.size()) != hipSuccess || b->head.alloc(4) != hipSuccess ||
+        b->gstack.alloc(sizeof(uint2) * lanes * spill) != hipSuccess)
+        return bail(fail(WSRT_E_OOM, "batch buffers"));
+    if (hipMemcpyAsync(b->table.p, table.data(), sizeof(FrameConst) * table.size(), hipMemcpyHostToDevice,
+                       b->stream) != hipSuccess)
+        return bail(fail(WSRT_E_HIP, "frame table upload"));
+    b->args.jobs = b->djobs.as<BatchJob>();
+    b->args.njobs = n_jobs;
+    b->args.table = b->table.as<FrameConst>();
+    b->args.head = b->head.as<uint32_t>();
+    b->args.gstack = b->gstack.as<uint2>();
+    b->args.spill = spill;
+    b->args.max_depth = c.max_depth;
+    rc = wsrt_batch_set_tuning(b, &b->tuning);
+    if (rc) return bail(rc);
+    *out = b;
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_batch_destroy(wsrt_batch* b) {
+    if (!b) return WSRT_OK;
+    batch_release(b);
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_batch_render(wsrt_batch* b) {
+    CHECK_ARG(b);
+    HIP_OR_FAIL(hipSetDevice(b->dev->ordinal));
+    const uint32_t blocks = (uint32_t)b->cus * b->tuning.blocks_per_cu;
+    if (!b->exec) {
+        if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            launch_batch(b->args, blocks, b->stream);
+            if (hipStreamEndCapture(b->stream, &b->graph) != hipSuccess ||
+                hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
+                batch_drop_graph(b);
+        }
+        (void)hipGetLastError();
+    }
+    if (b->exec) HIP_OR_FAIL(hipGraphLaunch(b->exec, b->stream));
+    else launch_batch(b->args, blocks, b->stream);
+    HIP_OR_FAIL(hipGetLastError());
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_batch_sync(wsrt_batch* b) {
+    CHECK_ARG(b);
+    HIP_OR_FAIL(hipSetDevice(b->dev->ordinal));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_batch_autotune(wsrt_batch* b, wsrt_batch_tuning* best, float* best_ms) {
+    CHECK_ARG(b);
+    HIP_OR_FAIL(hipSetDevice(b->dev->ordinal));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_OR_FAIL(hipEventCreate(&e0));
+    HIP_OR_FAIL(hipEventCreate(&e1));
+    wsrt_batch_tuning bt = b->tuning;
+    float bms = 0.0f;
+    int rc = WSRT_OK;
+    for (uint32_t mode = 0; mode < 2 && rc == WSRT_OK; ++mode) {
+        for (uint32_t bpc = 1; bpc <= 8 && rc == WSRT_OK; bpc *= 2) {
+            const wsrt_batch_tuning t{mode, bpc};
+            rc = wsrt_batch_set_tuning(b, &t);
+            if (rc) break;
+            BatchArgs a = b->args;
+            a.jobs = b->djobs_tune.as<BatchJob>();  // one sample per pixel
+            const uint32_t blocks = (uint32_t)b->cus * bpc;
+            launch_batch(a, blocks, b->stream);  // warm
+            (void)hipEventRecord(e0, b->stream);
+            launch_batch(a, blocks, b->stream);
+            (void)hipEventRecord(e1, b->stream);
+            float ms = 0.0f;
+            if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = fail(WSRT_E_HIP, "batch autotune timing");
+                break;
+            }
+            if (bms == 0.0f || ms < bms) {
+                bms = ms;
+                bt = t;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    rc = wsrt_batch_set_tuning(b, &bt);
+    if (rc) return rc;
+    // the tuning passes wrote frame-0 images: clear them for the real render
+    for (size_t j = 0; j < b->jobs.size(); ++j)
+        HIP_OR_FAIL(hipMemsetAsync(b->accum[j].p, 0, b->accum[j].bytes, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    if (best) *best = bt;
+    if (best_ms) *best_ms = bms;
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_batch_read_accum(wsrt_batch* b, uint32_t job, float* out) {
+    CHECK_ARG(b && out && job < b->jobs.size());
+    HIP_OR_FAIL(hipSetDevice(b->dev->ordinal));
+    HIP_OR_FAIL(hipMemcpyAsync(out, b->accum[job].p, b->accum[job].bytes, hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_batch_read_rgba8(wsrt_batch* b, uint32_t job, uint8_t* out) {
+    CHECK_ARG(b && out && job < b->jobs.size());
+    HIP_OR_FAIL(hipSetDevice(b->dev->ordinal));
+    HIP_OR_FAIL(hipMemcpyAsync(out, b->rgba[job].p, b->rgba[job].bytes, hipMemcpyDeviceToHost, b->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(b->stream));
     return WSRT_OK;
 }

@@ -1,0 +1,98 @@
+// batch.hip - many scenes and views of uneven size in one persistent launch.
+//
+// A work item is one 64-pixel tile of one job (a scene, a camera, a
+// resolution and a sample count); items are numbered job by job. A fixed
+// grid of blocks stays resident and each wave pulls the next item from a
+// device counter until all are taken, so large and small jobs, cheap and
+// expensive pixels, spread over the whole chip without a launch per job.
+// Each lane traces its pixel's samples one after the other with the same
+// per-path steps as the wavefront renderer (path_core.hpp), accumulating in
+// frame order, so a job's image equals the wavefront renderer's bit for bit.
+#include "kernels.hpp"
+#include "path_core.hpp"
+#include "traverse.hpp"
+
+namespace wsrt {
+namespace {
+
+constexpr int kBatchThreads = 256;
+
+__global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
+    __shared__ uint2 lds_stack[kShortStack * kBatchThreads];
+    __shared__ uint32_t first[kMaxBatchJobs + 1];
+    for (uint32_t j = threadIdx.x; j <= a.njobs; j += blockDim.x)
+        first[j] = j < a.njobs ? a.jobs[j].first_item : a.n_items;
+    __syncthreads();
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = __lane_id();
+    Stack st;
+    st.lds = lds_slot(lds_stack + threadIdx.x);
+    st.lds_stride = kBatchThreads;
+    st.glob = a.gstack + (uint64_t)gtid * a.spill;
+    uint32_t dn = 0, dt = 0;
+    while (true) {
+        uint32_t item = 0;
+        if (lane == 0) item = atomicAdd(a.head, 1u);
+        item = __shfl(item, 0, 64);
+        if (item >= a.n_items) break;
+        // job of the item: last j with first[j] <= item
+        uint32_t lo = 0, hi = a.njobs;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (first[mid] <= item) lo = mid;
+            else hi = mid;
+        }
+        const BatchJob& J = a.jobs[lo];
+        const uint32_t local = item - first[lo];
+        uint32_t x, y;
+        if (a.tile_mode == 0u) {  // 64 x 1 pixels
+            const uint32_t pix = local * 64u + (uint32_t)lane;
+            x = pix % J.width;
+            y = pix / J.width;
+        } else {  // 8 x 8 pixels
+            const uint32_t tx = local % J.tiles_x, ty = local / J.tiles_x;
+            x = tx * 8u + ((uint32_t)lane & 7u);
+            y = ty * 8u + ((uint32_t)lane >> 3);
+        }
+        if (x >= J.width || y >= J.height) continue;
+        const uint32_t pix = y * J.width + x;
+        const SceneView& sv = J.sv;
+        const GridView none{};
+        float4 acc = J.accum[pix];
+        for (uint32_t f = 0; f < J.frames; ++f) {
+            const FrameConst& fc = a.table[J.table_base + f];
+            PathRegs r;
+            camera_path(fc, x, y, r);
+            for (uint32_t b = 0; b < a.max_depth; ++b) {
+                const Hit h = trace_ray<false>(sv, xyz(r.o4), xyz(r.d4), 0.0001f, 100.0f, st, false, dn, dt);
+                if (h.tri == kInvalid) {
+                    if (sv.env) r.L = add4(r.L, mul4(r.tp, env_radiance(sv, xyz(r.d4), r.wl)));
+                    break;
+                }
+                const VertexOut vo = shade_vertex(sv, none, nullptr, 0u, fc.rr_after, b, false, h.t, h.tri,
+                                                  h.front, h.u, h.v, r);
+                if (vo.shadow) {
+                    const Hit s = trace_ray<true>(sv, vo.hp, vo.sdir, 0.001f, vo.stmax, st, false, dn, dt);
+                    if (s.tri == kInvalid) r.L = add4(r.L, vo.contrib);
+                }
+                if (!vo.alive) break;
+            }
+            acc = accumulate_sample(acc, fc, r.wl, r.L);
+        }
+        J.accum[pix] = acc;
+        reinterpret_cast<uchar4*>(J.rgba)[pix] = tone_map(acc);
+    }
+}
+
+__global__ void k_batch_reset(uint32_t* head) {
+    if (threadIdx.x == 0) *head = 0u;
+}
+
+}  // namespace
+
+void launch_batch(const BatchArgs& a, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.head);
+    hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(kBatchThreads), 0, s, a);
+}
+
+}  // namespace wsrt

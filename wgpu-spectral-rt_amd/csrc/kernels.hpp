@@ -118,6 +118,31 @@ void launch_guide_prepare(const PathState& ps, const GuideView& gv, const FrameA
 void launch_guide_infer(const GuideView& gv, const float4* in0, const float4* in1, uint32_t n, float* out,
                         hipStream_t s);
 
+// ---- persistent multi-scene batch (batch.hip) ---------------------------------
+constexpr uint32_t kMaxBatchJobs = 256;
+struct BatchJob {
+    SceneView sv;          // env null unless the config lights misses with it
+    float4* accum;         // width x height
+    uint8_t* rgba;         // width x height x 4
+    uint32_t width, height;
+    uint32_t frames;       // samples per pixel rendered by one batch launch
+    uint32_t table_base;   // FrameConst of frame f at table[table_base + f]
+    uint32_t first_item;   // first work item of the job
+    uint32_t tiles_x;      // 8 x 8 tiles per row
+};
+struct BatchArgs {
+    const BatchJob* jobs;
+    uint32_t njobs;
+    const FrameConst* table;
+    uint32_t n_items;      // work items over all jobs
+    uint32_t* head;        // next work item (device counter)
+    uint2* gstack;         // per-lane traversal stack spill
+    uint32_t spill;        // spill entries per lane
+    uint32_t max_depth;
+    uint32_t tile_mode;    // 0: 64 x 1 pixel items, 1: 8 x 8 pixel items
+};
+void launch_batch(const BatchArgs& a, uint32_t blocks, hipStream_t s);
+
 // ---- wavefront camera paths (path.hip) ----------------------------------------
 struct PathLaunch {
     SceneView sv;

@@ -16,13 +16,13 @@
 // bounce), so each path's result matches the per-pixel reference.
 #include "kernels.hpp"
 #include "queues.hpp"
+#include "path_core.hpp"
 #include "shading.hpp"
 
 namespace wsrt {
 
 namespace {
 
-constexpr uint32_t kFlagDiffuse = 1u, kFlagCaustic = 2u;
 constexpr int kShadeThreads = 256;
 constexpr int kShadeBlocks = (int)(kGridLanes / kShadeThreads);
 static_assert((uint64_t)kTraceBlocks * kTraceThreads == kGridLanes, "queue producers use kGridLanes");
@@ -49,47 +49,15 @@ __global__ __launch_bounds__(256) void k_generate(PathState ps, FrameArgs fa, ui
     if (p >= npx * nf) return;
     const uint32_t slot = p / npx, pix = p - slot * npx;
     const FrameConst& fc = fa.table[(*fa.frame + slot) & fa.mask];
-    const uint32_t x = pix % fc.width, y = pix / fc.width;
-    uint32_t rng = pcg_hash(x + y * fc.width + fc.frame * 927163u);
-    float jx = rand01(rng), jy = rand01(rng);
-    float ux = ((float)x + jx) / (float)fc.width, uy = ((float)y + jy) / (float)fc.height;
-    float v[4] = {ux * 2.0f - 1.0f, uy * 2.0f - 1.0f, 1.0f, 1.0f};
-    float tw[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        float s = fc.M[0 + i] * v[0];
-        s = s + fc.M[4 + i] * v[1];
-        s = s + fc.M[8 + i] * v[2];
-        s = s + fc.M[12 + i] * v[3];
-        tw[i] = s;
-    }
-    f3 origin = mk3(fc.origin[0], fc.origin[1], fc.origin[2]);
-    f3 dir = normalize(mk3(tw[0] / tw[3], tw[1] / tw[3], tw[2] / tw[3]) - origin);
-    float u = rand01(rng);
-    float rot = rand01(rng);
-    const CauchyConst cc{fc.cdf_min, fc.cdf_range};
-    float hero = cauchy_sample(u, cc);
-    const float q = kLambdaRange / 4.0f;
-    float4 w = make_float4(hero, hero + q * 1.0f, hero + q * 2.0f, hero + q * 3.0f);
-    w.x = kLambdaMin + fmod_wgsl(w.x - kLambdaMin, kLambdaRange);
-    w.y = kLambdaMin + fmod_wgsl(w.y - kLambdaMin, kLambdaRange);
-    w.z = kLambdaMin + fmod_wgsl(w.z - kLambdaMin, kLambdaRange);
-    w.w = kLambdaMin + fmod_wgsl(w.w - kLambdaMin, kLambdaRange);
-    if (rot < 0.25f) {
-    } else if (rot < 0.5f) {
-        w = make_float4(w.y, w.z, w.w, w.x);
-    } else if (rot < 0.75f) {
-        w = make_float4(w.z, w.w, w.x, w.y);
-    } else {
-        w = make_float4(w.w, w.x, w.y, w.z);
-    }
-    ps.o[p] = f4(origin, 0.0f);
-    ps.d[p] = f4(dir, 0.0f);
-    ps.tp[p] = splat4(1.0f);
-    ps.L[p] = splat4(0.0f);
-    ps.wl[p] = w;
-    ps.rng[p] = rng;
-    ps.flags[p] = 0u;
+    PathRegs r;
+    camera_path(fc, pix % fc.width, pix / fc.width, r);
+    ps.o[p] = r.o4;
+    ps.d[p] = r.d4;
+    ps.tp[p] = r.tp;
+    ps.L[p] = r.L;
+    ps.wl[p] = r.wl;
+    ps.rng[p] = r.rng;
+    ps.flags[p] = r.flags;
 }
 
 // Bounce 0 reads path ids 0 .. n0-1 directly (every path starts); later
@@ -170,105 +138,47 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = gtid & ~63u; base < total; base += stride) {
         const uint32_t i = base + __lane_id();
-        bool alive = false;
-        bool have = i < total;
+        const bool have = i < total;
         uint32_t p = 0;
-        float4 o4 = splat4(0.0f), d4 = splat4(0.0f), tp = splat4(0.0f), L = splat4(0.0f);
-        uint32_t rng = 0, flags = 0;
-        bool shadow = false, deferred = false;
-        f3 hp = mk3(0, 0, 0), sdir = mk3(0, 0, 0);
-        float stmax = 0.0f;
-        float4 contrib = splat4(0.0f);
+        VertexOut vo;
+        vo.alive = vo.shadow = vo.deferred = false;
         if (have) {
             p = ps.shade_q[seg_entry(pre, kMatBins * kSegs, i, qs.seg_cap)];
             const uint4 hr = ps.hit[p];
-            const float t = __uint_as_float(hr.x);
-            const uint32_t tri = hr.y & 0x7FFFFFFFu;
-            const bool front = (hr.y >> 31) != 0u;
-            o4 = ps.o[p];
-            d4 = ps.d[p];
-            tp = ps.tp[p];
-            L = ps.L[p];
-            const float4 wl = ps.wl[p];
-            rng = ps.rng[p];
-            flags = ps.flags[p];
-            uint32_t mat_id;
-            f3 wn = hit_world_normal(sv, tri, __uint_as_float(hr.z), __uint_as_float(hr.w), &mat_id);
-            f3 ff = front ? wn : -wn;
-            const wsrt_material& m = sv.materials[mat_id];
-            const float4 col = *reinterpret_cast<const float4*>(m.color);
-            const float4 ex = *reinterpret_cast<const float4*>(m.extra);
-            const uint32_t type = (uint32_t)ex.x;
-            const f3 o = xyz(o4), d = xyz(d4);
-            if (type == 3u) {  // emitter (shader.wgsl:586-631); the MIS weight is not applied
-                if (!(flags & kFlagCaustic)) {
-                    float4 em = mul4s(blackbody4(wl, col.x), col.y);
-                    if (front || ex.y > 0.0f) L = add4(L, mul4(em, tp));
-                }
-            } else {
-                if (type == 0u) {
-                    flags = (flags | kFlagDiffuse) & ~kFlagCaustic;
-                } else if (type == 2u) {
-                    if (flags & kFlagDiffuse) flags |= kFlagCaustic;
-                }
-                hp = o + d * t;
-                if (type == 0u) {
-                    float4 spec = rgb_to_spectrum4(mk3(col.x, col.y, col.z), wl);
-                    if (photons_on) {
-                        const uint32_t slot = p / npx;
-                        const FrameConst& fs = fa.table[(base_frame + slot) & fa.mask];
-                        float4 g = gather_photons(grid, fs, slot * fs.hash_size, hp, ff, wl);
-                        L = add4(L, mul4(mul4(tp, div4s(spec, kPi)), g));
-                    }
-                    LightSample ls = sample_light(sv, hp, wl, rng);
-                    if (ls.pdf > 0.0f) {
-                        float ct = fmaxf(dot(ff, ls.dir), 0.0f);
-                        float4 be = div4s(spec, kPi);
-                        float bp = ct / kPi;
-                        float w = ls.delta ? 1.0f / ls.pdf : (ls.pdf / (ls.pdf + bp)) / ls.pdf;
-                        contrib = mul4s(mul4s(mul4(mul4(tp, be), ls.radiance), ct), w);
-                        shadow = true;
-                        sdir = ls.dir;
-                        stmax = ls.dist - 0.001f;
-                    }
-                }
-                if (type == 0u && guided) {
-                    // guided scattering of this Lambert vertex follows in
-                    // k_guide_scatter, after the vertex's shadow ray
-                    deferred = true;
-                    ps.gn[p] = f4(ff, 0.0f);
-                    ps.galb[p] = rgb_to_spectrum4(mk3(col.x, col.y, col.z), wl);
-                    o4 = f4(hp, 0.0f);
-                }
-                Scatter sc;
-                sc.absorbed = true;
-                if (!deferred) sc = sample_bsdf(d, ff, m, front, wl, tp, rng);
-                if (!sc.absorbed) {
-                    tp = mul4(tp, sc.att);
-                    o4 = f4(hp, 0.0f);
-                    d4 = f4(sc.dir, 0.0f);
-                    alive = true;
-                    if (bounce > rr_after) {
-                        float pr = fmaxf(fmaxf(tp.x, tp.y), fmaxf(tp.z, tp.w));
-                        if (rand01(rng) > pr) alive = false;
-                        else tp = div4s(tp, pr);
-                    }
-                }
+            PathRegs r;
+            r.o4 = ps.o[p];
+            r.d4 = ps.d[p];
+            r.tp = ps.tp[p];
+            r.L = ps.L[p];
+            r.wl = ps.wl[p];
+            r.rng = ps.rng[p];
+            r.flags = ps.flags[p];
+            const uint32_t slot = p / npx;
+            const FrameConst* fs = photons_on ? &fa.table[(base_frame + slot) & fa.mask] : nullptr;
+            const uint32_t cell_base = photons_on ? slot * fs->hash_size : 0u;
+            vo = shade_vertex(sv, grid, fs, cell_base, rr_after, bounce, guided != 0u,
+                              __uint_as_float(hr.x), hr.y & 0x7FFFFFFFu, (hr.y >> 31) != 0u,
+                              __uint_as_float(hr.z), __uint_as_float(hr.w), r);
+            ps.L[p] = r.L;
+            if (vo.deferred) {
+                ps.gn[p] = f4(vo.ff, 0.0f);
+                ps.galb[p] = vo.galb;
+                ps.o[p] = r.o4;
+                ps.rng[p] = r.rng;
+                ps.flags[p] = r.flags;
             }
-            ps.L[p] = L;
-            if (deferred) {
-                ps.o[p] = o4;
-                ps.rng[p] = rng;
-                ps.flags[p] = flags;
-            }
-            if (alive) {
-                ps.o[p] = o4;
-                ps.d[p] = d4;
-                ps.tp[p] = tp;
-                ps.rng[p] = rng;
-                ps.flags[p] = flags;
+            if (vo.alive) {
+                ps.o[p] = r.o4;
+                ps.d[p] = r.d4;
+                ps.tp[p] = r.tp;
+                ps.rng[p] = r.rng;
+                ps.flags[p] = r.flags;
             }
         }
+        const bool shadow = vo.shadow, alive = vo.alive, deferred = vo.deferred;
+        const f3 hp = vo.hp, sdir = vo.sdir;
+        const float stmax = vo.stmax;
+        const float4 contrib = vo.contrib;
         const uint32_t se = seg_append(qs, bounce, kQShadow, qs.seg_cap, shadow);
         if (shadow) {
             float4* r = ps.shadow + 3ull * se;
@@ -324,30 +234,10 @@ __global__ __launch_bounds__(256) void k_accumulate(PathState ps, FrameArgs fa, 
     for (uint32_t s = 0; s < nf; ++s) {
         const FrameConst& fc = fa.table[(base_frame + s) & fa.mask];
         const uint64_t p = (uint64_t)s * npx + pix;
-        const float4 wl = ps.wl[p], L = ps.L[p];
-        f3 xz = mk3(0, 0, 0);
-        xz = xz + cie_xyz(wl.x) * L.x;
-        xz = xz + cie_xyz(wl.y) * L.y;
-        xz = xz + cie_xyz(wl.z) * L.z;
-        xz = xz + cie_xyz(wl.w) * L.w;
-        const CauchyConst cc{fc.cdf_min, fc.cdf_range};
-        float sp = cauchy_pdf(wl.x, cc) + cauchy_pdf(wl.y, cc) + cauchy_pdf(wl.z, cc) + cauchy_pdf(wl.w, cc);
-        xz = xz * (1.0f / sp);
-        f3 c = xyz_to_srgb(xz);
-        float4 cur = fc.frame > 0u ? na : splat4(0.0f);
-        na = add4(cur, make_float4(c.x, c.y, c.z, 1.0f));
+        na = accumulate_sample(na, fc, ps.wl[p], ps.L[p]);
     }
     accum[pix] = na;
-    f3 fcol = mk3(fmaxf(na.x / na.w * 1.0f, 0.0f), fmaxf(na.y / na.w * 1.0f, 0.0f),
-                  fmaxf(na.z / na.w * 1.0f, 0.0f));
-    float luma = dot(fcol, mk3(0.2126f, 0.7152f, 0.0722f));
-    float ml = luma / (luma + 1.0f);
-    if (luma > 0.00001f) fcol = fcol * (ml / luma);
-    uchar4 o;
-    o.x = (uint8_t)__float2int_rn(fminf(fmaxf(fcol.x, 0.0f), 1.0f) * 255.0f);
-    o.y = (uint8_t)__float2int_rn(fminf(fmaxf(fcol.y, 0.0f), 1.0f) * 255.0f);
-    o.z = (uint8_t)__float2int_rn(fminf(fmaxf(fcol.z, 0.0f), 1.0f) * 255.0f);
-    o.w = 255;
+    const uchar4 o = tone_map(na);
     reinterpret_cast<uchar4*>(rgba)[pix] = o;
 }
 

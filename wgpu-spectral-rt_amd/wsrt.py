@@ -51,6 +51,15 @@ class AutotuneResult(C.Structure):
                 ("ms_per_frame", C.c_float * 8), ("device_bytes", C.c_uint64 * 8), ("best", C.c_uint32)]
 
 
+class BatchJob(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("camera", CameraUniform), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("spp", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("tile_mode", C.c_uint32), ("blocks_per_cu", C.c_uint32)]
+
+
 class SceneInfo(C.Structure):
     _fields_ = [("n_triangles", C.c_uint32), ("n_instances", C.c_uint32),
                 ("n_lights", C.c_uint32), ("n_materials", C.c_uint32), ("n_nodes", C.c_uint32),
@@ -142,6 +151,14 @@ _SIGS = {
     "wsrt_guide_train_step": (C.c_int, [vp, vp, vp, vp, u32, C.c_float]),
     "wsrt_renderer_get_guide": (C.c_int, [vp, P(vp)]),
     "wsrt_autotune": (C.c_int, [vp, vp, u32, u32, vp, P(Config), C.c_uint64, P(AutotuneResult)]),
+    "wsrt_batch_create": (C.c_int, [vp, vp, u32, P(Config), P(vp)]),
+    "wsrt_batch_destroy": (C.c_int, [vp]),
+    "wsrt_batch_set_tuning": (C.c_int, [vp, P(BatchTuning)]),
+    "wsrt_batch_autotune": (C.c_int, [vp, P(BatchTuning), P(C.c_float)]),
+    "wsrt_batch_render": (C.c_int, [vp]),
+    "wsrt_batch_sync": (C.c_int, [vp]),
+    "wsrt_batch_read_accum": (C.c_int, [vp, u32, vp]),
+    "wsrt_batch_read_rgba8": (C.c_int, [vp, u32, vp]),
 }
 
 GUIDE_PARAMS = 27952
@@ -667,3 +684,58 @@ def autotune(device: "Device", scene: "Scene", width: int, height: int, config: 
             "candidates": [{"frames_in_flight": res.frames_in_flight[i],
                             "ms_per_frame": round(res.ms_per_frame[i], 4),
                             "device_mib": round(res.device_bytes[i] / 2**20, 1)} for i in range(n)]}
+
+
+class Batch:
+    """Jobs (scene, camera, width, height, spp) rendered by one persistent
+    launch (wsrt_batch_*); photon pass and guiding off."""
+
+    def __init__(self, device: "Device", jobs, config: Config | None = None):
+        arr = (BatchJob * len(jobs))()
+        self.sizes = []
+        self._scenes = []
+        for k, (scene, cam, w, h, spp) in enumerate(jobs):
+            arr[k].scene = scene.h
+            arr[k].camera = cam
+            arr[k].width, arr[k].height, arr[k].spp = w, h, spp
+            self.sizes.append((w, h))
+            self._scenes.append(scene)
+        cfg = config if config is not None else default_config(enable_photons=0)
+        h = vp()
+        _check(load().wsrt_batch_create(device.h, C.cast(arr, vp), len(jobs), C.byref(cfg), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().wsrt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_tuning(self, tile_mode: int, blocks_per_cu: int):
+        t = BatchTuning(tile_mode, blocks_per_cu)
+        _check(load().wsrt_batch_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self) -> dict:
+        t, ms = BatchTuning(), C.c_float()
+        _check(load().wsrt_batch_autotune(self.h, C.byref(t), C.byref(ms)))
+        return {"tile_mode": t.tile_mode, "blocks_per_cu": t.blocks_per_cu, "one_spp_ms": ms.value}
+
+    def render(self):
+        _check(load().wsrt_batch_render(self.h))
+
+    def sync(self):
+        _check(load().wsrt_batch_sync(self.h))
+
+    def read_accum(self, job: int) -> np.ndarray:
+        w, h = self.sizes[job]
+        out = np.zeros((w * h, 4), np.float32)
+        _check(load().wsrt_batch_read_accum(self.h, job, _p(out)))
+        return out
+
+    def read_rgba8(self, job: int) -> np.ndarray:
+        w, h = self.sizes[job]
+        out = np.zeros((w * h, 4), np.uint8)
+        _check(load().wsrt_batch_read_rgba8(self.h, job, _p(out)))
+        return out
