@@ -208,6 +208,68 @@ def time_to_relmse(frames: int = 32, warmup: int = 64, boxes: int | None = None,
     return out
 
 
+def heightfield_throughput(frames: int = 16) -> dict:
+    """Film-coated GGX heightfield (2,093,058 triangles) under the seeded sky,
+    1920 x 1080, depth 6, `frames` samples per pixel, plain tracer."""
+    import scenes
+    import wsrt
+
+    dev = wsrt.Device(0)
+    t = time.perf_counter()
+    scene = scenes.build_heightfield(wsrt.SceneBuilder()).build(dev)
+    host_s = time.perf_counter() - t
+    v = scenes.HEIGHTFIELD_VIEW
+    cfg = scenes.heightfield_config()
+    tune = wsrt.autotune(dev, scene, v.width, v.height, cfg, scenes.camera(v, 0))
+    r = wsrt.Renderer(dev, scene, v.width, v.height, cfg)
+    r.set_camera(scenes.camera(v, 0))
+    r.render_frames(cfg.frames_in_flight)  # captures the batch graph
+    r.sync()
+    r.frame_count = 0
+    t = time.perf_counter()
+    r.render_frames(frames)
+    r.sync()
+    dt = time.perf_counter() - t
+    r.close()
+    return {"workload": "heightfield 1024x1024 vertices (film-coated GGX tiles, sky environment), 1920x1080, "
+                        f"{frames} spp, depth 6", "triangles": scene.info.n_triangles,
+            "Msamples_per_s": round(v.width * v.height * frames / dt / 1e6, 3),
+            "ms_per_frame": round(1000 * dt / frames, 3), "frames_in_flight": cfg.frames_in_flight,
+            "autotune": tune, "scene_build_s": round(host_s, 2), "bvh_build_ms": round(scene.info.build_ms, 2)}
+
+
+def batch_throughput(n_scenes: int = 64) -> dict:
+    """64 procedurally generated scenes of uneven size (triangles log-uniform
+    1e3 .. 2e6, 128^2 .. 1024^2 pixels, 8 .. 64 spp) in one persistent launch."""
+    import scenes
+    import wsrt
+
+    dev = wsrt.Device(0)
+    jobs, tris, samples = [], 0, 0
+    t = time.perf_counter()
+    for k in range(n_scenes):
+        b, v, spp = scenes.build_batch_scene(wsrt.SceneBuilder(), k)
+        sc = b.build(dev)
+        jobs.append((sc, scenes.camera(v, 0), v.width, v.height, spp))
+        tris += sc.info.n_triangles
+        samples += v.width * v.height * spp
+    host_s = time.perf_counter() - t
+    cfg = wsrt.default_config(enable_photons=0, max_depth=8)
+    batch = wsrt.Batch(dev, jobs, cfg)
+    tune = batch.autotune()
+    batch.render()  # first call captures the graph
+    batch.sync()
+    t = time.perf_counter()
+    batch.render()
+    batch.sync()
+    dt = time.perf_counter() - t
+    batch.close()
+    return {"workload": f"{n_scenes} procedural scenes (displaced spheres and boxes, mixed materials), "
+                        "one persistent launch, depth 8", "scenes": n_scenes, "triangles": tris,
+            "samples": samples, "Msamples_per_s": round(samples / dt / 1e6, 3), "seconds": round(dt, 3),
+            "tuning": tune, "scene_build_s": round(host_s, 2)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +277,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-guided", action="store_true", help="skip the guided-vs-plain time-to-relMSE leg")
+    ap.add_argument("--no-other", action="store_true", help="skip the heightfield and batch throughput legs")
     ap.add_argument("--guided-only", action="store_true", help="only the guided-vs-plain leg (one JSON line)")
     ap.add_argument("--guided-frames", type=int, default=32)
     ap.add_argument("--frames-in-flight", type=int, default=0, help="frames per wavefront (default: config's)")
@@ -357,6 +420,12 @@ def main() -> int:
         "kernels": kern,
         "traversal_per_ray": per_ray,
     }
+    if not args.no_other:
+        log("heightfield ...")
+        r.close()
+        result["other_configs"] = {"heightfield": heightfield_throughput()}
+        log("batch ...")
+        result["other_configs"]["batch"] = batch_throughput()
     if not args.no_guided:
         log("guided vs plain time-to-relMSE ...")
         r.close()

@@ -18,6 +18,11 @@ drive any builder with that interface.
   custom        the reference's second scene (src/scene/custom.rs): a dark
                 room, a vertical slit light, a high-dispersion glass prism
                 (Cauchy B = 0.12), a thin-film bubble and a white board
+  batch_scene   one of the procedurally generated scenes of the batch
+                workload: seeded displaced icospheres and boxes on a floor
+                under an area light, triangle count log-uniform in
+                [1e3, 2e6], materials drawn from Lambert, GGX, dielectric
+                and thin film
   occluded_room two rooms split by a wall with a 0.05-wide full-height door
                 gap; the only light is on the ceiling of the far room, so the
                 near room is lit through the gap; both rooms are cluttered
@@ -376,3 +381,79 @@ def heightfield_config(**kw):
     for k, v in kw.items():
         setattr(c, k, v)
     return c
+
+
+_SPHERE_CACHE: dict = {}
+
+
+def _sphere(sub: int):
+    if sub not in _SPHERE_CACHE:
+        _SPHERE_CACHE[sub] = wsrt.create_sphere(sub)
+    return _SPHERE_CACHE[sub]
+
+
+def _displaced_sphere(sub: int, rng, amp: float):
+    """Icosphere(sub) with radial displacement 1 + amp * sum of 3 seeded
+    sinusoidal lobes of the direction; normals left radial."""
+    v, i = _sphere(sub)
+    v = v.copy()
+    p = v["pos"][:, :3].astype(np.float64)
+    d = p / np.linalg.norm(p, axis=1, keepdims=True)
+    disp = np.ones(len(p))
+    for _ in range(3):
+        k = rng.normal(size=3)
+        k *= rng.uniform(2.0, 6.0) / np.linalg.norm(k)
+        disp += amp * np.sin(d @ k + rng.uniform(0, 2 * math.pi)) / 3.0
+    v["pos"][:, :3] = (p * disp[:, None]).astype(np.float32)
+    return v, i
+
+
+def _random_material(b, rng):
+    kind = rng.integers(0, 4)
+    col = list(rng.uniform(0.2, 0.9, 3)) + [1.0]
+    if kind == 0:
+        return b.add_material(col, [0, 0, 0, 0])
+    if kind == 1:
+        return b.add_material(col, [1, float(rng.uniform(0.05, 0.8)), 0, 0])
+    if kind == 2:
+        return b.add_material([1.0, 1.0, 1.0, 1.0], [2, 0, float(rng.uniform(1.3, 1.8)), 0.0])
+    return b.add_material([1.0, 1.0, 1.0, 1.0], [4, float(rng.uniform(200, 800)), 1.33, 1.0])
+
+
+BATCH_RESOLUTIONS = (128, 256, 512, 1024)
+
+
+def build_batch_scene(b, index: int, seed: int = 11):
+    """Scene `index` of the batch workload. Returns (builder, view, spp)."""
+    rng = np.random.default_rng([seed, index])
+    target = float(np.exp(rng.uniform(math.log(1e3), math.log(2e6))))
+    floor = b.add_material([0.7, 0.7, 0.7, 1.0], [0, 0, 0, 0])
+    light = b.add_material([6500.0, 4.0, 0.0, 1.0], [3, 0, 0, 0])
+    plane = b.add_mesh(*wsrt.create_plane())
+    cube = b.add_mesh(*wsrt.create_cube())
+    b.add_instance(plane, floor, _trs((0, 0, 0), s=(8, 1, 8)))
+    b.add_light_instance(plane, light, _trs((0, 3.0, 0), "x", PI, (1.5, 1, 1.5)), [0.0, 6500.0, 4.0, 0.0])
+    tris, objects = 4, 0
+    while tris + 12 <= target and objects < 48:
+        left = target - tris
+        sub = int(min(8, max(0, math.floor(math.log(max(left / 20.0, 1.0), 4)))))
+        if 20 * 4**sub > left or rng.random() < 0.3:  # a box
+            r = float(rng.uniform(0.1, 0.5))
+            pos = (float(rng.uniform(-2.5, 2.5)), r, float(rng.uniform(-2.5, 2.5)))
+            b.add_instance(cube, _random_material(b, rng),
+                           _trs(pos, "y", float(rng.uniform(0, 2 * math.pi)), (2 * r, 2 * r, 2 * r)))
+            tris += 12
+        else:
+            v, i = _displaced_sphere(sub, rng, float(rng.uniform(0.0, 0.25)))
+            r = float(rng.uniform(0.2, 0.8))
+            pos = (float(rng.uniform(-2.5, 2.5)), r * 1.25, float(rng.uniform(-2.5, 2.5)))
+            b.add_instance(b.add_mesh(v, i), _random_material(b, rng), _trs(pos, s=(r, r, r)))
+            tris += 20 * 4**sub
+        objects += 1
+    res = int(BATCH_RESOLUTIONS[rng.integers(0, 4)])
+    spp = int(rng.integers(8, 65))
+    yaw = float(rng.uniform(-math.pi, math.pi))
+    dist = float(rng.uniform(4.0, 7.0))
+    pos = (-dist * math.cos(yaw), float(rng.uniform(1.5, 3.5)), -dist * math.sin(yaw))
+    pitch = -math.atan2(pos[1] - 0.5, dist)
+    return b, View(pos, yaw, pitch, res, res), spp
