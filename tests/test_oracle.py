@@ -318,3 +318,105 @@ def test_counting_sort_equals_linked_list():
                 p = nxt[p]
             assert list(seg) == sorted(chain)  # same set; the segment is in slot order
             assert np.all(cell[seg] == c)
+
+
+# ---- integrator closed forms ---------------------------------------------------
+PLANCK_H, PLANCK_C, PLANCK_K = 6.62607015e-34, 299792458.0, 1.380649e-23
+
+
+def planck_1e13(lam_nm, t):
+    lam = np.asarray(lam_nm, np.float64) * 1e-9
+    return 2 * PLANCK_H * PLANCK_C**2 / (lam**5 * np.expm1(PLANCK_H * PLANCK_C / (lam * PLANCK_K * t))) * 1e-13
+
+
+def _cauchy_pnorm():
+    cmin, crange, pnorm = (np.ctypeslib.ctypes.c_float() for _ in range(3))
+    O.lib.or_cauchy_constants(cmin, crange, pnorm)
+    return pnorm.value
+
+
+def _cie(lam):
+    out = np.zeros(3, np.float32)
+    O.lib.or_cie_xyz_f(float(lam), out)
+    return out.astype(np.float64)
+
+
+def test_first_hit_emission_and_accumulation_closed_form():
+    """A camera facing an emitter wall: each path's spectral radiance is
+    blackbody(lambda, T) * intensity at its four hero wavelengths, and the
+    pixel colour is XYZ (CIE fit) over the summed Cauchy pdf, to linear sRGB
+    (shader.wgsl:586-631, 771-799)."""
+    import wsrt
+
+    s = O.Scene()
+    light = s.add_material([6500.0, 2.5, 0.0, 1.0], [3, 0, 0, 0])
+    plane = s.add_mesh(*wsrt.create_plane())
+    m = wsrt.mat4_mul(wsrt.mat4_translation(0, 0, -1), wsrt.mat4_mul(wsrt.mat4_rotation("x", float(np.float32(np.pi / 2))),
+                                                                     wsrt.mat4_scale(40, 40, 40)))
+    s.add_light_instance(plane, light, m, [0.0, 6500.0, 2.5, 0.0])
+    sc = s.build()
+    cfg = O.default_config(enable_photons=0, max_depth=1)
+    f = O.Frame(sc, cfg, O.camera_uniform(), 16, 16)
+    wl, L, rgb = f.trace_pixels(np.arange(256, dtype=np.uint32))
+    want = planck_1e13(wl, 6500.0) * 2.5
+    np.testing.assert_allclose(L, want, rtol=2e-5)
+    # accumulation: sum_i cie(l_i) L_i / sum_i p(l_i), XYZ -> linear sRGB
+    pdf = _cauchy_pnorm() / (1 + ((wl.astype(np.float64) - 555.0) / 40.0) ** 2)
+    xyz = np.zeros((len(wl), 3))
+    for i in range(4):
+        xyz += np.array([_cie(lam) for lam in wl[:, i]]) * L[:, i:i + 1]
+    xyz /= pdf.sum(1, keepdims=True)
+    M = np.array([[3.2404542, -1.5371385, -0.4985314], [-0.9692660, 1.8760108, 0.0415560],
+                  [0.0556434, -0.2040259, 1.0572252]])
+    np.testing.assert_allclose(rgb, xyz @ M.T, rtol=2e-4, atol=1e-6)
+
+
+def test_point_light_direct_lighting_closed_form():
+    """Lambert floor (gray 0.5: a flat spectrum) lit by a point light at the
+    camera, one bounce: L = (0.5 / pi) B(lambda) I cos^3(phi) / h^2 with phi
+    the angle of the centre pixel's ray from the vertical (NEE of a delta
+    light has weight 1 / pdf = 1 / d^2, shader.wgsl:650-671)."""
+    import wsrt
+
+    s = O.Scene()
+    gray = s.add_material([0.5, 0.5, 0.5, 1.0], [0, 0, 0, 0])
+    s.add_instance(s.add_mesh(*wsrt.create_plane()), gray, wsrt.mat4_scale(50, 50, 50))
+    h, I = 2.0, 3.0
+    s.add_point_light([0.0, h, 0.0], [1.0, 6500.0, I, 0.0])
+    sc = s.build()
+    cfg = O.default_config(enable_photons=0, max_depth=1)
+    pitch = float(np.float32(-89.0 * np.pi / 180))
+    cam = O.camera_uniform(pos=(0.0, h, 0.0), pitch=pitch)
+    W = 101
+    f = O.Frame(sc, cfg, cam, W, W)
+    wl, L, _ = f.trace_pixels(np.array([50 + 50 * W], np.uint32))
+    phi = np.radians(1.0)
+    want = 0.5 / np.pi * planck_1e13(wl, 6500.0) * I * np.cos(phi) ** 3 / h**2
+    np.testing.assert_allclose(L, want, rtol=3e-4)
+
+
+def test_photon_gather_density_closed_form():
+    """Density estimate of shader.wgsl:297-342 for hand-placed photons in one
+    cell: E (1 - d/r) max(1 - |dl|/20, 0) 3/(pi r^2) / 20 / 2^20, photons
+    facing away from the normal or beyond the radius excluded."""
+    sc = O.cornell_box()
+    cfg = O.default_config()
+    ph = np.zeros(4, O.PHOTON_DTYPE)
+    p = np.array([0.005, 0.005, 0.005], np.float32)
+    ph[0] = (p, 550.0, (0.0, 1.0, 0.0), 2.0)                      # counted
+    ph[1] = (p + np.float32([0.0, 0.0, 0.004]), 565.0, (0.0, 1.0, 0.0), 3.0)  # counted
+    ph[2] = (p, 550.0, (0.0, -1.0, 0.0), 5.0)                     # faces away
+    ph[3] = (p + np.float32([0.0, 0.0, 0.0199]), 550.0, (0.0, 1.0, 0.0), 7.0)  # outside r once offset
+    f = O.Frame(sc, cfg, O.camera_uniform(), 8, 8, photons=ph)
+    q = p + np.float32([0.003, 0.0, 0.0])
+    wl = np.array([550.0, 560.0, 575.0, 590.0], np.float32)
+    got = f.gather(q, np.array([0.0, 1.0, 0.0], np.float32), wl)
+    r = 0.02
+    norm = 3.0 / (np.pi * r * r) / 20.0 / 2**20
+    want = np.zeros(4)
+    for pos, lam, e in ((ph[0]["position"], 550.0, 2.0), (ph[1]["position"], 565.0, 3.0)):
+        d = np.linalg.norm(q.astype(np.float64) - pos)
+        assert d < r
+        want += e * (1 - d / r) * np.maximum(1 - np.abs(wl - lam) / 20.0, 0) * norm
+    assert np.linalg.norm(q.astype(np.float64) - ph[3]["position"]) > r
+    np.testing.assert_allclose(got, want, rtol=1e-5)
