@@ -40,9 +40,9 @@ def cpu_baseline(seconds: float = 12.0) -> dict:
     import scenes
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    scene = scenes.build_c2_caustic(O.Scene()).build()
-    cfg = O.Config.from_buffer_copy(bytes(scenes.c2_config()))
-    v = scenes.C2_VIEW
+    scene = scenes.build_caustic(O.Scene()).build()
+    cfg = O.Config.from_buffer_copy(bytes(scenes.caustic_config()))
+    v = scenes.CAUSTIC_VIEW
     acc = None
     frames, t0 = 0, time.perf_counter()
     while True:
@@ -238,6 +238,33 @@ def heightfield_throughput(frames: int = 16) -> dict:
             "autotune": tune, "scene_build_s": round(host_s, 2), "bvh_build_ms": round(scene.info.build_ms, 2)}
 
 
+def small_box_throughput(frames: int = 16) -> dict:
+    """34-triangle Cornell-style box (5 walls, two seeded boxes, one GGX),
+    0.25 x 0.25 area light, 256 x 256, 16 spp, depth 8, photon pass on."""
+    import scenes
+    import wsrt
+
+    dev = wsrt.Device(0)
+    scene = scenes.build_small_box(wsrt.SceneBuilder()).build(dev)
+    v = scenes.SMALL_BOX_VIEW
+    cfg = scenes.small_box_config()
+    tune = wsrt.autotune(dev, scene, v.width, v.height, cfg, scenes.camera(v, 0))
+    r = wsrt.Renderer(dev, scene, v.width, v.height, cfg)
+    r.set_camera(scenes.camera(v, 0))
+    r.render_frames(cfg.frames_in_flight)
+    r.sync()
+    r.frame_count = 0
+    t = time.perf_counter()
+    r.render_frames(frames)
+    r.sync()
+    dt = time.perf_counter() - t
+    r.close()
+    return {"workload": "Cornell-style box, 34 triangles + area light, 256x256, 16 spp, depth 8, 2^20 photons/frame",
+            "triangles": scene.info.n_triangles, "Msamples_per_s": round(v.width * v.height * frames / dt / 1e6, 3),
+            "ms_per_frame": round(1000 * dt / frames, 3), "frames_in_flight": cfg.frames_in_flight,
+            "autotune_best": tune["best"]}
+
+
 def batch_throughput(n_scenes: int = 64) -> dict:
     """64 procedurally generated scenes of uneven size (triangles log-uniform
     1e3 .. 2e6, 128^2 .. 1024^2 pixels, 8 .. 64 spp) in one persistent launch."""
@@ -308,10 +335,10 @@ def main() -> int:
     import wsrt
 
     dev = wsrt.Device(0)
-    scene = scenes.build_c2_caustic(wsrt.SceneBuilder()).build(dev)
+    scene = scenes.build_caustic(wsrt.SceneBuilder()).build(dev)
     info = scene.info
-    v = scenes.C2_VIEW
-    cfg = scenes.c2_config()
+    v = scenes.CAUSTIC_VIEW
+    cfg = scenes.caustic_config()
     tune = None
     if args.frames_in_flight:
         cfg.frames_in_flight = args.frames_in_flight
@@ -423,7 +450,7 @@ def main() -> int:
     if not args.no_other:
         log("heightfield ...")
         r.close()
-        result["other_configs"] = {"heightfield": heightfield_throughput()}
+        result["other_configs"] = {"small_box": small_box_throughput(), "heightfield": heightfield_throughput()}
         log("batch ...")
         result["other_configs"]["batch"] = batch_throughput()
     if not args.no_guided:

@@ -32,9 +32,9 @@ def random_rays(n, seed, lo=-0.95, hi=0.95, tmin=1e-4, tmax=100.0):
     return r
 
 
-def c2_pair(device):
-    return scenes.build_c2_caustic(wsrt.SceneBuilder()).build(device), \
-        scenes.build_c2_caustic(O.Scene()).build()
+def caustic_pair(device):
+    return scenes.build_caustic(wsrt.SceneBuilder()).build(device), \
+        scenes.build_caustic(O.Scene()).build()
 
 
 def test_world_and_morton_bit_exact(device, cornell_pair):
@@ -49,9 +49,9 @@ def test_world_and_morton_bit_exact(device, cornell_pair):
     assert sorted(order.tolist()) == list(range(len(codes)))
 
 
-@pytest.mark.parametrize("scene", ["cornell", "c2"])
+@pytest.mark.parametrize("scene", ["cornell", "caustic"])
 def test_closest_and_any_hit_match_brute_force(device, cornell_pair, scene):
-    g, o = cornell_pair if scene == "cornell" else c2_pair(device)
+    g, o = cornell_pair if scene == "cornell" else caustic_pair(device)
     print(f"closest/any hit {scene}", flush=True)
     lo, hi = (-0.95, 0.95) if scene == "cornell" else (-1.5, 2.0)
     rays = random_rays(40000, 1, lo, hi)
@@ -85,9 +85,9 @@ NODE = np.dtype([("origin", np.float32, 3), ("exp", np.int8, 3), ("imask", np.ui
                  ("qlo", np.uint8, (3, 8)), ("qhi", np.uint8, (3, 8))])
 
 
-@pytest.mark.parametrize("scene", ["cornell", "c2"])
+@pytest.mark.parametrize("scene", ["cornell", "caustic"])
 def test_bvh8_structure_and_traversal_counts(device, cornell_pair, scene):
-    g, o = cornell_pair if scene == "cornell" else c2_pair(device)
+    g, o = cornell_pair if scene == "cornell" else caustic_pair(device)
     info = g.info
     nodes_b, tris_b = g.bvh()
     nodes = nodes_b.view(NODE)

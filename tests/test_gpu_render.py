@@ -38,6 +38,12 @@ def setup(device, name, W, H, photons_per_pass):
         o = O.cornell_box()
         cfg = wsrt.default_config(photons_per_pass=photons_per_pass)
         view = scenes.View((0.0, 0.0, 3.0), scenes.HALF_PI * -1, 0.0, W, H)
+    elif name == "small_box":
+        g = scenes.build_small_box(wsrt.SceneBuilder()).build(device)
+        o = scenes.build_small_box(O.Scene()).build()
+        cfg = scenes.small_box_config(photons_per_pass=photons_per_pass)
+        v = scenes.SMALL_BOX_VIEW
+        view = scenes.View(v.position, v.yaw, v.pitch, W, H)
     elif name == "heightfield":
         g = scenes.build_heightfield(wsrt.SceneBuilder(), n=65, tiles=4).build(device)
         o = scenes.build_heightfield(O.Scene(), n=65, tiles=4).build()
@@ -50,10 +56,10 @@ def setup(device, name, W, H, photons_per_pass):
         cfg = wsrt.default_config(photons_per_pass=photons_per_pass)
         view = scenes.View(scenes.DEFAULT_VIEW.position, scenes.DEFAULT_VIEW.yaw, scenes.DEFAULT_VIEW.pitch, W, H)
     else:
-        g = scenes.build_c2_caustic(wsrt.SceneBuilder()).build(device)
-        o = scenes.build_c2_caustic(O.Scene()).build()
-        cfg = scenes.c2_config(photons_per_pass=photons_per_pass)
-        view = scenes.View(scenes.C2_VIEW.position, scenes.C2_VIEW.yaw, scenes.C2_VIEW.pitch, W, H)
+        g = scenes.build_caustic(wsrt.SceneBuilder()).build(device)
+        o = scenes.build_caustic(O.Scene()).build()
+        cfg = scenes.caustic_config(photons_per_pass=photons_per_pass)
+        view = scenes.View(scenes.CAUSTIC_VIEW.position, scenes.CAUSTIC_VIEW.yaw, scenes.CAUSTIC_VIEW.pitch, W, H)
     return g, o, cfg, view
 
 
@@ -61,7 +67,7 @@ def rel_close(a, b, rel, floor):
     return np.all(np.abs(a - b) <= rel * np.maximum(np.abs(a), np.abs(b)) + floor, axis=-1)
 
 
-@pytest.mark.parametrize("name,frame", [("cornell", 0), ("cornell", 5), ("c2", 2), ("custom", 3)])
+@pytest.mark.parametrize("name,frame", [("cornell", 0), ("cornell", 5), ("caustic", 2), ("custom", 3)])
 def test_photon_pass_and_grid(device, name, frame):
     print(f"photons {name} frame {frame}", flush=True)
     g, o, cfg, view = setup(device, name, 64, 64, 1 << 16)
@@ -88,8 +94,8 @@ def test_photon_pass_and_grid(device, name, frame):
 
 
 @pytest.mark.parametrize("name,frame,photons", [("cornell", 0, 1 << 16), ("cornell", 9, 1 << 16),
-                                                ("c2", 1, 1 << 18), ("custom", 4, 1 << 17),
-                                                ("heightfield", 2, 1 << 10)])
+                                                ("caustic", 1, 1 << 18), ("custom", 4, 1 << 17),
+                                                ("heightfield", 2, 1 << 10), ("small_box", 3, 1 << 16)])
 def test_camera_paths_match_oracle(device, name, frame, photons):
     print(f"paths {name} frame {frame}", flush=True)
     W = H = 96
@@ -110,7 +116,7 @@ def test_camera_paths_match_oracle(device, name, frame, photons):
     assert np.abs(oL).sum() > 0
 
 
-@pytest.mark.parametrize("name", ["cornell", "c2", "custom", "heightfield"])
+@pytest.mark.parametrize("name", ["cornell", "caustic", "custom", "heightfield", "small_box"])
 def test_accumulated_image_matches_oracle(device, name):
     print(f"image {name}", flush=True)
     W = H = 64
@@ -181,7 +187,7 @@ def test_empty_scene(device):
     assert flags[0] == 0
 
 
-@pytest.mark.parametrize("name", ["cornell", "c2"])
+@pytest.mark.parametrize("name", ["cornell", "caustic"])
 def test_frame_graph_replays_match_eager(device, name):
     """Each frame is one captured HIP graph; every replay (with eager frames
     of another renderer and synchronizations between replays) must equal the
@@ -238,7 +244,7 @@ def test_screenshot_and_png(device, cornell_pair, tmp_path):
     np.testing.assert_array_equal(wsrt.load_png(path), shot)
 
 
-@pytest.mark.parametrize("name", ["cornell", "c2"])
+@pytest.mark.parametrize("name", ["cornell", "caustic"])
 def test_frames_in_flight_do_not_change_results(device, name):
     """Frames traced together in one wavefront accumulate bit for bit like
     frames traced one at a time."""
@@ -277,7 +283,7 @@ def test_batch_equals_single_renderers(device, tile_mode):
     """One persistent launch over jobs of different scenes, sizes and sample
     counts gives each job the image its own renderer gives, bit for bit."""
     print(f"batch tile mode {tile_mode}", flush=True)
-    specs = [("cornell", 40, 24, 3), ("c2", 32, 32, 2), ("custom", 24, 40, 4), ("cornell", 9, 13, 5)]
+    specs = [("cornell", 40, 24, 3), ("caustic", 32, 32, 2), ("custom", 24, 40, 4), ("cornell", 9, 13, 5)]
     jobs, refs = [], []
     for k, (name, W, H, spp) in enumerate(specs):
         g, _, cfg, view = setup(device, name, W, H, 1 << 10)

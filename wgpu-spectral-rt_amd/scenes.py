@@ -4,10 +4,10 @@ add_light_instance, src/scene/builder.rs:70-129) so the same definition can
 drive any builder with that interface.
 
   cornell       the reference's default scene (src/scene/cornell_box.rs)
-  c1_box        34-triangle Cornell-style box: 5 wall quads and two 12-triangle
+  small_box     34-triangle Cornell-style box: 5 wall quads and two 12-triangle
                 boxes rotated about Y by seeded angles in [0, 45] deg, one of
                 them GGX (alpha 0.3); a 0.25 x 0.25 blackbody 6500 K area light
-  c2_caustic    glass icosphere (subdivision 4, radius 0.5, Cauchy A = 1.5,
+  caustic       glass icosphere (subdivision 4, radius 0.5, Cauchy A = 1.5,
                 B = 0.0042 um^2) on a 2-triangle diffuse plane, point light
                 2 units above the sphere centre
   heightfield   1024 x 1024-vertex heightfield (2,093,058 triangles) displaced
@@ -86,8 +86,8 @@ def _walls(b, white, red, green):
     return plane
 
 
-def build_c1_box(b, seed: int = 0):
-    """34 triangles + a 2-triangle area light (config C1)."""
+def build_small_box(b, seed: int = 0):
+    """34 triangles + a 2-triangle area light."""
     light = b.add_material([6500.0, 1.0, 0.0, 1.0], [3, 0, 0, 0])
     red = b.add_material([0.65, 0.05, 0.05, 1.0], [0, 0, 0, 0])
     green = b.add_material([0.12, 0.45, 0.15, 1.0], [0, 0, 0, 0])
@@ -103,8 +103,8 @@ def build_c1_box(b, seed: int = 0):
     return b
 
 
-def build_c2_caustic(b):
-    """Glass icosphere over a diffuse plane under a point light (config C2)."""
+def build_caustic(b):
+    """Glass icosphere over a diffuse plane under a point light."""
     floor = b.add_material([0.8, 0.8, 0.8, 1.0], [0, 0, 0, 0])
     glass = b.add_material([1.0, 1.0, 1.0, 1.0], [2, 0, 1.5, 0.0042])
     plane = b.add_mesh(*wsrt.create_plane())
@@ -115,11 +115,11 @@ def build_c2_caustic(b):
     return b
 
 
-C2_VIEW = View((0.0, 2.0, 3.0), -HALF_PI, -math.atan2(1.5, 3.0), 512, 512)
-C1_VIEW = View((0.0, 0.0, 3.0), -HALF_PI, 0.0, 256, 256)
+CAUSTIC_VIEW = View((0.0, 2.0, 3.0), -HALF_PI, -math.atan2(1.5, 3.0), 512, 512)
+SMALL_BOX_VIEW = View((0.0, 0.0, 3.0), -HALF_PI, 0.0, 256, 256)
 
 
-def c2_config(**kw):
+def caustic_config(**kw):
     """2^20 photons per pass, 2^22 cells, cell size = gather radius = 0.01."""
     c = wsrt.default_config(cell_size=0.01, gather_radius=0.01, gather_radius_exponent=0.0)
     for k, v in kw.items():
@@ -127,7 +127,7 @@ def c2_config(**kw):
     return c
 
 
-def c1_config(**kw):
+def small_box_config(**kw):
     c = wsrt.default_config(max_depth=8)
     for k, v in kw.items():
         setattr(c, k, v)
