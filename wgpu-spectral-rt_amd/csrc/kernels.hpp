@@ -17,6 +17,13 @@ namespace wsrt {
 constexpr int kTraceThreads = 128;
 constexpr int kTraceBlocks = 4096;
 constexpr uint64_t kTraceLanes = (uint64_t)kTraceThreads * kTraceBlocks;
+// Register budgets of the traversal kernels, as waves per SIMD the compiler
+// must fit (gfx950: 512 VGPRs per lane and SIMD). Left free, the compiler
+// gives the extend kernel 122 VGPRs (4 waves); with 5 it fits 96 with a few
+// spills outside the node loop and runs 5 % faster on the caustic workload,
+// while 6 or 8 waves spill inside the loop and run slower. The photon bounce
+// fits 5 waves (96 VGPRs) without spills.
+#define WSRT_WAVES_PER_SIMD(n) __attribute__((amdgpu_waves_per_eu(n)))
 
 // ---- primitives (sort.hip) -------------------------------------------------
 size_t scan_temp_words(uint32_t n);

@@ -62,7 +62,7 @@ __global__ __launch_bounds__(256) void k_generate(PathState ps, FrameArgs fa, ui
 
 // Bounce 0 reads path ids 0 .. n0-1 directly (every path starts); later
 // bounces read the compacted queue of the previous shade.
-__global__ __launch_bounds__(kTraceThreads) void k_extend(SceneView sv, PathState ps, Queues qs,
+__global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend(SceneView sv, PathState ps, Queues qs,
                                                           uint32_t count_traversal,
                                                           Counters* __restrict__ ctr, uint32_t bounce,
                                                           uint32_t n0, const uint32_t* __restrict__ queue,

@@ -136,19 +136,150 @@ __global__ __launch_bounds__(kEmitThreads) void k_photon_emit(SceneView sv, Fram
     }
 }
 
+// Register state of a photon path between its load and the end of its ray.
+struct PhotonRay {
+    uint32_t sid, id;  // state slot, photon path id
+    float wl, energy;
+    uint32_t rng;
+    bool caustic;
+};
+
+// The surface interaction of photon_emit.wgsl:228-329 at the closest hit h
+// of the ray (pos, dir): store at a diffuse surface if the path is caustic,
+// else continue through metal, glass and thin film. Returns whether the
+// path continues (its state is then written back).
+__device__ __forceinline__ bool photon_interact(const SceneView& sv, const PhotonState& ps,
+                                                wsrt_photon* __restrict__ photons, PhotonRay& r,
+                                                f3 pos, f3 dir, const Hit& h, uint32_t& my_stored) {
+    if (h.tri == kInvalid) return false;
+    bool alive = false;
+    bool caustic = r.caustic;
+    uint32_t rng = r.rng;
+    const float wl = r.wl;
+    uint32_t mat_id;
+    f3 wn = hit_world_normal(sv, h.tri, h.u, h.v, &mat_id);
+    f3 ff = h.front ? wn : -wn;
+    const wsrt_material& m = sv.materials[mat_id];
+    const float4 ex = *reinterpret_cast<const float4*>(m.extra);
+    const uint32_t type = (uint32_t)ex.x;
+    f3 hp = pos + dir * h.t;
+    if (type == 0u) {
+        if (caustic) {
+            float4* dst = reinterpret_cast<float4*>(photons + r.id);
+            dst[0] = make_float4(hp.x, hp.y, hp.z, wl);
+            dst[1] = make_float4(-dir.x, -dir.y, -dir.z, r.energy);
+            my_stored++;
+        }
+    } else if (type == 1u) {
+        dir = reflect(normalize(dir), ff);
+        pos = hp + ff * 0.001f;
+        alive = true;
+    } else if (type == 2u) {
+        float lum = wl / 1000.0f;
+        float ir = ex.z + (ex.w / (lum * lum));
+        float ratio = h.front ? 1.0f / ir : ir;
+        f3 ud = normalize(dir);
+        float ct = fminf(dot(-ud, ff), 1.0f);
+        float sn = sqrtf(1.0f - ct * ct);
+        float r0 = (1.0f - ratio) / (1.0f + ratio);
+        r0 = r0 * r0;
+        float refl = r0 + (1.0f - r0) * powf(1.0f - ct, 5.0f);
+        // short-circuit: total internal reflection draws no random number
+        if (ratio * sn > 1.0f || refl > rand01(rng)) {
+            dir = reflect(ud, ff);
+        } else {
+            caustic = true;
+            dir = refract(ud, ff, ratio);
+        }
+        f3 off = dot(dir, ff) > 0.0f ? ff : -ff;
+        pos = hp + off * 0.001f;
+        alive = true;
+    } else if (type == 4u) {
+        float n2 = ex.z, n3 = ex.w, d = fmaxf(ex.y, 50.0f);
+        f3 ud = normalize(dir);
+        float ct = fminf(dot(-ud, ff), 1.0f);
+        float ratio = h.front ? n3 / 1.0f : 1.0f / n3;
+        float stsq = ratio * ratio * (1.0f - ct * ct);
+        if (stsq > 1.0f) {
+            dir = reflect(ud, ff);
+        } else {
+            float F = thin_film(ct, 1.0f, n2, n3, d, wl);
+            if (rand01(rng) < F) dir = reflect(ud, ff);
+            else dir = refract(ud, ff, ratio);
+        }
+        caustic = true;
+        alive = true;
+        // the origin is not advanced in this branch (photon_emit.wgsl:291-326)
+    }
+    if (alive) {
+        ps.pos[r.sid] = f4(pos, wl);
+        ps.dir[r.sid] = f4(dir, r.energy);
+        ps.rng[r.sid] = rng;
+        ps.flags[r.sid] = caustic ? 1u : 0u;
+    }
+    return alive;
+}
+
+// The rays of entries [begin, end) of a bounce, lanes refilled as above.
+template <bool kCount>
+__device__ __forceinline__ void photon_bounce_range(const SceneView& sv, uint32_t depth, const Queues& qs,
+                                                    const PhotonState& ps, const uint32_t* pre,
+                                                    const uint32_t* __restrict__ queue,
+                                                    uint32_t* __restrict__ next_queue,
+                                                    wsrt_photon* __restrict__ photons, Stack st,
+                                                    uint32_t begin, uint32_t end, uint32_t& my_rays,
+                                                    uint32_t& my_stored, uint32_t& my_nodes,
+                                                    uint32_t& my_tris) {
+    uint32_t cursor = begin;  // wave-uniform: the next entry not taken up
+    bool has = false;
+    PhotonRay r{};
+    Trav t;
+    while (true) {
+        const uint64_t idle = __ballot(!has);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (cursor < end && nidle >= kRefillLanes) {
+            if (!has) {
+                const uint32_t i = cursor + (uint32_t)__popcll(idle & lanemask_lt());
+                if (i < end) {
+                    // depth 0 reads state slots directly (in emission-direction
+                    // bin order within each emit block), later depths the queue
+                    r.sid = depth == 0 ? i : queue[seg_entry(pre, kSegs, i, qs.pseg_cap)];
+                    r.id = ps.pid[r.sid];
+                    const float4 pw = ps.pos[r.sid], de = ps.dir[r.sid];
+                    r.wl = pw.w;
+                    r.energy = de.w;
+                    r.rng = ps.rng[r.sid];
+                    r.caustic = ps.flags[r.sid] != 0u;
+                    trav_init(t, sv, xyz(pw), xyz(de), 0.0001f, 100.0f);
+                    has = true;
+                    my_rays++;
+                }
+            }
+            cursor += nidle;
+        }
+        if (__ballot(has) == 0ull) break;  // only once the range is used up
+        bool alive = false;
+        if (has) {
+            if (t.live) trav_step<false, kCount>(sv, t, st, my_nodes, my_tris);
+            if (!t.live) {
+                has = false;
+                alive = photon_interact(sv, ps, photons, r, t.r.o, t.r.d, t.best, my_stored);
+            }
+        }
+        const uint32_t e = seg_append(qs, depth + 1, kQPhoton, qs.pseg_cap, alive);
+        if (alive) next_queue[e] = r.sid;
+    }
+}
+
 // One bounce of every queued photon path (photon_emit.wgsl:219-330): closest
-// hit, then store at a diffuse surface if the path is caustic, or continue
-// through metal, glass and thin film. Survivors are compacted into the next
-// queue, so the few paths refracting through glass run in full waves.
-__global__ __launch_bounds__(kTraceThreads) void k_photon_bounce(SceneView sv, uint32_t depth,
-                                                                 uint32_t n0, Queues qs,
-                                                                 uint32_t count_traversal,
-                                                                 PhotonState ps,
-                                                                 const uint32_t* __restrict__ queue,
-                                                                 uint32_t* __restrict__ next_queue,
-                                                                 wsrt_photon* __restrict__ photons,
-                                                                 Counters* __restrict__ ctr,
-                                                                 uint2* __restrict__ gstack) {
+// hit, then the surface interaction. Survivors are compacted into the next
+// queue, so the few paths refracting through glass run in full waves. Each
+// wave takes a contiguous range of entries: at depth 0 every wave then sees
+// all emission-direction bins, and the paths of one bin share waves.
+__global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon_bounce(
+    SceneView sv, uint32_t depth, uint32_t n0, Queues qs, uint32_t count_traversal, PhotonState ps,
+    const uint32_t* __restrict__ queue, uint32_t* __restrict__ next_queue, wsrt_photon* __restrict__ photons,
+    Counters* __restrict__ ctr, uint2* __restrict__ gstack) {
     __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
     __shared__ uint32_t pre[kSegs + 1];
     __shared__ unsigned long long red[kTraceThreads / 64];
@@ -158,101 +289,20 @@ __global__ __launch_bounds__(kTraceThreads) void k_photon_bounce(SceneView sv, u
     st.lds_stride = kTraceThreads;
     st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
     const uint32_t n = depth == 0 ? n0 : seg_prefix(qs, depth, kQPhoton, 1, pre);
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const bool count = count_traversal != 0u;
-    uint32_t my_rays = 0, my_stored = 0, my_nodes = 0, my_tris = 0;
-    // Depth 0 reads state slots in bin order within each emit block. A wave
-    // takes a contiguous run of 64-path chunks there (so every wave sees all
-    // bins, and the paths of one direction bin share waves); later depths
-    // read the compacted queue grid-stride.
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     const uint32_t nchunks = (n + 63u) >> 6;
-    const uint32_t per_wave = depth == 0 ? (nchunks + (stride >> 6) - 1u) / (stride >> 6) : 0u;
+    const uint32_t per_wave = (nchunks + nwaves - 1u) / nwaves;
     const uint32_t wave = gtid >> 6;
-    uint32_t base = depth == 0 ? wave * per_wave * 64u : (gtid & ~63u);
-    const uint32_t end = depth == 0 ? min(n, (wave + 1u) * per_wave * 64u) : n;
-    const uint32_t step = depth == 0 ? 64u : stride;
-    for (; base < end; base += step) {
-        const uint32_t i = base + __lane_id();
-        bool alive = false;
-        uint32_t sid = 0;  // state slot; the photon path id is ps.pid[sid]
-        if (i < end) {
-            sid = depth == 0 ? i : queue[seg_entry(pre, kSegs, i, qs.pseg_cap)];
-            const uint32_t id = ps.pid[sid];
-            const float4 pw = ps.pos[sid], de = ps.dir[sid];
-            f3 pos = xyz(pw), dir = xyz(de);
-            const float wl = pw.w, energy = de.w;
-            uint32_t rng = ps.rng[sid];
-            bool caustic = ps.flags[sid] != 0u;
-            Hit h = trace_ray<false>(sv, pos, dir, 0.0001f, 100.0f, st, count, my_nodes, my_tris);
-            my_rays++;
-            if (h.tri != kInvalid) {
-                uint32_t mat_id;
-                f3 wn = hit_world_normal(sv, h.tri, h.u, h.v, &mat_id);
-                f3 ff = h.front ? wn : -wn;
-                const wsrt_material& m = sv.materials[mat_id];
-                const float4 ex = *reinterpret_cast<const float4*>(m.extra);
-                const uint32_t type = (uint32_t)ex.x;
-                f3 hp = pos + dir * h.t;
-                if (type == 0u) {
-                    if (caustic) {
-                        float4* dst = reinterpret_cast<float4*>(photons + id);
-                        dst[0] = make_float4(hp.x, hp.y, hp.z, wl);
-                        dst[1] = make_float4(-dir.x, -dir.y, -dir.z, energy);
-                        my_stored++;
-                    }
-                } else if (type == 1u) {
-                    dir = reflect(normalize(dir), ff);
-                    pos = hp + ff * 0.001f;
-                    alive = true;
-                } else if (type == 2u) {
-                    float lum = wl / 1000.0f;
-                    float ir = ex.z + (ex.w / (lum * lum));
-                    float ratio = h.front ? 1.0f / ir : ir;
-                    f3 ud = normalize(dir);
-                    float ct = fminf(dot(-ud, ff), 1.0f);
-                    float sn = sqrtf(1.0f - ct * ct);
-                    float r0 = (1.0f - ratio) / (1.0f + ratio);
-                    r0 = r0 * r0;
-                    float refl = r0 + (1.0f - r0) * powf(1.0f - ct, 5.0f);
-                    // short-circuit: total internal reflection draws no random number
-                    if (ratio * sn > 1.0f || refl > rand01(rng)) {
-                        dir = reflect(ud, ff);
-                    } else {
-                        caustic = true;
-                        dir = refract(ud, ff, ratio);
-                    }
-                    f3 off = dot(dir, ff) > 0.0f ? ff : -ff;
-                    pos = hp + off * 0.001f;
-                    alive = true;
-                } else if (type == 4u) {
-                    float n2 = ex.z, n3 = ex.w, d = fmaxf(ex.y, 50.0f);
-                    f3 ud = normalize(dir);
-                    float ct = fminf(dot(-ud, ff), 1.0f);
-                    float ratio = h.front ? n3 / 1.0f : 1.0f / n3;
-                    float stsq = ratio * ratio * (1.0f - ct * ct);
-                    if (stsq > 1.0f) {
-                        dir = reflect(ud, ff);
-                    } else {
-                        float F = thin_film(ct, 1.0f, n2, n3, d, wl);
-                        if (rand01(rng) < F) dir = reflect(ud, ff);
-                        else dir = refract(ud, ff, ratio);
-                    }
-                    caustic = true;
-                    alive = true;
-                    // the origin is not advanced in this branch (photon_emit.wgsl:291-326)
-                }
-                if (alive) {
-                    ps.pos[sid] = f4(pos, wl);
-                    ps.dir[sid] = f4(dir, energy);
-                    ps.rng[sid] = rng;
-                    ps.flags[sid] = caustic ? 1u : 0u;
-                }
-            }
-        }
-        const uint32_t e = seg_append(qs, depth + 1, kQPhoton, qs.pseg_cap, alive);
-        if (alive) next_queue[e] = sid;
-    }
-    if (count) flush_traversal_counts(ctr, 0, my_rays, my_nodes, my_tris);
+    const uint32_t begin = (uint32_t)min((uint64_t)n, (uint64_t)wave * per_wave * 64u);
+    const uint32_t end = (uint32_t)min((uint64_t)n, (uint64_t)(wave + 1u) * per_wave * 64u);
+    uint32_t my_rays = 0, my_stored = 0, my_nodes = 0, my_tris = 0;
+    if (count_traversal)
+        photon_bounce_range<true>(sv, depth, qs, ps, pre, queue, next_queue, photons, st, begin, end, my_rays,
+                                  my_stored, my_nodes, my_tris);
+    else
+        photon_bounce_range<false>(sv, depth, qs, ps, pre, queue, next_queue, photons, st, begin, end,
+                                   my_rays, my_stored, my_nodes, my_tris);
+    if (count_traversal) flush_traversal_counts(ctr, 0, my_rays, my_nodes, my_tris);
     // per-block statistics, summed on the host
     const unsigned long long br = block_sum(my_rays, red);
     const unsigned long long bs = block_sum(my_stored, red);
