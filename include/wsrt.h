@@ -312,11 +312,13 @@ int wsrt_renderer_reset_profile(wsrt_renderer* r);
 
 /* ---- parity / debug taps -------------------------------------------------- */
 /* Photon buffer of the last frame in photon-path order: slot i holds photon
- * path i's stored photon or energy 0 (n = photons_per_pass). *n_stored gets
- * the number with energy != 0. */
+ * path i's stored photon, or all zero when it stored none (n =
+ * photons_per_pass). *n_stored gets the number of stored photons. */
 int wsrt_debug_read_photons(wsrt_renderer* r, wsrt_photon* out, uint32_t cap, uint32_t* n_stored);
-/* Grid of the last frame: cell_id per photon slot (0xFFFFFFFF = skipped),
- * count and exclusive offset per cell, photon slot per sorted position. */
+/* Grid of the last frame in the form of wsrt_photon_grid_build: cell_id per
+ * photon slot (0xFFFFFFFF = skipped), count and exclusive offset per cell
+ * (the renderer keeps offsets of non-empty cells only; an empty cell reads
+ * the number of photons before it), photon slot per sorted position. */
 int wsrt_debug_read_grid(wsrt_renderer* r, uint32_t* cell_id, uint32_t* count, uint32_t* offset,
                          uint32_t* sorted_slot);
 /* Per-pixel hero wavelengths and spectral radiance of the last frame

@@ -938,7 +938,7 @@ struct wsrt_renderer {
         shadow;
     DevBuf accum, rgba, counters, gstack, qcounts;
     DevBuf ftable, dframe;           // per-frame constants, device frame counter
-    DevBuf ph_pos, ph_dir, ph_rng, ph_flags, ph_qa, ph_qb, ph_pid;
+    DevBuf ph_pos, ph_dir, ph_rng, ph_flags, ph_qa, ph_qb, ph_pid, ph_stored;
     PhotonState phs{};
     FrameConst* staging = nullptr;   // pinned, 2 halves of kUploadHalf entries
     hipEvent_t upload_ev[2] = {nullptr, nullptr};
@@ -1051,12 +1051,13 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     HIP_OR_FAIL(r->ph_rng.alloc(4 * nph));
     HIP_OR_FAIL(r->ph_flags.alloc(4 * nph));
     HIP_OR_FAIL(r->ph_pid.alloc(4 * nph));
+    HIP_OR_FAIL(r->ph_stored.alloc(4 * nph));
     const uint32_t pseg = seg_capacity(nph);
     HIP_OR_FAIL(r->ph_qa.alloc(4ull * kSegs * pseg));
     HIP_OR_FAIL(r->ph_qb.alloc(4ull * kSegs * pseg));
     r->phs = PhotonState{r->ph_pos.as<float4>(), r->ph_dir.as<float4>(), r->ph_rng.as<uint32_t>(),
                          r->ph_flags.as<uint32_t>(), r->ph_qa.as<uint32_t>(), r->ph_qb.as<uint32_t>(),
-                         r->ph_pid.as<uint32_t>()};
+                         r->ph_pid.as<uint32_t>(), r->ph_stored.as<uint32_t>()};
     HIP_OR_FAIL(r->path_o.alloc(16 * npaths));
     HIP_OR_FAIL(r->path_d.alloc(16 * npaths));
     HIP_OR_FAIL(r->path_tp.alloc(16 * npaths));
@@ -1092,6 +1093,7 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     HIP_OR_FAIL(hipMemsetAsync(r->grid_count.p, 0, sizeof(uint32_t) * cells, r->stream));
     HIP_OR_FAIL(hipMemsetAsync(r->grid_offset.p, 0, sizeof(uint32_t) * cells, r->stream));
     HIP_OR_FAIL(hipMemsetAsync(r->grid_nvalid.p, 0, sizeof(uint32_t), r->stream));
+    HIP_OR_FAIL(hipMemsetAsync(r->ph_stored.p, 0, 4 * nph, r->stream));
     HIP_OR_FAIL(hipStreamSynchronize(r->stream));
     r->gb.count = r->grid_count.as<uint32_t>();
     r->gb.offset = r->grid_offset.as<uint32_t>();
@@ -1228,7 +1230,7 @@ static int enqueue_batch(wsrt_renderer* r, uint32_t nf) {
     if (np) {
         {
             ProfScope p(r, K_PHOTON_EMIT);
-            launch_photon_emit(pl.sv, pl.fa, np, nf, r->phs, r->photons.as<wsrt_photon>(), s);
+            launch_photon_emit(pl.sv, pl.fa, np, nf, r->phs, s);
         }
         for (uint32_t d = 0; d < r->cfg.photon_max_depth; ++d) {
             ProfScope p(r, K_PHOTON_TRACE);
@@ -1236,7 +1238,8 @@ static int enqueue_batch(wsrt_renderer* r, uint32_t nf) {
                                  pl.ctr, pl.gstack, s);
         }
         ProfScope p(r, K_GRID_BUILD);
-        if (grid_build(r->photons.as<wsrt_photon>(), np, nf, r->cfg.hash_size, pl.fa, r->gb, s))
+        if (grid_build_stored(r->photons.as<wsrt_photon>(), r->phs.stored, np, nf, r->cfg.hash_size, pl.fa, r->gb,
+                              s))
             return fail(WSRT_E_HIP, "grid build launch failed");
     }
     {
@@ -1535,33 +1538,57 @@ extern "C" int wsrt_debug_read_photons(wsrt_renderer* r, wsrt_photon* out, uint3
     CHECK_ARG(r && out);
     uint32_t np = r->cfg.enable_photons ? r->cfg.photons_per_pass : 0u;
     if (cap < np) return fail(WSRT_E_INVALID, "photon buffer holds %u < %u", cap, np);
-    const wsrt_photon* src = r->photons.as<wsrt_photon>() + (size_t)r->last_slot * np;
-    int rc = read_back(r, out, src, sizeof(wsrt_photon) * np);
+    const size_t first = (size_t)r->last_slot * np;
+    int rc = read_back(r, out, r->photons.as<wsrt_photon>() + first, sizeof(wsrt_photon) * np);
+    std::vector<uint32_t> flag(np);
+    if (!rc && np) rc = read_back(r, flag.data(), r->ph_stored.as<uint32_t>() + first, 4ull * np);
     if (rc) return rc;
-    if (n_stored) {
-        uint32_t k = 0;
-        for (uint32_t i = 0; i < np; ++i) k += out[i].energy != 0.0f;
-        *n_stored = k;
+    // a slot holds a photon only when its stored flag is set; the others read
+    // as the reference's skipped photons (all zero)
+    uint32_t k = 0;
+    for (uint32_t i = 0; i < np; ++i) {
+        if (flag[i]) k++;
+        else memset(&out[i], 0, sizeof(wsrt_photon));
     }
+    if (n_stored) *n_stored = k;
     return WSRT_OK;
 }
 
+// The grid of the frame in the dense form of wsrt_photon_grid_build. The
+// renderer writes cell ids for stored photons only and offsets for
+// non-empty cells only (nothing else is read); here a slot without a photon
+// reads kInvalid, and an empty cell's offset is the number of photons in the
+// cells before it (what a dense exclusive scan gives). Non-empty cells report
+// the device's offsets.
 extern "C" int wsrt_debug_read_grid(wsrt_renderer* r, uint32_t* cell_id, uint32_t* count,
                                     uint32_t* offset, uint32_t* sorted_slot) {
     CHECK_ARG(r);
     const uint32_t np = r->cfg.enable_photons ? r->cfg.photons_per_pass : 0u;
     const uint32_t hs = r->cfg.hash_size, ls = r->last_slot;
     int rc = WSRT_OK;
-    if (cell_id && !rc) rc = read_back(r, cell_id, r->grid_cell.as<uint32_t>() + (size_t)ls * np, 4ull * np);
+    if (cell_id && np) {
+        std::vector<uint32_t> flag(np);
+        rc = read_back(r, cell_id, r->grid_cell.as<uint32_t>() + (size_t)ls * np, 4ull * np);
+        if (!rc) rc = read_back(r, flag.data(), r->ph_stored.as<uint32_t>() + (size_t)ls * np, 4ull * np);
+        if (rc) return rc;
+        for (uint32_t i = 0; i < np; ++i)
+            if (!flag[i]) cell_id[i] = kInvalid;
+    }
     std::vector<uint32_t> cnt(hs), off(hs);
     if (!rc) rc = read_back(r, cnt.data(), r->grid_count.as<uint32_t>() + (size_t)ls * hs, 4ull * hs);
     if (!rc) rc = read_back(r, off.data(), r->grid_offset.as<uint32_t>() + (size_t)ls * hs, 4ull * hs);
     if (rc) return rc;
-    const uint32_t base = hs ? off[0] : 0u;
+    // this frame's photons start at the offset of its first non-empty cell
+    uint32_t base = 0;
+    for (uint32_t c = 0; c < hs; ++c)
+        if (cnt[c]) {
+            base = off[c];
+            break;
+        }
     uint64_t valid = 0;
     for (uint32_t c = 0; c < hs; ++c) {
+        off[c] = cnt[c] ? off[c] - base : (uint32_t)valid;
         valid += cnt[c];
-        off[c] -= base;  // offsets relative to this frame's first cell
     }
     if (count) memcpy(count, cnt.data(), 4ull * hs);
     if (offset) memcpy(offset, off.data(), 4ull * hs);

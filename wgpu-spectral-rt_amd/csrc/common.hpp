@@ -102,6 +102,7 @@ struct PhotonState {
     uint32_t* queue_a;
     uint32_t* queue_b;
     uint32_t* pid;      // photon path id of a state slot
+    uint32_t* stored;   // per photon path: 1 once its photon is stored in its slot
 };
 
 // Photon grid (counting sort view of build_grid.wgsl).

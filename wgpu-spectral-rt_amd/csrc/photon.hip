@@ -36,8 +36,9 @@ __device__ __forceinline__ uint32_t direction_bin(f3 d) {
 constexpr int kEmitThreads = 1024;
 
 // Emission of photon path i (photon_emit.wgsl:159-216): light choice, point
-// on the light, cosine direction, hero wavelength and flux. Every slot is
-// cleared; with lights present every path takes part in the first bounce.
+// on the light, cosine direction, hero wavelength and flux. Every path's
+// stored flag is cleared (its photon slot holds a photon only once the flag
+// is set); with lights present every path takes part in the first bounce.
 // Each block also
 // orders its paths by emission-direction bin (a counting sort in LDS) and
 // stores their state in that order, with the path id beside it, so the waves
@@ -45,8 +46,7 @@ constexpr int kEmitThreads = 1024;
 // path's result depends only on its id, so the order in which the bounce
 // kernel takes paths up changes nothing but divergence.
 __global__ __launch_bounds__(kEmitThreads) void k_photon_emit(SceneView sv, FrameArgs fa, uint32_t np,
-                                                              uint32_t nf, PhotonState ps,
-                                                              wsrt_photon* __restrict__ photons) {
+                                                              uint32_t nf, PhotonState ps) {
     __shared__ uint32_t hist[64], cursor[64];
     __shared__ float4 s_pos[kEmitThreads], s_dir[kEmitThreads];
     __shared__ uint32_t s_rng[kEmitThreads], s_id[kEmitThreads];
@@ -62,9 +62,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_photon_emit(SceneView sv, Fram
         const FrameConst& f0 = fa.table[base_frame & fa.mask];
         const uint32_t slot = id / np, local = id - slot * np;
         uint32_t rng = pcg_hash(local + (base_frame + slot) * 927163u);
-        float4* dst = reinterpret_cast<float4*>(photons + id);
-        dst[0] = splat4(0.0f);
-        dst[1] = splat4(0.0f);
+        ps.stored[id] = 0u;
         const uint32_t nl = sv.n_lights;
         if (nl != 0u) {
             uint32_t li = (uint32_t)(rand01(rng) * (float)nl);
@@ -168,7 +166,11 @@ __device__ __forceinline__ bool photon_interact(const SceneView& sv, const Photo
             float4* dst = reinterpret_cast<float4*>(photons + r.id);
             dst[0] = make_float4(hp.x, hp.y, hp.z, wl);
             dst[1] = make_float4(-dir.x, -dir.y, -dir.z, r.energy);
-            my_stored++;
+            // a photon of energy 0 counts as skipped, as in the dense photon
+            // buffer that wsrt_photon_grid_build takes
+            const bool keep = r.energy != 0.0f;
+            ps.stored[r.id] = keep ? 1u : 0u;
+            my_stored += keep ? 1u : 0u;
         }
     } else if (type == 1u) {
         dir = reflect(normalize(dir), ff);
@@ -351,6 +353,41 @@ __global__ __launch_bounds__(256) void k_grid_compact(uint32_t n, const uint32_t
     cvals[pos[i]] = i;
 }
 
+// Cell and histogram of the stored photons only, compacted in slot order
+// (positions from an exclusive scan of the stored flags).
+__global__ __launch_bounds__(256) void k_grid_hash_stored(const wsrt_photon* __restrict__ photons,
+                                                          const uint32_t* __restrict__ stored,
+                                                          const uint32_t* __restrict__ pos, uint32_t n,
+                                                          uint32_t np, FrameArgs fa,
+                                                          uint32_t* __restrict__ cell_id,
+                                                          uint32_t* __restrict__ count,
+                                                          uint32_t* __restrict__ ckeys,
+                                                          uint32_t* __restrict__ cvals) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !stored[i]) return;
+    const uint32_t slot = i / np;
+    const FrameConst& fc = fa.table[(*fa.frame + slot) & fa.mask];
+    const float4 a = reinterpret_cast<const float4*>(photons + i)[0];
+    const uint32_t c = hash_cell(xyz(a), mk3(fc.jitter[0], fc.jitter[1], fc.jitter[2]), fc.cell_size, fc.hash_size);
+    const uint32_t key = slot * fc.hash_size + c;
+    atomicAdd(&count[key], 1u);
+    cell_id[i] = c;
+    ckeys[pos[i]] = key;
+    cvals[pos[i]] = i;
+}
+
+// Offset of every non-empty cell: the position of the first of its photons
+// in the sorted order (keys ascending).
+__global__ __launch_bounds__(256) void k_grid_run_offsets(const uint32_t* __restrict__ skeys,
+                                                          const uint32_t* __restrict__ n_valid,
+                                                          uint32_t* __restrict__ offset) {
+    const uint32_t nv = *n_valid;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nv; k += gridDim.x * blockDim.x) {
+        const uint32_t key = skeys[k];
+        if (k == 0 || skeys[k - 1] != key) offset[key] = k;
+    }
+}
+
 __global__ void k_grid_copy_count(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
     if (threadIdx.x == 0) *dst = *src;
 }
@@ -373,10 +410,10 @@ __global__ __launch_bounds__(256) void k_grid_copy(const wsrt_photon* __restrict
 }  // namespace
 
 void launch_photon_emit(const SceneView& sv, const FrameArgs& fa, uint32_t np, uint32_t nf, const PhotonState& ps,
-                        wsrt_photon* photons, hipStream_t s) {
+                        hipStream_t s) {
     const uint32_t n = np * nf;
     hipLaunchKernelGGL(k_photon_emit, dim3((n + kEmitThreads - 1) / kEmitThreads), dim3(kEmitThreads), 0, s, sv,
-                       fa, np, nf, ps, photons);
+                       fa, np, nf, ps);
 }
 
 void launch_photon_bounce(const SceneView& sv, uint32_t np, uint32_t nf, uint32_t depth,
@@ -423,6 +460,37 @@ int grid_build(const wsrt_photon* photons, uint32_t np, uint32_t nf, uint32_t ha
     bool in_tmp = false;
     if (radix_sort_pairs(ckeys, cvals, n, bits, work, s, &in_tmp, n_valid)) return -1;
     const uint32_t* sorted_vals = in_tmp ? work + n : cvals;
+    hipLaunchKernelGGL(k_grid_copy, dim3((n + 255) / 256), dim3(256), 0, s, photons, sorted_vals, n_valid,
+                       g.sorted_slot, g.sorted);
+    hipLaunchKernelGGL(k_grid_copy_count, dim3(1), dim3(64), 0, s, n_valid, g.n_valid);
+    return 0;
+}
+
+int grid_build_stored(const wsrt_photon* photons, const uint32_t* stored, uint32_t np, uint32_t nf,
+                      uint32_t hash_size, const FrameArgs& fa, GridBuffers& g, hipStream_t s) {
+    // g.temp: as grid_build (keys and flags unused)
+    const uint32_t n = np * nf, cells = hash_size * nf;
+    uint32_t* pos = g.temp + 2ull * n;
+    uint32_t* ckeys = pos + n;
+    uint32_t* cvals = ckeys + n;
+    uint32_t* work = cvals + n;
+    uint32_t* n_valid = g.temp + grid_temp_words(n, cells) - 1;
+    fill_u32(g.count, cells, 0u, s);
+    fill_u32(n_valid, 1, 0u, s);
+    if (n == 0) {
+        fill_u32(g.n_valid, 1, 0u, s);
+        return 0;
+    }
+    if (scan_exclusive(stored, pos, n, work, n_valid, s)) return -1;
+    hipLaunchKernelGGL(k_grid_hash_stored, dim3((n + 255) / 256), dim3(256), 0, s, photons, stored, pos, n, np, fa,
+                       g.cell_id, g.count, ckeys, cvals);
+    int bits = 1;
+    while ((1ull << bits) < (uint64_t)cells) bits++;  // keys in [0, cells)
+    bool in_tmp = false;
+    if (radix_sort_pairs(ckeys, cvals, n, bits, work, s, &in_tmp, n_valid)) return -1;
+    const uint32_t* sorted_keys = in_tmp ? work : ckeys;
+    const uint32_t* sorted_vals = in_tmp ? work + n : cvals;
+    hipLaunchKernelGGL(k_grid_run_offsets, dim3(1024), dim3(256), 0, s, sorted_keys, n_valid, g.offset);
     hipLaunchKernelGGL(k_grid_copy, dim3((n + 255) / 256), dim3(256), 0, s, photons, sorted_vals, n_valid,
                        g.sorted_slot, g.sorted);
     hipLaunchKernelGGL(k_grid_copy_count, dim3(1), dim3(64), 0, s, n_valid, g.n_valid);
