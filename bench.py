@@ -143,17 +143,21 @@ def copy_calibration(mib: int) -> int:
     return 0
 
 
-def time_to_relmse(frames: int = 32, warmup: int = 64, boxes: int | None = None, width: int = 1024) -> dict:
+def time_to_relmse(frames: int = 32, warmup: int = 256, boxes: int | None = None, width: int = 1024,
+                   ref_frames: int = 128) -> dict:
     """Guided vs plain tracer on the occluded-light room (2,097,182 triangles
     at the default box count), 1024 x 1024, depth 8.
 
     relMSE = mean over pixels and rgb of (I - R)^2 / (R^2 + 1e-2). Both
     estimators are unbiased for the same image, so relMSE after N samples
     per pixel is C / N with C = N * E[(A - B)^2 / (R^2 + 1e-2)] / 2 for two
-    independent N-spp images A, B of the same tracer (R: a 3N-spp plain image
-    of independent seeds for the denominator). Time-to-relMSE 0.01 is then
-    (C / 0.01) * (seconds per spp). The guided tracer is measured after
-    `warmup` frames of online training, and keeps training while measured."""
+    independent N-spp images A, B of the same tracer. R, the reference of
+    the denominator, is a plain image of `ref_frames` samples per pixel with
+    seeds of its own, independent of A and B: a reference that contained A
+    or B would grow with their outliers and hide them. Time-to-relMSE 0.01 is
+    then (C / 0.01) * (seconds per spp) plus, for the guided tracer, the
+    `warmup` frames of online training it runs before being measured (it
+    keeps training while measured)."""
     import numpy as np
     import scenes
     import wsrt
@@ -184,8 +188,9 @@ def time_to_relmse(frames: int = 32, warmup: int = 64, boxes: int | None = None,
     out = {"scene": f"occluded-light room, {scene.info.n_triangles} triangles, {width}x{width}, depth 8",
            "relmse": "mean (I - R)^2 / (R^2 + 1e-2)", "target": 0.01}
     rp = renderer(scenes.occluded_config(frames_in_flight=4), 1 << 22)
-    den, _ = frames_of(rp, 3 * frames)
+    R, _ = frames_of(rp, ref_frames)
     rp.close()
+    out["reference_spp"] = ref_frames
     res = {}
     for name, cfg in (("plain", scenes.occluded_config(frames_in_flight=4)),
                       ("guided", scenes.guided_config(frames_in_flight=4))):
@@ -196,7 +201,6 @@ def time_to_relmse(frames: int = 32, warmup: int = 64, boxes: int | None = None,
         a, ta = frames_of(r, frames)
         b, tb = frames_of(r, frames)
         r.close()
-        R = (den * 3 + a + b) / 5
         c = frames * float(np.mean((a - b) ** 2 / (R**2 + 1e-2))) / 2
         spp_s = (ta + tb) / (2 * frames)
         res[name] = {"C": round(c, 3), "ms_per_spp": round(1000 * spp_s, 3),
