@@ -143,7 +143,7 @@ def copy_calibration(mib: int) -> int:
     return 0
 
 
-def time_to_relmse(frames: int = 32, warmup: int = 512, boxes: int | None = None, width: int = 1024,
+def time_to_relmse(frames: int = 32, warmup: int = 256, boxes: int | None = None, width: int = 1024,
                    ref_frames: int = 128) -> dict:
     """Guided vs plain tracer on the occluded-light room (2,097,182 triangles
     at the default box count), 1024 x 1024, depth 8.
