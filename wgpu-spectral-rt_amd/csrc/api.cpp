@@ -404,6 +404,7 @@ struct wsrt_scene {
     std::atomic<int> renderers{0};
     DevBuf verts, idx, meshes, materials, lights, instances, env;
     DevBuf nodes, tris, wv, tri_inst, codes, sorted;
+    DevBuf spec_instances, spec_nodes, spec_tris;
     SceneView view{};
     wsrt_scene_info info{};
 };
@@ -518,6 +519,47 @@ extern "C" int wsrt_scene_build(wsrt_scene_builder* b, wsrt_device* dev, wsrt_sc
     v.env_w = b->env_w;
     v.env_h = b->env_h;
     v.stack_size = 1u + 7u * bo.max_depth;
+    // the specular BVH: instances of metal (1), glass (2) and thin-film (4)
+    // materials, numbered from 0 (the photon pass's first-bounce filter)
+    {
+        std::vector<Instance> spec;
+        uint32_t n_spec = 0;
+        for (const Instance& in : inst) {
+            const uint32_t type = (uint32_t)b->materials[in.material].extra[0];
+            if (type != 1u && type != 2u && type != 4u) continue;
+            Instance c = in;
+            c.first_tri = n_spec;
+            n_spec += c.n_tris;
+            spec.push_back(c);
+        }
+        BvhOutputs so;
+        so.n_nodes = 0;
+        so.max_depth = 0;
+        so.nodes = nullptr;
+        so.tris = nullptr;
+        if (n_spec) {
+            HIP_OR_FAIL(upload(sc->spec_instances, spec, s));
+            BvhInputs sb = bi;
+            sb.instances = sc->spec_instances.as<Instance>();
+            sb.n_instances = (uint32_t)spec.size();
+            sb.n_tris = n_spec;
+            rc = bvh_build(sb, &so, s);
+            HIP_OR_FAIL(hipStreamSynchronize(s));
+            // only the nodes and leaf triangles are kept
+            for (void* q : {(void*)so.wv, (void*)so.tri_inst, (void*)so.codes, (void*)so.sorted})
+                if (q) (void)hipFree(q);
+            if (rc == -2) return fail(WSRT_E_OOM, "specular BVH build: device allocation failed");
+            if (rc != 0 || so.placed_tris != n_spec) return fail(WSRT_E_HIP, "specular BVH build failed (%d)", rc);
+            sc->spec_nodes.p = so.nodes;
+            sc->spec_nodes.bytes = (size_t)so.n_nodes * sizeof(wsrt_bvh8_node);
+            sc->spec_tris.p = so.tris;
+            sc->spec_tris.bytes = (size_t)n_spec * sizeof(wsrt_bvh_tri);
+        }
+        v.spec_n_nodes = so.n_nodes;
+        v.spec_nodes = sc->spec_nodes.as<wsrt_bvh8_node>();
+        v.spec_tris = sc->spec_tris.as<float4>();
+        v.stack_size = std::max(v.stack_size, 1u + 7u * so.max_depth);
+    }
     wsrt_scene_info& I = sc->info;
     I.n_triangles = (uint32_t)n_tris;
     I.n_instances = (uint32_t)inst.size();
@@ -525,7 +567,7 @@ extern "C" int wsrt_scene_build(wsrt_scene_builder* b, wsrt_device* dev, wsrt_sc
     I.n_materials = (uint32_t)b->materials.size();
     I.n_nodes = bo.n_nodes;
     I.max_depth = bo.max_depth;
-    I.stack_size = v.stack_size;
+    I.stack_size = 1u + 7u * bo.max_depth;
     I.node_bytes = sizeof(wsrt_bvh8_node);
     I.tri_bytes = sizeof(wsrt_bvh_tri);
     I.build_ms = ms;
@@ -938,7 +980,7 @@ struct wsrt_renderer {
         shadow;
     DevBuf accum, rgba, counters, gstack, qcounts;
     DevBuf ftable, dframe;           // per-frame constants, device frame counter
-    DevBuf ph_pos, ph_dir, ph_rng, ph_flags, ph_qa, ph_qb, ph_pid, ph_stored;
+    DevBuf ph_pos, ph_dir, ph_rng, ph_flags, ph_qa, ph_qb, ph_stored;
     PhotonState phs{};
     FrameConst* staging = nullptr;   // pinned, 2 halves of kUploadHalf entries
     hipEvent_t upload_ev[2] = {nullptr, nullptr};
@@ -1050,14 +1092,13 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     HIP_OR_FAIL(r->ph_dir.alloc(16 * nph));
     HIP_OR_FAIL(r->ph_rng.alloc(4 * nph));
     HIP_OR_FAIL(r->ph_flags.alloc(4 * nph));
-    HIP_OR_FAIL(r->ph_pid.alloc(4 * nph));
     HIP_OR_FAIL(r->ph_stored.alloc(4 * nph));
     const uint32_t pseg = seg_capacity(nph);
     HIP_OR_FAIL(r->ph_qa.alloc(4ull * kSegs * pseg));
     HIP_OR_FAIL(r->ph_qb.alloc(4ull * kSegs * pseg));
     r->phs = PhotonState{r->ph_pos.as<float4>(), r->ph_dir.as<float4>(), r->ph_rng.as<uint32_t>(),
                          r->ph_flags.as<uint32_t>(), r->ph_qa.as<uint32_t>(), r->ph_qb.as<uint32_t>(),
-                         r->ph_pid.as<uint32_t>(), r->ph_stored.as<uint32_t>()};
+                         r->ph_stored.as<uint32_t>()};
     HIP_OR_FAIL(r->path_o.alloc(16 * npaths));
     HIP_OR_FAIL(r->path_d.alloc(16 * npaths));
     HIP_OR_FAIL(r->path_tp.alloc(16 * npaths));
@@ -1230,12 +1271,12 @@ static int enqueue_batch(wsrt_renderer* r, uint32_t nf) {
     if (np) {
         {
             ProfScope p(r, K_PHOTON_EMIT);
-            launch_photon_emit(pl.sv, pl.fa, np, nf, r->phs, s);
+            launch_photon_emit(pl.sv, pl.fa, np, nf, pl.qs, r->phs, pl.count_traversal, pl.ctr, pl.gstack, s);
         }
         for (uint32_t d = 0; d < r->cfg.photon_max_depth; ++d) {
             ProfScope p(r, K_PHOTON_TRACE);
-            launch_photon_bounce(pl.sv, np, nf, d, pl.count_traversal, r->phs, pl.qs, r->photons.as<wsrt_photon>(),
-                                 pl.ctr, pl.gstack, s);
+            launch_photon_bounce(pl.sv, d, pl.count_traversal, r->phs, pl.qs, r->photons.as<wsrt_photon>(), pl.ctr,
+                                 pl.gstack, s);
         }
         ProfScope p(r, K_GRID_BUILD);
         if (grid_build_stored(r->photons.as<wsrt_photon>(), r->phs.stored, np, nf, r->cfg.hash_size, pl.fa, r->gb,

@@ -40,9 +40,23 @@ struct SceneView {
     uint32_t n_lights;
     uint32_t n_instances;
     uint32_t env_w, env_h;
-    uint32_t stack_size;           // traversal stack entries required
-    uint32_t pad;
+    uint32_t stack_size;           // traversal stack entries required (either BVH)
+    // BVH over the triangles of metal, glass and thin-film instances (the
+    // surfaces a photon must reach first to ever become caustic); its leaf
+    // triangle ids are not world ids
+    uint32_t spec_n_nodes;
+    const wsrt_bvh8_node* spec_nodes;
+    const float4* spec_tris;
 };
+
+// The scene seen through its specular BVH (for any-hit queries only).
+__host__ __device__ inline SceneView specular_view(const SceneView& sv) {
+    SceneView v = sv;
+    v.nodes = sv.spec_nodes;
+    v.tris = sv.spec_tris;
+    v.n_nodes = sv.spec_n_nodes;
+    return v;
+}
 
 // Per-frame constants uploaded before a frame (kernels read them from device
 // memory so a captured frame graph stays valid across frames).
@@ -92,8 +106,8 @@ struct PathState {
     uint32_t capacity;  // path slots (frames in flight x pixels)
 };
 
-// Photon paths between bounces (structure of arrays indexed by state slot;
-// slot s carries photon path pid[s]).
+// Photon paths between bounces (structure of arrays indexed by photon path
+// id).
 struct PhotonState {
     float4* pos;        // origin xyz, wavelength
     float4* dir;        // direction xyz, energy
@@ -101,7 +115,6 @@ struct PhotonState {
     uint32_t* flags;    // 1: caustic path
     uint32_t* queue_a;
     uint32_t* queue_b;
-    uint32_t* pid;      // photon path id of a state slot
     uint32_t* stored;   // per photon path: 1 once its photon is stored in its slot
 };
 

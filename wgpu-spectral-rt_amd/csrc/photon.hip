@@ -18,125 +18,103 @@ static_assert(kTraceBlocks <= kStatBlocks, "per-block statistics of the photon b
 
 namespace {
 
-// Octahedral bin (8 x 8, +Y pole) of a unit direction.
-__device__ __forceinline__ uint32_t direction_bin(f3 d) {
-    const float l1 = fabsf(d.x) + fabsf(d.y) + fabsf(d.z);
-    float px = d.x / l1, pz = d.z / l1;
-    if (d.y < 0.0f) {
-        const float qx = (1.0f - fabsf(pz)) * (px < 0.0f ? -1.0f : 1.0f);
-        const float qz = (1.0f - fabsf(px)) * (pz < 0.0f ? -1.0f : 1.0f);
-        px = qx;
-        pz = qz;
-    }
-    const uint32_t bx = (uint32_t)fminf(fmaxf((px * 0.5f + 0.5f) * 8.0f, 0.0f), 7.0f);
-    const uint32_t bz = (uint32_t)fminf(fmaxf((pz * 0.5f + 0.5f) * 8.0f, 0.0f), 7.0f);
-    return bx * 8u + bz;
-}
-
-constexpr int kEmitThreads = 1024;
-
 // Emission of photon path i (photon_emit.wgsl:159-216): light choice, point
-// on the light, cosine direction, hero wavelength and flux. Every path's
+// on the light, cosine direction, hero wavelength and flux; every path's
 // stored flag is cleared (its photon slot holds a photon only once the flag
-// is set); with lights present every path takes part in the first bounce.
-// Each block also
-// orders its paths by emission-direction bin (a counting sort in LDS) and
-// stores their state in that order, with the path id beside it, so the waves
-// of the first bounce trace neighbouring directions from contiguous state: a
-// path's result depends only on its id, so the order in which the bounce
-// kernel takes paths up changes nothing but divergence.
-__global__ __launch_bounds__(kEmitThreads) void k_photon_emit(SceneView sv, FrameArgs fa, uint32_t np,
-                                                              uint32_t nf, PhotonState ps) {
-    __shared__ uint32_t hist[64], cursor[64];
-    __shared__ float4 s_pos[kEmitThreads], s_dir[kEmitThreads];
-    __shared__ uint32_t s_rng[kEmitThreads], s_id[kEmitThreads];
-    if (threadIdx.x < 64) hist[threadIdx.x] = 0u;
-    __syncthreads();
-    float4 my_pos = splat4(0.0f), my_dir = splat4(0.0f);
-    uint32_t my_rng = 0u;
-    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = np * nf;
-    uint32_t key = 0u;
-    if (id < n) {
-        const uint32_t base_frame = *fa.frame;
-        const FrameConst& f0 = fa.table[base_frame & fa.mask];
-        const uint32_t slot = id / np, local = id - slot * np;
-        uint32_t rng = pcg_hash(local + (base_frame + slot) * 927163u);
-        ps.stored[id] = 0u;
-        const uint32_t nl = sv.n_lights;
-        if (nl != 0u) {
-            uint32_t li = (uint32_t)(rand01(rng) * (float)nl);
-            if (li > nl - 1u) li = nl - 1u;
-            const wsrt_light_info& L = sv.lights[li];
-            const float4 v0 = *reinterpret_cast<const float4*>(L.v0);
-            const float4 prm = *reinterpret_cast<const float4*>(L.params);
-            f3 emit_pos, emit_dir;
-            float area;
-            if (prm.x == 1.0f) {  // point light (extension): uniform sphere
-                emit_pos = xyz(v0);
-                emit_dir = random_unit_vector(rng);
-                area = 4.0f;
-            } else {
-                const float4 v1 = *reinterpret_cast<const float4*>(L.v1);
-                const float4 v2 = *reinterpret_cast<const float4*>(L.v2);
-                f3 a = xyz(v0), b = xyz(v1), c = xyz(v2);
-                f3 e1 = b - a, e2 = c - a;
-                f3 nrm = normalize(cross(e1, e2));
-                area = length(cross(e1, e2)) * 0.5f;
-                float r1 = rand01(rng), r2 = rand01(rng), sr = sqrtf(r1);
-                float u = 1.0f - sr, v = r2 * sr, w = 1.0f - u - v;
-                emit_pos = a * u + b * v + c * w;
-                emit_dir = normalize(nrm + random_unit_vector(rng));
+// is set). A photon is stored only after a specular interaction, so a path
+// whose first ray meets no metal, glass or thin-film triangle ends at its
+// first hit without storing anything. A test against the root of the
+// specular BVH finds most of those here (the same cost for every lane), and
+// only the others enter the depth-0 queue with their state (at state slot
+// i). Grid-stride over exactly kGridLanes lanes, which bounds each queue
+// segment.
+__global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon_emit(SceneView sv, FrameArgs fa, uint32_t np,
+                                                                uint32_t nf, Queues qs, PhotonState ps,
+                                                                uint32_t count_traversal,
+                                                                Counters* __restrict__ ctr,
+                                                                uint2* __restrict__ gstack) {
+    __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
+    __shared__ unsigned long long red[kTraceThreads / 64];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack st;
+    st.lds = lds_slot(lds_stack + threadIdx.x);
+    st.lds_stride = kTraceThreads;
+    st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
+    const SceneView spec = specular_view(sv);
+    const uint32_t n = np * nf, nl = sv.n_lights;
+    const uint32_t base_frame = *fa.frame;
+    const FrameConst& f0 = fa.table[base_frame & fa.mask];
+    const CauchyConst cc{f0.cdf_min, f0.cdf_range};
+    const bool count = count_traversal != 0u;
+    uint32_t my_rays = 0, my_nodes = 0, my_tris = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = gtid & ~63u; base < n; base += stride) {
+        const uint32_t id = base + __lane_id();
+        bool go = false;
+        if (id < n) {
+            ps.stored[id] = 0u;
+            if (nl != 0u) {
+                const uint32_t slot = id / np, local = id - slot * np;
+                uint32_t rng = pcg_hash(local + (base_frame + slot) * 927163u);
+                uint32_t li = (uint32_t)(rand01(rng) * (float)nl);
+                if (li > nl - 1u) li = nl - 1u;
+                const wsrt_light_info& L = sv.lights[li];
+                const float4 v0 = *reinterpret_cast<const float4*>(L.v0);
+                const float4 prm = *reinterpret_cast<const float4*>(L.params);
+                f3 emit_pos, emit_dir;
+                float area;
+                if (prm.x == 1.0f) {  // point light (extension): uniform sphere
+                    emit_pos = xyz(v0);
+                    emit_dir = random_unit_vector(rng);
+                    area = 4.0f;
+                } else {
+                    const float4 v1 = *reinterpret_cast<const float4*>(L.v1);
+                    const float4 v2 = *reinterpret_cast<const float4*>(L.v2);
+                    f3 a = xyz(v0), b = xyz(v1), c = xyz(v2);
+                    f3 e1 = b - a, e2 = c - a;
+                    f3 nrm = normalize(cross(e1, e2));
+                    area = length(cross(e1, e2)) * 0.5f;
+                    float r1 = rand01(rng), r2 = rand01(rng), sr = sqrtf(r1);
+                    float u = 1.0f - sr, v = r2 * sr, w = 1.0f - u - v;
+                    emit_pos = a * u + b * v + c * w;
+                    emit_dir = normalize(nrm + random_unit_vector(rng));
+                }
+                const float wl = cauchy_sample(rand01(rng), cc);
+                const float pdf = cauchy_pdf(wl, cc);
+                const float radiance = blackbody(wl, prm.y);
+                const float flux = (radiance / pdf) * area * kPi;
+                const float energy = flux * prm.z;
+                const f3 pos = emit_pos + emit_dir * 0.001f;
+                // one visit of the specular BVH's root: a ray that meets none
+                // of its child boxes (nor a triangle of a leaf child) cannot
+                // reach a specular triangle; the others are traced in full
+                Trav t;
+                trav_init(t, spec, pos, emit_dir, 0.0001f, 100.0f);
+                if (t.live) {
+                    if (count) trav_step<true, true>(spec, t, st, my_nodes, my_tris);
+                    else trav_step<true, false>(spec, t, st, my_nodes, my_tris);
+                }
+                my_rays++;
+                if (t.live || t.best.tri != kInvalid) {
+                    go = true;
+                    ps.pos[id] = f4(pos, wl);
+                    ps.dir[id] = f4(emit_dir, energy);
+                    ps.rng[id] = rng;
+                    ps.flags[id] = 0u;
+                }
             }
-            const CauchyConst cc{f0.cdf_min, f0.cdf_range};
-            float wl = cauchy_sample(rand01(rng), cc);
-            float pdf = cauchy_pdf(wl, cc);
-            float radiance = blackbody(wl, prm.y);
-            float flux = (radiance / pdf) * area * kPi;
-            float energy = flux * prm.z;
-            f3 pos = emit_pos + emit_dir * 0.001f;
-            my_pos = f4(pos, wl);
-            my_dir = f4(emit_dir, energy);
-            my_rng = rng;
-            key = direction_bin(emit_dir);
         }
-        atomicAdd(&hist[key], 1u);
+        const uint32_t e = seg_append(qs, 0, kQPhoton, qs.pseg_cap, go);
+        if (go) ps.queue_a[e] = id;
     }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        const uint32_t v = hist[threadIdx.x];
-        uint32_t inc = v;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t t = __shfl_up(inc, off, 64);
-            if ((int)threadIdx.x >= off) inc += t;
-        }
-        cursor[threadIdx.x] = inc - v;
-    }
-    __syncthreads();
-    // the block's paths in bin order, staged in LDS and stored contiguously:
-    // state slot base + t holds photon path s_id[t]
-    if (id < n) {
-        const uint32_t rank = atomicAdd(&cursor[key], 1u);
-        s_pos[rank] = my_pos;
-        s_dir[rank] = my_dir;
-        s_rng[rank] = my_rng;
-        s_id[rank] = id;
-    }
-    __syncthreads();
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot < n) {
-        ps.pos[slot] = s_pos[threadIdx.x];
-        ps.dir[slot] = s_dir[threadIdx.x];
-        ps.rng[slot] = s_rng[threadIdx.x];
-        ps.flags[slot] = 0u;
-        ps.pid[slot] = s_id[threadIdx.x];
-    }
+    if (count) flush_traversal_counts(ctr, 0, my_rays, my_nodes, my_tris);
+    const unsigned long long br = block_sum(my_rays, red);
+    if (threadIdx.x == 0) ctr->photon_rays_part[blockIdx.x] += br;
 }
 
 // Register state of a photon path between its load and the end of its ray.
 struct PhotonRay {
-    uint32_t sid, id;  // state slot, photon path id
+    uint32_t id;  // photon path id, also its state slot
     float wl, energy;
     uint32_t rng;
     bool caustic;
@@ -214,10 +192,10 @@ __device__ __forceinline__ bool photon_interact(const SceneView& sv, const Photo
         // the origin is not advanced in this branch (photon_emit.wgsl:291-326)
     }
     if (alive) {
-        ps.pos[r.sid] = f4(pos, wl);
-        ps.dir[r.sid] = f4(dir, r.energy);
-        ps.rng[r.sid] = rng;
-        ps.flags[r.sid] = caustic ? 1u : 0u;
+        ps.pos[r.id] = f4(pos, wl);
+        ps.dir[r.id] = f4(dir, r.energy);
+        ps.rng[r.id] = rng;
+        ps.flags[r.id] = caustic ? 1u : 0u;
     }
     return alive;
 }
@@ -243,18 +221,15 @@ __device__ __forceinline__ void photon_bounce_range(const SceneView& sv, uint32_
             if (!has) {
                 const uint32_t i = cursor + (uint32_t)__popcll(idle & lanemask_lt());
                 if (i < end) {
-                    // depth 0 reads state slots directly (in emission-direction
-                    // bin order within each emit block), later depths the queue
-                    r.sid = depth == 0 ? i : queue[seg_entry(pre, kSegs, i, qs.pseg_cap)];
-                    r.id = ps.pid[r.sid];
-                    const float4 pw = ps.pos[r.sid], de = ps.dir[r.sid];
+                    r.id = queue[seg_entry(pre, kSegs, i, qs.pseg_cap)];
+                    const float4 pw = ps.pos[r.id], de = ps.dir[r.id];
                     r.wl = pw.w;
                     r.energy = de.w;
-                    r.rng = ps.rng[r.sid];
-                    r.caustic = ps.flags[r.sid] != 0u;
+                    r.rng = ps.rng[r.id];
+                    r.caustic = ps.flags[r.id] != 0u;
                     trav_init(t, sv, xyz(pw), xyz(de), 0.0001f, 100.0f);
                     has = true;
-                    my_rays++;
+                    if (depth != 0) my_rays++;  // the emit kernel counted the first rays
                 }
             }
             cursor += nidle;
@@ -269,17 +244,16 @@ __device__ __forceinline__ void photon_bounce_range(const SceneView& sv, uint32_
             }
         }
         const uint32_t e = seg_append(qs, depth + 1, kQPhoton, qs.pseg_cap, alive);
-        if (alive) next_queue[e] = r.sid;
+        if (alive) next_queue[e] = r.id;
     }
 }
 
 // One bounce of every queued photon path (photon_emit.wgsl:219-330): closest
 // hit, then the surface interaction. Survivors are compacted into the next
 // queue, so the few paths refracting through glass run in full waves. Each
-// wave takes a contiguous range of entries: at depth 0 every wave then sees
-// all emission-direction bins, and the paths of one bin share waves.
+// wave takes a contiguous range of the queue.
 __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon_bounce(
-    SceneView sv, uint32_t depth, uint32_t n0, Queues qs, uint32_t count_traversal, PhotonState ps,
+    SceneView sv, uint32_t depth, Queues qs, uint32_t count_traversal, PhotonState ps,
     const uint32_t* __restrict__ queue, uint32_t* __restrict__ next_queue, wsrt_photon* __restrict__ photons,
     Counters* __restrict__ ctr, uint2* __restrict__ gstack) {
     __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
@@ -290,7 +264,7 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon
     st.lds = lds_slot(lds_stack + threadIdx.x);
     st.lds_stride = kTraceThreads;
     st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
-    const uint32_t n = depth == 0 ? n0 : seg_prefix(qs, depth, kQPhoton, 1, pre);
+    const uint32_t n = seg_prefix(qs, depth, kQPhoton, 1, pre);
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     const uint32_t nchunks = (n + 63u) >> 6;
     const uint32_t per_wave = (nchunks + nwaves - 1u) / nwaves;
@@ -409,20 +383,18 @@ __global__ __launch_bounds__(256) void k_grid_copy(const wsrt_photon* __restrict
 
 }  // namespace
 
-void launch_photon_emit(const SceneView& sv, const FrameArgs& fa, uint32_t np, uint32_t nf, const PhotonState& ps,
+void launch_photon_emit(const SceneView& sv, const FrameArgs& fa, uint32_t np, uint32_t nf, const Queues& qs,
+                        const PhotonState& ps, uint32_t count_traversal, Counters* ctr, uint2* gstack,
                         hipStream_t s) {
-    const uint32_t n = np * nf;
-    hipLaunchKernelGGL(k_photon_emit, dim3((n + kEmitThreads - 1) / kEmitThreads), dim3(kEmitThreads), 0, s, sv,
-                       fa, np, nf, ps);
+    hipLaunchKernelGGL(k_photon_emit, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, sv, fa, np, nf, qs, ps,
+                       count_traversal, ctr, gstack);
 }
 
-void launch_photon_bounce(const SceneView& sv, uint32_t np, uint32_t nf, uint32_t depth,
-                          uint32_t count_traversal, const PhotonState& ps, const Queues& qs, wsrt_photon* photons,
-                          Counters* ctr, uint2* gstack, hipStream_t s) {
-    const uint32_t n0 = sv.n_lights ? np * nf : 0u;
+void launch_photon_bounce(const SceneView& sv, uint32_t depth, uint32_t count_traversal, const PhotonState& ps,
+                          const Queues& qs, wsrt_photon* photons, Counters* ctr, uint2* gstack, hipStream_t s) {
     const uint32_t* q = (depth & 1u) ? ps.queue_b : ps.queue_a;
     uint32_t* nq = (depth & 1u) ? ps.queue_a : ps.queue_b;
-    hipLaunchKernelGGL(k_photon_bounce, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, sv, depth, n0, qs,
+    hipLaunchKernelGGL(k_photon_bounce, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, sv, depth, qs,
                        count_traversal, ps, q, nq, photons, ctr, gstack);
 }
 
