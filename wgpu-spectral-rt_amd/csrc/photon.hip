@@ -1,13 +1,17 @@
 // photon.hip - the caustic photon pass (photon_emit.wgsl:154-331) and the
 // photon grid (build_grid.wgsl:25-73) rebuilt as a counting sort.
 //
-// Photon path i writes its stored photon (or energy 0) to slot i, so the
-// photon buffer is deterministic; the reference appends with atomicAdd in
-// arbitrary order, which holds the same multiset. The grid is then:
+// Photon path i writes its stored photon to slot i and sets its stored
+// flag, so the photon buffer is deterministic; the reference appends with
+// atomicAdd in arbitrary order, which holds the same multiset. The grid of a
+// dense photon buffer (wsrt_photon_grid_build; energy 0 marks an empty slot):
 //   hash    cell_id[i] and an atomic histogram count[cell]
 //   scan    offset = exclusive prefix sum of count (wave-level scans)
 //   sort    stable radix sort of (cell, slot), so cells list slots ascending
 //   copy    photons in cell order, so a gather reads one contiguous range.
+// The renderer's grid hashes only the slots whose stored flag is set and
+// writes the offsets of non-empty cells from the run starts of the sorted
+// keys (the gather reads an offset only when the cell's count is not 0).
 #include "kernels.hpp"
 #include "queues.hpp"
 #include "shading.hpp"
