@@ -162,7 +162,12 @@ typedef struct wsrt_config {
     uint32_t env_map;             /* 1: scene environment map lights misses    */
     uint32_t frame_graphs;        /* 1: submit each batch as one HIP graph     */
     uint32_t frames_in_flight;    /* consecutive frames traced in one wavefront */
-    uint32_t reserved[4];
+    uint32_t tail_paths;          /* once fewer camera paths than this enter a
+                                   * bounce (bounce 2 on), one launch finishes
+                                   * them; 0: every bounce in the wavefront.
+                                   * Not used with guiding. Results do not
+                                   * depend on it.                             */
+    uint32_t reserved[3];
 } wsrt_config;
 
 /* Fills *cfg with the reference's defaults. */

@@ -43,14 +43,14 @@ class Config(C.Structure):
                 ("guide_mix", C.c_float), ("guide_lr", C.c_float),
                 ("guide_train_samples", C.c_uint32), ("env_map", C.c_uint32),
                 ("frame_graphs", C.c_uint32), ("frames_in_flight", C.c_uint32),
-                ("reserved", C.c_uint32 * 4)]
+                ("tail_paths", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 def default_config(**kw) -> Config:
     """The reference's compile-time constants (renderer/mod.rs:18-19,
     compute_pass.rs:26, build_grid.wgsl:23, shader.wgsl:274-342,696)."""
     c = Config(C.sizeof(Config), 12, 5, 1 << 20, 1 << 22, 0.02, 0.02, -0.15, 20.0, 64, 3, 1,
-               1 << 20, 0, 0.5, 1e-3, 1 << 18, 0, 1, 8)
+               1 << 20, 0, 0.5, 1e-3, 1 << 18, 0, 1, 8, 1 << 18)
     for k, v in kw.items():
         setattr(c, k, v)
     return c

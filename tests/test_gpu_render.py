@@ -263,6 +263,32 @@ def test_frames_in_flight_do_not_change_results(device, name):
     np.testing.assert_array_equal(accs[0], accs[2])
 
 
+@pytest.mark.parametrize("name", ["cornell", "caustic", "heightfield"])
+def test_tail_launch_does_not_change_results(device, name):
+    """Paths finished by the tail launch (from the bounce where fewer than
+    tail_paths remain) accumulate bit for bit like paths traced bounce by
+    bounce in the wavefront: tail off, tail from bounce 2 on, and a threshold
+    that switches at a later bounce."""
+    print(f"tail launch {name}", flush=True)
+    W = H = 48
+    g, _, cfg, view = setup(device, name, W, H, 1 << 14)
+    out = []
+    for tail in (0, 0xFFFFFFFF, 300):
+        c = wsrt.Config.from_buffer_copy(bytes(cfg))
+        c.frames_in_flight = 2
+        c.tail_paths = tail
+        r = wsrt.Renderer(device, g, W, H, c)
+        r.set_camera(scenes.camera(view, 1))
+        r.render_frames(4)
+        out.append((r.read_accum(), r.read_paths()[1], r.stats()))
+    for acc, L, st in out[1:]:
+        np.testing.assert_array_equal(acc, out[0][0])
+        np.testing.assert_array_equal(L, out[0][1])
+        # the same rays, counted by whichever launch traced them
+        assert st["extend_rays"] == out[0][2]["extend_rays"]
+        assert st["shadow_rays"] == out[0][2]["shadow_rays"]
+
+
 def test_autotune_picks_a_measured_candidate(device, cornell_pair):
     g, _ = cornell_pair
     cfg = wsrt.default_config(photons_per_pass=1 << 14)

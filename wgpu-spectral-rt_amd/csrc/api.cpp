@@ -758,6 +758,7 @@ extern "C" int wsrt_config_default(wsrt_config* c) {
     c->env_map = 0;
     c->frame_graphs = 1;
     c->frames_in_flight = 8;
+    c->tail_paths = 1u << 18;
     return WSRT_OK;
 }
 
@@ -837,11 +838,11 @@ static FrameConst make_frame_const(const wsrt_config& c, const wsrt_camera_unifo
 namespace {
 enum KernelId {
     K_FRAME_PREP, K_PHOTON_EMIT, K_PHOTON_TRACE, K_GRID_BUILD, K_GENERATE, K_EXTEND, K_SHADE, K_SHADOW,
-    K_ACCUMULATE, K_GUIDE_SCATTER, K_GUIDE_TRAIN, K_COUNT
+    K_ACCUMULATE, K_GUIDE_SCATTER, K_GUIDE_TRAIN, K_TAIL, K_COUNT
 };
 const char* kKernelNames[K_COUNT] = {"frame_prep", "photon_emit", "photon_trace", "grid_build", "generate",
                                      "extend", "shade", "shadow", "accumulate", "guide_scatter",
-                                     "guide_train"};
+                                     "guide_train", "tail"};
 constexpr uint32_t kUploadHalf = kFrameTable / 2;  // frames per upload (staging half)
 struct Pending {
     int kid;
@@ -1115,8 +1116,9 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     HIP_OR_FAIL(r->shade_q.alloc(4ull * kSegs * seg * kMatBins));
     HIP_OR_FAIL(r->shadow.alloc(48ull * kSegs * seg));
     const uint32_t depths = std::max(c.max_depth, c.photon_max_depth) + 1;
-    HIP_OR_FAIL(r->qcounts.alloc(4 * queue_count_words(depths)));
-    HIP_OR_FAIL(hipMemsetAsync(r->qcounts.p, 0, 4 * queue_count_words(depths), r->stream));
+    // the counters, then the tail word on a line of its own
+    HIP_OR_FAIL(r->qcounts.alloc(4 * (queue_count_words(depths) + kSegStride)));
+    HIP_OR_FAIL(hipMemsetAsync(r->qcounts.p, 0, 4 * (queue_count_words(depths) + kSegStride), r->stream));
     HIP_OR_FAIL(r->accum.alloc(16 * npx));
     HIP_OR_FAIL(r->rgba.alloc(4 * npx));
     HIP_OR_FAIL(r->counters.alloc(sizeof(Counters)));
@@ -1174,6 +1176,10 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     pl.count_traversal = 0;
     pl.qs.counts = r->qcounts.as<uint32_t>();
     pl.qs.seg_cap = seg;
+    // guided paths scatter in their own kernel between shade and shadow, so
+    // they stay in the wavefront to the end
+    pl.qs.tail = (c.guiding || c.tail_paths == 0) ? nullptr : pl.qs.counts + queue_count_words(depths);
+    pl.qs.tail_paths = c.tail_paths;
     pl.guided = c.guiding;
     if (c.guiding) {
         const uint64_t nrec = npaths * guide::kRecBounces;
@@ -1306,6 +1312,10 @@ static int enqueue_batch(wsrt_renderer* r, uint32_t nf) {
             ProfScope p(r, K_GUIDE_SCATTER);
             launch_guide_scatter(pl.ps, pl.qs, pl.gv, pl.fa, b, nq, s);
         }
+    }
+    if (pl.qs.tail && r->cfg.max_depth > kTailFirstBounce) {
+        ProfScope p(r, K_TAIL);
+        launch_tail(pl, npx, r->cfg.max_depth, s);
     }
     {
         ProfScope p(r, K_ACCUMULATE);

@@ -29,7 +29,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
     st.lds = lds_slot(lds_stack + threadIdx.x);
     st.lds_stride = kBatchThreads;
     st.glob = a.gstack + (uint64_t)gtid * a.spill;
-    uint32_t dn = 0, dt = 0;
+    RayTally tally;
     while (true) {
         uint32_t item = 0;
         if (lane == 0) item = atomicAdd(a.head, 1u);
@@ -63,20 +63,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
             const FrameConst& fc = a.table[J.table_base + f];
             PathRegs r;
             camera_path(fc, x, y, r);
-            for (uint32_t b = 0; b < a.max_depth; ++b) {
-                const Hit h = trace_ray<false>(sv, xyz(r.o4), xyz(r.d4), 0.0001f, 100.0f, st, false, dn, dt);
-                if (h.tri == kInvalid) {
-                    if (sv.env) r.L = add4(r.L, mul4(r.tp, env_radiance(sv, xyz(r.d4), r.wl)));
-                    break;
-                }
-                const VertexOut vo = shade_vertex(sv, none, nullptr, 0u, fc.rr_after, b, false, h.t, h.tri,
-                                                  h.front, h.u, h.v, r);
-                if (vo.shadow) {
-                    const Hit s = trace_ray<true>(sv, vo.hp, vo.sdir, 0.001f, vo.stmax, st, false, dn, dt);
-                    if (s.tri == kInvalid) r.L = add4(r.L, vo.contrib);
-                }
-                if (!vo.alive) break;
-            }
+            trace_bounces(sv, none, nullptr, 0u, fc.rr_after, 0u, a.max_depth, r, st, false, tally);
             acc = accumulate_sample(acc, fc, r.wl, r.L);
         }
         J.accum[pix] = acc;

@@ -180,6 +180,9 @@ void launch_generate(const PathLaunch& pl, uint32_t npx, uint32_t nf, hipStream_
 void launch_extend(const PathLaunch& pl, uint32_t bounce, uint32_t n0, const uint32_t* queue, hipStream_t s);
 void launch_shade(const PathLaunch& pl, uint32_t npx, uint32_t bounce, uint32_t* next_queue, hipStream_t s);
 void launch_shadow(const PathLaunch& pl, uint32_t bounce, hipStream_t s);
+// The camera paths the wavefront left to the tail launch (Queues::tail),
+// each finished by one lane.
+void launch_tail(const PathLaunch& pl, uint32_t npx, uint32_t max_depth, hipStream_t s);
 void launch_accumulate(const PathLaunch& pl, uint32_t npx, uint32_t nf, hipStream_t s);
 
 }  // namespace wsrt

@@ -29,9 +29,10 @@ static_assert((uint64_t)kTraceBlocks * kTraceThreads == kGridLanes, "queue produ
 
 // Clears the queue counters of a batch (a fill kernel, so it can be
 // captured in a graph).
-__global__ void k_clear_queues(uint32_t* counts, uint32_t words) {
+__global__ void k_clear_queues(uint32_t* counts, uint32_t words, uint32_t* tail) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x)
         counts[i] = 0u;
+    if (tail && blockIdx.x == 0 && threadIdx.x == 0) *tail = 0u;
 }
 
 // Last node of a batch: the next batch starts at frame_count + nf.
@@ -75,6 +76,10 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend
     st.lds_stride = kTraceThreads;
     st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
     const uint32_t n = bounce == 0 ? n0 : seg_prefix(qs, bounce, kQActive, 1, pre);
+    if (qs.tail && bounce >= kTailFirstBounce && (*qs.tail != 0u || n < qs.tail_paths)) {
+        if (gtid == 0 && *qs.tail == 0u) *qs.tail = bounce + 1u;  // the tail launch takes these paths
+        return;
+    }
     if (gtid == 0) atomicAdd(&ctr->extend_rays, (unsigned long long)n);
     const uint32_t stride = gridDim.x * blockDim.x;
     const bool count = count_traversal != 0u;
@@ -130,6 +135,7 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
                                                          uint32_t bounce, uint32_t guided,
                                                          uint32_t* __restrict__ next_queue) {
     __shared__ uint32_t pre[kMatBins * kSegs + 1];
+    if (tail_started(qs, bounce)) return;
     const uint32_t base_frame = *fa.frame;
     const uint32_t rr_after = fa.table[base_frame & fa.mask].rr_after;
     const uint32_t photons_on = fa.table[base_frame & fa.mask].enable_photons;
@@ -204,6 +210,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_shadow(SceneView sv, PathStat
     st.lds = lds_slot(lds_stack + threadIdx.x);
     st.lds_stride = kTraceThreads;
     st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
+    if (tail_started(qs, bounce)) return;
     const uint32_t n = seg_prefix(qs, bounce, kQShadow, 1, pre);
     if (gtid == 0) atomicAdd(&ctr->shadow_rays, (unsigned long long)n);
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -220,6 +227,62 @@ __global__ __launch_bounds__(kTraceThreads) void k_shadow(SceneView sv, PathStat
         }
     }
     if (count) flush_traversal_counts(ctr, 2, my_rays, my_nodes, my_tris);
+}
+
+// The paths entering bounce t-1 = *qs.tail - 1 (if the wavefront handed
+// them over), each carried to its end by one lane with the per-path steps of
+// the wavefront kernels (trace_bounces), photon gather included.
+__global__ __launch_bounds__(kTraceThreads) void k_tail(SceneView sv, PathState ps, GridView grid, FrameArgs fa,
+                                                         uint32_t npx, Queues qs, uint32_t max_depth,
+                                                         uint32_t count_traversal, Counters* __restrict__ ctr,
+                                                         uint2* __restrict__ gstack) {
+    __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
+    __shared__ uint32_t pre[kSegs + 1];
+    const uint32_t ta = *qs.tail;
+    if (ta == 0u || ta - 1u >= max_depth) return;
+    const uint32_t first = ta - 1u;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack st;
+    st.lds = lds_slot(lds_stack + threadIdx.x);
+    st.lds_stride = kTraceThreads;
+    st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
+    const uint32_t n = seg_prefix(qs, first, kQActive, 1, pre);
+    const uint32_t* queue = (first & 1u) ? ps.queue_b : ps.queue_a;
+    const uint32_t base_frame = *fa.frame;
+    const uint32_t rr_after = fa.table[base_frame & fa.mask].rr_after;
+    const uint32_t photons_on = fa.table[base_frame & fa.mask].enable_photons;
+    const bool count = count_traversal != 0u;
+    RayTally tally;
+    for (uint32_t i = gtid; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t p = queue[seg_entry(pre, kSegs, i, qs.seg_cap)];
+        PathRegs r;
+        r.o4 = ps.o[p];
+        r.d4 = ps.d[p];
+        r.tp = ps.tp[p];
+        r.L = ps.L[p];
+        r.wl = ps.wl[p];
+        r.rng = ps.rng[p];
+        r.flags = ps.flags[p];
+        const uint32_t slot = p / npx;
+        const FrameConst* fs = photons_on ? &fa.table[(base_frame + slot) & fa.mask] : nullptr;
+        const uint32_t cell_base = photons_on ? slot * fs->hash_size : 0u;
+        trace_bounces(sv, grid, fs, cell_base, rr_after, first, max_depth, r, st, count, tally);
+        ps.L[p] = r.L;
+    }
+    // statistics: rays per wave, traversal work while counting
+    unsigned long long er = tally.extend_rays, sr = tally.shadow_rays;
+    for (int off = 32; off > 0; off >>= 1) {
+        er += __shfl_xor(er, off, 64);
+        sr += __shfl_xor(sr, off, 64);
+    }
+    if (__lane_id() == 0) {
+        if (er) atomicAdd(&ctr->extend_rays, er);
+        if (sr) atomicAdd(&ctr->shadow_rays, sr);
+    }
+    if (count) {
+        flush_traversal_counts(ctr, 1, tally.extend_rays, tally.extend_nodes, tally.extend_tris);
+        flush_traversal_counts(ctr, 2, tally.shadow_rays, tally.shadow_nodes, tally.shadow_tris);
+    }
 }
 
 // shader.wgsl:771-822, for the frames of the batch in frame order, so the
@@ -246,7 +309,7 @@ __global__ __launch_bounds__(256) void k_accumulate(PathState ps, FrameArgs fa, 
 void launch_clear_queues(const Queues& qs, uint32_t depths, hipStream_t s) {
     const uint32_t words = (uint32_t)queue_count_words(depths);
     hipLaunchKernelGGL(k_clear_queues, dim3((words + 255) / 256 < 1024 ? (words + 255) / 256 : 1024), dim3(256),
-                       0, s, qs.counts, words);
+                       0, s, qs.counts, words, qs.tail);
 }
 void launch_advance_frame(uint32_t* frame, uint32_t nf, hipStream_t s) {
     hipLaunchKernelGGL(k_advance_frame, dim3(1), dim3(64), 0, s, frame, nf);
@@ -269,6 +332,10 @@ void launch_shade(const PathLaunch& pl, uint32_t npx, uint32_t bounce, uint32_t*
 void launch_shadow(const PathLaunch& pl, uint32_t bounce, hipStream_t s) {
     hipLaunchKernelGGL(k_shadow, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, pl.sv, pl.ps, pl.qs,
                        pl.count_traversal, pl.ctr, bounce, pl.gstack);
+}
+void launch_tail(const PathLaunch& pl, uint32_t npx, uint32_t max_depth, hipStream_t s) {
+    hipLaunchKernelGGL(k_tail, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, pl.sv, pl.ps, pl.grid, pl.fa, npx,
+                       pl.qs, max_depth, pl.count_traversal, pl.ctr, pl.gstack);
 }
 void launch_accumulate(const PathLaunch& pl, uint32_t npx, uint32_t nf, hipStream_t s) {
     hipLaunchKernelGGL(k_accumulate, dim3((npx + 255) / 256), dim3(256), 0, s, pl.ps, pl.fa, npx, nf,

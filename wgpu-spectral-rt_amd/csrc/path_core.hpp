@@ -6,6 +6,7 @@
 #include "common.hpp"
 #include "dev_math.hpp"
 #include "shading.hpp"
+#include "traverse.hpp"
 
 namespace wsrt {
 
@@ -144,6 +145,40 @@ __device__ __forceinline__ VertexOut shade_vertex(const SceneView& sv, const Gri
         }
     }
     return out;
+}
+
+// Rays a path loop traced, and their traversal work while counting.
+struct RayTally {
+    uint32_t extend_rays = 0, extend_nodes = 0, extend_tris = 0;
+    uint32_t shadow_rays = 0, shadow_nodes = 0, shadow_tris = 0;
+};
+
+// Bounces first .. max_depth-1 of one path, each in turn: the closest hit
+// (a miss ends the path, adding the environment, shader.wgsl:566), the
+// vertex, and its shadow ray. The same steps, in the same order, as one
+// path's share of the wavefront kernels extend, shade and shadow.
+__device__ __forceinline__ void trace_bounces(const SceneView& sv, const GridView& grid, const FrameConst* fs,
+                                              uint32_t cell_base, uint32_t rr_after, uint32_t first,
+                                              uint32_t max_depth, PathRegs& r, Stack st, bool count,
+                                              RayTally& tally) {
+    for (uint32_t b = first; b < max_depth; ++b) {
+        const Hit h = trace_ray<false>(sv, xyz(r.o4), xyz(r.d4), 0.0001f, 100.0f, st, count, tally.extend_nodes,
+                                       tally.extend_tris);
+        tally.extend_rays++;
+        if (h.tri == kInvalid) {
+            if (sv.env) r.L = add4(r.L, mul4(r.tp, env_radiance(sv, xyz(r.d4), r.wl)));
+            break;
+        }
+        const VertexOut vo =
+            shade_vertex(sv, grid, fs, cell_base, rr_after, b, false, h.t, h.tri, h.front, h.u, h.v, r);
+        if (vo.shadow) {
+            const Hit s = trace_ray<true>(sv, vo.hp, vo.sdir, 0.001f, vo.stmax, st, count, tally.shadow_nodes,
+                                          tally.shadow_tris);
+            tally.shadow_rays++;
+            if (s.tri == kInvalid) r.L = add4(r.L, vo.contrib);
+        }
+        if (!vo.alive) break;
+    }
 }
 
 // One sample into the running accumulation (shader.wgsl:771-808): hero

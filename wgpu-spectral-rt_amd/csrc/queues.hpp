@@ -37,7 +37,22 @@ struct Queues {
     uint32_t* counts;  // [depth][kind][segment] counters, kSegStride apart
     uint32_t seg_cap;  // camera queues: entries per segment
     uint32_t pseg_cap; // photon queues: entries per segment
+    // camera bounce + 1 from which one persistent launch finishes the
+    // remaining paths (0: not yet); null where the tail launch is not used
+    uint32_t* tail = nullptr;
+    // once fewer camera paths than this enter a bounce (from bounce
+    // kTailFirstBounce on), the wavefront kernels of that and later bounces
+    // leave them to the tail launch: a launch per bounce for a few thousand
+    // paths costs its launch and latency, not its work
+    uint32_t tail_paths = 0;
 };
+
+constexpr uint32_t kTailFirstBounce = 2;
+
+// Whether the tail launch has taken over (uniform across a launch).
+__device__ __forceinline__ bool tail_started(const Queues& q, uint32_t bounce) {
+    return q.tail && bounce >= kTailFirstBounce && *q.tail != 0u;
+}
 
 __host__ __device__ inline uint32_t seg_capacity(uint64_t capacity) {
     return (uint32_t)((kGridLanes / kSegs) * ((capacity + kGridLanes - 1) / kGridLanes));
