@@ -377,11 +377,12 @@ def main() -> int:
     r.render_frames(args.steps)
     r.sync()
     dt_graph = time.perf_counter() - t1
-    # traversal work per ray from the in-kernel counters (pinned against the
-    # oracle's traversal of the same BVH8), measured on one extra frame
+    # traversal work per ray of each launching kernel from the in-kernel
+    # counters (pinned against the oracle's traversal of the same BVH8),
+    # measured on one extra batch
     r.reset_stats()
     r.set_traversal_counting(True)
-    r.render_frames(1)
+    r.render_frames(cfg.frames_in_flight)
     r.sync()
     cnt = r.stats()
     r.set_traversal_counting(False)
@@ -389,28 +390,35 @@ def main() -> int:
     samples = v.width * v.height * args.steps
     msps = samples / dt / 1e6
     rays = st["extend_rays"] + st["shadow_rays"] + st["photon_rays"]
-    kinds = {"photon_trace": 0, "extend": 1, "shadow": 2}
+    # the launching kernels' counter indices (wsrt_frame_stats.kernel_*)
+    kinds = {"photon_trace": [0], "extend": [1], "shadow": [2], "photon_emit": [3], "tail": [4, 5]}
     per_ray = {}
-    for k, i in kinds.items():
-        n = max(cnt["trav_rays"][i], 1)
-        per_ray[k] = {"nodes": cnt["trav_nodes"][i] / n, "tris": cnt["trav_tris"][i] / n}
-    ray_counts = {"photon_trace": st["photon_rays"], "extend": st["extend_rays"],
-                  "shadow": st["shadow_rays"]}
+    ray_counts, ray_bytes = {}, {}
+    for k, idx in kinds.items():
+        counted = sum(cnt["kernel_counted_rays"][i] for i in idx)
+        nodes = sum(cnt["kernel_nodes"][i] for i in idx)
+        tris = sum(cnt["kernel_tris"][i] for i in idx)
+        per_ray[k] = {"nodes": nodes / max(counted, 1), "tris": tris / max(counted, 1)}
+        ray_counts[k] = sum(st["kernel_rays"][i] for i in idx)
+        # bytes of the timed rays: each counter's rays at its own work per ray
+        ray_bytes[k] = sum(st["kernel_rays"][i] * (RAY_BYTES + (cnt["kernel_nodes"][i] * NODE_BYTES +
+                                                                cnt["kernel_tris"][i] * TRI_BYTES) /
+                                                   max(cnt["kernel_counted_rays"][i], 1)) for i in idx)
     kern = {}
     for name, p in prof.items():
         if p["launches"] == 0:
             continue
         e = {"ms": round(p["total_ms"], 3), "launches": p["launches"],
              "avg_us": round(1000 * p["total_ms"] / p["launches"], 2)}
-        if name in per_ray:
-            b = RAY_BYTES + per_ray[name]["nodes"] * NODE_BYTES + per_ray[name]["tris"] * TRI_BYTES
-            e["bytes_per_ray"] = round(b, 1)
-            e["GBps"] = round(ray_counts[name] * b / (p["total_ms"] * 1e-3) / 1e9, 1)
+        if name in per_ray and ray_counts[name]:
+            e["rays"] = ray_counts[name]
+            e["bytes_per_ray"] = round(ray_bytes[name] / ray_counts[name], 1)
+            e["GBps"] = round(ray_bytes[name] / (p["total_ms"] * 1e-3) / 1e9, 1)
         if name == "grid_build":  # 60 N + 12 C bytes per frame, launches cover frames_in_flight frames
             n, c = cfg.photons_per_pass, cfg.hash_size
             e["GBps"] = round((60 * n + 12 * c) * args.steps / (p["total_ms"] * 1e-3) / 1e9, 1)
         kern[name] = e
-    dom = max((k for k in kern if k in per_ray), key=lambda k: kern[k]["ms"])
+    dom = max((k for k in kern if "GBps" in kern[k] and k in per_ray), key=lambda k: kern[k]["ms"])
     d = kern[dom]
     achieved = d["GBps"]
     result = {

@@ -292,10 +292,20 @@ typedef struct wsrt_frame_stats {
     uint64_t photons_stored; /* photons written to the grid                */
     uint64_t guide_samples;  /* guide-sampled directions                   */
     /* BVH8 nodes visited and triangles tested, while counting is enabled:
-     * [0] photon rays, [1] camera extend rays, [2] shadow rays */
+     * [0] photon rays, [1] camera extend rays, [2] shadow rays; trav_rays
+     * counts traversals (a photon's first ray is tested against the
+     * specular BVH's root, and traversed in full if it may reach it) */
     uint64_t trav_nodes[3];
     uint64_t trav_tris[3];
     uint64_t trav_rays[3];
+    /* The same per launching kernel: [0] photon bounce, [1] extend, [2]
+     * shadow, [3] photon emission (specular root test), [4] tail closest
+     * hit, [5] tail any hit. kernel_rays is always counted, the others
+     * while counting is enabled. */
+    uint64_t kernel_rays[6];
+    uint64_t kernel_nodes[6];
+    uint64_t kernel_tris[6];
+    uint64_t kernel_counted_rays[6];
 } wsrt_frame_stats;
 int wsrt_renderer_get_stats(wsrt_renderer* r, wsrt_frame_stats* out);
 int wsrt_renderer_reset_stats(wsrt_renderer* r);

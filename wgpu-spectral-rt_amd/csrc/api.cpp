@@ -1525,10 +1525,20 @@ extern "C" int wsrt_renderer_get_stats(wsrt_renderer* r, wsrt_frame_stats* out) 
         out->photons_stored += c.photons_stored_part[b];
     }
     out->guide_samples = c.guide_samples;
+    // by ray type: photon (bounce + emit root tests), camera extend (wavefront
+    // + tail), shadow (wavefront + tail)
+    const int by_type[3][2] = {{kRkPhotonBounce, kRkPhotonEmit}, {kRkExtend, kRkTailExtend},
+                               {kRkShadow, kRkTailShadow}};
     for (int k = 0; k < 3; ++k) {
-        out->trav_nodes[k] = c.trav_nodes[k];
-        out->trav_tris[k] = c.trav_tris[k];
-        out->trav_rays[k] = c.trav_rays[k];
+        out->trav_nodes[k] = c.trav_nodes[by_type[k][0]] + c.trav_nodes[by_type[k][1]];
+        out->trav_tris[k] = c.trav_tris[by_type[k][0]] + c.trav_tris[by_type[k][1]];
+        out->trav_rays[k] = c.trav_rays[by_type[k][0]] + c.trav_rays[by_type[k][1]];
+    }
+    for (int k = 0; k < 6; ++k) {
+        out->kernel_rays[k] = c.kernel_rays[k];
+        out->kernel_nodes[k] = c.trav_nodes[k];
+        out->kernel_tris[k] = c.trav_tris[k];
+        out->kernel_counted_rays[k] = c.trav_rays[k];
     }
     return WSRT_OK;
 }

@@ -79,11 +79,24 @@ struct Counters {
     unsigned long long extend_rays;
     unsigned long long shadow_rays;
     unsigned long long guide_samples;
-    unsigned long long trav_nodes[3];  // photon, extend, shadow (while counting)
-    unsigned long long trav_tris[3];
-    unsigned long long trav_rays[3];
+    // per launching kernel (RayKernel): rays traversed, always counted; nodes
+    // visited and triangles tested, while counting
+    unsigned long long kernel_rays[6];
+    unsigned long long trav_nodes[6];
+    unsigned long long trav_tris[6];
+    unsigned long long trav_rays[6];
     unsigned long long photon_rays_part[kStatBlocks];
     unsigned long long photons_stored_part[kStatBlocks];
+};
+
+// Kernels that traverse the BVH, as indices of the per-kernel counters.
+enum RayKernel : int {
+    kRkPhotonBounce = 0,  // k_photon_bounce: closest hit
+    kRkExtend = 1,        // k_extend: closest hit
+    kRkShadow = 2,        // k_shadow: any hit
+    kRkPhotonEmit = 3,    // k_photon_emit: the specular BVH root
+    kRkTailExtend = 4,    // k_tail: closest hit
+    kRkTailShadow = 5,    // k_tail: any hit
 };
 
 // Wavefront path state, structure of arrays indexed by path id (= pixel).

@@ -80,7 +80,10 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend
         if (gtid == 0 && *qs.tail == 0u) *qs.tail = bounce + 1u;  // the tail launch takes these paths
         return;
     }
-    if (gtid == 0) atomicAdd(&ctr->extend_rays, (unsigned long long)n);
+    if (gtid == 0) {
+        atomicAdd(&ctr->extend_rays, (unsigned long long)n);
+        atomicAdd(&ctr->kernel_rays[kRkExtend], (unsigned long long)n);
+    }
     const uint32_t stride = gridDim.x * blockDim.x;
     const bool count = count_traversal != 0u;
     uint32_t my_rays = 0, my_nodes = 0, my_tris = 0;
@@ -127,7 +130,7 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend
             pending &= ~__ballot(mine);
         }
     }
-    if (count) flush_traversal_counts(ctr, 1, my_rays, my_nodes, my_tris);
+    if (count) flush_traversal_counts(ctr, kRkExtend, my_rays, my_nodes, my_tris);
 }
 
 __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState ps, GridView grid,
@@ -212,7 +215,10 @@ __global__ __launch_bounds__(kTraceThreads) void k_shadow(SceneView sv, PathStat
     st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
     if (tail_started(qs, bounce)) return;
     const uint32_t n = seg_prefix(qs, bounce, kQShadow, 1, pre);
-    if (gtid == 0) atomicAdd(&ctr->shadow_rays, (unsigned long long)n);
+    if (gtid == 0) {
+        atomicAdd(&ctr->shadow_rays, (unsigned long long)n);
+        atomicAdd(&ctr->kernel_rays[kRkShadow], (unsigned long long)n);
+    }
     const uint32_t stride = gridDim.x * blockDim.x;
     const bool count = count_traversal != 0u;
     uint32_t my_rays = 0, my_nodes = 0, my_tris = 0;
@@ -226,7 +232,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_shadow(SceneView sv, PathStat
             ps.L[p] = add4(ps.L[p], r[2]);
         }
     }
-    if (count) flush_traversal_counts(ctr, 2, my_rays, my_nodes, my_tris);
+    if (count) flush_traversal_counts(ctr, kRkShadow, my_rays, my_nodes, my_tris);
 }
 
 // The paths entering bounce t-1 = *qs.tail - 1 (if the wavefront handed
@@ -278,12 +284,18 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(4) void k_tail(S
         sr += __shfl_xor(sr, off, 64);
     }
     if (__lane_id() == 0) {
-        if (er) atomicAdd(&ctr->extend_rays, er);
-        if (sr) atomicAdd(&ctr->shadow_rays, sr);
+        if (er) {
+            atomicAdd(&ctr->extend_rays, er);
+            atomicAdd(&ctr->kernel_rays[kRkTailExtend], er);
+        }
+        if (sr) {
+            atomicAdd(&ctr->shadow_rays, sr);
+            atomicAdd(&ctr->kernel_rays[kRkTailShadow], sr);
+        }
     }
     if (count) {
-        flush_traversal_counts(ctr, 1, tally.extend_rays, tally.extend_nodes, tally.extend_tris);
-        flush_traversal_counts(ctr, 2, tally.shadow_rays, tally.shadow_nodes, tally.shadow_tris);
+        flush_traversal_counts(ctr, kRkTailExtend, tally.extend_rays, tally.extend_nodes, tally.extend_tris);
+        flush_traversal_counts(ctr, kRkTailShadow, tally.shadow_rays, tally.shadow_nodes, tally.shadow_tris);
     }
 }
 

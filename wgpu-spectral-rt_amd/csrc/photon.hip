@@ -111,7 +111,9 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon
         const uint32_t e = seg_append(qs, 0, kQPhoton, qs.pseg_cap, go);
         if (go) ps.queue_a[e] = id;
     }
-    if (count) flush_traversal_counts(ctr, 0, my_rays, my_nodes, my_tris);
+    if (gtid == 0 && nl != 0u) atomicAdd(&ctr->kernel_rays[kRkPhotonEmit], (unsigned long long)n);
+    if (count) flush_traversal_counts(ctr, kRkPhotonEmit, my_rays, my_nodes, my_tris);
+    // every photon path's first ray
     const unsigned long long br = block_sum(my_rays, red);
     if (threadIdx.x == 0) ctr->photon_rays_part[blockIdx.x] += br;
 }
@@ -233,7 +235,7 @@ __device__ __forceinline__ void photon_bounce_range(const SceneView& sv, uint32_
                     r.caustic = ps.flags[r.id] != 0u;
                     trav_init(t, sv, xyz(pw), xyz(de), 0.0001f, 100.0f);
                     has = true;
-                    if (depth != 0) my_rays++;  // the emit kernel counted the first rays
+                    my_rays++;
                 }
             }
             cursor += nidle;
@@ -282,9 +284,11 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon
     else
         photon_bounce_range<false>(sv, depth, qs, ps, pre, queue, next_queue, photons, st, begin, end,
                                    my_rays, my_stored, my_nodes, my_tris);
-    if (count_traversal) flush_traversal_counts(ctr, 0, my_rays, my_nodes, my_tris);
-    // per-block statistics, summed on the host
-    const unsigned long long br = block_sum(my_rays, red);
+    if (gtid == 0) atomicAdd(&ctr->kernel_rays[kRkPhotonBounce], (unsigned long long)n);
+    if (count_traversal) flush_traversal_counts(ctr, kRkPhotonBounce, my_rays, my_nodes, my_tris);
+    // per-block statistics, summed on the host; the emit kernel counted the
+    // first rays
+    const unsigned long long br = block_sum(depth != 0 ? my_rays : 0u, red);
     const unsigned long long bs = block_sum(my_stored, red);
     if (threadIdx.x == 0) {
         ctr->photon_rays_part[blockIdx.x] += br;

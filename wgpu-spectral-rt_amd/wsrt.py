@@ -72,7 +72,9 @@ class FrameStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("frames", "camera_paths", "extend_rays", "shadow_rays",
                                            "photon_paths", "photon_rays", "photons_stored",
                                            "guide_samples")] + \
-        [("trav_nodes", C.c_uint64 * 3), ("trav_tris", C.c_uint64 * 3), ("trav_rays", C.c_uint64 * 3)]
+        [("trav_nodes", C.c_uint64 * 3), ("trav_tris", C.c_uint64 * 3), ("trav_rays", C.c_uint64 * 3),
+         ("kernel_rays", C.c_uint64 * 6), ("kernel_nodes", C.c_uint64 * 6), ("kernel_tris", C.c_uint64 * 6),
+         ("kernel_counted_rays", C.c_uint64 * 6)]
 
 
 class KernelStat(C.Structure):
@@ -567,7 +569,8 @@ class Renderer:
         s = FrameStats()
         _check(load().wsrt_renderer_get_stats(self.h, C.byref(s)))
         out = {k: getattr(s, k) for k, _ in FrameStats._fields_[:8]}
-        for k in ("trav_nodes", "trav_tris", "trav_rays"):
+        for k in ("trav_nodes", "trav_tris", "trav_rays", "kernel_rays", "kernel_nodes", "kernel_tris",
+                  "kernel_counted_rays"):
             out[k] = list(getattr(s, k))
         return out
 
