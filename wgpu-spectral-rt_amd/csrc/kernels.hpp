@@ -22,7 +22,7 @@ constexpr uint64_t kTraceLanes = (uint64_t)kTraceThreads * kTraceBlocks;
 // gives the extend kernel 122 VGPRs (4 waves); with 5 it fits 96 with a few
 // spills outside the node loop and runs 5 % faster on the caustic workload,
 // while 6 or 8 waves spill inside the loop and run slower. The photon bounce
-// fits 5 waves (96 VGPRs) without spills.
+// fits 4 waves (128 VGPRs) without spills.
 #define WSRT_WAVES_PER_SIMD(n) __attribute__((amdgpu_waves_per_eu(n)))
 
 // ---- primitives (sort.hip) -------------------------------------------------

@@ -89,17 +89,13 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon
                 const float flux = (radiance / pdf) * area * kPi;
                 const float energy = flux * prm.z;
                 const f3 pos = emit_pos + emit_dir * 0.001f;
-                // one visit of the specular BVH's root: a ray that meets none
-                // of its child boxes (nor a triangle of a leaf child) cannot
-                // reach a specular triangle; the others are traced in full
-                Trav t;
-                trav_init(t, spec, pos, emit_dir, 0.0001f, 100.0f);
-                if (t.live) {
-                    if (count) trav_step<true, true>(spec, t, st, my_nodes, my_tris);
-                    else trav_step<true, false>(spec, t, st, my_nodes, my_tris);
-                }
+                // a ray that meets no child box of the specular BVH's root
+                // cannot reach a specular triangle; the others are traced in
+                // full
+                const bool meets = meets_root_child(spec, pos, emit_dir, 0.0001f, 100.0f);
+                if (count && spec.n_nodes) my_nodes++;
                 my_rays++;
-                if (t.live || t.best.tri != kInvalid) {
+                if (meets) {
                     go = true;
                     ps.pos[id] = f4(pos, wl);
                     ps.dir[id] = f4(emit_dir, energy);
@@ -206,48 +202,29 @@ __device__ __forceinline__ bool photon_interact(const SceneView& sv, const Photo
     return alive;
 }
 
-// The rays of entries [begin, end) of a bounce, lanes refilled as above.
-template <bool kCount>
+// The rays of entries [begin, end) of a bounce, one per lane at a time.
 __device__ __forceinline__ void photon_bounce_range(const SceneView& sv, uint32_t depth, const Queues& qs,
                                                     const PhotonState& ps, const uint32_t* pre,
                                                     const uint32_t* __restrict__ queue,
                                                     uint32_t* __restrict__ next_queue,
                                                     wsrt_photon* __restrict__ photons, Stack st,
-                                                    uint32_t begin, uint32_t end, uint32_t& my_rays,
-                                                    uint32_t& my_stored, uint32_t& my_nodes,
+                                                    uint32_t begin, uint32_t end, bool count,
+                                                    uint32_t& my_rays, uint32_t& my_stored, uint32_t& my_nodes,
                                                     uint32_t& my_tris) {
-    uint32_t cursor = begin;  // wave-uniform: the next entry not taken up
-    bool has = false;
-    PhotonRay r{};
-    Trav t;
-    while (true) {
-        const uint64_t idle = __ballot(!has);
-        const uint32_t nidle = (uint32_t)__popcll(idle);
-        if (cursor < end && nidle >= kRefillLanes) {
-            if (!has) {
-                const uint32_t i = cursor + (uint32_t)__popcll(idle & lanemask_lt());
-                if (i < end) {
-                    r.id = queue[seg_entry(pre, kSegs, i, qs.pseg_cap)];
-                    const float4 pw = ps.pos[r.id], de = ps.dir[r.id];
-                    r.wl = pw.w;
-                    r.energy = de.w;
-                    r.rng = ps.rng[r.id];
-                    r.caustic = ps.flags[r.id] != 0u;
-                    trav_init(t, sv, xyz(pw), xyz(de), 0.0001f, 100.0f);
-                    has = true;
-                    my_rays++;
-                }
-            }
-            cursor += nidle;
-        }
-        if (__ballot(has) == 0ull) break;  // only once the range is used up
+    for (uint32_t base = begin; base < end; base += 64u) {
+        const uint32_t i = base + __lane_id();
         bool alive = false;
-        if (has) {
-            if (t.live) trav_step<false, kCount>(sv, t, st, my_nodes, my_tris);
-            if (!t.live) {
-                has = false;
-                alive = photon_interact(sv, ps, photons, r, t.r.o, t.r.d, t.best, my_stored);
-            }
+        PhotonRay r{};
+        if (i < end) {
+            r.id = queue[seg_entry(pre, kSegs, i, qs.pseg_cap)];
+            const float4 pw = ps.pos[r.id], de = ps.dir[r.id];
+            r.wl = pw.w;
+            r.energy = de.w;
+            r.rng = ps.rng[r.id];
+            r.caustic = ps.flags[r.id] != 0u;
+            const Hit h = trace_ray<false>(sv, xyz(pw), xyz(de), 0.0001f, 100.0f, st, count, my_nodes, my_tris);
+            my_rays++;
+            alive = photon_interact(sv, ps, photons, r, xyz(pw), xyz(de), h, my_stored);
         }
         const uint32_t e = seg_append(qs, depth + 1, kQPhoton, qs.pseg_cap, alive);
         if (alive) next_queue[e] = r.id;
@@ -258,7 +235,7 @@ __device__ __forceinline__ void photon_bounce_range(const SceneView& sv, uint32_
 // hit, then the surface interaction. Survivors are compacted into the next
 // queue, so the few paths refracting through glass run in full waves. Each
 // wave takes a contiguous range of the queue.
-__global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon_bounce(
+__global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(4) void k_photon_bounce(
     SceneView sv, uint32_t depth, Queues qs, uint32_t count_traversal, PhotonState ps,
     const uint32_t* __restrict__ queue, uint32_t* __restrict__ next_queue, wsrt_photon* __restrict__ photons,
     Counters* __restrict__ ctr, uint2* __restrict__ gstack) {
@@ -278,12 +255,8 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon
     const uint32_t begin = (uint32_t)min((uint64_t)n, (uint64_t)wave * per_wave * 64u);
     const uint32_t end = (uint32_t)min((uint64_t)n, (uint64_t)(wave + 1u) * per_wave * 64u);
     uint32_t my_rays = 0, my_stored = 0, my_nodes = 0, my_tris = 0;
-    if (count_traversal)
-        photon_bounce_range<true>(sv, depth, qs, ps, pre, queue, next_queue, photons, st, begin, end, my_rays,
-                                  my_stored, my_nodes, my_tris);
-    else
-        photon_bounce_range<false>(sv, depth, qs, ps, pre, queue, next_queue, photons, st, begin, end,
-                                   my_rays, my_stored, my_nodes, my_tris);
+    photon_bounce_range(sv, depth, qs, ps, pre, queue, next_queue, photons, st, begin, end, count_traversal != 0u,
+                        my_rays, my_stored, my_nodes, my_tris);
     if (gtid == 0) atomicAdd(&ctr->kernel_rays[kRkPhotonBounce], (unsigned long long)n);
     if (count_traversal) flush_traversal_counts(ctr, kRkPhotonBounce, my_rays, my_nodes, my_tris);
     // per-block statistics, summed on the host; the emit kernel counted the

@@ -90,68 +90,36 @@ __device__ __forceinline__ uint32_t byte_of(uint32_t lo, uint32_t hi, int c) {
     return ((c < 4 ? lo : hi) >> (8 * (c & 3))) & 0xFFu;
 }
 
-// The ray of a traversal: origin, direction, reciprocal direction, octant
-// (bit a set when d[a] < 0) and the lower end of the interval.
-struct TravRay {
-    f3 o, d;
-    float inv[3];
-    float tmin;
-    int oct;
-};
-
-__device__ __forceinline__ TravRay make_trav_ray(f3 o, f3 d, float tmin) {
-    TravRay r;
-    r.o = o;
-    r.d = d;
-    r.inv[0] = safe_inv(d.x);
-    r.inv[1] = safe_inv(d.y);
-    r.inv[2] = safe_inv(d.z);
-    r.tmin = tmin;
-    r.oct = (d.x < 0.0f ? 1 : 0) | (d.y < 0.0f ? 2 : 0) | (d.z < 0.0f ? 4 : 0);
-    return r;
-}
-
-__device__ __forceinline__ Hit no_hit(float tmax) {
-    Hit h;
-    h.t = tmax;
-    h.tri = kInvalid;
-    h.u = h.v = 0.0f;
-    h.front = false;
-    return h;
-}
-
-// The traversal loop. The node to visit next is kept in a register: the
-// root first, then the first internal child in visiting order of each node
-// (it would be pushed and popped at once, and with best.t unchanged in
-// between its pop-time cull test could not fail). With kSingle the loop
-// returns after one node visit, once it has found the node to visit after
-// it (live stays true) or found that none is left (live turns false); a
-// full traversal is one call without it.
-template <bool kAny, bool kCount, bool kSingle>
-__device__ __forceinline__ void trav_loop(const SceneView& sv, const TravRay& r, Hit& best, Stack st, int& sp,
-                                          uint32_t& next, bool& live, uint32_t& nvis, uint32_t& ntest) {
-    const uint4* nodes = reinterpret_cast<const uint4*>(sv.nodes);
-    const float inv[3] = {r.inv[0], r.inv[1], r.inv[2]};
-    const float oo[3] = {r.o.x, r.o.y, r.o.z};
+template <bool kAny, bool kCount>
+__device__ Hit traverse(const SceneView& sv, f3 o, f3 d, float tmin, float tmax, Stack st,
+                        uint32_t* n_nodes_visited = nullptr, uint32_t* n_tris_tested = nullptr) {
+    Hit best;
+    best.t = tmax;
+    best.tri = kInvalid;
+    best.u = best.v = 0.0f;
+    best.front = false;
+    if (sv.n_nodes == 0) return best;
+    const float inv[3] = {safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
+    const float oo[3] = {o.x, o.y, o.z};
     const bool neg[3] = {inv[0] < 0.0f, inv[1] < 0.0f, inv[2] < 0.0f};
-    const int oct = r.oct;
-    const f3 o = r.o, d = r.d;
-    const float tmin = r.tmin;
-    bool have_next = live, visited = false;
+    const int oct = (d.x < 0.0f ? 1 : 0) | (d.y < 0.0f ? 2 : 0) | (d.z < 0.0f ? 4 : 0);
+    uint32_t nvis = 0, ntest = 0;
+    int sp = 0;
+    // the node to visit next is kept in a register: the root first, then the
+    // first internal child in visiting order of each node (it would be
+    // pushed and popped at once, and with best.t unchanged in between its
+    // pop-time cull test could not fail)
+    uint32_t next = 0u;
+    bool have_next = true;
+    const uint4* nodes = reinterpret_cast<const uint4*>(sv.nodes);
     while (true) {
         if (!have_next) {
-            if (sp == 0) {
-                live = false;
-                break;
-            }
+            if (sp == 0) break;
             const uint2 e = st.get(--sp);
             if (!kAny && __uint_as_float(e.y) > best.t * kBoxPad) continue;
             next = e.x;
-            if (kSingle) break;  // after the visit: the next node is found
         }
-        if (kSingle && visited) break;
         have_next = false;
-        visited = true;
         const uint4* np = nodes + 5ull * next;
         uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
         if (kCount) nvis++;
@@ -209,8 +177,8 @@ __device__ __forceinline__ void trav_loop(const SceneView& sv, const TravRay& r,
                 if (kCount) ntest++;
                 float tt, u, v;
                 bool fr;
-                if (!tri_intersect(o, d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), tt, u,
-                                   v, fr))
+                if (!tri_intersect(o, d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z),
+                                   tt, u, v, fr))
                     continue;
                 if (!(tt > tmin)) continue;
                 uint32_t id = __float_as_uint(a.w);
@@ -221,8 +189,8 @@ __device__ __forceinline__ void trav_loop(const SceneView& sv, const TravRay& r,
                     best.v = v;
                     best.front = fr;
                     if (kAny) {
-                        live = false;
-                        return;
+                        if (kCount) { *n_nodes_visited = nvis; *n_tris_tested = ntest; }
+                        return best;
                     }
                 }
             }
@@ -257,18 +225,6 @@ __device__ __forceinline__ void trav_loop(const SceneView& sv, const TravRay& r,
             have_next = true;
         }
     }
-}
-
-template <bool kAny, bool kCount>
-__device__ Hit traverse(const SceneView& sv, f3 o, f3 d, float tmin, float tmax, Stack st,
-                        uint32_t* n_nodes_visited = nullptr, uint32_t* n_tris_tested = nullptr) {
-    Hit best = no_hit(tmax);
-    if (sv.n_nodes == 0) return best;
-    const TravRay r = make_trav_ray(o, d, tmin);
-    uint32_t nvis = 0, ntest = 0, next = 0u;
-    int sp = 0;
-    bool live = true;
-    trav_loop<kAny, kCount, false>(sv, r, best, st, sp, next, live, nvis, ntest);
     if (kCount) {
         *n_nodes_visited = nvis;
         *n_tris_tested = ntest;
@@ -276,41 +232,46 @@ __device__ Hit traverse(const SceneView& sv, f3 o, f3 d, float tmin, float tmax,
     return best;
 }
 
-// Lanes of a wave whose ray is finished stay idle until at least this many
-// are, then take up the next rays of the wave's work together. Every ray is
-// traversed exactly as on its own, so only the interleaving changes. A batch
-// of incoherent rays mixes many one-node misses with the few rays that
-// descend deep into a mesh; refilling keeps the lanes of a wave busy instead
-// of waiting for its longest ray. (Measured on the photon bounce: 16 beat 1,
-// 32 and 48; camera rays, being coherent, ran slower with refilling.)
-constexpr uint32_t kRefillLanes = 16;
-
-// Traversal state of one ray advanced one node visit at a time by
-// trav_step, so a caller can interleave the rays of a wave freely (a lane
-// whose ray is finished can take up a new one while the others continue).
-// Each ray's visits are exactly those of traverse().
-struct Trav {
-    TravRay r;
-    Hit best;
-    int sp;          // stack entries in use
-    uint32_t next;   // node to visit next
-    bool live;       // a node is waiting to be visited
-};
-
-__device__ __forceinline__ void trav_init(Trav& t, const SceneView& sv, f3 o, f3 d, float tmin, float tmax) {
-    t.r = make_trav_ray(o, d, tmin);
-    t.best = no_hit(tmax);
-    t.sp = 0;
-    t.next = 0u;  // the root
-    t.live = sv.n_nodes != 0u;
-}
-
-// Visits t.next, then finds the node to visit after it; t.live turns false
-// when nothing is left, or at the first accepted hit of an any-hit query.
-template <bool kAny, bool kCount>
-__device__ __forceinline__ void trav_step(const SceneView& sv, Trav& t, Stack st, uint32_t& nvis,
-                                          uint32_t& ntest) {
-    trav_loop<kAny, kCount, true>(sv, t.r, t.best, st, t.sp, t.next, t.live, nvis, ntest);
+// Whether a ray (o, d) on (tmin, tmax) meets any child box of the root of
+// sv's BVH: the same slab test as traverse() on one node. A ray that meets
+// none reaches no triangle; one that meets a box may still miss them all.
+__device__ __forceinline__ bool meets_root_child(const SceneView& sv, f3 o, f3 d, float tmin, float tmax) {
+    if (sv.n_nodes == 0) return false;
+    const float inv[3] = {safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
+    const float oo[3] = {o.x, o.y, o.z};
+    const uint4* np = reinterpret_cast<const uint4*>(sv.nodes);
+    const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
+    const float org[3] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)};
+    const uint32_t lo_l[3] = {w2.x, w2.z, w3.x}, lo_h[3] = {w2.y, w2.w, w3.y};
+    const uint32_t hi_l[3] = {w3.z, w4.x, w4.z}, hi_h[3] = {w3.w, w4.y, w4.w};
+    float as[3], ao[3], sl = 0.0f;
+    uint32_t nl[3], nh[3], fl[3], fh[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const bool neg = inv[a] < 0.0f;
+        const int ex = (int)(int8_t)((w0.w >> (8 * a)) & 0xFFu);
+        as[a] = __uint_as_float((uint32_t)(ex + 127) << 23) * inv[a];
+        ao[a] = (org[a] - oo[a]) * inv[a];
+        sl = fmaxf(sl, fabsf(as[a]) * 255.0f + fabsf(ao[a]));
+        nl[a] = neg ? hi_l[a] : lo_l[a];
+        nh[a] = neg ? hi_h[a] : lo_h[a];
+        fl[a] = neg ? lo_l[a] : hi_l[a];
+        fh[a] = neg ? lo_h[a] : hi_h[a];
+    }
+    sl = sl * 4.0e-7f;
+    const float sl2 = sl * (1.0f + kBoxPad);
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        float tn = tmin, tf = tmax;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            tn = fmaxf(tn, __fmaf_rn((float)byte_of(nl[a], nh[a], c), as[a], ao[a]));
+            tf = fminf(tf, __fmaf_rn((float)byte_of(fl[a], fh[a], c), as[a], ao[a]));
+        }
+        any |= byte_of(w1.z, w1.w, c) != 0u && tn <= __fmaf_rn(tf, kBoxPad, sl2);
+    }
+    return any;
 }
 
 // Closest or any hit, counting node visits and triangle tests into the
