@@ -160,10 +160,9 @@ __device__ Hit traverse(const SceneView& sv, f3 o, f3 d, float tmin, float tmax,
             cnear[c] = tn - sl;
             if (byte_of(w1.z, w1.w, c) != 0u && tn <= __fmaf_rn(tf, kBoxPad, sl2)) hitmask |= 1u << c;
         }
-        // slot classes: internal children have meta bit 7 set
-        uint32_t imask = 0;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) imask |= ((byte_of(w1.z, w1.w, c) >> 7) & 1u) << c;
+        // slot classes: bit c of the node's imask byte is set when child c is
+        // internal (meta bit 7), stored by the builder
+        const uint32_t imask = w0.w >> 24;
         // leaf children, in slot order
         uint32_t leaves = hitmask & ~imask;
         while (leaves) {
