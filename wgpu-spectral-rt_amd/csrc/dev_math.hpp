@@ -81,11 +81,19 @@ __device__ __forceinline__ f3 random_unit_vector(uint32_t& s) {  // shader.wgsl:
 // pow(x, 2.0) of a possibly negative x is evaluated as x * x.
 __device__ __forceinline__ float sq(float x) { return x * x; }
 
+// The smooth spectral fits below (CIE lobes, rgb -> spectrum, Planck) run
+// with the hardware exponential and multiplications by reciprocal constants:
+// a few ulp from the IEEE forms the oracle uses, far inside the 1e-4
+// relative parity bar (WGSL's own exp and division are not correctly rounded
+// either), and several times fewer instructions.
+__device__ __forceinline__ float gauss_lobe(float l, float mu, float inv_sigma) {
+    return __expf(-sq((l - mu) * inv_sigma));
+}
 __device__ __forceinline__ f3 cie_xyz(float l) {
-    float x = expf(-sq((l - 599.8f) / 37.9f)) * 1.056f + expf(-sq((l - 442.0f) / 16.0f)) * 0.362f -
-              expf(-sq((l - 501.1f) / 26.7f)) * 0.065f;
-    float y = expf(-sq((l - 568.8f) / 46.9f)) * 0.821f + expf(-sq((l - 530.9f) / 16.3f)) * 0.286f;
-    float z = expf(-sq((l - 437.0f) / 11.8f)) * 1.217f + expf(-sq((l - 459.0f) / 26.0f)) * 0.681f;
+    float x = gauss_lobe(l, 599.8f, 1.0f / 37.9f) * 1.056f + gauss_lobe(l, 442.0f, 1.0f / 16.0f) * 0.362f -
+              gauss_lobe(l, 501.1f, 1.0f / 26.7f) * 0.065f;
+    float y = gauss_lobe(l, 568.8f, 1.0f / 46.9f) * 0.821f + gauss_lobe(l, 530.9f, 1.0f / 16.3f) * 0.286f;
+    float z = gauss_lobe(l, 437.0f, 1.0f / 11.8f) * 1.217f + gauss_lobe(l, 459.0f, 1.0f / 26.0f) * 0.681f;
     return mk3(fmaxf(0.0f, x), fmaxf(0.0f, y), fmaxf(0.0f, z));
 }
 __device__ __forceinline__ f3 xyz_to_srgb(f3 c) {
@@ -97,9 +105,9 @@ __device__ __forceinline__ float rgb_to_spectrum(f3 rgb, float l) {
     float white = fminf(rgb.x, fminf(rgb.y, rgb.z));
     float cr = fmaxf(rgb.x - white, 0.0f), cg = fmaxf(rgb.y - white, 0.0f),
           cb = fmaxf(rgb.z - white, 0.0f);
-    float r = cr * expf(-sq((l - 630.0f) / 20.0f));
-    float g = cg * expf(-sq((l - 525.0f) / 20.0f));
-    float b = cb * expf(-sq((l - 460.0f) / 20.0f));
+    float r = cr * gauss_lobe(l, 630.0f, 1.0f / 20.0f);
+    float g = cg * gauss_lobe(l, 525.0f, 1.0f / 20.0f);
+    float b = cb * gauss_lobe(l, 460.0f, 1.0f / 20.0f);
     return fminf(fmaxf(white + r + g + b, 0.0f), 0.99f);
 }
 __device__ __forceinline__ float4 rgb_to_spectrum4(f3 rgb, float4 wl) {
@@ -111,7 +119,9 @@ __device__ __forceinline__ float blackbody(float lambda_nm, float temp_k) {
     const float h = 6.62607015e-34f, c = 299792458.0f, k = 1.380649e-23f;
     float c1 = 2.0f * h * c * c;
     float c2 = (h * c) / (k * temp_k);
-    float radiance = c1 / (powf(lambda_m, 5.0f) * (expf(c2 / lambda_m) - 1.0f));
+    const float l2 = lambda_m * lambda_m;
+    const float inv_l = __builtin_amdgcn_rcpf(lambda_m);
+    float radiance = c1 * __builtin_amdgcn_rcpf(l2 * l2 * lambda_m * (__expf(c2 * inv_l) - 1.0f));
     return radiance * 1e-13f;
 }
 __device__ __forceinline__ float4 blackbody4(float4 wl, float t) {

@@ -308,10 +308,26 @@ __global__ __launch_bounds__(256) void k_accumulate(PathState ps, FrameArgs fa, 
     if (pix >= npx) return;
     const uint32_t base_frame = *fa.frame;
     float4 na = accum[pix];
-    for (uint32_t s = 0; s < nf; ++s) {
-        const FrameConst& fc = fa.table[(base_frame + s) & fa.mask];
-        const uint64_t p = (uint64_t)s * npx + pix;
-        na = accumulate_sample(na, fc, ps.wl[p], ps.L[p]);
+    // eight frames' samples are loaded and converted together (the loads of
+    // one frame do not wait for the previous one), then added in frame order
+    constexpr uint32_t kGroup = 8;
+    for (uint32_t s0 = 0; s0 < nf; s0 += kGroup) {
+        float4 col[kGroup];
+        bool reset[kGroup];
+#pragma unroll
+        for (uint32_t k = 0; k < kGroup; ++k) {
+            col[k] = splat4(0.0f);
+            reset[k] = false;
+            if (s0 + k < nf) {
+                const FrameConst& fc = fa.table[(base_frame + s0 + k) & fa.mask];
+                const uint64_t p = (uint64_t)(s0 + k) * npx + pix;
+                col[k] = sample_color(fc, ps.wl[p], ps.L[p]);
+                reset[k] = fc.frame == 0u;
+            }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kGroup; ++k)
+            if (s0 + k < nf) na = add4(reset[k] ? splat4(0.0f) : na, col[k]);
     }
     accum[pix] = na;
     const uchar4 o = tone_map(na);

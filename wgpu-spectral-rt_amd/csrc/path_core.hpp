@@ -184,18 +184,29 @@ __device__ __forceinline__ void trace_bounces(const SceneView& sv, const GridVie
 // One sample into the running accumulation (shader.wgsl:771-808): hero
 // wavelengths to XYZ over the sum of their pdfs, to linear sRGB, added as
 // (rgb, 1); frame 0 restarts the accumulation.
-__device__ __forceinline__ float4 accumulate_sample(float4 na, const FrameConst& fc, float4 wl, float4 L) {
+__device__ __forceinline__ float4 sample_color(const FrameConst& fc, float4 wl, float4 L) {
     f3 xz = mk3(0, 0, 0);
     xz = xz + cie_xyz(wl.x) * L.x;
     xz = xz + cie_xyz(wl.y) * L.y;
     xz = xz + cie_xyz(wl.z) * L.z;
     xz = xz + cie_xyz(wl.w) * L.w;
-    const CauchyConst cc{fc.cdf_min, fc.cdf_range};
-    float sp = cauchy_pdf(wl.x, cc) + cauchy_pdf(wl.y, cc) + cauchy_pdf(wl.z, cc) + cauchy_pdf(wl.w, cc);
-    xz = xz * (1.0f / sp);
+    // sum of the hero wavelengths' Cauchy pdfs (cauchy_pdf), with hardware
+    // reciprocals
+    float q = 0.0f;
+    const float wv[4] = {wl.x, wl.y, wl.z, wl.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float d = (wv[i] - 555.0f) * (1.0f / 40.0f);
+        q += __builtin_amdgcn_rcpf(1.0f + d * d);
+    }
+    const float sp = q * __builtin_amdgcn_rcpf(fc.cdf_range * kPi * 40.0f);
+    xz = xz * __builtin_amdgcn_rcpf(sp);
     f3 c = xyz_to_srgb(xz);
+    return make_float4(c.x, c.y, c.z, 1.0f);
+}
+__device__ __forceinline__ float4 accumulate_sample(float4 na, const FrameConst& fc, float4 wl, float4 L) {
     float4 cur = fc.frame > 0u ? na : splat4(0.0f);
-    return add4(cur, make_float4(c.x, c.y, c.z, 1.0f));
+    return add4(cur, sample_color(fc, wl, L));
 }
 
 // Mean, clamp at 0, Reinhard on luma, RGBA8 without gamma (shader.wgsl:809-822).

@@ -83,19 +83,20 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon
                     emit_pos = a * u + b * v + c * w;
                     emit_dir = normalize(nrm + random_unit_vector(rng));
                 }
-                const float wl = cauchy_sample(rand01(rng), cc);
-                const float pdf = cauchy_pdf(wl, cc);
-                const float radiance = blackbody(wl, prm.y);
-                const float flux = (radiance / pdf) * area * kPi;
-                const float energy = flux * prm.z;
                 const f3 pos = emit_pos + emit_dir * 0.001f;
                 // a ray that meets no child box of the specular BVH's root
                 // cannot reach a specular triangle; the others are traced in
-                // full
+                // full, and only they need a wavelength and an energy (drawn
+                // after the direction, in the reference's order)
                 const bool meets = meets_root_child(spec, pos, emit_dir, 0.0001f, 100.0f);
                 if (count && spec.n_nodes) my_nodes++;
                 my_rays++;
                 if (meets) {
+                    const float wl = cauchy_sample(rand01(rng), cc);
+                    const float pdf = cauchy_pdf(wl, cc);
+                    const float radiance = blackbody(wl, prm.y);
+                    const float flux = (radiance / pdf) * area * kPi;
+                    const float energy = flux * prm.z;
                     go = true;
                     ps.pos[id] = f4(pos, wl);
                     ps.dir[id] = f4(emit_dir, energy);
