@@ -1093,7 +1093,7 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     HIP_OR_FAIL(r->ph_dir.alloc(16 * nph));
     HIP_OR_FAIL(r->ph_rng.alloc(4 * nph));
     HIP_OR_FAIL(r->ph_flags.alloc(4 * nph));
-    HIP_OR_FAIL(r->ph_stored.alloc(4 * nph));
+    HIP_OR_FAIL(r->ph_stored.alloc(4 * ((nph + 31) / 32)));
     const uint32_t pseg = seg_capacity(nph);
     HIP_OR_FAIL(r->ph_qa.alloc(4ull * kSegs * pseg));
     HIP_OR_FAIL(r->ph_qb.alloc(4ull * kSegs * pseg));
@@ -1136,7 +1136,7 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     HIP_OR_FAIL(hipMemsetAsync(r->grid_count.p, 0, sizeof(uint32_t) * cells, r->stream));
     HIP_OR_FAIL(hipMemsetAsync(r->grid_offset.p, 0, sizeof(uint32_t) * cells, r->stream));
     HIP_OR_FAIL(hipMemsetAsync(r->grid_nvalid.p, 0, sizeof(uint32_t), r->stream));
-    HIP_OR_FAIL(hipMemsetAsync(r->ph_stored.p, 0, 4 * nph, r->stream));
+    HIP_OR_FAIL(hipMemsetAsync(r->ph_stored.p, 0, 4 * ((nph + 31) / 32), r->stream));
     HIP_OR_FAIL(hipStreamSynchronize(r->stream));
     r->gb.count = r->grid_count.as<uint32_t>();
     r->gb.offset = r->grid_offset.as<uint32_t>();
@@ -1594,6 +1594,21 @@ extern "C" int wsrt_renderer_reset_profile(wsrt_renderer* r) {
 // ---------------------------------------------------------------------------
 // The debug taps read the last frame of the last batch (frame slot
 // last_slot of the batch's buffers).
+// The stored bits of photon slots first .. first+np-1, one flag per slot.
+static int read_stored_flags(wsrt_renderer* r, size_t first, uint32_t np, std::vector<uint8_t>& flag) {
+    flag.assign(np, 0);
+    if (np == 0) return WSRT_OK;
+    const size_t w0 = first / 32, w1 = (first + np + 31) / 32;
+    std::vector<uint32_t> words(w1 - w0);
+    int rc = read_back(r, words.data(), r->ph_stored.as<uint32_t>() + w0, 4 * (w1 - w0));
+    if (rc) return rc;
+    for (uint32_t i = 0; i < np; ++i) {
+        const size_t b = first + i;
+        flag[i] = (words[b / 32 - w0] >> (b % 32)) & 1u;
+    }
+    return WSRT_OK;
+}
+
 extern "C" int wsrt_debug_read_photons(wsrt_renderer* r, wsrt_photon* out, uint32_t cap,
                                        uint32_t* n_stored) {
     CHECK_ARG(r && out);
@@ -1601,8 +1616,8 @@ extern "C" int wsrt_debug_read_photons(wsrt_renderer* r, wsrt_photon* out, uint3
     if (cap < np) return fail(WSRT_E_INVALID, "photon buffer holds %u < %u", cap, np);
     const size_t first = (size_t)r->last_slot * np;
     int rc = read_back(r, out, r->photons.as<wsrt_photon>() + first, sizeof(wsrt_photon) * np);
-    std::vector<uint32_t> flag(np);
-    if (!rc && np) rc = read_back(r, flag.data(), r->ph_stored.as<uint32_t>() + first, 4ull * np);
+    std::vector<uint8_t> flag;
+    if (!rc) rc = read_stored_flags(r, first, np, flag);
     if (rc) return rc;
     // a slot holds a photon only when its stored flag is set; the others read
     // as the reference's skipped photons (all zero)
@@ -1628,9 +1643,9 @@ extern "C" int wsrt_debug_read_grid(wsrt_renderer* r, uint32_t* cell_id, uint32_
     const uint32_t hs = r->cfg.hash_size, ls = r->last_slot;
     int rc = WSRT_OK;
     if (cell_id && np) {
-        std::vector<uint32_t> flag(np);
+        std::vector<uint8_t> flag;
         rc = read_back(r, cell_id, r->grid_cell.as<uint32_t>() + (size_t)ls * np, 4ull * np);
-        if (!rc) rc = read_back(r, flag.data(), r->ph_stored.as<uint32_t>() + (size_t)ls * np, 4ull * np);
+        if (!rc) rc = read_stored_flags(r, (size_t)ls * np, np, flag);
         if (rc) return rc;
         for (uint32_t i = 0; i < np; ++i)
             if (!flag[i]) cell_id[i] = kInvalid;

@@ -128,7 +128,7 @@ struct PhotonState {
     uint32_t* flags;    // 1: caustic path
     uint32_t* queue_a;
     uint32_t* queue_b;
-    uint32_t* stored;   // per photon path: 1 once its photon is stored in its slot
+    uint32_t* stored;   // bit i (word i / 32): photon path i stored its photon in slot i
 };
 
 // Photon grid (counting sort view of build_grid.wgsl).

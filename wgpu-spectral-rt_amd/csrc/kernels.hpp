@@ -75,7 +75,7 @@ struct GridBuffers {
     uint32_t* temp;         // grid_temp_words(n, hash_size)
 };
 // Photon paths of nf consecutive frames, np per frame: path i is photon
-// i % np of frame slot i / np, stored at slot i with ps.stored[i] set.
+// i % np of frame slot i / np, stored at slot i with bit i of ps.stored set.
 // Emission (which queues the paths whose first ray meets specular geometry),
 // then one launch per bounce depth.
 void launch_photon_emit(const SceneView& sv, const FrameArgs& fa, uint32_t np, uint32_t nf, const Queues& qs,
@@ -88,7 +88,7 @@ size_t grid_temp_words(uint32_t n, uint32_t cells);
 // = slot * hash_size + hash; count/offset hold nf * hash_size cells.
 int grid_build(const wsrt_photon* photons, uint32_t np, uint32_t nf, uint32_t hash_size,
                const FrameArgs& fa, GridBuffers& g, hipStream_t s);
-// The same grid over the slots whose `stored` flag is set (the photon pass's
+// The same grid over the slots whose `stored` bit is set (the photon pass's
 // output; other slots are not read), with offsets written for non-empty
 // cells only: the gather reads a cell's offset only when its count is not 0.
 int grid_build_stored(const wsrt_photon* photons, const uint32_t* stored, uint32_t np, uint32_t nf,

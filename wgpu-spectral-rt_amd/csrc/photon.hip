@@ -2,14 +2,14 @@
 // photon grid (build_grid.wgsl:25-73) rebuilt as a counting sort.
 //
 // Photon path i writes its stored photon to slot i and sets its stored
-// flag, so the photon buffer is deterministic; the reference appends with
+// bit, so the photon buffer is deterministic; the reference appends with
 // atomicAdd in arbitrary order, which holds the same multiset. The grid of a
 // dense photon buffer (wsrt_photon_grid_build; energy 0 marks an empty slot):
 //   hash    cell_id[i] and an atomic histogram count[cell]
 //   scan    offset = exclusive prefix sum of count (wave-level scans)
 //   sort    stable radix sort of (cell, slot), so cells list slots ascending
 //   copy    photons in cell order, so a gather reads one contiguous range.
-// The renderer's grid hashes only the slots whose stored flag is set and
+// The renderer's grid hashes only the slots whose stored bit is set and
 // writes the offsets of non-empty cells from the run starts of the sorted
 // keys (the gather reads an offset only when the cell's count is not 0).
 #include "kernels.hpp"
@@ -24,7 +24,7 @@ namespace {
 
 // Emission of photon path i (photon_emit.wgsl:159-216): light choice, point
 // on the light, cosine direction, hero wavelength and flux; every path's
-// stored flag is cleared (its photon slot holds a photon only once the flag
+// stored bit is cleared (its photon slot holds a photon only once the bit
 // is set). A photon is stored only after a specular interaction, so a path
 // whose first ray meets no metal, glass or thin-film triangle ends at its
 // first hit without storing anything. A test against the root of the
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon
         const uint32_t id = base + __lane_id();
         bool go = false;
         if (id < n) {
-            ps.stored[id] = 0u;
+            if ((id & 31u) == 0u) ps.stored[id >> 5] = 0u;  // this lane's word of stored bits
             if (nl != 0u) {
                 const uint32_t slot = id / np, local = id - slot * np;
                 uint32_t rng = pcg_hash(local + (base_frame + slot) * 927163u);
@@ -150,7 +150,7 @@ __device__ __forceinline__ bool photon_interact(const SceneView& sv, const Photo
             // a photon of energy 0 counts as skipped, as in the dense photon
             // buffer that wsrt_photon_grid_build takes
             const bool keep = r.energy != 0.0f;
-            ps.stored[r.id] = keep ? 1u : 0u;
+            if (keep) atomicOr(&ps.stored[r.id >> 5], 1u << (r.id & 31u));
             my_stored += keep ? 1u : 0u;
         }
     } else if (type == 1u) {
@@ -309,27 +309,43 @@ __global__ __launch_bounds__(256) void k_grid_compact(uint32_t n, const uint32_t
     cvals[pos[i]] = i;
 }
 
-// Cell and histogram of the stored photons only, compacted in slot order
-// (positions from an exclusive scan of the stored flags).
+// Stored photons per 32-slot word of the stored bits.
+__global__ __launch_bounds__(256) void k_word_popc(const uint32_t* __restrict__ bits, uint32_t words,
+                                                   uint32_t* __restrict__ cnt) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < words) cnt[w] = (uint32_t)__popc(bits[w]);
+}
+
+// Cell and histogram of the stored photons only, compacted in slot order:
+// one lane per 32-slot word, positions from the exclusive scan of the words'
+// counts plus the rank of the bit within its word.
 __global__ __launch_bounds__(256) void k_grid_hash_stored(const wsrt_photon* __restrict__ photons,
-                                                          const uint32_t* __restrict__ stored,
-                                                          const uint32_t* __restrict__ pos, uint32_t n,
+                                                          const uint32_t* __restrict__ bits,
+                                                          const uint32_t* __restrict__ wpos, uint32_t words,
                                                           uint32_t np, FrameArgs fa,
                                                           uint32_t* __restrict__ cell_id,
                                                           uint32_t* __restrict__ count,
                                                           uint32_t* __restrict__ ckeys,
                                                           uint32_t* __restrict__ cvals) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !stored[i]) return;
-    const uint32_t slot = i / np;
-    const FrameConst& fc = fa.table[(*fa.frame + slot) & fa.mask];
-    const float4 a = reinterpret_cast<const float4*>(photons + i)[0];
-    const uint32_t c = hash_cell(xyz(a), mk3(fc.jitter[0], fc.jitter[1], fc.jitter[2]), fc.cell_size, fc.hash_size);
-    const uint32_t key = slot * fc.hash_size + c;
-    atomicAdd(&count[key], 1u);
-    cell_id[i] = c;
-    ckeys[pos[i]] = key;
-    cvals[pos[i]] = i;
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words) return;
+    uint32_t m = bits[w];
+    uint32_t k = m ? wpos[w] : 0u;
+    while (m) {
+        const uint32_t i = w * 32u + (uint32_t)(__ffs(m) - 1);
+        m &= m - 1u;
+        const uint32_t slot = i / np;
+        const FrameConst& fc = fa.table[(*fa.frame + slot) & fa.mask];
+        const float4 a = reinterpret_cast<const float4*>(photons + i)[0];
+        const uint32_t c =
+            hash_cell(xyz(a), mk3(fc.jitter[0], fc.jitter[1], fc.jitter[2]), fc.cell_size, fc.hash_size);
+        const uint32_t key = slot * fc.hash_size + c;
+        atomicAdd(&count[key], 1u);
+        cell_id[i] = c;
+        ckeys[k] = key;
+        cvals[k] = i;
+        k++;
+    }
 }
 
 // Offset of every non-empty cell: the position of the first of its photons
@@ -435,9 +451,14 @@ int grid_build_stored(const wsrt_photon* photons, const uint32_t* stored, uint32
         fill_u32(g.n_valid, 1, 0u, s);
         return 0;
     }
-    if (scan_exclusive(stored, pos, n, work, n_valid, s)) return -1;
-    hipLaunchKernelGGL(k_grid_hash_stored, dim3((n + 255) / 256), dim3(256), 0, s, photons, stored, pos, n, np, fa,
-                       g.cell_id, g.count, ckeys, cvals);
+    // pos: count of stored photons per 32-slot word, then their exclusive scan
+    const uint32_t words = (n + 31u) / 32u;
+    uint32_t* wcnt = pos;
+    uint32_t* wpos = pos + words;
+    hipLaunchKernelGGL(k_word_popc, dim3((words + 255) / 256), dim3(256), 0, s, stored, words, wcnt);
+    if (scan_exclusive(wcnt, wpos, words, work, n_valid, s)) return -1;
+    hipLaunchKernelGGL(k_grid_hash_stored, dim3((words + 255) / 256), dim3(256), 0, s, photons, stored, wpos, words,
+                       np, fa, g.cell_id, g.count, ckeys, cvals);
     int bits = 1;
     while ((1ull << bits) < (uint64_t)cells) bits++;  // keys in [0, cells)
     bool in_tmp = false;
