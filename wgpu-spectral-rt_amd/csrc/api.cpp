@@ -976,7 +976,7 @@ struct wsrt_renderer {
     bool have_cam = false;
     uint32_t frame_count = 0;
     // buffers
-    DevBuf photons, grid_count, grid_offset, grid_cell, grid_slot, grid_sorted, grid_nvalid, grid_temp;
+    DevBuf photons, grid_count, grid_offset, grid_cell, grid_slot, grid_sorted, grid_nvalid, grid_temp, grid_touched;
     DevBuf path_o, path_d, path_tp, path_L, path_wl, path_rng, path_flags, path_hit, q_a, q_b, shade_q,
         shadow;
     DevBuf accum, rgba, counters, gstack, qcounts;
@@ -1088,6 +1088,7 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     HIP_OR_FAIL(r->grid_slot.alloc(sizeof(uint32_t) * nph));
     HIP_OR_FAIL(r->grid_sorted.alloc(sizeof(wsrt_photon) * nph));
     HIP_OR_FAIL(r->grid_nvalid.alloc(sizeof(uint32_t)));
+    HIP_OR_FAIL(r->grid_touched.alloc(sizeof(uint32_t) * nph));
     HIP_OR_FAIL(r->grid_temp.alloc(sizeof(uint32_t) * grid_temp_words((uint32_t)nph, (uint32_t)cells)));
     HIP_OR_FAIL(r->ph_pos.alloc(16 * nph));
     HIP_OR_FAIL(r->ph_dir.alloc(16 * nph));
@@ -1145,6 +1146,7 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     r->gb.sorted = r->grid_sorted.as<wsrt_photon>();
     r->gb.n_valid = r->grid_nvalid.as<uint32_t>();
     r->gb.temp = r->grid_temp.as<uint32_t>();
+    r->gb.touched = r->grid_touched.as<uint32_t>();
     PathLaunch& pl = r->pl;
     pl.sv = scene->view;
     if (!c.env_map) pl.sv.env = nullptr;  // the environment lights misses only on request

@@ -73,6 +73,9 @@ struct GridBuffers {
     wsrt_photon* sorted;    // n
     uint32_t* n_valid;      // 1
     uint32_t* temp;         // grid_temp_words(n, hash_size)
+    // renderer grid: the cell of each photon of the last build (n_valid of
+    // them), so the next build clears only those counts
+    uint32_t* touched = nullptr;
 };
 // Photon paths of nf consecutive frames, np per frame: path i is photon
 // i % np of frame slot i / np, stored at slot i with bit i of ps.stored set.
@@ -91,6 +94,8 @@ int grid_build(const wsrt_photon* photons, uint32_t np, uint32_t nf, uint32_t ha
 // The same grid over the slots whose `stored` bit is set (the photon pass's
 // output; other slots are not read), with offsets written for non-empty
 // cells only: the gather reads a cell's offset only when its count is not 0.
+// The counts must be zero except at g.touched[0 .. *g.n_valid) (all zero on
+// allocation); each build leaves them so for the next.
 int grid_build_stored(const wsrt_photon* photons, const uint32_t* stored, uint32_t np, uint32_t nf,
                       uint32_t hash_size, const FrameArgs& fa, GridBuffers& g, hipStream_t s);
 

@@ -349,15 +349,27 @@ __global__ __launch_bounds__(256) void k_grid_hash_stored(const wsrt_photon* __r
 }
 
 // Offset of every non-empty cell: the position of the first of its photons
-// in the sorted order (keys ascending).
+// in the sorted order (keys ascending), and each photon's cell kept for the
+// next build's clear.
 __global__ __launch_bounds__(256) void k_grid_run_offsets(const uint32_t* __restrict__ skeys,
                                                           const uint32_t* __restrict__ n_valid,
-                                                          uint32_t* __restrict__ offset) {
+                                                          uint32_t* __restrict__ offset,
+                                                          uint32_t* __restrict__ touched) {
     const uint32_t nv = *n_valid;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nv; k += gridDim.x * blockDim.x) {
         const uint32_t key = skeys[k];
         if (k == 0 || skeys[k - 1] != key) offset[key] = k;
+        touched[k] = key;
     }
+}
+
+// Zeroes the counts the last build left non-zero (the rest are zero).
+__global__ __launch_bounds__(256) void k_grid_clear_touched(const uint32_t* __restrict__ touched,
+                                                            const uint32_t* __restrict__ n_last,
+                                                            uint32_t* __restrict__ count) {
+    const uint32_t nl = *n_last;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nl; k += gridDim.x * blockDim.x)
+        count[touched[k]] = 0u;
 }
 
 __global__ void k_grid_copy_count(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
@@ -445,7 +457,8 @@ int grid_build_stored(const wsrt_photon* photons, const uint32_t* stored, uint32
     uint32_t* cvals = ckeys + n;
     uint32_t* work = cvals + n;
     uint32_t* n_valid = g.temp + grid_temp_words(n, cells) - 1;
-    fill_u32(g.count, cells, 0u, s);
+    // the counts are zero except the cells of the last build's photons
+    hipLaunchKernelGGL(k_grid_clear_touched, dim3(1024), dim3(256), 0, s, g.touched, g.n_valid, g.count);
     fill_u32(n_valid, 1, 0u, s);
     if (n == 0) {
         fill_u32(g.n_valid, 1, 0u, s);
@@ -465,7 +478,7 @@ int grid_build_stored(const wsrt_photon* photons, const uint32_t* stored, uint32
     if (radix_sort_pairs(ckeys, cvals, n, bits, work, s, &in_tmp, n_valid)) return -1;
     const uint32_t* sorted_keys = in_tmp ? work : ckeys;
     const uint32_t* sorted_vals = in_tmp ? work + n : cvals;
-    hipLaunchKernelGGL(k_grid_run_offsets, dim3(1024), dim3(256), 0, s, sorted_keys, n_valid, g.offset);
+    hipLaunchKernelGGL(k_grid_run_offsets, dim3(1024), dim3(256), 0, s, sorted_keys, n_valid, g.offset, g.touched);
     hipLaunchKernelGGL(k_grid_copy, dim3((n + 255) / 256), dim3(256), 0, s, photons, sorted_vals, n_valid,
                        g.sorted_slot, g.sorted);
     hipLaunchKernelGGL(k_grid_copy_count, dim3(1), dim3(64), 0, s, n_valid, g.n_valid);
