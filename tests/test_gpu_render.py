@@ -77,8 +77,11 @@ def test_photon_pass_and_grid(device, name, frame):
     ph, stored = r.read_photons()
     ref = o.photon_trace(to_oracle_cfg(cfg), frame, 0, cfg.photons_per_pass)
     gs, os_ = ph["energy"] != 0, ref["energy"] != 0
+    print(f"  stored {gs.sum()} oracle {os_.sum()} same decision {(gs == os_).mean():.6f}", flush=True)
     assert stored == gs.sum() and stored > 100
-    assert (gs == os_).mean() >= 0.99
+    # only paths that diverge numerically may differ (none did on these
+    # cases); the emission's specular-root filter must drop no stored photon
+    assert (gs == os_).mean() >= 0.999
     both = gs & os_
     ok = np.all(np.abs(ph["position"][both] - ref["position"][both]) <= 1e-3, axis=1)
     assert ok.mean() >= 0.99
