@@ -96,6 +96,27 @@ def test_photon_pass_and_grid(device, name, frame):
     np.testing.assert_array_equal(slot[: len(oslot)], oslot)
 
 
+def test_photon_pass_without_specular_geometry(device):
+    """With no metal, glass or thin-film instance the specular BVH is empty:
+    no photon can become caustic, so none is stored (as in the oracle) and
+    the grid is empty."""
+    print("photons, diffuse-only scene", flush=True)
+    g = scenes.build_occluded_room(wsrt.SceneBuilder(), n_boxes=0).build(device)
+    o = scenes.build_occluded_room(O.Scene(), n_boxes=0).build()
+    cfg = scenes.occluded_config(enable_photons=1, photons_per_pass=1 << 14, frames_in_flight=2)
+    v = scenes.OCCLUDED_VIEW
+    view = scenes.View(v.position, v.yaw, v.pitch, 32, 32)
+    r = wsrt.Renderer(device, g, 32, 32, cfg)
+    r.set_camera(scenes.camera(view, 3))
+    r.render()
+    ph, stored = r.read_photons()
+    ref = o.photon_trace(to_oracle_cfg(cfg), 3, 0, cfg.photons_per_pass)
+    assert stored == 0 and np.count_nonzero(ref["energy"]) == 0
+    assert not np.any(ph["energy"])
+    _, count, _, _ = r.read_grid()
+    assert count.sum() == 0
+
+
 @pytest.mark.parametrize("name,frame,photons", [("cornell", 0, 1 << 16), ("cornell", 9, 1 << 16),
                                                 ("caustic", 1, 1 << 18), ("custom", 4, 1 << 17),
                                                 ("heightfield", 2, 1 << 10), ("small_box", 3, 1 << 16)])
