@@ -161,7 +161,7 @@ typedef struct wsrt_config {
     uint32_t guide_train_samples; /* training samples per frame                */
     uint32_t env_map;             /* 1: scene environment map lights misses    */
     uint32_t frame_graphs;        /* 1: submit each batch as one HIP graph     */
-    uint32_t frames_in_flight;    /* consecutive frames traced in one wavefront */
+    uint32_t frames_in_flight;    /* consecutive frames traced in one wavefront (1..256) */
     uint32_t tail_paths;          /* once fewer camera paths than this enter a
                                    * bounce (bounce 2 on), one launch finishes
                                    * them; 0: every bounce in the wavefront.
@@ -372,7 +372,7 @@ int wsrt_debug_read_world(const wsrt_scene* s, float* tris, uint32_t* ids);
 
 /* ---- autotuner (no reference counterpart: the reference renders one frame
  *      per submission) -------------------------------------------------------
- * Times candidate frames_in_flight values (1, 2, 4, .. 64: frames traced in
+ * Times candidate frames_in_flight values (1, 2, 4, .. 256: frames traced in
  * one wavefront, one graph per batch) for a scene, camera (null: the default
  * camera), resolution and config by
  * rendering 2 warm batches each, within a device-memory budget (bytes; 0 =
@@ -380,9 +380,9 @@ int wsrt_debug_read_world(const wsrt_scene* s, float* tris, uint32_t* ids);
  * Results do not depend on frames_in_flight. */
 typedef struct wsrt_autotune_result {
     uint32_t n_candidates;
-    uint32_t frames_in_flight[8];
-    float ms_per_frame[8];        /* 0: not run (over budget or allocation failed) */
-    uint64_t device_bytes[8];     /* device memory the candidate's renderer took */
+    uint32_t frames_in_flight[12];
+    float ms_per_frame[12];       /* 0: not run (over budget or allocation failed) */
+    uint64_t device_bytes[12];    /* device memory the candidate's renderer took */
     uint32_t best;
 } wsrt_autotune_result;
 int wsrt_autotune(wsrt_device* dev, const wsrt_scene* scene, uint32_t width, uint32_t height,

@@ -774,8 +774,8 @@ static int validate_config(const wsrt_config* c) {
     if (!(c->cell_size > 0.0f) || !(c->gather_radius > 0.0f) || !(c->spectral_radius > 0.0f))
         return fail(WSRT_E_INVALID, "cell_size, gather_radius and spectral_radius must be > 0");
     if (c->photon_norm == 0) return fail(WSRT_E_INVALID, "photon_norm must be > 0");
-    if (c->frames_in_flight == 0 || c->frames_in_flight > 64)
-        return fail(WSRT_E_INVALID, "frames_in_flight %u not in 1..64", c->frames_in_flight);
+    if (c->frames_in_flight == 0 || c->frames_in_flight > 256)
+        return fail(WSRT_E_INVALID, "frames_in_flight %u not in 1..256", c->frames_in_flight);
     if ((uint64_t)c->photons_per_pass * c->frames_in_flight > (1ull << 28) ||
         (uint64_t)c->hash_size * c->frames_in_flight > (1ull << 30))
         return fail(WSRT_E_INVALID, "photons or cells times frames_in_flight too large");
@@ -1986,10 +1986,10 @@ extern "C" int wsrt_autotune(wsrt_device* dev, const wsrt_scene* scene, uint32_t
     HIP_OR_FAIL(hipMemGetInfo(&free0, &total));
     if (mem_budget == 0) mem_budget = total / 4;
     memset(out, 0, sizeof *out);
-    const uint32_t cands[7] = {1, 2, 4, 8, 16, 32, 64};
+    const uint32_t cands[9] = {1, 2, 4, 8, 16, 32, 64, 128, 256};
     float best_ms = 0.0f;
     uint32_t best = cfg->frames_in_flight, slower = 0;
-    for (uint32_t k = 0; k < 7; ++k) {
+    for (uint32_t k = 0; k < 9; ++k) {
         const uint32_t S = cands[k];
         out->frames_in_flight[k] = S;
         out->n_candidates = k + 1;

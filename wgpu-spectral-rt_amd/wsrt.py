@@ -47,8 +47,8 @@ class Config(C.Structure):
 
 
 class AutotuneResult(C.Structure):
-    _fields_ = [("n_candidates", C.c_uint32), ("frames_in_flight", C.c_uint32 * 8),
-                ("ms_per_frame", C.c_float * 8), ("device_bytes", C.c_uint64 * 8), ("best", C.c_uint32)]
+    _fields_ = [("n_candidates", C.c_uint32), ("frames_in_flight", C.c_uint32 * 12),
+                ("ms_per_frame", C.c_float * 12), ("device_bytes", C.c_uint64 * 12), ("best", C.c_uint32)]
 
 
 class BatchJob(C.Structure):
