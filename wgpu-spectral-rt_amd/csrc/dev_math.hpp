@@ -73,8 +73,10 @@ __device__ __forceinline__ f3 random_unit_vector(uint32_t& s) {  // shader.wgsl:
     float u2 = rand01(s);
     float z = 1.0f - 2.0f * u1;
     float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-    float phi = 2.0f * kPi * u2;
-    return mk3(r * cosf(phi), r * sinf(phi), z);
+    // cos / sin of phi = 2 pi u2 with the hardware functions, which take
+    // their argument in revolutions (u2 in [0, 1)): a few ulp from the
+    // correctly rounded forms, inside the parity bars
+    return mk3(r * __builtin_amdgcn_cosf(u2), r * __builtin_amdgcn_sinf(u2), z);
 }
 
 // ---- spectral (shader.wgsl:85-139) -----------------------------------------
