@@ -154,28 +154,30 @@ constexpr int kRadix = 256;
 
 // n_dev (optional): the element count in device memory (at most n), so a
 // sort whose size is produced on the device can be launched for its maximum.
+// The kernels loop over the tiles that hold elements (a fixed grid); the
+// histogram entries of the others stay zero from one fill per sort.
 __global__ __launch_bounds__(kSortThreads) void k_radix_histogram(const uint32_t* __restrict__ keys,
                                                                   uint32_t n,
                                                                   const uint32_t* __restrict__ n_dev,
                                                                   int shift, uint32_t num_tiles,
                                                                   uint32_t* __restrict__ hist) {
     if (n_dev) n = min(n, *n_dev);
-    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-    if (base >= n) {  // a tile past the device count: an empty histogram
-        hist[(uint64_t)threadIdx.x * num_tiles + blockIdx.x] = 0u;
-        return;
-    }
+    const uint32_t used = (uint32_t)(((uint64_t)n + kSortTile - 1) / kSortTile);
     __shared__ uint32_t h[kRadix];
-    h[threadIdx.x] = 0;
-    __syncthreads();
+    for (uint32_t tile = blockIdx.x; tile < used; tile += gridDim.x) {
+        const uint64_t base = (uint64_t)tile * kSortTile;
+        h[threadIdx.x] = 0;
+        __syncthreads();
 #pragma unroll 4
-    for (int r = 0; r < kSortRounds; ++r) {
-        uint64_t i = base + (uint64_t)r * kSortThreads + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xFFu], 1u);
+        for (int r = 0; r < kSortRounds; ++r) {
+            uint64_t i = base + (uint64_t)r * kSortThreads + threadIdx.x;
+            if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xFFu], 1u);
+        }
+        __syncthreads();
+        // digit-major layout: hist[digit * num_tiles + tile]
+        hist[(uint64_t)threadIdx.x * num_tiles + tile] = h[threadIdx.x];
+        __syncthreads();
     }
-    __syncthreads();
-    // digit-major layout: hist[digit * num_tiles + tile]
-    hist[(uint64_t)threadIdx.x * num_tiles + blockIdx.x] = h[threadIdx.x];
 }
 
 // Stable scatter of one tile. Keys are taken in input order 256 at a time;
@@ -187,50 +189,52 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint32_t* __restrict__ n_dev, int shift, uint32_t num_tiles,
     const uint32_t* __restrict__ hist_scanned) {
     if (n_dev) n = min(n, *n_dev);
-    if ((uint64_t)blockIdx.x * kSortTile >= n) return;  // block-uniform: nothing to place
+    const uint32_t used = (uint32_t)(((uint64_t)n + kSortTile - 1) / kSortTile);
     constexpr int kWaves = kSortThreads / 64;
     __shared__ uint32_t run[kRadix];
     __shared__ uint32_t wave_cnt[kWaves][kRadix];
-    run[threadIdx.x] = hist_scanned[(uint64_t)threadIdx.x * num_tiles + blockIdx.x];
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) wave_cnt[w][threadIdx.x] = 0;
-    __syncthreads();
     const int wave = threadIdx.x >> 6;
     const uint64_t lt = lanemask_lt();
-    const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
-    for (int r = 0; r < kSortRounds; ++r) {
-        uint64_t i = base + (uint64_t)r * kSortThreads + threadIdx.x;
-        bool valid = i < n;
-        uint32_t key = valid ? keys_in[i] : 0u;
-        uint32_t val = valid ? vals_in[i] : 0u;
-        uint32_t dig = (key >> shift) & 0xFFu;
-        uint64_t peers = __ballot(valid);
+    for (uint32_t tile = blockIdx.x; tile < used; tile += gridDim.x) {
+        run[threadIdx.x] = hist_scanned[(uint64_t)threadIdx.x * num_tiles + tile];
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            uint64_t m = __ballot((dig >> b) & 1u);
-            peers &= ((dig >> b) & 1u) ? m : ~m;
-        }
-        uint32_t rank = (uint32_t)__popcll(peers & lt);
-        bool leader = valid && (peers & lt) == 0;
-        if (leader) wave_cnt[wave][dig] = (uint32_t)__popcll(peers);
+        for (int w = 0; w < kWaves; ++w) wave_cnt[w][threadIdx.x] = 0;
         __syncthreads();
-        if (valid) {
-            uint32_t pos = run[dig] + rank;
-            for (int w = 0; w < wave; ++w) pos += wave_cnt[w][dig];
-            keys_out[pos] = key;
-            vals_out[pos] = val;
-        }
-        __syncthreads();
-        {
-            uint32_t add = 0;
+        const uint64_t base = (uint64_t)tile * kSortTile;
+        for (int r = 0; r < kSortRounds; ++r) {
+            uint64_t i = base + (uint64_t)r * kSortThreads + threadIdx.x;
+            bool valid = i < n;
+            uint32_t key = valid ? keys_in[i] : 0u;
+            uint32_t val = valid ? vals_in[i] : 0u;
+            uint32_t dig = (key >> shift) & 0xFFu;
+            uint64_t peers = __ballot(valid);
 #pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
-                add += wave_cnt[w][threadIdx.x];
-                wave_cnt[w][threadIdx.x] = 0;
+            for (int b = 0; b < 8; ++b) {
+                uint64_t m = __ballot((dig >> b) & 1u);
+                peers &= ((dig >> b) & 1u) ? m : ~m;
             }
-            run[threadIdx.x] += add;
+            uint32_t rank = (uint32_t)__popcll(peers & lt);
+            bool leader = valid && (peers & lt) == 0;
+            if (leader) wave_cnt[wave][dig] = (uint32_t)__popcll(peers);
+            __syncthreads();
+            if (valid) {
+                uint32_t pos = run[dig] + rank;
+                for (int w = 0; w < wave; ++w) pos += wave_cnt[w][dig];
+                keys_out[pos] = key;
+                vals_out[pos] = val;
+            }
+            __syncthreads();
+            {
+                uint32_t add = 0;
+#pragma unroll
+                for (int w = 0; w < kWaves; ++w) {
+                    add += wave_cnt[w][threadIdx.x];
+                    wave_cnt[w][threadIdx.x] = 0;
+                }
+                run[threadIdx.x] += add;
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
@@ -291,12 +295,14 @@ int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, int bits, uint3
     uint32_t* stemp = hscan + hist_n;
     uint32_t *ki = keys, *vi = vals, *ko = kalt, *vo = valt;
     int passes = (bits + 7) / 8;
+    const uint32_t grid = tiles < 2048u ? tiles : 2048u;
+    if (n_dev) fill_u32(hist, hist_n, 0u, s);  // entries of tiles past the device count
     for (int p = 0; p < passes; ++p) {
         int shift = 8 * p;
-        hipLaunchKernelGGL(k_radix_histogram, dim3(tiles), dim3(kSortThreads), 0, s, ki, n, n_dev, shift,
+        hipLaunchKernelGGL(k_radix_histogram, dim3(grid), dim3(kSortThreads), 0, s, ki, n, n_dev, shift,
                            tiles, hist);
         if (scan_exclusive(hist, hscan, hist_n, stemp, nullptr, s)) return -1;
-        hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kSortThreads), 0, s, ki, vi, ko, vo,
+        hipLaunchKernelGGL(k_radix_scatter, dim3(grid), dim3(kSortThreads), 0, s, ki, vi, ko, vo,
                            n, n_dev, shift, tiles, hscan);
         uint32_t* t;
         t = ki; ki = ko; ko = t;
