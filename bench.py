@@ -145,8 +145,8 @@ def copy_calibration(mib: int) -> int:
     return 0
 
 
-def time_to_relmse(frames: int = 32, warmup: int = 256, boxes: int | None = None, width: int = 1024,
-                   ref_frames: int = 128) -> dict:
+def time_to_relmse(frames: int = 64, warmup: int = 256, boxes: int | None = None, width: int = 1024,
+                   ref_frames: int = 128, frames_in_flight: int = 64) -> dict:
     """Guided vs plain tracer on the occluded-light room (2,097,182 triangles
     at the default box count), 1024 x 1024, depth 8.
 
@@ -159,7 +159,10 @@ def time_to_relmse(frames: int = 32, warmup: int = 256, boxes: int | None = None
     or B would grow with their outliers and hide them. Time-to-relMSE 0.01 is
     then (C / 0.01) * (seconds per spp) plus, for the guided tracer, the
     `warmup` frames of online training it runs before being measured (it
-    keeps training while measured)."""
+    keeps training while measured). Both tracers run `frames_in_flight`
+    frames per wavefront (the faster setting for both on this scene, whose
+    2M-triangle traversal needs many rays in flight), and every measured
+    window is whole batches."""
     import numpy as np
     import scenes
     import wsrt
@@ -189,13 +192,16 @@ def time_to_relmse(frames: int = 32, warmup: int = 256, boxes: int | None = None
 
     out = {"scene": f"occluded-light room, {scene.info.n_triangles} triangles, {width}x{width}, depth 8",
            "relmse": "mean (I - R)^2 / (R^2 + 1e-2)", "target": 0.01}
-    rp = renderer(scenes.occluded_config(frames_in_flight=4), 1 << 22)
+    S = frames_in_flight
+    rp = renderer(scenes.occluded_config(frames_in_flight=S), 1 << 22)
     R, _ = frames_of(rp, ref_frames)
     rp.close()
     out["reference_spp"] = ref_frames
+    out["frames_in_flight"] = S
+    out["measured_spp"] = frames
     res = {}
-    for name, cfg in (("plain", scenes.occluded_config(frames_in_flight=4)),
-                      ("guided", scenes.guided_config(frames_in_flight=4))):
+    for name, cfg in (("plain", scenes.occluded_config(frames_in_flight=S)),
+                      ("guided", scenes.guided_config(frames_in_flight=S))):
         r = renderer(cfg, 0)
         tw = 0.0
         if name == "guided":
@@ -312,7 +318,7 @@ def main() -> int:
     ap.add_argument("--no-guided", action="store_true", help="skip the guided-vs-plain time-to-relMSE leg")
     ap.add_argument("--no-other", action="store_true", help="skip the heightfield and batch throughput legs")
     ap.add_argument("--guided-only", action="store_true", help="only the guided-vs-plain leg (one JSON line)")
-    ap.add_argument("--guided-frames", type=int, default=32)
+    ap.add_argument("--guided-frames", type=int, default=64)
     ap.add_argument("--frames-in-flight", type=int, default=0, help="frames per wavefront (default: config's)")
     ap.add_argument("--guided-boxes", type=int, default=None)
     ap.add_argument("--guided-width", type=int, default=1024)
