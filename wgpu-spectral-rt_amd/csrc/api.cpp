@@ -1621,7 +1621,7 @@ extern "C" int wsrt_debug_read_photons(wsrt_renderer* r, wsrt_photon* out, uint3
     std::vector<uint8_t> flag;
     if (!rc) rc = read_stored_flags(r, first, np, flag);
     if (rc) return rc;
-    // a slot holds a photon only when its stored flag is set; the others read
+    // a slot holds a photon only when its stored bit is set; the others read
     // as the reference's skipped photons (all zero)
     uint32_t k = 0;
     for (uint32_t i = 0; i < np; ++i) {
