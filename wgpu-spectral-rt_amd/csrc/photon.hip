@@ -30,7 +30,7 @@ namespace {
 // first hit without storing anything. A test against the root of the
 // specular BVH finds most of those here (the same cost for every lane), and
 // only the others enter the depth-0 queue with their state (at state slot
-// i). Grid-stride over exactly kGridLanes lanes, which bounds each queue
+// i). Grid-stride over at most kGridLanes lanes, which bounds each queue
 // segment.
 __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon_emit(SceneView sv, FrameArgs fa, uint32_t np,
                                                                 uint32_t nf, Queues qs, PhotonState ps,
@@ -404,7 +404,7 @@ void launch_photon_bounce(const SceneView& sv, uint32_t depth, uint32_t count_tr
                           const Queues& qs, wsrt_photon* photons, Counters* ctr, uint2* gstack, hipStream_t s) {
     const uint32_t* q = (depth & 1u) ? ps.queue_b : ps.queue_a;
     uint32_t* nq = (depth & 1u) ? ps.queue_a : ps.queue_b;
-    hipLaunchKernelGGL(k_photon_bounce, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, sv, depth, qs,
+    hipLaunchKernelGGL(k_photon_bounce, dim3(trace_grid(4)), dim3(kTraceThreads), 0, s, sv, depth, qs,
                        count_traversal, ps, q, nq, photons, ctr, gstack);
 }
 

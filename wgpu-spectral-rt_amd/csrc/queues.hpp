@@ -7,11 +7,13 @@
 // queue is therefore split into kSegs segments: producer block b appends to
 // segment b % kSegs (one wave-aggregated atomic on a counter that sits on
 // its own 128-byte line), and a consumer turns its item index into (segment,
-// offset) through a prefix sum of the segment counts held in LDS. Producers
-// are grid-stride kernels of exactly kGridLanes lanes, so segment s receives
-// at most kGridLanes / kSegs items per pass over the input and a segment of
-// seg_cap = (kGridLanes / kSegs) * ceil(capacity / kGridLanes) entries never
-// overflows. Items enter a queue in an order that depends on scheduling;
+// offset) through a prefix sum of the segment counts held in LDS. A
+// producer of L <= kGridLanes lanes (blocks of whole waves, L / kSegs lanes
+// per segment) gives each lane at most ceil(n / L) of its n items, grid-stride
+// or as a wave's contiguous range, so segment s receives at most
+// (L / kSegs) * (n / L + 1) = n / kSegs + L / kSegs items: a segment of
+// seg_cap = ceil(capacity / kSegs) + kGridLanes / kSegs entries never
+// overflows, whatever grid the producer runs. Items enter a queue in an order that depends on scheduling;
 // every consumer treats items independently, so results do not depend on it.
 #pragma once
 
@@ -22,7 +24,7 @@ namespace wsrt {
 
 constexpr int kSegs = 64;
 constexpr int kSegStride = 32;             // uint32 counters 128 bytes apart
-constexpr uint32_t kGridLanes = 524288;     // lanes of every queue-producing launch
+constexpr uint32_t kGridLanes = 524288;     // most lanes of a queue-producing launch
 
 enum QueueKind : uint32_t {
     kQActive = 0,                      // camera paths entering a bounce
@@ -55,7 +57,7 @@ __device__ __forceinline__ bool tail_started(const Queues& q, uint32_t bounce) {
 }
 
 __host__ __device__ inline uint32_t seg_capacity(uint64_t capacity) {
-    return (uint32_t)((kGridLanes / kSegs) * ((capacity + kGridLanes - 1) / kGridLanes));
+    return (uint32_t)((capacity + kSegs - 1) / kSegs + kGridLanes / kSegs);
 }
 __host__ __device__ inline size_t queue_count_words(uint32_t depths) {
     return (size_t)depths * kQKinds * kSegs * kSegStride;
