@@ -208,6 +208,8 @@ def time_to_relmse(frames: int = 64, warmup: int = 256, boxes: int | None = None
             _, tw = frames_of(r, warmup)
         a, ta = frames_of(r, frames)
         b, tb = frames_of(r, frames)
+        if name == "guided":
+            out["mlp"] = mlp_rates(r, S, cfg.guide_train_samples)
         r.close()
         c = frames * float(np.mean((a - b) ** 2 / (R**2 + 1e-2))) / 2
         spp_s = (ta + tb) / (2 * frames)
@@ -217,6 +219,43 @@ def time_to_relmse(frames: int = 64, warmup: int = 256, boxes: int | None = None
                      "training_frames_before_measuring": warmup if name == "guided" else 0}
     out.update(res)
     out["speedup_guided_vs_plain"] = round(res["plain"]["time_to_relmse_s"] / res["guided"]["time_to_relmse_s"], 3)
+    return out
+
+
+# Algorithmic work of the guide network per sample: 168 encoded inputs, four
+# hidden layers of 64, 40 outputs; forward 2 (168*64 + 3*64^2 + 64*40) FLOP,
+# training (forward, backward, weight gradients) three times that.
+MLP_FWD_FLOP = 2 * (168 * 64 + 3 * 64 * 64 + 64 * 40)
+MLP_TRAIN_FLOP = 3 * MLP_FWD_FLOP
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def mlp_rates(r, frames: int, train_samples: int) -> dict:
+    """BF16 MFMA rate of the guide network inside the guided tracer, over
+    `frames` frames with per-kernel HIP events: inference in k_guide_scatter
+    (one evaluation per Lambert vertex queued to it, counted on the device)
+    and the training step (preparation, forward, backward, weight gradients,
+    Adam) on `train_samples` records per frame. The scatter kernel also
+    encodes inputs, samples the mixture and queues the next rays, so its rate
+    understates the MFMA work alone."""
+    s0 = r.stats()["guide_samples"]
+    r.sync()
+    r.reset_profile()
+    r.set_profiling(True)
+    r.render_frames(frames)
+    r.sync()
+    r.set_profiling(False)
+    prof = r.profile()
+    n_inf = r.stats()["guide_samples"] - s0
+    n_train = frames * train_samples
+    out = {"unit": "TFLOP/s", "peak": BF16_PEAK_TFLOPS, "frames": frames,
+           "flop_per_sample": {"inference": MLP_FWD_FLOP, "training": MLP_TRAIN_FLOP}}
+    for key, kname, n, flop in (("inference", "guide_scatter", n_inf, MLP_FWD_FLOP),
+                                ("training", "guide_train", n_train, MLP_TRAIN_FLOP)):
+        ms = prof.get(kname, {}).get("total_ms", 0.0)
+        tf = n * flop / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+        out[key] = {"samples": int(n), "ms": round(ms, 3), "achieved": round(tf, 2),
+                    "frac": round(tf / BF16_PEAK_TFLOPS, 4)}
     return out
 
 

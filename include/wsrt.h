@@ -290,7 +290,7 @@ typedef struct wsrt_frame_stats {
     uint64_t photon_paths;   /* photon paths traced                        */
     uint64_t photon_rays;    /* closest-hit rays of photon paths           */
     uint64_t photons_stored; /* photons written to the grid                */
-    uint64_t guide_samples;  /* guide-sampled directions                   */
+    uint64_t guide_samples;  /* vertices the guide network evaluated      */
     /* BVH8 nodes visited and triangles tested, while counting is enabled:
      * [0] photon rays, [1] camera extend rays, [2] shadow rays; trav_rays
      * counts traversals (a photon's first ray is tested against the

@@ -981,7 +981,7 @@ struct wsrt_renderer {
         shadow;
     DevBuf accum, rgba, counters, gstack, qcounts;
     DevBuf ftable, dframe;           // per-frame constants, device frame counter
-    DevBuf ph_pos, ph_dir, ph_rng, ph_flags, ph_qa, ph_qb, ph_stored;
+    DevBuf ph_id[2], ph_pos[2], ph_dir[2], ph_rng[2], ph_flags[2], ph_stored;
     PhotonState phs{};
     FrameConst* staging = nullptr;   // pinned, 2 halves of kUploadHalf entries
     hipEvent_t upload_ev[2] = {nullptr, nullptr};
@@ -1090,17 +1090,19 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     HIP_OR_FAIL(r->grid_nvalid.alloc(sizeof(uint32_t)));
     HIP_OR_FAIL(r->grid_touched.alloc(sizeof(uint32_t) * nph));
     HIP_OR_FAIL(r->grid_temp.alloc(sizeof(uint32_t) * grid_temp_words((uint32_t)nph, (uint32_t)cells)));
-    HIP_OR_FAIL(r->ph_pos.alloc(16 * nph));
-    HIP_OR_FAIL(r->ph_dir.alloc(16 * nph));
-    HIP_OR_FAIL(r->ph_rng.alloc(4 * nph));
-    HIP_OR_FAIL(r->ph_flags.alloc(4 * nph));
     HIP_OR_FAIL(r->ph_stored.alloc(4 * ((nph + 31) / 32)));
-    const uint32_t pseg = seg_capacity(nph);
-    HIP_OR_FAIL(r->ph_qa.alloc(4ull * kSegs * pseg));
-    HIP_OR_FAIL(r->ph_qb.alloc(4ull * kSegs * pseg));
-    r->phs = PhotonState{r->ph_pos.as<float4>(), r->ph_dir.as<float4>(), r->ph_rng.as<uint32_t>(),
-                         r->ph_flags.as<uint32_t>(), r->ph_qa.as<uint32_t>(), r->ph_qb.as<uint32_t>(),
-                         r->ph_stored.as<uint32_t>()};
+    // photon paths in queue order, one set per depth parity
+    const uint64_t pq = (uint64_t)kSegs * seg_capacity(nph);
+    for (int k = 0; k < 2; ++k) {
+        HIP_OR_FAIL(r->ph_id[k].alloc(4 * pq));
+        HIP_OR_FAIL(r->ph_pos[k].alloc(16 * pq));
+        HIP_OR_FAIL(r->ph_dir[k].alloc(16 * pq));
+        HIP_OR_FAIL(r->ph_rng[k].alloc(4 * pq));
+        HIP_OR_FAIL(r->ph_flags[k].alloc(4 * pq));
+        r->phs.q[k] = PhotonQueue{r->ph_id[k].as<uint32_t>(), r->ph_pos[k].as<float4>(), r->ph_dir[k].as<float4>(),
+                                  r->ph_rng[k].as<uint32_t>(), r->ph_flags[k].as<uint32_t>()};
+    }
+    r->phs.stored = r->ph_stored.as<uint32_t>();
     HIP_OR_FAIL(r->path_o.alloc(16 * npaths));
     HIP_OR_FAIL(r->path_d.alloc(16 * npaths));
     HIP_OR_FAIL(r->path_tp.alloc(16 * npaths));
@@ -1312,7 +1314,7 @@ static int enqueue_batch(wsrt_renderer* r, uint32_t nf) {
         }
         if (pl.guided) {
             ProfScope p(r, K_GUIDE_SCATTER);
-            launch_guide_scatter(pl.ps, pl.qs, pl.gv, pl.fa, b, nq, s);
+            launch_guide_scatter(pl.ps, pl.qs, pl.gv, pl.fa, b, nq, pl.ctr, s);
         }
     }
     if (pl.qs.tail && r->cfg.max_depth > kTailFirstBounce) {

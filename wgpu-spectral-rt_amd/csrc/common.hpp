@@ -121,13 +121,18 @@ struct PathState {
 
 // Photon paths between bounces (structure of arrays indexed by photon path
 // id).
-struct PhotonState {
+// The photon paths of one depth's queue, in queue order: entry e holds the
+// path id (also its photon slot) and the state of the ray it traces next,
+// so the queue's producer writes and its consumer reads whole lines.
+struct PhotonQueue {
+    uint32_t* id;
     float4* pos;        // origin xyz, wavelength
     float4* dir;        // direction xyz, energy
     uint32_t* rng;
     uint32_t* flags;    // 1: caustic path
-    uint32_t* queue_a;
-    uint32_t* queue_b;
+};
+struct PhotonState {
+    PhotonQueue q[2];   // depth d reads q[d & 1] and fills q[(d + 1) & 1]
     uint32_t* stored;   // bit i (word i / 32): photon path i stored its photon in slot i
 };
 

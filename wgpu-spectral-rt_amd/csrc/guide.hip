@@ -245,7 +245,8 @@ __device__ __forceinline__ void load_biases(float* dst, const float* params) {
 // training record of the path's first Lambert vertex.
 __global__ __launch_bounds__(64 * kScatterWaves) void k_guide_scatter(PathState ps, Queues qs, GuideView gv,
                                                                       FrameArgs fa, uint32_t bounce,
-                                                                      uint32_t* __restrict__ next_queue) {
+                                                                      uint32_t* __restrict__ next_queue,
+                                                                      Counters* __restrict__ ctr) {
     __shared__ bf16x8 Wf[kFwdPack / 8];
     __shared__ float bias[4 * kHid + kOutPad];
     __shared__ float stage_all[kScatterWaves][64 * kStage];
@@ -253,6 +254,7 @@ __global__ __launch_bounds__(64 * kScatterWaves) void k_guide_scatter(PathState 
     copy_to_lds(reinterpret_cast<uint4*>(Wf), reinterpret_cast<const uint4*>(gv.wf), kFwdPack / 8);
     load_biases(bias, gv.params);
     const uint32_t n = seg_prefix(qs, bounce, kQGuide, 1, pre);  // syncs the block
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctr->guide_samples, (unsigned long long)n);
     const uint32_t rr_after = fa.table[*fa.frame & fa.mask].rr_after;
     const int wave = threadIdx.x >> 6, l = __lane_id();
     float* stage = stage_all[wave];
@@ -680,9 +682,9 @@ __global__ __launch_bounds__(64 * kScatterWaves) void k_guide_infer(GuideView gv
 size_t guide_act_words(uint32_t ns) { return (size_t)kActRows * ns / 2; }  // in uint32 words
 
 void launch_guide_scatter(const PathState& ps, const Queues& qs, const GuideView& gv, const FrameArgs& fa,
-                          uint32_t bounce, uint32_t* next_queue, hipStream_t s) {
+                          uint32_t bounce, uint32_t* next_queue, Counters* ctr, hipStream_t s) {
     hipLaunchKernelGGL(k_guide_scatter, dim3(256), dim3(64 * kScatterWaves), 0, s, ps, qs, gv, fa, bounce,
-                       next_queue);
+                       next_queue, ctr);
 }
 
 void launch_guide_pack(const GuideView& gv, float* params, hipStream_t s) {

@@ -131,7 +131,7 @@ struct GuideTrain {
 };
 size_t guide_act_words(uint32_t ns);
 void launch_guide_scatter(const PathState& ps, const Queues& qs, const GuideView& gv, const FrameArgs& fa,
-                          uint32_t bounce, uint32_t* next_queue, hipStream_t s);
+                          uint32_t bounce, uint32_t* next_queue, Counters* ctr, hipStream_t s);
 // Writes the packed operands of params (after an upload or an initialisation).
 void launch_guide_pack(const GuideView& gv, float* params, hipStream_t s);
 // One step over ns samples (a multiple of 64 and of gt.chunk), the loss

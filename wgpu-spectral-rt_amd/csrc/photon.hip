@@ -29,8 +29,8 @@ namespace {
 // whose first ray meets no metal, glass or thin-film triangle ends at its
 // first hit without storing anything. A test against the root of the
 // specular BVH finds most of those here (the same cost for every lane), and
-// only the others enter the depth-0 queue with their state (at state slot
-// i). Grid-stride over at most kGridLanes lanes, which bounds each queue
+// only the others enter the depth-0 queue with their state (in queue
+// order). Grid-stride over at most kGridLanes lanes, which bounds each queue
 // segment.
 __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon_emit(SceneView sv, FrameArgs fa, uint32_t np,
                                                                 uint32_t nf, Queues qs, PhotonState ps,
@@ -55,6 +55,8 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon
     for (uint32_t base = gtid & ~63u; base < n; base += stride) {
         const uint32_t id = base + __lane_id();
         bool go = false;
+        float4 sp, sd;
+        uint32_t srng = 0u;
         if (id < n) {
             if ((id & 31u) == 0u) ps.stored[id >> 5] = 0u;  // this lane's word of stored bits
             if (nl != 0u) {
@@ -98,15 +100,21 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon
                     const float flux = (radiance / pdf) * area * kPi;
                     const float energy = flux * prm.z;
                     go = true;
-                    ps.pos[id] = f4(pos, wl);
-                    ps.dir[id] = f4(emit_dir, energy);
-                    ps.rng[id] = rng;
-                    ps.flags[id] = 0u;
+                    sp = f4(pos, wl);
+                    sd = f4(emit_dir, energy);
+                    srng = rng;
                 }
             }
         }
         const uint32_t e = seg_append(qs, 0, kQPhoton, qs.pseg_cap, go);
-        if (go) ps.queue_a[e] = id;
+        if (go) {
+            const PhotonQueue& q = ps.q[0];
+            q.id[e] = id;
+            q.pos[e] = sp;
+            q.dir[e] = sd;
+            q.rng[e] = srng;
+            q.flags[e] = 0u;
+        }
     }
     if (gtid == 0 && nl != 0u) atomicAdd(&ctr->kernel_rays[kRkPhotonEmit], (unsigned long long)n);
     if (count) flush_traversal_counts(ctr, kRkPhotonEmit, my_rays, my_nodes, my_tris);
@@ -126,10 +134,10 @@ struct PhotonRay {
 // The surface interaction of photon_emit.wgsl:228-329 at the closest hit h
 // of the ray (pos, dir): store at a diffuse surface if the path is caustic,
 // else continue through metal, glass and thin film. Returns whether the
-// path continues (its state is then written back).
+// path continues; pos, dir, r.rng and r.caustic are then its next ray's.
 __device__ __forceinline__ bool photon_interact(const SceneView& sv, const PhotonState& ps,
                                                 wsrt_photon* __restrict__ photons, PhotonRay& r,
-                                                f3 pos, f3 dir, const Hit& h, uint32_t& my_stored) {
+                                                f3& pos, f3& dir, const Hit& h, uint32_t& my_stored) {
     if (h.tri == kInvalid) return false;
     bool alive = false;
     bool caustic = r.caustic;
@@ -194,41 +202,47 @@ __device__ __forceinline__ bool photon_interact(const SceneView& sv, const Photo
         alive = true;
         // the origin is not advanced in this branch (photon_emit.wgsl:291-326)
     }
-    if (alive) {
-        ps.pos[r.id] = f4(pos, wl);
-        ps.dir[r.id] = f4(dir, r.energy);
-        ps.rng[r.id] = rng;
-        ps.flags[r.id] = caustic ? 1u : 0u;
-    }
+    r.rng = rng;
+    r.caustic = caustic;
     return alive;
 }
 
 // The rays of entries [begin, end) of a bounce, one per lane at a time.
 __device__ __forceinline__ void photon_bounce_range(const SceneView& sv, uint32_t depth, const Queues& qs,
                                                     const PhotonState& ps, const uint32_t* pre,
-                                                    const uint32_t* __restrict__ queue,
-                                                    uint32_t* __restrict__ next_queue,
                                                     wsrt_photon* __restrict__ photons, Stack st,
                                                     uint32_t begin, uint32_t end, bool count,
                                                     uint32_t& my_rays, uint32_t& my_stored, uint32_t& my_nodes,
                                                     uint32_t& my_tris) {
+    const PhotonQueue& cur = ps.q[depth & 1u];
+    const PhotonQueue& nxt = ps.q[(depth + 1u) & 1u];
     for (uint32_t base = begin; base < end; base += 64u) {
         const uint32_t i = base + __lane_id();
         bool alive = false;
         PhotonRay r{};
+        f3 pos, dir;
         if (i < end) {
-            r.id = queue[seg_entry(pre, kSegs, i, qs.pseg_cap)];
-            const float4 pw = ps.pos[r.id], de = ps.dir[r.id];
+            const uint32_t p = seg_entry(pre, kSegs, i, qs.pseg_cap);
+            r.id = cur.id[p];
+            const float4 pw = cur.pos[p], de = cur.dir[p];
             r.wl = pw.w;
             r.energy = de.w;
-            r.rng = ps.rng[r.id];
-            r.caustic = ps.flags[r.id] != 0u;
-            const Hit h = trace_ray<false>(sv, xyz(pw), xyz(de), 0.0001f, 100.0f, st, count, my_nodes, my_tris);
+            r.rng = cur.rng[p];
+            r.caustic = cur.flags[p] != 0u;
+            pos = xyz(pw);
+            dir = xyz(de);
+            const Hit h = trace_ray<false>(sv, pos, dir, 0.0001f, 100.0f, st, count, my_nodes, my_tris);
             my_rays++;
-            alive = photon_interact(sv, ps, photons, r, xyz(pw), xyz(de), h, my_stored);
+            alive = photon_interact(sv, ps, photons, r, pos, dir, h, my_stored);
         }
         const uint32_t e = seg_append(qs, depth + 1, kQPhoton, qs.pseg_cap, alive);
-        if (alive) next_queue[e] = r.id;
+        if (alive) {
+            nxt.id[e] = r.id;
+            nxt.pos[e] = f4(pos, r.wl);
+            nxt.dir[e] = f4(dir, r.energy);
+            nxt.rng[e] = r.rng;
+            nxt.flags[e] = r.caustic ? 1u : 0u;
+        }
     }
 }
 
@@ -238,7 +252,7 @@ __device__ __forceinline__ void photon_bounce_range(const SceneView& sv, uint32_
 // wave takes a contiguous range of the queue.
 __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(4) void k_photon_bounce(
     SceneView sv, uint32_t depth, Queues qs, uint32_t count_traversal, PhotonState ps,
-    const uint32_t* __restrict__ queue, uint32_t* __restrict__ next_queue, wsrt_photon* __restrict__ photons,
+    wsrt_photon* __restrict__ photons,
     Counters* __restrict__ ctr, uint2* __restrict__ gstack) {
     __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
     __shared__ uint32_t pre[kSegs + 1];
@@ -256,7 +270,7 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(4) void k_photon
     const uint32_t begin = (uint32_t)min((uint64_t)n, (uint64_t)wave * per_wave * 64u);
     const uint32_t end = (uint32_t)min((uint64_t)n, (uint64_t)(wave + 1u) * per_wave * 64u);
     uint32_t my_rays = 0, my_stored = 0, my_nodes = 0, my_tris = 0;
-    photon_bounce_range(sv, depth, qs, ps, pre, queue, next_queue, photons, st, begin, end, count_traversal != 0u,
+    photon_bounce_range(sv, depth, qs, ps, pre, photons, st, begin, end, count_traversal != 0u,
                         my_rays, my_stored, my_nodes, my_tris);
     if (gtid == 0) atomicAdd(&ctr->kernel_rays[kRkPhotonBounce], (unsigned long long)n);
     if (count_traversal) flush_traversal_counts(ctr, kRkPhotonBounce, my_rays, my_nodes, my_tris);
@@ -402,10 +416,8 @@ void launch_photon_emit(const SceneView& sv, const FrameArgs& fa, uint32_t np, u
 
 void launch_photon_bounce(const SceneView& sv, uint32_t depth, uint32_t count_traversal, const PhotonState& ps,
                           const Queues& qs, wsrt_photon* photons, Counters* ctr, uint2* gstack, hipStream_t s) {
-    const uint32_t* q = (depth & 1u) ? ps.queue_b : ps.queue_a;
-    uint32_t* nq = (depth & 1u) ? ps.queue_a : ps.queue_b;
     hipLaunchKernelGGL(k_photon_bounce, dim3(trace_grid(4)), dim3(kTraceThreads), 0, s, sv, depth, qs,
-                       count_traversal, ps, q, nq, photons, ctr, gstack);
+                       count_traversal, ps, photons, ctr, gstack);
 }
 
 size_t grid_temp_words(uint32_t n, uint32_t cells) {
