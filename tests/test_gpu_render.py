@@ -67,10 +67,15 @@ def rel_close(a, b, rel, floor):
     return np.all(np.abs(a - b) <= rel * np.maximum(np.abs(a), np.abs(b)) + floor, axis=-1)
 
 
-@pytest.mark.parametrize("name,frame", [("cornell", 0), ("cornell", 5), ("caustic", 2), ("custom", 3)])
-def test_photon_pass_and_grid(device, name, frame):
-    print(f"photons {name} frame {frame}", flush=True)
-    g, o, cfg, view = setup(device, name, 64, 64, 1 << 16)
+# the caustic case at 2^20 photons is the caustic workload's full photon pass
+# (2^22 cells): its ~14k stored photons all come through the emission's cone
+# and root-box filters, so a filter that dropped a path would show here
+@pytest.mark.parametrize("name,frame,photons", [("cornell", 0, 1 << 16), ("cornell", 5, 1 << 16),
+                                                ("caustic", 2, 1 << 16), ("custom", 3, 1 << 16),
+                                                ("caustic", 7, 1 << 20)])
+def test_photon_pass_and_grid(device, name, frame, photons):
+    print(f"photons {name} frame {frame} photons {photons}", flush=True)
+    g, o, cfg, view = setup(device, name, 64, 64, photons)
     r = wsrt.Renderer(device, g, view.width, view.height, cfg)
     r.set_camera(scenes.camera(view, frame))
     r.render()

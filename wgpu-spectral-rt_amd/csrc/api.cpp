@@ -1281,7 +1281,7 @@ static int enqueue_batch(wsrt_renderer* r, uint32_t nf) {
     if (np) {
         {
             ProfScope p(r, K_PHOTON_EMIT);
-            launch_photon_emit(pl.sv, pl.fa, np, nf, pl.qs, r->phs, pl.count_traversal, pl.ctr, pl.gstack, s);
+            launch_photon_emit(pl.sv, pl.fa, np, nf, pl.qs, r->phs, pl.count_traversal, pl.ctr, s);
         }
         for (uint32_t d = 0; d < r->cfg.photon_max_depth; ++d) {
             ProfScope p(r, K_PHOTON_TRACE);

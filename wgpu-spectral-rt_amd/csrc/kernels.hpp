@@ -91,8 +91,7 @@ struct GridBuffers {
 // Emission (which queues the paths whose first ray meets specular geometry),
 // then one launch per bounce depth.
 void launch_photon_emit(const SceneView& sv, const FrameArgs& fa, uint32_t np, uint32_t nf, const Queues& qs,
-                        const PhotonState& ps, uint32_t count_traversal, Counters* ctr, uint2* gstack,
-                        hipStream_t s);
+                        const PhotonState& ps, uint32_t count_traversal, Counters* ctr, hipStream_t s);
 void launch_photon_bounce(const SceneView& sv, uint32_t depth, uint32_t count_traversal, const PhotonState& ps,
                           const Queues& qs, wsrt_photon* photons, Counters* ctr, uint2* gstack, hipStream_t s);
 size_t grid_temp_words(uint32_t n, uint32_t cells);

@@ -22,100 +22,230 @@ static_assert(kTraceBlocks <= kStatBlocks, "per-block statistics of the photon b
 
 namespace {
 
+constexpr int kConeLights = 64;  // lights with an emission cone in LDS
+constexpr int kCandCap = 128;     // candidates a wave buffers (< 64 left + 64 new)
+
+// Bounding sphere (centre, radius) of the child boxes of the root of sv's
+// BVH. The radius is padded by 1e-3 of the centre's distance from the
+// origin plus the radius, far beyond the slab test's own padding.
+__device__ float4 root_sphere(const SceneView& sv) {
+    const uint4* np = reinterpret_cast<const uint4*>(sv.nodes);
+    const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
+    const float org[3] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)};
+    const uint32_t lo_l[3] = {w2.x, w2.z, w3.x}, lo_h[3] = {w2.y, w2.w, w3.y};
+    const uint32_t hi_l[3] = {w3.z, w4.x, w4.z}, hi_h[3] = {w3.w, w4.y, w4.w};
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    for (int c = 0; c < 8; ++c) {
+        if (byte_of(w1.z, w1.w, c) == 0u) continue;
+        for (int a = 0; a < 3; ++a) {
+            const float sc = __uint_as_float((uint32_t)((int)(int8_t)((w0.w >> (8 * a)) & 0xFFu) + 127) << 23);
+            lo[a] = fminf(lo[a], org[a] + (float)byte_of(lo_l[a], lo_h[a], c) * sc);
+            hi[a] = fmaxf(hi[a], org[a] + (float)byte_of(hi_l[a], hi_h[a], c) * sc);
+        }
+    }
+    const f3 c = mk3(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2]));
+    const float r = 0.5f * length(mk3(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]));
+    return f4(c, r + 1e-3f * (length(c) + r) + 1e-4f);
+}
+
+// Cone (unit axis, cosine of the half-angle) holding the direction u of
+// every ray from light L's emission points (pos = point + 0.001 u) that
+// meets the sphere sph at t > 0: from a point within r_l of the light's
+// centre, a ray meeting the sphere passes within R = r_l + r of the centre
+// d away, so its angle to the axis has sine <= R / d. Cosine -2 (no test)
+// when the light is too close for a cone.
+__device__ float4 light_cone(const wsrt_light_info& L, float4 sph) {
+    const float4 v0 = *reinterpret_cast<const float4*>(L.v0);
+    const float4 prm = *reinterpret_cast<const float4*>(L.params);
+    f3 cl = xyz(v0);
+    float rl = 0.0f;
+    if (prm.x != 1.0f) {  // triangle: centroid and farthest corner
+        const f3 a = xyz(v0), b = xyz(*reinterpret_cast<const float4*>(L.v1));
+        const f3 c = xyz(*reinterpret_cast<const float4*>(L.v2));
+        cl = (a + b + c) * (1.0f / 3.0f);
+        rl = fmaxf(length(a - cl), fmaxf(length(b - cl), length(c - cl)));
+    }
+    const f3 D = xyz(sph) - cl;
+    const float d = length(D);
+    const float R = rl * 1.001f + 0.0011f + sph.w + 1e-3f * d;
+    if (!(d > 1.01f * R)) return make_float4(0.0f, 0.0f, 1.0f, -2.0f);
+    const float s = R / d;
+    return f4(D * (1.0f / d), sqrtf(1.0f - s * s) - 1e-4f);
+}
+
+// The first ray of photon paths buffered in a wave's LDS (one entry per
+// candidate): id, origin, direction, rng state after the direction, light.
+struct Cand {
+    uint32_t id[kCandCap];
+    float px[kCandCap], py[kCandCap], pz[kCandCap];
+    float dx[kCandCap], dy[kCandCap], dz[kCandCap];
+    uint32_t rng[kCandCap], li[kCandCap];
+};
+
+// Root test and the rest of the emission (photon_emit.wgsl:199-216) for
+// buffered candidates [0, cnt), one per lane; the paths that may reach
+// specular geometry enter the depth-0 queue.
+__device__ __forceinline__ void emit_candidates(const SceneView& sv, const SceneView& spec, const Cand& cb,
+                                                uint32_t cnt, const CauchyConst& cc, const Queues& qs,
+                                                const PhotonState& ps) {
+    const uint32_t l = __lane_id();
+    bool go = false;
+    float4 sp, sd;
+    uint32_t srng = 0u, id = 0u;
+    if (l < cnt) {
+        id = cb.id[l];
+        const f3 pos = mk3(cb.px[l], cb.py[l], cb.pz[l]);
+        const f3 dir = mk3(cb.dx[l], cb.dy[l], cb.dz[l]);
+        if (meets_root_child(spec, pos, dir, 0.0001f, 100.0f)) {
+            uint32_t rng = cb.rng[l];
+            const wsrt_light_info& L = sv.lights[cb.li[l]];
+            const float4 prm = *reinterpret_cast<const float4*>(L.params);
+            float area = 4.0f;
+            if (prm.x != 1.0f) {
+                const f3 a = xyz(*reinterpret_cast<const float4*>(L.v0));
+                const f3 e1 = xyz(*reinterpret_cast<const float4*>(L.v1)) - a;
+                const f3 e2 = xyz(*reinterpret_cast<const float4*>(L.v2)) - a;
+                area = length(cross(e1, e2)) * 0.5f;
+            }
+            const float wl = cauchy_sample(rand01(rng), cc);
+            const float pdf = cauchy_pdf(wl, cc);
+            const float radiance = blackbody(wl, prm.y);
+            const float flux = (radiance / pdf) * area * kPi;
+            const float energy = flux * prm.z;
+            go = true;
+            sp = f4(pos, wl);
+            sd = f4(dir, energy);
+            srng = rng;
+        }
+    }
+    const uint32_t e = seg_append(qs, 0, kQPhoton, qs.pseg_cap, go);
+    if (go) {
+        const PhotonQueue& q = ps.q[0];
+        q.id[e] = id;
+        q.pos[e] = sp;
+        q.dir[e] = sd;
+        q.rng[e] = srng;
+        q.flags[e] = 0u;
+    }
+}
+
 // Emission of photon path i (photon_emit.wgsl:159-216): light choice, point
 // on the light, cosine direction, hero wavelength and flux; every path's
 // stored bit is cleared (its photon slot holds a photon only once the bit
 // is set). A photon is stored only after a specular interaction, so a path
 // whose first ray meets no metal, glass or thin-film triangle ends at its
-// first hit without storing anything. A test against the root of the
-// specular BVH finds most of those here (the same cost for every lane), and
-// only the others enter the depth-0 queue with their state (in queue
-// order). Grid-stride over at most kGridLanes lanes, which bounds each queue
-// segment.
+// first hit without storing anything. Such rays are found in two steps:
+// a cone per light around the specular BVH's root sphere rejects most of
+// them with one dot product, and the others (buffered per wave in LDS until
+// 64 fill a wave) take the slab test against the root's child boxes. Only
+// the paths that pass it enter the depth-0 queue, with their state in queue
+// order; a ray outside the cone meets no child box, so the queue holds the
+// same paths as with the slab test alone. Grid-stride over at most
+// kGridLanes lanes, and a wave queues at most as many paths as it emits,
+// which bounds each queue segment.
 __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon_emit(SceneView sv, FrameArgs fa, uint32_t np,
                                                                 uint32_t nf, Queues qs, PhotonState ps,
                                                                 uint32_t count_traversal,
-                                                                Counters* __restrict__ ctr,
-                                                                uint2* __restrict__ gstack) {
-    __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
+                                                                Counters* __restrict__ ctr) {
     __shared__ unsigned long long red[kTraceThreads / 64];
+    __shared__ float4 cone[kConeLights];
+    __shared__ Cand cand_all[kTraceThreads / 64];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack st;
-    st.lds = lds_slot(lds_stack + threadIdx.x);
-    st.lds_stride = kTraceThreads;
-    st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
     const SceneView spec = specular_view(sv);
     const uint32_t n = np * nf, nl = sv.n_lights;
+    const bool use_cone = spec.n_nodes != 0u && nl <= (uint32_t)kConeLights;
+    if (use_cone) {
+        const float4 sph = root_sphere(spec);
+        for (uint32_t k = threadIdx.x; k < nl; k += blockDim.x) cone[k] = light_cone(sv.lights[k], sph);
+    }
+    __syncthreads();
+    Cand& cb = cand_all[threadIdx.x >> 6];
     const uint32_t base_frame = *fa.frame;
     const FrameConst& f0 = fa.table[base_frame & fa.mask];
     const CauchyConst cc{f0.cdf_min, f0.cdf_range};
     const bool count = count_traversal != 0u;
     uint32_t my_rays = 0, my_nodes = 0, my_tris = 0;
+    uint32_t ncand = 0;  // wave-uniform
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = gtid & ~63u; base < n; base += stride) {
         const uint32_t id = base + __lane_id();
-        bool go = false;
-        float4 sp, sd;
-        uint32_t srng = 0u;
+        bool c = false;
+        f3 pos, emit_dir;
+        uint32_t rng = 0u, li = 0u;
         if (id < n) {
             if ((id & 31u) == 0u) ps.stored[id >> 5] = 0u;  // this lane's word of stored bits
-            if (nl != 0u) {
+            if (nl != 0u && spec.n_nodes != 0u) {
                 const uint32_t slot = id / np, local = id - slot * np;
-                uint32_t rng = pcg_hash(local + (base_frame + slot) * 927163u);
-                uint32_t li = (uint32_t)(rand01(rng) * (float)nl);
+                rng = pcg_hash(local + (base_frame + slot) * 927163u);
+                li = (uint32_t)(rand01(rng) * (float)nl);
                 if (li > nl - 1u) li = nl - 1u;
                 const wsrt_light_info& L = sv.lights[li];
                 const float4 v0 = *reinterpret_cast<const float4*>(L.v0);
                 const float4 prm = *reinterpret_cast<const float4*>(L.params);
-                f3 emit_pos, emit_dir;
-                float area;
+                f3 emit_pos;
                 if (prm.x == 1.0f) {  // point light (extension): uniform sphere
                     emit_pos = xyz(v0);
                     emit_dir = random_unit_vector(rng);
-                    area = 4.0f;
                 } else {
                     const float4 v1 = *reinterpret_cast<const float4*>(L.v1);
                     const float4 v2 = *reinterpret_cast<const float4*>(L.v2);
-                    f3 a = xyz(v0), b = xyz(v1), c = xyz(v2);
-                    f3 e1 = b - a, e2 = c - a;
+                    f3 a = xyz(v0), b = xyz(v1), cc3 = xyz(v2);
+                    f3 e1 = b - a, e2 = cc3 - a;
                     f3 nrm = normalize(cross(e1, e2));
-                    area = length(cross(e1, e2)) * 0.5f;
                     float r1 = rand01(rng), r2 = rand01(rng), sr = sqrtf(r1);
                     float u = 1.0f - sr, v = r2 * sr, w = 1.0f - u - v;
-                    emit_pos = a * u + b * v + c * w;
+                    emit_pos = a * u + b * v + cc3 * w;
                     emit_dir = normalize(nrm + random_unit_vector(rng));
                 }
-                const f3 pos = emit_pos + emit_dir * 0.001f;
-                // a ray that meets no child box of the specular BVH's root
-                // cannot reach a specular triangle; the others are traced in
-                // full, and only they need a wavelength and an energy (drawn
-                // after the direction, in the reference's order)
-                const bool meets = meets_root_child(spec, pos, emit_dir, 0.0001f, 100.0f);
-                if (count && spec.n_nodes) my_nodes++;
-                my_rays++;
-                if (meets) {
-                    const float wl = cauchy_sample(rand01(rng), cc);
-                    const float pdf = cauchy_pdf(wl, cc);
-                    const float radiance = blackbody(wl, prm.y);
-                    const float flux = (radiance / pdf) * area * kPi;
-                    const float energy = flux * prm.z;
-                    go = true;
-                    sp = f4(pos, wl);
-                    sd = f4(emit_dir, energy);
-                    srng = rng;
+                pos = emit_pos + emit_dir * 0.001f;
+                c = true;
+                if (use_cone) {
+                    const float4 k = cone[li];
+                    c = dot(emit_dir, xyz(k)) >= k.w;
                 }
             }
+            if (nl != 0u) {
+                if (count && spec.n_nodes) my_nodes++;
+                my_rays++;
+            }
         }
-        const uint32_t e = seg_append(qs, 0, kQPhoton, qs.pseg_cap, go);
-        if (go) {
-            const PhotonQueue& q = ps.q[0];
-            q.id[e] = id;
-            q.pos[e] = sp;
-            q.dir[e] = sd;
-            q.rng[e] = srng;
-            q.flags[e] = 0u;
+        // append this wave's candidates to its buffer; test 64 at a time
+        const uint64_t m = __ballot(c);
+        if (c) {
+            const uint32_t k = ncand + (uint32_t)__popcll(m & lanemask_lt());
+            cb.id[k] = id;
+            cb.px[k] = pos.x;
+            cb.py[k] = pos.y;
+            cb.pz[k] = pos.z;
+            cb.dx[k] = emit_dir.x;
+            cb.dy[k] = emit_dir.y;
+            cb.dz[k] = emit_dir.z;
+            cb.rng[k] = rng;
+            cb.li[k] = li;
+        }
+        ncand += (uint32_t)__popcll(m);
+        if (ncand >= 64u) {
+            emit_candidates(sv, spec, cb, 64u, cc, qs, ps);
+            ncand -= 64u;
+            // move the rest (< 64) to the front
+            const uint32_t l = __lane_id();
+            if (l < ncand) {
+                const uint32_t a = cb.id[64 + l], r = cb.rng[64 + l], li2 = cb.li[64 + l];
+                const float x = cb.px[64 + l], y = cb.py[64 + l], z = cb.pz[64 + l];
+                const float dx = cb.dx[64 + l], dy = cb.dy[64 + l], dz = cb.dz[64 + l];
+                cb.id[l] = a;
+                cb.rng[l] = r;
+                cb.li[l] = li2;
+                cb.px[l] = x;
+                cb.py[l] = y;
+                cb.pz[l] = z;
+                cb.dx[l] = dx;
+                cb.dy[l] = dy;
+                cb.dz[l] = dz;
+            }
         }
     }
+    if (ncand) emit_candidates(sv, spec, cb, ncand, cc, qs, ps);
     if (gtid == 0 && nl != 0u) atomicAdd(&ctr->kernel_rays[kRkPhotonEmit], (unsigned long long)n);
     if (count) flush_traversal_counts(ctr, kRkPhotonEmit, my_rays, my_nodes, my_tris);
     // every photon path's first ray
@@ -408,10 +538,9 @@ __global__ __launch_bounds__(256) void k_grid_copy(const wsrt_photon* __restrict
 }  // namespace
 
 void launch_photon_emit(const SceneView& sv, const FrameArgs& fa, uint32_t np, uint32_t nf, const Queues& qs,
-                        const PhotonState& ps, uint32_t count_traversal, Counters* ctr, uint2* gstack,
-                        hipStream_t s) {
+                        const PhotonState& ps, uint32_t count_traversal, Counters* ctr, hipStream_t s) {
     hipLaunchKernelGGL(k_photon_emit, dim3(kTraceBlocks), dim3(kTraceThreads), 0, s, sv, fa, np, nf, qs, ps,
-                       count_traversal, ctr, gstack);
+                       count_traversal, ctr);
 }
 
 void launch_photon_bounce(const SceneView& sv, uint32_t depth, uint32_t count_traversal, const PhotonState& ps,
