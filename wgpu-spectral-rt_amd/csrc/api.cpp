@@ -867,7 +867,7 @@ struct wsrt_guide {
 
 namespace {
 
-constexpr uint32_t kGuideChunk = 256;    // samples per partial gradient sum
+constexpr uint32_t kGuideChunk = 128;    // samples per partial gradient sum (768 gradient waves per 16384-sample step)
 constexpr uint32_t kGuideBatch = 16384;  // samples per Adam step while rendering
 
 // Training-step size for n samples: a multiple of 64, and of the partial-sum

@@ -26,7 +26,11 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kStage = 41;            // floats per sample row of the output staging
 constexpr int kScatterWaves = 8;      // waves per block of the scattering kernel
 constexpr int kTrainWaves = 4;        // waves per block of the training kernel
-constexpr int kTrainTiles = 2;        // 16-sample tiles per wave while training
+// 16-sample tiles per wave in the training forward and backward passes. A
+// training step covers 16384 samples; one tile per wave gives it 1024 waves,
+// one per SIMD, and the two passes run 27 % faster than with 2 and 4 tiles.
+constexpr int kTrainTiles = 1;
+constexpr int kBwdTiles = 1;
 // path flags 4 << b: the Lambert vertex of bounce b is recorded for training
 constexpr uint32_t kFlagRecorded = 4u;
 
@@ -465,14 +469,15 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_guide_train_bwd(GuideView 
     copy_to_lds(reinterpret_cast<uint4*>(Wt), reinterpret_cast<const uint4*>(gv.wt), kBwdPack / 8);
     __syncthreads();
     const int wave = threadIdx.x >> 6, l = __lane_id(), g = l >> 4, n = l & 15;
-    const uint32_t wstride = gridDim.x * kTrainWaves * 64;
-    for (uint32_t col0 = (blockIdx.x * kTrainWaves + wave) * 64; col0 < ns; col0 += wstride) {
+    constexpr uint32_t kWs = 16 * kBwdTiles;  // samples per wave
+    const uint32_t wstride = gridDim.x * kTrainWaves * kWs;
+    for (uint32_t col0 = (blockIdx.x * kTrainWaves + wave) * kWs; col0 < ns; col0 += wstride) {
         // output gradient, natural row order, as the B operand (rows >= 48 zero)
-        bf16x8 B[2][4];
+        bf16x8 B[2][kBwdTiles];
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
+            for (int t = 0; t < kBwdTiles; ++t)
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     const int row = 32 * kb + 8 * g + j;
@@ -481,25 +486,25 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_guide_train_bwd(GuideView 
 #pragma unroll 1
         for (int layer = 4; layer >= 1; --layer) {
             const int off = layer == 4 ? kT5 : layer == 3 ? kT4 : layer == 2 ? kT3 : kT2;
-            f32x4 C[4][4];
+            f32x4 C[4][kBwdTiles];
 #pragma unroll
             for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
-                for (int t = 0; t < 4; ++t) C[mb][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                for (int t = 0; t < kBwdTiles; ++t) C[mb][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
                 for (int mb = 0; mb < 4; ++mb) {
                     const bf16x8 A = Wt[off / 8 + (mb * 2 + kb) * 64 + l];
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) C[mb][t] = mfma(A, B[kb][t], C[mb][t]);
+                    for (int t = 0; t < kBwdTiles; ++t) C[mb][t] = mfma(A, B[kb][t], C[mb][t]);
                 }
             const uint32_t drow = kD1 + 64 * (layer - 1);
             const uint32_t hrow = kH1 + 64 * (layer - 1);
 #pragma unroll
             for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
-                for (int t = 0; t < 4; ++t)
+                for (int t = 0; t < kBwdTiles; ++t)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int row = 16 * mb + 4 * g + r;
@@ -537,6 +542,9 @@ __global__ __launch_bounds__(64) void k_guide_wgrad(Act act, uint32_t chunk, flo
     float bsum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     const bool do_bias = part != 1;
     const uint32_t k_begin = c * chunk, k_end = k_begin + chunk;
+    // two k-steps per iteration keep two steps' loads in flight (chunk is a
+    // multiple of 64)
+#pragma unroll 2
     for (uint32_t k0 = k_begin; k0 < k_end; k0 += 32) {
         bf16x8 A[4];
 #pragma unroll
@@ -607,6 +615,7 @@ __global__ __launch_bounds__(256) void k_guide_adam(const float* __restrict__ pa
     float w = params[i];
     if (update && !is_padding(i)) {
         float gsum = 0.0f;
+#pragma unroll 8
         for (uint32_t c = 0; c < nchunks; ++c) gsum += partial[(size_t)c * kParams + i];
         const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
         const float t = (float)step[0];
@@ -700,7 +709,8 @@ void launch_guide_train_step(const GuideView& gv, const GuideTrain& gt, const fl
     const uint32_t blocks = std::min<uint32_t>(512u, (ns + per_block - 1) / per_block);
     hipLaunchKernelGGL(k_guide_train_fwd, dim3(blocks), dim3(64 * kTrainWaves), 0, s, gv, in0, in1, dir, ns, norm,
                        act, gt.step, update ? 1u : 0u);
-    const uint32_t bblocks = std::min<uint32_t>(1024u, (ns + 64 * kTrainWaves - 1) / (64 * kTrainWaves));
+    const uint32_t bper = 16 * kBwdTiles * kTrainWaves;
+    const uint32_t bblocks = std::min<uint32_t>(1024u, (ns + bper - 1) / bper);
     hipLaunchKernelGGL(k_guide_train_bwd, dim3(bblocks), dim3(64 * kTrainWaves), 0, s, gv, ns, act);
     const uint32_t nchunks = ns / gt.chunk;
     hipLaunchKernelGGL(k_guide_wgrad, dim3(6 * nchunks), dim3(64), 0, s, act, gt.chunk, gt.partial);
