@@ -339,6 +339,12 @@ int wsrt_debug_read_grid(wsrt_renderer* r, uint32_t* cell_id, uint32_t* count, u
 /* Per-pixel hero wavelengths and spectral radiance of the last frame
  * (float4 each, pixel = y*width + x). */
 int wsrt_debug_read_paths(wsrt_renderer* r, float* wavelengths, float* radiance);
+/* The same for n chosen pixels (pixel_ids[k] < width*height): hero
+ * wavelengths and spectral radiance of each pixel's path in the last frame,
+ * float4 each, in the order of pixel_ids. A parity tap; the reference has no
+ * counterpart (its paths live in a shader). */
+int wsrt_debug_trace_paths(wsrt_renderer* r, const uint32_t* pixel_ids, uint32_t n, float* wavelengths,
+                           float* radiance);
 
 /* Standalone photon grid build on a fixed photon buffer (the build_grid pass,
  * build_grid.wgsl:25-73, as a counting sort). photons: n host records.

@@ -143,6 +143,14 @@ def test_camera_paths_match_oracle(device, name, frame, photons):
     print(f"  paths within 1e-4: {same.mean():.5f}; nonzero {np.mean(np.abs(oL).sum(1) > 0):.3f}")
     assert same.mean() >= 0.99
     assert np.abs(oL).sum() > 0
+    # the per-pixel tap returns the same records for chosen pixels, and
+    # rejects a pixel outside the image
+    ids = np.array([W * H - 1, 0, 7, W * H // 2], np.uint32)
+    twl, tL = r.trace_paths(ids)
+    np.testing.assert_array_equal(twl, wl[ids])
+    np.testing.assert_array_equal(tL, L[ids])
+    with pytest.raises(wsrt.WsrtError):
+        r.trace_paths([W * H])
 
 
 @pytest.mark.parametrize("name", ["cornell", "caustic", "custom", "heightfield", "small_box"])

@@ -1691,6 +1691,23 @@ extern "C" int wsrt_debug_read_paths(wsrt_renderer* r, float* wl, float* L) {
     return rc;
 }
 
+extern "C" int wsrt_debug_trace_paths(wsrt_renderer* r, const uint32_t* pixel_ids, uint32_t n, float* wl,
+                                      float* L) {
+    CHECK_ARG(r && (pixel_ids || n == 0) && ((wl && L) || n == 0));
+    const size_t npx = (size_t)r->W * r->H;
+    for (uint32_t k = 0; k < n; ++k)
+        if (pixel_ids[k] >= npx) return fail(WSRT_E_INVALID, "pixel id %u >= %zu pixels", pixel_ids[k], npx);
+    if (n == 0) return WSRT_OK;
+    std::vector<float> all_wl(4 * npx), all_L(4 * npx);
+    int rc = wsrt_debug_read_paths(r, all_wl.data(), all_L.data());
+    if (rc) return rc;
+    for (uint32_t k = 0; k < n; ++k) {
+        memcpy(wl + 4ull * k, all_wl.data() + 4ull * pixel_ids[k], 16);
+        memcpy(L + 4ull * k, all_L.data() + 4ull * pixel_ids[k], 16);
+    }
+    return WSRT_OK;
+}
+
 extern "C" int wsrt_photon_grid_build(wsrt_device* dev, const wsrt_photon* photons, uint32_t n,
                                       uint32_t frame_count, float cell_size, uint32_t hash_size,
                                       uint32_t* cell_id, uint32_t* count, uint32_t* offset,

@@ -137,6 +137,7 @@ _SIGS = {
     "wsrt_debug_read_photons": (C.c_int, [vp, vp, u32, P(u32)]),
     "wsrt_debug_read_grid": (C.c_int, [vp, vp, vp, vp, vp]),
     "wsrt_debug_read_paths": (C.c_int, [vp, vp, vp]),
+    "wsrt_debug_trace_paths": (C.c_int, [vp, vp, C.c_uint32, vp, vp]),
     "wsrt_photon_grid_build": (C.c_int, [vp, vp, u32, u32, C.c_float, u32, vp, vp, vp, vp]),
     "wsrt_debug_closest_hit": (C.c_int, [vp, vp, u32, vp, vp]),
     "wsrt_debug_any_hit": (C.c_int, [vp, vp, u32, vp]),
@@ -616,6 +617,15 @@ class Renderer:
         L = np.zeros((npx, 4), np.float32)
         _check(load().wsrt_debug_read_paths(self.h, _p(wl), _p(L)))
         return wl, L
+
+    def trace_paths(self, pixel_ids):
+        """Hero wavelengths and spectral radiance (n x 4 each) of the chosen
+        pixels' paths in the last frame."""
+        ids = np.ascontiguousarray(pixel_ids, np.uint32).reshape(-1)
+        wl = np.zeros((max(len(ids), 1), 4), np.float32)
+        L = np.zeros((max(len(ids), 1), 4), np.float32)
+        _check(load().wsrt_debug_trace_paths(self.h, _p(ids), len(ids), _p(wl), _p(L)))
+        return wl[: len(ids)], L[: len(ids)]
 
 
 def photon_grid_build(device: Device, photons: np.ndarray, frame_count: int, cell_size: float,
