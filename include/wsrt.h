@@ -278,6 +278,17 @@ int wsrt_renderer_read_rgba8(wsrt_renderer* r, uint8_t* out);
  * width*height RGBA8, rows top to bottom, sRGB OETF applied to the tone-mapped
  * RGBA8 channels, alpha 255. Writing the PNG file is left to the caller. */
 int wsrt_renderer_read_screenshot(wsrt_renderer* r, uint8_t* out);
+/* The presented frame of Renderer::render (mod.rs:193-200): the blit pass
+ * (blit_pass.rs:114-180, blit.wgsl) draws the storage image into a
+ * win_w x win_h window with its aspect kept (scale_x = target/window when
+ * the window is wider, else scale_y = window/target) over a blue clear,
+ * nearest-sampled, into an sRGB surface. With overlay_photons set, the
+ * photon overlay of debug_photons_pass.rs and debug_photons.wgsl (recorded
+ * only in a commented-out line of render) is drawn over it: each stored
+ * photon of the last frame as a one-pixel point of (1, 0.6, 0.1), alpha 0.3,
+ * blended additively in linear space. out: win_w*win_h RGBA8, rows top to
+ * bottom, window size 1..16384 per axis. */
+int wsrt_renderer_present(wsrt_renderer* r, uint32_t win_w, uint32_t win_h, int overlay_photons, uint8_t* out);
 /* Accumulation state export/import (float4 per pixel) for resume. */
 int wsrt_renderer_write_accum(wsrt_renderer* r, const float* in);
 

@@ -357,3 +357,96 @@ class Frame:
         lib.or_frame_gather(self.h, np.asarray(pos, np.float32), np.asarray(n, np.float32),
                             np.asarray(wl, np.float32), out)
         return out
+
+
+# ---- presented frame: blit pass and photon overlay ---------------------------
+def _srgb_oetf8(v: np.ndarray) -> np.ndarray:
+    """Linear [0, 1] -> 8-bit sRGB (IEC 61966-2-1 OETF), in float64."""
+    v = np.clip(np.asarray(v, np.float64), 0.0, 1.0)
+    e = np.where(v <= 0.0031308, v * 12.92, 1.055 * np.power(v, 1.0 / 2.4) - 0.055)
+    return np.floor(np.clip(e, 0.0, 1.0) * 255.0 + 0.5).astype(np.uint8)
+
+
+def _srgb_eotf(c: np.ndarray) -> np.ndarray:
+    """8-bit sRGB -> linear, in float64."""
+    e = np.asarray(c, np.float64) / 255.0
+    return np.where(e <= 0.04045, e / 12.92, np.power((e + 0.055) / 1.055, 2.4))
+
+
+def present(image: np.ndarray, win_w: int, win_h: int, photons=None, view=None, proj=None) -> np.ndarray:
+    """The presented frame, [win_h, win_w, 4] uint8, rows top to bottom.
+
+    image: the storage texture, [H, W, 4] uint8 with row 0 the image bottom
+    (shader.wgsl:721-722). The blit (blit_pass.rs:114-180, blit.wgsl:1-34)
+    draws a quad scaled by (sx, sy) -- sx = target/window aspect when the
+    window is wider, else sy = window/target, in f32 -- over a blue clear
+    (blit_pass.rs:160); a fragment at the pixel centre samples the texel
+    floor(uv * size) (wgpu's default sampler: nearest, clamp to edge), where
+    uv = (ndc / scale + 1) / 2 and uv = (0, 0) is texture row 0 at the window
+    bottom. The surface is sRGB, so written values pass through the OETF.
+    With photons (32-byte records, energy 0 = skipped) and the camera's view
+    and proj (column-major), each photon is drawn as a one-pixel point
+    (debug_photons.wgsl:28-46, debug_photons_pass.rs:84-96): clip = proj *
+    view * p, clipped to the view volume, colour (1, 0.6, 0.1) with alpha
+    0.3 added to the linear destination and re-encoded, point by point;
+    alpha is replaced by the f32 0.3 (unorm8 77)."""
+    H, W = image.shape[:2]
+    f32 = np.float32
+    target, window = f32(W) / f32(H), f32(win_w) / f32(win_h)
+    sx, sy = f32(1.0), f32(1.0)
+    if window > target:
+        sx = target / window
+    else:
+        sy = window / target
+    xs = np.arange(win_w, dtype=np.float32)
+    ys = np.arange(win_h, dtype=np.float32)
+    ndx = (xs + f32(0.5)) / f32(win_w) * f32(2.0) - f32(1.0)
+    ndy = f32(1.0) - (ys + f32(0.5)) / f32(win_h) * f32(2.0)
+    inx = (ndx >= -sx) & (ndx < sx)
+    iny = (ndy > -sy) & (ndy <= sy)
+    u = (ndx / sx + f32(1.0)) * f32(0.5)
+    v = (ndy / sy + f32(1.0)) * f32(0.5)
+    tx = np.minimum(np.floor(u * f32(W)).clip(0).astype(np.int64), W - 1)
+    ty = np.minimum(np.floor(v * f32(H)).clip(0).astype(np.int64), H - 1)
+    enc = _srgb_oetf8(np.arange(256) / 255.0)
+    out = np.zeros((win_h, win_w, 4), np.uint8)
+    out[..., 2] = 255
+    out[..., 3] = 255
+    tex = image[ty[:, None], tx[None, :]]
+    inside = iny[:, None] & inx[None, :]
+    out[inside, 0] = enc[tex[inside, 0]]
+    out[inside, 1] = enc[tex[inside, 1]]
+    out[inside, 2] = enc[tex[inside, 2]]
+    out[inside, 3] = tex[inside, 3]
+    if photons is None:
+        return out
+    pos = np.ascontiguousarray(photons["position"], np.float32)
+    keep = photons["energy"] != 0
+    pos = pos[keep]
+    V = np.asarray(view, np.float32).reshape(4, 4)  # V[c, r]: column c
+    P = np.asarray(proj, np.float32).reshape(4, 4)
+    p4 = [pos[:, 0], pos[:, 1], pos[:, 2], np.ones(len(pos), np.float32)]
+    vv = [((V[0, r] * p4[0] + V[1, r] * p4[1]) + V[2, r] * p4[2]) + V[3, r] * p4[3] for r in range(4)]
+    cc = [((P[0, r] * vv[0] + P[1, r] * vv[1]) + P[2, r] * vv[2]) + P[3, r] * vv[3] for r in range(4)]
+    w = cc[3]
+    ok = (w > 0) & (cc[0] >= -w) & (cc[0] <= w) & (cc[1] >= -w) & (cc[1] <= w) & (cc[2] >= 0) & (cc[2] <= w)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        nx, ny = cc[0] / w, cc[1] / w
+        fx = (nx + f32(1.0)) * f32(0.5) * f32(win_w)
+        fy = (f32(1.0) - ny) * f32(0.5) * f32(win_h)
+    px, py = np.floor(fx[ok]).astype(np.int64), np.floor(fy[ok]).astype(np.int64)
+    on = (px >= 0) & (py >= 0) & (px < win_w) & (py < win_h)
+    count = np.zeros((win_h, win_w), np.int64)
+    np.add.at(count, (py[on], px[on]), 1)
+    add = (0.3 * 1.0, 0.3 * 0.6, 0.3 * 0.1)
+    steps = [_srgb_oetf8(_srgb_eotf(np.arange(256)) + a) for a in add]
+    hit = count > 0
+    for ch in range(3):
+        c = out[..., ch][hit].astype(np.int64)
+        k = np.minimum(count[hit], 256)
+        for _ in range(int(k.max(initial=0))):
+            c = np.where(k > 0, steps[ch][c], c)
+            k = k - 1
+        out[..., ch][hit] = c
+    out[..., 3][hit] = int(np.floor(float(np.float32(0.3) * np.float32(255.0)) + 0.5))  # f32 alpha 0.3 as unorm8
+    return out

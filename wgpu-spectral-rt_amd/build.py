@@ -16,7 +16,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libwsrt.so")
-SOURCES = ["api.cpp", "bvh.hip", "sort.hip", "photon.hip", "path.hip", "query.hip", "guide.hip", "batch.hip"]
+SOURCES = ["api.cpp", "bvh.hip", "sort.hip", "photon.hip", "path.hip", "query.hip", "guide.hip", "batch.hip",
+           "present.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("WSRT_ARCH", "gfx950")

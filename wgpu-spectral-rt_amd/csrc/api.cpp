@@ -1503,6 +1503,68 @@ extern "C" int wsrt_renderer_read_screenshot(wsrt_renderer* r, uint8_t* out) {
     return WSRT_OK;
 }
 
+// The presented frame. The blit (blit_pass.rs:114-180) keeps the image's
+// aspect: scale_x = target / window when the window is wider, else scale_y =
+// window / target, both in f32. The surface is wgpu's default sRGB format,
+// so every value written is encoded with the sRGB OETF; the encodings and
+// the overlay's per-point blend step are 256-entry tables computed here in
+// double precision.
+extern "C" int wsrt_renderer_present(wsrt_renderer* r, uint32_t win_w, uint32_t win_h, int overlay_photons,
+                                     uint8_t* out) {
+    CHECK_ARG(r && out);
+    if (win_w == 0 || win_h == 0 || win_w > 16384 || win_h > 16384)
+        return fail(WSRT_E_INVALID, "window %ux%u outside 1..16384", win_w, win_h);
+    HIP_OR_FAIL(hipSetDevice(r->dev->ordinal));
+    auto oetf8 = [](double v) {
+        v = std::min(std::max(v, 0.0), 1.0);
+        const double e = v <= 0.0031308 ? v * 12.92 : 1.055 * std::pow(v, 1.0 / 2.4) - 0.055;
+        return (uint8_t)std::lround(std::min(std::max(e, 0.0), 1.0) * 255.0);
+    };
+    auto eotf = [](int c) {
+        const double e = c / 255.0;
+        return e <= 0.04045 ? e / 12.92 : std::pow((e + 0.055) / 1.055, 2.4);
+    };
+    uint8_t lut[1024];
+    const double add[3] = {0.3 * 1.0, 0.3 * 0.6, 0.3 * 0.1};  // debug_photons.wgsl colour times alpha
+    for (int c = 0; c < 256; ++c) {
+        lut[c] = oetf8(c / 255.0);
+        for (int k = 0; k < 3; ++k) lut[256 * (k + 1) + c] = oetf8(eotf(c) + add[k]);
+    }
+    PresentArgs a{};
+    memcpy(a.view, r->cam.view, sizeof a.view);
+    memcpy(a.proj, r->cam.proj, sizeof a.proj);
+    const float target_aspect = (float)r->W / (float)r->H, window_aspect = (float)win_w / (float)win_h;
+    a.sx = 1.0f;
+    a.sy = 1.0f;
+    if (window_aspect > target_aspect) a.sx = target_aspect / window_aspect;
+    else a.sy = window_aspect / target_aspect;
+    a.win_w = win_w;
+    a.win_h = win_h;
+    a.img_w = r->W;
+    a.img_h = r->H;
+    a.point_alpha = (uint8_t)std::lround((double)(0.3f * 255.0f));  // the shader's f32 alpha 0.3 as unorm8: 77
+    const size_t npx = (size_t)win_w * win_h;
+    const uint32_t np = r->cfg.enable_photons ? r->cfg.photons_per_pass : 0u;
+    DevBuf dlut, dout, dcount;
+    HIP_OR_FAIL(dlut.alloc(sizeof lut));
+    HIP_OR_FAIL(dout.alloc(4 * npx));
+    HIP_OR_FAIL(hipMemcpyAsync(dlut.p, lut, sizeof lut, hipMemcpyHostToDevice, r->stream));
+    a.lut = dlut.as<uint8_t>();
+    uint32_t* count = nullptr;
+    if (overlay_photons) {
+        HIP_OR_FAIL(dcount.alloc(4 * npx));
+        HIP_OR_FAIL(hipMemsetAsync(dcount.p, 0, 4 * npx, r->stream));
+        count = dcount.as<uint32_t>();
+    }
+    const size_t first = (size_t)r->last_slot * np;
+    launch_present(r->rgba.as<uint8_t>(), r->photons.as<wsrt_photon>() + first, r->ph_stored.as<uint32_t>(),
+                   (uint32_t)first, np, a, count, dout.as<uint8_t>(), r->stream);
+    HIP_OR_FAIL(hipGetLastError());
+    HIP_OR_FAIL(hipMemcpyAsync(out, dout.p, 4 * npx, hipMemcpyDeviceToHost, r->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(r->stream));
+    return WSRT_OK;
+}
+
 extern "C" int wsrt_renderer_write_accum(wsrt_renderer* r, const float* in) {
     CHECK_ARG(r && in);
     HIP_OR_FAIL(hipSetDevice(r->dev->ordinal));

@@ -542,9 +542,6 @@ __global__ __launch_bounds__(64) void k_guide_wgrad(Act act, uint32_t chunk, flo
     float bsum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     const bool do_bias = part != 1;
     const uint32_t k_begin = c * chunk, k_end = k_begin + chunk;
-    // two k-steps per iteration keep two steps' loads in flight (chunk is a
-    // multiple of 64)
-#pragma unroll 2
     for (uint32_t k0 = k_begin; k0 < k_end; k0 += 32) {
         bf16x8 A[4];
 #pragma unroll

@@ -198,4 +198,21 @@ void launch_shadow(const PathLaunch& pl, uint32_t bounce, hipStream_t s);
 void launch_tail(const PathLaunch& pl, uint32_t npx, uint32_t max_depth, hipStream_t s);
 void launch_accumulate(const PathLaunch& pl, uint32_t npx, uint32_t nf, hipStream_t s);
 
+// ---- presented frame (present.hip) -------------------------------------------
+struct PresentArgs {
+    float view[16], proj[16];  // camera matrices, column-major
+    float sx, sy;              // quad scale of the blit (aspect fit)
+    uint32_t win_w, win_h, img_w, img_h;
+    // 4 x 256 bytes: the sRGB encoding of an 8-bit linear value, then for
+    // r, g, b the 8-bit sRGB value after one overlay point is blended in
+    const uint8_t* lut;
+    uint8_t point_alpha;
+};
+// Blit of the RGBA8 image (img_w x img_h) into out (win_w x win_h RGBA8,
+// row 0 = window top); with count (win_w x win_h zeroed words) the photons
+// of slots 0 .. n_photons-1 whose stored bit (first + slot) is set are drawn
+// over it.
+void launch_present(const uint8_t* image, const wsrt_photon* photons, const uint32_t* stored, uint32_t first,
+                    uint32_t n_photons, const PresentArgs& a, uint32_t* count, uint8_t* out, hipStream_t s);
+
 }  // namespace wsrt

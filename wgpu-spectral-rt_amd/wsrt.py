@@ -128,6 +128,7 @@ _SIGS = {
     "wsrt_renderer_read_rgba8": (C.c_int, [vp, vp]),
     "wsrt_renderer_write_accum": (C.c_int, [vp, vp]),
     "wsrt_renderer_read_screenshot": (C.c_int, [vp, vp]),
+    "wsrt_renderer_present": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_int, vp]),
     "wsrt_renderer_get_stats": (C.c_int, [vp, P(FrameStats)]),
     "wsrt_renderer_reset_stats": (C.c_int, [vp]),
     "wsrt_renderer_set_traversal_counting": (C.c_int, [vp, C.c_int]),
@@ -550,6 +551,14 @@ class Renderer:
     def write_accum(self, acc: np.ndarray):
         acc = np.ascontiguousarray(acc, np.float32)
         _check(load().wsrt_renderer_write_accum(self.h, _p(acc)))
+
+    def present(self, window_width: int, window_height: int, overlay_photons: bool = False) -> np.ndarray:
+        """[window_height, window_width, 4] uint8: the presented frame (the
+        blit pass into an sRGB window surface, blit_pass.rs / blit.wgsl), with
+        the photon overlay of debug_photons.wgsl drawn over it on request."""
+        out = np.zeros((window_height, window_width, 4), np.uint8)
+        _check(load().wsrt_renderer_present(self.h, window_width, window_height, int(overlay_photons), _p(out)))
+        return out
 
     def screenshot(self) -> np.ndarray:
         """[H, W, 4] uint8: the screenshot image (screenshot.rs:47-73)."""
