@@ -24,14 +24,17 @@ constexpr int kTraceBlocks = 4096;
 // segment count, which the segment capacity bound needs.
 constexpr int kCUs = 256;
 constexpr int trace_grid(int waves) { return kCUs * 4 * waves / (kTraceThreads / 64); }
-static_assert(trace_grid(5) <= kTraceBlocks, "resident traversal grids fit the stack spill area");
+static_assert(trace_grid(6) <= kTraceBlocks, "resident traversal grids fit the stack spill area");
 constexpr uint64_t kTraceLanes = (uint64_t)kTraceThreads * kTraceBlocks;
 // Register budgets of the traversal kernels, as waves per SIMD the compiler
 // must fit (gfx950: 512 VGPRs per lane and SIMD). Left free, the compiler
 // gives the extend kernel 122 VGPRs (4 waves); with 5 it fits 96 with a few
 // spills outside the node loop and runs 5 % faster on the caustic workload,
 // while 6 or 8 waves spill inside the loop and run slower. The photon bounce
-// fits 4 waves (128 VGPRs) without spills.
+// fits 4 waves (128 VGPRs) without spills, but with its grid sized to
+// resident waves 5 waves (96 VGPRs, a few spills) run it 5 % faster. The
+// any-hit shadow kernel runs 4 % faster at 6 waves (80 VGPRs) than at the
+// compiler's 5.
 #define WSRT_WAVES_PER_SIMD(n) __attribute__((amdgpu_waves_per_eu(n)))
 
 // ---- primitives (sort.hip) -------------------------------------------------

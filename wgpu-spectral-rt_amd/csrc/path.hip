@@ -202,7 +202,7 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
     }
 }
 
-__global__ __launch_bounds__(kTraceThreads) void k_shadow(SceneView sv, PathState ps, Queues qs,
+__global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(6) void k_shadow(SceneView sv, PathState ps, Queues qs,
                                                           uint32_t count_traversal,
                                                           Counters* __restrict__ ctr, uint32_t bounce,
                                                           uint2* __restrict__ gstack) {
@@ -360,7 +360,7 @@ void launch_shade(const PathLaunch& pl, uint32_t npx, uint32_t bounce, uint32_t*
                        pl.fa, npx, pl.qs, bounce, pl.guided, next_queue);
 }
 void launch_shadow(const PathLaunch& pl, uint32_t bounce, hipStream_t s) {
-    hipLaunchKernelGGL(k_shadow, dim3(trace_grid(5)), dim3(kTraceThreads), 0, s, pl.sv, pl.ps, pl.qs,
+    hipLaunchKernelGGL(k_shadow, dim3(trace_grid(6)), dim3(kTraceThreads), 0, s, pl.sv, pl.ps, pl.qs,
                        pl.count_traversal, pl.ctr, bounce, pl.gstack);
 }
 void launch_tail(const PathLaunch& pl, uint32_t npx, uint32_t max_depth, hipStream_t s) {

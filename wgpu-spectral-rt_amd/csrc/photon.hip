@@ -380,7 +380,7 @@ __device__ __forceinline__ void photon_bounce_range(const SceneView& sv, uint32_
 // hit, then the surface interaction. Survivors are compacted into the next
 // queue, so the few paths refracting through glass run in full waves. Each
 // wave takes a contiguous range of the queue.
-__global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(4) void k_photon_bounce(
+__global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon_bounce(
     SceneView sv, uint32_t depth, Queues qs, uint32_t count_traversal, PhotonState ps,
     wsrt_photon* __restrict__ photons,
     Counters* __restrict__ ctr, uint2* __restrict__ gstack) {
@@ -545,7 +545,7 @@ void launch_photon_emit(const SceneView& sv, const FrameArgs& fa, uint32_t np, u
 
 void launch_photon_bounce(const SceneView& sv, uint32_t depth, uint32_t count_traversal, const PhotonState& ps,
                           const Queues& qs, wsrt_photon* photons, Counters* ctr, uint2* gstack, hipStream_t s) {
-    hipLaunchKernelGGL(k_photon_bounce, dim3(trace_grid(4)), dim3(kTraceThreads), 0, s, sv, depth, qs,
+    hipLaunchKernelGGL(k_photon_bounce, dim3(trace_grid(5)), dim3(kTraceThreads), 0, s, sv, depth, qs,
                        count_traversal, ps, photons, ctr, gstack);
 }
 
