@@ -525,15 +525,20 @@ __global__ __launch_bounds__(256) void k_grid_copy(const wsrt_photon* __restrict
                                                    const uint32_t* __restrict__ n_valid,
                                                    uint32_t* __restrict__ sorted_slot,
                                                    wsrt_photon* __restrict__ sorted) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= *n_valid) return;
-    const uint32_t slot = sorted_vals[k];
-    sorted_slot[k] = slot;
-    const float4* src = reinterpret_cast<const float4*>(photons + slot);
-    float4* dst = reinterpret_cast<float4*>(sorted + k);
-    dst[0] = src[0];
-    dst[1] = src[1];
+    // grid-stride over the valid photons only (their count is on the device)
+    const uint32_t nv = *n_valid;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nv; k += gridDim.x * blockDim.x) {
+        const uint32_t slot = sorted_vals[k];
+        sorted_slot[k] = slot;
+        const float4* src = reinterpret_cast<const float4*>(photons + slot);
+        float4* dst = reinterpret_cast<float4*>(sorted + k);
+        dst[0] = src[0];
+        dst[1] = src[1];
+    }
 }
+// blocks of a launch over at most n valid photons: enough to fill the chip,
+// no more than the photons need
+inline uint32_t valid_grid(uint32_t n) { return std::max(1u, std::min((n + 255u) / 256u, 2048u)); }
 
 }  // namespace
 
@@ -583,7 +588,7 @@ int grid_build(const wsrt_photon* photons, uint32_t np, uint32_t nf, uint32_t ha
     bool in_tmp = false;
     if (radix_sort_pairs(ckeys, cvals, n, bits, work, s, &in_tmp, n_valid)) return -1;
     const uint32_t* sorted_vals = in_tmp ? work + n : cvals;
-    hipLaunchKernelGGL(k_grid_copy, dim3((n + 255) / 256), dim3(256), 0, s, photons, sorted_vals, n_valid,
+    hipLaunchKernelGGL(k_grid_copy, dim3(valid_grid(n)), dim3(256), 0, s, photons, sorted_vals, n_valid,
                        g.sorted_slot, g.sorted);
     hipLaunchKernelGGL(k_grid_copy_count, dim3(1), dim3(64), 0, s, n_valid, g.n_valid);
     return 0;
@@ -620,7 +625,7 @@ int grid_build_stored(const wsrt_photon* photons, const uint32_t* stored, uint32
     const uint32_t* sorted_keys = in_tmp ? work : ckeys;
     const uint32_t* sorted_vals = in_tmp ? work + n : cvals;
     hipLaunchKernelGGL(k_grid_run_offsets, dim3(1024), dim3(256), 0, s, sorted_keys, n_valid, g.offset, g.touched);
-    hipLaunchKernelGGL(k_grid_copy, dim3((n + 255) / 256), dim3(256), 0, s, photons, sorted_vals, n_valid,
+    hipLaunchKernelGGL(k_grid_copy, dim3(valid_grid(n)), dim3(256), 0, s, photons, sorted_vals, n_valid,
                        g.sorted_slot, g.sorted);
     hipLaunchKernelGGL(k_grid_copy_count, dim3(1), dim3(64), 0, s, n_valid, g.n_valid);
     return 0;
