@@ -249,7 +249,7 @@ __device__ __forceinline__ float4 gather_photons(const GridView& g, const FrameC
     uint32_t h = cell_base + hash_cell(hp, j, fc.cell_size, fc.hash_size);
     uint32_t cnt = g.count[h];
     if (cnt > fc.gather_cap) cnt = fc.gather_cap;
-    const uint32_t off = g.offset[h];
+    const uint32_t off = cnt ? g.offset[h] : 0u;  // offsets are written for non-empty cells only
     const float4* ph = reinterpret_cast<const float4*>(g.sorted);
     const float r = fc.radius, r2 = fc.radius_sq, sr = fc.spectral_radius;
     for (uint32_t k = 0; k < cnt; ++k) {
