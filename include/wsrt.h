@@ -378,7 +378,10 @@ int wsrt_debug_any_hit(const wsrt_scene* s, const float* rays, uint32_t n, uint3
 /* Per-ray count of BVH8 nodes and triangles the closest-hit traversal visits. */
 int wsrt_debug_traversal_counts(const wsrt_scene* s, const float* rays, uint32_t n,
                                 uint32_t* nodes, uint32_t* tris);
-/* 30-bit Morton code per world triangle (triangle order) and the sorted order. */
+/* 30-bit Morton code per world triangle (triangle order), and the build
+ * order: triangles sorted by (size class, code), where a triangle is large
+ * when its box spans more than 1/8 of the centroid bounds on some axis and
+ * large triangles come after all others. */
 int wsrt_debug_read_morton(const wsrt_scene* s, uint32_t* codes, uint32_t* sorted_tris);
 /* BVH8 arrays: nodes (n_nodes * node_bytes), leaf triangle records
  * (n_triangles * tri_bytes). Either pointer may be NULL. */

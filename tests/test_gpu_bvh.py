@@ -45,7 +45,16 @@ def test_world_and_morton_bit_exact(device, cornell_pair):
     np.testing.assert_array_equal(gi, oi)
     codes, order = g.morton()
     np.testing.assert_array_equal(codes, o.morton())
-    assert np.all(np.diff(codes[order].astype(np.int64)) >= 0)
+    # build order: by (size class, code); a triangle is large when its box
+    # spans more than 1/8 of the centroid bounds on some axis (exact in f32)
+    v = gt.reshape(-1, 3, 3)
+    lo, hi = v.min(axis=1), v.max(axis=1)
+    cent = (lo + hi) * np.float32(0.5)
+    ext = cent.max(axis=0) - cent.min(axis=0)
+    large = ((ext > 0) & ((hi - lo) > np.float32(0.125) * ext)).any(axis=1)
+    assert 0 < large.sum() < len(large)  # the Cornell box: walls and boxes large, the sphere small
+    keys = codes.astype(np.int64) | (large.astype(np.int64) << 30)
+    assert np.all(np.diff(keys[order]) >= 0)
     assert sorted(order.tolist()) == list(range(len(codes)))
 
 

@@ -3,7 +3,8 @@
 //
 //  1. world triangles: every instance's mesh transformed by its matrix (f32,
 //     glam's mul_vec4 summation order), triangle AABBs and centroids;
-//  2. 30-bit Morton codes of the centroids over the centroid bounds;
+//  2. 30-bit Morton codes of the centroids over the centroid bounds, and
+//     sort keys that add a size-class bit above them (large triangles last);
 //  3. stable radix sort of (code, triangle);
 //  4. binary radix tree (Karras 2012), duplicates split by position;
 //  5. bottom-up boxes and triangle counts (one atomic flag per node);
@@ -110,12 +111,19 @@ __device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
     return v;
 }
 
-__global__ void k_morton(const float* __restrict__ cent, uint32_t n,
-                         const uint32_t* __restrict__ cbounds, uint32_t* __restrict__ codes,
-                         uint32_t* __restrict__ vals) {
+// 30-bit Morton code of each triangle's box centroid over the centroid
+// bounds (codes), and its sort key: the code with bit 30 set for a large
+// triangle, one whose box spans more than 1/8 of the centroid bounds on some
+// axis. The radix tree's first split then puts the large triangles (walls,
+// floors, ground planes) in a subtree of their own, so their boxes do not
+// inflate every ancestor of the small triangles near them.
+__global__ void k_morton(const float* __restrict__ cent, const float* __restrict__ box, uint32_t n,
+                         const uint32_t* __restrict__ cbounds, uint32_t* __restrict__ keys,
+                         uint32_t* __restrict__ codes, uint32_t* __restrict__ vals) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     uint32_t q[3];
+    bool large = false;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         float mn = ord2f(cbounds[a]), mx = ord2f(cbounds[3 + a]);
@@ -124,8 +132,12 @@ __global__ void k_morton(const float* __restrict__ cent, uint32_t n,
         float y = x * 1024.0f;
         uint32_t qi = (uint32_t)y;
         q[a] = qi > 1023u ? 1023u : qi;
+        const float bext = box[6ull * t + 3 + a] - box[6ull * t + a];
+        large = large || (ext > 0.0f && bext > 0.125f * ext);
     }
-    codes[t] = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
+    const uint32_t code = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
+    codes[t] = code;
+    keys[t] = code | (large ? 1u << 30 : 0u);
     vals[t] = t;
 }
 
@@ -443,13 +455,12 @@ int bvh_build(const BvhInputs& in, BvhOutputs* out, hipStream_t s) {
     uint32_t blocks = (n + 255) / 256;
     hipLaunchKernelGGL(k_world_tris, dim3(blocks), dim3(256), 0, s, in.instances, in.n_instances,
                        in.verts, in.indices, in.meshes, n, wv, box, cent, tri_inst, cbounds);
-    hipLaunchKernelGGL(k_morton, dim3(blocks), dim3(256), 0, s, cent, n, cbounds, codes, vals);
-    // keep the unsorted codes for inspection
+    // the unsorted codes are kept for inspection; the sort keys go to codes
     uint32_t* codes_copy;
     if (hipMalloc((void**)&codes_copy, sizeof(uint32_t) * n) != hipSuccess) return -2;
-    if (hipMemcpyAsync(codes_copy, codes, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s)) return -3;
+    hipLaunchKernelGGL(k_morton, dim3(blocks), dim3(256), 0, s, cent, box, n, cbounds, codes, codes_copy, vals);
     bool in_tmp = false;
-    if (radix_sort_pairs(codes, vals, n, 30, sort_tmp, s, &in_tmp)) return -1;
+    if (radix_sort_pairs(codes, vals, n, 31, sort_tmp, s, &in_tmp)) return -1;
     uint32_t* scodes = in_tmp ? sort_tmp : codes;
     uint32_t* order = in_tmp ? sort_tmp + n : vals;
     fill_u32(flags, n, 0u, s);
