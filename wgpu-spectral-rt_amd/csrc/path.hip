@@ -100,8 +100,9 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend
         const uint32_t i = base + __lane_id();
         bool have = i < n;
         uint32_t p = 0, bin = 0;
+        const uint32_t s0 = bounce == 0 ? 0u : seg_of_wave_base(pre, base);  // whole wave
         if (have) {
-            p = bounce == 0 ? i : queue[seg_entry(pre, kSegs, i, qs.seg_cap)];
+            p = bounce == 0 ? i : queue[seg_entry_from(pre, s0, i, qs.seg_cap)];
             const float4 o = ps.o[p], d = ps.d[p];
             Hit h = trace_ray<false>(sv, xyz(o), xyz(d), 0.0001f, 100.0f, st, count, my_nodes, my_tris);
             my_rays++;

@@ -351,8 +351,9 @@ __device__ __forceinline__ void photon_bounce_range(const SceneView& sv, uint32_
         bool alive = false;
         PhotonRay r{};
         f3 pos, dir;
+        const uint32_t s0 = seg_of_wave_base(pre, base);  // whole wave
         if (i < end) {
-            const uint32_t p = seg_entry(pre, kSegs, i, qs.pseg_cap);
+            const uint32_t p = seg_entry_from(pre, s0, i, qs.pseg_cap);
             r.id = cur.id[p];
             const float4 pw = cur.pos[p], de = cur.dir[p];
             r.wl = pw.w;

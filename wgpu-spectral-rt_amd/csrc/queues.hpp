@@ -129,6 +129,22 @@ __device__ __forceinline__ uint32_t seg_entry(const uint32_t* pre, uint32_t ne, 
     return lo * cap + (i - pre[lo]);
 }
 
+// Segment holding item base of a queue of kSegs segments (pre: kSegs + 1
+// prefix entries): lane l reads pre[l], and the segments starting at or
+// before base form a prefix of the wave's ballot. All 64 lanes must call it
+// with the same base.
+__device__ __forceinline__ uint32_t seg_of_wave_base(const uint32_t* pre, uint32_t base) {
+    const uint64_t m = __ballot(pre[__lane_id()] <= base);
+    return (uint32_t)__popcll(m) - 1u;
+}
+// Entry of item i (base <= i < total) from the segment s0 of base: steps
+// over the segments that end at or before i, usually none or one.
+__device__ __forceinline__ uint32_t seg_entry_from(const uint32_t* pre, uint32_t s0, uint32_t i, uint32_t cap) {
+    uint32_t s = s0;
+    while (s + 1u < (uint32_t)kSegs && pre[s + 1u] <= i) ++s;
+    return s * cap + (i - pre[s]);
+}
+
 // Block sum of one value per thread (blockDim a multiple of 64, <= 1024),
 // valid in thread 0.
 __device__ __forceinline__ unsigned long long block_sum(unsigned long long v, unsigned long long* lds) {
