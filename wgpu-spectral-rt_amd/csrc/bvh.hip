@@ -8,9 +8,10 @@
 //  3. stable radix sort of (code, triangle);
 //  4. binary radix tree (Karras 2012), duplicates split by position;
 //  5. bottom-up boxes and triangle counts (one atomic flag per node);
-//  6. top-down collapse, one level per launch: each BVH8 node opens the
-//     largest children of its binary node until it has 8, subtrees of <= 3
-//     triangles become leaf children, the others new BVH8 nodes; child boxes
+//  6. top-down collapse, one level per launch: each BVH8 node opens its
+//     child of largest surface area (whatever its triangle count) until it
+//     has 8, subtrees of <= 3 triangles become leaf children, the others new
+//     BVH8 nodes; child boxes
 //     are quantized to 8 bits per axis against a power-of-two grid, rounded
 //     outward in f64 so every dequantized box contains its child.
 #include <math.h>
@@ -288,7 +289,7 @@ __global__ void k_collapse(CollapseArgs A, const uint4* __restrict__ qin, uint32
             float best_s = -1.0f;
             for (int k = 0; k < m; ++k) {
                 if (ent[k] & 0x80000000u) continue;
-                float s = half_area(eb[k]) + (ec[k] > 3u ? 1e30f : 0.0f);
+                float s = half_area(eb[k]);
                 if (s > best_s) { best_s = s; best = k; }
             }
             if (best < 0) break;
