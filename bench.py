@@ -259,6 +259,34 @@ def mlp_rates(r, frames: int, train_samples: int) -> dict:
     return out
 
 
+def guide_train_steps(n: int, samples: int = 16384) -> int:
+    """n Adam steps of the standalone guide network on `samples` seeded
+    records (scalars in [0, 1), unit directions, weights in [0, 1)), through
+    wsrt_guide_train_step: the same four kernels as one training step of the
+    renderer. Prints the mean time per step (host buffers uploaded each
+    step, so this is an upper bound; a kernel trace gives the kernels'
+    own times)."""
+    import numpy as np
+    import wsrt
+
+    dev = wsrt.Device(0)
+    g = wsrt.Guide(dev, samples, 7)
+    rng = np.random.default_rng(0)
+    x = rng.random((samples, 7), dtype=np.float32)
+    d = rng.normal(size=(samples, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    w = rng.random(samples, dtype=np.float32)
+    for _ in range(5):
+        g.train_step(x, d, w)
+    t = time.perf_counter()
+    for _ in range(n):
+        g.train_step(x, d, w)
+    dt = (time.perf_counter() - t) / n
+    print(json.dumps({"guide_train_steps": n, "samples_per_step": samples,
+                      "ms_per_step_with_upload": round(dt * 1e3, 4)}))
+    return 0
+
+
 def heightfield_throughput(frames: int = 16) -> dict:
     """Film-coated GGX heightfield (2,093,058 triangles) under the seeded sky,
     1920 x 1080, depth 6, `frames` samples per pixel, plain tracer."""
@@ -365,6 +393,10 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (the reference point that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiled run) and exit")
+    ap.add_argument("--guide-train-steps", type=int, default=0, metavar="N",
+                    help="only run N Adam steps of the guide network over 16384 synthetic samples "
+                         "(the renderer's step size), for a kernel profile of the training kernels, "
+                         "and exit")
     ap.add_argument("--summarize-traffic", nargs=3, metavar=("PMC_DIR", "KERNEL", "COMMAND"),
                     help="write the per-launch HBM traffic summary of KERNEL from rocprofv3 --pmc "
                          "output and exit")
@@ -373,6 +405,8 @@ def main() -> int:
         return summarize_traffic(*args.summarize_traffic)
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
+    if args.guide_train_steps:
+        return guide_train_steps(args.guide_train_steps)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the render loop runs "
               "on a single MI355X; multi-GPU sharding is a design note only)", file=sys.stderr)
