@@ -317,11 +317,17 @@ typedef struct wsrt_frame_stats {
     uint64_t kernel_nodes[6];
     uint64_t kernel_tris[6];
     uint64_t kernel_counted_rays[6];
+    /* The wavefront extend kernel's node visits in lockstep, while counting
+     * is enabled: per wave and round of rays, the most visits of any lane
+     * times the lanes that traversed, summed. Its ratio to kernel_nodes[1]
+     * (>= 1) is the visits a wave spends waiting on its longest ray. */
+    uint64_t extend_wave_nodes;
 } wsrt_frame_stats;
 int wsrt_renderer_get_stats(wsrt_renderer* r, wsrt_frame_stats* out);
 int wsrt_renderer_reset_stats(wsrt_renderer* r);
 /* Count BVH8 node visits and triangle tests of every ray (adds one atomic
- * per wave per kernel; off by default). */
+ * per wave per kernel, and one per wave and round of rays in the extend
+ * kernel; off by default). */
 int wsrt_renderer_set_traversal_counting(wsrt_renderer* r, int enable);
 
 /* Per-kernel device time, measured with HIP events recorded on the

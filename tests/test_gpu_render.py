@@ -200,6 +200,24 @@ def test_frame_count_semantics(device, cornell_pair):
     assert r.frame_count == 3
 
 
+def test_traversal_counting(device, cornell_pair):
+    """Counting node visits leaves the image unchanged; the extend kernel's
+    lockstep visits lie between its summed visits and 64 times them."""
+    g, _ = cornell_pair
+    cfg = wsrt.default_config(photons_per_pass=1 << 12)
+    plain = wsrt.Renderer(device, g, 32, 32, cfg)
+    counted = wsrt.Renderer(device, g, 32, 32, cfg)
+    counted.set_traversal_counting(True)
+    plain.render_frames(2)
+    counted.render_frames(2)
+    np.testing.assert_array_equal(plain.read_accum(), counted.read_accum())
+    assert plain.stats()["extend_wave_nodes"] == 0
+    s = counted.stats()
+    nodes = s["kernel_nodes"][1]
+    assert s["kernel_counted_rays"][1] == s["kernel_rays"][1] > 0
+    assert 0 < nodes <= s["extend_wave_nodes"] <= 64 * nodes
+
+
 def test_zero_light_scene_renders_black(device):
     b = wsrt.SceneBuilder()
     m = b.add_material([0.5, 0.5, 0.5, 1], [0, 0, 0, 0])

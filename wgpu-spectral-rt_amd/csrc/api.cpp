@@ -1606,6 +1606,7 @@ extern "C" int wsrt_renderer_get_stats(wsrt_renderer* r, wsrt_frame_stats* out) 
         out->kernel_tris[k] = c.trav_tris[k];
         out->kernel_counted_rays[k] = c.trav_rays[k];
     }
+    out->extend_wave_nodes = c.extend_wave_nodes;
     return WSRT_OK;
 }
 

@@ -85,6 +85,9 @@ struct Counters {
     unsigned long long trav_nodes[6];
     unsigned long long trav_tris[6];
     unsigned long long trav_rays[6];
+    // wavefront extend, while counting: per wave and round, the most node
+    // visits of any lane times the lanes that traversed, summed
+    unsigned long long extend_wave_nodes;
     unsigned long long photon_rays_part[kStatBlocks];
     unsigned long long photons_stored_part[kStatBlocks];
 };

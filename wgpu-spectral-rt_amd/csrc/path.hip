@@ -99,12 +99,14 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend
         if (base >= n) continue;
         const uint32_t i = base + __lane_id();
         bool have = i < n;
-        uint32_t p = 0, bin = 0;
+        uint32_t p = 0, bin = 0, ray_nodes = 0, ray_tris = 0;
         const uint32_t s0 = bounce == 0 ? 0u : seg_of_wave_base(pre, base);  // whole wave
         if (have) {
             p = bounce == 0 ? i : queue[seg_entry_from(pre, s0, i, qs.seg_cap)];
             const float4 o = ps.o[p], d = ps.d[p];
-            Hit h = trace_ray<false>(sv, xyz(o), xyz(d), 0.0001f, 100.0f, st, count, my_nodes, my_tris);
+            Hit h = trace_ray<false>(sv, xyz(o), xyz(d), 0.0001f, 100.0f, st, count, ray_nodes, ray_tris);
+            my_nodes += ray_nodes;
+            my_tris += ray_tris;
             my_rays++;
             if (h.tri == kInvalid) {
                 have = false;  // miss: the path ends (shader.wgsl:566)
@@ -116,6 +118,10 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend
                 uint32_t type = (uint32_t)sv.materials[mat].extra[0];
                 bin = type < (uint32_t)kMatBins ? type : 0u;
             }
+        }
+        if (count) {
+            const uint32_t w = wave_lockstep_nodes(ray_nodes, i < n);
+            if (__lane_id() == 0 && w) atomicAdd(&ctr->extend_wave_nodes, (unsigned long long)w);
         }
         // sort-by-material: one segmented queue per material type
         uint64_t pending = __ballot(have);

@@ -288,6 +288,15 @@ __device__ __forceinline__ Hit trace_ray(const SceneView& sv, f3 o, f3 d, float 
     return traverse<kAny, false>(sv, o, d, tmin, tmax, st);
 }
 
+// The node visits a wave spends in lockstep on one round of rays: the most
+// visits of any lane times the lanes that traversed. Every lane of the wave
+// calls it, in wave-uniform control flow.
+__device__ __forceinline__ uint32_t wave_lockstep_nodes(uint32_t nodes, bool traced) {
+    uint32_t m = traced ? nodes : 0u;
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor(m, off, 64));
+    return m * (uint32_t)__popcll(__ballot(traced));
+}
+
 // Adds per-lane traversal totals of a wave to the frame counters.
 __device__ __forceinline__ void flush_traversal_counts(Counters* ctr, int kind, uint32_t rays,
                                                        uint32_t nodes, uint32_t tris) {

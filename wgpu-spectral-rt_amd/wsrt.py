@@ -74,7 +74,7 @@ class FrameStats(C.Structure):
                                            "guide_samples")] + \
         [("trav_nodes", C.c_uint64 * 3), ("trav_tris", C.c_uint64 * 3), ("trav_rays", C.c_uint64 * 3),
          ("kernel_rays", C.c_uint64 * 6), ("kernel_nodes", C.c_uint64 * 6), ("kernel_tris", C.c_uint64 * 6),
-         ("kernel_counted_rays", C.c_uint64 * 6)]
+         ("kernel_counted_rays", C.c_uint64 * 6), ("extend_wave_nodes", C.c_uint64)]
 
 
 class KernelStat(C.Structure):
@@ -582,6 +582,7 @@ class Renderer:
         for k in ("trav_nodes", "trav_tris", "trav_rays", "kernel_rays", "kernel_nodes", "kernel_tris",
                   "kernel_counted_rays"):
             out[k] = list(getattr(s, k))
+        out["extend_wave_nodes"] = s.extend_wave_nodes
         return out
 
     def set_traversal_counting(self, on: bool):
