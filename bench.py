@@ -4,16 +4,18 @@ Workload (the caustic photon-mapping configuration): glass icosphere
 (subdivision 4, 5120 triangles, radius 0.5, Cauchy A = 1.5, B = 0.0042 um^2)
 on a 2-triangle diffuse plane under a point light 2 units above the sphere;
 512 x 512 pixels, 2^20 photon paths per frame, 2^22 grid cells, cell size =
-gather radius = 0.01, max depth 12. One step is one frame of the render
-loop: photon pass, grid build, one camera path per pixel (1 spp) and
-accumulation; the default --steps 256 times one batch of the frames in flight
-the autotuner picks (256 on this workload), i.e. the 256 spp image; frames in
-flight never exceed the timed steps, so the timed region is whole batches.
+gather radius = 0.01, max depth 12. One step is the workload's image at 64
+spp: 64 frames of the render loop (each a photon pass, grid build, one
+camera path per pixel and accumulation). The timed steps run at the
+frames in flight the autotuner picks, every batch one replay of its
+captured HIP graph, exactly as the library renders; a batch that does not
+fill frames_in_flight runs as a smaller batch with its own graph.
 
 Prints one JSON line: Msamples/s (samples = camera paths) with rays/s and
-photons/s beside it, the roofline of the dominant kernel measured with HIP
-events on the renderer's stream during the timed region, and the CPU oracle
-timed on a bounded sample of the same workload on the host cores.
+photons/s beside it, the roofline of the dominant kernel (its device time
+from HIP events on the renderer's stream over one batch at the same
+setting), and the CPU oracle timed on a bounded sample of the same workload
+on the host cores.
 """
 from __future__ import annotations
 
@@ -27,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "wgpu-spectral-rt_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+STEP_FRAMES = 64  # one step: the caustic workload's 512 x 512 image at 64 spp, one frame per spp
 NODE_BYTES, TRI_BYTES, RAY_BYTES = 80, 48, 48  # BVH8 node, leaf triangle, ray in + hit out
 
 
@@ -127,6 +130,108 @@ def hbm_traffic(kernel: str):
             k, v = line.split("=", 1)
             vals[k.strip()] = float(v)
     return vals.get("traffic_bytes_per_launch")
+
+
+ISSUE_FILE = os.path.join(ROOT, "profiles", "issue_{}_caustic512.txt")
+ISSUE_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_SMEM", "SQ_INSTS_VMEM", "SQ_INSTS_LDS")
+
+
+def summarize_issue(pmc_dir: str, kernel: str, command: str) -> int:
+    """Writes the instructions issued per launch of one kernel (wave-level
+    counts, summed over the chip) from one rocprofv3 --pmc run of the SQ
+    instruction counters in pmc_dir (issue_counter_collection.csv)."""
+    sym = KERNEL_SYMBOLS[kernel]
+    path = os.path.join(pmc_dir, "issue_counter_collection.csv")
+    with open(ISSUE_FILE.format(kernel), "w") as f:
+        f.write(f"# instructions issued per launch of {sym} (caustic photon mapping, 512x512), "
+                "wave-level counts summed over the chip\n")
+        f.write(f"# command: {command}\n")
+        for c in ISSUE_COUNTERS:
+            per = _per_dispatch(path, c)
+            if sym in per:
+                f.write(f"{c.lower()}_per_launch = {per[sym]:.1f}\n")
+    print(open(ISSUE_FILE.format(kernel)).read())
+    return 0
+
+
+def issue_rate(kernel: str, cnt: dict, frames_in_flight: int):
+    """VALU and SALU instructions per BVH8 node visit of `kernel`: the
+    committed per-launch instruction counts (rocprofv3 SQ counters at this
+    workload and frames_in_flight) times the launches of one batch, over the
+    node visits of one batch from the in-kernel counters. A wave instruction
+    serves 64 lanes, so 64 x VALU per lane-visit is the issue cost of one
+    useful node visit; lockstep idling inflates it."""
+    path = ISSUE_FILE.format(kernel)
+    idx = {"extend": 1, "shadow": 2, "photon_trace": 0}.get(kernel)
+    if not os.path.exists(path) or idx is None:
+        return None
+    vals = {}
+    for line in open(path):
+        if "=" in line and not line.startswith("#"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = float(v)
+    launches = {"extend": 12, "shadow": 12, "photon_trace": 5}[kernel]
+    nodes = cnt["kernel_nodes"][idx]
+    if not nodes or "sq_insts_valu_per_launch" not in vals:
+        return None
+    out = {"source": os.path.relpath(path, ROOT), "launches_per_batch": launches,
+           "node_visits_per_batch": nodes, "frames_in_flight": frames_in_flight}
+    for c in ("valu", "salu", "vmem", "lds"):
+        k = f"sq_insts_{c}_per_launch"
+        if k in vals:
+            out[f"{c}_wave_insts_per_lane_node_visit"] = round(vals[k] * launches * 64 / nodes, 1)
+    if idx == 1 and cnt.get("extend_wave_nodes"):
+        out["lockstep_ratio"] = round(cnt["extend_wave_nodes"] / nodes, 3)
+    return out
+
+
+def summarize_kernels(trace_dir: str, first_batch: int, n_batches: int, frames_per_batch: int, command: str) -> int:
+    """Writes the rocprofv3 kernel statistics of a bench run (kernel_stats.csv)
+    and the per-kernel times of batches [first_batch, first_batch + n_batches)
+    of the renderer (kernel_trace.csv; a batch runs from one k_clear_queues
+    to the next), with their span from the first kernel's start to the last
+    kernel's end."""
+    import csv
+    import collections
+
+    stats = os.path.join(trace_dir, "bench_kernel_stats.csv")
+    trace = os.path.join(trace_dir, "bench_kernel_trace.csv")
+    out = os.path.join(ROOT, "profiles", "kernels_caustic512.txt")
+    rows = []
+    with open(trace) as f:
+        for row in csv.DictReader(f):
+            rows.append((int(row["Start_Timestamp"]), int(row["End_Timestamp"]), _short(row["Kernel_Name"])))
+    rows.sort()
+    starts = [i for i, r in enumerate(rows) if r[2] == "k_clear_queues"]
+    lo = starts[first_batch]
+    hi = starts[first_batch + n_batches] if first_batch + n_batches < len(starts) else len(rows)
+    sel = rows[lo:hi]
+    per = collections.defaultdict(list)
+    for a, b, k in sel:
+        per[k].append((b - a) / 1000.0)
+    span = (max(b for _, b, _ in sel) - sel[0][0]) / 1000.0
+    with open(out, "w") as f:
+        f.write("# rocprofv3 kernel statistics, caustic photon mapping 512x512 (glass icosphere 5120 tris, "
+                "2^20 photons/frame, 2^22 cells)\n")
+        f.write(f"# command: {command}\n")
+        with open(stats) as g:
+            rd = list(csv.DictReader(g))
+        f.write(f"{'kernel':34s} {'calls':>7s} {'total_us':>11s} {'avg_us':>9s} {'pct':>6s}\n")
+        for row in rd:
+            f.write(f"{_short(row['Name'])[:34]:34s} {int(row['Calls']):7d} {float(row['TotalDurationNs']) / 1e3:11.1f} "
+                    f"{float(row['AverageNs']) / 1e3:9.2f} {float(row['Percentage']):6.2f}\n")
+        f.write(f"\n# batches {first_batch}..{first_batch + n_batches - 1} of the run (the timed region), "
+                "per kernel:\n")
+        for k, ts in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+            f.write(f"#   {k[:28]:28s} launches {len(ts):4d}  total {sum(ts):10.1f} us  "
+                    f"avg {sum(ts) / len(ts):9.2f} us\n")
+        busy = sum(sum(ts) for ts in per.values())
+        frames = n_batches * frames_per_batch
+        f.write(f"#   span of the batches {span:.1f} us ({span / frames:.2f} us per frame under the tracer)\n")
+        f.write(f"#   kernel time of the batches {busy:.1f} us ({busy / frames:.2f} us per frame, "
+                f"{frames} frames)\n")
+    print(open(out).read())
+    return 0
 
 
 def copy_calibration(mib: int) -> int:
@@ -396,8 +501,8 @@ def batch_throughput(n_scenes: int = 64) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=256)
-    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=8, help="timed 64-spp images")
+    ap.add_argument("--warmup", type=int, default=4, help="untimed 64-spp images")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-guided", action="store_true", help="skip the guided-vs-plain time-to-relMSE leg")
     ap.add_argument("--no-other", action="store_true", help="skip the heightfield and batch throughput legs")
@@ -417,9 +522,21 @@ def main() -> int:
     ap.add_argument("--summarize-traffic", nargs=3, metavar=("PMC_DIR", "KERNEL", "COMMAND"),
                     help="write the per-launch HBM traffic summary of KERNEL from rocprofv3 --pmc "
                          "output and exit")
+    ap.add_argument("--summarize-issue", nargs=3, metavar=("PMC_DIR", "KERNEL", "COMMAND"),
+                    help="write the per-launch instruction counts of KERNEL from a rocprofv3 --pmc run "
+                         "of the SQ instruction counters and exit")
+    ap.add_argument("--summarize-kernels", nargs=5,
+                    metavar=("TRACE_DIR", "FIRST_BATCH", "N_BATCHES", "FRAMES_PER_BATCH", "COMMAND"),
+                    help="write the kernel statistics and the timed batches' per-kernel times from a "
+                         "rocprofv3 --kernel-trace --stats run and exit")
     args = ap.parse_args()
     if args.summarize_traffic:
         return summarize_traffic(*args.summarize_traffic)
+    if args.summarize_issue:
+        return summarize_issue(*args.summarize_issue)
+    if args.summarize_kernels:
+        d, a, n, fpb, c = args.summarize_kernels
+        return summarize_kernels(d, int(a), int(n), int(fpb), c)
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
     if args.guide_train_steps:
@@ -446,53 +563,55 @@ def main() -> int:
         cfg.frames_in_flight = args.frames_in_flight
     else:
         # frames per wavefront picked by the library's autotuner (results do
-        # not depend on it; the timed frames are the same)
+        # not depend on it); the timed frames run at exactly that setting
         tune = wsrt.autotune(dev, scene, v.width, v.height, cfg, scenes.camera(v, 0))
         log(f"autotune: {tune}")
-    # the timed steps are whole batches
-    while cfg.frames_in_flight > 1 and args.steps % cfg.frames_in_flight:
-        cfg.frames_in_flight //= 2
+    S = cfg.frames_in_flight
+    frames = args.steps * STEP_FRAMES
     r = wsrt.Renderer(dev, scene, v.width, v.height, cfg)
     r.set_camera(scenes.camera(v, 0))
-    r.render_frames(args.warmup)
+    # warm-up: at least one full batch, which captures its graph, and the
+    # batch size the timed frames end with, so no capture falls in the
+    # timed region
+    r.render_frames(max(args.warmup * STEP_FRAMES, S))
+    if frames % S:
+        r.render_frames(frames % S)
     r.sync()
-    # timed region: exactly `steps` frames, every kernel (group) bracketed by
-    # HIP events recorded on the renderer's stream, which is the stream the
-    # kernels run on (profiling mode launches the batches eagerly)
+    r.reset_stats()
+    # timed region: exactly `steps` images of STEP_FRAMES frames, each batch
+    # of frames_in_flight frames one replay of its captured HIP graph, as the
+    # library ships
+    t0 = time.perf_counter()
+    r.render_frames(frames)
+    r.sync()
+    dt = time.perf_counter() - t0
+    st = r.stats()
+    # per-kernel device time: one batch of the same frames_in_flight, each
+    # kernel (group) bracketed by HIP events on the renderer's stream, the
+    # stream the kernels run on (profiling launches the batch eagerly)
     r.reset_stats()
     r.reset_profile()
     r.set_profiling(True)
+    r.render_frames(S)
     r.sync()
-    t0 = time.perf_counter()
-    r.render_frames(args.steps)
-    r.sync()
-    dt = time.perf_counter() - t0
     r.set_profiling(False)
-    st = r.stats()
+    pst = r.stats()
     prof = r.profile()
-    # the same number of frames replayed as captured batch graphs (no events);
-    # one untimed batch first captures the graphs
-    r.render_frames(cfg.frames_in_flight)
-    r.sync()
-    t1 = time.perf_counter()
-    r.render_frames(args.steps)
-    r.sync()
-    dt_graph = time.perf_counter() - t1
     # traversal work per ray of each launching kernel from the in-kernel
     # counters (pinned against the oracle's traversal of the same BVH8),
-    # measured on one extra batch
+    # measured on one more batch
     r.reset_stats()
     r.set_traversal_counting(True)
-    r.render_frames(cfg.frames_in_flight)
+    r.render_frames(S)
     r.sync()
     cnt = r.stats()
     r.set_traversal_counting(False)
 
-    samples = v.width * v.height * args.steps
+    samples = v.width * v.height * frames
     msps = samples / dt / 1e6
-    rays = st["extend_rays"] + st["shadow_rays"] + st["photon_rays"]
+    rays = st["extend_rays"] + st["shadow_rays"] + st["kernel_rays"][0]
     # the launching kernels' counter indices (wsrt_frame_stats.kernel_*)
-    kinds = {"photon_trace": [0], "extend": [1], "shadow": [2], "photon_emit": [3], "tail": [4, 5]}
+    kinds = {"photon_trace": [0], "extend": [1], "shadow": [2], "tail": [4, 5]}
     per_ray = {}
     ray_counts, ray_bytes = {}, {}
     for k, idx in kinds.items():
@@ -500,26 +619,50 @@ def main() -> int:
         nodes = sum(cnt["kernel_nodes"][i] for i in idx)
         tris = sum(cnt["kernel_tris"][i] for i in idx)
         per_ray[k] = {"nodes": nodes / max(counted, 1), "tris": tris / max(counted, 1)}
-        ray_counts[k] = sum(st["kernel_rays"][i] for i in idx)
-        # bytes of the timed rays: each counter's rays at its own work per ray
-        ray_bytes[k] = sum(st["kernel_rays"][i] * (RAY_BYTES + (cnt["kernel_nodes"][i] * NODE_BYTES +
-                                                                cnt["kernel_tris"][i] * TRI_BYTES) /
-                                                   max(cnt["kernel_counted_rays"][i], 1)) for i in idx)
+        ray_counts[k] = sum(pst["kernel_rays"][i] for i in idx)
+        # bytes of the profiled batch's rays: each counter's rays at its own
+        # work per ray
+        ray_bytes[k] = sum(pst["kernel_rays"][i] * (RAY_BYTES + (cnt["kernel_nodes"][i] * NODE_BYTES +
+                                                                 cnt["kernel_tris"][i] * TRI_BYTES) /
+                                                    max(cnt["kernel_counted_rays"][i], 1)) for i in idx)
+    # emission: one stored-bit word per 32 paths, and for each path that
+    # reaches the slab test (the cone rejects the others with one dot
+    # product on registers and LDS) the root node read and its 44-byte
+    # queued state
+    n_emit = pst["kernel_rays"][3]
+    cand = cnt["kernel_nodes"][3] / max(cnt["kernel_counted_rays"][3], 1)
+    per_ray["photon_emit"] = {"slab_tested_fraction": cand}
+    emit_bytes = n_emit * (4.0 / 32 + cand * (NODE_BYTES + 44))
+    stored_batch = pst["photons_stored"]
     kern = {}
+    over_peak = []
     for name, p in prof.items():
         if p["launches"] == 0:
             continue
         e = {"ms": round(p["total_ms"], 3), "launches": p["launches"],
              "avg_us": round(1000 * p["total_ms"] / p["launches"], 2)}
-        if name in per_ray and ray_counts[name]:
+        gbs = None
+        if name in per_ray and name in ray_counts and ray_counts[name]:
             e["rays"] = ray_counts[name]
             e["bytes_per_ray"] = round(ray_bytes[name] / ray_counts[name], 1)
-            e["GBps"] = round(ray_bytes[name] / (p["total_ms"] * 1e-3) / 1e9, 1)
-        if name == "grid_build":  # 60 N + 12 C bytes per frame, launches cover frames_in_flight frames
-            n, c = cfg.photons_per_pass, cfg.hash_size
-            e["GBps"] = round((60 * n + 12 * c) * args.steps / (p["total_ms"] * 1e-3) / 1e9, 1)
+            gbs = ray_bytes[name] / (p["total_ms"] * 1e-3) / 1e9
+        elif name == "photon_emit":
+            e["paths"] = n_emit
+            gbs = emit_bytes / (p["total_ms"] * 1e-3) / 1e9
+        elif name == "grid_build":
+            # the stored-bit scan (one bit per photon slot), then per stored
+            # photon the 60-byte hash, histogram, scan and scatter work and
+            # 12 bytes per touched cell (at most one per stored photon)
+            e["photons_stored"] = stored_batch
+            gbs = (cfg.photons_per_pass * S / 8 + 72 * stored_batch) / (p["total_ms"] * 1e-3) / 1e9
+        if gbs is not None:
+            e["GBps"] = round(gbs, 1)
+            if gbs > HBM_PEAK_GBS:
+                over_peak.append(name)
         kern[name] = e
-    dom = max((k for k in kern if "GBps" in kern[k] and k in per_ray), key=lambda k: kern[k]["ms"])
+    if over_peak:
+        raise AssertionError(f"byte model of {over_peak} exceeds the HBM peak: fix the model")
+    dom = max((k for k in kern if "rays" in kern[k]), key=lambda k: kern[k]["ms"])
     d = kern[dom]
     achieved = d["GBps"]
     result = {
@@ -537,18 +680,22 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": "caustic photon mapping: glass icosphere(4) 5120 tris + 2-tri plane + point "
-                        "light, 512x512, 1 spp per step, 2^20 photons/pass, 2^22 cells, cell=radius=0.01, "
+                        f"light, 512x512, {STEP_FRAMES} spp per step (one frame per spp: photon pass of "
+                        "2^20 paths, 2^22-cell grid, one camera path per pixel), cell=radius=0.01, "
                         "max depth 12",
             "triangles": info.n_triangles, "bvh8_nodes": info.n_nodes,
-            "width": v.width, "height": v.height, "photons_per_pass": cfg.photons_per_pass,
+            "width": v.width, "height": v.height, "spp_per_step": STEP_FRAMES,
+            "photons_per_pass": cfg.photons_per_pass,
             "hash_size": cfg.hash_size, "max_depth": cfg.max_depth,
-            "frames_in_flight": cfg.frames_in_flight,
+            "frames_in_flight": S, "timed_frames": frames,
+            "launch": "one captured HIP graph per batch of frames_in_flight frames",
         },
         "autotune": tune,
-        "graph_replay_ms_per_step": round(1000 * dt_graph / args.steps, 4),
+        "ms_per_frame": round(1000 * dt / frames, 4),
         "rays_per_s": round(rays / dt, 1),
-        "photons_per_s": round(st["photon_paths"] / dt, 1),
-        "photons_stored_per_frame": round(st["photons_stored"] / args.steps, 1),
+        "photon_paths_per_s": round(st["photon_paths"] / dt, 1),
+        "photon_rays_traced_per_s": round(st["kernel_rays"][0] / dt, 1),
+        "photons_stored_per_frame": round(st["photons_stored"] / frames, 1),
         "roofline": {
             "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": hbm_traffic(dom),
@@ -556,7 +703,9 @@ def main() -> int:
             "algorithmic_bytes_per_launch": round(d["bytes_per_ray"] * ray_counts[dom] / d["launches"]),
             "algorithmic_bytes_per_ray": d["bytes_per_ray"],
             "avg_launch_us": d["avg_us"],
+            "issue": issue_rate(dom, cnt, S),
         },
+        "event_timed_batch_ms": round(sum(p["total_ms"] for p in prof.values()), 3),
         "kernels": kern,
         "traversal_per_ray": per_ray,
     }

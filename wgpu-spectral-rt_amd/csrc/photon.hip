@@ -205,7 +205,8 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon
                 }
             }
             if (nl != 0u) {
-                if (count && spec.n_nodes) my_nodes++;
+                // the root node is read only by the slab test of a candidate
+                if (count && c) my_nodes++;
                 my_rays++;
             }
         }
