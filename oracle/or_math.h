@@ -18,6 +18,11 @@
 #define OR_LAMBDA_MAX 780.0f
 #define OR_LAMBDA_RANGE 400.0f
 
+/* Storage type of the f32 fields of the ABI structs (wsrt.h). The
+ * double-precision build of the oracle rewrites every other float to
+ * double and keeps the lines marked KEEP_F32. */
+typedef float or_f32; /* KEEP_F32 */
+
 typedef struct { float x, y, z; } v3;
 typedef struct { float x, y, z, w; } v4;
 

@@ -15,9 +15,11 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_LIB64_PATH = os.path.join(_HERE, "liboracle64.so")
 
 u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
-f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+f32p = _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 
 
@@ -64,36 +66,47 @@ LIGHT_DTYPE = np.dtype([("v0", np.float32, 4), ("v1", np.float32, 4), ("v2", np.
                         ("params", np.float32, 4)])
 
 
-def _load():
-    if not os.path.exists(_LIB_PATH):
+def _load(fp64: bool = False):
+    """liboracle.so (f32), or with fp64 liboracle64.so: the same source built
+    with double arithmetic, whose float arrays and scalars are double."""
+    path = _LIB64_PATH if fp64 else _LIB_PATH
+    if not os.path.exists(path):
         build()
-    lib = C.CDLL(_LIB_PATH)
+    lib = C.CDLL(path)
     vp = C.c_void_p
+    f32p = f64p if fp64 else _f32p
+    C_float = C.c_double if fp64 else C.c_float
+
+    class _C:  # ctypes with c_float replaced where the build uses double
+        c_float = C_float
+        c_uint32 = C.c_uint32
+        c_int = C.c_int
+        POINTER = staticmethod(C.POINTER)
     sig = {
         "or_pcg_hash": (C.c_uint32, [C.c_uint32]),
         "or_rand_stream": (None, [C.c_uint32, C.c_uint32, f32p]),
-        "or_cie_xyz_f": (None, [C.c_float, f32p]),
-        "or_rgb_to_spectrum_f": (C.c_float, [f32p, C.c_float]),
-        "or_blackbody_f": (C.c_float, [C.c_float, C.c_float]),
-        "or_thin_film_f": (C.c_float, [C.c_float] * 6),
-        "or_schlick_f": (C.c_float, [C.c_float, C.c_float]),
-        "or_cauchy_constants": (None, [C.POINTER(C.c_float)] * 3),
-        "or_cauchy_sample_f": (C.c_float, [C.c_float]),
-        "or_grid_jitter_f": (None, [C.c_uint32, C.c_float, f32p]),
-        "or_hash_position_f": (C.c_uint32, [f32p, C.c_uint32, C.c_float, C.c_uint32]),
-        "or_hero_wavelengths": (None, [C.c_float, C.c_float, f32p]),
+        "or_cie_xyz_f": (None, [_C.c_float, f32p]),
+        "or_rgb_to_spectrum_f": (_C.c_float, [f32p, _C.c_float]),
+        "or_blackbody_f": (_C.c_float, [_C.c_float, _C.c_float]),
+        "or_thin_film_f": (_C.c_float, [_C.c_float] * 6),
+        "or_schlick_f": (_C.c_float, [_C.c_float, _C.c_float]),
+        "or_cauchy_constants": (None, [C.POINTER(_C.c_float)] * 3),
+        "or_cauchy_sample_f": (_C.c_float, [_C.c_float]),
+        "or_grid_jitter_f": (None, [C.c_uint32, _C.c_float, f32p]),
+        "or_hash_position_f": (C.c_uint32, [f32p, C.c_uint32, _C.c_float, C.c_uint32]),
+        "or_hero_wavelengths": (None, [_C.c_float, _C.c_float, f32p]),
         "or_geometry_plane": (C.c_int, [vp, vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
         "or_geometry_cube": (C.c_int, [vp, vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
         "or_geometry_sphere": (C.c_int, [C.c_uint32, vp, vp, C.POINTER(C.c_uint32),
                                          C.POINTER(C.c_uint32)]),
         "or_mat4_identity": (None, [f32p]),
         "or_mat4_mul": (None, [f32p, f32p, f32p]),
-        "or_mat4_translation": (None, [C.c_float] * 3 + [f32p]),
-        "or_mat4_scale": (None, [C.c_float] * 3 + [f32p]),
-        "or_mat4_rotation_x": (None, [C.c_float, f32p]),
-        "or_mat4_rotation_y": (None, [C.c_float, f32p]),
-        "or_mat4_rotation_z": (None, [C.c_float, f32p]),
-        "or_camera_uniform": (None, [f32p, C.c_float, C.c_float, C.c_float, C.c_uint32,
+        "or_mat4_translation": (None, [_C.c_float] * 3 + [f32p]),
+        "or_mat4_scale": (None, [_C.c_float] * 3 + [f32p]),
+        "or_mat4_rotation_x": (None, [_C.c_float, f32p]),
+        "or_mat4_rotation_y": (None, [_C.c_float, f32p]),
+        "or_mat4_rotation_z": (None, [_C.c_float, f32p]),
+        "or_camera_uniform": (None, [f32p, _C.c_float, _C.c_float, _C.c_float, C.c_uint32,
                                      C.POINTER(CameraUniform)]),
         "or_scene_create": (vp, []),
         "or_scene_destroy": (None, [vp]),
@@ -115,9 +128,9 @@ def _load():
         "or_morton_codes": (None, [vp, u32p]),
         "or_photon_trace": (C.c_uint32, [vp, C.POINTER(Config), C.c_uint32, C.c_uint32,
                                          C.c_uint32, vp]),
-        "or_grid_counting": (C.c_uint32, [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32,
+        "or_grid_counting": (C.c_uint32, [vp, C.c_uint32, C.c_uint32, _C.c_float, C.c_uint32,
                                           u32p, u32p, u32p, u32p]),
-        "or_grid_linked": (None, [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, u32p,
+        "or_grid_linked": (None, [vp, C.c_uint32, C.c_uint32, _C.c_float, C.c_uint32, u32p,
                                   u32p]),
         "or_frame_create": (vp, [vp, C.POINTER(Config), C.POINTER(CameraUniform), C.c_uint32,
                                  C.c_uint32]),
@@ -137,6 +150,15 @@ def _load():
 
 
 lib = _load()
+_lib64 = None
+
+
+def lib64():
+    """The double-precision build (loaded on first use)."""
+    global _lib64
+    if _lib64 is None:
+        _lib64 = _load(True)
+    return _lib64
 
 
 def _ptr(a):
@@ -194,91 +216,93 @@ def camera_uniform(pos=(0.0, 0.0, 3.0), yaw=None, pitch=0.0, aspect=1.0, frame=0
 class Scene:
     """Oracle scene mirroring SceneBuilder (builder.rs:47-266)."""
 
-    def __init__(self):
-        self.h = lib.or_scene_create()
+    def __init__(self, fp64: bool = False):
+        self.lib = lib64() if fp64 else lib
+        self.fdt = np.float64 if fp64 else np.float32
+        self.h = self.lib.or_scene_create()
         self.built = False
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib.or_scene_destroy(self.h)
+            self.lib.or_scene_destroy(self.h)
             self.h = None
 
     def add_material(self, color, extra) -> int:
         m = np.zeros(1, MATERIAL_DTYPE)
         m["color"] = color
         m["extra"] = extra
-        return lib.or_scene_add_material(self.h, _ptr(m))
+        return self.lib.or_scene_add_material(self.h, _ptr(m))
 
     def add_mesh(self, verts, idx) -> int:
         verts = np.ascontiguousarray(verts)
         idx = np.ascontiguousarray(idx, np.uint32)
-        return lib.or_scene_add_mesh(self.h, _ptr(verts), len(verts), idx, len(idx))
+        return self.lib.or_scene_add_mesh(self.h, _ptr(verts), len(verts), idx, len(idx))
 
     def add_instance(self, mesh, mat, m16):
-        lib.or_scene_add_instance(self.h, mesh, mat, np.ascontiguousarray(m16, np.float32))
+        self.lib.or_scene_add_instance(self.h, mesh, mat, np.ascontiguousarray(m16, self.fdt))
 
     def add_light_instance(self, mesh, mat, m16, params):
-        lib.or_scene_add_light_instance(self.h, mesh, mat, np.ascontiguousarray(m16, np.float32),
-                                        np.asarray(params, np.float32))
+        self.lib.or_scene_add_light_instance(self.h, mesh, mat, np.ascontiguousarray(m16, self.fdt),
+                                        np.asarray(params, self.fdt))
 
     def set_environment(self, rgb):
-        rgb = np.ascontiguousarray(rgb, np.float32)
+        rgb = np.ascontiguousarray(rgb, self.fdt)
         h, w = rgb.shape[:2]
-        lib.or_scene_set_environment(self.h, rgb, w, h)
+        self.lib.or_scene_set_environment(self.h, rgb, w, h)
 
     def add_point_light(self, pos, params):
-        lib.or_scene_add_point_light(self.h, np.asarray(pos, np.float32),
-                                     np.asarray(params, np.float32))
+        self.lib.or_scene_add_point_light(self.h, np.asarray(pos, self.fdt),
+                                     np.asarray(params, self.fdt))
 
     def build(self):
-        assert lib.or_scene_build(self.h) == 0
+        assert self.lib.or_scene_build(self.h) == 0
         self.built = True
         return self
 
     @property
     def n_triangles(self) -> int:
-        return lib.or_scene_num_triangles(self.h)
+        return self.lib.or_scene_num_triangles(self.h)
 
     def lights(self) -> np.ndarray:
-        out = np.zeros(lib.or_scene_num_lights(self.h), LIGHT_DTYPE)
-        lib.or_scene_get_lights(self.h, _ptr(out))
+        out = np.zeros(self.lib.or_scene_num_lights(self.h), LIGHT_DTYPE)
+        self.lib.or_scene_get_lights(self.h, _ptr(out))
         return out
 
     def world(self):
         n = self.n_triangles
-        tris = np.zeros((n, 9), np.float32)
+        tris = np.zeros((n, 9), self.fdt)
         ids = np.zeros((n, 3), np.uint32)
-        lib.or_scene_get_world(self.h, _ptr(tris), _ptr(ids))
+        self.lib.or_scene_get_world(self.h, _ptr(tris), _ptr(ids))
         return tris, ids
 
     def closest_hit(self, rays: np.ndarray, brute: bool = False):
-        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        rays = np.ascontiguousarray(rays, self.fdt).reshape(-1, 8)
         n = len(rays)
         hits = np.zeros((n, 4), np.uint32)
         flags = np.zeros(n, np.uint32)
-        lib.or_closest_hit(self.h, rays, n, int(brute), hits, flags)
+        self.lib.or_closest_hit(self.h, rays, n, int(brute), hits, flags)
         return hits, flags
 
     def any_hit(self, rays: np.ndarray, brute: bool = False):
-        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        rays = np.ascontiguousarray(rays, self.fdt).reshape(-1, 8)
         occ = np.zeros(len(rays), np.uint32)
-        lib.or_any_hit(self.h, rays, len(rays), int(brute), occ)
+        self.lib.or_any_hit(self.h, rays, len(rays), int(brute), occ)
         return occ
 
     def morton(self) -> np.ndarray:
         out = np.zeros(self.n_triangles, np.uint32)
-        lib.or_morton_codes(self.h, out)
+        self.lib.or_morton_codes(self.h, out)
         return out
 
     def photon_trace(self, cfg: Config, frame: int, first: int, n: int) -> np.ndarray:
         out = np.zeros(n, PHOTON_DTYPE)
-        lib.or_photon_trace(self.h, C.byref(cfg), frame, first, n, _ptr(out))
+        self.lib.or_photon_trace(self.h, C.byref(cfg), frame, first, n, _ptr(out))
         return out
 
 
-def cornell_box() -> Scene:
-    s = Scene()
-    lib.or_scene_cornell_box(s.h)
+def cornell_box(fp64: bool = False) -> Scene:
+    s = Scene(fp64)
+    s.lib.or_scene_cornell_box(s.h)
     return s.build()
 
 
@@ -320,42 +344,43 @@ class Frame:
     def __init__(self, scene: Scene, cfg: Config, cam: CameraUniform, width: int, height: int,
                  photons: np.ndarray | None = None):
         self.scene, self.W, self.H = scene, width, height
+        self.lib, self.fdt = scene.lib, scene.fdt
         if photons is None:
-            self.h = lib.or_frame_create(scene.h, C.byref(cfg), C.byref(cam), width, height)
+            self.h = self.lib.or_frame_create(scene.h, C.byref(cfg), C.byref(cam), width, height)
         else:
             photons = np.ascontiguousarray(photons, PHOTON_DTYPE)
-            self.h = lib.or_frame_create_with_photons(scene.h, C.byref(cfg), C.byref(cam), width,
+            self.h = self.lib.or_frame_create_with_photons(scene.h, C.byref(cfg), C.byref(cam), width,
                                                       height, _ptr(photons), len(photons))
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib.or_frame_destroy(self.h)
+            self.lib.or_frame_destroy(self.h)
             self.h = None
 
     @property
     def photons_stored(self) -> int:
-        return lib.or_frame_photons_stored(self.h)
+        return self.lib.or_frame_photons_stored(self.h)
 
     def trace_pixels(self, pixels):
         px = np.ascontiguousarray(pixels, np.uint32)
         n = len(px)
-        wl = np.zeros((n, 4), np.float32)
-        rad = np.zeros((n, 4), np.float32)
-        rgb = np.zeros((n, 3), np.float32)
-        lib.or_frame_trace_pixels(self.h, px, n, _ptr(wl), _ptr(rad), _ptr(rgb))
+        wl = np.zeros((n, 4), self.fdt)
+        rad = np.zeros((n, 4), self.fdt)
+        rgb = np.zeros((n, 3), self.fdt)
+        self.lib.or_frame_trace_pixels(self.h, px, n, _ptr(wl), _ptr(rad), _ptr(rgb))
         return wl, rad, rgb
 
     def render(self, accum: np.ndarray | None = None, want_rgba8: bool = False):
         if accum is None:
-            accum = np.zeros((self.H * self.W, 4), np.float32)
+            accum = np.zeros((self.H * self.W, 4), self.fdt)
         rgba = np.zeros((self.H * self.W, 4), np.uint8) if want_rgba8 else None
-        lib.or_frame_render(self.h, accum, _ptr(rgba))
+        self.lib.or_frame_render(self.h, accum, _ptr(rgba))
         return accum, rgba
 
     def gather(self, pos, n, wl):
-        out = np.zeros(4, np.float32)
-        lib.or_frame_gather(self.h, np.asarray(pos, np.float32), np.asarray(n, np.float32),
-                            np.asarray(wl, np.float32), out)
+        out = np.zeros(4, self.fdt)
+        self.lib.or_frame_gather(self.h, np.asarray(pos, self.fdt), np.asarray(n, self.fdt),
+                            np.asarray(wl, self.fdt), out)
         return out
 
 

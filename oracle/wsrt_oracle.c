@@ -180,8 +180,8 @@ int or_geometry_sphere(uint32_t subdiv, wsrt_vertex* v, uint32_t* idx, uint32_t*
                 int found;
                 uint32_t* slot = mp_get(&map, key, &found);
                 if (!found) { /* geometry.rs:201-232 */
-                    const float* a = v[p1].pos;
-                    const float* b = v[p2].pos;
+                    const or_f32* a = v[p1].pos;
+                    const or_f32* b = v[p2].pos;
                     float mx = (a[0] + b[0]) * 0.5f, my = (a[1] + b[1]) * 0.5f, mz = (a[2] + b[2]) * 0.5f;
                     sphere_push(v, &n, mx, my, mz);
                     *slot = n - 1;
@@ -251,7 +251,7 @@ void or_mat4_rotation_z(float a, float* m) {
 }
 
 /* General 4x4 inverse by cofactor expansion (f64), rounded to f32. */
-static void mat4_inverse(const float* mf, float* out) {
+static void mat4_inverse(const float* mf, or_f32* out) {
     double m[16], inv[16];
     for (int i = 0; i < 16; ++i) m[i] = mf[i];
     inv[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] + m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
@@ -295,8 +295,10 @@ void or_camera_uniform(const float* pos, float yaw, float pitch, float aspect, u
     float sf = sinf(0.5f * fov), cf = cosf(0.5f * fov);
     float h = cf / sf, w = h / aspect, r = 100.0f / (0.1f - 100.0f);
     float proj[16] = {w, 0, 0, 0, 0, h, 0, 0, 0, 0, r, -1, 0, 0, r * 0.1f, 0};
-    memcpy(out->view, view, sizeof view);
-    memcpy(out->proj, proj, sizeof proj);
+    for (int i = 0; i < 16; ++i) {
+        out->view[i] = view[i];
+        out->proj[i] = proj[i];
+    }
     mat4_inverse(view, out->view_inverse);
     mat4_inverse(proj, out->proj_inverse);
     out->frame_count = frame;
@@ -407,14 +409,14 @@ void or_scene_add_light_instance(or_scene* s, uint32_t mesh, uint32_t mat, const
     const wsrt_mesh_info* mi = &s->meshes[mesh];
     for (uint32_t i = 0; i + 2 < s->mesh_ni[mesh]; i += 3) {
         wsrt_light_info L;
-        float* dst[3] = {L.v0, L.v1, L.v2};
+        or_f32* dst[3] = {L.v0, L.v1, L.v2};
         for (int k = 0; k < 3; ++k) {
-            const float* p = s->verts[mi->vertex_offset + s->idx[mi->index_offset + i + k]].pos;
+            const or_f32* p = s->verts[mi->vertex_offset + s->idx[mi->index_offset + i + k]].pos;
             float v[4] = {p[0], p[1], p[2], 1.0f}, r[4];
             mat4_mul_vec4(m16, v, r);
             dst[k][0] = r[0]; dst[k][1] = r[1]; dst[k][2] = r[2]; dst[k][3] = 1.0f;
         }
-        memcpy(L.params, params, sizeof L.params);
+        for (int k = 0; k < 4; ++k) L.params[k] = params[k];
         GROW(s->lights, s->nlights, s->clights, 1);
         s->lights[s->nlights++] = L;
     }
@@ -423,7 +425,7 @@ void or_scene_add_point_light(or_scene* s, const float* pos, const float* params
     wsrt_light_info L;
     memset(&L, 0, sizeof L);
     L.v0[0] = pos[0]; L.v0[1] = pos[1]; L.v0[2] = pos[2]; L.v0[3] = 1.0f;
-    memcpy(L.params, params, sizeof L.params);
+    for (int k = 0; k < 4; ++k) L.params[k] = params[k];
     L.params[0] = 1.0f;
     GROW(s->lights, s->nlights, s->clights, 1);
     s->lights[s->nlights++] = L;
@@ -518,7 +520,7 @@ int or_scene_build(or_scene* s) {
         for (uint32_t p = 0; p < nt; ++p, ++t) {
             float w[3][4];
             for (int k = 0; k < 3; ++k) {
-                const float* pos = s->verts[mi->vertex_offset + s->idx[mi->index_offset + 3 * p + k]].pos;
+                const or_f32* pos = s->verts[mi->vertex_offset + s->idx[mi->index_offset + 3 * p + k]].pos;
                 float v[4] = {pos[0], pos[1], pos[2], 1.0f};
                 mat4_mul_vec4(in->m, v, w[k]);
             }
@@ -656,8 +658,8 @@ static int scene_any(const or_scene* s, v3 o, v3 d, float tmin, float tmax, int 
     return 0;
 }
 
-static inline uint32_t fbits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
-static inline float bitsf(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static inline uint32_t fbits(float f) { or_f32 g = (or_f32)f; uint32_t u; memcpy(&u, &g, 4); return u; }
+static inline float bitsf(uint32_t u) { or_f32 f; memcpy(&f, &u, 4); return f; }
 
 void or_closest_hit(const or_scene* s, const float* rays, uint32_t n, int brute, uint32_t* hits,
                     uint32_t* flags) {
@@ -821,9 +823,9 @@ static v3 hit_world_normal(const or_scene* s, uint32_t tri, float u, float v) {
     const or_instance* in = &s->inst[inst];
     const wsrt_mesh_info* mi = &s->meshes[in->mesh];
     uint32_t io = mi->index_offset + prim * 3u;
-    const float* n0 = s->verts[s->idx[io] + mi->vertex_offset].normal;
-    const float* n1 = s->verts[s->idx[io + 1] + mi->vertex_offset].normal;
-    const float* n2 = s->verts[s->idx[io + 2] + mi->vertex_offset].normal;
+    const or_f32* n0 = s->verts[s->idx[io] + mi->vertex_offset].normal;
+    const or_f32* n1 = s->verts[s->idx[io + 1] + mi->vertex_offset].normal;
+    const or_f32* n2 = s->verts[s->idx[io + 2] + mi->vertex_offset].normal;
     float w = 1.0f - u - v;
     v3 ln = V3(n0[0] * w + n1[0] * u + n2[0] * v, n0[1] * w + n1[1] * u + n2[1] * v,
                n0[2] * w + n1[2] * u + n2[2] * v);
@@ -1296,7 +1298,12 @@ static v4 ray_color(const or_frame* f, v3 o, v3 d, v4 wl, uint32_t* rng) {
 }
 
 static void frame_matrix(const wsrt_camera_uniform* cam, float* M) {
-    or_mat4_mul(cam->view_inverse, cam->proj_inverse, M);
+    float vi[16], pi[16];
+    for (int i = 0; i < 16; ++i) {
+        vi[i] = cam->view_inverse[i];
+        pi[i] = cam->proj_inverse[i];
+    }
+    or_mat4_mul(vi, pi, M);
 }
 
 /* shader.wgsl:709-799: one camera path; returns rgb, wavelengths, radiance. */
