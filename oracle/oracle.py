@@ -141,6 +141,15 @@ def _load(fp64: bool = False):
         "or_frame_trace_pixels": (None, [vp, u32p, C.c_uint32, vp, vp, vp]),
         "or_frame_render": (None, [vp, f32p, vp]),
         "or_frame_gather": (None, [vp, f32p, f32p, f32p, f32p]),
+        "or_unit_vectors_f": (None, [C.c_uint32, C.c_uint32, f32p]),
+        "or_sample_bsdf_f": (C.c_int, [f32p, f32p, vp, C.c_int, f32p, f32p, C.c_uint32, f32p, f32p,
+                                       C.POINTER(C.c_uint32)]),
+        "or_sample_light_f": (C.c_int, [vp, f32p, f32p, C.c_uint32, f32p, C.POINTER(_C.c_float),
+                                        C.POINTER(_C.c_float), f32p, C.POINTER(C.c_uint32)]),
+        "or_photon_emit_f": (None, [vp, C.c_uint32, C.c_uint32, f32p, f32p, C.POINTER(_C.c_float),
+                                    C.POINTER(_C.c_float)]),
+        "or_ray_color_f": (None, [vp, f32p, f32p, f32p, C.c_uint32, f32p]),
+        "or_roulette_f": (C.c_int, [f32p, C.c_uint32, f32p, C.POINTER(C.c_uint32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -294,10 +303,63 @@ class Scene:
         self.lib.or_morton_codes(self.h, out)
         return out
 
+    def sample_light(self, hp, wl, rng):
+        """One call of sample_random_light: (delta, direction, distance, pdf,
+        radiance, rng after)."""
+        d = np.zeros(3, self.fdt)
+        rad = np.zeros(4, self.fdt)
+        F = C.c_double if self.fdt == np.float64 else C.c_float
+        dist, pdf, r = F(), F(), C.c_uint32()
+        delta = self.lib.or_sample_light_f(self.h, np.asarray(hp, self.fdt), np.asarray(wl, self.fdt), rng, d,
+                                           C.byref(dist), C.byref(pdf), rad, C.byref(r))
+        return bool(delta), d, dist.value, pdf.value, rad, r.value
+
+    def photon_emit(self, frame: int, pid: int):
+        """Emission of photon path pid: (position, direction, wavelength, energy)."""
+        p = np.zeros(3, self.fdt)
+        d = np.zeros(3, self.fdt)
+        F = C.c_double if self.fdt == np.float64 else C.c_float
+        wl, en = F(), F()
+        self.lib.or_photon_emit_f(self.h, frame, pid, p, d, C.byref(wl), C.byref(en))
+        return p, d, wl.value, en.value
+
     def photon_trace(self, cfg: Config, frame: int, first: int, n: int) -> np.ndarray:
         out = np.zeros(n, PHOTON_DTYPE)
         self.lib.or_photon_trace(self.h, C.byref(cfg), frame, first, n, _ptr(out))
         return out
+
+
+def unit_vectors(seed: int, n: int) -> np.ndarray:
+    """n directions of random_unit_vector from the stream seeded with pcg_hash(seed)."""
+    out = np.zeros((n, 3), np.float32)
+    lib.or_unit_vectors_f(seed, n, out)
+    return out
+
+
+def material(color, extra) -> np.ndarray:
+    m = np.zeros(1, MATERIAL_DTYPE)
+    m["color"] = color
+    m["extra"] = extra
+    return m
+
+
+def sample_bsdf(rdir, ff, mat, front, wl, tp, rng):
+    """One call of sample_bsdf: (absorbed, direction, attenuation, rng after)."""
+    f = np.float32
+    d = np.zeros(3, f)
+    a = np.zeros(4, f)
+    r = C.c_uint32()
+    ab = lib.or_sample_bsdf_f(np.asarray(rdir, f), np.asarray(ff, f), _ptr(mat), int(front), np.asarray(wl, f),
+                              np.asarray(tp, f), rng, d, a, C.byref(r))
+    return bool(ab), d, a, r.value
+
+
+def roulette(tp, rng):
+    """One Russian roulette step: (survived, throughput after, rng after)."""
+    out = np.zeros(4, np.float32)
+    r = C.c_uint32()
+    alive = lib.or_roulette_f(np.asarray(tp, np.float32), rng, out, C.byref(r))
+    return bool(alive), out, r.value
 
 
 def cornell_box(fp64: bool = False) -> Scene:
@@ -376,6 +438,13 @@ class Frame:
         rgba = np.zeros((self.H * self.W, 4), np.uint8) if want_rgba8 else None
         self.lib.or_frame_render(self.h, accum, _ptr(rgba))
         return accum, rgba
+
+    def ray_color(self, o, d, wl, rng):
+        """ray_color of one ray: spectral radiance at the 4 wavelengths."""
+        out = np.zeros(4, self.fdt)
+        self.lib.or_ray_color_f(self.h, np.asarray(o, self.fdt), np.asarray(d, self.fdt),
+                                np.asarray(wl, self.fdt), rng, out)
+        return out
 
     def gather(self, pos, n, wl):
         out = np.zeros(4, self.fdt)

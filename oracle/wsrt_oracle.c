@@ -848,12 +848,14 @@ static v3 random_unit_vector(uint32_t* rng) { /* shader.wgsl:162-169 */
 /* ====================================================================== */
 /* Photon pass: photon_emit.wgsl:154-331                                     */
 /* ====================================================================== */
-static void trace_photon(const or_scene* s, const wsrt_config* cfg, uint32_t frame, uint32_t id,
-                         wsrt_photon* out) {
-    memset(out, 0, sizeof *out);
+/* Emission of photon path id (photon_emit.wgsl:159-208): the light, a point
+ * on it, the cosine direction, the hero wavelength and the energy; returns
+ * 0 when the scene has no light. */
+static int photon_emit(const or_scene* s, uint32_t frame, uint32_t id, uint32_t* rng_io, v3* pos_out,
+                       v3* dir_out, float* wl_out, float* energy_out) {
     uint32_t rng = or_pcg(id + frame * 927163u);
     uint32_t nl = s->nlights;
-    if (nl == 0) return;
+    if (nl == 0) return 0;
     uint32_t li = (uint32_t)(or_rand(&rng) * (float)nl);
     if (li > nl - 1) li = nl - 1;
     const wsrt_light_info* L = &s->lights[li];
@@ -882,8 +884,32 @@ static void trace_photon(const or_scene* s, const wsrt_config* cfg, uint32_t fra
     float pdf = or_cauchy_pdf(wl, crange);
     float radiance = or_blackbody(wl, temp);
     float flux = (radiance / pdf) * area_factor * OR_PI;
-    float energy = flux * intensity;
+    *energy_out = flux * intensity;
+    *wl_out = wl;
+    *pos_out = emit_pos;
+    *dir_out = emit_dir;
+    *rng_io = rng;
+    return 1;
+}
 
+void or_photon_emit_f(const or_scene* s, uint32_t frame, uint32_t id, float* pos, float* dir, float* wl,
+                      float* energy) {
+    uint32_t rng;
+    v3 p = V3(0, 0, 0), d = V3(0, 0, 0);
+    *wl = 0.0f;
+    *energy = 0.0f;
+    photon_emit(s, frame, id, &rng, &p, &d, wl, energy);
+    pos[0] = p.x; pos[1] = p.y; pos[2] = p.z;
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+}
+
+static void trace_photon(const or_scene* s, const wsrt_config* cfg, uint32_t frame, uint32_t id,
+                         wsrt_photon* out) {
+    memset(out, 0, sizeof *out);
+    uint32_t rng;
+    v3 emit_pos, emit_dir;
+    float wl, energy;
+    if (!photon_emit(s, frame, id, &rng, &emit_pos, &emit_dir, &wl, &energy)) return;
     v3 pos = add3(emit_pos, mul3s(emit_dir, 0.001f));
     v3 dir = emit_dir;
     int caustic = 0;
@@ -1237,6 +1263,24 @@ static v4 env_radiance(const or_scene* s, v3 d, v4 wl) {
     return mul4s(or_rgb_to_spectrum4(V3(t[0] * im, t[1] * im, t[2] * im), wl), mx);
 }
 
+/* Russian roulette (shader.wgsl:696-700): survival probability max(tp),
+ * not clamped to 1 (so a throughput above 1 always survives and is divided
+ * down); returns 0 when the path ends. */
+static int roulette(v4* tp, uint32_t* rng) {
+    float ps = maxf(maxf(tp->x, tp->y), maxf(tp->z, tp->w));
+    if (or_rand(rng) > ps) return 0;
+    *tp = div4s(*tp, ps);
+    return 1;
+}
+
+int or_roulette_f(const float* tp4, uint32_t rng, float* tp_out4, uint32_t* rng_out) {
+    v4 tp = V4(tp4[0], tp4[1], tp4[2], tp4[3]);
+    int alive = roulette(&tp, &rng);
+    tp_out4[0] = tp.x; tp_out4[1] = tp.y; tp_out4[2] = tp.z; tp_out4[3] = tp.w;
+    *rng_out = rng;
+    return alive;
+}
+
 /* shader.wgsl:547-703 */
 static v4 ray_color(const or_frame* f, v3 o, v3 d, v4 wl, uint32_t* rng) {
     const or_scene* s = f->s;
@@ -1288,11 +1332,7 @@ static v4 ray_color(const or_frame* f, v3 o, v3 d, v4 wl, uint32_t* rng) {
         tp = mul4(tp, sc.att);
         o = hp;
         d = sc.dir;
-        if (i > c->rr_after) {
-            float ps = maxf(maxf(tp.x, tp.y), maxf(tp.z, tp.w));
-            if (or_rand(rng) > ps) break;
-            tp = div4s(tp, ps);
-        }
+        if (i > c->rr_after && !roulette(&tp, rng)) break;
     }
     return acc;
 }
@@ -1486,4 +1526,42 @@ void or_scene_cornell_box(or_scene* s) {
     or_mat4_mul(T, S, M);
     or_scene_add_instance(s, sphere, bubble, M);
 #undef TRS
+}
+
+/* ====================================================================== */
+/* Taps of single steps, for the pins of the oracle's transport code        */
+/* ====================================================================== */
+void or_unit_vectors_f(uint32_t seed, uint32_t n, float* out) {
+    uint32_t rng = or_pcg(seed);
+    for (uint32_t i = 0; i < n; ++i) {
+        v3 u = random_unit_vector(&rng);
+        out[3 * i] = u.x; out[3 * i + 1] = u.y; out[3 * i + 2] = u.z;
+    }
+}
+
+int or_sample_bsdf_f(const float* rdir, const float* ffn, const wsrt_material* m, int front, const float* wl4,
+                     const float* tp4, uint32_t rng, float* dir_out, float* att_out, uint32_t* rng_out) {
+    scatter_t sc = sample_bsdf(V3(rdir[0], rdir[1], rdir[2]), V3(ffn[0], ffn[1], ffn[2]), m, front,
+                               V4(wl4[0], wl4[1], wl4[2], wl4[3]), V4(tp4[0], tp4[1], tp4[2], tp4[3]), &rng);
+    dir_out[0] = sc.dir.x; dir_out[1] = sc.dir.y; dir_out[2] = sc.dir.z;
+    att_out[0] = sc.att.x; att_out[1] = sc.att.y; att_out[2] = sc.att.z; att_out[3] = sc.att.w;
+    *rng_out = rng;
+    return sc.absorbed;
+}
+
+int or_sample_light_f(const or_scene* s, const float* hp, const float* wl4, uint32_t rng, float* dir,
+                      float* dist, float* pdf, float* rad4, uint32_t* rng_out) {
+    light_sample ls = sample_light(s, V3(hp[0], hp[1], hp[2]), V4(wl4[0], wl4[1], wl4[2], wl4[3]), &rng);
+    dir[0] = ls.dir.x; dir[1] = ls.dir.y; dir[2] = ls.dir.z;
+    *dist = ls.dist;
+    *pdf = ls.pdf;
+    rad4[0] = ls.radiance.x; rad4[1] = ls.radiance.y; rad4[2] = ls.radiance.z; rad4[3] = ls.radiance.w;
+    *rng_out = rng;
+    return ls.delta;
+}
+
+void or_ray_color_f(const or_frame* f, const float* o, const float* d, const float* wl4, uint32_t rng,
+                    float* out4) {
+    v4 L = ray_color(f, V3(o[0], o[1], o[2]), V3(d[0], d[1], d[2]), V4(wl4[0], wl4[1], wl4[2], wl4[3]), &rng);
+    out4[0] = L.x; out4[1] = L.y; out4[2] = L.z; out4[3] = L.w;
 }

@@ -128,6 +128,33 @@ void or_frame_render(const or_frame* f, float* accum, uint8_t* rgba8);
 void or_frame_gather(const or_frame* f, const float* pos3, const float* n3, const float* wl4,
                      float* out4);
 
+/* ---- taps of single steps (for the pins of the transport code) ---- */
+/* n directions of random_unit_vector (shader.wgsl:162-169) from the stream
+ * seeded with pcg_hash(seed). */
+void or_unit_vectors_f(uint32_t seed, uint32_t n, float* out3);
+/* sample_bsdf (shader.wgsl:347-471) for incoming direction rdir at a hit
+ * with forward-facing normal ffn; rng is the state before the call. Returns
+ * 1 when the path is absorbed. */
+int or_sample_bsdf_f(const float* rdir, const float* ffn, const wsrt_material* m, int front,
+                     const float* wl4, const float* tp4, uint32_t rng, float* dir_out3,
+                     float* att_out4, uint32_t* rng_out);
+/* sample_random_light (shader.wgsl:477-528) from point hp. Returns 1 for a
+ * point light (a delta distribution: pdf is then the distance squared over
+ * the light count). pdf 0: no sample. */
+int or_sample_light_f(const or_scene* s, const float* hp3, const float* wl4, uint32_t rng,
+                      float* dir3, float* dist, float* pdf, float* rad4, uint32_t* rng_out);
+/* Emission of photon path id of a frame (photon_emit.wgsl:159-208): point,
+ * direction, wavelength, energy (energy 0 when the scene has no light). */
+void or_photon_emit_f(const or_scene* s, uint32_t frame, uint32_t id, float* pos3, float* dir3,
+                      float* wl, float* energy);
+/* Russian roulette of a path's throughput (shader.wgsl:696-700). Returns 1
+ * when the path survives. */
+int or_roulette_f(const float* tp4, uint32_t rng, float* tp_out4, uint32_t* rng_out);
+/* ray_color (shader.wgsl:547-703) of one ray with given hero wavelengths
+ * and rng state, in the frame's scene and photon grid. */
+void or_ray_color_f(const or_frame* f, const float* o3, const float* d3, const float* wl4,
+                    uint32_t rng, float* out4);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,447 @@
+"""Pins of the oracle's light-transport code by what the mathematics fixes.
+
+The oracle's camera and photon paths are restatements of the reference's
+shaders; the GPU is checked against them path by path. Here each transport
+step is checked against something other than its own formula, so that a
+slip shared by both restatements fails:
+
+  random_unit_vector ... shader.wgsl:162-169      uniform on the sphere (KS tests)
+  Lambert sampling ..... shader.wgsl:462-470      cos-weighted about the normal (KS)
+  GGX sampling/weight .. shader.wgsl:193-220,356-378
+                         half vectors follow the textbook GGX NDF (chi-square
+                         against its quadrature), the mean weight equals the
+                         directional albedo of the textbook microfacet BRDF
+                         (f64 quadrature), alpha -> 0 is a mirror with Schlick F
+  ONB sign(0) (Q3) ..... shader.wgsl:171-188      a normal with z == 0 is absorbed
+  dielectric ........... shader.wgsl:380-431      Snell, TIR at asin(1/n) with no
+                         random number drawn (Q21), reflection probability =
+                         Schlick at the angle in air, refraction weight eta^2,
+                         hero collapse x4 / x1 (Q4)
+  thin film ............ shader.wgsl:432-462      straight transmission (Q5),
+                         reflection probability = Airy R at the hero
+                         wavelength, every wavelength's weight has mean 1
+  Russian roulette ..... shader.wgsl:696-700      unbiased for max(tp) <= 1; above 1
+                         always survives and divides the throughput (Q2)
+  light sampling + NEE . shader.wgsl:477-528,650-671
+                         E[1/pdf] = solid angle of the emitter, depth-1
+                         radiance = balance-weighted direct light, depth 2 adds
+                         the unweighted BSDF-hit emission (Q1), all against f64
+                         quadrature over a square emitter
+  photon emission ...... photon_emit.wgsl:159-208 mean energy = pi A_tri I integral
+                         of B over 380..780 nm (no light-count factor, Q6),
+                         cosine directions, uniform points
+  multi-bounce paths ... shader.wgsl:547-703      white furnace: a point light at
+                         the centre of a closed Lambert sphere of albedo rho gives
+                         rho B I / (pi R^2) (1 - rho^D) / (1 - rho), with and
+                         without roulette
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate, stats
+
+import oracle as O
+
+WL = np.array([450.0, 530.0, 610.0, 700.0], np.float32)
+TP1 = np.ones(4, np.float32)
+
+
+def seeds(n, base=1):
+    return [O.pcg_hash(base * 7919 + k) for k in range(n)]
+
+
+def ks_uniform(x):
+    return stats.kstest(np.asarray(x, np.float64), "uniform").pvalue
+
+
+def test_unit_vector_is_uniform_on_the_sphere():
+    v = O.unit_vectors(12345, 200000).astype(np.float64)
+    np.testing.assert_allclose(np.linalg.norm(v, axis=1), 1.0, atol=3e-6)
+    assert ks_uniform((v[:, 2] + 1) / 2) > 1e-3
+    phi = np.arctan2(v[:, 1], v[:, 0])
+    assert ks_uniform((phi + np.pi) / (2 * np.pi)) > 1e-3
+    assert np.all(np.abs(v.mean(0)) < 5 / math.sqrt(3 * len(v)))
+
+
+def test_lambert_sampling_is_cosine_weighted():
+    m = O.material([0.5, 0.5, 0.5, 1.0], [0, 0, 0, 0])
+    rng = np.random.default_rng(3)
+    n = rng.normal(size=3)
+    n /= np.linalg.norm(n)
+    t = np.cross(n, [0.3, 0.5, 0.8])
+    t /= np.linalg.norm(t)
+    b = np.cross(n, t)
+    cos, phi = [], []
+    for s in seeds(30000):
+        ab, d, att, _ = O.sample_bsdf(-n, n, m, 1, WL, TP1, s)
+        assert not ab
+        np.testing.assert_array_equal(att, np.float32(0.5))
+        cos.append(float(np.dot(d, n)))
+        phi.append(math.atan2(float(np.dot(d, b)), float(np.dot(d, t))))
+    cos = np.array(cos)
+    # p(cos) = 2 cos on [0, 1]: cos^2 is uniform
+    assert cos.min() > -1e-6
+    assert ks_uniform(np.clip(cos, 0, 1) ** 2) > 1e-3
+    assert ks_uniform((np.array(phi) + np.pi) / (2 * np.pi)) > 1e-3
+
+
+# ---- GGX: textbook microfacet model ----------------------------------------
+def ggx_d(mu, a):
+    a2 = a * a
+    return a2 / (math.pi * ((a2 - 1.0) * mu * mu + 1.0) ** 2)
+
+
+def schlick(f0, c):
+    return f0 + (1.0 - f0) * (1.0 - c) ** 5
+
+
+def g1(x, a):
+    k = a * a / 2.0  # Smith with Schlick's k = alpha^2 / 2
+    return x / (x * (1.0 - k) + k)
+
+
+def ggx_albedo(theta_v, a, f0):
+    """Directional albedo of the textbook GGX BRDF D F G / (4 n.l n.v), by
+    quadrature over half vectors (d omega_l = 4 v.h d omega_h)."""
+    v = np.array([math.sin(theta_v), 0.0, math.cos(theta_v)])
+
+    def f(phi, th):
+        h = np.array([math.sin(th) * math.cos(phi), math.sin(th) * math.sin(phi), math.cos(th)])
+        vh = float(v @ h)
+        if vh <= 0:
+            return 0.0
+        l_ = 2 * vh * h - v
+        nl, nv = l_[2], v[2]
+        if nl <= 0:
+            return 0.0
+        brdf = ggx_d(h[2], a) * schlick(f0, vh) * g1(nl, a) * g1(nv, a) / (4 * nl * nv)
+        return brdf * nl * 4 * vh * math.sin(th)
+
+    val, _ = integrate.dblquad(f, 0.0, math.pi / 2, 0.0, 2 * math.pi, epsabs=1e-7, epsrel=1e-6)
+    return val
+
+
+def test_ggx_half_vectors_follow_the_ndf():
+    a = 0.3
+    m = O.material([0.8, 0.8, 0.8, 1.0], [1, a, 0, 0])
+    n = np.array([0.0, 0.0, 1.0], np.float32)
+    mus, absorbed = [], 0
+    N = 40000
+    for s in seeds(N, 2):
+        ab, d, _, _ = O.sample_bsdf(-n, n, m, 1, WL, TP1, s)  # view along the normal
+        if ab:
+            absorbed += 1
+            continue
+        h = (d + n) / np.linalg.norm(d + n)
+        mus.append(float(h[2]))
+    # p(mu) d mu = 2 pi D(mu) mu d mu; with v = n, l is above the surface iff
+    # the half vector is within 45 degrees of n
+    mu45 = math.cos(math.pi / 4)
+    p_abs = integrate.quad(lambda mu: 2 * math.pi * ggx_d(mu, a) * mu, 0.0, mu45)[0]
+    assert integrate.quad(lambda mu: 2 * math.pi * ggx_d(mu, a) * mu, 0.0, 1.0)[0] == pytest.approx(1, abs=1e-9)
+    se = math.sqrt(p_abs * (1 - p_abs) / N)
+    assert abs(absorbed / N - p_abs) < 5 * se + 1e-4
+    edges = np.linspace(mu45, 1.0, 21)
+    want = np.array([integrate.quad(lambda mu: 2 * math.pi * ggx_d(mu, a) * mu, lo, hi)[0]
+                     for lo, hi in zip(edges[:-1], edges[1:])]) * N
+    got, _ = np.histogram(mus, edges)
+    chi2 = float(np.sum((got - want) ** 2 / want))
+    assert stats.chi2.sf(chi2, len(got) - 1) > 1e-3
+
+
+@pytest.mark.parametrize("theta_deg,alpha", [(30.0, 0.3), (60.0, 0.5)])
+def test_ggx_weight_mean_is_the_directional_albedo(theta_deg, alpha):
+    f0 = 0.8
+    m = O.material([f0, f0, f0, 1.0], [1, alpha, 0, 0])
+    th = math.radians(theta_deg)
+    v = np.array([math.sin(th), 0.0, math.cos(th)], np.float32)
+    n = np.array([0.0, 0.0, 1.0], np.float32)
+    w = []
+    for s in seeds(40000, 3):
+        ab, _, att, _ = O.sample_bsdf(-v, n, m, 1, WL, TP1, s)
+        w.append(0.0 if ab else float(att[0]))
+    w = np.array(w)
+    want = ggx_albedo(th, alpha, f0)
+    assert abs(w.mean() - want) < 4 * w.std() / math.sqrt(len(w)) + 1e-4, (w.mean(), want)
+
+
+def test_ggx_smooth_limit_is_a_mirror_with_schlick_fresnel():
+    f0 = 0.6
+    m = O.material([f0, f0, f0, 1.0], [1, 0.0, 0, 0])  # alpha clamps to 1e-3
+    n = np.array([0.0, 0.0, 1.0], np.float32)
+    th = math.radians(40.0)
+    v = np.array([math.sin(th), 0.0, math.cos(th)], np.float32)
+    mirror = np.array([-v[0], 0.0, v[2]])
+    dev = []
+    for s in seeds(2000, 4):
+        ab, d, att, _ = O.sample_bsdf(-v, n, m, 1, WL, TP1, s)
+        assert not ab
+        dev.append(np.linalg.norm(d - mirror))
+        if dev[-1] < 1e-3:  # the weight tends to Schlick's F at the mirror angle
+            np.testing.assert_allclose(att, schlick(f0, math.cos(th)), rtol=1e-3)
+    assert np.sum(np.array(dev) < 1e-3) > 100
+    # the half vector's polar angle has P(tan > t) = a^2 / (a^2 + t^2): the
+    # reflected direction is within 2 atan(3 a) of the mirror direction for
+    # 90 % of samples (the tail is GGX's)
+    assert np.median(dev) < 4e-3 and np.mean(np.array(dev) < 2 * math.atan(3e-3) * 1.01) > 0.88
+
+
+def test_onb_sign_zero_absorbs_ggx_paths():
+    """Q3: WGSL sign(0) = 0 makes the basis of a normal with z == 0 infinite,
+    so every GGX sample at such a normal is absorbed; z != 0 is unaffected."""
+    m = O.material([0.8, 0.8, 0.8, 1.0], [1, 0.3, 0, 0])
+    for n in ([1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.6, -0.8, 0.0]):
+        n = np.array(n, np.float32)
+        assert all(O.sample_bsdf(-n, n, m, 1, WL, TP1, s)[0] for s in seeds(50, 5))
+    for n in ([0.0, 0.0, -1.0], [0.6, 0.0, 0.8]):
+        n = np.array(n, np.float32)
+        alive = [not O.sample_bsdf(-n, n, m, 1, WL, TP1, s)[0] for s in seeds(200, 6)]
+        assert np.mean(alive) > 0.9
+
+
+# ---- dielectric -------------------------------------------------------------
+A, B = 1.5, 0.0042
+
+
+def ior(lam):
+    return A + B / (lam / 1000.0) ** 2
+
+
+def incoming(theta):
+    return np.array([math.sin(theta), 0.0, -math.cos(theta)], np.float32)
+
+
+def test_dielectric_total_internal_reflection_threshold():
+    glass = O.material([1.0, 1.0, 1.0, 1.0], [2, 0, A, B])
+    n = np.array([0.0, 0.0, 1.0], np.float32)
+    crit = math.asin(1.0 / ior(float(WL[0])))
+    d_in = incoming(crit + 2e-3)
+    for s in seeds(50, 7):
+        ab, d, att, r = O.sample_bsdf(d_in, n, glass, 0, WL, TP1, s)
+        assert not ab and r == s  # TIR draws no random number (Q21)
+        np.testing.assert_allclose(d, [d_in[0], 0.0, -d_in[2]], atol=1e-6)
+        np.testing.assert_array_equal(att, 1.0)
+    d_in = incoming(crit - 2e-2)
+    out = [O.sample_bsdf(d_in, n, glass, 0, WL, TP1, s) for s in seeds(300, 8)]
+    assert all(r == O.pcg_hash(s) for (_, _, _, r), s in zip(out, seeds(300, 8)))
+    assert any(d[2] < 0 for _, d, _, _ in out)  # some paths refract out
+
+
+@pytest.mark.parametrize("front,theta_deg", [(1, 50.0), (0, 30.0)])
+def test_dielectric_snell_fresnel_and_weight(front, theta_deg):
+    glass = O.material([1.0, 1.0, 1.0, 1.0], [2, 0, A, 0.0])  # B = 0: no hero collapse
+    n = np.array([0.0, 0.0, 1.0], np.float32)
+    th = math.radians(theta_deg)
+    d_in = incoming(th)
+    eta = 1.0 / A if front else A  # n_incident / n_transmitted
+    sin_t = eta * math.sin(th)
+    cos_air = math.cos(th) if front else math.sqrt(1 - sin_t**2)
+    r0 = ((1 - A) / (1 + A)) ** 2
+    p_refl = r0 + (1 - r0) * (1 - cos_air) ** 5  # Schlick at the angle in air
+    N, nrefl = 20000, 0
+    for s in seeds(N, 9 + front):
+        ab, d, att, _ = O.sample_bsdf(d_in, n, glass, front, WL, TP1, s)
+        assert not ab
+        if d[2] > 0:
+            nrefl += 1
+            np.testing.assert_array_equal(att, 1.0)
+        else:
+            assert d[0] == pytest.approx(sin_t, abs=2e-6)  # Snell, in the plane of incidence
+            assert abs(d[1]) < 1e-7
+            # radiance scales by eta^2 on refraction; color (1,1,1) -> 0.99
+            np.testing.assert_allclose(att, 0.99 * eta * eta, rtol=1e-6)
+    se = math.sqrt(p_refl * (1 - p_refl) / N)
+    assert abs(nrefl / N - p_refl) < 5 * se
+
+
+def test_dielectric_hero_collapse():
+    """Q4: with dispersion (B > 0) a refraction keeps only the hero
+    wavelength, x4 unless the path was already collapsed."""
+    glass = O.material([1.0, 1.0, 1.0, 1.0], [2, 0, A, B])
+    n = np.array([0.0, 0.0, 1.0], np.float32)
+    d_in = incoming(0.2)
+    eta = 1.0 / ior(float(WL[0]))
+    for tp, mult in ((TP1, 4.0), (np.array([0.7, 0, 0, 0], np.float32), 1.0)):
+        got = [O.sample_bsdf(d_in, n, glass, 1, WL, tp, s) for s in seeds(100, 11)]
+        refr = [att for _, d, att, _ in got if d[2] < 0]
+        assert refr
+        for att in refr:
+            np.testing.assert_allclose(att, [mult * 0.99 * eta * eta, 0, 0, 0], rtol=1e-5)
+
+
+def test_thin_film_transmits_straight_and_conserves_energy():
+    film = O.material([1.0, 1.0, 1.0, 1.0], [4, 400.0, 1.33, 1.5])
+    n = np.array([0.0, 0.0, 1.0], np.float32)
+    th = math.radians(30.0)
+    d_in = incoming(th)
+    F = [O.lib.or_thin_film_f(math.cos(th), 1.0, 1.33, 1.5, 400.0, float(w)) for w in WL]
+    N, atts, nrefl = 20000, [], 0
+    for s in seeds(N, 12):
+        ab, d, att, _ = O.sample_bsdf(d_in, n, film, 1, WL, TP1, s)
+        assert not ab
+        if d[2] > 0:
+            nrefl += 1
+            np.testing.assert_allclose(d, [d_in[0], 0, -d_in[2]], atol=1e-6)
+        else:
+            np.testing.assert_allclose(d, d_in, atol=1e-7)  # Q5: no refraction
+        atts.append(att)
+    se = math.sqrt(F[0] * (1 - F[0]) / N)
+    assert abs(nrefl / N - F[0]) < 5 * se
+    atts = np.array(atts, np.float64)
+    m, e = atts.mean(0), atts.std(0) / math.sqrt(N)
+    assert np.all(np.abs(m - 1.0) < 5 * e + 1e-6), (m, e)
+
+
+def test_russian_roulette():
+    tp = np.array([0.3, 0.6, 0.2, 0.1], np.float32)
+    out, alive = [], 0
+    for s in seeds(40000, 13):
+        a, t, _ = O.roulette(tp, s)
+        alive += a
+        out.append(t if a else np.zeros(4))
+    out = np.array(out, np.float64)
+    assert alive / len(out) == pytest.approx(0.6, abs=0.01)
+    np.testing.assert_allclose(out.mean(0), tp, rtol=0.03)  # unbiased
+    # Q2: the survival probability is not clamped to 1
+    big = np.array([2.0, 1.0, 0.5, 0.5], np.float32)
+    for s in seeds(100, 14):
+        a, t, _ = O.roulette(big, s)
+        assert a
+        np.testing.assert_allclose(t, big / 2.0)
+
+
+# ---- light sampling, next-event estimation, emission -----------------------
+SIDE, HEIGHT, T_K, INTENSITY = 0.5, 1.0, 6500.0, 1.0
+ALBEDO = 0.5
+
+
+def emitter_scene():
+    """A Lambert floor (albedo 0.5, y = 0, 20 x 20) and a square emitter of
+    side 0.5 at height 1, facing down, as two light triangles."""
+    s = O.Scene()
+    pv, pi = O.geometry("plane")
+    plane = s.add_mesh(pv, pi)
+    floor = s.add_material([ALBEDO, ALBEDO, ALBEDO, 1.0], [0, 0, 0, 0])
+    light = s.add_material([T_K, INTENSITY, 0.0, 1.0], [3, 0, 0, 0])
+    s.add_instance(plane, floor, O.mat4("scale", 20.0, 1.0, 20.0))
+    m = O.mat4_mul(O.mat4_mul(O.mat4("translation", 0.0, HEIGHT, 0.0), O.mat4("rotation_x", math.pi)),
+                   O.mat4("scale", SIDE, 1.0, SIDE))
+    s.add_light_instance(plane, light, m, [0.0, T_K, INTENSITY, 0.0])
+    return s.build()
+
+
+def light_integral(x0, z0, weight=None):
+    """f64 quadrature over the emitter of cos cos' / d^2 (times weight(d2, c))."""
+    h = HEIGHT
+
+    def f(v, u):
+        d2 = (u - x0) ** 2 + (v - z0) ** 2 + h * h
+        c = h / math.sqrt(d2)  # cosine at the floor and at the emitter
+        g = c * c / d2
+        return g * (weight(d2, c) if weight else 1.0)
+
+    return integrate.dblquad(f, -SIDE / 2, SIDE / 2, -SIDE / 2, SIDE / 2, epsabs=1e-10, epsrel=1e-8)[0]
+
+
+def test_light_sample_pdf_is_solid_angle_density():
+    s = emitter_scene()
+    assert len(s.lights()) == 2
+    hp = np.array([0.3, 0.0, -0.2], np.float32)
+    inv, cosw, pts = [], [], []
+    for r in seeds(40000, 15):
+        delta, d, dist, pdf, rad, _ = s.sample_light(hp, WL, r)
+        assert not delta and pdf > 0
+        inv.append(1.0 / pdf)
+        cosw.append(float(d[1]) / pdf)
+        pts.append(hp + d * dist)
+    inv, cosw, pts = np.array(inv), np.array(cosw), np.array(pts)
+    h = HEIGHT
+    omega = integrate.dblquad(lambda v, u: h / ((u - 0.3) ** 2 + (v + 0.2) ** 2 + h * h) ** 1.5,
+                              -SIDE / 2, SIDE / 2, -SIDE / 2, SIDE / 2)[0]
+    assert abs(inv.mean() - omega) < 4 * inv.std() / math.sqrt(len(inv))
+    proj = light_integral(0.3, -0.2)  # projected solid angle
+    assert abs(cosw.mean() - proj) < 4 * cosw.std() / math.sqrt(len(cosw))
+    # points are uniform over the emitter
+    np.testing.assert_allclose(pts[:, 1], HEIGHT, atol=1e-5)
+    assert ks_uniform(pts[:, 0] / SIDE + 0.5) > 1e-3
+    assert ks_uniform(pts[:, 2] / SIDE + 0.5) > 1e-3
+    bb = np.array([O.lib.or_blackbody_f(float(w), T_K) for w in WL])
+    np.testing.assert_allclose(rad, bb * INTENSITY, rtol=1e-6)
+
+
+@pytest.mark.parametrize("depth", [1, 2])
+def test_direct_light_against_quadrature(depth):
+    """Camera ray straight down onto the floor at (x0, 0, z0). Depth 1: only
+    next-event estimation, weighted by the balance heuristic against the
+    cosine pdf. Depth 2 also follows the cosine-sampled ray, and a hit on the
+    emitter adds its emission without a MIS weight (Q1)."""
+    s = emitter_scene()
+    cfg = O.default_config(max_depth=depth, enable_photons=0)
+    f = O.Frame(s, cfg, O.camera_uniform(), 4, 4)
+    x0, z0 = 0.3, -0.2
+    area = SIDE * SIDE
+
+    def w_light(d2, c):
+        pl = d2 / (area * c)  # solid-angle pdf of the light sample (both triangles)
+        return pl / (pl + c / math.pi)
+
+    nee = light_integral(x0, z0, w_light)
+    direct = light_integral(x0, z0)
+    bb = np.array([O.lib.or_blackbody_f(float(w), T_K) for w in WL], np.float64)
+    want = ALBEDO / math.pi * bb * INTENSITY * (nee + (direct if depth == 2 else 0.0))
+    L = np.array([f.ray_color([x0, 0.5, z0], [0, -1, 0], WL, r) for r in seeds(60000, 16 + depth)], np.float64)
+    m, e = L.mean(0), L.std(0) / math.sqrt(len(L))
+    assert np.all(np.abs(m - want) < 4 * e + 1e-6 * want), (m, want, e)
+    if depth == 2:  # the unweighted BSDF hit over-counts: more than the plain direct light
+        assert np.all(m > ALBEDO / math.pi * bb * direct * 1.2)
+
+
+def test_photon_emission_flux_direction_and_position():
+    s = emitter_scene()
+    lam = np.linspace(380.0, 780.0, 4001)
+    bint = np.trapezoid(np.array([O.lib.or_blackbody_f(float(x), T_K) for x in lam], np.float64), lam)
+    N = 40000
+    e, cos, pos = [], [], []
+    for k in range(N):
+        p, d, wl, en = s.photon_emit(3, k)
+        assert 380.0 <= wl <= 780.0
+        e.append(en)
+        cos.append(-float(d[1]))  # emitter normal is -y
+        pos.append(p)
+    e, cos, pos = np.array(e), np.array(cos), np.array(pos)
+    # Q6: each photon carries one triangle's power (pi A_tri I int B), with
+    # no factor for the two triangles it was chosen among
+    want = math.pi * (SIDE * SIDE / 2) * INTENSITY * bint
+    assert abs(e.mean() - want) < 4 * e.std() / math.sqrt(N) + 1e-4 * want, (e.mean(), want)
+    assert ks_uniform(np.clip(cos, 0, 1) ** 2) > 1e-3  # cosine-distributed
+    np.testing.assert_allclose(pos[:, 1], HEIGHT, atol=1e-5)
+    assert ks_uniform(pos[:, 0] / SIDE + 0.5) > 1e-3
+    assert ks_uniform(pos[:, 2] / SIDE + 0.5) > 1e-3
+
+
+# ---- white furnace ----------------------------------------------------------
+@pytest.mark.parametrize("rr_after", [3, 0])
+def test_white_furnace_point_light_in_lambert_sphere(rr_after):
+    """Inside a closed Lambert sphere of radius R and albedo rho with a point
+    light of intensity I at its centre, every wall point receives I / R^2
+    directly and the wall radiance after k bounces is rho^k times the first:
+    L = rho B I / (pi R^2) * (1 - rho^D) / (1 - rho) over D = max_depth path
+    vertices. The sphere is an icosphere (5120 faces); its faceting moves the
+    distance to the centre by < 0.2 %."""
+    R, rho, D = 1.0, 0.5, 12
+    s = O.Scene()
+    sv, si = O.geometry("sphere", 4)
+    mesh = s.add_mesh(sv, si)
+    wall = s.add_material([rho, rho, rho, 1.0], [0, 0, 0, 0])
+    s.add_instance(mesh, wall, O.mat4("scale", 2 * R, 2 * R, 2 * R))
+    s.add_point_light([0.0, 0.0, 0.0], [1.0, T_K, INTENSITY, 0.0])
+    s.build()
+    f = O.Frame(s, O.default_config(max_depth=D, enable_photons=0, rr_after=rr_after), O.camera_uniform(), 4, 4)
+    d = np.array([0.3, -0.5, 0.81], np.float32)
+    d /= np.linalg.norm(d)
+    L = np.array([f.ray_color([0.1, 0.05, 0.0], d, WL, r) for r in seeds(30000, 20 + rr_after)], np.float64)
+    bb = np.array([O.lib.or_blackbody_f(float(w), T_K) for w in WL], np.float64)
+    want = rho * bb * INTENSITY / (math.pi * R * R) * (1 - rho**D) / (1 - rho)
+    m, e = L.mean(0), L.std(0) / math.sqrt(len(L))
+    assert np.all(np.abs(m - want) < 4 * e + 4e-3 * want), (m / want, e / want)
