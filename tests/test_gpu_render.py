@@ -92,6 +92,14 @@ def test_photon_pass_and_grid(device, name, frame, photons):
     assert ok.mean() >= 0.99
     wl_ok = np.abs(ph["wavelength"][both] - ref["wavelength"][both]) <= 1e-3
     assert wl_ok.mean() >= 0.99
+    # the rest of each record: direction (the reversed incoming ray) and
+    # energy (flux of photon_emit.wgsl:184-208; the GPU's exp differs from
+    # libm by a few ulp)
+    dir_ok = np.all(np.abs(ph["direction"][both] - ref["direction"][both]) <= 1e-3, axis=1)
+    assert dir_ok.mean() >= 0.99
+    e_ok = np.abs(ph["energy"][both] - ref["energy"][both]) <= 1e-4 * np.abs(ref["energy"][both])
+    print(f"  direction within 1e-3 {dir_ok.mean():.5f}, energy within 1e-4 {e_ok.mean():.5f}", flush=True)
+    assert e_ok.mean() >= 0.99
     # grid of the frame: bit-exact with the oracle's counting sort of the GPU photons
     cell, count, offset, slot = r.read_grid()
     oc, ocount, ooff, oslot = O.grid_counting(ph, frame, cfg.cell_size, cfg.hash_size)
