@@ -108,7 +108,7 @@ def check_hits(g, o, seed):
     assert 0.05 < occ.mean() < 0.95
 
 
-def check_paths(device, g, o, cfg, view, frame, n_pixels, seed):
+def check_paths(device, g, o, cfg, view, frame, n_pixels, seed, min_lit=0.2):
     W, H = view.width, view.height
     r = wsrt.Renderer(device, g, W, H, cfg)
     cam = scenes.camera(view, frame)
@@ -124,7 +124,7 @@ def check_paths(device, g, o, cfg, view, frame, n_pixels, seed):
     print(f"  {n_pixels} paths of {W}x{H}: within 1e-4 {same.mean():.5f}; nonzero {np.mean(np.abs(oL).sum(1) > 0):.3f}",
           flush=True)
     assert same.mean() >= 0.99
-    assert np.mean(np.abs(oL).sum(1) > 0) > 0.2
+    assert np.mean(np.abs(oL).sum(1) > 0) > min_lit
     return r
 
 
@@ -149,7 +149,9 @@ def test_occluded_room_at_full_size(device, full_room):
     g, o = full_room
     assert g.info.n_triangles == o.n_triangles == 2097182
     check_hits(g, o, 41)
-    check_paths(device, g, o, scenes.occluded_config(frames_in_flight=4), scenes.OCCLUDED_VIEW, 5, 4096, 42)
+    # lit only through the door gap: about 5 % of one-sample paths carry light
+    check_paths(device, g, o, scenes.occluded_config(frames_in_flight=4), scenes.OCCLUDED_VIEW, 5, 16384, 42,
+                min_lit=0.02)
 
 
 def test_guided_mean_equals_plain_on_the_full_room(device, full_room):
