@@ -372,6 +372,21 @@ int wsrt_photon_grid_build(wsrt_device* dev, const wsrt_photon* photons, uint32_
                            uint32_t* cell_id, uint32_t* count, uint32_t* offset,
                            uint32_t* sorted_slot);
 
+/* No counterpart in the reference. The grid build of wsrt_photon_grid_build
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc): it allocates
+ * nothing. photons, cell_id, count, offset and sorted_slot are DEVICE
+ * pointers of the same meaning and sizes; the call is asynchronous on
+ * `stream` (a hipStream_t; null: the device's own stream). Work buffers live
+ * in a file-scope device array of 32 MiB per GPU (held until the process
+ * ends), which limits n to 2^20 photons and hash_size to 2^22 cells
+ * (WSRT_E_INVALID beyond). Calls on one GPU share that array: calls that
+ * overlap in time on one GPU race. */
+int wsrt_photon_grid_build_noalloc(wsrt_device* dev, const wsrt_photon* photons, uint32_t n,
+                                   uint32_t frame_count, float cell_size, uint32_t hash_size,
+                                   uint32_t* cell_id, uint32_t* count, uint32_t* offset,
+                                   uint32_t* sorted_slot, void* stream);
+
 /* Closest-hit query for n rays: rays = n x 8 floats (ox,oy,oz,tmin,dx,dy,dz,tmax).
  * hits = n x 4 words: t (float bits, +inf on miss), triangle id (0xFFFFFFFF on
  * miss), u (float), v (float) ; flags[n]: bit0 hit, bit1 front face.

@@ -119,11 +119,20 @@ void mat4_mul(const float* a, const float* b, float* out) {
 // ---------------------------------------------------------------------------
 // device
 // ---------------------------------------------------------------------------
+// Work memory of wsrt_photon_grid_build_noalloc: one copy per GPU, held
+// until the process ends. Host code uses only the device address that
+// hipGetSymbolAddress returns for it (wsrt_device::scratch), never the name.
+constexpr uint32_t kNoAllocMaxPhotons = 1u << 20;  // MAX_PHOTONS, renderer/mod.rs:18
+constexpr uint32_t kNoAllocMaxCells = 1u << 22;    // HASH_SIZE, renderer/mod.rs:19
+constexpr size_t kScratchBytes = 32ull << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
 struct wsrt_device {
     int ordinal = 0;
     hipStream_t stream = nullptr;  // builds and debug queries
     char name[256] = {0};
     std::atomic<int> live{0};      // scenes and renderers created on this device
+    uint8_t* scratch = nullptr;    // this GPU's copy of the file-scope scratch array
 };
 
 extern "C" int wsrt_device_count(void) {
@@ -146,6 +155,10 @@ extern "C" int wsrt_device_create(int ordinal, wsrt_device** out) {
     d->ordinal = ordinal;
     snprintf(d->name, sizeof d->name, "%s", prop.gcnArchName);
     HIP_OR_FAIL(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    // this device's address of the scratch array (the device is current)
+    void* sp = nullptr;
+    HIP_OR_FAIL(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+    d->scratch = static_cast<uint8_t*>(sp);
     *out = d.release();
     return WSRT_OK;
 }
@@ -1828,6 +1841,51 @@ extern "C" int wsrt_photon_grid_build(wsrt_device* dev, const wsrt_photon* photo
     HIP_OR_FAIL(hipStreamSynchronize(s));
     // slots past the valid ones are not part of the grid
     for (uint32_t k = valid; k < n; ++k) sorted_slot[k] = kInvalid;
+    return WSRT_OK;
+}
+
+// Scratch layout: the frame's constants, the device frame counter, the
+// valid-photon count, then the grid's work words.
+extern "C" int wsrt_photon_grid_build_noalloc(wsrt_device* dev, const wsrt_photon* photons, uint32_t n,
+                                              uint32_t frame_count, float cell_size, uint32_t hash_size,
+                                              uint32_t* cell_id, uint32_t* count, uint32_t* offset,
+                                              uint32_t* sorted_slot, void* stream) {
+    CHECK_ARG(dev && dev->scratch && (photons || n == 0) && count && offset);
+    CHECK_ARG((cell_id && sorted_slot) || n == 0);
+    if (hash_size == 0 || (hash_size & (hash_size - 1)) || hash_size > kNoAllocMaxCells)
+        return fail(WSRT_E_INVALID, "hash_size %u must be a power of two <= 2^22 (scratch limit)", hash_size);
+    if (n > kNoAllocMaxPhotons) return fail(WSRT_E_INVALID, "%u photons > 2^20 (scratch limit)", n);
+    if (!(cell_size > 0.0f)) return fail(WSRT_E_INVALID, "cell_size must be > 0");
+    const size_t words = grid_temp_words(n ? n : 1u, hash_size);
+    if (768 + 4 * words > kScratchBytes) return fail(WSRT_E_INVALID, "grid work exceeds the scratch array");
+    HIP_OR_FAIL(hipSetDevice(dev->ordinal));
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : dev->stream;
+    wsrt_config c;
+    wsrt_config_default(&c);
+    c.cell_size = cell_size;
+    c.hash_size = hash_size;
+    wsrt_camera_uniform cam;
+    memset(&cam, 0, sizeof cam);
+    cam.frame_count = frame_count;
+    const FrameConst fc = make_frame_const(c, cam, 1, 1);
+    static_assert(sizeof(FrameConst) <= 256, "frame constants fit their scratch slot");
+    FrameConst* tab = reinterpret_cast<FrameConst*>(dev->scratch);
+    uint32_t* frame = reinterpret_cast<uint32_t*>(dev->scratch + 256);
+    GridBuffers g;
+    g.count = count;
+    g.offset = offset;
+    g.cell_id = cell_id;
+    g.sorted_slot = sorted_slot;
+    g.sorted = nullptr;
+    g.n_valid = reinterpret_cast<uint32_t*>(dev->scratch + 512);
+    g.temp = reinterpret_cast<uint32_t*>(dev->scratch + 768);
+    launch_store_frame(fc, tab, frame, s);
+    FrameArgs fa{tab, frame, 0u};
+    if (grid_build(photons, n, 1, hash_size, fa, g, s)) return fail(WSRT_E_HIP, "grid build failed");
+    // slots past the valid photons are not part of the grid
+    if (n) launch_fill_after(sorted_slot, g.n_valid, n, kInvalid, s);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(WSRT_E_HIP, "grid build launch: %s", hipGetErrorString(e));
     return WSRT_OK;
 }
 

@@ -82,7 +82,7 @@ struct GridBuffers {
     uint32_t* offset;       // hash_size
     uint32_t* cell_id;      // n
     uint32_t* sorted_slot;  // n
-    wsrt_photon* sorted;    // n
+    wsrt_photon* sorted;    // n (null: grid_build writes the sorted slots only)
     uint32_t* n_valid;      // 1
     uint32_t* temp;         // grid_temp_words(n, hash_size)
     // renderer grid: the cell of each photon of the last build (n_valid of
@@ -98,6 +98,10 @@ void launch_photon_emit(const SceneView& sv, const FrameArgs& fa, uint32_t np, u
 void launch_photon_bounce(const SceneView& sv, uint32_t depth, uint32_t count_traversal, const PhotonState& ps,
                           const Queues& qs, wsrt_photon* photons, Counters* ctr, uint2* gstack, hipStream_t s);
 size_t grid_temp_words(uint32_t n, uint32_t cells);
+// Writes fc to *dst and sets *frame = 0, on the device (no host copy).
+void launch_store_frame(const FrameConst& fc, FrameConst* dst, uint32_t* frame, hipStream_t s);
+// p[k] = v for *start <= k < n (start read on the device).
+void launch_fill_after(uint32_t* p, const uint32_t* start, uint32_t n, uint32_t v, hipStream_t s);
 // One counting sort over the grids of nf frames: cell (frame slot, hash)
 // = slot * hash_size + hash; count/offset hold nf * hash_size cells.
 int grid_build(const wsrt_photon* photons, uint32_t np, uint32_t nf, uint32_t hash_size,

@@ -532,11 +532,26 @@ __global__ __launch_bounds__(256) void k_grid_copy(const wsrt_photon* __restrict
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nv; k += gridDim.x * blockDim.x) {
         const uint32_t slot = sorted_vals[k];
         sorted_slot[k] = slot;
-        const float4* src = reinterpret_cast<const float4*>(photons + slot);
-        float4* dst = reinterpret_cast<float4*>(sorted + k);
-        dst[0] = src[0];
-        dst[1] = src[1];
+        if (sorted) {  // null: slots only
+            const float4* src = reinterpret_cast<const float4*>(photons + slot);
+            float4* dst = reinterpret_cast<float4*>(sorted + k);
+            dst[0] = src[0];
+            dst[1] = src[1];
+        }
     }
+}
+// One frame's constants into a table entry, and the device frame counter
+// set to 0 (so entry 0 is the frame's).
+__global__ void k_store_frame(FrameConst fc, FrameConst* __restrict__ dst, uint32_t* __restrict__ frame) {
+    if (threadIdx.x == 0) {
+        *dst = fc;
+        *frame = 0u;
+    }
+}
+// p[k] = v for *start <= k < n.
+__global__ __launch_bounds__(256) void k_fill_after(uint32_t* __restrict__ p, const uint32_t* __restrict__ start,
+                                                    uint32_t n, uint32_t v) {
+    for (uint32_t k = *start + blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = v;
 }
 // blocks of a launch over at most n valid photons: enough to fill the chip,
 // no more than the photons need
@@ -554,6 +569,14 @@ void launch_photon_bounce(const SceneView& sv, uint32_t depth, uint32_t count_tr
                           const Queues& qs, wsrt_photon* photons, Counters* ctr, uint2* gstack, hipStream_t s) {
     hipLaunchKernelGGL(k_photon_bounce, dim3(trace_grid(5)), dim3(kTraceThreads), 0, s, sv, depth, qs,
                        count_traversal, ps, photons, ctr, gstack);
+}
+
+void launch_store_frame(const FrameConst& fc, FrameConst* dst, uint32_t* frame, hipStream_t s) {
+    hipLaunchKernelGGL(k_store_frame, dim3(1), dim3(64), 0, s, fc, dst, frame);
+}
+
+void launch_fill_after(uint32_t* p, const uint32_t* start, uint32_t n, uint32_t v, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_fill_after, dim3(valid_grid(n)), dim3(256), 0, s, p, start, n, v);
 }
 
 size_t grid_temp_words(uint32_t n, uint32_t cells) {

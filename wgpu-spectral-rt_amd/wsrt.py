@@ -140,6 +140,7 @@ _SIGS = {
     "wsrt_debug_read_paths": (C.c_int, [vp, vp, vp]),
     "wsrt_debug_trace_paths": (C.c_int, [vp, vp, C.c_uint32, vp, vp]),
     "wsrt_photon_grid_build": (C.c_int, [vp, vp, u32, u32, C.c_float, u32, vp, vp, vp, vp]),
+    "wsrt_photon_grid_build_noalloc": (C.c_int, [vp, vp, u32, u32, C.c_float, u32, vp, vp, vp, vp, vp]),
     "wsrt_debug_closest_hit": (C.c_int, [vp, vp, u32, vp, vp]),
     "wsrt_debug_any_hit": (C.c_int, [vp, vp, u32, vp]),
     "wsrt_debug_traversal_counts": (C.c_int, [vp, vp, u32, vp, vp]),
