@@ -167,8 +167,28 @@ typedef struct wsrt_config {
                                    * them; 0: every bounce in the wavefront.
                                    * Not used with guiding. Results do not
                                    * depend on it.                             */
-    uint32_t reserved[3];
+    uint32_t quirks;              /* WSRT_FIX_* bits; 0 renders as the
+                                   * reference does                          */
+    uint32_t reserved[2];
 } wsrt_config;
+
+/* wsrt_config.quirks: each bit replaces one behaviour of the reference's
+ * shader by the textbook estimator (default 0: the reference's behaviour).
+ *   WSRT_FIX_EMISSION_MIS  shader.wgsl:601-628 computes a balance-heuristic
+ *     weight for emission reached by a cosine-sampled bounce but does not
+ *     apply it, so direct light is counted by both strategies. The bit
+ *     applies prev_pdf / (prev_pdf + light_pdf), with light_pdf = t^2 /
+ *     (A cos_light n_lights) of the hit emitter triangle's world-space area
+ *     A (the reference's unapplied weight uses the object-space area); the
+ *     two strategies' weights then sum to one. Not with guiding.
+ *   WSRT_FIX_RR_CLAMP  shader.wgsl:696-700 divides by a survival probability
+ *     max(throughput) that may exceed 1; the bit clamps it to 1.
+ *   WSRT_FIX_ONB_SIGN  shader.wgsl:171-188 uses WGSL sign(0) = 0, so the
+ *     basis of a normal with z == 0 is infinite and GGX paths there are
+ *     absorbed; the bit takes sign(0) = +1. */
+#define WSRT_FIX_EMISSION_MIS 1u
+#define WSRT_FIX_RR_CLAMP 2u
+#define WSRT_FIX_ONB_SIGN 4u
 
 /* Fills *cfg with the reference's defaults. */
 int wsrt_config_default(wsrt_config* cfg);

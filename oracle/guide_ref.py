@@ -122,7 +122,8 @@ def loss_grad_out(out, dirs, weights, norm_factor):
     p = (pi * v).sum(axis=1, keepdims=True)
     wt = np.asarray(weights, np.float64).reshape(-1, 1) * norm_factor
     r = np.where(wt != 0, pi * v / np.maximum(p, 1e-30), 0.0)
-    dlk = 1 / kap - 2 / np.expm1(2 * kap) + (c - 1)
+    # 2 / (e^(2 kappa) - 1), written so that a large kappa does not overflow
+    dlk = 1 / kap - 2 * np.exp(-2 * kap) / -np.expm1(-2 * kap) + (c - 1)
     in_range = (sv > LOG_KAPPA_MIN) & (sv < LOG_KAPPA_MAX)
     gs = np.where(in_range, -wt * r * kap * dlk * KAPPA_SCALE, 0.0)
     gm = (-wt * r * kap / ln)[:, :, None] * (w - mu * c[:, :, None])

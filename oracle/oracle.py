@@ -45,7 +45,7 @@ class Config(C.Structure):
                 ("guide_mix", C.c_float), ("guide_lr", C.c_float),
                 ("guide_train_samples", C.c_uint32), ("env_map", C.c_uint32),
                 ("frame_graphs", C.c_uint32), ("frames_in_flight", C.c_uint32),
-                ("tail_paths", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("tail_paths", C.c_uint32), ("quirks", C.c_uint32), ("reserved", C.c_uint32 * 2)]
 
 
 def default_config(**kw) -> Config:
@@ -142,14 +142,15 @@ def _load(fp64: bool = False):
         "or_frame_render": (None, [vp, f32p, vp]),
         "or_frame_gather": (None, [vp, f32p, f32p, f32p, f32p]),
         "or_unit_vectors_f": (None, [C.c_uint32, C.c_uint32, f32p]),
-        "or_sample_bsdf_f": (C.c_int, [f32p, f32p, vp, C.c_int, f32p, f32p, C.c_uint32, f32p, f32p,
-                                       C.POINTER(C.c_uint32)]),
+        "or_sample_bsdf_f": (C.c_int, [f32p, f32p, vp, C.c_int, f32p, f32p, C.c_uint32, C.c_uint32, f32p,
+                                       f32p, C.POINTER(C.c_uint32)]),
         "or_sample_light_f": (C.c_int, [vp, f32p, f32p, C.c_uint32, f32p, C.POINTER(_C.c_float),
                                         C.POINTER(_C.c_float), f32p, C.POINTER(C.c_uint32)]),
         "or_photon_emit_f": (None, [vp, C.c_uint32, C.c_uint32, f32p, f32p, C.POINTER(_C.c_float),
                                     C.POINTER(_C.c_float)]),
         "or_ray_color_f": (None, [vp, f32p, f32p, f32p, C.c_uint32, f32p]),
-        "or_roulette_f": (C.c_int, [f32p, C.c_uint32, f32p, C.POINTER(C.c_uint32)]),
+        "or_tone_map_f": (None, [f32p, C.c_uint32, u8p]),
+        "or_roulette_f": (C.c_int, [f32p, C.c_uint32, C.c_uint32, f32p, C.POINTER(C.c_uint32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -343,22 +344,33 @@ def material(color, extra) -> np.ndarray:
     return m
 
 
-def sample_bsdf(rdir, ff, mat, front, wl, tp, rng):
+FIX_EMISSION_MIS, FIX_RR_CLAMP, FIX_ONB_SIGN = 1, 2, 4  # wsrt_config.quirks bits
+
+
+def sample_bsdf(rdir, ff, mat, front, wl, tp, rng, quirks=0):
     """One call of sample_bsdf: (absorbed, direction, attenuation, rng after)."""
     f = np.float32
     d = np.zeros(3, f)
     a = np.zeros(4, f)
     r = C.c_uint32()
     ab = lib.or_sample_bsdf_f(np.asarray(rdir, f), np.asarray(ff, f), _ptr(mat), int(front), np.asarray(wl, f),
-                              np.asarray(tp, f), rng, d, a, C.byref(r))
+                              np.asarray(tp, f), rng, quirks, d, a, C.byref(r))
     return bool(ab), d, a, r.value
 
 
-def roulette(tp, rng):
+def tone_map(accum) -> np.ndarray:
+    """RGBA8 of accumulated pixels (n x 4 floats), shader.wgsl:809-822."""
+    a = np.ascontiguousarray(accum, np.float32).reshape(-1, 4)
+    out = np.zeros((len(a), 4), np.uint8)
+    lib.or_tone_map_f(a, len(a), out)
+    return out
+
+
+def roulette(tp, rng, quirks=0):
     """One Russian roulette step: (survived, throughput after, rng after)."""
     out = np.zeros(4, np.float32)
     r = C.c_uint32()
-    alive = lib.or_roulette_f(np.asarray(tp, np.float32), rng, out, C.byref(r))
+    alive = lib.or_roulette_f(np.asarray(tp, np.float32), rng, quirks, out, C.byref(r))
     return bool(alive), out, r.value
 
 

@@ -1123,8 +1123,8 @@ static light_sample sample_light(const or_scene* s, v3 hp, v4 wl, uint32_t* rng)
 }
 
 /* shader.wgsl:171-188, WGSL sign(0) = 0 */
-static void onb(v3 n, v3* right, v3* up) {
-    float sg = n.z > 0.0f ? 1.0f : (n.z < 0.0f ? -1.0f : 0.0f);
+static void onb(v3 n, v3* right, v3* up, int sign_fix) {
+    float sg = n.z > 0.0f ? 1.0f : (n.z < 0.0f ? -1.0f : (sign_fix ? 1.0f : 0.0f));
     float a = -1.0f / (sg + n.z);
     float b = n.x * n.y * a;
     *right = V3(1.0f + sg * n.x * n.x * a, sg * b, -sg * n.x);
@@ -1140,7 +1140,7 @@ typedef struct { v3 dir; v4 att; int absorbed; } scatter_t;
 
 /* shader.wgsl:347-471 */
 static scatter_t sample_bsdf(v3 rdir, v3 ff, const wsrt_material* m, int front, v4 wl, v4 tp,
-                             uint32_t* rng) {
+                             uint32_t* rng, uint32_t quirks) {
     scatter_t rec;
     rec.absorbed = 0;
     rec.dir = V3(0, 0, 0);
@@ -1159,7 +1159,7 @@ static scatter_t sample_bsdf(v3 rdir, v3 ff, const wsrt_material* m, int front, 
         float st = sqrtf(maxf(0.0f, 1.0f - ct * ct));
         v3 ht = V3(cosf(phi) * st, sinf(phi) * st, ct);
         v3 rt, up;
-        onb(ff, &rt, &up);
+        onb(ff, &rt, &up, (quirks & WSRT_FIX_ONB_SIGN) != 0);
         v3 h = normalize3(add3(add3(mul3s(rt, ht.x), mul3s(up, ht.y)), mul3s(ff, ht.z)));
         v3 l = reflect3(neg3(v), h);
         float ndl = dot3(ff, l), ndv = dot3(ff, v);
@@ -1266,16 +1266,17 @@ static v4 env_radiance(const or_scene* s, v3 d, v4 wl) {
 /* Russian roulette (shader.wgsl:696-700): survival probability max(tp),
  * not clamped to 1 (so a throughput above 1 always survives and is divided
  * down); returns 0 when the path ends. */
-static int roulette(v4* tp, uint32_t* rng) {
+static int roulette(v4* tp, uint32_t* rng, uint32_t quirks) {
     float ps = maxf(maxf(tp->x, tp->y), maxf(tp->z, tp->w));
+    if (quirks & WSRT_FIX_RR_CLAMP) ps = minf(ps, 1.0f);
     if (or_rand(rng) > ps) return 0;
     *tp = div4s(*tp, ps);
     return 1;
 }
 
-int or_roulette_f(const float* tp4, uint32_t rng, float* tp_out4, uint32_t* rng_out) {
+int or_roulette_f(const float* tp4, uint32_t rng, uint32_t quirks, float* tp_out4, uint32_t* rng_out) {
     v4 tp = V4(tp4[0], tp4[1], tp4[2], tp4[3]);
-    int alive = roulette(&tp, &rng);
+    int alive = roulette(&tp, &rng, quirks);
     tp_out4[0] = tp.x; tp_out4[1] = tp.y; tp_out4[2] = tp.z; tp_out4[3] = tp.w;
     *rng_out = rng;
     return alive;
@@ -1287,6 +1288,7 @@ static v4 ray_color(const or_frame* f, v3 o, v3 d, v4 wl, uint32_t* rng) {
     const wsrt_config* c = &f->cfg;
     v4 acc = V4s(0.0f), tp = V4s(1.0f);
     int has_diffuse = 0, caustic = 0;
+    float prev_pdf = 0.0f; /* cosine pdf of the last Lambert scattering; 0 after a specular one */
     for (uint32_t i = 0; i < c->max_depth; ++i) {
         hit_t h = scene_closest(s, o, d, 0.0001f, 100.0f, 0);
         if (h.tri == NO_HIT) {
@@ -1300,7 +1302,20 @@ static v4 ray_color(const or_frame* f, v3 o, v3 d, v4 wl, uint32_t* rng) {
         if (type == 3u) {
             if (caustic) break;
             v4 em = mul4s(or_blackbody4(wl, m->color[0]), m->color[1]);
-            if (h.front || m->extra[1] > 0.0f) acc = add4(acc, mul4(em, tp));
+            if (h.front || m->extra[1] > 0.0f) {
+                if ((c->quirks & WSRT_FIX_EMISSION_MIS) && prev_pdf > 0.0f) {
+                    /* shader.wgsl:601-626's balance weight, applied, with the
+                     * world-space area of the hit triangle */
+                    v3 e1 = V3(s->e1[3 * h.tri], s->e1[3 * h.tri + 1], s->e1[3 * h.tri + 2]);
+                    v3 e2 = V3(s->e2[3 * h.tri], s->e2[3 * h.tri + 1], s->e2[3 * h.tri + 2]);
+                    float area = length3(cross3(e1, e2)) * 0.5f;
+                    float cl = maxf(dot3(ff, neg3(d)), 0.0001f);
+                    float lp = (h.t * h.t) / (area * cl * (float)s->nlights);
+                    acc = add4(acc, mul4s(mul4(em, tp), prev_pdf / (prev_pdf + lp)));
+                } else {
+                    acc = add4(acc, mul4(em, tp));
+                }
+            }
             break;
         }
         if (type == 0u) {
@@ -1327,12 +1342,13 @@ static v4 ray_color(const or_frame* f, v3 o, v3 d, v4 wl, uint32_t* rng) {
                 }
             }
         }
-        scatter_t sc = sample_bsdf(d, ff, m, h.front, wl, tp, rng);
+        scatter_t sc = sample_bsdf(d, ff, m, h.front, wl, tp, rng, c->quirks);
         if (sc.absorbed) break;
+        prev_pdf = type == 0u ? maxf(dot3(ff, sc.dir), 0.0001f) / OR_PI : 0.0f;
         tp = mul4(tp, sc.att);
         o = hp;
         d = sc.dir;
-        if (i > c->rr_after && !roulette(&tp, rng)) break;
+        if (i > c->rr_after && !roulette(&tp, rng, c->quirks)) break;
     }
     return acc;
 }
@@ -1439,6 +1455,17 @@ void or_frame_trace_pixels(const or_frame* f, const uint32_t* px, uint32_t n, fl
 }
 
 /* shader.wgsl:801-822 */
+/* shader.wgsl:809-822: mean, clamp at 0, Reinhard on luma, RGBA8 without
+ * gamma. */
+static void tone_map(const float* na, uint8_t* out) {
+    v3 fc = V3(maxf(na[0] / na[3] * 1.0f, 0.0f), maxf(na[1] / na[3] * 1.0f, 0.0f), maxf(na[2] / na[3] * 1.0f, 0.0f));
+    float luma = dot3(fc, V3(0.2126f, 0.7152f, 0.0722f));
+    float ml = luma / (luma + 1.0f);
+    if (luma > 0.00001f) fc = mul3s(fc, ml / luma);
+    float ch[4] = {fc.x, fc.y, fc.z, 1.0f};
+    for (int k = 0; k < 4; ++k) out[k] = (uint8_t)lrintf(clampf(ch[k], 0.0f, 1.0f) * 255.0f);
+}
+
 void or_frame_render(const or_frame* f, float* accum, uint8_t* rgba8) {
     int64_t npx = (int64_t)f->W * f->H;
 #pragma omp parallel for schedule(dynamic, 64)
@@ -1449,16 +1476,12 @@ void or_frame_render(const or_frame* f, float* accum, uint8_t* rgba8) {
         v4 cur = f->cam.frame_count > 0 ? V4(a[0], a[1], a[2], a[3]) : V4s(0.0f);
         v4 na = add4(cur, V4(c.x, c.y, c.z, 1.0f));
         a[0] = na.x; a[1] = na.y; a[2] = na.z; a[3] = na.w;
-        if (rgba8) {
-            v3 fc = V3(maxf(na.x / na.w * 1.0f, 0.0f), maxf(na.y / na.w * 1.0f, 0.0f),
-                       maxf(na.z / na.w * 1.0f, 0.0f));
-            float luma = dot3(fc, V3(0.2126f, 0.7152f, 0.0722f));
-            float ml = luma / (luma + 1.0f);
-            if (luma > 0.00001f) fc = mul3s(fc, ml / luma);
-            float ch[4] = {fc.x, fc.y, fc.z, 1.0f};
-            for (int k = 0; k < 4; ++k) rgba8[4 * p + k] = (uint8_t)lrintf(clampf(ch[k], 0.0f, 1.0f) * 255.0f);
-        }
+        if (rgba8) tone_map(a, rgba8 + 4 * p);
     }
+}
+
+void or_tone_map_f(const float* accum, uint32_t n, uint8_t* rgba8) {
+    for (uint32_t p = 0; p < n; ++p) tone_map(accum + 4 * p, rgba8 + 4 * p);
 }
 
 void or_frame_gather(const or_frame* f, const float* p, const float* n, const float* wl, float* out) {
@@ -1540,9 +1563,11 @@ void or_unit_vectors_f(uint32_t seed, uint32_t n, float* out) {
 }
 
 int or_sample_bsdf_f(const float* rdir, const float* ffn, const wsrt_material* m, int front, const float* wl4,
-                     const float* tp4, uint32_t rng, float* dir_out, float* att_out, uint32_t* rng_out) {
+                     const float* tp4, uint32_t rng, uint32_t quirks, float* dir_out, float* att_out,
+                     uint32_t* rng_out) {
     scatter_t sc = sample_bsdf(V3(rdir[0], rdir[1], rdir[2]), V3(ffn[0], ffn[1], ffn[2]), m, front,
-                               V4(wl4[0], wl4[1], wl4[2], wl4[3]), V4(tp4[0], tp4[1], tp4[2], tp4[3]), &rng);
+                               V4(wl4[0], wl4[1], wl4[2], wl4[3]), V4(tp4[0], tp4[1], tp4[2], tp4[3]), &rng,
+                               quirks);
     dir_out[0] = sc.dir.x; dir_out[1] = sc.dir.y; dir_out[2] = sc.dir.z;
     att_out[0] = sc.att.x; att_out[1] = sc.att.y; att_out[2] = sc.att.z; att_out[3] = sc.att.w;
     *rng_out = rng;

@@ -124,6 +124,8 @@ void or_frame_trace_pixels(const or_frame* f, const uint32_t* pixels, uint32_t n
 /* Full frame: accum (width*height*4) updated as shader.wgsl:801-808;
  * rgba8 optional. */
 void or_frame_render(const or_frame* f, float* accum, uint8_t* rgba8);
+/* Tone map of n accumulated pixels (shader.wgsl:809-822) to RGBA8. */
+void or_tone_map_f(const float* accum, uint32_t n, uint8_t* rgba8);
 /* Photon gather at a point (shader.wgsl:297-342) against the frame's grid. */
 void or_frame_gather(const or_frame* f, const float* pos3, const float* n3, const float* wl4,
                      float* out4);
@@ -133,11 +135,11 @@ void or_frame_gather(const or_frame* f, const float* pos3, const float* n3, cons
  * seeded with pcg_hash(seed). */
 void or_unit_vectors_f(uint32_t seed, uint32_t n, float* out3);
 /* sample_bsdf (shader.wgsl:347-471) for incoming direction rdir at a hit
- * with forward-facing normal ffn; rng is the state before the call. Returns
- * 1 when the path is absorbed. */
+ * with forward-facing normal ffn; rng is the state before the call; quirks
+ * as wsrt_config.quirks. Returns 1 when the path is absorbed. */
 int or_sample_bsdf_f(const float* rdir, const float* ffn, const wsrt_material* m, int front,
-                     const float* wl4, const float* tp4, uint32_t rng, float* dir_out3,
-                     float* att_out4, uint32_t* rng_out);
+                     const float* wl4, const float* tp4, uint32_t rng, uint32_t quirks,
+                     float* dir_out3, float* att_out4, uint32_t* rng_out);
 /* sample_random_light (shader.wgsl:477-528) from point hp. Returns 1 for a
  * point light (a delta distribution: pdf is then the distance squared over
  * the light count). pdf 0: no sample. */
@@ -149,7 +151,7 @@ void or_photon_emit_f(const or_scene* s, uint32_t frame, uint32_t id, float* pos
                       float* wl, float* energy);
 /* Russian roulette of a path's throughput (shader.wgsl:696-700). Returns 1
  * when the path survives. */
-int or_roulette_f(const float* tp4, uint32_t rng, float* tp_out4, uint32_t* rng_out);
+int or_roulette_f(const float* tp4, uint32_t rng, uint32_t quirks, float* tp_out4, uint32_t* rng_out);
 /* ray_color (shader.wgsl:547-703) of one ray with given hero wavelengths
  * and rng state, in the frame's scene and photon grid. */
 void or_ray_color_f(const or_frame* f, const float* o3, const float* d3, const float* wl4,

@@ -184,6 +184,8 @@ def test_accumulated_image_matches_oracle(device, name):
     assert relmse <= 1e-4
     rgba = r.read_rgba8()
     assert rgba[:, 3].min() == 255 and rgba[:, :3].max() > 0
+    # the storage image (shader.wgsl:809-822) of the GPU's own accumulation
+    np.testing.assert_array_equal(rgba, O.tone_map(acc))
 
 
 def test_frame_count_semantics(device, cornell_pair):
@@ -407,3 +409,41 @@ def test_batch_rejects_photons_and_guiding(device, cornell_pair):
     cam = scenes.camera(scenes.DEFAULT_VIEW, 0)
     with pytest.raises(wsrt.WsrtError):
         wsrt.Batch(device, [(g, cam, 8, 8, 1)], wsrt.default_config(enable_photons=1))
+
+
+@pytest.mark.parametrize("name", ["cornell", "small_box"])
+def test_quirk_fixes_match_oracle(device, name):
+    """wsrt_config.quirks = all three WSRT_FIX_* bits: emission weighted by
+    the balance heuristic after a Lambert bounce, roulette clamped at 1, and
+    sign(0) = +1 in the GGX basis (the small box's GGX box tops and walls
+    have z == 0 normals). Paths match the oracle with the same bits, and
+    differ from the reference behaviour."""
+    print(f"quirk fixes {name}", flush=True)
+    W = H = 64
+    g, o, cfg, view = setup(device, name, W, H, 1 << 14)
+    out = {}
+    for q in (0, 7):
+        c = wsrt.Config.from_buffer_copy(bytes(cfg))
+        c.quirks = q
+        r = wsrt.Renderer(device, g, W, H, c)
+        cam = scenes.camera(view, 2)
+        r.set_camera(cam)
+        r.render()
+        wl, L = r.read_paths()
+        ph, _ = r.read_photons()
+        f = O.Frame(o, to_oracle_cfg(c), to_oracle_cam(cam), W, H, photons=ph)
+        owl, oL, _ = f.trace_pixels(np.arange(W * H, dtype=np.uint32))
+        floor = 1e-6 * max(float(np.abs(oL).mean()), 1e-12)
+        same = rel_close(L, oL, 1e-4, floor).mean()
+        print(f"  quirks {q}: paths within 1e-4 {same:.5f}", flush=True)
+        assert same >= 0.99
+        out[q] = L
+    assert np.mean(np.any(out[0] != out[7], axis=1)) > 0.01
+
+
+def test_quirk_bits_are_validated(device, cornell_pair):
+    g, _ = cornell_pair
+    with pytest.raises(wsrt.WsrtError):
+        wsrt.Renderer(device, g, 8, 8, wsrt.default_config(quirks=8))
+    with pytest.raises(wsrt.WsrtError):
+        wsrt.Renderer(device, g, 8, 8, wsrt.default_config(quirks=1, guiding=1))

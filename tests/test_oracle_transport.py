@@ -30,6 +30,13 @@ slip shared by both restatements fails:
   photon emission ...... photon_emit.wgsl:159-208 mean energy = pi A_tri I integral
                          of B over 380..780 nm (no light-count factor, Q6),
                          cosine directions, uniform points
+  quirk bits ........... wsrt_config.quirks: WSRT_FIX_EMISSION_MIS makes depth-2
+                         direct light equal the quadrature exactly (the two
+                         strategies' weights sum to one), WSRT_FIX_RR_CLAMP keeps
+                         a throughput above 1, WSRT_FIX_ONB_SIGN gives a z == 0
+                         normal an orthonormal basis (GGX albedo as at any normal)
+  tone map ............. shader.wgsl:809-822      gray v -> round(255 v / (1 + v)), clamp
+                         at 0 and 255, hue kept (channel ratios), alpha 255
   multi-bounce paths ... shader.wgsl:547-703      white furnace: a point light at
                          the centre of a closed Lambert sphere of albedo rho gives
                          rho B I / (pi R^2) (1 - rho^D) / (1 - rho), with and
@@ -187,6 +194,23 @@ def test_ggx_smooth_limit_is_a_mirror_with_schlick_fresnel():
     assert np.median(dev) < 4e-3 and np.mean(np.array(dev) < 2 * math.atan(3e-3) * 1.01) > 0.88
 
 
+def test_onb_sign_fix_gives_the_albedo_at_a_z0_normal():
+    """WSRT_FIX_ONB_SIGN: with sign(0) = +1 the basis of n = (1, 0, 0) is
+    orthonormal, so GGX at that normal reflects like at any other: viewed
+    along the normal, the mean weight is the directional albedo at 0 deg."""
+    a, f0 = 0.3, 0.8
+    m = O.material([f0, f0, f0, 1.0], [1, a, 0, 0])
+    n = np.array([1.0, 0.0, 0.0], np.float32)
+    w = []
+    for s in seeds(40000, 30):
+        ab, d, att, _ = O.sample_bsdf(-n, n, m, 1, WL, TP1, s, O.FIX_ONB_SIGN)
+        w.append(0.0 if ab else float(att[0]))
+        assert ab or d[0] > 0
+    w = np.array(w)
+    want = ggx_albedo(0.0, a, f0)
+    assert abs(w.mean() - want) < 4 * w.std() / math.sqrt(len(w)) + 1e-4, (w.mean(), want)
+
+
 def test_onb_sign_zero_absorbs_ggx_paths():
     """Q3: WGSL sign(0) = 0 makes the basis of a normal with z == 0 infinite,
     so every GGX sample at such a normal is absorbed; z != 0 is unaffected."""
@@ -303,12 +327,16 @@ def test_russian_roulette():
     out = np.array(out, np.float64)
     assert alive / len(out) == pytest.approx(0.6, abs=0.01)
     np.testing.assert_allclose(out.mean(0), tp, rtol=0.03)  # unbiased
-    # Q2: the survival probability is not clamped to 1
+    # Q2: the survival probability is not clamped to 1; WSRT_FIX_RR_CLAMP
+    # clamps it, and the throughput then passes unchanged
     big = np.array([2.0, 1.0, 0.5, 0.5], np.float32)
     for s in seeds(100, 14):
         a, t, _ = O.roulette(big, s)
         assert a
         np.testing.assert_allclose(t, big / 2.0)
+        a, t, _ = O.roulette(big, s, O.FIX_RR_CLAMP)
+        assert a
+        np.testing.assert_array_equal(t, big)
 
 
 # ---- light sampling, next-event estimation, emission -----------------------
@@ -370,14 +398,16 @@ def test_light_sample_pdf_is_solid_angle_density():
     np.testing.assert_allclose(rad, bb * INTENSITY, rtol=1e-6)
 
 
-@pytest.mark.parametrize("depth", [1, 2])
-def test_direct_light_against_quadrature(depth):
+@pytest.mark.parametrize("depth,fix", [(1, False), (2, False), (2, True)])
+def test_direct_light_against_quadrature(depth, fix):
     """Camera ray straight down onto the floor at (x0, 0, z0). Depth 1: only
     next-event estimation, weighted by the balance heuristic against the
     cosine pdf. Depth 2 also follows the cosine-sampled ray, and a hit on the
-    emitter adds its emission without a MIS weight (Q1)."""
+    emitter adds its emission without a MIS weight (Q1). With
+    WSRT_FIX_EMISSION_MIS it is weighted by the balance heuristic too, and
+    the estimate is the direct light itself."""
     s = emitter_scene()
-    cfg = O.default_config(max_depth=depth, enable_photons=0)
+    cfg = O.default_config(max_depth=depth, enable_photons=0, quirks=O.FIX_EMISSION_MIS if fix else 0)
     f = O.Frame(s, cfg, O.camera_uniform(), 4, 4)
     x0, z0 = 0.3, -0.2
     area = SIDE * SIDE
@@ -390,10 +420,12 @@ def test_direct_light_against_quadrature(depth):
     direct = light_integral(x0, z0)
     bb = np.array([O.lib.or_blackbody_f(float(w), T_K) for w in WL], np.float64)
     want = ALBEDO / math.pi * bb * INTENSITY * (nee + (direct if depth == 2 else 0.0))
+    if fix:
+        want = ALBEDO / math.pi * bb * INTENSITY * direct
     L = np.array([f.ray_color([x0, 0.5, z0], [0, -1, 0], WL, r) for r in seeds(60000, 16 + depth)], np.float64)
     m, e = L.mean(0), L.std(0) / math.sqrt(len(L))
     assert np.all(np.abs(m - want) < 4 * e + 1e-6 * want), (m, want, e)
-    if depth == 2:  # the unweighted BSDF hit over-counts: more than the plain direct light
+    if depth == 2 and not fix:  # the unweighted BSDF hit over-counts: more than the plain direct light
         assert np.all(m > ALBEDO / math.pi * bb * direct * 1.2)
 
 
@@ -445,3 +477,22 @@ def test_white_furnace_point_light_in_lambert_sphere(rr_after):
     want = rho * bb * INTENSITY / (math.pi * R * R) * (1 - rho**D) / (1 - rho)
     m, e = L.mean(0), L.std(0) / math.sqrt(len(L))
     assert np.all(np.abs(m - want) < 4 * e + 4e-3 * want), (m / want, e / want)
+
+
+def test_tone_map_is_reinhard_on_luma():
+    rng = np.random.default_rng(9)
+    v = rng.uniform(0.0, 5.0, 1000).astype(np.float32)
+    n = rng.integers(1, 100, 1000).astype(np.float32)
+    acc = np.stack([v * n, v * n, v * n, n], 1).astype(np.float32)
+    out = O.tone_map(acc)
+    want = np.round(255 * v.astype(np.float64) / (1 + v.astype(np.float64)))
+    assert np.all(np.abs(out[:, 0].astype(int) - want) <= 1)
+    assert np.all(out[:, 0] == out[:, 1]) and np.all(out[:, 1] == out[:, 2]) and np.all(out[:, 3] == 255)
+    # colour: luma l -> l / (1 + l), channels scaled alike (hue kept) until they clip
+    c = np.array([[0.2, 0.1, 0.05, 1.0]], np.float32)
+    l = 0.2126 * 0.2 + 0.7152 * 0.1 + 0.0722 * 0.05
+    s = 1 / (1 + l)
+    np.testing.assert_allclose(O.tone_map(c)[0, :3], np.round(255 * np.array([0.2, 0.1, 0.05]) * s), atol=1)
+    # negative means clamp to 0; a saturated channel clips at 255
+    assert O.tone_map(np.array([[-1.0, -1.0, -1.0, 1.0]], np.float32))[0, :3].tolist() == [0, 0, 0]
+    assert O.tone_map(np.array([[50.0, 0.0, 0.0, 1.0]], np.float32))[0, 0] == 255

@@ -793,6 +793,10 @@ static int validate_config(const wsrt_config* c) {
         (uint64_t)c->hash_size * c->frames_in_flight > (1ull << 30))
         return fail(WSRT_E_INVALID, "photons or cells times frames_in_flight too large");
     if (c->guiding > 1) return fail(WSRT_E_INVALID, "guiding %u not 0 or 1", c->guiding);
+    const uint32_t known = WSRT_FIX_EMISSION_MIS | WSRT_FIX_RR_CLAMP | WSRT_FIX_ONB_SIGN;
+    if (c->quirks & ~known) return fail(WSRT_E_INVALID, "quirks 0x%x has unknown bits", c->quirks);
+    if ((c->quirks & WSRT_FIX_EMISSION_MIS) && c->guiding)
+        return fail(WSRT_E_UNSUPPORTED, "WSRT_FIX_EMISSION_MIS is not available with guiding");
     if (c->guiding) {
         if (!(c->guide_mix >= 0.0f && c->guide_mix <= 1.0f))
             return fail(WSRT_E_INVALID, "guide_mix %g not in [0, 1]", (double)c->guide_mix);
@@ -838,6 +842,7 @@ static FrameConst make_frame_const(const wsrt_config& c, const wsrt_camera_unifo
     fc.rr_after = c.rr_after;
     fc.enable_photons = c.enable_photons;
     fc.n_photons = c.enable_photons ? c.photons_per_pass : 0u;
+    fc.quirks = c.quirks;
     mat4_mul(cam.view_inverse, cam.proj_inverse, fc.M);
     fc.origin[0] = cam.view_inverse[12];
     fc.origin[1] = cam.view_inverse[13];

@@ -63,7 +63,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
             const FrameConst& fc = a.table[J.table_base + f];
             PathRegs r;
             camera_path(fc, x, y, r);
-            trace_bounces(sv, none, nullptr, 0u, fc.rr_after, 0u, a.max_depth, r, st, false, tally);
+            trace_bounces(sv, none, nullptr, 0u, fc.rr_after, fc.quirks, 0u, a.max_depth, r, st, false, tally);
             acc = accumulate_sample(acc, fc, r.wl, r.L);
         }
         J.accum[pix] = acc;

@@ -167,6 +167,7 @@ struct FrameConst {
     float M[16];               // view_inverse * proj_inverse
     float origin[3];           // camera position (view_inverse column 3)
     uint32_t count_traversal;  // 1: count node visits / triangle tests
+    uint32_t quirks;           // wsrt_config.quirks (WSRT_FIX_* bits)
 };
 
 constexpr uint32_t kFrameTable = 4096;  // entries; frames wrap modulo this

@@ -36,6 +36,9 @@ constexpr uint32_t kFlagRecorded = 4u;
 
 __device__ __forceinline__ short bf16_bits(float f) {  // round to nearest even
     uint32_t u = __float_as_uint(f);
+    // a NaN stays a (quiet) NaN: rounding its payload could carry into the
+    // exponent and sign
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (short)((u >> 16) | 0x40u);
     u += 0x7FFFu + ((u >> 16) & 1u);
     return (short)(u >> 16);
 }

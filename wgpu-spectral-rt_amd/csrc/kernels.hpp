@@ -16,15 +16,19 @@ namespace wsrt {
 // fixed size): 4096 x 128 lanes = 32 waves on each of the 256 CUs.
 constexpr int kTraceThreads = 128;
 constexpr int kTraceBlocks = 4096;
+// Compute units of the current device (cached per device ordinal).
+int device_cus();
 // Grid of a traversal kernel whose register budget keeps `waves` waves per
-// SIMD resident: exactly one round of waves on the 256 CUs x 4 SIMDs (blocks
-// of 2 waves), so no partly filled second round trails the launch. With 5
-// waves per SIMD this runs the caustic workload's extend kernel 4 % faster
-// than the full 4096-block grid. The block count is a multiple of the queue
-// segment count, which the segment capacity bound needs.
-constexpr int kCUs = 256;
-constexpr int trace_grid(int waves) { return kCUs * 4 * waves / (kTraceThreads / 64); }
-static_assert(trace_grid(6) <= kTraceBlocks, "resident traversal grids fit the stack spill area");
+// SIMD resident: exactly one round of waves on the device's CUs x 4 SIMDs
+// (blocks of 2 waves; 256 CUs on MI355X), so no partly filled second round
+// trails the launch. With 5 waves per SIMD this runs the caustic workload's
+// extend kernel 4 % faster than the full 4096-block grid. The block count is
+// a multiple of the queue segment count, which the segment capacity bound
+// needs, and at most kTraceBlocks (the stack spill area).
+inline int trace_grid(int waves) {
+    const int g = device_cus() * 4 * waves / (kTraceThreads / 64) / kSegs * kSegs;
+    return g < kSegs ? kSegs : (g > kTraceBlocks ? kTraceBlocks : g);
+}
 constexpr uint64_t kTraceLanes = (uint64_t)kTraceThreads * kTraceBlocks;
 // Register budgets of the traversal kernels, as waves per SIMD the compiler
 // must fit (gfx950: 512 VGPRs per lane and SIMD). Left free, the compiler

@@ -148,6 +148,7 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
     if (tail_started(qs, bounce)) return;
     const uint32_t base_frame = *fa.frame;
     const uint32_t rr_after = fa.table[base_frame & fa.mask].rr_after;
+    const uint32_t quirks = fa.table[base_frame & fa.mask].quirks;
     const uint32_t photons_on = fa.table[base_frame & fa.mask].enable_photons;
     const uint32_t total = seg_prefix(qs, bounce, kQShade0, kMatBins, pre);
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
             const uint32_t slot = p / npx;
             const FrameConst* fs = photons_on ? &fa.table[(base_frame + slot) & fa.mask] : nullptr;
             const uint32_t cell_base = photons_on ? slot * fs->hash_size : 0u;
-            vo = shade_vertex(sv, grid, fs, cell_base, rr_after, bounce, guided != 0u,
+            vo = shade_vertex(sv, grid, fs, cell_base, rr_after, quirks, bounce, guided != 0u,
                               __uint_as_float(hr.x), hr.y & 0x7FFFFFFFu, (hr.y >> 31) != 0u,
                               __uint_as_float(hr.z), __uint_as_float(hr.w), r);
             ps.L[p] = r.L;
@@ -265,6 +266,7 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(4) void k_tail(S
     const uint32_t* queue = (first & 1u) ? ps.queue_b : ps.queue_a;
     const uint32_t base_frame = *fa.frame;
     const uint32_t rr_after = fa.table[base_frame & fa.mask].rr_after;
+    const uint32_t quirks = fa.table[base_frame & fa.mask].quirks;
     const uint32_t photons_on = fa.table[base_frame & fa.mask].enable_photons;
     const bool count = count_traversal != 0u;
     RayTally tally;
@@ -281,7 +283,7 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(4) void k_tail(S
         const uint32_t slot = p / npx;
         const FrameConst* fs = photons_on ? &fa.table[(base_frame + slot) & fa.mask] : nullptr;
         const uint32_t cell_base = photons_on ? slot * fs->hash_size : 0u;
-        trace_bounces(sv, grid, fs, cell_base, rr_after, first, max_depth, r, st, count, tally);
+        trace_bounces(sv, grid, fs, cell_base, rr_after, quirks, first, max_depth, r, st, count, tally);
         ps.L[p] = r.L;
     }
     // statistics: rays per wave, traversal work while counting

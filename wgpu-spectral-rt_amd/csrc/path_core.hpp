@@ -75,10 +75,13 @@ struct VertexOut {
 
 // One hit of a camera path (shader.wgsl:568-701): emission, photon gather
 // (fs non-null), next-event estimation, BSDF sampling, Russian roulette.
+// r.d4.w carries the cosine pdf of the last Lambert scattering (0 after a
+// camera ray or a specular bounce) when the WSRT_FIX_EMISSION_MIS bit is set.
 __device__ __forceinline__ VertexOut shade_vertex(const SceneView& sv, const GridView& grid,
                                                   const FrameConst* fs, uint32_t cell_base,
-                                                  uint32_t rr_after, uint32_t bounce, bool guided, float t,
-                                                  uint32_t tri, bool front, float hu, float hv, PathRegs& r) {
+                                                  uint32_t rr_after, uint32_t quirks, uint32_t bounce, bool guided,
+                                                  float t, uint32_t tri, bool front, float hu, float hv,
+                                                  PathRegs& r) {
     VertexOut out;
     out.alive = out.shadow = out.deferred = false;
     out.hp = out.sdir = out.ff = mk3(0, 0, 0);
@@ -96,7 +99,16 @@ __device__ __forceinline__ VertexOut shade_vertex(const SceneView& sv, const Gri
     if (type == 3u) {  // emitter (shader.wgsl:586-631); the MIS weight is not applied
         if (!(r.flags & kFlagCaustic)) {
             float4 em = mul4s(blackbody4(wl, col.x), col.y);
-            if (front || ex.y > 0.0f) r.L = add4(r.L, mul4(em, r.tp));
+            if (front || ex.y > 0.0f) {
+                if ((quirks & WSRT_FIX_EMISSION_MIS) && r.d4.w > 0.0f) {
+                    // balance heuristic against the light sample's pdf
+                    const float cl = fmaxf(dot(ff, -d), 0.0001f);
+                    const float lp = (t * t) / (world_tri_area(sv, tri) * cl * (float)sv.n_lights);
+                    r.L = add4(r.L, mul4s(mul4(em, r.tp), r.d4.w / (r.d4.w + lp)));
+                } else {
+                    r.L = add4(r.L, mul4(em, r.tp));
+                }
+            }
         }
         return out;
     }
@@ -132,14 +144,18 @@ __device__ __forceinline__ VertexOut shade_vertex(const SceneView& sv, const Gri
             return out;
         }
     }
-    Scatter sc = sample_bsdf(d, ff, m, front, wl, r.tp, r.rng);
+    Scatter sc = sample_bsdf(d, ff, m, front, wl, r.tp, r.rng, quirks);
     if (!sc.absorbed) {
         r.tp = mul4(r.tp, sc.att);
         r.o4 = f4(hp, 0.0f);
-        r.d4 = f4(sc.dir, 0.0f);
+        const float prev_pdf = ((quirks & WSRT_FIX_EMISSION_MIS) && type == 0u)
+                                   ? fmaxf(dot(ff, sc.dir), 0.0001f) / kPi
+                                   : 0.0f;
+        r.d4 = f4(sc.dir, prev_pdf);
         out.alive = true;
         if (bounce > rr_after) {
             float pr = fmaxf(fmaxf(r.tp.x, r.tp.y), fmaxf(r.tp.z, r.tp.w));
+            if (quirks & WSRT_FIX_RR_CLAMP) pr = fminf(pr, 1.0f);
             if (rand01(r.rng) > pr) out.alive = false;
             else r.tp = div4s(r.tp, pr);
         }
@@ -158,7 +174,7 @@ struct RayTally {
 // vertex, and its shadow ray. The same steps, in the same order, as one
 // path's share of the wavefront kernels extend, shade and shadow.
 __device__ __forceinline__ void trace_bounces(const SceneView& sv, const GridView& grid, const FrameConst* fs,
-                                              uint32_t cell_base, uint32_t rr_after, uint32_t first,
+                                              uint32_t cell_base, uint32_t rr_after, uint32_t quirks, uint32_t first,
                                               uint32_t max_depth, PathRegs& r, Stack st, bool count,
                                               RayTally& tally) {
     for (uint32_t b = first; b < max_depth; ++b) {
@@ -170,7 +186,7 @@ __device__ __forceinline__ void trace_bounces(const SceneView& sv, const GridVie
             break;
         }
         const VertexOut vo =
-            shade_vertex(sv, grid, fs, cell_base, rr_after, b, false, h.t, h.tri, h.front, h.u, h.v, r);
+            shade_vertex(sv, grid, fs, cell_base, rr_after, quirks, b, false, h.t, h.tri, h.front, h.u, h.v, r);
         if (vo.shadow) {
             const Hit s = trace_ray<true>(sv, vo.hp, vo.sdir, 0.001f, vo.stmax, st, count, tally.shadow_nodes,
                                           tally.shadow_tris);

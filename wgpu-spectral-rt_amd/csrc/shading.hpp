@@ -17,6 +17,22 @@ __device__ __forceinline__ FrameConst frame_const(const FrameArgs& fa) {
 
 // shader.wgsl:530-545 and :573-579: interpolated object normal, world normal
 // through the transposed world-to-object matrix.
+// World-space area of world triangle tri: its object-space edges through
+// the linear part of the instance transform.
+__device__ __forceinline__ float world_tri_area(const SceneView& sv, uint32_t tri) {
+    const Instance& in = sv.instances[sv.tri_instance[tri]];
+    const wsrt_mesh_info mi = sv.meshes[in.mesh];
+    const uint32_t io = mi.index_offset + (tri - in.first_tri) * 3u;
+    const float4 p0 = *reinterpret_cast<const float4*>(sv.verts[sv.indices[io] + mi.vertex_offset].pos);
+    const float4 p1 = *reinterpret_cast<const float4*>(sv.verts[sv.indices[io + 1] + mi.vertex_offset].pos);
+    const float4 p2 = *reinterpret_cast<const float4*>(sv.verts[sv.indices[io + 2] + mi.vertex_offset].pos);
+    const float* M = in.m;
+    const f3 c0 = mk3(M[0], M[1], M[2]), c1 = mk3(M[4], M[5], M[6]), c2 = mk3(M[8], M[9], M[10]);
+    const f3 a = xyz(p1) - xyz(p0), b = xyz(p2) - xyz(p0);
+    const f3 wa = c0 * a.x + c1 * a.y + c2 * a.z, wb = c0 * b.x + c1 * b.y + c2 * b.z;
+    return length(cross(wa, wb)) * 0.5f;
+}
+
 __device__ __forceinline__ f3 hit_world_normal(const SceneView& sv, uint32_t tri, float u, float v,
                                                uint32_t* material) {
     const uint32_t ii = sv.tri_instance[tri];
@@ -101,9 +117,10 @@ __device__ __forceinline__ LightSample sample_light(const SceneView& sv, f3 hp, 
     return ls;
 }
 
-// shader.wgsl:171-188, WGSL sign(0) = 0 (normals with z == 0 give a NaN basis).
-__device__ __forceinline__ void onb(f3 n, f3& right, f3& up) {
-    float sg = n.z > 0.0f ? 1.0f : (n.z < 0.0f ? -1.0f : 0.0f);
+// shader.wgsl:171-188, WGSL sign(0) = 0 (normals with z == 0 give a NaN
+// basis), or sign(0) = +1 with sign_fix (WSRT_FIX_ONB_SIGN).
+__device__ __forceinline__ void onb(f3 n, f3& right, f3& up, bool sign_fix) {
+    float sg = n.z > 0.0f ? 1.0f : (n.z < 0.0f ? -1.0f : (sign_fix ? 1.0f : 0.0f));
     float a = -1.0f / (sg + n.z);
     float b = n.x * n.y * a;
     right = mk3(1.0f + sg * n.x * n.x * a, sg * b, -sg * n.x);
@@ -123,7 +140,7 @@ struct Scatter {
 
 // shader.wgsl:347-471
 __device__ __forceinline__ Scatter sample_bsdf(f3 rdir, f3 ff, const wsrt_material& m, bool front,
-                                               float4 wl, float4 tp, uint32_t& rng) {
+                                               float4 wl, float4 tp, uint32_t& rng, uint32_t quirks) {
     Scatter rec;
     rec.absorbed = false;
     rec.dir = mk3(0, 0, 0);
@@ -145,7 +162,7 @@ __device__ __forceinline__ Scatter sample_bsdf(f3 rdir, f3 ff, const wsrt_materi
         float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
         f3 ht = mk3(cosf(phi) * st, sinf(phi) * st, ct);
         f3 rt, up;
-        onb(ff, rt, up);
+        onb(ff, rt, up, (quirks & WSRT_FIX_ONB_SIGN) != 0u);
         f3 h = normalize(rt * ht.x + up * ht.y + ff * ht.z);
         f3 l = reflect(-v, h);
         float ndl = dot(ff, l), ndv = dot(ff, v);

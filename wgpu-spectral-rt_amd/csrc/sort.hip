@@ -255,6 +255,18 @@ __global__ void k_iota(uint32_t* __restrict__ p, uint32_t n) {
 
 }  // namespace
 
+int device_cus() {
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cached[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cached[dev] = n;
+    }
+    return cached[dev];
+}
+
 size_t scan_temp_words(uint32_t n) {
     uint32_t blocks = (n + kScanTile - 1) / kScanTile;
     return (size_t)blocks + 1;
