@@ -1204,7 +1204,13 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     pl.qs.tail_paths = c.tail_paths;
     pl.guided = c.guiding;
     if (c.guiding) {
-        const uint64_t nrec = npaths * guide::kRecBounces;
+        const uint32_t want = (uint32_t)std::min<uint64_t>(c.guide_train_samples, npx * guide::kRecBounces);
+        // whole chunks of samples per step (at least one step of 64)
+        const uint32_t ns = want >= kGuideBatch   ? want / kGuideBatch * kGuideBatch
+                            : want >= kGuideChunk ? want / kGuideChunk * kGuideChunk
+                                                  : (want + 63u) & ~63u;
+        // the records a frame's training reads, per frame slot
+        const uint64_t nrec = (uint64_t)ns * r->S;
         HIP_OR_FAIL(r->g_in0.alloc(16 * nrec));
         HIP_OR_FAIL(r->g_in1.alloc(16 * nrec));
         HIP_OR_FAIL(r->g_dir.alloc(16 * nrec));
@@ -1218,11 +1224,6 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
         g->dev = dev;
         g->stream = r->stream;
         g->owned_by_renderer = true;
-        const uint32_t want = (uint32_t)std::min<uint64_t>(c.guide_train_samples, npx * guide::kRecBounces);
-        // whole chunks of samples per step (at least one step of 64)
-        uint32_t ns = want >= kGuideBatch   ? want / kGuideBatch * kGuideBatch
-                      : want >= kGuideChunk ? want / kGuideChunk * kGuideChunk
-                                            : (want + 63u) & ~63u;
         int grc = guide_setup(g, ns, 0x5eedu, c.guide_lr, c.guide_mix);
         if (grc) return grc;
         r->guide_ns = ns;
@@ -1235,7 +1236,8 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
             g->gv.binv[a] = ext > 0.0f ? 1.0f / ext : 1.0f;
         }
         g->gv.rec = guide::Records{r->g_in0.as<float4>(), r->g_in1.as<float4>(), r->g_dir.as<float4>(),
-                                   r->g_tp.as<float4>(), r->g_Lb.as<float4>(), (uint32_t)npaths};
+                                   r->g_tp.as<float4>(), r->g_Lb.as<float4>(), (uint32_t)npx, ns,
+                                   r->guide_stride};
         pl.gv = g->gv;
     }
     pl.ps.guide_q = r->g_q.as<uint32_t>();

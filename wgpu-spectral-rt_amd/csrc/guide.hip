@@ -262,7 +262,8 @@ __global__ __launch_bounds__(64 * kScatterWaves) void k_guide_scatter(PathState 
     load_biases(bias, gv.params);
     const uint32_t n = seg_prefix(qs, bounce, kQGuide, 1, pre);  // syncs the block
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctr->guide_samples, (unsigned long long)n);
-    const uint32_t rr_after = fa.table[*fa.frame & fa.mask].rr_after;
+    const uint32_t base_frame = *fa.frame;
+    const uint32_t rr_after = fa.table[base_frame & fa.mask].rr_after;
     const int wave = threadIdx.x >> 6, l = __lane_id();
     float* stage = stage_all[wave];
     const uint32_t wstride = gridDim.x * kScatterWaves * 64;
@@ -327,13 +328,20 @@ __global__ __launch_bounds__(64 * kScatterWaves) void k_guide_scatter(PathState 
                 const float pm = gv.mix * mixture_pdf(m, w) + (1.0f - gv.mix) * (cw / kPi);
                 tp = mul4(mul4s(spec, (cw / kPi) / pm), tp);
                 if (bounce < kRecBounces && gv.rec.in0) {
-                    const size_t ri = (size_t)bounce * gv.rec.capacity + p;
-                    gv.rec.in0[ri] = make_float4(s[0], s[1], s[2], s[3]);
-                    gv.rec.in1[ri] = make_float4(s[4], s[5], s[6], pm);
-                    gv.rec.dir[ri] = make_float4(w.x, w.y, w.z, cw);
-                    gv.rec.tp[ri] = tp;
-                    gv.rec.Lb[ri] = ps.L[p];
-                    flags |= kFlagRecorded << bounce;
+                    // kept only if a training step of its frame reads it
+                    const uint32_t slot = p / gv.rec.npx;
+                    const uint32_t j = bounce * gv.rec.npx + (p - slot * gv.rec.npx);
+                    const uint32_t phase = (base_frame + slot) % gv.rec.stride;
+                    const uint32_t k = j >= phase ? (j - phase) / gv.rec.stride : gv.rec.ns;
+                    if (k < gv.rec.ns && k * gv.rec.stride + phase == j) {
+                        const size_t ri = (size_t)slot * gv.rec.ns + k;
+                        gv.rec.in0[ri] = make_float4(s[0], s[1], s[2], s[3]);
+                        gv.rec.in1[ri] = make_float4(s[4], s[5], s[6], pm);
+                        gv.rec.dir[ri] = make_float4(w.x, w.y, w.z, cw);
+                        gv.rec.tp[ri] = tp;
+                        gv.rec.Lb[ri] = ps.L[p];
+                        flags |= kFlagRecorded << bounce;
+                    }
                 }
                 alive = true;
                 if (bounce > rr_after) {
@@ -359,8 +367,9 @@ __global__ __launch_bounds__(64 * kScatterWaves) void k_guide_scatter(PathState 
 }
 
 // Training inputs of one step from the records of frame slot `slot`:
-// sample k is record j = (k * stride + phase) mod (kRecBounces npx), the
-// vertex of bounce j / npx on pixel j % npx. The weight of a sample is
+// sample k is record k of the slot, the vertex j = k * stride + phase
+// (< kRecBounces npx, since stride = floor(kRecBounces npx / ns)) of bounce
+// j / npx on pixel j % npx. The weight of a sample is
 // L_in cos / p_mix, L_in the incident radiance along the sampled direction
 // (radiance gathered after the vertex over the throughput after it, mean
 // over the hero wavelengths that carry throughput); records a path did not
@@ -372,13 +381,13 @@ __global__ __launch_bounds__(256) void k_guide_prepare(PathState ps, GuideView g
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ns) return;
     const uint32_t phase = (*fa.frame + slot) % stride;
-    const uint64_t j = ((uint64_t)k * stride + phase) % ((uint64_t)kRecBounces * npx);
+    const uint64_t j = (uint64_t)k * stride + phase;
     const uint32_t bounce = (uint32_t)(j / npx);
     const uint32_t p = slot * npx + (uint32_t)(j % npx);
-    const size_t ri = (size_t)bounce * gv.rec.capacity + p;
+    const size_t ri = (size_t)slot * ns + k;
     float wgt = 0.0f;
     float4 a = splat4(0.0f), b = splat4(0.0f), d = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
-    if (ps.flags[p] & (kFlagRecorded << bounce)) {
+    if (bounce < kRecBounces && (ps.flags[p] & (kFlagRecorded << bounce))) {
         a = gv.rec.in0[ri];
         b = gv.rec.in1[ri];
         d = gv.rec.dir[ri];

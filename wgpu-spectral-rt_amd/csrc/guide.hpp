@@ -98,13 +98,19 @@ constexpr float kKappaScale = 8.0f, kLogitScale = 4.0f;
 // Training records of the Lambert vertices at bounces 0 .. kRecBounces-1 of
 // each path (structure of arrays, index bounce * capacity + path id).
 constexpr uint32_t kRecBounces = 4;
+// Training records: only the vertices a training step will read are kept.
+// Vertex (bounce b, pixel x) of frame slot s is record k of the slot when
+// j = b * npx + x satisfies j = k * stride + phase with k < ns and phase =
+// (frame of the slot) mod stride; it is stored at s * ns + k.
 struct Records {
     float4* in0;   // scalars 0..3
     float4* in1;   // scalars 4..6, pdf of the mixture the direction came from
     float4* dir;   // sampled direction, cos to the normal
     float4* tp;    // throughput after the scatter (before roulette)
     float4* Lb;    // radiance accumulated before the scatter
-    uint32_t capacity;  // path slots
+    uint32_t npx;     // pixels of a frame
+    uint32_t ns;      // records (training samples) per frame slot
+    uint32_t stride;  // vertices between consecutive records
 };
 
 }  // namespace guide
