@@ -558,18 +558,22 @@ typedef struct { float t, u, v; uint32_t tri; int front; } hit_t;
 
 static inline int tri_intersect(v3 o, v3 d, v3 v0, v3 e1, v3 e2, float* t, float* u, float* v,
                                 int* front) {
+    /* Moller-Trumbore; the barycentric tests on the unscaled numerators
+     * with det's sign normalised, the division only for a triangle that
+     * passes them (the HIP path's order of operations) */
     v3 pvec = cross3(d, e2);
     float det = dot3(e1, pvec);
     if (det == 0.0f) return 0;
-    float inv = 1.0f / det;
+    float sg = det > 0.0f ? 1.0f : -1.0f, adet = det * sg;
     v3 tvec = sub3(o, v0);
-    float uu = dot3(tvec, pvec) * inv;
-    if (uu < 0.0f || uu > 1.0f) return 0;
+    float U = dot3(tvec, pvec);
+    if (U * sg < 0.0f || U * sg > adet) return 0;
     v3 qvec = cross3(tvec, e1);
-    float vv = dot3(d, qvec) * inv;
-    if (vv < 0.0f || uu + vv > 1.0f) return 0;
+    float V = dot3(d, qvec);
+    if (V * sg < 0.0f || (U + V) * sg > adet) return 0;
+    float inv = 1.0f / det;
     *t = dot3(e2, qvec) * inv;
-    *u = uu; *v = vv; *front = det > 0.0f;
+    *u = U * inv; *v = V * inv; *front = det > 0.0f;
     return 1;
 }
 
@@ -684,7 +688,6 @@ void or_any_hit(const or_scene* s, const float* rays, uint32_t n, int brute, uin
 }
 
 /* --- BVH8 traversal over the HIP path's arrays (format in wsrt.h) ------- */
-static inline float exp2i(int e) { return bitsf((uint32_t)(e + 127) << 23); }
 
 void or_bvh8_closest(const wsrt_bvh8_node* nodes, uint32_t n_nodes, const wsrt_bvh_tri* tris,
                      const float* rays, uint32_t n, uint32_t* hits, uint32_t* flags,
@@ -712,28 +715,35 @@ void or_bvh8_closest(const wsrt_bvh8_node* nodes, uint32_t n_nodes, const wsrt_b
             if (st_t[sp] > best_t * BOX_PAD) continue;
             const wsrt_bvh8_node* nd = &nodes[ni];
             nv++;
-            float as[3], ao[3], sl = 0.0f;
+            float as[3], ao[3], fs[3], fo[3], sl = 0.0f;
             for (int a = 0; a < 3; ++a) {
-                as[a] = exp2i(nd->exp[a]) * inv[a];
+                as[a] = ldexpf(inv[a], nd->exp[a]); /* 2^exp * inv, exact */
                 ao[a] = (nd->origin[a] - oo[a]) * inv[a];
                 sl = fmaxf(sl, fabsf(as[a]) * 255.0f + fabsf(ao[a]));
             }
-            /* bound of the f32 error of the slab values, applied to both ends */
+            /* bound of the f32 error of the slab values, applied to both
+             * ends: hit when near <= far * BOX_PAD + sl2, the padding folded
+             * into the far planes' constants */
             sl = sl * 4.0e-7f;
             const float sl2 = sl * (1.0f + BOX_PAD);
-            float cnear[8];
+            for (int a = 0; a < 3; ++a) {
+                fs[a] = as[a] * BOX_PAD;
+                fo[a] = fmaf(ao[a], BOX_PAD, sl2);
+            }
+            const float tcap = fmaf(best_t, BOX_PAD, sl2);
+            float tnear[8];
             int chit[8];
             for (int c = 0; c < 8; ++c) {
-                float tn = tmin, tf = best_t;
+                float tn = tmin, tf = tcap;
                 for (int a = 0; a < 3; ++a) {
                     int ng = inv[a] < 0.0f;
                     float qn = (float)(ng ? nd->qhi[a][c] : nd->qlo[a][c]);
                     float qf = (float)(ng ? nd->qlo[a][c] : nd->qhi[a][c]);
                     tn = fmaxf(tn, fmaf(qn, as[a], ao[a]));
-                    tf = fminf(tf, fmaf(qf, as[a], ao[a]));
+                    tf = fminf(tf, fmaf(qf, fs[a], fo[a]));
                 }
-                cnear[c] = tn - sl;
-                chit[c] = nd->meta[c] != 0 && tn <= fmaf(tf, BOX_PAD, sl2);
+                tnear[c] = tn;
+                chit[c] = tn <= tf; /* an empty slot is a leaf of no triangles */
             }
             /* leaf children first, in slot order */
             for (int c = 0; c < 8; ++c) {
@@ -757,12 +767,13 @@ void or_bvh8_closest(const wsrt_bvh8_node* nodes, uint32_t n_nodes, const wsrt_b
             }
             /* internal children in the ray-octant order of the slots: slot
              * (k ^ oct) pushed for k = 7 .. 0, so slot oct is popped first */
+            const float cull = fmaf(best_t, BOX_PAD, sl);
             for (int k = 7; k >= 0; --k) {
                 int c = k ^ oct;
                 if (!chit[c] || !(nd->meta[c] & 0x80)) continue;
-                if (cnear[c] > best_t * BOX_PAD) continue;
+                if (tnear[c] > cull) continue;
                 st_node[sp] = nd->child_base + (nd->meta[c] & 0x7F);
-                st_t[sp] = cnear[c];
+                st_t[sp] = tnear[c] - sl;
                 sp++;
             }
         }

@@ -358,8 +358,9 @@ __global__ void k_collapse(CollapseArgs A, const uint4* __restrict__ qin, uint32
         }
         for (int sl = 0; sl < 8; ++sl) {
             if (used & (1u << sl)) continue;
+            // empty: an inverted box (lo above hi), which the slab test rejects
             out.meta[sl] = 0;
-            for (int a = 0; a < 3; ++a) { out.qlo[a][sl] = 0; out.qhi[a][sl] = 0; }
+            for (int a = 0; a < 3; ++a) { out.qlo[a][sl] = 255; out.qhi[a][sl] = 0; }
         }
     }
     uint8_t imask = 0;

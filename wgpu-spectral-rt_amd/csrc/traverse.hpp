@@ -8,7 +8,8 @@
 //    dot(dir, cross(v1 - v0, v2 - v0)) < 0 (the DXR/Vulkan default winding);
 //  * any hit terminates on the first accepted triangle.
 // Node boxes are tested with fused slab values plus a bound of their f32
-// error, so culling never rejects a box the exact box would accept; the
+// error (folded into the far planes), so culling never rejects a box the
+// exact box would accept; the
 // visiting order (leaf children first, then internal children in the order
 // of the ray's direction octant over the build-time octant slots) is what
 // the oracle's BVH8 traversal counts.
@@ -34,21 +35,24 @@ __device__ __forceinline__ float safe_inv(float d) {
     return 1.0f / d;
 }
 
+// Moller-Trumbore with the barycentric tests on the unscaled numerators
+// (signs normalised by det's), so only a triangle that passes them divides.
 __device__ __forceinline__ bool tri_intersect(f3 o, f3 d, f3 v0, f3 e1, f3 e2, float& t, float& u,
                                               float& v, bool& front) {
     f3 pvec = cross(d, e2);
     float det = dot(e1, pvec);
     if (det == 0.0f) return false;
-    float inv = 1.0f / det;
+    const float sg = det > 0.0f ? 1.0f : -1.0f, adet = det * sg;
     f3 tvec = o - v0;
-    float uu = dot(tvec, pvec) * inv;
-    if (uu < 0.0f || uu > 1.0f) return false;
+    float U = dot(tvec, pvec);
+    if (U * sg < 0.0f || U * sg > adet) return false;
     f3 qvec = cross(tvec, e1);
-    float vv = dot(d, qvec) * inv;
-    if (vv < 0.0f || uu + vv > 1.0f) return false;
+    float V = dot(d, qvec);
+    if (V * sg < 0.0f || (U + V) * sg > adet) return false;
+    float inv = 1.0f / det;
     t = dot(e2, qvec) * inv;
-    u = uu;
-    v = vv;
+    u = U * inv;
+    v = V * inv;
     front = det > 0.0f;
     return true;
 }
@@ -125,7 +129,7 @@ __device__ Hit traverse(const SceneView& sv, f3 o, f3 d, float tmin, float tmax,
         if (kCount) nvis++;
         // per axis: slab scale/offset, and the byte plane of each child's near
         // and far bound (qlo or qhi by the sign of the direction)
-        float as[3], ao[3], sl = 0.0f;
+        float as[3], ao[3], fs[3], fo[3], sl = 0.0f;
         const float org[3] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)};
         // quantized bytes: qlo x/y/z = (w2.xy, w2.zw, w3.xy), qhi x/y/z = (w3.zw, w4.xy, w4.zw)
         const uint32_t lo_l[3] = {w2.x, w2.z, w3.x}, lo_h[3] = {w2.y, w2.w, w3.y};
@@ -133,9 +137,8 @@ __device__ Hit traverse(const SceneView& sv, f3 o, f3 d, float tmin, float tmax,
         uint32_t nl[3], nh[3], fl[3], fh[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            int ex = (int)(int8_t)((w0.w >> (8 * a)) & 0xFFu);
-            float scale = __uint_as_float((uint32_t)(ex + 127) << 23);
-            as[a] = scale * inv[a];
+            const int ex = (int)(int8_t)((w0.w >> (8 * a)) & 0xFFu);
+            as[a] = ldexpf(inv[a], ex);  // 2^ex * inv, exact
             ao[a] = (org[a] - oo[a]) * inv[a];
             sl = fmaxf(sl, fabsf(as[a]) * 255.0f + fabsf(ao[a]));
             nl[a] = neg[a] ? hi_l[a] : lo_l[a];
@@ -143,22 +146,33 @@ __device__ Hit traverse(const SceneView& sv, f3 o, f3 d, float tmin, float tmax,
             fl[a] = neg[a] ? lo_l[a] : hi_l[a];
             fh[a] = neg[a] ? lo_h[a] : hi_h[a];
         }
-        // bound of the f32 error of the slab values, applied to both ends
+        // bound of the f32 error of the slab values, applied to both ends:
+        // a child is hit when its near distance is at most its padded far
+        // distance, far * kBoxPad + sl2, with the padding folded into the
+        // far planes' constants
         sl = sl * 4.0e-7f;
         const float sl2 = sl * (1.0f + kBoxPad);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            fs[a] = as[a] * kBoxPad;
+            fo[a] = __fmaf_rn(ao[a], kBoxPad, sl2);
+        }
+        const float tcap = __fmaf_rn(best.t, kBoxPad, sl2);
         const uint32_t child_base = w1.x, tri_base = w1.y;
-        float cnear[8];
+        float tnear[8];
         uint32_t hitmask = 0;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            float tn = tmin, tf = best.t;
+            float tn = tmin, tf = tcap;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 tn = fmaxf(tn, __fmaf_rn((float)byte_of(nl[a], nh[a], c), as[a], ao[a]));
-                tf = fminf(tf, __fmaf_rn((float)byte_of(fl[a], fh[a], c), as[a], ao[a]));
+                tf = fminf(tf, __fmaf_rn((float)byte_of(fl[a], fh[a], c), fs[a], fo[a]));
             }
-            cnear[c] = tn - sl;
-            if (byte_of(w1.z, w1.w, c) != 0u && tn <= __fmaf_rn(tf, kBoxPad, sl2)) hitmask |= 1u << c;
+            tnear[c] = tn;
+            // an empty slot (meta 0) is a leaf of no triangles; its box is
+            // written inverted, so it is not hit in practice either
+            if (tn <= tf) hitmask |= 1u << c;
         }
         // slot classes: bit c of the node's imask byte is set when child c is
         // internal (meta bit 7), stored by the builder
@@ -197,13 +211,18 @@ __device__ Hit traverse(const SceneView& sv, f3 o, f3 d, float tmin, float tmax,
         // internal children in the order of the ray's octant over the slots:
         // slot c is pushed at rank (c ^ oct) from the top, so slot oct is
         // popped first; positions come from a mask permuted by oct
-        uint32_t push = 0, pm = 0;
+        // a child is culled when its near distance, less the error bound, is
+        // beyond the closest hit so far (padded)
+        const float cull = __fmaf_rn(best.t, kBoxPad, sl);
+        uint32_t push = 0;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const bool pc = ((hitmask & imask) >> c) & 1u && !(cnear[c] > best.t * kBoxPad);
-            push |= (pc ? 1u : 0u) << c;
-            pm |= (pc ? 1u : 0u) << (c ^ oct);
-        }
+        for (int c = 0; c < 8; ++c) push |= (!(tnear[c] > cull) ? 1u : 0u) << c;
+        push &= hitmask & imask;
+        // ranks: bit c of push at position c ^ oct (swap bits, pairs, nibbles)
+        uint32_t pm = push;
+        if (oct & 1) pm = ((pm & 0x55u) << 1) | ((pm >> 1) & 0x55u);
+        if (oct & 2) pm = ((pm & 0x33u) << 2) | ((pm >> 2) & 0x33u);
+        if (oct & 4) pm = ((pm & 0x0Fu) << 4) | ((pm >> 4) & 0x0Fu);
         if (pm) {
             // the lowest rank goes to the register, the others onto the stack
             // with lower ranks nearer the top
@@ -218,7 +237,7 @@ __device__ Hit traverse(const SceneView& sv, f3 o, f3 d, float tmin, float tmax,
                     continue;
                 }
                 const int pos = sp + __popc(rest >> ((c ^ oct) + 1));
-                st.put(pos, make_uint2(child_base + (meta & 0x7Fu), __float_as_uint(cnear[c])));
+                st.put(pos, make_uint2(child_base + (meta & 0x7Fu), __float_as_uint(tnear[c] - sl)));
             }
             sp += __popc(rest);
             have_next = true;
@@ -249,7 +268,7 @@ __device__ __forceinline__ bool meets_root_child(const SceneView& sv, f3 o, f3 d
     for (int a = 0; a < 3; ++a) {
         const bool neg = inv[a] < 0.0f;
         const int ex = (int)(int8_t)((w0.w >> (8 * a)) & 0xFFu);
-        as[a] = __uint_as_float((uint32_t)(ex + 127) << 23) * inv[a];
+        as[a] = ldexpf(inv[a], ex);
         ao[a] = (org[a] - oo[a]) * inv[a];
         sl = fmaxf(sl, fabsf(as[a]) * 255.0f + fabsf(ao[a]));
         nl[a] = neg ? hi_l[a] : lo_l[a];
@@ -259,16 +278,23 @@ __device__ __forceinline__ bool meets_root_child(const SceneView& sv, f3 o, f3 d
     }
     sl = sl * 4.0e-7f;
     const float sl2 = sl * (1.0f + kBoxPad);
+    float fs[3], fo[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        fs[a] = as[a] * kBoxPad;
+        fo[a] = __fmaf_rn(ao[a], kBoxPad, sl2);
+    }
+    const float tcap = __fmaf_rn(tmax, kBoxPad, sl2);
     bool any = false;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        float tn = tmin, tf = tmax;
+        float tn = tmin, tf = tcap;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             tn = fmaxf(tn, __fmaf_rn((float)byte_of(nl[a], nh[a], c), as[a], ao[a]));
-            tf = fminf(tf, __fmaf_rn((float)byte_of(fl[a], fh[a], c), as[a], ao[a]));
+            tf = fminf(tf, __fmaf_rn((float)byte_of(fl[a], fh[a], c), fs[a], fo[a]));
         }
-        any |= byte_of(w1.z, w1.w, c) != 0u && tn <= __fmaf_rn(tf, kBoxPad, sl2);
+        any |= byte_of(w1.z, w1.w, c) != 0u && tn <= tf;
     }
     return any;
 }
