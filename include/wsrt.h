@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define WSRT_ABI_VERSION 1
+#define WSRT_ABI_VERSION 2
 
 /* ---- status codes ---------------------------------------------------- */
 enum {
@@ -169,7 +169,10 @@ typedef struct wsrt_config {
                                    * depend on it.                             */
     uint32_t quirks;              /* WSRT_FIX_* bits; 0 renders as the
                                    * reference does                          */
-    uint32_t reserved[2];
+    uint32_t trace_rounds;        /* traversal launches: rounds of resident
+                                   * waves (1..4; 0 means 1). Results do not
+                                   * depend on it.                             */
+    uint32_t reserved[1];
 } wsrt_config;
 
 /* wsrt_config.quirks: each bit replaces one behaviour of the reference's
@@ -438,13 +441,23 @@ int wsrt_debug_read_world(const wsrt_scene* s, float* tris, uint32_t* ids);
  * camera), resolution and config by
  * rendering 2 warm batches each, within a device-memory budget (bytes; 0 =
  * a quarter of the device), and writes the fastest into cfg->frames_in_flight.
- * Results do not depend on frames_in_flight. */
+ * Then, at that setting, times tail_paths in {0, 2^16, 2^18, 2^20} x
+ * trace_rounds in {1, 2} and writes the fastest into cfg. Results do not
+ * depend on any of them. */
 typedef struct wsrt_autotune_result {
     uint32_t n_candidates;
     uint32_t frames_in_flight[12];
     float ms_per_frame[12];       /* 0: not run (over budget or allocation failed) */
     uint64_t device_bytes[12];    /* device memory the candidate's renderer took */
     uint32_t best;
+    /* second stage, at the best frames_in_flight: the queue size below which
+     * one launch finishes the remaining bounces (tail_paths) and the
+     * traversal grids (trace_rounds) */
+    uint32_t n_stage2;
+    uint32_t tail_paths[8];
+    uint32_t trace_rounds[8];
+    float stage2_ms_per_frame[8]; /* 0: not run */
+    uint32_t best_tail_paths, best_trace_rounds;
 } wsrt_autotune_result;
 int wsrt_autotune(wsrt_device* dev, const wsrt_scene* scene, uint32_t width, uint32_t height,
                   const wsrt_camera_uniform* cam, wsrt_config* cfg, uint64_t mem_budget,

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version_and_struct_sizes():
-    assert wsrt.load().wsrt_abi_version() == 1
+    assert wsrt.load().wsrt_abi_version() == 2
     assert C.sizeof(wsrt.CameraUniform) == 272
     assert wsrt.VERTEX_DTYPE.itemsize == 32
     assert wsrt.MATERIAL_DTYPE.itemsize == 32

@@ -361,6 +361,22 @@ def test_autotune_picks_a_measured_candidate(device, cornell_pair):
     ms = {c["frames_in_flight"]: c["ms_per_frame"] for c in res["candidates"]}
     assert cfg.frames_in_flight == res["best"] and ms[res["best"]] > 0
     assert ms[res["best"]] <= min(v for v in ms.values() if v > 0) * 1.02 + 1e-9
+    # second stage: tail threshold and traversal grid at that setting
+    assert len(res["stage2"]) == 7 and all(c["ms_per_frame"] > 0 for c in res["stage2"])
+    assert (cfg.tail_paths, cfg.trace_rounds) == (res["best_tail_paths"], res["best_trace_rounds"])
+    assert cfg.trace_rounds in (1, 2) and cfg.tail_paths in (0, 1 << 16, 1 << 18, 1 << 20)
+
+
+def test_trace_rounds_do_not_change_results(device, cornell_pair):
+    g, _ = cornell_pair
+    out = []
+    for rounds in (1, 2, 4):
+        r = wsrt.Renderer(device, g, 40, 40, wsrt.default_config(photons_per_pass=1 << 14, frames_in_flight=2,
+                                                                  trace_rounds=rounds))
+        r.render_frames(4)
+        out.append(r.read_accum())
+    np.testing.assert_array_equal(out[0], out[1])
+    np.testing.assert_array_equal(out[0], out[2])
 
 
 def test_environment_requires_a_map(device, cornell_pair):
@@ -404,11 +420,45 @@ def test_batch_equals_single_renderers(device, tile_mode):
         np.testing.assert_array_equal(b.read_accum(k), refs[k])
 
 
-def test_batch_rejects_photons_and_guiding(device, cornell_pair):
+def test_batch_rejects_guiding(device, cornell_pair):
     g, _ = cornell_pair
     cam = scenes.camera(scenes.DEFAULT_VIEW, 0)
     with pytest.raises(wsrt.WsrtError):
-        wsrt.Batch(device, [(g, cam, 8, 8, 1)], wsrt.default_config(enable_photons=1))
+        wsrt.Batch(device, [(g, cam, 8, 8, 1)], wsrt.default_config(guiding=1))
+
+
+@pytest.mark.parametrize("graphs", [1, 0])
+def test_batch_with_photons_equals_single_renderers(device, graphs):
+    """With the photon pass on, a batch renders frame by frame (each job's
+    photon pass and grid, then one persistent launch over all pixels): each
+    job's image equals its own renderer's, photon gather included, bit for
+    bit, with and without per-frame graphs and after autotuning."""
+    print(f"batch with photons, graphs {graphs}", flush=True)
+    specs = [("caustic", 32, 32, 3), ("cornell", 40, 24, 5), ("custom", 24, 40, 2), ("caustic", 9, 13, 4)]
+    jobs, refs = [], []
+    for name, W, H, spp in specs:
+        g, _, cfg, view = setup(device, name, W, H, 1 << 14)
+        cam = scenes.camera(view, 0)
+        c = wsrt.Config.from_buffer_copy(bytes(cfg))
+        c.photons_per_pass, c.cell_size, c.gather_radius, c.gather_radius_exponent = 1 << 14, 0.02, 0.02, -0.15
+        c.frames_in_flight = 2
+        r = wsrt.Renderer(device, g, W, H, c)
+        r.set_camera(cam)
+        r.render_frames(spp)
+        refs.append(r.read_accum())
+        jobs.append((g, cam, W, H, spp))
+    bcfg = wsrt.default_config(photons_per_pass=1 << 14, frame_graphs=graphs)
+    b = wsrt.Batch(device, jobs, bcfg)
+    for _ in range(2):  # a second render restarts every job at frame 0
+        b.render()
+        b.sync()
+        for k in range(len(specs)):
+            np.testing.assert_array_equal(b.read_accum(k), refs[k])
+    b.autotune()
+    b.render()
+    b.sync()
+    for k in range(len(specs)):
+        np.testing.assert_array_equal(b.read_accum(k), refs[k])
 
 
 @pytest.mark.parametrize("name", ["cornell", "small_box"])

@@ -197,11 +197,14 @@ class _Materials:
         return getattr(self.b, name)
 
 
-def test_batch_of_procedural_scenes(device):
+@pytest.mark.parametrize("photons", [0, 1])
+def test_batch_of_procedural_scenes(device, photons):
     """Scenes 0..7 of the batch workload (1.5 M triangles at most) at their
-    own resolutions and sample counts in one persistent launch."""
-    print("batch: 8 procedural scenes", flush=True)
-    cfg = wsrt.default_config(enable_photons=0, max_depth=8)
+    own resolutions and sample counts in one persistent launch (with the
+    photon pass: 2^20 photon paths per job and frame, then one persistent
+    launch per frame)."""
+    print(f"batch: 8 procedural scenes, photons {photons}", flush=True)
+    cfg = wsrt.default_config(enable_photons=photons, max_depth=8)
     jobs, refs, types, tris = [], [], set(), []
     for k in range(8):
         rec = _Materials(wsrt.SceneBuilder())
@@ -220,8 +223,9 @@ def test_batch_of_procedural_scenes(device):
             ids = np.sort(np.random.default_rng(k).choice(v.width * v.height, 2048, replace=False)).astype(np.uint32)
             wl, L = r.trace_paths(ids)
             lc = scenes.camera(v, spp - 1)
+            ph = r.read_photons()[0] if photons else None
             f = O.Frame(o, O.Config.from_buffer_copy(bytes(cfg)), O.CameraUniform.from_buffer_copy(bytes(lc)),
-                        v.width, v.height)
+                        v.width, v.height, photons=ph)
             owl, oL, _ = f.trace_pixels(ids)
             floor = 1e-6 * max(float(np.abs(oL).mean()), 1e-12)
             same = rel_close(L, oL, 1e-4, floor).mean()

@@ -793,6 +793,7 @@ static int validate_config(const wsrt_config* c) {
         (uint64_t)c->hash_size * c->frames_in_flight > (1ull << 30))
         return fail(WSRT_E_INVALID, "photons or cells times frames_in_flight too large");
     if (c->guiding > 1) return fail(WSRT_E_INVALID, "guiding %u not 0 or 1", c->guiding);
+    if (c->trace_rounds > 4) return fail(WSRT_E_INVALID, "trace_rounds %u not in 0..4", c->trace_rounds);
     const uint32_t known = WSRT_FIX_EMISSION_MIS | WSRT_FIX_RR_CLAMP | WSRT_FIX_ONB_SIGN;
     if (c->quirks & ~known) return fail(WSRT_E_INVALID, "quirks 0x%x has unknown bits", c->quirks);
     if ((c->quirks & WSRT_FIX_EMISSION_MIS) && c->guiding)
@@ -1202,6 +1203,7 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     // they stay in the wavefront to the end
     pl.qs.tail = (c.guiding || c.tail_paths == 0) ? nullptr : pl.qs.counts + queue_count_words(depths);
     pl.qs.tail_paths = c.tail_paths;
+    pl.qs.trace_rounds = c.trace_rounds ? c.trace_rounds : 1u;
     pl.guided = c.guiding;
     if (c.guiding) {
         const uint32_t want = (uint32_t)std::min<uint64_t>(c.guide_train_samples, npx * guide::kRecBounces);
@@ -2122,6 +2124,40 @@ extern "C" int wsrt_renderer_get_guide(wsrt_renderer* r, wsrt_guide** out) {
 // ---------------------------------------------------------------------------
 // autotuner: frames in flight
 // ---------------------------------------------------------------------------
+// Milliseconds per frame of a renderer with config c (first batch warms and
+// captures, then 2 batches timed with events), and its device memory; 0 ms
+// when it does not fit the budget or fails.
+static float autotune_time(wsrt_device* dev, const wsrt_scene* scene, uint32_t width, uint32_t height,
+                           const wsrt_camera_uniform* cam, const wsrt_config& c, uint64_t mem_budget,
+                           uint64_t* bytes) {
+    size_t fa = 0, fb = 0, total = 0;
+    (void)hipMemGetInfo(&fa, &total);
+    wsrt_renderer* r = nullptr;
+    *bytes = 0;
+    if (wsrt_renderer_create(dev, scene, width, height, &c, &r) != WSRT_OK) return 0.0f;
+    (void)hipMemGetInfo(&fb, &total);
+    *bytes = fa > fb ? (uint64_t)(fa - fb) : 0;
+    const uint32_t S = c.frames_in_flight;
+    bool ok = *bytes <= mem_budget;
+    if (ok && cam) ok = wsrt_renderer_set_camera(r, cam) == WSRT_OK;
+    float ms = 0.0f;
+    if (ok) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+        ok = ok && wsrt_renderer_render_frames(r, S) == WSRT_OK;  // first batch captures the graph
+        ok = ok && hipEventRecord(e0, r->stream) == hipSuccess;
+        ok = ok && wsrt_renderer_render_frames(r, 2 * S) == WSRT_OK;
+        ok = ok && hipEventRecord(e1, r->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+        if (ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) ms /= (float)(2 * S);
+        else ms = 0.0f;
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+    wsrt_renderer_destroy(r);
+    (void)hipGetLastError();
+    return ok ? ms : 0.0f;
+}
+
 extern "C" int wsrt_autotune(wsrt_device* dev, const wsrt_scene* scene, uint32_t width, uint32_t height,
                              const wsrt_camera_uniform* cam, wsrt_config* cfg, uint64_t mem_budget,
                              wsrt_autotune_result* out) {
@@ -2143,30 +2179,8 @@ extern "C" int wsrt_autotune(wsrt_device* dev, const wsrt_scene* scene, uint32_t
         wsrt_config c = *cfg;
         c.frames_in_flight = S;
         if (validate_config(&c) != WSRT_OK) break;
-        size_t fa = 0, fb = 0;
-        (void)hipMemGetInfo(&fa, &total);
-        wsrt_renderer* r = nullptr;
-        if (wsrt_renderer_create(dev, scene, width, height, &c, &r) != WSRT_OK) break;
-        (void)hipMemGetInfo(&fb, &total);
-        out->device_bytes[k] = fa > fb ? (uint64_t)(fa - fb) : 0;
-        bool ok = out->device_bytes[k] <= mem_budget;
-        if (ok && cam) ok = wsrt_renderer_set_camera(r, cam) == WSRT_OK;
-        float ms = 0.0f;
-        if (ok) {
-            hipEvent_t e0 = nullptr, e1 = nullptr;
-            ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
-            ok = ok && wsrt_renderer_render_frames(r, S) == WSRT_OK;  // first batch captures the graph
-            ok = ok && hipEventRecord(e0, r->stream) == hipSuccess;
-            ok = ok && wsrt_renderer_render_frames(r, 2 * S) == WSRT_OK;
-            ok = ok && hipEventRecord(e1, r->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
-            if (ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) ms /= (float)(2 * S);
-            else ok = false;
-            if (e0) (void)hipEventDestroy(e0);
-            if (e1) (void)hipEventDestroy(e1);
-        }
-        wsrt_renderer_destroy(r);
-        (void)hipGetLastError();
-        if (!ok) break;
+        const float ms = autotune_time(dev, scene, width, height, cam, c, mem_budget, &out->device_bytes[k]);
+        if (ms == 0.0f) break;
         out->ms_per_frame[k] = ms;
         if (best_ms == 0.0f || ms < best_ms * 0.98f) {  // a larger batch must win by 2 %
             best_ms = ms;
@@ -2179,6 +2193,35 @@ extern "C" int wsrt_autotune(wsrt_device* dev, const wsrt_scene* scene, uint32_t
     if (best_ms == 0.0f) return fail(WSRT_E_OOM, "no frames_in_flight candidate fits the memory budget");
     out->best = best;
     cfg->frames_in_flight = best;
+    // second stage at the chosen frames in flight: the tail launch's queue
+    // threshold and the traversal grids; the config as given (a measured
+    // first-stage candidate) is kept unless another is 2 % faster
+    const uint32_t tails[4] = {0u, 1u << 16, 1u << 18, 1u << 20};
+    float best2 = best_ms;
+    uint32_t bt = cfg->tail_paths, br = cfg->trace_rounds ? cfg->trace_rounds : 1u;
+    for (uint32_t rounds = 1; rounds <= 2; ++rounds) {
+        for (uint32_t t = 0; t < 4; ++t) {
+            if (rounds == br && tails[t] == bt) continue;  // measured in the first stage
+            wsrt_config c = *cfg;
+            c.tail_paths = tails[t];
+            c.trace_rounds = rounds;
+            const uint32_t i = out->n_stage2++;
+            out->tail_paths[i] = tails[t];
+            out->trace_rounds[i] = rounds;
+            uint64_t bytes = 0;
+            const float ms = autotune_time(dev, scene, width, height, cam, c, mem_budget, &bytes);
+            out->stage2_ms_per_frame[i] = ms;
+            if (ms > 0.0f && ms < best2 * 0.98f) {
+                best2 = ms;
+                bt = tails[t];
+                br = rounds;
+            }
+        }
+    }
+    cfg->tail_paths = bt;
+    cfg->trace_rounds = br;
+    out->best_tail_paths = bt;
+    out->best_trace_rounds = br;
     return WSRT_OK;
 }
 
@@ -2198,6 +2241,16 @@ struct wsrt_batch {
     int cus = 256;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // photon pass (cfg.enable_photons): each job's grid of the frame being
+    // traced, and the trace buffers the jobs' passes share (one job at a time)
+    bool photons = false;
+    uint32_t max_spp = 0;
+    std::vector<DevBuf> gcount, goffset, gsorted, gtouched, gnvalid;
+    DevBuf ph, ph_stored, g_cell, g_slot, g_temp, qcounts, ctr, pgstack, dframe;
+    DevBuf ph_id[2], ph_pos[2], ph_dir[2], ph_rng[2], ph_flags[2];
+    PhotonState phs{};
+    Queues qs{};
+    std::vector<std::pair<hipGraph_t, hipGraphExec_t>> frame_graph;  // by frame, photon mode
 };
 
 static void batch_drop_graph(wsrt_batch* b) {
@@ -2205,6 +2258,102 @@ static void batch_drop_graph(wsrt_batch* b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
+    for (auto& g : b->frame_graph) {
+        if (g.second) (void)hipGraphExecDestroy(g.second);
+        if (g.first) (void)hipGraphDestroy(g.first);
+    }
+    b->frame_graph.clear();
+}
+
+// Frame f of every job that renders it: each job's photon pass and grid
+// build for that frame (photon_emit.wgsl, build_grid.wgsl: the renderer's
+// kernels with one frame in flight), then one persistent launch tracing
+// frame f of all jobs' pixels.
+static void batch_enqueue_frame(wsrt_batch* b, uint32_t f, const BatchArgs& base, uint32_t blocks) {
+    hipStream_t s = b->stream;
+    const uint32_t np = b->cfg.photons_per_pass;
+    for (size_t j = 0; j < b->jobs.size(); ++j) {
+        const BatchJob& B = b->hjobs[j];
+        if (f >= B.frames) continue;
+        const FrameArgs fa{b->table.as<FrameConst>() + B.table_base, b->dframe.as<uint32_t>(), 0xFFFFFFFFu};
+        launch_set_frame(b->dframe.as<uint32_t>(), f, s);
+        launch_clear_queues(b->qs, b->cfg.photon_max_depth + 1, s);
+        launch_photon_emit(B.sv, fa, np, 1, b->qs, b->phs, 0u, b->ctr.as<Counters>(), s);
+        for (uint32_t d = 0; d < b->cfg.photon_max_depth; ++d)
+            launch_photon_bounce(B.sv, d, 0u, b->phs, b->qs, b->ph.as<wsrt_photon>(), b->ctr.as<Counters>(),
+                                 b->pgstack.as<uint2>(), s);
+        GridBuffers g;
+        g.count = b->gcount[j].as<uint32_t>();
+        g.offset = b->goffset[j].as<uint32_t>();
+        g.cell_id = b->g_cell.as<uint32_t>();
+        g.sorted_slot = b->g_slot.as<uint32_t>();
+        g.sorted = b->gsorted[j].as<wsrt_photon>();
+        g.n_valid = b->gnvalid[j].as<uint32_t>();
+        g.temp = b->g_temp.as<uint32_t>();
+        g.touched = b->gtouched[j].as<uint32_t>();
+        grid_build_stored(b->ph.as<wsrt_photon>(), b->phs.stored, np, 1, b->cfg.hash_size, fa, g, s);
+    }
+    BatchArgs a = base;
+    a.one_frame = 1u;
+    a.frame = f;
+    launch_batch(a, blocks, s);
+}
+
+// The photon pass's buffers: per job, a grid (counts, offsets, photons in
+// cell order, the cells the last build touched); shared, the photon trace
+// state of one pass and the grid build's work memory.
+static int batch_alloc_photons(wsrt_batch* b, uint32_t max_stack) {
+    const wsrt_config& c = b->cfg;
+    const uint32_t np = c.photons_per_pass ? c.photons_per_pass : 1u;
+    const size_t nj = b->jobs.size();
+    b->gcount.resize(nj);
+    b->goffset.resize(nj);
+    b->gsorted.resize(nj);
+    b->gtouched.resize(nj);
+    b->gnvalid.resize(nj);
+    for (size_t j = 0; j < nj; ++j) {
+        HIP_OR_FAIL(b->gcount[j].alloc(4ull * c.hash_size));
+        HIP_OR_FAIL(b->goffset[j].alloc(4ull * c.hash_size));
+        HIP_OR_FAIL(b->gsorted[j].alloc(sizeof(wsrt_photon) * np));
+        HIP_OR_FAIL(b->gtouched[j].alloc(4ull * np));
+        HIP_OR_FAIL(b->gnvalid[j].alloc(4));
+        HIP_OR_FAIL(hipMemsetAsync(b->gcount[j].p, 0, 4ull * c.hash_size, b->stream));
+        HIP_OR_FAIL(hipMemsetAsync(b->goffset[j].p, 0, 4ull * c.hash_size, b->stream));
+        HIP_OR_FAIL(hipMemsetAsync(b->gnvalid[j].p, 0, 4, b->stream));
+        BatchJob& B = b->hjobs[j];
+        B.grid = GridView{b->gcount[j].as<uint32_t>(), b->goffset[j].as<uint32_t>(),
+                          b->gsorted[j].as<wsrt_photon>(), c.hash_size, c.cell_size};
+    }
+    HIP_OR_FAIL(b->ph.alloc(sizeof(wsrt_photon) * np));
+    HIP_OR_FAIL(b->ph_stored.alloc(4ull * ((np + 31) / 32)));
+    HIP_OR_FAIL(b->g_cell.alloc(4ull * np));
+    HIP_OR_FAIL(b->g_slot.alloc(4ull * np));
+    HIP_OR_FAIL(b->g_temp.alloc(4 * grid_temp_words(np, c.hash_size)));
+    const uint64_t pq = (uint64_t)kSegs * seg_capacity(np);
+    for (int k = 0; k < 2; ++k) {
+        HIP_OR_FAIL(b->ph_id[k].alloc(4 * pq));
+        HIP_OR_FAIL(b->ph_pos[k].alloc(16 * pq));
+        HIP_OR_FAIL(b->ph_dir[k].alloc(16 * pq));
+        HIP_OR_FAIL(b->ph_rng[k].alloc(4 * pq));
+        HIP_OR_FAIL(b->ph_flags[k].alloc(4 * pq));
+        b->phs.q[k] = PhotonQueue{b->ph_id[k].as<uint32_t>(), b->ph_pos[k].as<float4>(), b->ph_dir[k].as<float4>(),
+                                  b->ph_rng[k].as<uint32_t>(), b->ph_flags[k].as<uint32_t>()};
+    }
+    b->phs.stored = b->ph_stored.as<uint32_t>();
+    const uint32_t depths = c.photon_max_depth + 1;
+    HIP_OR_FAIL(b->qcounts.alloc(4 * (queue_count_words(depths) + kSegStride)));
+    HIP_OR_FAIL(hipMemsetAsync(b->qcounts.p, 0, b->qcounts.bytes, b->stream));
+    b->qs.counts = b->qcounts.as<uint32_t>();
+    b->qs.seg_cap = 0;
+    b->qs.pseg_cap = seg_capacity(np);
+    b->qs.tail = nullptr;
+    b->qs.trace_rounds = c.trace_rounds ? c.trace_rounds : 1u;
+    HIP_OR_FAIL(b->ctr.alloc(sizeof(Counters)));
+    HIP_OR_FAIL(hipMemsetAsync(b->ctr.p, 0, sizeof(Counters), b->stream));
+    const uint32_t spill = max_stack > (uint32_t)kShortStack ? max_stack - kShortStack : 1u;
+    HIP_OR_FAIL(b->pgstack.alloc(sizeof(uint2) * kTraceLanes * spill));
+    HIP_OR_FAIL(b->dframe.alloc(4));
+    return WSRT_OK;
 }
 
 static void batch_release(wsrt_batch* b) {
@@ -2250,8 +2399,7 @@ extern "C" int wsrt_batch_create(wsrt_device* dev, const wsrt_batch_job* jobs, u
     else wsrt_config_default(&c);
     int rc = validate_config(&c);
     if (rc) return rc;
-    if (c.enable_photons || c.guiding)
-        return fail(WSRT_E_UNSUPPORTED, "batches trace without the photon pass and guiding");
+    if (c.guiding) return fail(WSRT_E_UNSUPPORTED, "batches trace without guiding");
     uint64_t total_frames = 0;
     uint32_t max_stack = 1;
     for (uint32_t j = 0; j < n_jobs; ++j) {
@@ -2301,6 +2449,8 @@ extern "C" int wsrt_batch_create(wsrt_device* dev, const wsrt_batch_job* jobs, u
         B.table_base = (uint32_t)table.size();
         B.tiles_x = (J.width + 7u) / 8u;
         B.first_item = 0;
+        B.grid = GridView{nullptr, nullptr, nullptr, c.hash_size, c.cell_size};
+        b->max_spp = std::max(b->max_spp, J.spp);
         for (uint32_t f = 0; f < J.spp; ++f) {
             wsrt_camera_uniform cam = J.camera;
             cam.frame_count = f;
@@ -2324,6 +2474,13 @@ extern "C" int wsrt_batch_create(wsrt_device* dev, const wsrt_batch_job* jobs, u
     b->args.gstack = b->gstack.as<uint2>();
     b->args.spill = spill;
     b->args.max_depth = c.max_depth;
+    b->args.one_frame = 0;
+    b->args.frame = 0;
+    b->photons = c.enable_photons != 0;
+    if (b->photons) {
+        rc = batch_alloc_photons(b, max_stack);
+        if (rc) return bail(rc);
+    }
     rc = wsrt_batch_set_tuning(b, &b->tuning);
     if (rc) return bail(rc);
     *out = b;
@@ -2340,6 +2497,28 @@ extern "C" int wsrt_batch_render(wsrt_batch* b) {
     CHECK_ARG(b);
     HIP_OR_FAIL(hipSetDevice(b->dev->ordinal));
     const uint32_t blocks = (uint32_t)b->cus * b->tuning.blocks_per_cu;
+    if (b->photons) {
+        // frame by frame: every job's photon pass, then one persistent
+        // launch over all jobs' pixels; each frame one captured graph
+        if (b->frame_graph.empty()) b->frame_graph.assign(b->max_spp, {nullptr, nullptr});
+        for (uint32_t f = 0; f < b->max_spp; ++f) {
+            auto& fg = b->frame_graph[f];
+            if (!fg.second && b->cfg.frame_graphs &&
+                hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+                batch_enqueue_frame(b, f, b->args, blocks);
+                if (hipStreamEndCapture(b->stream, &fg.first) != hipSuccess ||
+                    hipGraphInstantiate(&fg.second, fg.first, nullptr, nullptr, 0) != hipSuccess) {
+                    if (fg.first) (void)hipGraphDestroy(fg.first);
+                    fg = {nullptr, nullptr};
+                }
+                (void)hipGetLastError();
+            }
+            if (fg.second) HIP_OR_FAIL(hipGraphLaunch(fg.second, b->stream));
+            else batch_enqueue_frame(b, f, b->args, blocks);
+        }
+        HIP_OR_FAIL(hipGetLastError());
+        return WSRT_OK;
+    }
     if (!b->exec) {
         if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
             launch_batch(b->args, blocks, b->stream);
@@ -2379,6 +2558,12 @@ extern "C" int wsrt_batch_autotune(wsrt_batch* b, wsrt_batch_tuning* best, float
             BatchArgs a = b->args;
             a.jobs = b->djobs_tune.as<BatchJob>();  // one sample per pixel
             const uint32_t blocks = (uint32_t)b->cus * bpc;
+            if (b->photons) {
+                // frame 0's grids, then frame 0 of every job
+                batch_enqueue_frame(b, 0, a, blocks);
+                a.one_frame = 1u;
+                a.frame = 0u;
+            }
             launch_batch(a, blocks, b->stream);  // warm
             (void)hipEventRecord(e0, b->stream);
             launch_batch(a, blocks, b->stream);

@@ -8,6 +8,8 @@
 // Each lane traces its pixel's samples one after the other with the same
 // per-path steps as the wavefront renderer (path_core.hpp), accumulating in
 // frame order, so a job's image equals the wavefront renderer's bit for bit.
+// With the photon pass, the launch traces one frame of every job, after the
+// host has run each job's photon pass and grid build for that frame.
 #include "kernels.hpp"
 #include "path_core.hpp"
 #include "traverse.hpp"
@@ -43,6 +45,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
             else hi = mid;
         }
         const BatchJob& J = a.jobs[lo];
+        if (a.one_frame && a.frame >= J.frames) continue;  // the job has no such sample
         const uint32_t local = item - first[lo];
         uint32_t x, y;
         if (a.tile_mode == 0u) {  // 64 x 1 pixels
@@ -57,13 +60,15 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
         if (x >= J.width || y >= J.height) continue;
         const uint32_t pix = y * J.width + x;
         const SceneView& sv = J.sv;
-        const GridView none{};
         float4 acc = J.accum[pix];
-        for (uint32_t f = 0; f < J.frames; ++f) {
+        const uint32_t f1 = a.one_frame ? a.frame + 1u : J.frames;
+        for (uint32_t f = a.one_frame ? a.frame : 0u; f < f1; ++f) {
             const FrameConst& fc = a.table[J.table_base + f];
             PathRegs r;
             camera_path(fc, x, y, r);
-            trace_bounces(sv, none, nullptr, 0u, fc.rr_after, fc.quirks, 0u, a.max_depth, r, st, false, tally);
+            // the photon gather reads the job's grid, built for this frame
+            const FrameConst* fs = (fc.enable_photons && J.grid.count) ? &fc : nullptr;
+            trace_bounces(sv, J.grid, fs, 0u, fc.rr_after, fc.quirks, 0u, a.max_depth, r, st, false, tally);
             acc = accumulate_sample(acc, fc, r.wl, r.L);
         }
         J.accum[pix] = acc;

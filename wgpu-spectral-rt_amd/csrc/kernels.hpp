@@ -25,8 +25,8 @@ int device_cus();
 // extend kernel 4 % faster than the full 4096-block grid. The block count is
 // a multiple of the queue segment count, which the segment capacity bound
 // needs, and at most kTraceBlocks (the stack spill area).
-inline int trace_grid(int waves) {
-    const int g = device_cus() * 4 * waves / (kTraceThreads / 64) / kSegs * kSegs;
+inline int trace_grid(int waves, uint32_t rounds = 1) {
+    const int g = device_cus() * 4 * waves * (int)(rounds ? rounds : 1) / (kTraceThreads / 64) / kSegs * kSegs;
     return g < kSegs ? kSegs : (g > kTraceBlocks ? kTraceBlocks : g);
 }
 constexpr uint64_t kTraceLanes = (uint64_t)kTraceThreads * kTraceBlocks;
@@ -159,6 +159,7 @@ void launch_guide_infer(const GuideView& gv, const float4* in0, const float4* in
 constexpr uint32_t kMaxBatchJobs = 256;
 struct BatchJob {
     SceneView sv;          // env null unless the config lights misses with it
+    GridView grid;         // the job's photon grid of the frame being traced (count null: no photon pass)
     float4* accum;         // width x height
     uint8_t* rgba;         // width x height x 4
     uint32_t width, height;
@@ -177,6 +178,8 @@ struct BatchArgs {
     uint32_t spill;        // spill entries per lane
     uint32_t max_depth;
     uint32_t tile_mode;    // 0: 64 x 1 pixel items, 1: 8 x 8 pixel items
+    uint32_t one_frame;    // 1: trace frame `frame` only (jobs with more samples than that)
+    uint32_t frame;
 };
 void launch_batch(const BatchArgs& a, uint32_t blocks, hipStream_t s);
 

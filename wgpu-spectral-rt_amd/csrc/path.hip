@@ -361,7 +361,7 @@ void launch_generate(const PathLaunch& pl, uint32_t npx, uint32_t nf, hipStream_
     hipLaunchKernelGGL(k_generate, dim3((n + 255) / 256), dim3(256), 0, s, pl.ps, pl.fa, npx, nf);
 }
 void launch_extend(const PathLaunch& pl, uint32_t bounce, uint32_t n0, const uint32_t* queue, hipStream_t s) {
-    hipLaunchKernelGGL(k_extend, dim3(trace_grid(5)), dim3(kTraceThreads), 0, s, pl.sv, pl.ps, pl.qs,
+    hipLaunchKernelGGL(k_extend, dim3(trace_grid(5, pl.qs.trace_rounds)), dim3(kTraceThreads), 0, s, pl.sv, pl.ps, pl.qs,
                        pl.count_traversal, pl.ctr, bounce, n0, queue, pl.gstack);
 }
 void launch_shade(const PathLaunch& pl, uint32_t npx, uint32_t bounce, uint32_t* next_queue, hipStream_t s) {
@@ -369,11 +369,11 @@ void launch_shade(const PathLaunch& pl, uint32_t npx, uint32_t bounce, uint32_t*
                        pl.fa, npx, pl.qs, bounce, pl.guided, next_queue);
 }
 void launch_shadow(const PathLaunch& pl, uint32_t bounce, hipStream_t s) {
-    hipLaunchKernelGGL(k_shadow, dim3(trace_grid(6)), dim3(kTraceThreads), 0, s, pl.sv, pl.ps, pl.qs,
+    hipLaunchKernelGGL(k_shadow, dim3(trace_grid(6, pl.qs.trace_rounds)), dim3(kTraceThreads), 0, s, pl.sv, pl.ps, pl.qs,
                        pl.count_traversal, pl.ctr, bounce, pl.gstack);
 }
 void launch_tail(const PathLaunch& pl, uint32_t npx, uint32_t max_depth, hipStream_t s) {
-    hipLaunchKernelGGL(k_tail, dim3(trace_grid(4)), dim3(kTraceThreads), 0, s, pl.sv, pl.ps, pl.grid, pl.fa, npx,
+    hipLaunchKernelGGL(k_tail, dim3(trace_grid(4, pl.qs.trace_rounds)), dim3(kTraceThreads), 0, s, pl.sv, pl.ps, pl.grid, pl.fa, npx,
                        pl.qs, max_depth, pl.count_traversal, pl.ctr, pl.gstack);
 }
 void launch_accumulate(const PathLaunch& pl, uint32_t npx, uint32_t nf, hipStream_t s) {

@@ -567,7 +567,7 @@ void launch_photon_emit(const SceneView& sv, const FrameArgs& fa, uint32_t np, u
 
 void launch_photon_bounce(const SceneView& sv, uint32_t depth, uint32_t count_traversal, const PhotonState& ps,
                           const Queues& qs, wsrt_photon* photons, Counters* ctr, uint2* gstack, hipStream_t s) {
-    hipLaunchKernelGGL(k_photon_bounce, dim3(trace_grid(5)), dim3(kTraceThreads), 0, s, sv, depth, qs,
+    hipLaunchKernelGGL(k_photon_bounce, dim3(trace_grid(5, qs.trace_rounds)), dim3(kTraceThreads), 0, s, sv, depth, qs,
                        count_traversal, ps, photons, ctr, gstack);
 }
 

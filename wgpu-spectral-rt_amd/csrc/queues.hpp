@@ -47,6 +47,10 @@ struct Queues {
     // leave them to the tail launch: a launch per bounce for a few thousand
     // paths costs its launch and latency, not its work
     uint32_t tail_paths = 0;
+    // traversal launches (extend, shadow, tail, photon bounce) use this many
+    // rounds of resident waves (wsrt_config.trace_rounds; results do not
+    // depend on it)
+    uint32_t trace_rounds = 1;
 };
 
 constexpr uint32_t kTailFirstBounce = 2;
