@@ -251,7 +251,7 @@ def copy_calibration(mib: int) -> int:
 
 
 def time_to_relmse(frames: int = 64, warmup: int = 256, boxes: int | None = None, width: int = 1024,
-                   ref_frames: int = 128, frames_in_flight: int = 64, rate_frames_in_flight=(64, 128)) -> dict:
+                   ref_frames: int = 2048, frames_in_flight: int = 64, rate_frames_in_flight=(64, 128)) -> dict:
     """Guided vs plain tracer on the occluded-light room (2,097,182 triangles
     at the default box count), 1024 x 1024, depth 8.
 
@@ -266,7 +266,9 @@ def time_to_relmse(frames: int = 64, warmup: int = 256, boxes: int | None = None
 
     Time-to-relMSE 0.01 is (C / 0.01) * (seconds per spp) plus, for the
     guided tracer, the `warmup` frames of online training it runs before
-    being measured (it keeps training while measured). Seconds per spp is
+    being measured (after them the network stays fixed:
+    guide_train_frames = warmup; training on kept C unchanged and cost 7 %
+    more per sample). Seconds per spp is
     each tracer's steady-state rate at its faster setting among
     `rate_frames_in_flight`: before any timed window the plain tracer
     renders one untimed batch (graph capture and first touches of its
@@ -307,22 +309,28 @@ def time_to_relmse(frames: int = 64, warmup: int = 256, boxes: int | None = None
     out = {"scene": f"occluded-light room, {scene.info.n_triangles} triangles, {width}x{width}, depth 8",
            "relmse": "mean (I - R)^2 / (R^2 + 1e-2)", "target": 0.01}
     S = frames_in_flight
-    rp = renderer(scenes.occluded_config(frames_in_flight=S), 1 << 22)
+    rp = renderer(scenes.occluded_config(frames_in_flight=128), 1 << 22)
     R, _ = frames_of(rp, ref_frames)
     rp.close()
     out["reference_spp"] = ref_frames
     out["frames_in_flight_for_variance"] = S
     out["measured_spp"] = frames
     res = {}
-    for name, make in (("plain", scenes.occluded_config), ("guided", scenes.guided_config)):
+    def guided(**kw):
+        return scenes.guided_config(guide_train_frames=warmup, **kw)
+
+    for name, make in (("plain", scenes.occluded_config), ("guided", guided)):
         cfg = make(frames_in_flight=S)
         r = renderer(cfg, 0)
         tw = {S: warm(r, name, S)}
         a, ta = frames_of(r, frames)
         b, tb = frames_of(r, frames)
-        if name == "guided":
-            out["mlp"] = mlp_rates(r, S, cfg.guide_train_samples)
         r.close()
+        if name == "guided":  # the network's rates with training on, one batch after a warm one
+            rm = renderer(scenes.guided_config(frames_in_flight=S), 0)
+            rm.render_frames(S)
+            out["mlp"] = mlp_rates(rm, S, cfg.guide_train_samples)
+            rm.close()
         c = frames * float(np.mean((a - b) ** 2 / (R**2 + 1e-2))) / 2
         rate = {S: (ta + tb) / (2 * frames)}
         for S2 in rate_frames_in_flight:
@@ -468,7 +476,8 @@ def small_box_throughput(frames: int = 16) -> dict:
 
 def batch_throughput(n_scenes: int = 64) -> dict:
     """64 procedurally generated scenes of uneven size (triangles log-uniform
-    1e3 .. 2e6, 128^2 .. 1024^2 pixels, 8 .. 64 spp) in one persistent launch."""
+    1e3 .. 2e6, 128^2 .. 1024^2 pixels, 8 .. 64 spp) rendered as one batch,
+    without and with the photon pass."""
     import scenes
     import wsrt
 
@@ -482,20 +491,27 @@ def batch_throughput(n_scenes: int = 64) -> dict:
         tris += sc.info.n_triangles
         samples += v.width * v.height * spp
     host_s = time.perf_counter() - t
-    cfg = wsrt.default_config(enable_photons=0, max_depth=8)
-    batch = wsrt.Batch(dev, jobs, cfg)
-    tune = batch.autotune()
-    batch.render()  # first call captures the graph
-    batch.sync()
-    t = time.perf_counter()
-    batch.render()
-    batch.sync()
-    dt = time.perf_counter() - t
-    batch.close()
-    return {"workload": f"{n_scenes} procedural scenes (displaced spheres and boxes, mixed materials), "
-                        "one persistent launch, depth 8", "scenes": n_scenes, "triangles": tris,
-            "samples": samples, "Msamples_per_s": round(samples / dt / 1e6, 3), "seconds": round(dt, 3),
-            "tuning": tune, "scene_build_s": round(host_s, 2)}
+    out = {"workload": f"{n_scenes} procedural scenes (displaced spheres and boxes, mixed materials), depth 8",
+           "scenes": n_scenes, "triangles": tris, "samples": samples, "scene_build_s": round(host_s, 2)}
+    for photons in (0, 1):
+        # without the photon pass: every sample of every job in one persistent
+        # launch; with it (the reference's frame: photon pass, grid, camera
+        # paths): each frame every job's pass of 2^20 photon paths and grid,
+        # then one persistent launch, one captured graph per frame
+        cfg = wsrt.default_config(enable_photons=photons, max_depth=8)
+        batch = wsrt.Batch(dev, jobs, cfg)
+        tune = batch.autotune()
+        batch.render()  # the first call captures the graph(s)
+        batch.sync()
+        t = time.perf_counter()
+        batch.render()
+        batch.sync()
+        dt = time.perf_counter() - t
+        batch.close()
+        out["photons" if photons else "no_photons"] = {
+            "Msamples_per_s": round(samples / dt / 1e6, 3), "seconds": round(dt, 3), "tuning": tune,
+            "photon_paths": (2**20 * sum(j[4] for j in jobs)) if photons else 0}
+    return out
 
 
 def main() -> int:

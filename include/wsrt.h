@@ -172,7 +172,10 @@ typedef struct wsrt_config {
     uint32_t trace_rounds;        /* traversal launches: rounds of resident
                                    * waves (1..4; 0 means 1). Results do not
                                    * depend on it.                             */
-    uint32_t reserved[1];
+    uint32_t guide_train_frames;  /* guiding: train online during the first
+                                   * this many frames (rounded up to whole
+                                   * batches), then keep the network fixed;
+                                   * 0: train every frame                     */
 } wsrt_config;
 
 /* wsrt_config.quirks: each bit replaces one behaviour of the reference's

@@ -1024,6 +1024,7 @@ struct wsrt_renderer {
     std::unique_ptr<wsrt_guide> guide;
     DevBuf g_in0, g_in1, g_dir, g_tp, g_Lb, g_n, g_alb, g_q;
     uint32_t guide_ns = 0, guide_stride = 1, guide_batch = 0;
+    uint64_t frames_trained = 0;  // frames whose records trained the network
 };
 
 namespace {
@@ -1239,7 +1240,8 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
         }
         g->gv.rec = guide::Records{r->g_in0.as<float4>(), r->g_in1.as<float4>(), r->g_dir.as<float4>(),
                                    r->g_tp.as<float4>(), r->g_Lb.as<float4>(), (uint32_t)npx, ns,
-                                   r->guide_stride};
+                                   r->guide_stride, guide::fast_div((uint32_t)npx),
+                                   guide::fast_div(r->guide_stride)};
         pl.gv = g->gv;
     }
     pl.ps.guide_q = r->g_q.as<uint32_t>();
@@ -1291,7 +1293,12 @@ extern "C" int wsrt_renderer_set_camera(wsrt_renderer* r, const wsrt_camera_unif
 // one wavefront; kernels read each frame's constants through the device
 // frame counter, which the batch advances by nf at its end. Used directly
 // when profiling or counting, and captured into one graph per batch size.
-static int enqueue_batch(wsrt_renderer* r, uint32_t nf) {
+// Whether the next batch trains the guide network (guide_train_frames).
+static bool batch_trains(const wsrt_renderer* r) {
+    return r->pl.guided && (r->cfg.guide_train_frames == 0 || r->frames_trained < r->cfg.guide_train_frames);
+}
+
+static int enqueue_batch(wsrt_renderer* r, uint32_t nf, bool train) {
     hipStream_t s = r->stream;
     const PathLaunch& pl = r->pl;
     const uint32_t npx = r->W * r->H;
@@ -1347,7 +1354,7 @@ static int enqueue_batch(wsrt_renderer* r, uint32_t nf) {
         ProfScope p(r, K_ACCUMULATE);
         launch_accumulate(pl, npx, nf, s);
     }
-    if (pl.guided) {
+    if (pl.guided && train) {
         // each frame's records train the network in Adam steps over
         // minibatches of guide_batch samples
         ProfScope p(r, K_GUIDE_TRAIN);
@@ -1369,8 +1376,9 @@ static int enqueue_batch(wsrt_renderer* r, uint32_t nf) {
     return WSRT_OK;
 }
 
-static hipGraphExec_t batch_graph(wsrt_renderer* r, uint32_t nf) {
-    auto it = r->graphs.find(nf);
+static hipGraphExec_t batch_graph(wsrt_renderer* r, uint32_t nf, bool train) {
+    const uint32_t key = nf | (train ? 0x10000u : 0u);
+    auto it = r->graphs.find(key);
     if (it != r->graphs.end()) return it->second.second;
     if (r->graph_failed) return nullptr;
     hipGraph_t g = nullptr;
@@ -1378,7 +1386,7 @@ static hipGraphExec_t batch_graph(wsrt_renderer* r, uint32_t nf) {
     bool ok = hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
     bool prof = r->profiling;
     r->profiling = false;
-    int rc = ok ? enqueue_batch(r, nf) : WSRT_E_HIP;
+    int rc = ok ? enqueue_batch(r, nf, train) : WSRT_E_HIP;
     r->profiling = prof;
     if (ok) ok = hipStreamEndCapture(r->stream, &g) == hipSuccess && rc == WSRT_OK;
     if (ok) ok = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) == hipSuccess;
@@ -1388,7 +1396,7 @@ static hipGraphExec_t batch_graph(wsrt_renderer* r, uint32_t nf) {
         (void)hipGetLastError();
         return nullptr;
     }
-    r->graphs[nf] = {g, ge};
+    r->graphs[key] = {g, ge};
     return ge;
 }
 
@@ -1426,13 +1434,15 @@ static int submit_frames(wsrt_renderer* r, uint32_t first, uint32_t n, const wsr
         launch_set_frame(r->dframe.as<uint32_t>(), f0, s);
         for (uint32_t i = 0; i < m;) {
             const uint32_t nf = std::min(m - i, r->S);
-            hipGraphExec_t ge = eager ? nullptr : batch_graph(r, nf);
+            const bool train = batch_trains(r);
+            hipGraphExec_t ge = eager ? nullptr : batch_graph(r, nf, train);
             if (ge) {
                 HIP_OR_FAIL(hipGraphLaunch(ge, s));
             } else {
-                int rc = enqueue_batch(r, nf);
+                int rc = enqueue_batch(r, nf, train);
                 if (rc) return rc;
             }
+            if (train) r->frames_trained += nf;
             r->last_slot = nf - 1;
             i += nf;
         }

@@ -329,10 +329,11 @@ __global__ __launch_bounds__(64 * kScatterWaves) void k_guide_scatter(PathState 
                 tp = mul4(mul4s(spec, (cw / kPi) / pm), tp);
                 if (bounce < kRecBounces && gv.rec.in0) {
                     // kept only if a training step of its frame reads it
-                    const uint32_t slot = p / gv.rec.npx;
+                    const uint32_t slot = fdiv(p, gv.rec.div_npx);
                     const uint32_t j = bounce * gv.rec.npx + (p - slot * gv.rec.npx);
-                    const uint32_t phase = (base_frame + slot) % gv.rec.stride;
-                    const uint32_t k = j >= phase ? (j - phase) / gv.rec.stride : gv.rec.ns;
+                    const uint32_t fs = base_frame + slot;
+                    const uint32_t phase = fs - fdiv(fs, gv.rec.div_stride) * gv.rec.stride;
+                    const uint32_t k = j >= phase ? fdiv(j - phase, gv.rec.div_stride) : gv.rec.ns;
                     if (k < gv.rec.ns && k * gv.rec.stride + phase == j) {
                         const size_t ri = (size_t)slot * gv.rec.ns + k;
                         gv.rec.in0[ri] = make_float4(s[0], s[1], s[2], s[3]);

@@ -98,6 +98,21 @@ constexpr float kKappaScale = 8.0f, kLogitScale = 4.0f;
 // Training records of the Lambert vertices at bounces 0 .. kRecBounces-1 of
 // each path (structure of arrays, index bounce * capacity + path id).
 constexpr uint32_t kRecBounces = 4;
+// Unsigned 32-bit division by a divisor fixed at set-up: q = (hi32(m n) + n)
+// >> l in 64 bits, with l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1
+// (Granlund and Montgomery 1994, round-up multiplier); exact for every n.
+struct FastDiv {
+    uint32_t m, l;
+};
+inline FastDiv fast_div(uint32_t d) {
+    uint32_t l = 0;
+    while (l < 32 && (1ull << l) < d) ++l;
+    return FastDiv{(uint32_t)(((1ull << 32) * ((1ull << l) - d)) / d + 1), l};
+}
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv f) {
+    return (uint32_t)(((uint64_t)__umulhi(f.m, n) + n) >> f.l);
+}
+
 // Training records: only the vertices a training step will read are kept.
 // Vertex (bounce b, pixel x) of frame slot s is record k of the slot when
 // j = b * npx + x satisfies j = k * stride + phase with k < ns and phase =
@@ -111,6 +126,7 @@ struct Records {
     uint32_t npx;     // pixels of a frame
     uint32_t ns;      // records (training samples) per frame slot
     uint32_t stride;  // vertices between consecutive records
+    FastDiv div_npx, div_stride;
 };
 
 }  // namespace guide
