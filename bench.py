@@ -154,7 +154,10 @@ def summarize_issue(pmc_dir: str, kernel: str, command: str) -> int:
     return 0
 
 
-def issue_rate(kernel: str, cnt: dict, frames_in_flight: int):
+VALU_PEAK_WAVE_INSTS = 256 * 2.4e9  # MI355X: 256 CUs x 2.4 GHz, one wave64 VALU instruction per CU and clock
+
+
+def issue_rate(kernel: str, cnt: dict, frames_in_flight: int, batch_ms: float | None = None):
     """VALU and SALU instructions per BVH8 node visit of `kernel`: the
     committed per-launch instruction counts (rocprofv3 SQ counters at this
     workload and frames_in_flight) times the launches of one batch, over the
@@ -182,6 +185,13 @@ def issue_rate(kernel: str, cnt: dict, frames_in_flight: int):
             out[f"{c}_wave_insts_per_lane_node_visit"] = round(vals[k] * launches * 64 / nodes, 1)
     if idx == 1 and cnt.get("extend_wave_nodes"):
         out["lockstep_ratio"] = round(cnt["extend_wave_nodes"] / nodes, 3)
+    if batch_ms and "sq_insts_valu_per_launch" in vals:
+        # the kernel's VALU issue rate against the chip's (unpacked fp32
+        # arithmetic: 64 lanes per CU and clock)
+        rate = vals["sq_insts_valu_per_launch"] * launches / (batch_ms * 1e-3)
+        out["valu_wave_insts_per_s"] = round(rate, 1)
+        out["valu_issue_peak"] = VALU_PEAK_WAVE_INSTS
+        out["valu_issue_frac"] = round(rate / VALU_PEAK_WAVE_INSTS, 4)
     return out
 
 
@@ -719,7 +729,7 @@ def main() -> int:
             "algorithmic_bytes_per_launch": round(d["bytes_per_ray"] * ray_counts[dom] / d["launches"]),
             "algorithmic_bytes_per_ray": d["bytes_per_ray"],
             "avg_launch_us": d["avg_us"],
-            "issue": issue_rate(dom, cnt, S),
+            "issue": issue_rate(dom, cnt, S, d["ms"]),
         },
         "event_timed_batch_ms": round(sum(p["total_ms"] for p in prof.values()), 3),
         "kernels": kern,
