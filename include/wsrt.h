@@ -176,6 +176,12 @@ typedef struct wsrt_config {
                                    * this many frames (rounded up to whole
                                    * batches), then keep the network fixed;
                                    * 0: train every frame                     */
+    uint32_t extend_refill;       /* k > 0: the closest-hit kernel gives new
+                                   * rays to a wave's idle lanes once k of
+                                   * them idle (1..64); 0: a wave's rays
+                                   * start and end together. Results do not
+                                   * depend on it.                             */
+    uint32_t reserved[3];
 } wsrt_config;
 
 /* wsrt_config.quirks: each bit replaces one behaviour of the reference's
@@ -461,6 +467,12 @@ typedef struct wsrt_autotune_result {
     uint32_t trace_rounds[8];
     float stage2_ms_per_frame[8]; /* 0: not run */
     uint32_t best_tail_paths, best_trace_rounds;
+    /* third stage, at the second stage's choice: the closest-hit kernel's
+     * lane refill threshold (extend_refill 0, 16 and 48) */
+    uint32_t n_stage3;
+    uint32_t extend_refill[4];
+    float stage3_ms_per_frame[4]; /* 0: not run */
+    uint32_t best_extend_refill;
 } wsrt_autotune_result;
 int wsrt_autotune(wsrt_device* dev, const wsrt_scene* scene, uint32_t width, uint32_t height,
                   const wsrt_camera_uniform* cam, wsrt_config* cfg, uint64_t mem_budget,

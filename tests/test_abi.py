@@ -141,3 +141,31 @@ def test_png_round_trip(tmp_path):
     wsrt.save_png(p, img)
     np.testing.assert_array_equal(wsrt.load_png(p), img)
     assert open(p, "rb").read()[:8] == b"\x89PNG\r\n\x1a\n"
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Every ctypes mirror has the size and field offsets the C compiler gives
+    the header's struct (a field added on one side only fails here)."""
+    import subprocess
+
+    mirrors = {"wsrt_camera_uniform": wsrt.CameraUniform, "wsrt_config": wsrt.Config,
+               "wsrt_autotune_result": wsrt.AutotuneResult, "wsrt_batch_job": wsrt.BatchJob,
+               "wsrt_batch_tuning": wsrt.BatchTuning, "wsrt_scene_info": wsrt.SceneInfo,
+               "wsrt_frame_stats": wsrt.FrameStats, "wsrt_kernel_stat": wsrt.KernelStat}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "wsrt.h"', "int main(void) {"]
+    for cname, cls in mirrors.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f in cls._fields_:
+            lines.append(f'printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines.append("return 0; }")
+    src, exe = tmp_path / "layout.c", tmp_path / "layout"
+    src.write_text("\n".join(lines))
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        s, f, v = line.split()
+        got[(s, f)] = int(v)
+    for cname, cls in mirrors.items():
+        assert got[(cname, "size")] == C.sizeof(cls), cname
+        for f in cls._fields_:
+            assert got[(cname, f[0])] == getattr(cls, f[0]).offset, (cname, f[0])

@@ -365,6 +365,10 @@ def test_autotune_picks_a_measured_candidate(device, cornell_pair):
     assert len(res["stage2"]) == 7 and all(c["ms_per_frame"] > 0 for c in res["stage2"])
     assert (cfg.tail_paths, cfg.trace_rounds) == (res["best_tail_paths"], res["best_trace_rounds"])
     assert cfg.trace_rounds in (1, 2) and cfg.tail_paths in (0, 1 << 16, 1 << 18, 1 << 20)
+    # third stage: lane refill threshold
+    assert [c["extend_refill"] for c in res["stage3"]] == [16, 48]
+    assert all(c["ms_per_frame"] > 0 for c in res["stage3"])
+    assert cfg.extend_refill == res["best_extend_refill"] in (0, 16, 48)
 
 
 def test_trace_rounds_do_not_change_results(device, cornell_pair):
@@ -497,3 +501,35 @@ def test_quirk_bits_are_validated(device, cornell_pair):
         wsrt.Renderer(device, g, 8, 8, wsrt.default_config(quirks=8))
     with pytest.raises(wsrt.WsrtError):
         wsrt.Renderer(device, g, 8, 8, wsrt.default_config(quirks=1, guiding=1))
+
+
+@pytest.mark.parametrize("name", ["cornell", "caustic", "heightfield", "custom"])
+def test_extend_refill_does_not_change_results(device, name):
+    """The closest-hit kernel that refills lanes as rays end traces the same
+    rays to the same hits: images, path radiance and ray counts equal the
+    plain extend kernel's bit for bit (the order of paths in the shade
+    queues differs; paths are independent). With counting on, both count
+    the same node visits."""
+    print(f"extend refill {name}", flush=True)
+    W = H = 48
+    g, _, cfg, view = setup(device, name, W, H, 1 << 14)
+    out = []
+    for refill in (0, 1, 24):
+        c = wsrt.Config.from_buffer_copy(bytes(cfg))
+        c.frames_in_flight = 3
+        c.extend_refill = refill
+        r = wsrt.Renderer(device, g, W, H, c)
+        r.set_camera(scenes.camera(view, 1))
+        r.render_frames(6)
+        acc, L = r.read_accum(), r.read_paths()[1]
+        r.reset_stats()
+        r.set_traversal_counting(True)
+        r.render_frames(3)
+        out.append((acc, L, r.stats()))
+    a = out[0][2]
+    for acc, L, b in out[1:]:
+        np.testing.assert_array_equal(out[0][0], acc)
+        np.testing.assert_array_equal(out[0][1], L)
+        assert a["extend_rays"] == b["extend_rays"] and a["shadow_rays"] == b["shadow_rays"]
+        assert a["kernel_nodes"][1] == b["kernel_nodes"][1] and a["kernel_tris"][1] == b["kernel_tris"][1]
+        assert b["kernel_nodes"][1] <= b["extend_wave_nodes"] <= 64 * b["kernel_nodes"][1]

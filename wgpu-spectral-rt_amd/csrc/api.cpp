@@ -794,6 +794,7 @@ static int validate_config(const wsrt_config* c) {
         return fail(WSRT_E_INVALID, "photons or cells times frames_in_flight too large");
     if (c->guiding > 1) return fail(WSRT_E_INVALID, "guiding %u not 0 or 1", c->guiding);
     if (c->trace_rounds > 4) return fail(WSRT_E_INVALID, "trace_rounds %u not in 0..4", c->trace_rounds);
+    if (c->extend_refill > 64) return fail(WSRT_E_INVALID, "extend_refill %u not in 0..64", c->extend_refill);
     const uint32_t known = WSRT_FIX_EMISSION_MIS | WSRT_FIX_RR_CLAMP | WSRT_FIX_ONB_SIGN;
     if (c->quirks & ~known) return fail(WSRT_E_INVALID, "quirks 0x%x has unknown bits", c->quirks);
     if ((c->quirks & WSRT_FIX_EMISSION_MIS) && c->guiding)
@@ -1206,6 +1207,7 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     pl.qs.tail_paths = c.tail_paths;
     pl.qs.trace_rounds = c.trace_rounds ? c.trace_rounds : 1u;
     pl.guided = c.guiding;
+    pl.extend_refill = c.extend_refill;
     if (c.guiding) {
         const uint32_t want = (uint32_t)std::min<uint64_t>(c.guide_train_samples, npx * guide::kRecBounces);
         // whole chunks of samples per step (at least one step of 64)
@@ -2232,6 +2234,27 @@ extern "C" int wsrt_autotune(wsrt_device* dev, const wsrt_scene* scene, uint32_t
     cfg->trace_rounds = br;
     out->best_tail_paths = bt;
     out->best_trace_rounds = br;
+    // third stage: lane refill in the closest-hit kernel, which pays where
+    // ray costs within a wave differ widely (occluded interiors)
+    const uint32_t refills[3] = {0u, 16u, 48u};
+    float best3 = best2;
+    uint32_t bx = cfg->extend_refill;
+    for (uint32_t k = 0; k < 3; ++k) {
+        if (refills[k] == cfg->extend_refill) continue;  // measured above
+        wsrt_config c = *cfg;
+        c.extend_refill = refills[k];
+        const uint32_t i = out->n_stage3++;
+        out->extend_refill[i] = refills[k];
+        uint64_t bytes = 0;
+        const float ms = autotune_time(dev, scene, width, height, cam, c, mem_budget, &bytes);
+        out->stage3_ms_per_frame[i] = ms;
+        if (ms > 0.0f && ms < best3 * 0.98f) {
+            best3 = ms;
+            bx = refills[k];
+        }
+    }
+    cfg->extend_refill = bx;
+    out->best_extend_refill = bx;
     return WSRT_OK;
 }
 

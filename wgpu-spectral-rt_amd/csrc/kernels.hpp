@@ -196,6 +196,7 @@ struct PathLaunch {
     FrameArgs fa;
     Queues qs;
     uint32_t guided;  // 1: Lambert vertices scatter through k_guide_scatter
+    uint32_t extend_refill;  // > 0: k_extend_refill, refilling when this many lanes idle
     GuideView gv;
 };
 void launch_clear_queues(const Queues& qs, uint32_t depths, hipStream_t s);

@@ -140,6 +140,185 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend
     if (count) flush_traversal_counts(ctr, kRkExtend, my_rays, my_nodes, my_tris);
 }
 
+// The same extend with lanes refilled as their rays end. A wave owns the
+// rays k_extend gives it (round k: 64 queue entries at k * stride + chunk),
+// taken in that order from a wave-uniform cursor; when at least `idle_min`
+// lanes idle, the idle lanes take the next rays, so a wave no longer waits
+// on its longest ray before starting new ones. Each loop iteration is one
+// traversal step (trav_step) of every lane holding a ray. A finished ray's
+// hit is written at once; its (path, material bin) waits in the wave's LDS
+// buffer until 64 have finished, then the 64 are sorted by path and binned
+// together (one segmented append per bin, as in k_extend), so each bin's
+// queue stays in path order within a group. Hits are the same as
+// k_extend's; only the order of paths within the shade queues differs, and
+// paths are independent.
+struct DoneBuf {
+    uint32_t p[128];
+    uint32_t bin[128];
+};
+
+__device__ __forceinline__ void bin_paths(const Queues& qs, const PathState& ps, uint32_t bounce, bool have,
+                                          uint32_t p, uint32_t bin) {
+    uint64_t pending = __ballot(have);
+    while (pending) {
+        const uint32_t leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t b = __shfl(bin, leader);
+        const bool mine = have && bin == b;
+        const uint32_t e = seg_append(qs, bounce, kQShade0 + b, qs.seg_cap, mine);
+        if (mine) {
+            ps.shade_q[(size_t)b * kSegs * qs.seg_cap + e] = p;
+            have = false;
+        }
+        pending &= ~__ballot(mine);
+    }
+}
+
+// Sorts the wave's 64 (key, value) pairs by key, ascending across lanes
+// (bitonic network over lane shuffles).
+__device__ __forceinline__ void wave_sort64(uint32_t& key, uint32_t& val) {
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (uint32_t k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t ok = (uint32_t)__shfl_xor((int)key, (int)j, 64);
+            const uint32_t ov = (uint32_t)__shfl_xor((int)val, (int)j, 64);
+            const bool up = (lane & k) == 0;       // this pair sorts ascending
+            const bool low = (lane & j) == 0;      // this lane holds the lower position
+            const bool take = low == up ? ok < key : ok > key;
+            if (take) {
+                key = ok;
+                val = ov;
+            }
+        }
+    }
+}
+
+template <bool kCount>
+__device__ __forceinline__ void extend_refill_loop(const SceneView& sv, const PathState& ps, const Queues& qs,
+                                                   Counters* __restrict__ ctr, uint32_t bounce, uint32_t n,
+                                                   const uint32_t* __restrict__ queue, const uint32_t* pre,
+                                                   Stack st, DoneBuf& db, uint32_t idle_min) {
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t nw = stride >> 6, wave = gtid >> 6, lane = __lane_id();
+    const uint32_t seq_len = (n + stride - 1) / stride * 64u;  // the wave's rounds x 64 positions
+    uint32_t cursor = 0, ndone = 0, iters = 0;                 // wave-uniform
+    uint32_t my_rays = 0, my_nodes = 0, my_tris = 0;
+    bool active = false;
+    uint32_t p = 0;
+    TravState ts;
+    while (true) {
+        const uint64_t idle = __ballot(!active);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (cursor < seq_len && (nidle >= idle_min || nidle == 64u)) {
+            if (!active) {
+                const uint32_t q = cursor + (uint32_t)__popcll(idle & lanemask_lt());
+                const uint32_t k = q >> 6;
+                const uint32_t chunk = bounce == 0 ? (wave + 977u * k) % nw : wave;
+                const uint32_t i = k * stride + (chunk << 6) + (q & 63u);
+                if (q < seq_len && i < n) {
+                    p = bounce == 0 ? i : queue[seg_entry(pre, kSegs, i, qs.seg_cap)];
+                    const float4 o = ps.o[p], d = ps.d[p];
+                    trav_init(sv, ts, xyz(o), xyz(d), 0.0001f, 100.0f);
+                    active = true;
+                    my_rays++;
+                }
+            }
+            cursor += nidle;
+        }
+        if (__ballot(active) == 0ull) {
+            if (cursor >= seq_len) break;
+            continue;  // positions past the queue's end: take the next ones
+        }
+        iters++;
+        bool to_bin = false;
+        uint32_t bin = 0;
+        if (active && !trav_step<false, kCount>(sv, ts, st)) {
+            active = false;
+            my_nodes += ts.nvis;
+            my_tris += ts.ntest;
+            const Hit& h = ts.best;
+            if (h.tri == kInvalid) {  // miss: the path ends (shader.wgsl:566)
+                if (sv.env) ps.L[p] = add4(ps.L[p], mul4(ps.tp[p], env_radiance(sv, ts.d, ps.wl[p])));
+            } else {
+                ps.hit[p] = make_uint4(__float_as_uint(h.t), h.tri | (h.front ? 0x80000000u : 0u),
+                                       __float_as_uint(h.u), __float_as_uint(h.v));
+                const uint32_t mat = sv.instances[sv.tri_instance[h.tri]].material;
+                const uint32_t type = (uint32_t)sv.materials[mat].extra[0];
+                bin = type < (uint32_t)kMatBins ? type : 0u;
+                to_bin = true;
+            }
+        }
+        // finished rays wait in the wave's buffer; 64 are binned together
+        const uint64_t tb = __ballot(to_bin);
+        if (tb) {
+            if (to_bin) {
+                const uint32_t slot = ndone + (uint32_t)__popcll(tb & lanemask_lt());
+                db.p[slot] = p;
+                db.bin[slot] = bin;
+            }
+            ndone += (uint32_t)__popcll(tb);
+            if (ndone >= 64u) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();  // other lanes' entries are read below
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                uint32_t key = db.p[lane], val = db.bin[lane];
+                wave_sort64(key, val);
+                bin_paths(qs, ps, bounce, true, key, val);
+                ndone -= 64u;
+                if (lane < ndone) {
+                    const uint32_t pp = db.p[64 + lane], bb = db.bin[64 + lane];
+                    db.p[lane] = pp;
+                    db.bin[lane] = bb;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        }
+    }
+    // the rest, sorted with empty lanes last
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    uint32_t key = lane < ndone ? db.p[lane] : 0xFFFFFFFFu, val = lane < ndone ? db.bin[lane] : 0u;
+    wave_sort64(key, val);
+    bin_paths(qs, ps, bounce, lane < ndone, key, val);
+    if (kCount) {
+        // lane slots the wave spent in the traversal loop
+        if (lane == 0 && iters) atomicAdd(&ctr->extend_wave_nodes, (unsigned long long)iters * 64ull);
+        flush_traversal_counts(ctr, kRkExtend, my_rays, my_nodes, my_tris);
+    }
+}
+
+// kCount: the traversal-counting build, a kernel of its own so that its
+// counters do not take registers from the plain one.
+template <bool kCount>
+__global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend_refill(
+    SceneView sv, PathState ps, Queues qs, Counters* __restrict__ ctr, uint32_t bounce,
+    uint32_t n0, const uint32_t* __restrict__ queue, uint2* __restrict__ gstack, uint32_t idle_min) {
+    __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
+    __shared__ uint32_t pre[kSegs + 1];
+    __shared__ DoneBuf done_all[kTraceThreads / 64];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack st;
+    st.lds = lds_slot(lds_stack + threadIdx.x);
+    st.lds_stride = kTraceThreads;
+    st.glob = gstack + (uint64_t)gtid * (sv.stack_size > kShortStack ? sv.stack_size - kShortStack : 0);
+    const uint32_t n = bounce == 0 ? n0 : seg_prefix(qs, bounce, kQActive, 1, pre);
+    if (qs.tail && bounce >= kTailFirstBounce && (*qs.tail != 0u || n < qs.tail_paths)) {
+        if (gtid == 0 && *qs.tail == 0u) *qs.tail = bounce + 1u;  // the tail launch takes these paths
+        return;
+    }
+    if (gtid == 0) {
+        atomicAdd(&ctr->extend_rays, (unsigned long long)n);
+        atomicAdd(&ctr->kernel_rays[kRkExtend], (unsigned long long)n);
+    }
+    DoneBuf& db = done_all[threadIdx.x >> 6];
+    extend_refill_loop<kCount>(sv, ps, qs, ctr, bounce, n, queue, pre, st, db, idle_min);
+}
+
 __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState ps, GridView grid,
                                                          FrameArgs fa, uint32_t npx, Queues qs,
                                                          uint32_t bounce, uint32_t guided,
@@ -361,8 +540,13 @@ void launch_generate(const PathLaunch& pl, uint32_t npx, uint32_t nf, hipStream_
     hipLaunchKernelGGL(k_generate, dim3((n + 255) / 256), dim3(256), 0, s, pl.ps, pl.fa, npx, nf);
 }
 void launch_extend(const PathLaunch& pl, uint32_t bounce, uint32_t n0, const uint32_t* queue, hipStream_t s) {
-    hipLaunchKernelGGL(k_extend, dim3(trace_grid(5, pl.qs.trace_rounds)), dim3(kTraceThreads), 0, s, pl.sv, pl.ps, pl.qs,
-                       pl.count_traversal, pl.ctr, bounce, n0, queue, pl.gstack);
+    if (pl.extend_refill)
+        hipLaunchKernelGGL(pl.count_traversal ? k_extend_refill<true> : k_extend_refill<false>,
+                           dim3(trace_grid(5, pl.qs.trace_rounds)), dim3(kTraceThreads), 0, s, pl.sv, pl.ps, pl.qs,
+                           pl.ctr, bounce, n0, queue, pl.gstack, pl.extend_refill);
+    else
+        hipLaunchKernelGGL(k_extend, dim3(trace_grid(5, pl.qs.trace_rounds)), dim3(kTraceThreads), 0, s, pl.sv,
+                           pl.ps, pl.qs, pl.count_traversal, pl.ctr, bounce, n0, queue, pl.gstack);
 }
 void launch_shade(const PathLaunch& pl, uint32_t npx, uint32_t bounce, uint32_t* next_queue, hipStream_t s) {
     hipLaunchKernelGGL(k_shade, dim3(kShadeBlocks), dim3(kShadeThreads), 0, s, pl.sv, pl.ps, pl.grid,

@@ -94,160 +94,187 @@ __device__ __forceinline__ uint32_t byte_of(uint32_t lo, uint32_t hi, int c) {
     return ((c < 4 ? lo : hi) >> (8 * (c & 3))) & 0xFFu;
 }
 
-template <bool kAny, bool kCount>
-__device__ Hit traverse(const SceneView& sv, f3 o, f3 d, float tmin, float tmax, Stack st,
-                        uint32_t* n_nodes_visited = nullptr, uint32_t* n_tris_tested = nullptr) {
+// Traversal state of one ray between node visits.
+struct TravState {
+    f3 o, d;
+    float inv[3];
+    float tmin;
+    int oct;          // direction octant: bit a set when d[a] < 0
+    int sp;           // stack entries
+    uint32_t next;    // node to visit next when have_next
+    bool have_next;
     Hit best;
-    best.t = tmax;
-    best.tri = kInvalid;
-    best.u = best.v = 0.0f;
-    best.front = false;
-    if (sv.n_nodes == 0) return best;
-    const float inv[3] = {safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
-    const float oo[3] = {o.x, o.y, o.z};
-    const bool neg[3] = {inv[0] < 0.0f, inv[1] < 0.0f, inv[2] < 0.0f};
-    const int oct = (d.x < 0.0f ? 1 : 0) | (d.y < 0.0f ? 2 : 0) | (d.z < 0.0f ? 4 : 0);
-    uint32_t nvis = 0, ntest = 0;
-    int sp = 0;
+    uint32_t nvis, ntest;
+};
+
+__device__ __forceinline__ void trav_init(const SceneView& sv, TravState& s, f3 o, f3 d, float tmin, float tmax) {
+    s.best.t = tmax;
+    s.best.tri = kInvalid;
+    s.best.u = s.best.v = 0.0f;
+    s.best.front = false;
+    s.o = o;
+    s.d = d;
+    s.inv[0] = safe_inv(d.x);
+    s.inv[1] = safe_inv(d.y);
+    s.inv[2] = safe_inv(d.z);
+    s.tmin = tmin;
+    s.oct = (d.x < 0.0f ? 1 : 0) | (d.y < 0.0f ? 2 : 0) | (d.z < 0.0f ? 4 : 0);
+    s.sp = 0;
     // the node to visit next is kept in a register: the root first, then the
     // first internal child in visiting order of each node (it would be
     // pushed and popped at once, and with best.t unchanged in between its
     // pop-time cull test could not fail)
-    uint32_t next = 0u;
-    bool have_next = true;
-    const uint4* nodes = reinterpret_cast<const uint4*>(sv.nodes);
-    while (true) {
-        if (!have_next) {
-            if (sp == 0) break;
-            const uint2 e = st.get(--sp);
-            if (!kAny && __uint_as_float(e.y) > best.t * kBoxPad) continue;
-            next = e.x;
-        }
-        have_next = false;
-        const uint4* np = nodes + 5ull * next;
-        uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
-        if (kCount) nvis++;
-        // per axis: slab scale/offset, and the byte plane of each child's near
-        // and far bound (qlo or qhi by the sign of the direction)
-        float as[3], ao[3], fs[3], fo[3], sl = 0.0f;
-        const float org[3] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)};
-        // quantized bytes: qlo x/y/z = (w2.xy, w2.zw, w3.xy), qhi x/y/z = (w3.zw, w4.xy, w4.zw)
-        const uint32_t lo_l[3] = {w2.x, w2.z, w3.x}, lo_h[3] = {w2.y, w2.w, w3.y};
-        const uint32_t hi_l[3] = {w3.z, w4.x, w4.z}, hi_h[3] = {w3.w, w4.y, w4.w};
-        uint32_t nl[3], nh[3], fl[3], fh[3];
+    s.next = 0u;
+    s.have_next = sv.n_nodes != 0u;
+    s.nvis = s.ntest = 0;
+}
+
+// One step of a traversal: pop the next node (skipping culled entries is a
+// step of its own) and visit it: box tests of its 8 children, triangle tests
+// of its leaf children, the hit internal children onto the stack. Returns
+// false once the traversal has ended (stack empty, or kAny and a hit).
+template <bool kAny, bool kCount>
+__device__ __forceinline__ bool trav_step(const SceneView& sv, TravState& s, Stack st) {
+    if (!s.have_next) {
+        if (s.sp == 0) return false;
+        const uint2 e = st.get(--s.sp);
+        if (!kAny && __uint_as_float(e.y) > s.best.t * kBoxPad) return true;
+        s.next = e.x;
+    }
+    s.have_next = false;
+    const uint4* np = reinterpret_cast<const uint4*>(sv.nodes) + 5ull * s.next;
+    uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
+    if (kCount) s.nvis++;
+    // per axis: slab scale/offset, and the byte plane of each child's near
+    // and far bound (qlo or qhi by the sign of the direction)
+    float as[3], ao[3], fs[3], fo[3], sl = 0.0f;
+    const float org[3] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)};
+    const float oo[3] = {s.o.x, s.o.y, s.o.z};
+    // quantized bytes: qlo x/y/z = (w2.xy, w2.zw, w3.xy), qhi x/y/z = (w3.zw, w4.xy, w4.zw)
+    const uint32_t lo_l[3] = {w2.x, w2.z, w3.x}, lo_h[3] = {w2.y, w2.w, w3.y};
+    const uint32_t hi_l[3] = {w3.z, w4.x, w4.z}, hi_h[3] = {w3.w, w4.y, w4.w};
+    uint32_t nl[3], nh[3], fl[3], fh[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const bool neg = (s.oct >> a) & 1;
+        const int ex = (int)(int8_t)((w0.w >> (8 * a)) & 0xFFu);
+        as[a] = ldexpf(s.inv[a], ex);  // 2^ex * inv, exact
+        ao[a] = (org[a] - oo[a]) * s.inv[a];
+        sl = fmaxf(sl, fabsf(as[a]) * 255.0f + fabsf(ao[a]));
+        nl[a] = neg ? hi_l[a] : lo_l[a];
+        nh[a] = neg ? hi_h[a] : lo_h[a];
+        fl[a] = neg ? lo_l[a] : hi_l[a];
+        fh[a] = neg ? lo_h[a] : hi_h[a];
+    }
+    // bound of the f32 error of the slab values, applied to both ends:
+    // a child is hit when its near distance is at most its padded far
+    // distance, far * kBoxPad + sl2, with the padding folded into the
+    // far planes' constants
+    sl = sl * 4.0e-7f;
+    const float sl2 = sl * (1.0f + kBoxPad);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        fs[a] = as[a] * kBoxPad;
+        fo[a] = __fmaf_rn(ao[a], kBoxPad, sl2);
+    }
+    const float tcap = __fmaf_rn(s.best.t, kBoxPad, sl2);
+    const uint32_t child_base = w1.x, tri_base = w1.y;
+    float tnear[8];
+    uint32_t hitmask = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        float tn = s.tmin, tf = tcap;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const int ex = (int)(int8_t)((w0.w >> (8 * a)) & 0xFFu);
-            as[a] = ldexpf(inv[a], ex);  // 2^ex * inv, exact
-            ao[a] = (org[a] - oo[a]) * inv[a];
-            sl = fmaxf(sl, fabsf(as[a]) * 255.0f + fabsf(ao[a]));
-            nl[a] = neg[a] ? hi_l[a] : lo_l[a];
-            nh[a] = neg[a] ? hi_h[a] : lo_h[a];
-            fl[a] = neg[a] ? lo_l[a] : hi_l[a];
-            fh[a] = neg[a] ? lo_h[a] : hi_h[a];
+            tn = fmaxf(tn, __fmaf_rn((float)byte_of(nl[a], nh[a], c), as[a], ao[a]));
+            tf = fminf(tf, __fmaf_rn((float)byte_of(fl[a], fh[a], c), fs[a], fo[a]));
         }
-        // bound of the f32 error of the slab values, applied to both ends:
-        // a child is hit when its near distance is at most its padded far
-        // distance, far * kBoxPad + sl2, with the padding folded into the
-        // far planes' constants
-        sl = sl * 4.0e-7f;
-        const float sl2 = sl * (1.0f + kBoxPad);
+        tnear[c] = tn;
+        // an empty slot (meta 0) is a leaf of no triangles; its box is
+        // written inverted, so it is not hit in practice either
+        if (tn <= tf) hitmask |= 1u << c;
+    }
+    // slot classes: bit c of the node's imask byte is set when child c is
+    // internal (meta bit 7), stored by the builder
+    const uint32_t imask = w0.w >> 24;
+    // leaf children, in slot order
+    uint32_t leaves = hitmask & ~imask;
+    while (leaves) {
+        const int c = __ffs(leaves) - 1;
+        leaves &= leaves - 1;
+        const uint32_t meta = ((c < 4 ? w1.z : w1.w) >> (8 * (c & 3))) & 0xFFu;
+        const uint32_t cnt = meta >> 5, off = meta & 0x1Fu;
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const float4* tr = sv.tris + 3ull * (tri_base + off + k);
+            float4 a = tr[0], b = tr[1], cc = tr[2];
+            if (kCount) s.ntest++;
+            float tt, u, v;
+            bool fr;
+            if (!tri_intersect(s.o, s.d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), tt, u, v,
+                               fr))
+                continue;
+            if (!(tt > s.tmin)) continue;
+            uint32_t id = __float_as_uint(a.w);
+            if (tt < s.best.t || (tt == s.best.t && s.best.tri != kInvalid && id < s.best.tri)) {
+                s.best.t = tt;
+                s.best.tri = id;
+                s.best.u = u;
+                s.best.v = v;
+                s.best.front = fr;
+                if (kAny) return false;
+            }
+        }
+    }
+    // internal children in the order of the ray's octant over the slots:
+    // slot c is pushed at rank (c ^ oct) from the top, so slot oct is
+    // popped first; positions come from a mask permuted by oct.
+    // A child is culled when its near distance, less the error bound, is
+    // beyond the closest hit so far (padded).
+    const float cull = __fmaf_rn(s.best.t, kBoxPad, sl);
+    uint32_t push = 0;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            fs[a] = as[a] * kBoxPad;
-            fo[a] = __fmaf_rn(ao[a], kBoxPad, sl2);
-        }
-        const float tcap = __fmaf_rn(best.t, kBoxPad, sl2);
-        const uint32_t child_base = w1.x, tri_base = w1.y;
-        float tnear[8];
-        uint32_t hitmask = 0;
+    for (int c = 0; c < 8; ++c) push |= (!(tnear[c] > cull) ? 1u : 0u) << c;
+    push &= hitmask & imask;
+    // ranks: bit c of push at position c ^ oct (swap bits, pairs, nibbles)
+    const int oct = s.oct;
+    uint32_t pm = push;
+    if (oct & 1) pm = ((pm & 0x55u) << 1) | ((pm >> 1) & 0x55u);
+    if (oct & 2) pm = ((pm & 0x33u) << 2) | ((pm >> 2) & 0x33u);
+    if (oct & 4) pm = ((pm & 0x0Fu) << 4) | ((pm >> 4) & 0x0Fu);
+    if (pm) {
+        // the lowest rank goes to the register, the others onto the stack
+        // with lower ranks nearer the top
+        const int first = (__ffs(pm) - 1) ^ oct;
+        const uint32_t rest = pm & (pm - 1u);
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            float tn = tmin, tf = tcap;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                tn = fmaxf(tn, __fmaf_rn((float)byte_of(nl[a], nh[a], c), as[a], ao[a]));
-                tf = fminf(tf, __fmaf_rn((float)byte_of(fl[a], fh[a], c), fs[a], fo[a]));
+            if (!((push >> c) & 1u)) continue;
+            const uint32_t meta = byte_of(w1.z, w1.w, c);
+            if (c == first) {
+                s.next = child_base + (meta & 0x7Fu);
+                continue;
             }
-            tnear[c] = tn;
-            // an empty slot (meta 0) is a leaf of no triangles; its box is
-            // written inverted, so it is not hit in practice either
-            if (tn <= tf) hitmask |= 1u << c;
+            const int pos = s.sp + __popc(rest >> ((c ^ oct) + 1));
+            st.put(pos, make_uint2(child_base + (meta & 0x7Fu), __float_as_uint(tnear[c] - sl)));
         }
-        // slot classes: bit c of the node's imask byte is set when child c is
-        // internal (meta bit 7), stored by the builder
-        const uint32_t imask = w0.w >> 24;
-        // leaf children, in slot order
-        uint32_t leaves = hitmask & ~imask;
-        while (leaves) {
-            const int c = __ffs(leaves) - 1;
-            leaves &= leaves - 1;
-            const uint32_t meta = ((c < 4 ? w1.z : w1.w) >> (8 * (c & 3))) & 0xFFu;
-            const uint32_t cnt = meta >> 5, off = meta & 0x1Fu;
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const float4* tr = sv.tris + 3ull * (tri_base + off + k);
-                float4 a = tr[0], b = tr[1], cc = tr[2];
-                if (kCount) ntest++;
-                float tt, u, v;
-                bool fr;
-                if (!tri_intersect(o, d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z),
-                                   tt, u, v, fr))
-                    continue;
-                if (!(tt > tmin)) continue;
-                uint32_t id = __float_as_uint(a.w);
-                if (tt < best.t || (tt == best.t && best.tri != kInvalid && id < best.tri)) {
-                    best.t = tt;
-                    best.tri = id;
-                    best.u = u;
-                    best.v = v;
-                    best.front = fr;
-                    if (kAny) {
-                        if (kCount) { *n_nodes_visited = nvis; *n_tris_tested = ntest; }
-                        return best;
-                    }
-                }
-            }
-        }
-        // internal children in the order of the ray's octant over the slots:
-        // slot c is pushed at rank (c ^ oct) from the top, so slot oct is
-        // popped first; positions come from a mask permuted by oct
-        // a child is culled when its near distance, less the error bound, is
-        // beyond the closest hit so far (padded)
-        const float cull = __fmaf_rn(best.t, kBoxPad, sl);
-        uint32_t push = 0;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) push |= (!(tnear[c] > cull) ? 1u : 0u) << c;
-        push &= hitmask & imask;
-        // ranks: bit c of push at position c ^ oct (swap bits, pairs, nibbles)
-        uint32_t pm = push;
-        if (oct & 1) pm = ((pm & 0x55u) << 1) | ((pm >> 1) & 0x55u);
-        if (oct & 2) pm = ((pm & 0x33u) << 2) | ((pm >> 2) & 0x33u);
-        if (oct & 4) pm = ((pm & 0x0Fu) << 4) | ((pm >> 4) & 0x0Fu);
-        if (pm) {
-            // the lowest rank goes to the register, the others onto the stack
-            // with lower ranks nearer the top
-            const int first = (__ffs(pm) - 1) ^ oct;
-            const uint32_t rest = pm & (pm - 1u);
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                if (!((push >> c) & 1u)) continue;
-                const uint32_t meta = byte_of(w1.z, w1.w, c);
-                if (c == first) {
-                    next = child_base + (meta & 0x7Fu);
-                    continue;
-                }
-                const int pos = sp + __popc(rest >> ((c ^ oct) + 1));
-                st.put(pos, make_uint2(child_base + (meta & 0x7Fu), __float_as_uint(tnear[c] - sl)));
-            }
-            sp += __popc(rest);
-            have_next = true;
-        }
+        s.sp += __popc(rest);
+        s.have_next = true;
+    }
+    return true;
+}
+
+template <bool kAny, bool kCount>
+__device__ Hit traverse(const SceneView& sv, f3 o, f3 d, float tmin, float tmax, Stack st,
+                        uint32_t* n_nodes_visited = nullptr, uint32_t* n_tris_tested = nullptr) {
+    TravState s;
+    trav_init(sv, s, o, d, tmin, tmax);
+    while (trav_step<kAny, kCount>(sv, s, st)) {
     }
     if (kCount) {
-        *n_nodes_visited = nvis;
-        *n_tris_tested = ntest;
+        *n_nodes_visited = s.nvis;
+        *n_tris_tested = s.ntest;
     }
-    return best;
+    return s.best;
 }
 
 // Whether a ray (o, d) on (tmin, tmax) meets any child box of the root of

@@ -43,7 +43,8 @@ class Config(C.Structure):
                 ("guide_mix", C.c_float), ("guide_lr", C.c_float),
                 ("guide_train_samples", C.c_uint32), ("env_map", C.c_uint32),
                 ("frame_graphs", C.c_uint32), ("frames_in_flight", C.c_uint32),
-                ("tail_paths", C.c_uint32), ("quirks", C.c_uint32), ("trace_rounds", C.c_uint32), ("guide_train_frames", C.c_uint32)]
+                ("tail_paths", C.c_uint32), ("quirks", C.c_uint32), ("trace_rounds", C.c_uint32), ("guide_train_frames", C.c_uint32),
+                ("extend_refill", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 class AutotuneResult(C.Structure):
@@ -51,7 +52,8 @@ class AutotuneResult(C.Structure):
                 ("ms_per_frame", C.c_float * 12), ("device_bytes", C.c_uint64 * 12), ("best", C.c_uint32),
                 ("n_stage2", C.c_uint32), ("tail_paths", C.c_uint32 * 8), ("trace_rounds", C.c_uint32 * 8),
                 ("stage2_ms_per_frame", C.c_float * 8), ("best_tail_paths", C.c_uint32),
-                ("best_trace_rounds", C.c_uint32)]
+                ("best_trace_rounds", C.c_uint32), ("n_stage3", C.c_uint32), ("extend_refill", C.c_uint32 * 4),
+                ("stage3_ms_per_frame", C.c_float * 4), ("best_extend_refill", C.c_uint32)]
 
 
 class BatchJob(C.Structure):
@@ -700,8 +702,8 @@ def load_png(path: str) -> np.ndarray:
 
 def autotune(device: "Device", scene: "Scene", width: int, height: int, config: Config,
              camera: CameraUniform | None = None, mem_budget: int = 0) -> dict:
-    """Picks config.frames_in_flight, then tail_paths and trace_rounds, by
-    timing candidates (wsrt_autotune) for the given camera; updates config in
+    """Picks config.frames_in_flight, then tail_paths and trace_rounds, then
+    extend_refill, by timing candidates (wsrt_autotune) for the given camera; updates config in
     place and returns the timings."""
     res = AutotuneResult()
     cam = C.cast(C.byref(camera), vp) if camera is not None else None
@@ -714,7 +716,10 @@ def autotune(device: "Device", scene: "Scene", width: int, height: int, config: 
                             "device_mib": round(res.device_bytes[i] / 2**20, 1)} for i in range(n)],
             "best_tail_paths": res.best_tail_paths, "best_trace_rounds": res.best_trace_rounds,
             "stage2": [{"tail_paths": res.tail_paths[i], "trace_rounds": res.trace_rounds[i],
-                        "ms_per_frame": round(res.stage2_ms_per_frame[i], 4)} for i in range(res.n_stage2)]}
+                        "ms_per_frame": round(res.stage2_ms_per_frame[i], 4)} for i in range(res.n_stage2)],
+            "best_extend_refill": res.best_extend_refill,
+            "stage3": [{"extend_refill": res.extend_refill[i], "ms_per_frame": round(res.stage3_ms_per_frame[i], 4)}
+                       for i in range(res.n_stage3)]}
 
 
 class Batch:
