@@ -181,7 +181,11 @@ typedef struct wsrt_config {
                                    * them idle (1..64); 0: a wave's rays
                                    * start and end together. Results do not
                                    * depend on it.                             */
-    uint32_t reserved[3];
+    uint32_t ordered_queues;      /* 1: each bounce's rays are queued in path
+                                   * order (from per-path tags) instead of in
+                                   * the order shading appended them. Results
+                                   * do not depend on it.                      */
+    uint32_t reserved[2];
 } wsrt_config;
 
 /* wsrt_config.quirks: each bit replaces one behaviour of the reference's
@@ -354,6 +358,10 @@ typedef struct wsrt_frame_stats {
      * times the lanes that traversed, summed. Its ratio to kernel_nodes[1]
      * (>= 1) is the visits a wave spends waiting on its longest ray. */
     uint64_t extend_wave_nodes;
+    /* While counting is enabled: the most node visits of a single ray, per
+     * launching kernel as kernel_nodes (kept for [1] extend and [2] shadow;
+     * 0 elsewhere). */
+    uint64_t ray_max_nodes[6];
 } wsrt_frame_stats;
 int wsrt_renderer_get_stats(wsrt_renderer* r, wsrt_frame_stats* out);
 int wsrt_renderer_reset_stats(wsrt_renderer* r);

@@ -46,7 +46,8 @@ class Config(C.Structure):
                 ("guide_train_samples", C.c_uint32), ("env_map", C.c_uint32),
                 ("frame_graphs", C.c_uint32), ("frames_in_flight", C.c_uint32),
                 ("tail_paths", C.c_uint32), ("quirks", C.c_uint32), ("trace_rounds", C.c_uint32), ("guide_train_frames", C.c_uint32),
-                ("extend_refill", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("extend_refill", C.c_uint32), ("ordered_queues", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
 
 def default_config(**kw) -> Config:

@@ -715,22 +715,21 @@ void or_bvh8_closest(const wsrt_bvh8_node* nodes, uint32_t n_nodes, const wsrt_b
             if (st_t[sp] > best_t * BOX_PAD) continue;
             const wsrt_bvh8_node* nd = &nodes[ni];
             nv++;
-            float as[3], ao[3], fs[3], fo[3], sl = 0.0f;
+            float as[3], an[3], fs[3], fo[3];
             for (int a = 0; a < 3; ++a) {
                 as[a] = ldexpf(inv[a], nd->exp[a]); /* 2^exp * inv, exact */
-                ao[a] = (nd->origin[a] - oo[a]) * inv[a];
-                sl = fmaxf(sl, fabsf(as[a]) * 255.0f + fabsf(ao[a]));
-            }
-            /* bound of the f32 error of the slab values, applied to both
-             * ends: hit when near <= far * BOX_PAD + sl2, the padding folded
-             * into the far planes' constants */
-            sl = sl * 4.0e-7f;
-            const float sl2 = sl * (1.0f + BOX_PAD);
-            for (int a = 0; a < 3; ++a) {
+                const float ao = (nd->origin[a] - oo[a]) * inv[a];
+                /* bound of the f32 error of this axis' slab values, off its
+                 * near planes and onto its far planes (padding folded into
+                 * the far planes' constants): hit when near - e <= far *
+                 * BOX_PAD + e (1 + BOX_PAD) on every axis and near - e <=
+                 * best_t * BOX_PAD */
+                const float e = (fabsf(as[a]) * 255.0f + fabsf(ao)) * 4.0e-7f;
+                an[a] = ao - e;
                 fs[a] = as[a] * BOX_PAD;
-                fo[a] = fmaf(ao[a], BOX_PAD, sl2);
+                fo[a] = fmaf(ao, BOX_PAD, e * (1.0f + BOX_PAD));
             }
-            const float tcap = fmaf(best_t, BOX_PAD, sl2);
+            const float tcap = best_t * BOX_PAD;
             float tnear[8];
             int chit[8];
             for (int c = 0; c < 8; ++c) {
@@ -739,7 +738,7 @@ void or_bvh8_closest(const wsrt_bvh8_node* nodes, uint32_t n_nodes, const wsrt_b
                     int ng = inv[a] < 0.0f;
                     float qn = (float)(ng ? nd->qhi[a][c] : nd->qlo[a][c]);
                     float qf = (float)(ng ? nd->qlo[a][c] : nd->qhi[a][c]);
-                    tn = fmaxf(tn, fmaf(qn, as[a], ao[a]));
+                    tn = fmaxf(tn, fmaf(qn, as[a], an[a]));
                     tf = fminf(tf, fmaf(qf, fs[a], fo[a]));
                 }
                 tnear[c] = tn;
@@ -767,13 +766,13 @@ void or_bvh8_closest(const wsrt_bvh8_node* nodes, uint32_t n_nodes, const wsrt_b
             }
             /* internal children in the ray-octant order of the slots: slot
              * (k ^ oct) pushed for k = 7 .. 0, so slot oct is popped first */
-            const float cull = fmaf(best_t, BOX_PAD, sl);
+            const float cull = best_t * BOX_PAD;
             for (int k = 7; k >= 0; --k) {
                 int c = k ^ oct;
                 if (!chit[c] || !(nd->meta[c] & 0x80)) continue;
                 if (tnear[c] > cull) continue;
                 st_node[sp] = nd->child_base + (nd->meta[c] & 0x7F);
-                st_t[sp] = tnear[c] - sl;
+                st_t[sp] = tnear[c];
                 sp++;
             }
         }

@@ -149,16 +149,20 @@ def frame_means(r, frames):
     return np.array(out)
 
 
-def test_guided_estimate_equals_plain(device):
+@pytest.mark.parametrize("quirks", [0, wsrt.FIX_EMISSION_MIS])
+def test_guided_estimate_equals_plain(device, quirks):
     """One-sample MIS keeps the estimator unbiased: the mean image of the
-    guided tracer equals the plain tracer's within the Monte Carlo error."""
-    print("guided vs plain mean", flush=True)
+    guided tracer equals the plain tracer's within the Monte Carlo error,
+    with the reference's emission weighting and with the balance heuristic
+    (light samples weighted against the guide's mixture pdf)."""
+    print(f"guided vs plain mean, quirks {quirks}", flush=True)
     scene = occluded_small().build(device)
     view = scenes.View(scenes.OCCLUDED_VIEW.position, scenes.OCCLUDED_VIEW.yaw, scenes.OCCLUDED_VIEW.pitch,
                        64, 64)
     means = {}
-    for name, cfg in (("plain", scenes.occluded_config(frames_in_flight=1)),
-                      ("guided", scenes.guided_config(frames_in_flight=1, guide_train_samples=4096))):
+    for name, cfg in (("plain", scenes.occluded_config(frames_in_flight=1, quirks=quirks)),
+                      ("guided", scenes.guided_config(frames_in_flight=1, guide_train_samples=4096,
+                                                      quirks=quirks))):
         r = wsrt.Renderer(device, scene, 64, 64, cfg)
         r.set_camera(scenes.camera(view, 0))
         means[name] = frame_means(r, 96)

@@ -154,17 +154,20 @@ def test_occluded_room_at_full_size(device, full_room):
                 min_lit=0.02)
 
 
-def test_guided_mean_equals_plain_on_the_full_room(device, full_room):
+@pytest.mark.parametrize("quirks", [0, wsrt.FIX_EMISSION_MIS])
+def test_guided_mean_equals_plain_on_the_full_room(device, full_room, quirks):
     """One-sample MIS keeps the guided estimator unbiased on the workload's
     own scene: per-frame image means of guided and plain tracers agree
-    within their Monte Carlo error (z < 4.5 per channel) at 256 x 256."""
-    print("guided vs plain mean, full room, 256x256", flush=True)
+    within their Monte Carlo error (z < 4.5 per channel) at 256 x 256, with
+    the reference's emission weighting and with the balance heuristic."""
+    print(f"guided vs plain mean, full room, 256x256, quirks {quirks}", flush=True)
     g, _ = full_room
     v = scenes.OCCLUDED_VIEW
     view = scenes.View(v.position, v.yaw, v.pitch, 256, 256)
     means = {}
-    for name, cfg in (("plain", scenes.occluded_config(frames_in_flight=1)),
-                      ("guided", scenes.guided_config(frames_in_flight=1, guide_train_samples=1 << 14))):
+    for name, cfg in (("plain", scenes.occluded_config(frames_in_flight=1, quirks=quirks)),
+                      ("guided", scenes.guided_config(frames_in_flight=1, guide_train_samples=1 << 14,
+                                                      quirks=quirks))):
         r = wsrt.Renderer(device, g, 256, 256, cfg)
         r.set_camera(scenes.camera(view, 0))
         out, prev = [], np.zeros(3)

@@ -795,10 +795,9 @@ static int validate_config(const wsrt_config* c) {
     if (c->guiding > 1) return fail(WSRT_E_INVALID, "guiding %u not 0 or 1", c->guiding);
     if (c->trace_rounds > 4) return fail(WSRT_E_INVALID, "trace_rounds %u not in 0..4", c->trace_rounds);
     if (c->extend_refill > 64) return fail(WSRT_E_INVALID, "extend_refill %u not in 0..64", c->extend_refill);
+    if (c->ordered_queues > 1) return fail(WSRT_E_INVALID, "ordered_queues %u not 0 or 1", c->ordered_queues);
     const uint32_t known = WSRT_FIX_EMISSION_MIS | WSRT_FIX_RR_CLAMP | WSRT_FIX_ONB_SIGN;
     if (c->quirks & ~known) return fail(WSRT_E_INVALID, "quirks 0x%x has unknown bits", c->quirks);
-    if ((c->quirks & WSRT_FIX_EMISSION_MIS) && c->guiding)
-        return fail(WSRT_E_UNSUPPORTED, "WSRT_FIX_EMISSION_MIS is not available with guiding");
     if (c->guiding) {
         if (!(c->guide_mix >= 0.0f && c->guide_mix <= 1.0f))
             return fail(WSRT_E_INVALID, "guide_mix %g not in [0, 1]", (double)c->guide_mix);
@@ -858,11 +857,11 @@ static FrameConst make_frame_const(const wsrt_config& c, const wsrt_camera_unifo
 namespace {
 enum KernelId {
     K_FRAME_PREP, K_PHOTON_EMIT, K_PHOTON_TRACE, K_GRID_BUILD, K_GENERATE, K_EXTEND, K_SHADE, K_SHADOW,
-    K_ACCUMULATE, K_GUIDE_SCATTER, K_GUIDE_TRAIN, K_TAIL, K_COUNT
+    K_ACCUMULATE, K_GUIDE_SCATTER, K_GUIDE_TRAIN, K_TAIL, K_ORDER, K_COUNT
 };
 const char* kKernelNames[K_COUNT] = {"frame_prep", "photon_emit", "photon_trace", "grid_build", "generate",
                                      "extend", "shade", "shadow", "accumulate", "guide_scatter",
-                                     "guide_train", "tail"};
+                                     "guide_train", "tail", "order"};
 constexpr uint32_t kUploadHalf = kFrameTable / 2;  // frames per upload (staging half)
 struct Pending {
     int kid;
@@ -997,6 +996,7 @@ struct wsrt_renderer {
     uint32_t frame_count = 0;
     // buffers
     DevBuf photons, grid_count, grid_offset, grid_cell, grid_slot, grid_sorted, grid_nvalid, grid_temp, grid_touched;
+    DevBuf path_tag;  // ordered queues: per-path bounce tags (whole 256-path tiles)
     DevBuf path_o, path_d, path_tp, path_L, path_wl, path_rng, path_flags, path_hit, q_a, q_b, shade_q,
         shadow;
     DevBuf accum, rgba, counters, gstack, qcounts;
@@ -1131,6 +1131,11 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     HIP_OR_FAIL(r->path_wl.alloc(16 * npaths));
     HIP_OR_FAIL(r->path_rng.alloc(4 * npaths));
     HIP_OR_FAIL(r->path_flags.alloc(4 * npaths));
+    if (c.ordered_queues) {
+        const uint64_t tag_bytes = (npaths + 255) / 256 * 256;
+        HIP_OR_FAIL(r->path_tag.alloc(tag_bytes));
+        HIP_OR_FAIL(hipMemsetAsync(r->path_tag.p, 0, tag_bytes, r->stream));
+    }
     HIP_OR_FAIL(r->path_hit.alloc(16 * npaths));
     // with guiding, k_shade and k_guide_scatter both fill a bounce's active
     // queue, so each segment may receive two producers' worth of entries
@@ -1180,6 +1185,7 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     pl.ps.wl = r->path_wl.as<float4>();
     pl.ps.rng = r->path_rng.as<uint32_t>();
     pl.ps.flags = r->path_flags.as<uint32_t>();
+    pl.ps.tag = c.ordered_queues ? r->path_tag.as<uint8_t>() : nullptr;
     pl.ps.hit = r->path_hit.as<uint4>();
     pl.ps.queue_a = r->q_a.as<uint32_t>();
     pl.ps.queue_b = r->q_b.as<uint32_t>();
@@ -1208,6 +1214,7 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     pl.qs.trace_rounds = c.trace_rounds ? c.trace_rounds : 1u;
     pl.guided = c.guiding;
     pl.extend_refill = c.extend_refill;
+    pl.guide_mis = (c.guiding && (c.quirks & WSRT_FIX_EMISSION_MIS)) ? 1u : 0u;
     if (c.guiding) {
         const uint32_t want = (uint32_t)std::min<uint64_t>(c.guide_train_samples, npx * guide::kRecBounces);
         // whole chunks of samples per step (at least one step of 64)
@@ -1346,6 +1353,10 @@ static int enqueue_batch(wsrt_renderer* r, uint32_t nf, bool train) {
         if (pl.guided) {
             ProfScope p(r, K_GUIDE_SCATTER);
             launch_guide_scatter(pl.ps, pl.qs, pl.gv, pl.fa, b, nq, pl.ctr, s);
+        }
+        if (pl.ps.tag && b + 1 < r->cfg.max_depth) {
+            ProfScope p(r, K_ORDER);
+            launch_order_active(pl, b + 1, npx * nf, nq, s);
         }
     }
     if (pl.qs.tail && r->cfg.max_depth > kTailFirstBounce) {
@@ -1641,6 +1652,7 @@ extern "C" int wsrt_renderer_get_stats(wsrt_renderer* r, wsrt_frame_stats* out) 
         out->kernel_counted_rays[k] = c.trav_rays[k];
     }
     out->extend_wave_nodes = c.extend_wave_nodes;
+    for (int k = 0; k < 6; ++k) out->ray_max_nodes[k] = c.ray_max_nodes[k];
     return WSRT_OK;
 }
 

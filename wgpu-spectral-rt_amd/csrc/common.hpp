@@ -88,6 +88,8 @@ struct Counters {
     // wavefront extend, while counting: per wave and round, the most node
     // visits of any lane times the lanes that traversed, summed
     unsigned long long extend_wave_nodes;
+    // while counting: the most node visits of one ray, per launching kernel
+    unsigned long long ray_max_nodes[6];
     unsigned long long photon_rays_part[kStatBlocks];
     unsigned long long photons_stored_part[kStatBlocks];
 };
@@ -119,6 +121,9 @@ struct PathState {
     uint32_t* guide_q;  // Lambert vertices for guided scattering (segmented, 64 * seg_cap)
     float4* gn;         // forward-facing shading normal at a guided vertex
     float4* galb;       // spectral albedo at a guided vertex
+    // ordered queues (null: off): tag[p] = b when path p enters bounce b;
+    // the active queue of b is then built from the tags in path order
+    uint8_t* tag;
     uint32_t capacity;  // path slots (frames in flight x pixels)
 };
 

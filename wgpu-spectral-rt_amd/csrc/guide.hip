@@ -264,20 +264,25 @@ __global__ __launch_bounds__(64 * kScatterWaves) void k_guide_scatter(PathState 
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctr->guide_samples, (unsigned long long)n);
     const uint32_t base_frame = *fa.frame;
     const uint32_t rr_after = fa.table[base_frame & fa.mask].rr_after;
+    const bool mis = (fa.table[base_frame & fa.mask].quirks & WSRT_FIX_EMISSION_MIS) != 0u;
     const int wave = threadIdx.x >> 6, l = __lane_id();
     float* stage = stage_all[wave];
     const uint32_t wstride = gridDim.x * kScatterWaves * 64;
     for (uint32_t base = (blockIdx.x * kScatterWaves + wave) * 64; base < n; base += wstride) {
         const uint32_t i = base + l;
         const bool have = i < n;
-        uint32_t p = 0;
+        uint32_t p = 0, se = 0xFFFFFFFFu;
         f3 hp = mk3(0, 0, 0), ff = mk3(0, 0, 1);
         float4 wl = splat4(500.0f);
+        float lp = 0.0f;
         float s[kScalars];
         if (have) {
             p = ps.guide_q[seg_entry(pre, kSegs, i, qs.seg_cap)];
-            hp = xyz(ps.o[p]);
-            ff = xyz(ps.gn[p]);
+            const float4 o4 = ps.o[p], n4 = ps.gn[p];
+            hp = xyz(o4);
+            lp = o4.w;
+            ff = xyz(n4);
+            se = __float_as_uint(n4.w);
             wl = ps.wl[p];
         }
         scalars_of(gv, hp, ff, wl.x, s);
@@ -297,6 +302,23 @@ __global__ __launch_bounds__(64 * kScatterWaves) void k_guide_scatter(PathState 
             uint32_t flags = ps.flags[p];
             float4 tp = ps.tp[p];
             const float4 spec = ps.galb[p];
+            float4 Lcur = ps.L[p];
+            if (mis && lp > 0.0f && se != 0xFFFFFFFFu) {
+                // balance heuristic of the light sample against the
+                // directions this vertex samples (guide mixed with cosine):
+                // lp / (lp + p_mix) in place of the shade kernel's
+                // lp / (lp + cos / pi), applied to a contribution the shadow
+                // kernel found unoccluded
+                const float4* sr = ps.shadow + 3ull * se;
+                if (sr[0].w < 0.0f) {
+                    const float4 lb = sr[1];
+                    const f3 ld = mk3(lb.x, lb.y, lb.z);
+                    const float bp = fmaxf(dot(ff, ld), 0.0f) / kPi;
+                    const float pl = gv.mix * mixture_pdf(m, ld) + (1.0f - gv.mix) * bp;
+                    Lcur = add4(Lcur, mul4s(sr[2], (lp + bp) / (lp + pl) - 1.0f));
+                    ps.L[p] = Lcur;
+                }
+            }
             f3 w;
             if (rand01(rng) < gv.mix) {
                 const float xl = rand01(rng);
@@ -340,7 +362,7 @@ __global__ __launch_bounds__(64 * kScatterWaves) void k_guide_scatter(PathState 
                         gv.rec.in1[ri] = make_float4(s[4], s[5], s[6], pm);
                         gv.rec.dir[ri] = make_float4(w.x, w.y, w.z, cw);
                         gv.rec.tp[ri] = tp;
-                        gv.rec.Lb[ri] = ps.L[p];
+                        gv.rec.Lb[ri] = Lcur;
                         flags |= kFlagRecorded << bounce;
                     }
                 }
@@ -355,15 +377,21 @@ __global__ __launch_bounds__(64 * kScatterWaves) void k_guide_scatter(PathState 
                     else tp = div4s(tp, pr);
                 }
                 if (alive) {
-                    ps.d[p] = f4(w, 0.0f);
+                    // w: the mixture pdf of the direction, for the balance
+                    // weight of emission it reaches (WSRT_FIX_EMISSION_MIS)
+                    ps.d[p] = f4(w, mis ? pm : 0.0f);
                     ps.tp[p] = tp;
                     ps.rng[p] = rng;
                     ps.flags[p] = flags;
                 }
             }
         }
-        const uint32_t ne = seg_append(qs, bounce + 1, kQActive, qs.seg_cap, alive);
-        if (alive) next_queue[ne] = p;
+        if (ps.tag) {
+            if (alive) ps.tag[p] = (uint8_t)(bounce + 1);
+        } else {
+            const uint32_t ne = seg_append(qs, bounce + 1, kQActive, qs.seg_cap, alive);
+            if (alive) next_queue[ne] = p;
+        }
     }
 }
 

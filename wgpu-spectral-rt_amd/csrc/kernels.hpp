@@ -197,6 +197,7 @@ struct PathLaunch {
     Queues qs;
     uint32_t guided;  // 1: Lambert vertices scatter through k_guide_scatter
     uint32_t extend_refill;  // > 0: k_extend_refill, refilling when this many lanes idle
+    uint32_t guide_mis;      // guided with WSRT_FIX_EMISSION_MIS: NEE weights use the guide's pdf
     GuideView gv;
 };
 void launch_clear_queues(const Queues& qs, uint32_t depths, hipStream_t s);
@@ -207,6 +208,9 @@ void launch_generate(const PathLaunch& pl, uint32_t npx, uint32_t nf, hipStream_
 // bounce 0 reads path ids 0 .. n0-1 directly
 void launch_extend(const PathLaunch& pl, uint32_t bounce, uint32_t n0, const uint32_t* queue, hipStream_t s);
 void launch_shade(const PathLaunch& pl, uint32_t npx, uint32_t bounce, uint32_t* next_queue, hipStream_t s);
+// Active queue of bounce `next` in path order from the path tags (ps.tag
+// set), after the shading of bounce next - 1; n = path slots in use.
+void launch_order_active(const PathLaunch& pl, uint32_t next, uint32_t n, uint32_t* queue, hipStream_t s);
 void launch_shadow(const PathLaunch& pl, uint32_t bounce, hipStream_t s);
 // The camera paths the wavefront left to the tail launch (Queues::tail),
 // each finished by one lane.

@@ -59,6 +59,7 @@ __global__ __launch_bounds__(256) void k_generate(PathState ps, FrameArgs fa, ui
     ps.wl[p] = r.wl;
     ps.rng[p] = r.rng;
     ps.flags[p] = r.flags;
+    if (ps.tag) ps.tag[p] = 0;  // tags of an earlier batch are stale
 }
 
 // Bounce 0 reads path ids 0 .. n0-1 directly (every path starts); later
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend
     }
     const uint32_t stride = gridDim.x * blockDim.x;
     const bool count = count_traversal != 0u;
-    uint32_t my_rays = 0, my_nodes = 0, my_tris = 0;
+    uint32_t my_rays = 0, my_nodes = 0, my_tris = 0, my_max = 0;
     // wave-uniform trip count, so the binning ballots see the whole wave.
     // Bounce 0 takes path ids in pixel order; a plain grid stride would give
     // a wave the same pixel chunk of every frame in flight, so wave w takes
@@ -107,6 +108,7 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend
             Hit h = trace_ray<false>(sv, xyz(o), xyz(d), 0.0001f, 100.0f, st, count, ray_nodes, ray_tris);
             my_nodes += ray_nodes;
             my_tris += ray_tris;
+            my_max = max(my_max, ray_nodes);
             my_rays++;
             if (h.tri == kInvalid) {
                 have = false;  // miss: the path ends (shader.wgsl:566)
@@ -137,7 +139,7 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend
             pending &= ~__ballot(mine);
         }
     }
-    if (count) flush_traversal_counts(ctr, kRkExtend, my_rays, my_nodes, my_tris);
+    if (count) flush_traversal_counts(ctr, kRkExtend, my_rays, my_nodes, my_tris, my_max);
 }
 
 // The same extend with lanes refilled as their rays end. A wave owns the
@@ -204,7 +206,7 @@ __device__ __forceinline__ void extend_refill_loop(const SceneView& sv, const Pa
     const uint32_t nw = stride >> 6, wave = gtid >> 6, lane = __lane_id();
     const uint32_t seq_len = (n + stride - 1) / stride * 64u;  // the wave's rounds x 64 positions
     uint32_t cursor = 0, ndone = 0, iters = 0;                 // wave-uniform
-    uint32_t my_rays = 0, my_nodes = 0, my_tris = 0;
+    uint32_t my_rays = 0, my_nodes = 0, my_tris = 0, my_max = 0;
     bool active = false;
     uint32_t p = 0;
     TravState ts;
@@ -238,6 +240,7 @@ __device__ __forceinline__ void extend_refill_loop(const SceneView& sv, const Pa
             active = false;
             my_nodes += ts.nvis;
             my_tris += ts.ntest;
+            if (kCount) my_max = max(my_max, ts.nvis);
             const Hit& h = ts.best;
             if (h.tri == kInvalid) {  // miss: the path ends (shader.wgsl:566)
                 if (sv.env) ps.L[p] = add4(ps.L[p], mul4(ps.tp[p], env_radiance(sv, ts.d, ps.wl[p])));
@@ -288,7 +291,7 @@ __device__ __forceinline__ void extend_refill_loop(const SceneView& sv, const Pa
     if (kCount) {
         // lane slots the wave spent in the traversal loop
         if (lane == 0 && iters) atomicAdd(&ctr->extend_wave_nodes, (unsigned long long)iters * 64ull);
-        flush_traversal_counts(ctr, kRkExtend, my_rays, my_nodes, my_tris);
+        flush_traversal_counts(ctr, kRkExtend, my_rays, my_nodes, my_tris, my_max);
     }
 }
 
@@ -357,7 +360,6 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
                               __uint_as_float(hr.z), __uint_as_float(hr.w), r);
             ps.L[p] = r.L;
             if (vo.deferred) {
-                ps.gn[p] = f4(vo.ff, 0.0f);
                 ps.galb[p] = vo.galb;
                 ps.o[p] = r.o4;
                 ps.rng[p] = r.rng;
@@ -382,17 +384,66 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
             r[1] = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(p));
             r[2] = contrib;
         }
-        const uint32_t ne = seg_append(qs, bounce + 1, kQActive, qs.seg_cap, alive);
-        if (alive) next_queue[ne] = p;
+        if (ps.tag) {
+            if (alive) ps.tag[p] = (uint8_t)(bounce + 1);
+        } else {
+            const uint32_t ne = seg_append(qs, bounce + 1, kQActive, qs.seg_cap, alive);
+            if (alive) next_queue[ne] = p;
+        }
         const uint32_t ge = seg_append(qs, bounce, kQGuide, qs.seg_cap, deferred);
-        if (deferred) ps.guide_q[ge] = p;
+        if (deferred) {
+            ps.guide_q[ge] = p;
+            // normal and the vertex's shadow record (none: all ones)
+            ps.gn[p] = f4(vo.ff, __uint_as_float(shadow ? se : 0xFFFFFFFFu));
+        }
     }
 }
 
+// Active queue of bounce `next` (>= 1) in path order, from the tags the
+// previous bounce's shading wrote: path range [s R, (s + 1) R) fills segment
+// s (R = the paths over kSegs, rounded up to whole 256-path tiles, so R <=
+// seg_cap); a wave takes a tile, each lane 4 consecutive paths, and appends
+// the tile's entering paths in order at one wave-aggregated offset.
+__global__ __launch_bounds__(256) void k_order_active(PathState ps, Queues qs, uint32_t next, uint32_t n,
+                                                      uint32_t* __restrict__ queue) {
+    if (tail_started(qs, next - 1u)) return;
+    const uint32_t lane = __lane_id();
+    const uint32_t R = ((n + kSegs - 1) / kSegs + 255u) & ~255u;
+    const uint32_t tiles = (n + 255u) / 256u;
+    const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); t < tiles; t += nwaves) {
+        const uint32_t p0 = t * 256u + lane * 4u;  // tag array padded to whole tiles
+        const uint32_t w = *reinterpret_cast<const uint32_t*>(ps.tag + p0);
+        uint32_t m = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j)
+            if (p0 + j < n && ((w >> (8 * j)) & 0xFFu) == next) m |= 1u << j;
+        const uint32_t c = (uint32_t)__popc(m);
+        uint32_t inc = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t v = (uint32_t)__shfl_up((int)inc, off, 64);
+            if ((int)lane >= off) inc += v;
+        }
+        const uint32_t total = (uint32_t)__shfl((int)inc, 63, 64);
+        if (total == 0u) continue;
+        const uint32_t seg = (t * 256u) / R;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(qcount(qs, next, kQActive, seg), total);
+        base = (uint32_t)__shfl((int)base, 0, 64) + inc - c;
+        uint32_t* out = queue + (size_t)seg * qs.seg_cap + base;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j)
+            if (m & (1u << j)) *out++ = p0 + j;
+    }
+}
+
+// mark_visible: an unoccluded record gets a negative tmax, which the guided
+// scatter reads to reweight the contribution the kernel added.
 __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(6) void k_shadow(SceneView sv, PathState ps, Queues qs,
                                                           uint32_t count_traversal,
                                                           Counters* __restrict__ ctr, uint32_t bounce,
-                                                          uint2* __restrict__ gstack) {
+                                                          uint2* __restrict__ gstack, uint32_t mark_visible) {
     __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
     __shared__ uint32_t pre[kSegs + 1];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -408,18 +459,21 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(6) void k_shadow
     }
     const uint32_t stride = gridDim.x * blockDim.x;
     const bool count = count_traversal != 0u;
-    uint32_t my_rays = 0, my_nodes = 0, my_tris = 0;
+    uint32_t my_rays = 0, my_nodes = 0, my_tris = 0, my_max = 0;
     for (uint32_t i = gtid; i < n; i += stride) {
-        const float4* r = ps.shadow + 3ull * seg_entry(pre, kSegs, i, qs.seg_cap);
+        float4* r = ps.shadow + 3ull * seg_entry(pre, kSegs, i, qs.seg_cap);
         const float4 a = r[0], b = r[1];
+        const uint32_t n_before = my_nodes;
         Hit h = trace_ray<true>(sv, xyz(a), xyz(b), 0.001f, a.w, st, count, my_nodes, my_tris);
+        my_max = max(my_max, my_nodes - n_before);
         my_rays++;
         if (h.tri == kInvalid) {
             const uint32_t p = __float_as_uint(b.w);
             ps.L[p] = add4(ps.L[p], r[2]);
+            if (mark_visible) r[0].w = -1.0f;
         }
     }
-    if (count) flush_traversal_counts(ctr, kRkShadow, my_rays, my_nodes, my_tris);
+    if (count) flush_traversal_counts(ctr, kRkShadow, my_rays, my_nodes, my_tris, my_max);
 }
 
 // The paths entering bounce t-1 = *qs.tail - 1 (if the wavefront handed
@@ -552,9 +606,12 @@ void launch_shade(const PathLaunch& pl, uint32_t npx, uint32_t bounce, uint32_t*
     hipLaunchKernelGGL(k_shade, dim3(kShadeBlocks), dim3(kShadeThreads), 0, s, pl.sv, pl.ps, pl.grid,
                        pl.fa, npx, pl.qs, bounce, pl.guided, next_queue);
 }
+void launch_order_active(const PathLaunch& pl, uint32_t next, uint32_t n, uint32_t* queue, hipStream_t s) {
+    hipLaunchKernelGGL(k_order_active, dim3(2048), dim3(256), 0, s, pl.ps, pl.qs, next, n, queue);
+}
 void launch_shadow(const PathLaunch& pl, uint32_t bounce, hipStream_t s) {
     hipLaunchKernelGGL(k_shadow, dim3(trace_grid(6, pl.qs.trace_rounds)), dim3(kTraceThreads), 0, s, pl.sv, pl.ps, pl.qs,
-                       pl.count_traversal, pl.ctr, bounce, pl.gstack);
+                       pl.count_traversal, pl.ctr, bounce, pl.gstack, pl.guide_mis);
 }
 void launch_tail(const PathLaunch& pl, uint32_t npx, uint32_t max_depth, hipStream_t s) {
     hipLaunchKernelGGL(k_tail, dim3(trace_grid(4, pl.qs.trace_rounds)), dim3(kTraceThreads), 0, s, pl.sv, pl.ps, pl.grid, pl.fa, npx,

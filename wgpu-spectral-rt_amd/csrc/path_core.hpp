@@ -137,10 +137,12 @@ __device__ __forceinline__ VertexOut shade_vertex(const SceneView& sv, const Gri
             out.stmax = ls.dist - 0.001f;
         }
         if (guided) {
+            // the scatter comes later; o.w keeps the light sample's pdf (0:
+            // none, or a point light) for the guide-aware NEE weight
             out.deferred = true;
             out.ff = ff;
             out.galb = rgb_to_spectrum4(mk3(col.x, col.y, col.z), wl);
-            r.o4 = f4(hp, 0.0f);
+            r.o4 = f4(hp, out.shadow && !ls.delta ? ls.pdf : 0.0f);
             return out;
         }
     }

@@ -147,7 +147,7 @@ __device__ __forceinline__ bool trav_step(const SceneView& sv, TravState& s, Sta
     if (kCount) s.nvis++;
     // per axis: slab scale/offset, and the byte plane of each child's near
     // and far bound (qlo or qhi by the sign of the direction)
-    float as[3], ao[3], fs[3], fo[3], sl = 0.0f;
+    float as[3], ao[3], an[3], fs[3], fo[3];
     const float org[3] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)};
     const float oo[3] = {s.o.x, s.o.y, s.o.z};
     // quantized bytes: qlo x/y/z = (w2.xy, w2.zw, w3.xy), qhi x/y/z = (w3.zw, w4.xy, w4.zw)
@@ -160,24 +160,23 @@ __device__ __forceinline__ bool trav_step(const SceneView& sv, TravState& s, Sta
         const int ex = (int)(int8_t)((w0.w >> (8 * a)) & 0xFFu);
         as[a] = ldexpf(s.inv[a], ex);  // 2^ex * inv, exact
         ao[a] = (org[a] - oo[a]) * s.inv[a];
-        sl = fmaxf(sl, fabsf(as[a]) * 255.0f + fabsf(ao[a]));
+        // bound of the f32 error of this axis' slab values, taken off its
+        // near planes and added to its far planes (with the padding folded
+        // into the far planes' constants): each axis keeps its own bound, so
+        // a near-zero direction component (inv ~ 1e20) widens only its own
+        // slab, which is then all or nothing, and not the others
+        const float e = (fabsf(as[a]) * 255.0f + fabsf(ao[a])) * 4.0e-7f;
+        an[a] = ao[a] - e;
+        fs[a] = as[a] * kBoxPad;
+        fo[a] = __fmaf_rn(ao[a], kBoxPad, e * (1.0f + kBoxPad));
         nl[a] = neg ? hi_l[a] : lo_l[a];
         nh[a] = neg ? hi_h[a] : lo_h[a];
         fl[a] = neg ? lo_l[a] : hi_l[a];
         fh[a] = neg ? lo_h[a] : hi_h[a];
     }
-    // bound of the f32 error of the slab values, applied to both ends:
-    // a child is hit when its near distance is at most its padded far
-    // distance, far * kBoxPad + sl2, with the padding folded into the
-    // far planes' constants
-    sl = sl * 4.0e-7f;
-    const float sl2 = sl * (1.0f + kBoxPad);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        fs[a] = as[a] * kBoxPad;
-        fo[a] = __fmaf_rn(ao[a], kBoxPad, sl2);
-    }
-    const float tcap = __fmaf_rn(s.best.t, kBoxPad, sl2);
+    // a child is hit when its near distance (a lower bound) is at most its
+    // padded far distance (an upper bound) and the padded closest hit
+    const float tcap = s.best.t * kBoxPad;
     const uint32_t child_base = w1.x, tri_base = w1.y;
     float tnear[8];
     uint32_t hitmask = 0;
@@ -186,7 +185,7 @@ __device__ __forceinline__ bool trav_step(const SceneView& sv, TravState& s, Sta
         float tn = s.tmin, tf = tcap;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            tn = fmaxf(tn, __fmaf_rn((float)byte_of(nl[a], nh[a], c), as[a], ao[a]));
+            tn = fmaxf(tn, __fmaf_rn((float)byte_of(nl[a], nh[a], c), as[a], an[a]));
             tf = fminf(tf, __fmaf_rn((float)byte_of(fl[a], fh[a], c), fs[a], fo[a]));
         }
         tnear[c] = tn;
@@ -228,9 +227,9 @@ __device__ __forceinline__ bool trav_step(const SceneView& sv, TravState& s, Sta
     // internal children in the order of the ray's octant over the slots:
     // slot c is pushed at rank (c ^ oct) from the top, so slot oct is
     // popped first; positions come from a mask permuted by oct.
-    // A child is culled when its near distance, less the error bound, is
-    // beyond the closest hit so far (padded).
-    const float cull = __fmaf_rn(s.best.t, kBoxPad, sl);
+    // A child is culled when its near distance (a lower bound) is beyond
+    // the closest hit so far (padded).
+    const float cull = s.best.t * kBoxPad;
     uint32_t push = 0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) push |= (!(tnear[c] > cull) ? 1u : 0u) << c;
@@ -255,7 +254,7 @@ __device__ __forceinline__ bool trav_step(const SceneView& sv, TravState& s, Sta
                 continue;
             }
             const int pos = s.sp + __popc(rest >> ((c ^ oct) + 1));
-            st.put(pos, make_uint2(child_base + (meta & 0x7Fu), __float_as_uint(tnear[c] - sl)));
+            st.put(pos, make_uint2(child_base + (meta & 0x7Fu), __float_as_uint(tnear[c])));
         }
         s.sp += __popc(rest);
         s.have_next = true;
@@ -289,36 +288,31 @@ __device__ __forceinline__ bool meets_root_child(const SceneView& sv, f3 o, f3 d
     const float org[3] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)};
     const uint32_t lo_l[3] = {w2.x, w2.z, w3.x}, lo_h[3] = {w2.y, w2.w, w3.y};
     const uint32_t hi_l[3] = {w3.z, w4.x, w4.z}, hi_h[3] = {w3.w, w4.y, w4.w};
-    float as[3], ao[3], sl = 0.0f;
+    float as[3], an[3], fs[3], fo[3];
     uint32_t nl[3], nh[3], fl[3], fh[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const bool neg = inv[a] < 0.0f;
         const int ex = (int)(int8_t)((w0.w >> (8 * a)) & 0xFFu);
         as[a] = ldexpf(inv[a], ex);
-        ao[a] = (org[a] - oo[a]) * inv[a];
-        sl = fmaxf(sl, fabsf(as[a]) * 255.0f + fabsf(ao[a]));
+        const float ao = (org[a] - oo[a]) * inv[a];
+        const float e = (fabsf(as[a]) * 255.0f + fabsf(ao)) * 4.0e-7f;  // as in trav_step
+        an[a] = ao - e;
+        fs[a] = as[a] * kBoxPad;
+        fo[a] = __fmaf_rn(ao, kBoxPad, e * (1.0f + kBoxPad));
         nl[a] = neg ? hi_l[a] : lo_l[a];
         nh[a] = neg ? hi_h[a] : lo_h[a];
         fl[a] = neg ? lo_l[a] : hi_l[a];
         fh[a] = neg ? lo_h[a] : hi_h[a];
     }
-    sl = sl * 4.0e-7f;
-    const float sl2 = sl * (1.0f + kBoxPad);
-    float fs[3], fo[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        fs[a] = as[a] * kBoxPad;
-        fo[a] = __fmaf_rn(ao[a], kBoxPad, sl2);
-    }
-    const float tcap = __fmaf_rn(tmax, kBoxPad, sl2);
+    const float tcap = tmax * kBoxPad;
     bool any = false;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         float tn = tmin, tf = tcap;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            tn = fmaxf(tn, __fmaf_rn((float)byte_of(nl[a], nh[a], c), as[a], ao[a]));
+            tn = fmaxf(tn, __fmaf_rn((float)byte_of(nl[a], nh[a], c), as[a], an[a]));
             tf = fminf(tf, __fmaf_rn((float)byte_of(fl[a], fh[a], c), fs[a], fo[a]));
         }
         any |= byte_of(w1.z, w1.w, c) != 0u && tn <= tf;
@@ -351,18 +345,22 @@ __device__ __forceinline__ uint32_t wave_lockstep_nodes(uint32_t nodes, bool tra
 }
 
 // Adds per-lane traversal totals of a wave to the frame counters.
+// max_nodes: the most node visits of one of the lane's rays (0: not kept).
 __device__ __forceinline__ void flush_traversal_counts(Counters* ctr, int kind, uint32_t rays,
-                                                       uint32_t nodes, uint32_t tris) {
+                                                       uint32_t nodes, uint32_t tris, uint32_t max_nodes = 0) {
     unsigned long long r = rays, n = nodes, t = tris;
+    uint32_t m = max_nodes;
     for (int off = 32; off > 0; off >>= 1) {
         r += __shfl_xor(r, off, 64);
         n += __shfl_xor(n, off, 64);
         t += __shfl_xor(t, off, 64);
+        m = max(m, (uint32_t)__shfl_xor(m, off, 64));
     }
     if (__lane_id() == 0 && r) {
         atomicAdd(&ctr->trav_rays[kind], r);
         atomicAdd(&ctr->trav_nodes[kind], n);
         atomicAdd(&ctr->trav_tris[kind], t);
+        if (m) atomicMax(&ctr->ray_max_nodes[kind], (unsigned long long)m);
     }
 }
 

@@ -44,7 +44,13 @@ class Config(C.Structure):
                 ("guide_train_samples", C.c_uint32), ("env_map", C.c_uint32),
                 ("frame_graphs", C.c_uint32), ("frames_in_flight", C.c_uint32),
                 ("tail_paths", C.c_uint32), ("quirks", C.c_uint32), ("trace_rounds", C.c_uint32), ("guide_train_frames", C.c_uint32),
-                ("extend_refill", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("extend_refill", C.c_uint32), ("ordered_queues", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
+
+
+# wsrt_config.quirks bits (WSRT_FIX_*): each replaces one behaviour of the
+# reference's shader by the textbook estimator
+FIX_EMISSION_MIS, FIX_RR_CLAMP, FIX_ONB_SIGN = 1, 2, 4
 
 
 class AutotuneResult(C.Structure):
@@ -79,7 +85,8 @@ class FrameStats(C.Structure):
                                            "guide_samples")] + \
         [("trav_nodes", C.c_uint64 * 3), ("trav_tris", C.c_uint64 * 3), ("trav_rays", C.c_uint64 * 3),
          ("kernel_rays", C.c_uint64 * 6), ("kernel_nodes", C.c_uint64 * 6), ("kernel_tris", C.c_uint64 * 6),
-         ("kernel_counted_rays", C.c_uint64 * 6), ("extend_wave_nodes", C.c_uint64)]
+         ("kernel_counted_rays", C.c_uint64 * 6), ("extend_wave_nodes", C.c_uint64),
+         ("ray_max_nodes", C.c_uint64 * 6)]
 
 
 class KernelStat(C.Structure):
@@ -586,7 +593,7 @@ class Renderer:
         _check(load().wsrt_renderer_get_stats(self.h, C.byref(s)))
         out = {k: getattr(s, k) for k, _ in FrameStats._fields_[:8]}
         for k in ("trav_nodes", "trav_tris", "trav_rays", "kernel_rays", "kernel_nodes", "kernel_tris",
-                  "kernel_counted_rays"):
+                  "kernel_counted_rays", "ray_max_nodes"):
             out[k] = list(getattr(s, k))
         out["extend_wave_nodes"] = s.extend_wave_nodes
         return out
