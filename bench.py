@@ -312,53 +312,74 @@ def time_to_relmse(frames: int = 64, warmup: int = 256, boxes: int | None = None
         return r
 
     def warm(r, name, S):
-        """Untimed steady state for the plain tracer; timed training for the guided one."""
+        """Untimed steady state for the plain tracer; timed training for the
+        guided one, then one untimed batch that captures the graph of its
+        frames with the network fixed (a one-time cost like the plain
+        tracer's capture)."""
         _, t = frames_of(r, warmup if name == "guided" else S)
+        if name == "guided":
+            frames_of(r, S)
         return t if name == "guided" else 0.0
 
     out = {"scene": f"occluded-light room, {scene.info.n_triangles} triangles, {width}x{width}, depth 8",
            "relmse": "mean (I - R)^2 / (R^2 + 1e-2)", "target": 0.01}
     S = frames_in_flight
-    rp = renderer(scenes.occluded_config(frames_in_flight=128), 1 << 22)
-    R, _ = frames_of(rp, ref_frames)
-    rp.close()
     out["reference_spp"] = ref_frames
     out["frames_in_flight_for_variance"] = S
     out["measured_spp"] = frames
-    res = {}
-    def guided(**kw):
-        return scenes.guided_config(guide_train_frames=warmup, **kw)
 
-    for name, make in (("plain", scenes.occluded_config), ("guided", guided)):
-        cfg = make(frames_in_flight=S)
-        r = renderer(cfg, 0)
-        tw = {S: warm(r, name, S)}
-        a, ta = frames_of(r, frames)
-        b, tb = frames_of(r, frames)
-        r.close()
-        if name == "guided":  # the network's rates with training on, one batch after a warm one
-            rm = renderer(scenes.guided_config(frames_in_flight=S), 0)
-            rm.render_frames(S)
-            out["mlp"] = mlp_rates(rm, S, cfg.guide_train_samples)
-            rm.close()
-        c = frames * float(np.mean((a - b) ** 2 / (R**2 + 1e-2))) / 2
-        rate = {S: (ta + tb) / (2 * frames)}
-        for S2 in rate_frames_in_flight:
-            if S2 == S:
-                continue
-            r = renderer(make(frames_in_flight=S2), 0)
-            tw[S2] = warm(r, name, S2)
-            _, t2 = frames_of(r, S2)
-            rate[S2] = t2 / S2
+    def pair(quirks):
+        """Plain and guided tracer with the given emission weighting."""
+        rp = renderer(scenes.occluded_config(frames_in_flight=128, quirks=quirks), 1 << 22)
+        R, _ = frames_of(rp, ref_frames)
+        rp.close()
+        res = {}
+
+        def plain(**kw):
+            return scenes.occluded_config(quirks=quirks, **kw)
+
+        def guided(**kw):
+            return scenes.guided_config(guide_train_frames=warmup, quirks=quirks, **kw)
+
+        for name, make in (("plain", plain), ("guided", guided)):
+            cfg = make(frames_in_flight=S)
+            r = renderer(cfg, 0)
+            tw = {S: warm(r, name, S)}
+            a, ta = frames_of(r, frames)
+            b, tb = frames_of(r, frames)
             r.close()
-        best = min(rate, key=rate.get)
-        res[name] = {"C": round(c, 3), "ms_per_spp": round(1000 * rate[best], 3), "frames_in_flight": best,
-                     "ms_per_spp_by_frames_in_flight": {str(k): round(1000 * x, 3) for k, x in sorted(rate.items())},
-                     "relmse_at_measured_spp": round(c / frames, 4),
-                     "time_to_relmse_s": round(tw[best] + c / 0.01 * rate[best], 2),
-                     "training_frames_before_measuring": warmup if name == "guided" else 0}
-    out.update(res)
-    out["speedup_guided_vs_plain"] = round(res["plain"]["time_to_relmse_s"] / res["guided"]["time_to_relmse_s"], 3)
+            if name == "guided" and quirks == 0:  # the network's rates with training on, one batch after a warm one
+                rm = renderer(scenes.guided_config(frames_in_flight=S), 0)
+                rm.render_frames(S)
+                out["mlp"] = mlp_rates(rm, S, cfg.guide_train_samples)
+                rm.close()
+            c = frames * float(np.mean((a - b) ** 2 / (R**2 + 1e-2))) / 2
+            rate = {S: (ta + tb) / (2 * frames)}
+            for S2 in rate_frames_in_flight:
+                if S2 == S:
+                    continue
+                r = renderer(make(frames_in_flight=S2), 0)
+                tw[S2] = warm(r, name, S2)
+                _, t2 = frames_of(r, S2)
+                rate[S2] = t2 / S2
+                r.close()
+            best = min(rate, key=rate.get)
+            res[name] = {"C": round(c, 3), "ms_per_spp": round(1000 * rate[best], 3), "frames_in_flight": best,
+                         "ms_per_spp_by_frames_in_flight": {str(k): round(1000 * x, 3)
+                                                            for k, x in sorted(rate.items())},
+                         "relmse_at_measured_spp": round(c / frames, 4),
+                         "time_to_relmse_s": round(tw[best] + c / 0.01 * rate[best], 2),
+                         "training_frames_before_measuring": warmup if name == "guided" else 0}
+        res["speedup_guided_vs_plain"] = round(res["plain"]["time_to_relmse_s"] / res["guided"]["time_to_relmse_s"],
+                                               3)
+        return res
+
+    # the reference's estimator (emission reached by a Lambert bounce counted
+    # whole beside next-event estimation), then both tracers with the
+    # balance heuristic (WSRT_FIX_EMISSION_MIS; the guided tracer weighs
+    # light samples against its mixture pdf)
+    out.update(pair(0))
+    out["emission_mis"] = pair(wsrt.FIX_EMISSION_MIS)
     return out
 
 

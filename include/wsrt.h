@@ -476,11 +476,13 @@ typedef struct wsrt_autotune_result {
     float stage2_ms_per_frame[8]; /* 0: not run */
     uint32_t best_tail_paths, best_trace_rounds;
     /* third stage, at the second stage's choice: the closest-hit kernel's
-     * lane refill threshold (extend_refill 0, 16 and 48) */
+     * lane refill threshold (extend_refill 0, 16 and 48) with and without
+     * path-ordered queues (ordered_queues) */
     uint32_t n_stage3;
-    uint32_t extend_refill[4];
-    float stage3_ms_per_frame[4]; /* 0: not run */
-    uint32_t best_extend_refill;
+    uint32_t extend_refill[8];
+    uint32_t ordered_queues[8];
+    float stage3_ms_per_frame[8]; /* 0: not run */
+    uint32_t best_extend_refill, best_ordered_queues;
 } wsrt_autotune_result;
 int wsrt_autotune(wsrt_device* dev, const wsrt_scene* scene, uint32_t width, uint32_t height,
                   const wsrt_camera_uniform* cam, wsrt_config* cfg, uint64_t mem_budget,

@@ -366,9 +366,11 @@ def test_autotune_picks_a_measured_candidate(device, cornell_pair):
     assert (cfg.tail_paths, cfg.trace_rounds) == (res["best_tail_paths"], res["best_trace_rounds"])
     assert cfg.trace_rounds in (1, 2) and cfg.tail_paths in (0, 1 << 16, 1 << 18, 1 << 20)
     # third stage: lane refill threshold
-    assert [c["extend_refill"] for c in res["stage3"]] == [16, 48]
+    assert [(c["extend_refill"], c["ordered_queues"]) for c in res["stage3"]] == \
+        [(16, 0), (48, 0), (0, 1), (16, 1), (48, 1)]
     assert all(c["ms_per_frame"] > 0 for c in res["stage3"])
     assert cfg.extend_refill == res["best_extend_refill"] in (0, 16, 48)
+    assert cfg.ordered_queues == res["best_ordered_queues"] in (0, 1)
 
 
 def test_trace_rounds_do_not_change_results(device, cornell_pair):

@@ -2248,25 +2248,34 @@ extern "C" int wsrt_autotune(wsrt_device* dev, const wsrt_scene* scene, uint32_t
     out->best_trace_rounds = br;
     // third stage: lane refill in the closest-hit kernel, which pays where
     // ray costs within a wave differ widely (occluded interiors)
+    // and path-ordered queues, which shorten shading where paths scatter
+    // widely
     const uint32_t refills[3] = {0u, 16u, 48u};
     float best3 = best2;
-    uint32_t bx = cfg->extend_refill;
-    for (uint32_t k = 0; k < 3; ++k) {
-        if (refills[k] == cfg->extend_refill) continue;  // measured above
-        wsrt_config c = *cfg;
-        c.extend_refill = refills[k];
-        const uint32_t i = out->n_stage3++;
-        out->extend_refill[i] = refills[k];
-        uint64_t bytes = 0;
-        const float ms = autotune_time(dev, scene, width, height, cam, c, mem_budget, &bytes);
-        out->stage3_ms_per_frame[i] = ms;
-        if (ms > 0.0f && ms < best3 * 0.98f) {
-            best3 = ms;
-            bx = refills[k];
+    uint32_t bx = cfg->extend_refill, bo = cfg->ordered_queues;
+    for (uint32_t o = 0; o < 2; ++o) {
+        for (uint32_t k = 0; k < 3; ++k) {
+            if (refills[k] == cfg->extend_refill && o == cfg->ordered_queues) continue;  // measured above
+            wsrt_config c = *cfg;
+            c.extend_refill = refills[k];
+            c.ordered_queues = o;
+            const uint32_t i = out->n_stage3++;
+            out->extend_refill[i] = refills[k];
+            out->ordered_queues[i] = o;
+            uint64_t bytes = 0;
+            const float ms = autotune_time(dev, scene, width, height, cam, c, mem_budget, &bytes);
+            out->stage3_ms_per_frame[i] = ms;
+            if (ms > 0.0f && ms < best3 * 0.98f) {
+                best3 = ms;
+                bx = refills[k];
+                bo = o;
+            }
         }
     }
     cfg->extend_refill = bx;
+    cfg->ordered_queues = bo;
     out->best_extend_refill = bx;
+    out->best_ordered_queues = bo;
     return WSRT_OK;
 }
 

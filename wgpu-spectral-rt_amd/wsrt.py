@@ -58,8 +58,9 @@ class AutotuneResult(C.Structure):
                 ("ms_per_frame", C.c_float * 12), ("device_bytes", C.c_uint64 * 12), ("best", C.c_uint32),
                 ("n_stage2", C.c_uint32), ("tail_paths", C.c_uint32 * 8), ("trace_rounds", C.c_uint32 * 8),
                 ("stage2_ms_per_frame", C.c_float * 8), ("best_tail_paths", C.c_uint32),
-                ("best_trace_rounds", C.c_uint32), ("n_stage3", C.c_uint32), ("extend_refill", C.c_uint32 * 4),
-                ("stage3_ms_per_frame", C.c_float * 4), ("best_extend_refill", C.c_uint32)]
+                ("best_trace_rounds", C.c_uint32), ("n_stage3", C.c_uint32), ("extend_refill", C.c_uint32 * 8),
+                ("ordered_queues", C.c_uint32 * 8), ("stage3_ms_per_frame", C.c_float * 8),
+                ("best_extend_refill", C.c_uint32), ("best_ordered_queues", C.c_uint32)]
 
 
 class BatchJob(C.Structure):
@@ -710,7 +711,7 @@ def load_png(path: str) -> np.ndarray:
 def autotune(device: "Device", scene: "Scene", width: int, height: int, config: Config,
              camera: CameraUniform | None = None, mem_budget: int = 0) -> dict:
     """Picks config.frames_in_flight, then tail_paths and trace_rounds, then
-    extend_refill, by timing candidates (wsrt_autotune) for the given camera; updates config in
+    extend_refill and ordered_queues, by timing candidates (wsrt_autotune) for the given camera; updates config in
     place and returns the timings."""
     res = AutotuneResult()
     cam = C.cast(C.byref(camera), vp) if camera is not None else None
@@ -724,9 +725,9 @@ def autotune(device: "Device", scene: "Scene", width: int, height: int, config: 
             "best_tail_paths": res.best_tail_paths, "best_trace_rounds": res.best_trace_rounds,
             "stage2": [{"tail_paths": res.tail_paths[i], "trace_rounds": res.trace_rounds[i],
                         "ms_per_frame": round(res.stage2_ms_per_frame[i], 4)} for i in range(res.n_stage2)],
-            "best_extend_refill": res.best_extend_refill,
-            "stage3": [{"extend_refill": res.extend_refill[i], "ms_per_frame": round(res.stage3_ms_per_frame[i], 4)}
-                       for i in range(res.n_stage3)]}
+            "best_extend_refill": res.best_extend_refill, "best_ordered_queues": res.best_ordered_queues,
+            "stage3": [{"extend_refill": res.extend_refill[i], "ordered_queues": res.ordered_queues[i],
+                        "ms_per_frame": round(res.stage3_ms_per_frame[i], 4)} for i in range(res.n_stage3)]}
 
 
 class Batch:
