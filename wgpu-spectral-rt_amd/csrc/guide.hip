@@ -24,7 +24,10 @@ typedef short bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kStage = 41;            // floats per sample row of the output staging
-constexpr int kScatterWaves = 8;      // waves per block of the scattering kernel
+constexpr int kScatterWaves = 8;      // waves per block of the standalone inference kernel
+// waves per block of the scattering kernel: 3 per SIMD, which its register
+// budget (<= 168 VGPRs) and LDS (weights + FP16 output staging) allow
+constexpr int kScatterBlockWaves = 12;
 constexpr int kTrainWaves = 4;        // waves per block of the training kernel
 // 16-sample tiles per wave in the training forward and backward passes. A
 // training step covers 16384 samples; one tile per wave gives it 1024 waves,
@@ -34,13 +37,10 @@ constexpr int kBwdTiles = 1;
 // path flags 4 << b: the Lambert vertex of bounce b is recorded for training
 constexpr uint32_t kFlagRecorded = 4u;
 
-__device__ __forceinline__ short bf16_bits(float f) {  // round to nearest even
-    uint32_t u = __float_as_uint(f);
-    // a NaN stays a (quiet) NaN: rounding its payload could carry into the
-    // exponent and sign
-    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (short)((u >> 16) | 0x40u);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (short)(u >> 16);
+// Round to nearest even: gfx950's v_cvt_pk_bf16_f32 (a NaN stays a quiet
+// NaN), paired into one instruction per two values where the compiler can.
+__device__ __forceinline__ short bf16_bits(float f) {
+    return __builtin_bit_cast(short, (__bf16)f);
 }
 __device__ __forceinline__ float bf16_float(short h) {
     return __uint_as_float((uint32_t)(uint16_t)h << 16);
@@ -75,16 +75,18 @@ constexpr uint32_t kD1 = 448, kD2 = 512, kD3 = 576, kD4 = 640, kD5 = 704, kActRo
 // lane's own sample (sample index = lane). Leaves the 40 outputs of sample
 // k at stage[k * kStage + o]. With kTrain, also stores the layer inputs to
 // act (columns col0 .. col0 + 16 NT - 1) and returns the ReLU masks.
-template <bool kTrain, int NT>
+// T0: the first 16-sample tile (samples 16 T0 .. 16 (T0 + NT) - 1 of the
+// wave's lanes); StageT: the staging type of the outputs.
+template <bool kTrain, int NT, int T0 = 0, typename StageT = float>
 __device__ void mlp_forward(const bf16x8* __restrict__ Wf, const float* __restrict__ bias,
-                            const float (&s_own)[kScalars], float* __restrict__ stage,
+                            const float (&s_own)[kScalars], StageT* __restrict__ stage,
                             uint32_t (&mask)[4][2], const Act& act, uint32_t col0) {
     const int l = __lane_id(), g = l >> 4, n = l & 15;
     float x[NT][kScalars];
 #pragma unroll
     for (int t = 0; t < NT; ++t)
 #pragma unroll
-        for (int a = 0; a < kScalars; ++a) x[t][a] = __shfl(s_own[a], 16 * t + n, 64);
+        for (int a = 0; a < kScalars; ++a) x[t][a] = __shfl(s_own[a], 16 * (T0 + t) + n, 64);
 
     f32x4 C[4][NT];
 #pragma unroll
@@ -174,7 +176,7 @@ __device__ void mlp_forward(const bf16x8* __restrict__ Wf, const float* __restri
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = 16 * mb + 4 * g + r;
-                if (row < kOut) stage[(16 * t + n) * kStage + row] = C[mb][t][r] + b5[row];
+                if (row < kOut) stage[(16 * (T0 + t) + n) * kStage + row] = (StageT)(C[mb][t][r] + b5[row]);
             }
 }
 
@@ -250,13 +252,15 @@ __device__ __forceinline__ void load_biases(float* dst, const float* params) {
 // guide (probability mix) and the reference's cosine sampling, throughput
 // update f cos / p_mix, Russian roulette (shader.wgsl:694-700) and the
 // training record of the path's first Lambert vertex.
-__global__ __launch_bounds__(64 * kScatterWaves) void k_guide_scatter(PathState ps, Queues qs, GuideView gv,
+__global__ __launch_bounds__(64 * kScatterBlockWaves) void k_guide_scatter(PathState ps, Queues qs, GuideView gv,
                                                                       FrameArgs fa, uint32_t bounce,
                                                                       uint32_t* __restrict__ next_queue,
                                                                       Counters* __restrict__ ctr) {
     __shared__ bf16x8 Wf[kFwdPack / 8];
     __shared__ float bias[4 * kHid + kOutPad];
-    __shared__ float stage_all[kScatterWaves][64 * kStage];
+    // outputs staged as FP16: sampling and the pdf both read these values,
+    // so the mixture a vertex samples is the one its weight uses
+    __shared__ _Float16 stage_all[kScatterBlockWaves][64 * kStage];
     __shared__ uint32_t pre[kSegs + 1];
     copy_to_lds(reinterpret_cast<uint4*>(Wf), reinterpret_cast<const uint4*>(gv.wf), kFwdPack / 8);
     load_biases(bias, gv.params);
@@ -266,9 +270,9 @@ __global__ __launch_bounds__(64 * kScatterWaves) void k_guide_scatter(PathState 
     const uint32_t rr_after = fa.table[base_frame & fa.mask].rr_after;
     const bool mis = (fa.table[base_frame & fa.mask].quirks & WSRT_FIX_EMISSION_MIS) != 0u;
     const int wave = threadIdx.x >> 6, l = __lane_id();
-    float* stage = stage_all[wave];
-    const uint32_t wstride = gridDim.x * kScatterWaves * 64;
-    for (uint32_t base = (blockIdx.x * kScatterWaves + wave) * 64; base < n; base += wstride) {
+    _Float16* stage = stage_all[wave];
+    const uint32_t wstride = gridDim.x * kScatterBlockWaves * 64;
+    for (uint32_t base = (blockIdx.x * kScatterBlockWaves + wave) * 64; base < n; base += wstride) {
         const uint32_t i = base + l;
         const bool have = i < n;
         uint32_t p = 0, se = 0xFFFFFFFFu;
@@ -288,11 +292,15 @@ __global__ __launch_bounds__(64 * kScatterWaves) void k_guide_scatter(PathState 
         scalars_of(gv, hp, ff, wl.x, s);
         uint32_t mask[4][2];
         const Act none{nullptr, 0};
-        mlp_forward<false, 4>(Wf, bias, s, stage, mask, none, 0);
+        // two passes of 32 samples: half the accumulator registers of one
+        // 64-sample pass
+        mlp_forward<false, 2, 0>(Wf, bias, s, stage, mask, none, 0);
+        __builtin_amdgcn_sched_barrier(0);  // the passes do not interleave
+        mlp_forward<false, 2, 2>(Wf, bias, s, stage, mask, none, 0);
         wave_sync();
         float o[kOut];
 #pragma unroll
-        for (int k = 0; k < kOut; ++k) o[k] = stage[l * kStage + k];
+        for (int k = 0; k < kOut; ++k) o[k] = (float)stage[l * kStage + k];
         wave_sync();
         bool alive = false;
         if (have) {
@@ -730,7 +738,7 @@ size_t guide_act_words(uint32_t ns) { return (size_t)kActRows * ns / 2; }  // in
 
 void launch_guide_scatter(const PathState& ps, const Queues& qs, const GuideView& gv, const FrameArgs& fa,
                           uint32_t bounce, uint32_t* next_queue, Counters* ctr, hipStream_t s) {
-    hipLaunchKernelGGL(k_guide_scatter, dim3(256), dim3(64 * kScatterWaves), 0, s, ps, qs, gv, fa, bounce,
+    hipLaunchKernelGGL(k_guide_scatter, dim3(256), dim3(64 * kScatterBlockWaves), 0, s, ps, qs, gv, fa, bounce,
                        next_queue, ctr);
 }
 
