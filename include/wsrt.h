@@ -44,7 +44,9 @@ enum {
     WSRT_E_OOM = -2,         /* device or host allocation failed            */
     WSRT_E_HIP = -3,         /* HIP runtime error (message has the detail)  */
     WSRT_E_UNSUPPORTED = -4, /* feature or device not supported             */
-    WSRT_E_STATE = -5        /* call not valid in the object's current state */
+    WSRT_E_STATE = -5,       /* call not valid in the object's current state */
+    WSRT_E_ABORTED = -6      /* a render stopped before its work was done
+                              * (canceled, or past its time limit)         */
 };
 
 /* ---- binary contract shared with the reference (byte-identical) ------- */
@@ -225,6 +227,14 @@ int wsrt_device_create(int ordinal, wsrt_device** out);
 int wsrt_device_destroy(wsrt_device* dev);
 /* Number of HIP devices visible (0 when none). Never fails. */
 int wsrt_device_count(void);
+/* Device memory held by the library's objects (scenes, renderers, batches,
+ * guides) on this device, and a cap on it (no reference counterpart: the
+ * reference leaves allocation to wgpu). Past the cap an allocation fails as
+ * if the device were full: the creating call returns WSRT_E_OOM and frees
+ * what it had allocated. 0 removes the cap. Temporary buffers of a BVH build
+ * are not counted. */
+int wsrt_device_set_memory_limit(wsrt_device* dev, uint64_t bytes);
+int wsrt_device_get_memory_used(const wsrt_device* dev, uint64_t* out);
 
 /* ---- geometry generators (src/scene/geometry.rs) ------------------------
  * Two-call pattern: pass NULL buffers to get the sizes, then buffers of at
@@ -518,7 +528,15 @@ int wsrt_batch_autotune(wsrt_batch* b, wsrt_batch_tuning* best, float* best_ms);
 /* All jobs, all samples: one persistent launch (captured as a graph after
  * the first call), asynchronous. */
 int wsrt_batch_render(wsrt_batch* b);
+/* Waits for the render; WSRT_E_ABORTED when it stopped early (its images are
+ * then partial; the next render starts from empty images). */
 int wsrt_batch_sync(wsrt_batch* b);
+/* Stops the render in flight (callable from any thread): every wave drops
+ * the next work item it takes and exits. The next render clears the flag. */
+int wsrt_batch_cancel(wsrt_batch* b);
+/* A wall-clock budget for each render from its start, in microseconds
+ * (0: none): past it, waves stop taking work items. */
+int wsrt_batch_set_time_limit(wsrt_batch* b, uint64_t microseconds);
 int wsrt_batch_read_accum(wsrt_batch* b, uint32_t job, float* out);
 int wsrt_batch_read_rgba8(wsrt_batch* b, uint32_t job, uint8_t* out);
 

@@ -708,7 +708,10 @@ void or_bvh8_closest(const wsrt_bvh8_node* nodes, uint32_t n_nodes, const wsrt_b
         uint32_t st_node[256];
         float st_t[256];
         int sp = 0;
-        if (n_nodes) { st_node[sp] = 0; st_t[sp] = tmin; sp++; }
+        /* a ray with a NaN in its origin or direction meets no triangle (its
+         * Moller-Trumbore values fail every test) and is not traversed */
+        const float sum = o.x + o.y + o.z + d.x + d.y + d.z;
+        if (n_nodes && sum == sum) { st_node[sp] = 0; st_t[sp] = tmin; sp++; }
         while (sp) {
             --sp;
             uint32_t ni = st_node[sp];

@@ -569,3 +569,91 @@ def test_ordered_queues_do_not_change_results(device, name):
     np.testing.assert_array_equal(a[1], b[1])
     assert a[2:] == b[2:]
     assert np.abs(a[0][:, :3]).sum() > 0
+
+
+def test_device_memory_limit_forces_out_of_memory(device, cornell_pair):
+    """An allocation cap makes renderer creation fail with WSRT_E_OOM, frees
+    what the failing call had allocated, and lifting it lets the same call
+    succeed."""
+    g, _ = cornell_pair
+    used = device.memory_used
+    try:
+        device.set_memory_limit(used + (1 << 20))
+        with pytest.raises(wsrt.WsrtError) as e:
+            wsrt.Renderer(device, g, 256, 256, wsrt.default_config(photons_per_pass=1 << 14))
+        assert e.value.code == -2
+        assert device.memory_used == used
+    finally:
+        device.set_memory_limit(0)
+    r = wsrt.Renderer(device, g, 256, 256, wsrt.default_config(photons_per_pass=1 << 14))
+    assert device.memory_used > used + (1 << 20)
+    r.render()
+    r.close()
+    assert device.memory_used == used
+
+
+def test_batch_time_limit_and_cancel(device, cornell_pair):
+    """A persistent batch launch stops when its wall-clock budget runs out or
+    the host sets its stop flag: sync returns WSRT_E_ABORTED (-6), every wave
+    still exits normally, and the next render, without the limit, gives each
+    job's full image."""
+    g, _ = cornell_pair
+    cam = wsrt.CameraController().build_uniform(1.0, 0)
+    jobs = [(g, cam, 512, 512, 64), (g, cam, 256, 256, 16)]
+    cfg = wsrt.default_config(enable_photons=0, max_depth=8)
+    ref = wsrt.Renderer(device, g, 256, 256, cfg)
+    ref.set_camera(cam)
+    ref.render_frames(16)
+    want = ref.read_accum()
+    ref.close()
+    b = wsrt.Batch(device, jobs, cfg)
+    b.set_time_limit(1)
+    b.render()
+    with pytest.raises(wsrt.WsrtError) as e:
+        b.sync()
+    assert e.value.code == -6 and "time limit" in str(e.value)
+    b.set_time_limit(0)
+    b.render()
+    b.cancel()
+    with pytest.raises(wsrt.WsrtError) as e:
+        b.sync()
+    assert e.value.code == -6 and "canceled" in str(e.value)
+    b.render()
+    b.sync()
+    np.testing.assert_array_equal(b.read_accum(1), want)
+    assert np.all(b.read_accum(0)[:, 3] == 64)
+
+
+def test_nan_normals_render_without_error(device):
+    """A mesh whose vertex normals are NaN: the renderer returns no error,
+    the image stays finite (its paths end at the NaN-normal surface: their
+    next rays are not traversed) and every path equals the oracle's."""
+    def build(b):
+        white = b.add_material([0.8, 0.8, 0.8, 1.0], [0, 0, 0, 0])
+        light = b.add_material([6500.0, 2.0, 0.0, 1.0], [3, 0, 0, 0])
+        v, i = wsrt.create_plane()
+        bad = v.copy()
+        bad["normal"][:, :3] = np.nan
+        good = b.add_mesh(v, i)
+        nan = b.add_mesh(bad, i)
+        b.add_instance(good, white, scenes._trs((0, -1, 0), s=(4, 1, 4)))
+        b.add_instance(nan, white, scenes._trs((0.0, 0.0, -1.0), "x", scenes.HALF_PI, (1, 1, 1)))
+        b.add_light_instance(good, light, scenes._trs((0, 1.5, 0), "x", np.pi, (1, 1, 1)), [0.0, 6500.0, 2.0, 0.0])
+        return b
+    g = build(wsrt.SceneBuilder()).build(device)
+    o = build(O.Scene()).build()
+    cfg = wsrt.default_config(photons_per_pass=1 << 12, max_depth=6)
+    W = H = 48
+    r = wsrt.Renderer(device, g, W, H, cfg)
+    cam = wsrt.CameraController().build_uniform(1.0, 0)
+    r.set_camera(cam)
+    r.render_frames(4)
+    acc = r.read_accum()
+    assert np.all(np.isfinite(acc)) and acc[:, :3].sum() > 0
+    wl, L = r.read_paths()
+    f = O.Frame(o, O.Config.from_buffer_copy(bytes(cfg)), O.CameraUniform.from_buffer_copy(bytes(
+        wsrt.CameraController().build_uniform(1.0, 3))), W, H, photons=r.read_photons()[0])
+    owl, oL, _ = f.trace_pixels(np.arange(W * H, dtype=np.uint32))
+    assert np.all(np.isfinite(L)) and np.all(np.isfinite(oL))
+    tol = 1e-4 * np.maximum(np.abs(L), np.abs(oL)) + 1e-6 * float(np.abs(oL).mean())
+    assert np.mean(np.all(np.abs(L - oL) <= tol, axis=1)) >= 0.99

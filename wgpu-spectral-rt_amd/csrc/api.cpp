@@ -60,36 +60,72 @@ extern "C" int wsrt_abi_version(void) { return WSRT_ABI_VERSION; }
 // ---------------------------------------------------------------------------
 namespace {
 
+// Device memory the library's objects hold, per device ordinal, and the cap
+// wsrt_device_set_memory_limit puts on it (0: none). An allocation past the
+// cap fails as hipMalloc would when the device is full.
+constexpr int kMaxOrdinals = 64;
+std::atomic<uint64_t> g_mem_used[kMaxOrdinals];
+std::atomic<uint64_t> g_mem_limit[kMaxOrdinals];
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    size_t held = 0;  // bytes counted against the device (the allocation's size)
+    int ordinal = -1;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), held(o.held), ordinal(o.ordinal) {
         o.p = nullptr;
-        o.bytes = 0;
+        o.bytes = o.held = 0;
+        o.ordinal = -1;
     }
     DevBuf& operator=(DevBuf&& o) noexcept {
         if (this != &o) {
             release();
             p = o.p;
             bytes = o.bytes;
+            held = o.held;
+            ordinal = o.ordinal;
             o.p = nullptr;
-            o.bytes = 0;
+            o.bytes = o.held = 0;
+            o.ordinal = -1;
         }
         return *this;
     }
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            if (ordinal >= 0) g_mem_used[ordinal] -= held;
+        }
         p = nullptr;
-        bytes = 0;
+        bytes = held = 0;
+        ordinal = -1;
     }
+    // On the current device.
     hipError_t alloc(size_t n) {
         release();
+        int d = -1;
+        if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= kMaxOrdinals) d = -1;
+        const size_t size = n < 256 ? 256 : n;
+        if (d >= 0) {
+            const uint64_t used = g_mem_used[d].fetch_add(size) + size, cap = g_mem_limit[d].load();
+            if (cap && used > cap) {
+                g_mem_used[d] -= size;
+                return hipErrorOutOfMemory;
+            }
+        }
+        const hipError_t e = hipMalloc(&p, size);
+        if (e != hipSuccess) {
+            p = nullptr;
+            if (d >= 0) g_mem_used[d] -= size;
+            return e;
+        }
         bytes = n;
-        return hipMalloc(&p, n < 256 ? 256 : n);
+        held = size;
+        ordinal = d;
+        return hipSuccess;
     }
     template <class T>
     T* as() const {
@@ -134,6 +170,18 @@ struct wsrt_device {
     std::atomic<int> live{0};      // scenes and renderers created on this device
     uint8_t* scratch = nullptr;    // this GPU's copy of the file-scope scratch array
 };
+
+extern "C" int wsrt_device_set_memory_limit(wsrt_device* dev, uint64_t bytes) {
+    CHECK_ARG(dev && dev->ordinal >= 0 && dev->ordinal < kMaxOrdinals);
+    g_mem_limit[dev->ordinal] = bytes;
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_device_get_memory_used(const wsrt_device* dev, uint64_t* out) {
+    CHECK_ARG(dev && out && dev->ordinal >= 0 && dev->ordinal < kMaxOrdinals);
+    *out = g_mem_used[dev->ordinal].load();
+    return WSRT_OK;
+}
 
 extern "C" int wsrt_device_count(void) {
     int n = 0;
@@ -2305,6 +2353,11 @@ struct wsrt_batch {
     PhotonState phs{};
     Queues qs{};
     std::vector<std::pair<hipGraph_t, hipGraphExec_t>> frame_graph;  // by frame, photon mode
+    // stopping a render: the pinned stop flag, the launch's abort counter and
+    // start time, and the device's wall-clock rate (kHz)
+    uint32_t* stop_host = nullptr;
+    DevBuf abort_buf;
+    int wall_khz = 100000;
 };
 
 static void batch_drop_graph(wsrt_batch* b) {
@@ -2350,7 +2403,7 @@ static void batch_enqueue_frame(wsrt_batch* b, uint32_t f, const BatchArgs& base
     BatchArgs a = base;
     a.one_frame = 1u;
     a.frame = f;
-    launch_batch(a, blocks, s);
+    launch_batch(a, blocks, s, f == 0);
 }
 
 // The photon pass's buffers: per job, a grid (counts, offsets, photons in
@@ -2416,6 +2469,7 @@ static void batch_release(wsrt_batch* b) {
     batch_drop_graph(b);
     for (const wsrt_batch_job& J : b->jobs) const_cast<wsrt_scene*>(J.scene)->renderers--;
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->stop_host) (void)hipHostFree(b->stop_host);
     b->dev->live--;
     delete b;
 }
@@ -2530,6 +2584,20 @@ extern "C" int wsrt_batch_create(wsrt_device* dev, const wsrt_batch_job* jobs, u
     b->args.max_depth = c.max_depth;
     b->args.one_frame = 0;
     b->args.frame = 0;
+    if (b->abort_buf.alloc(16) != hipSuccess || hipMemsetAsync(b->abort_buf.p, 0, 16, b->stream) != hipSuccess)
+        return bail(fail(WSRT_E_OOM, "batch abort counter"));
+    void* stop_dev = nullptr;
+    if (hipHostMalloc((void**)&b->stop_host, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+        return bail(fail(WSRT_E_OOM, "batch stop flag"));
+    __atomic_store_n(b->stop_host, 0u, __ATOMIC_SEQ_CST);
+    if (hipHostGetDevicePointer(&stop_dev, b->stop_host, 0) != hipSuccess)
+        return bail(fail(WSRT_E_HIP, "batch stop flag mapping"));
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev->ordinal) == hipSuccess && khz > 0)
+        b->wall_khz = khz;
+    b->args.stop = reinterpret_cast<const volatile uint32_t*>(stop_dev);
+    b->args.abort = b->abort_buf.as<uint32_t>();
+    b->args.time_limit = 0;
     b->photons = c.enable_photons != 0;
     if (b->photons) {
         rc = batch_alloc_photons(b, max_stack);
@@ -2551,6 +2619,10 @@ extern "C" int wsrt_batch_render(wsrt_batch* b) {
     CHECK_ARG(b);
     HIP_OR_FAIL(hipSetDevice(b->dev->ordinal));
     const uint32_t blocks = (uint32_t)b->cus * b->tuning.blocks_per_cu;
+    __atomic_store_n(b->stop_host, 0u, __ATOMIC_SEQ_CST);
+    // the render starts from empty images (a stopped render leaves partial ones)
+    for (size_t j = 0; j < b->jobs.size(); ++j)
+        HIP_OR_FAIL(hipMemsetAsync(b->accum[j].p, 0, b->accum[j].bytes, b->stream));
     if (b->photons) {
         // frame by frame: every job's photon pass, then one persistent
         // launch over all jobs' pixels; each frame one captured graph
@@ -2575,7 +2647,7 @@ extern "C" int wsrt_batch_render(wsrt_batch* b) {
     }
     if (!b->exec) {
         if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-            launch_batch(b->args, blocks, b->stream);
+            launch_batch(b->args, blocks, b->stream, true);
             if (hipStreamEndCapture(b->stream, &b->graph) != hipSuccess ||
                 hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
                 batch_drop_graph(b);
@@ -2583,7 +2655,7 @@ extern "C" int wsrt_batch_render(wsrt_batch* b) {
         (void)hipGetLastError();
     }
     if (b->exec) HIP_OR_FAIL(hipGraphLaunch(b->exec, b->stream));
-    else launch_batch(b->args, blocks, b->stream);
+    else launch_batch(b->args, blocks, b->stream, true);
     HIP_OR_FAIL(hipGetLastError());
     return WSRT_OK;
 }
@@ -2592,6 +2664,27 @@ extern "C" int wsrt_batch_sync(wsrt_batch* b) {
     CHECK_ARG(b);
     HIP_OR_FAIL(hipSetDevice(b->dev->ordinal));
     HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+    uint32_t dropped = 0;
+    HIP_OR_FAIL(hipMemcpy(&dropped, b->abort_buf.p, sizeof dropped, hipMemcpyDeviceToHost));
+    if (dropped)
+        return fail(WSRT_E_ABORTED, "batch render stopped (%s): %u work items not traced",
+                    __atomic_load_n(b->stop_host, __ATOMIC_SEQ_CST) ? "canceled" : "time limit", dropped);
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_batch_cancel(wsrt_batch* b) {
+    CHECK_ARG(b && b->stop_host);
+    __atomic_store_n(b->stop_host, 1u, __ATOMIC_SEQ_CST);
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_batch_set_time_limit(wsrt_batch* b, uint64_t microseconds) {
+    CHECK_ARG(b);
+    if (microseconds > (1ull << 40)) return fail(WSRT_E_INVALID, "time limit %llu us too large",
+                                                 (unsigned long long)microseconds);
+    const uint64_t ticks = microseconds * (uint64_t)b->wall_khz / 1000u;
+    b->args.time_limit = microseconds ? std::max<uint64_t>(ticks, 1u) : 0u;
+    batch_drop_graph(b);  // captured launches hold the old limit
     return WSRT_OK;
 }
 
@@ -2611,6 +2704,8 @@ extern "C" int wsrt_batch_autotune(wsrt_batch* b, wsrt_batch_tuning* best, float
             if (rc) break;
             BatchArgs a = b->args;
             a.jobs = b->djobs_tune.as<BatchJob>();  // one sample per pixel
+            a.stop = nullptr;                       // tuning passes always finish
+            a.time_limit = 0;
             const uint32_t blocks = (uint32_t)b->cus * bpc;
             if (b->photons) {
                 // frame 0's grids, then frame 0 of every job
@@ -2618,9 +2713,9 @@ extern "C" int wsrt_batch_autotune(wsrt_batch* b, wsrt_batch_tuning* best, float
                 a.one_frame = 1u;
                 a.frame = 0u;
             }
-            launch_batch(a, blocks, b->stream);  // warm
+            launch_batch(a, blocks, b->stream, true);  // warm
             (void)hipEventRecord(e0, b->stream);
-            launch_batch(a, blocks, b->stream);
+            launch_batch(a, blocks, b->stream, true);
             (void)hipEventRecord(e1, b->stream);
             float ms = 0.0f;
             if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {

@@ -19,6 +19,16 @@ namespace {
 
 constexpr int kBatchThreads = 256;
 
+// Whether the render must stop taking work items (host flag or time budget).
+__device__ __forceinline__ bool batch_stopping(const BatchArgs& a) {
+    if (a.stop && *a.stop != 0u) return true;
+    if (a.time_limit) {
+        const uint64_t t0 = *reinterpret_cast<const volatile uint64_t*>(a.abort + 2);
+        return (uint64_t)wall_clock64() - t0 > a.time_limit;
+    }
+    return false;
+}
+
 __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
     __shared__ uint2 lds_stack[kShortStack * kBatchThreads];
     __shared__ uint32_t first[kMaxBatchJobs + 1];
@@ -34,7 +44,13 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
     RayTally tally;
     while (true) {
         uint32_t item = 0;
-        if (lane == 0) item = atomicAdd(a.head, 1u);
+        if (lane == 0) {
+            item = atomicAdd(a.head, 1u);
+            if (item < a.n_items && (a.stop || a.time_limit) && batch_stopping(a)) {
+                atomicAdd(a.abort, 1u);  // the item is left untraced
+                item = 0xFFFFFFFFu;
+            }
+        }
         item = __shfl(item, 0, 64);
         if (item >= a.n_items) break;
         // job of the item: last j with first[j] <= item
@@ -76,14 +92,20 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
     }
 }
 
-__global__ void k_batch_reset(uint32_t* head) {
-    if (threadIdx.x == 0) *head = 0u;
+__global__ void k_batch_reset(uint32_t* head, uint32_t* abort, uint32_t restart) {
+    if (threadIdx.x == 0) {
+        *head = 0u;
+        if (restart && abort) {
+            abort[0] = 0u;
+            *reinterpret_cast<uint64_t*>(abort + 2) = (uint64_t)wall_clock64();
+        }
+    }
 }
 
 }  // namespace
 
-void launch_batch(const BatchArgs& a, uint32_t blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.head);
+void launch_batch(const BatchArgs& a, uint32_t blocks, hipStream_t s, bool restart) {
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.head, a.abort, restart ? 1u : 0u);
     hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(kBatchThreads), 0, s, a);
 }
 

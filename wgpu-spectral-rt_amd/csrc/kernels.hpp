@@ -180,8 +180,16 @@ struct BatchArgs {
     uint32_t tile_mode;    // 0: 64 x 1 pixel items, 1: 8 x 8 pixel items
     uint32_t one_frame;    // 1: trace frame `frame` only (jobs with more samples than that)
     uint32_t frame;
+    // stopping a render early: a host-written flag in pinned memory (null:
+    // none) and a wall-clock budget from the render's start (0: none). A
+    // wave that takes an item after either fires drops it and counts it in
+    // abort[0]; abort[2..3] hold the start time (wall-clock ticks).
+    const volatile uint32_t* stop;
+    uint32_t* abort;
+    uint64_t time_limit;
 };
-void launch_batch(const BatchArgs& a, uint32_t blocks, hipStream_t s);
+// restart: the launch starts a render (clears abort[0], stamps the start).
+void launch_batch(const BatchArgs& a, uint32_t blocks, hipStream_t s, bool restart);
 
 // ---- wavefront camera paths (path.hip) ----------------------------------------
 struct PathLaunch {

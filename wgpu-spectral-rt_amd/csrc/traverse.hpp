@@ -125,7 +125,12 @@ __device__ __forceinline__ void trav_init(const SceneView& sv, TravState& s, f3 
     // pushed and popped at once, and with best.t unchanged in between its
     // pop-time cull test could not fail)
     s.next = 0u;
-    s.have_next = sv.n_nodes != 0u;
+    // a ray with a NaN in its origin or direction (a NaN shading normal
+    // gives one) meets no triangle: its Moller-Trumbore values are NaN and
+    // fail every test, while its slab values (NaN, dropped by fmin/fmax)
+    // would open every box. It is not traversed.
+    const float sum = o.x + o.y + o.z + d.x + d.y + d.z;
+    s.have_next = sv.n_nodes != 0u && sum == sum;
     s.nvis = s.ntest = 0;
 }
 

@@ -111,6 +111,8 @@ _SIGS = {
     "wsrt_device_count": (C.c_int, []),
     "wsrt_device_create": (C.c_int, [C.c_int, P(vp)]),
     "wsrt_device_destroy": (C.c_int, [vp]),
+    "wsrt_device_set_memory_limit": (C.c_int, [vp, C.c_uint64]),
+    "wsrt_device_get_memory_used": (C.c_int, [vp, P(C.c_uint64)]),
     "wsrt_geometry_plane": (C.c_int, [vp, vp, P(u32), P(u32)]),
     "wsrt_geometry_cube": (C.c_int, [vp, vp, P(u32), P(u32)]),
     "wsrt_geometry_sphere": (C.c_int, [u32, vp, vp, P(u32), P(u32)]),
@@ -173,6 +175,8 @@ _SIGS = {
     "wsrt_batch_destroy": (C.c_int, [vp]),
     "wsrt_batch_set_tuning": (C.c_int, [vp, P(BatchTuning)]),
     "wsrt_batch_autotune": (C.c_int, [vp, P(BatchTuning), P(C.c_float)]),
+    "wsrt_batch_cancel": (C.c_int, [vp]),
+    "wsrt_batch_set_time_limit": (C.c_int, [vp, C.c_uint64]),
     "wsrt_batch_render": (C.c_int, [vp]),
     "wsrt_batch_sync": (C.c_int, [vp]),
     "wsrt_batch_read_accum": (C.c_int, [vp, u32, vp]),
@@ -288,6 +292,16 @@ class Device:
         h = vp()
         _check(load().wsrt_device_create(ordinal, C.byref(h)))
         self.h = h
+
+    def set_memory_limit(self, nbytes: int):
+        """Caps the device memory the library's objects hold (0: no cap)."""
+        _check(load().wsrt_device_set_memory_limit(self.h, nbytes))
+
+    @property
+    def memory_used(self) -> int:
+        out = C.c_uint64()
+        _check(load().wsrt_device_get_memory_used(self.h, C.byref(out)))
+        return out.value
 
     def close(self):
         """Destroys the device; a no-op while scenes or renderers created on it
@@ -760,6 +774,14 @@ class Batch:
     def set_tuning(self, tile_mode: int, blocks_per_cu: int):
         t = BatchTuning(tile_mode, blocks_per_cu)
         _check(load().wsrt_batch_set_tuning(self.h, C.byref(t)))
+
+    def cancel(self):
+        """Stops the render in flight; its sync then raises WsrtError (-6)."""
+        _check(load().wsrt_batch_cancel(self.h))
+
+    def set_time_limit(self, microseconds: int):
+        """Wall-clock budget of each render (0: none)."""
+        _check(load().wsrt_batch_set_time_limit(self.h, microseconds))
 
     def autotune(self) -> dict:
         t, ms = BatchTuning(), C.c_float()
