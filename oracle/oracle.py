@@ -71,7 +71,7 @@ LIGHT_DTYPE = np.dtype([("v0", np.float32, 4), ("v1", np.float32, 4), ("v2", np.
 def _load(fp64: bool = False):
     """liboracle.so (f32), or with fp64 liboracle64.so: the same source built
     with double arithmetic, whose float arrays and scalars are double."""
-    path = _LIB64_PATH if fp64 else _LIB_PATH
+    path = _LIB64_PATH if fp64 else os.environ.get("WSRT_ORACLE_LIB", _LIB_PATH)
     if not os.path.exists(path):
         build()
     lib = C.CDLL(path)

@@ -66,3 +66,40 @@ def clean() -> None:
 
 if __name__ == "__main__":
     print(build())
+
+
+def build_sanitized(out_dir: str) -> str:
+    """libwsrt.so with its host code (api.cpp: the C ABI, validation, the
+    scene builder, geometry and camera) built with AddressSanitizer and
+    UndefinedBehaviorSanitizer; the other objects are the regular build's
+    (run build() first). Load it with clang's ASan runtime preloaded. Device
+    code is unchanged: -Xarch_host puts the sanitizers on the host pass only."""
+    os.makedirs(out_dir, exist_ok=True)
+    objs = [os.path.join(BUILD, os.path.splitext(src)[0] + ".o") for src in SOURCES if src != "api.cpp"]
+    missing = [o for o in objs if not os.path.exists(o)]
+    if missing:
+        raise RuntimeError(f"regular objects missing ({missing[0]}): run build() first")
+    san = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+           "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer"]
+    api = os.path.join(out_dir, "api.o")
+    flags = [f for f in FLAGS if f != "-O3"] + ["-O1", "-g"]
+    res = subprocess.run([HIPCC, *flags, *san, "-c", os.path.join(CSRC, "api.cpp"), "-o", api],
+                         capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"sanitized compile failed:\n{res.stderr}")
+    lib = os.path.join(out_dir, "libwsrt.so")
+    res = subprocess.run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *san, "-shared-libsan", "-o", lib,
+                          api, *objs], capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"sanitized link failed:\n{res.stderr}")
+    return lib
+
+
+def asan_runtime() -> str:
+    """The ASan runtime of the compiler build_sanitized uses."""
+    out = subprocess.run([HIPCC, "-print-file-name=libclang_rt.asan-x86_64.so"], capture_output=True,
+                         text=True).stdout.strip()
+    if not os.path.isabs(out):
+        res = subprocess.run([HIPCC, "--print-runtime-dir"], capture_output=True, text=True).stdout.strip()
+        out = os.path.join(res, "libclang_rt.asan-x86_64.so")
+    return out

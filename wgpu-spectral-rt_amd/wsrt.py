@@ -16,7 +16,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libwsrt.so")
+# WSRT_LIB: another build of the library (the sanitized host build)
+LIB_PATH = os.environ.get("WSRT_LIB", os.path.join(_HERE, "libwsrt.so"))
 
 
 class WsrtError(RuntimeError):
