@@ -571,6 +571,18 @@ int wsrt_guide_train_step(wsrt_guide* g, const float* scalars, const float* dirs
                           uint32_t n, float lr);
 /* The network a guiding renderer trains (owned by the renderer). */
 int wsrt_renderer_get_guide(wsrt_renderer* r, wsrt_guide** out);
+/* Checkpoint and resume of training (no reference counterpart: the
+ * reference's only state is the accumulation and frame_count, which
+ * wsrt_renderer_read/write_accum and get/set_frame_count cover). The state
+ * is the parameters, both Adam moments (WSRT_GUIDE_PARAMS floats each) and
+ * the Adam step; set_state restores it exactly. With the frames a guiding
+ * renderer has trained on (guide_train_frames counts them), a renderer
+ * restored at a batch boundary renders the frames that follow bit for bit
+ * as the original would have. */
+int wsrt_guide_get_state(wsrt_guide* g, float* params, float* m1, float* m2, uint32_t* step);
+int wsrt_guide_set_state(wsrt_guide* g, const float* params, const float* m1, const float* m2, uint32_t step);
+int wsrt_renderer_get_guide_frames_trained(const wsrt_renderer* r, uint64_t* out);
+int wsrt_renderer_set_guide_frames_trained(wsrt_renderer* r, uint64_t frames);
 
 #ifdef __cplusplus
 }

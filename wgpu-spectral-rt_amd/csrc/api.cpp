@@ -2079,6 +2079,42 @@ extern "C" int wsrt_guide_get_params(wsrt_guide* g, float* out) {
     return WSRT_OK;
 }
 
+extern "C" int wsrt_guide_get_state(wsrt_guide* g, float* params, float* m1, float* m2, uint32_t* step) {
+    CHECK_ARG(g && params && m1 && m2 && step);
+    HIP_OR_FAIL(hipSetDevice(g->dev->ordinal));
+    HIP_OR_FAIL(hipMemcpyAsync(params, g->params.p, 4ull * guide::kParams, hipMemcpyDeviceToHost, g->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(m1, g->m1.p, 4ull * guide::kParams, hipMemcpyDeviceToHost, g->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(m2, g->m2.p, 4ull * guide::kParams, hipMemcpyDeviceToHost, g->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(step, g->step.p, 4, hipMemcpyDeviceToHost, g->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(g->stream));
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_guide_set_state(wsrt_guide* g, const float* params, const float* m1, const float* m2,
+                                    uint32_t step) {
+    CHECK_ARG(g && params && m1 && m2);
+    int rc = wsrt_guide_set_params(g, params);  // packs the operands, zeroes the padding
+    if (rc) return rc;
+    HIP_OR_FAIL(hipMemcpyAsync(g->m1.p, m1, 4ull * guide::kParams, hipMemcpyHostToDevice, g->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(g->m2.p, m2, 4ull * guide::kParams, hipMemcpyHostToDevice, g->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(g->step.p, &step, 4, hipMemcpyHostToDevice, g->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(g->stream));
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_renderer_get_guide_frames_trained(const wsrt_renderer* r, uint64_t* out) {
+    CHECK_ARG(r && out);
+    *out = r->frames_trained;
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_renderer_set_guide_frames_trained(wsrt_renderer* r, uint64_t frames) {
+    CHECK_ARG(r);
+    if (!r->guide) return fail(WSRT_E_STATE, "the renderer was created without guiding");
+    r->frames_trained = frames;
+    return WSRT_OK;
+}
+
 extern "C" int wsrt_guide_set_params(wsrt_guide* g, const float* in) {
     CHECK_ARG(g && in);
     HIP_OR_FAIL(hipSetDevice(g->dev->ordinal));

@@ -171,6 +171,10 @@ _SIGS = {
     "wsrt_guide_gradients": (C.c_int, [vp, vp, vp, vp, u32, vp]),
     "wsrt_guide_train_step": (C.c_int, [vp, vp, vp, vp, u32, C.c_float]),
     "wsrt_renderer_get_guide": (C.c_int, [vp, P(vp)]),
+    "wsrt_guide_get_state": (C.c_int, [vp, vp, vp, vp, P(u32)]),
+    "wsrt_guide_set_state": (C.c_int, [vp, vp, vp, vp, u32]),
+    "wsrt_renderer_get_guide_frames_trained": (C.c_int, [vp, P(C.c_uint64)]),
+    "wsrt_renderer_set_guide_frames_trained": (C.c_int, [vp, C.c_uint64]),
     "wsrt_autotune": (C.c_int, [vp, vp, u32, u32, vp, P(Config), C.c_uint64, P(AutotuneResult)]),
     "wsrt_batch_create": (C.c_int, [vp, vp, u32, P(Config), P(vp)]),
     "wsrt_batch_destroy": (C.c_int, [vp]),
@@ -499,6 +503,18 @@ class Guide:
         assert w.shape == (GUIDE_PARAMS,)
         _check(load().wsrt_guide_set_params(self.h, _p(w)))
 
+    def state(self) -> dict:
+        """Parameters, Adam moments and step (a training checkpoint)."""
+        p, m1, m2 = (np.zeros(GUIDE_PARAMS, np.float32) for _ in range(3))
+        step = u32()
+        _check(load().wsrt_guide_get_state(self.h, _p(p), _p(m1), _p(m2), C.byref(step)))
+        return {"params": p, "m1": m1, "m2": m2, "step": step.value}
+
+    def set_state(self, st: dict):
+        p, m1, m2 = (np.ascontiguousarray(st[k], np.float32) for k in ("params", "m1", "m2"))
+        assert p.size == m1.size == m2.size == GUIDE_PARAMS
+        _check(load().wsrt_guide_set_state(self.h, _p(p), _p(m1), _p(m2), int(st["step"])))
+
     def forward(self, scalars) -> np.ndarray:
         x = np.ascontiguousarray(scalars, np.float32).reshape(-1, 7)
         out = np.zeros((len(x), GUIDE_OUTPUTS), np.float32)
@@ -553,6 +569,16 @@ class Renderer:
 
     def set_camera(self, cam: CameraUniform):
         _check(load().wsrt_renderer_set_camera(self.h, C.byref(cam)))
+
+    @property
+    def guide_frames_trained(self) -> int:
+        out = C.c_uint64()
+        _check(load().wsrt_renderer_get_guide_frames_trained(self.h, C.byref(out)))
+        return out.value
+
+    @guide_frames_trained.setter
+    def guide_frames_trained(self, n: int):
+        _check(load().wsrt_renderer_set_guide_frames_trained(self.h, n))
 
     def guide(self) -> Guide:
         """The guiding network this renderer trains (config.guiding = 1)."""
