@@ -63,8 +63,27 @@ def cpu_baseline(seconds: float = 12.0) -> dict:
             "sample": f"{frames} full frames (512x512, 2^20 photons each) in {dt:.1f} s"}
 
 
-KERNEL_SYMBOLS = {"extend": "k_extend", "shadow": "k_shadow", "photon_trace": "k_photon_bounce"}
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_{}_caustic512.txt")
+# the kernels a summary names, by their short symbol (the closest-hit kernel
+# is the lane-refill one where the workload's configuration uses it)
+KERNEL_SYMBOLS = {"extend": ("k_extend", "void k_extend_refill<false>"), "shadow": ("k_shadow",),
+                  "photon_trace": ("k_photon_bounce",)}
+
+
+def _symbol(kernel: str, *per_kernel: dict) -> str:
+    """The short symbol of `kernel` present in every one of the per-kernel
+    dictionaries."""
+    for sym in KERNEL_SYMBOLS[kernel]:
+        if all(sym in d for d in per_kernel):
+            return sym
+    raise KeyError(f"no symbol of {kernel} in the profile")
+# profiled workloads: file tag -> description (--profile-workload picks the
+# one a summary is written for)
+PROFILE_WORKLOADS = {
+    "caustic512": "caustic photon mapping, 512x512 (glass icosphere 5120 tris, 2^20 photons/frame, 2^22 cells)",
+    "room1024": "occluded-light room, 2,097,182 tris, 1024x1024, plain tracer, 128 frames in flight",
+}
+PROFILE_TAG = "caustic512"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_{}_{}.txt")
 
 
 def _short(name: str) -> str:
@@ -101,12 +120,13 @@ def summarize_traffic(pmc_dir: str, kernel: str, command: str) -> int:
     cw = _per_dispatch(os.path.join(pmc_dir, "calib_write_counter_collection.csv"), "WRITE_SIZE")
     copy_name = max(cf, key=lambda k: cf[k])
     f_scale, w_scale = copy_kib / cf[copy_name], copy_kib / cw[copy_name]
-    sym = KERNEL_SYMBOLS[kernel]
-    fk = _per_dispatch(os.path.join(pmc_dir, "fetch_counter_collection.csv"), "FETCH_SIZE")[sym]
-    wk = _per_dispatch(os.path.join(pmc_dir, "write_counter_collection.csv"), "WRITE_SIZE")[sym]
+    fd = _per_dispatch(os.path.join(pmc_dir, "fetch_counter_collection.csv"), "FETCH_SIZE")
+    wd = _per_dispatch(os.path.join(pmc_dir, "write_counter_collection.csv"), "WRITE_SIZE")
+    sym = _symbol(kernel, fd, wd)
+    fk, wk = fd[sym], wd[sym]
     fb, wb = fk * f_scale * 1024, wk * w_scale * 1024
-    with open(TRAFFIC_FILE.format(kernel), "w") as f:
-        f.write(f"# HBM traffic per launch of {sym} (caustic photon mapping, 512x512)\n")
+    with open(TRAFFIC_FILE.format(kernel, PROFILE_TAG), "w") as f:
+        f.write(f"# HBM traffic per launch of {sym} ({PROFILE_WORKLOADS[PROFILE_TAG]})\n")
         f.write(f"# command: {command}\n")
         f.write(f"# calibration: {copy_name} of {copy_kib} KiB reads FETCH_SIZE {cf[copy_name]:.1f} KiB, "
                 f"WRITE_SIZE {cw[copy_name]:.1f} KiB per dispatch\n")
@@ -114,14 +134,14 @@ def summarize_traffic(pmc_dir: str, kernel: str, command: str) -> int:
         f.write(f"fetch_kib_per_dispatch_raw = {fk:.1f}\nwrite_kib_per_dispatch_raw = {wk:.1f}\n")
         f.write(f"fetch_bytes_per_launch = {fb:.0f}\nwrite_bytes_per_launch = {wb:.0f}\n")
         f.write(f"traffic_bytes_per_launch = {fb + wb:.0f}\n")
-    print(open(TRAFFIC_FILE.format(kernel)).read())
+    print(open(TRAFFIC_FILE.format(kernel, PROFILE_TAG)).read())
     return 0
 
 
-def hbm_traffic(kernel: str):
+def hbm_traffic(kernel: str, tag: str = "caustic512"):
     """HBM bytes per launch of `kernel` from its committed counter summary,
     or None when there is none."""
-    path = TRAFFIC_FILE.format(kernel)
+    path = TRAFFIC_FILE.format(kernel, tag)
     if not os.path.exists(path):
         return None
     vals = {}
@@ -132,7 +152,7 @@ def hbm_traffic(kernel: str):
     return vals.get("traffic_bytes_per_launch")
 
 
-ISSUE_FILE = os.path.join(ROOT, "profiles", "issue_{}_caustic512.txt")
+ISSUE_FILE = os.path.join(ROOT, "profiles", "issue_{}_{}.txt")
 ISSUE_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_SMEM", "SQ_INSTS_VMEM", "SQ_INSTS_LDS")
 
 
@@ -140,31 +160,32 @@ def summarize_issue(pmc_dir: str, kernel: str, command: str) -> int:
     """Writes the instructions issued per launch of one kernel (wave-level
     counts, summed over the chip) from one rocprofv3 --pmc run of the SQ
     instruction counters in pmc_dir (issue_counter_collection.csv)."""
-    sym = KERNEL_SYMBOLS[kernel]
     path = os.path.join(pmc_dir, "issue_counter_collection.csv")
-    with open(ISSUE_FILE.format(kernel), "w") as f:
-        f.write(f"# instructions issued per launch of {sym} (caustic photon mapping, 512x512), "
+    sym = _symbol(kernel, _per_dispatch(path, "SQ_INSTS_VALU"))
+    with open(ISSUE_FILE.format(kernel, PROFILE_TAG), "w") as f:
+        f.write(f"# instructions issued per launch of {sym} ({PROFILE_WORKLOADS[PROFILE_TAG]}), "
                 "wave-level counts summed over the chip\n")
         f.write(f"# command: {command}\n")
         for c in ISSUE_COUNTERS:
             per = _per_dispatch(path, c)
             if sym in per:
                 f.write(f"{c.lower()}_per_launch = {per[sym]:.1f}\n")
-    print(open(ISSUE_FILE.format(kernel)).read())
+    print(open(ISSUE_FILE.format(kernel, PROFILE_TAG)).read())
     return 0
 
 
 VALU_PEAK_WAVE_INSTS = 256 * 2.4e9  # MI355X: 256 CUs x 2.4 GHz, one wave64 VALU instruction per CU and clock
 
 
-def issue_rate(kernel: str, cnt: dict, frames_in_flight: int, batch_ms: float | None = None):
+def issue_rate(kernel: str, cnt: dict, frames_in_flight: int, batch_ms: float | None = None,
+               launches: int | None = None, tag: str = "caustic512"):
     """VALU and SALU instructions per BVH8 node visit of `kernel`: the
     committed per-launch instruction counts (rocprofv3 SQ counters at this
     workload and frames_in_flight) times the launches of one batch, over the
     node visits of one batch from the in-kernel counters. A wave instruction
     serves 64 lanes, so 64 x VALU per lane-visit is the issue cost of one
     useful node visit; lockstep idling inflates it."""
-    path = ISSUE_FILE.format(kernel)
+    path = ISSUE_FILE.format(kernel, tag)
     idx = {"extend": 1, "shadow": 2, "photon_trace": 0}.get(kernel)
     if not os.path.exists(path) or idx is None:
         return None
@@ -173,7 +194,8 @@ def issue_rate(kernel: str, cnt: dict, frames_in_flight: int, batch_ms: float | 
         if "=" in line and not line.startswith("#"):
             k, v = line.split("=", 1)
             vals[k.strip()] = float(v)
-    launches = {"extend": 12, "shadow": 12, "photon_trace": 5}[kernel]
+    if launches is None:
+        launches = {"extend": 12, "shadow": 12, "photon_trace": 5}[kernel]
     nodes = cnt["kernel_nodes"][idx]
     if not nodes or "sq_insts_valu_per_launch" not in vals:
         return None
@@ -206,7 +228,7 @@ def summarize_kernels(trace_dir: str, first_batch: int, n_batches: int, frames_p
 
     stats = os.path.join(trace_dir, "bench_kernel_stats.csv")
     trace = os.path.join(trace_dir, "bench_kernel_trace.csv")
-    out = os.path.join(ROOT, "profiles", "kernels_caustic512.txt")
+    out = os.path.join(ROOT, "profiles", f"kernels_{PROFILE_TAG}.txt")
     rows = []
     with open(trace) as f:
         for row in csv.DictReader(f):
@@ -221,8 +243,7 @@ def summarize_kernels(trace_dir: str, first_batch: int, n_batches: int, frames_p
         per[k].append((b - a) / 1000.0)
     span = (max(b for _, b, _ in sel) - sel[0][0]) / 1000.0
     with open(out, "w") as f:
-        f.write("# rocprofv3 kernel statistics, caustic photon mapping 512x512 (glass icosphere 5120 tris, "
-                "2^20 photons/frame, 2^22 cells)\n")
+        f.write(f"# rocprofv3 kernel statistics, {PROFILE_WORKLOADS[PROFILE_TAG]}\n")
         f.write(f"# command: {command}\n")
         with open(stats) as g:
             rd = list(csv.DictReader(g))
@@ -448,6 +469,76 @@ def guide_train_steps(n: int, samples: int = 16384) -> int:
     return 0
 
 
+def room_throughput(batches: int = 2, warmup: int = 2) -> dict:
+    """The plain tracer on the occluded-light room (2,097,182 triangles,
+    1024 x 1024, depth 8, its shipped configuration: 128 frames in flight,
+    lane refill, path-ordered queues): `batches` batches timed as graph
+    replays after `warmup` untimed ones, then one batch with per-kernel HIP
+    events and one with the traversal counters."""
+    import scenes
+    import wsrt
+
+    dev = wsrt.Device(0)
+    scene = scenes.build_occluded_room(wsrt.SceneBuilder()).build(dev)
+    v = scenes.OCCLUDED_VIEW
+    W = 1024
+    view = scenes.View(v.position, v.yaw, v.pitch, W, W)
+    cfg = scenes.occluded_config(frames_in_flight=128)
+    S = cfg.frames_in_flight
+    r = wsrt.Renderer(dev, scene, W, W, cfg)
+    r.set_camera(scenes.camera(view, 0))
+    r.render_frames(S * max(warmup, 1))
+    r.sync()
+    r.reset_stats()
+    t0 = time.perf_counter()
+    r.render_frames(S * batches)
+    r.sync()
+    dt = time.perf_counter() - t0
+    st = r.stats()
+    r.reset_stats()
+    r.reset_profile()
+    r.set_profiling(True)
+    r.render_frames(S)
+    r.sync()
+    r.set_profiling(False)
+    pst, prof = r.stats(), r.profile()
+    r.reset_stats()
+    r.set_traversal_counting(True)
+    r.render_frames(S)
+    r.sync()
+    cnt = r.stats()
+    r.close()
+    frames = S * batches
+    kern = {}
+    for name, idx in (("extend", 1), ("shadow", 2)):
+        p = prof[name]
+        counted = max(cnt["kernel_counted_rays"][idx], 1)
+        nodes, tris = cnt["kernel_nodes"][idx] / counted, cnt["kernel_tris"][idx] / counted
+        bpr = RAY_BYTES + nodes * NODE_BYTES + tris * TRI_BYTES
+        rays = pst["kernel_rays"][idx]
+        gbs = rays * bpr / (p["total_ms"] * 1e-3) / 1e9
+        kern[name] = {"ms": round(p["total_ms"], 3), "launches": p["launches"], "rays": rays,
+                      "rays_per_s": round(rays / (p["total_ms"] * 1e-3), 1),
+                      "nodes_per_ray": round(nodes, 3), "tris_per_ray": round(tris, 3),
+                      "max_nodes_of_a_ray": cnt["ray_max_nodes"][idx], "bytes_per_ray": round(bpr, 1),
+                      "GBps": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        traffic = hbm_traffic(name, "room1024")
+        if traffic:
+            kern[name]["traffic_bytes_per_launch"] = traffic
+        iss = issue_rate(name, cnt, S, p["total_ms"], launches=p["launches"], tag="room1024")
+        if iss:
+            kern[name]["issue"] = iss
+    for name in ("shade", "generate", "accumulate", "order", "tail"):
+        if name in prof and prof[name]["launches"]:
+            kern[name] = {"ms": round(prof[name]["total_ms"], 3), "launches": prof[name]["launches"]}
+    return {"workload": "occluded-light room, 1024x1024, depth 8, plain tracer, 128 frames in flight, "
+                        "lane refill 16, path-ordered queues",
+            "triangles": scene.info.n_triangles, "frames": frames,
+            "Msamples_per_s": round(W * W * frames / dt / 1e6, 3), "ms_per_spp": round(1000 * dt / frames, 3),
+            "rays_per_s": round((st["extend_rays"] + st["shadow_rays"]) / dt, 1),
+            "event_timed_batch_ms": round(sum(p["total_ms"] for p in prof.values()), 3), "kernels": kern}
+
+
 def heightfield_throughput(frames: int = 16) -> dict:
     """Film-coated GGX heightfield (2,093,058 triangles) under the seeded sky,
     1920 x 1080, depth 6, `frames` samples per pixel, plain tracer."""
@@ -572,11 +663,18 @@ def main() -> int:
     ap.add_argument("--summarize-issue", nargs=3, metavar=("PMC_DIR", "KERNEL", "COMMAND"),
                     help="write the per-launch instruction counts of KERNEL from a rocprofv3 --pmc run "
                          "of the SQ instruction counters and exit")
+    ap.add_argument("--profile-workload", choices=sorted(PROFILE_WORKLOADS), default="caustic512",
+                    help="the workload a --summarize-* summary is written for")
+    ap.add_argument("--room-only", action="store_true",
+                    help="only the plain tracer on the occluded-light room (one JSON line; --steps batches "
+                         "of 128 frames timed after --warmup)")
     ap.add_argument("--summarize-kernels", nargs=5,
                     metavar=("TRACE_DIR", "FIRST_BATCH", "N_BATCHES", "FRAMES_PER_BATCH", "COMMAND"),
                     help="write the kernel statistics and the timed batches' per-kernel times from a "
                          "rocprofv3 --kernel-trace --stats run and exit")
     args = ap.parse_args()
+    global PROFILE_TAG
+    PROFILE_TAG = args.profile_workload
     if args.summarize_traffic:
         return summarize_traffic(*args.summarize_traffic)
     if args.summarize_issue:
@@ -595,6 +693,9 @@ def main() -> int:
 
     if args.guided_only:
         print(json.dumps(time_to_relmse(args.guided_frames, boxes=args.guided_boxes, width=args.guided_width)))
+        return 0
+    if args.room_only:
+        print(json.dumps(room_throughput(args.steps, args.warmup)))
         return 0
 
     import scenes
@@ -762,6 +863,8 @@ def main() -> int:
         result["other_configs"] = {"small_box": small_box_throughput(), "heightfield": heightfield_throughput()}
         log("batch ...")
         result["other_configs"]["batch"] = batch_throughput()
+        log("room ...")
+        result["other_configs"]["room"] = room_throughput()
     if not args.no_guided:
         log("guided vs plain time-to-relMSE ...")
         r.close()
