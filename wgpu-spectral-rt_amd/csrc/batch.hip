@@ -19,9 +19,11 @@ namespace {
 
 constexpr int kBatchThreads = 256;
 
-// Whether the render must stop taking work items (host flag or time budget).
-__device__ __forceinline__ bool batch_stopping(const BatchArgs& a) {
-    if (a.stop && *a.stop != 0u) return true;
+// Whether the render must stop taking work items: the time budget at every
+// item, the host flag (a read over the host link, ~1 us) at every 32nd item
+// a wave takes.
+__device__ __forceinline__ bool batch_stopping(const BatchArgs& a, uint32_t taken) {
+    if (a.stop && (taken & 31u) == 0u && *a.stop != 0u) return true;
     if (a.time_limit) {
         const uint64_t t0 = *reinterpret_cast<const volatile uint64_t*>(a.abort + 2);
         return (uint64_t)wall_clock64() - t0 > a.time_limit;
@@ -42,11 +44,12 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
     st.lds_stride = kBatchThreads;
     st.glob = a.gstack + (uint64_t)gtid * a.spill;
     RayTally tally;
+    uint32_t taken = 0;  // items this wave has taken
     while (true) {
         uint32_t item = 0;
         if (lane == 0) {
             item = atomicAdd(a.head, 1u);
-            if (item < a.n_items && (a.stop || a.time_limit) && batch_stopping(a)) {
+            if (item < a.n_items && (a.stop || a.time_limit) && batch_stopping(a, taken++)) {
                 atomicAdd(a.abort, 1u);  // the item is left untraced
                 item = 0xFFFFFFFFu;
             }
