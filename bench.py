@@ -647,6 +647,9 @@ def main() -> int:
     ap.add_argument("--guided-only", action="store_true", help="only the guided-vs-plain leg (one JSON line)")
     ap.add_argument("--guided-frames", type=int, default=64)
     ap.add_argument("--frames-in-flight", type=int, default=0, help="frames per wavefront (default: config's)")
+    ap.add_argument("--extend-refill", type=int, default=-1,
+                    help="with --frames-in-flight: the closest-hit kernel's lane refill threshold "
+                         "(default: the configuration's)")
     ap.add_argument("--guided-boxes", type=int, default=None)
     ap.add_argument("--guided-width", type=int, default=1024)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -709,6 +712,8 @@ def main() -> int:
     tune = None
     if args.frames_in_flight:
         cfg.frames_in_flight = args.frames_in_flight
+        if args.extend_refill >= 0:
+            cfg.extend_refill = args.extend_refill
     else:
         # frames per wavefront picked by the library's autotuner (results do
         # not depend on it); the timed frames run at exactly that setting

@@ -468,6 +468,7 @@ struct wsrt_scene {
     DevBuf spec_instances, spec_nodes, spec_tris;
     SceneView view{};
     wsrt_scene_info info{};
+    uint32_t n_point_lights = 0;  // lights of kind 1 (the guided light proposal needs none)
 };
 
 static void linear_inverse(const float* m, float* w) {
@@ -524,6 +525,7 @@ extern "C" int wsrt_scene_build(wsrt_scene_builder* b, wsrt_device* dev, wsrt_sc
     HIP_OR_FAIL(upload(sc->meshes, mi, s));
     HIP_OR_FAIL(upload(sc->materials, b->materials, s));
     HIP_OR_FAIL(upload(sc->lights, b->lights, s));
+    for (const wsrt_light_info& L : b->lights) sc->n_point_lights += L.params[0] == 1.0f ? 1u : 0u;
     HIP_OR_FAIL(upload(sc->instances, inst, s));
     HIP_OR_FAIL(upload(sc->env, b->env, s));
     hipEvent_t e0, e1;
@@ -892,6 +894,7 @@ static FrameConst make_frame_const(const wsrt_config& c, const wsrt_camera_unifo
     fc.enable_photons = c.enable_photons;
     fc.n_photons = c.enable_photons ? c.photons_per_pass : 0u;
     fc.quirks = c.quirks;
+    fc.guide_light_mix = 1.0f;
     mat4_mul(cam.view_inverse, cam.proj_inverse, fc.M);
     fc.origin[0] = cam.view_inverse[12];
     fc.origin[1] = cam.view_inverse[13];
@@ -1071,9 +1074,12 @@ struct wsrt_renderer {
     uint64_t frames = 0, samples = 0, photon_paths = 0;
     // path guiding (cfg.guiding)
     std::unique_ptr<wsrt_guide> guide;
-    DevBuf g_in0, g_in1, g_dir, g_tp, g_Lb, g_n, g_alb, g_q;
+    DevBuf g_in0, g_in1, g_dir, g_tp, g_Lb, g_nee, g_n, g_alb, g_q;
     uint32_t guide_ns = 0, guide_stride = 1, guide_batch = 0;
     uint64_t frames_trained = 0;  // frames whose records trained the network
+    // guided next-event estimation: probability of the uniform light sample
+    // (1: the plain tracer's light sampling only)
+    float guide_light_mix = 1.0f;
 };
 
 namespace {
@@ -1262,7 +1268,7 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
     pl.qs.trace_rounds = c.trace_rounds ? c.trace_rounds : 1u;
     pl.guided = c.guiding;
     pl.extend_refill = c.extend_refill;
-    pl.guide_mis = (c.guiding && (c.quirks & WSRT_FIX_EMISSION_MIS)) ? 1u : 0u;
+    pl.guide_mis = c.guiding ? 1u : 0u;  // the shadow kernel flags visible guided light samples
     if (c.guiding) {
         const uint32_t want = (uint32_t)std::min<uint64_t>(c.guide_train_samples, npx * guide::kRecBounces);
         // whole chunks of samples per step (at least one step of 64)
@@ -1276,6 +1282,10 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
         HIP_OR_FAIL(r->g_dir.alloc(16 * nrec));
         HIP_OR_FAIL(r->g_tp.alloc(16 * nrec));
         HIP_OR_FAIL(r->g_Lb.alloc(16 * nrec));
+        HIP_OR_FAIL(r->g_nee.alloc(16 * nrec));
+        // the light proposal draws from the guide when every light is an
+        // area light and there are few (one triangle test per light)
+        r->guide_light_mix = (scene->n_point_lights == 0 && scene->view.n_lights <= kGuideLightsMax) ? 0.5f : 1.0f;
         HIP_OR_FAIL(r->g_n.alloc(16 * npaths));
         HIP_OR_FAIL(r->g_alb.alloc(16 * npaths));
         HIP_OR_FAIL(r->g_q.alloc(4ull * kSegs * seg));
@@ -1296,7 +1306,8 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
             g->gv.binv[a] = ext > 0.0f ? 1.0f / ext : 1.0f;
         }
         g->gv.rec = guide::Records{r->g_in0.as<float4>(), r->g_in1.as<float4>(), r->g_dir.as<float4>(),
-                                   r->g_tp.as<float4>(), r->g_Lb.as<float4>(), (uint32_t)npx, ns,
+                                   r->g_tp.as<float4>(), r->g_Lb.as<float4>(), r->g_nee.as<float4>(),
+                                   (uint32_t)npx, ns,
                                    r->guide_stride, guide::fast_div((uint32_t)npx),
                                    guide::fast_div(r->guide_stride)};
         pl.gv = g->gv;
@@ -1394,13 +1405,15 @@ static int enqueue_batch(wsrt_renderer* r, uint32_t nf, bool train) {
             ProfScope p(r, K_SHADE);
             launch_shade(pl, npx, b, nq, s);
         }
+        if (pl.guided) {
+            // scattering and next-event estimation of the Lambert vertices,
+            // whose shadow rays the shadow kernel then traces
+            ProfScope p(r, K_GUIDE_SCATTER);
+            launch_guide_scatter(pl.sv, pl.ps, pl.qs, pl.gv, pl.fa, b, nq, pl.ctr, s);
+        }
         {
             ProfScope p(r, K_SHADOW);
             launch_shadow(pl, b, s);
-        }
-        if (pl.guided) {
-            ProfScope p(r, K_GUIDE_SCATTER);
-            launch_guide_scatter(pl.ps, pl.qs, pl.gv, pl.fa, b, nq, pl.ctr, s);
         }
         if (pl.ps.tag && b + 1 < r->cfg.max_depth) {
             ProfScope p(r, K_ORDER);
@@ -1482,6 +1495,7 @@ static int submit_frames(wsrt_renderer* r, uint32_t first, uint32_t n, const wsr
             wsrt_camera_uniform c = cam;
             c.frame_count = f0 + i;
             st[i] = make_frame_const(r->cfg, c, r->W, r->H);
+            st[i].guide_light_mix = r->guide_light_mix;
         }
         // device slots f & mask: at most two contiguous ranges
         uint32_t slot = f0 & (kFrameTable - 1);

@@ -105,6 +105,13 @@ enum RayKernel : int {
 };
 
 // Wavefront path state, structure of arrays indexed by path id (= pixel).
+// Path flags of the guided tracer, for the Lambert vertices of bounces
+// 0 .. kGuideRecordBounces - 1: kFlagRecorded << b marks bounce b's vertex as
+// recorded for training; kFlagNeeVisible << b marks its light sample as
+// unoccluded (its contribution is then excluded from the training target).
+constexpr uint32_t kGuideRecordBounces = 4;
+constexpr uint32_t kFlagRecorded = 4u, kFlagNeeVisible = 64u;
+
 struct PathState {
     float4* o;          // origin xyz
     float4* d;          // direction xyz
@@ -173,6 +180,9 @@ struct FrameConst {
     float origin[3];           // camera position (view_inverse column 3)
     uint32_t count_traversal;  // 1: count node visits / triangle tests
     uint32_t quirks;           // wsrt_config.quirks (WSRT_FIX_* bits)
+    // guided next-event estimation: probability of the uniform light sample
+    // against the guide's light proposal (1: uniform only)
+    float guide_light_mix;
 };
 
 constexpr uint32_t kFrameTable = 4096;  // entries; frames wrap modulo this

@@ -35,7 +35,6 @@ constexpr int kTrainWaves = 4;        // waves per block of the training kernel
 constexpr int kTrainTiles = 1;
 constexpr int kBwdTiles = 1;
 // path flags 4 << b: the Lambert vertex of bounce b is recorded for training
-constexpr uint32_t kFlagRecorded = 4u;
 
 // Round to nearest even: gfx950's v_cvt_pk_bf16_f32 (a NaN stays a quiet
 // NaN), paired into one instruction per two values where the compiler can.
@@ -252,8 +251,119 @@ __device__ __forceinline__ void load_biases(float* dst, const float* params) {
 // guide (probability mix) and the reference's cosine sampling, throughput
 // update f cos / p_mix, Russian roulette (shader.wgsl:694-700) and the
 // training record of the path's first Lambert vertex.
-__global__ __launch_bounds__(64 * kScatterBlockWaves) void k_guide_scatter(PathState ps, Queues qs, GuideView gv,
-                                                                      FrameArgs fa, uint32_t bounce,
+// A direction from the mixture: lobe k with probability pi_k, then its vMF.
+__device__ __forceinline__ f3 mixture_sample(const Mixture& m, uint32_t& rng) {
+    const float xl = rand01(rng);
+    int k = 0;
+    float c = m.pi[0];
+#pragma unroll
+    for (int j = 1; j < kLobes; ++j)
+        if (xl >= c) {
+            k = j;
+            c += m.pi[j];
+        }
+    f3 mu = m.mu[0];
+    float kap = m.kappa[0];
+#pragma unroll
+    for (int j = 1; j < kLobes; ++j)
+        if (k == j) {
+            mu = m.mu[j];
+            kap = m.kappa[j];
+        }
+    const float x1 = rand01(rng), x2 = rand01(rng);
+    return vmf_sample(mu, kap, x1, x2);
+}
+
+// Next-event estimation at a guided Lambert vertex with a learned light
+// proposal. With probability light_mix the light point comes from the plain
+// tracer's uniform light sampling (shader.wgsl:477-528); otherwise a
+// direction is drawn from the vertex's mixture and the nearest area-light
+// triangle along it is the light point (none, or its back: no sample). The
+// sample is weighted by the density of the two proposals together, q =
+// light_mix lp + (1 - light_mix) p_guide (solid angle; lp the uniform
+// sampler's), so the estimate of the reference's NEE integral -- L f cos
+// weighted by lp / (lp + cos/pi), shader.wgsl:650-671 -- is unbiased for any
+// mixture; with mis (WSRT_FIX_EMISSION_MIS) the weight is the balance
+// heuristic against the vertex's own scattering density p_mix instead.
+struct GuidedNee {
+    bool shadow;
+    f3 dir;
+    float dist;
+    float4 contrib;
+};
+__device__ __forceinline__ GuidedNee guided_nee(const SceneView& sv, const Mixture& m, float guide_mix,
+                                                float light_mix, bool mis, f3 hp, f3 ff, float4 wl, float4 tp,
+                                                float4 spec, uint32_t& rng) {
+    GuidedNee g;
+    g.shadow = false;
+    g.dir = mk3(0, 0, 0);
+    g.dist = 0.0f;
+    g.contrib = splat4(0.0f);
+    const uint32_t nl = sv.n_lights;
+    if (nl == 0u) return g;
+    f3 dir = mk3(0, 0, 1);
+    float dist = 0.0f, lp = 0.0f;
+    float4 rad = splat4(0.0f);
+    bool delta = false;
+    if (light_mix >= 1.0f || rand01(rng) < light_mix) {
+        const LightSample ls = sample_light(sv, hp, wl, rng);
+        if (ls.pdf <= 0.0f) return g;
+        dir = ls.dir;
+        dist = ls.dist;
+        lp = ls.pdf;
+        rad = ls.radiance;
+        delta = ls.delta;
+    } else {
+        dir = mixture_sample(m, rng);
+        // the nearest light triangle along dir (every light is an area
+        // light here: light_mix < 1 only then)
+        float best = 3.0e38f;
+        uint32_t bi = nl;
+        f3 bn = mk3(0, 0, 0);
+        float barea = 0.0f;
+        for (uint32_t li = 0; li < nl; ++li) {
+            const wsrt_light_info& L = sv.lights[li];
+            const f3 a = xyz(*reinterpret_cast<const float4*>(L.v0));
+            const f3 e1 = xyz(*reinterpret_cast<const float4*>(L.v1)) - a;
+            const f3 e2 = xyz(*reinterpret_cast<const float4*>(L.v2)) - a;
+            float t, u, v;
+            bool fr;
+            if (tri_intersect(hp, dir, a, e1, e2, t, u, v, fr) && t > 0.001f && t < best) {
+                best = t;
+                bi = li;
+                const f3 c = cross(e1, e2);
+                bn = normalize(c);
+                barea = length(c) * 0.5f;
+            }
+        }
+        if (bi == nl) return g;
+        const float cl = fmaxf(dot(bn, -dir), 0.0f);
+        if (cl <= 0.0001f) return g;  // its back: the uniform sampler gives it no density either
+        dist = best;
+        lp = (best * best) / (barea * (float)nl * cl);
+        const float4 prm = *reinterpret_cast<const float4*>(sv.lights[bi].params);
+        rad = mul4s(blackbody4(wl, prm.y), prm.z);
+    }
+    const float ct = fmaxf(dot(ff, dir), 0.0f);
+    if (ct <= 0.0f) return g;
+    const float bp = ct / kPi;
+    float w;
+    if (delta) {
+        w = 1.0f / lp;  // a point light (light_mix is 1 then)
+    } else {
+        const float pg = mixture_pdf(m, dir);
+        const float q = light_mix * lp + (1.0f - light_mix) * pg;
+        w = mis ? 1.0f / (q + guide_mix * pg + (1.0f - guide_mix) * bp) : (lp / (lp + bp)) / q;
+    }
+    g.shadow = true;
+    g.dir = dir;
+    g.dist = dist;
+    g.contrib = mul4s(mul4s(mul4(mul4(tp, div4s(spec, kPi)), rad), ct), w);
+    return g;
+}
+
+__global__ __launch_bounds__(64 * kScatterBlockWaves) void k_guide_scatter(SceneView sv, PathState ps, Queues qs,
+                                                                      GuideView gv, FrameArgs fa, uint32_t bounce,
                                                                       uint32_t* __restrict__ next_queue,
                                                                       Counters* __restrict__ ctr) {
     __shared__ bf16x8 Wf[kFwdPack / 8];
@@ -269,24 +379,21 @@ __global__ __launch_bounds__(64 * kScatterBlockWaves) void k_guide_scatter(PathS
     const uint32_t base_frame = *fa.frame;
     const uint32_t rr_after = fa.table[base_frame & fa.mask].rr_after;
     const bool mis = (fa.table[base_frame & fa.mask].quirks & WSRT_FIX_EMISSION_MIS) != 0u;
+    const float light_mix = fa.table[base_frame & fa.mask].guide_light_mix;
     const int wave = threadIdx.x >> 6, l = __lane_id();
     _Float16* stage = stage_all[wave];
     const uint32_t wstride = gridDim.x * kScatterBlockWaves * 64;
     for (uint32_t base = (blockIdx.x * kScatterBlockWaves + wave) * 64; base < n; base += wstride) {
         const uint32_t i = base + l;
         const bool have = i < n;
-        uint32_t p = 0, se = 0xFFFFFFFFu;
+        uint32_t p = 0;
         f3 hp = mk3(0, 0, 0), ff = mk3(0, 0, 1);
         float4 wl = splat4(500.0f);
-        float lp = 0.0f;
         float s[kScalars];
         if (have) {
             p = ps.guide_q[seg_entry(pre, kSegs, i, qs.seg_cap)];
-            const float4 o4 = ps.o[p], n4 = ps.gn[p];
-            hp = xyz(o4);
-            lp = o4.w;
-            ff = xyz(n4);
-            se = __float_as_uint(n4.w);
+            hp = xyz(ps.o[p]);
+            ff = xyz(ps.gn[p]);
             wl = ps.wl[p];
         }
         scalars_of(gv, hp, ff, wl.x, s);
@@ -302,7 +409,8 @@ __global__ __launch_bounds__(64 * kScatterBlockWaves) void k_guide_scatter(PathS
 #pragma unroll
         for (int k = 0; k < kOut; ++k) o[k] = (float)stage[l * kStage + k];
         wave_sync();
-        bool alive = false;
+        bool alive = false, shadow = false;
+        GuidedNee nee;
         if (have) {
             Mixture m;
             mixture_from(o, m);
@@ -310,44 +418,12 @@ __global__ __launch_bounds__(64 * kScatterBlockWaves) void k_guide_scatter(PathS
             uint32_t flags = ps.flags[p];
             float4 tp = ps.tp[p];
             const float4 spec = ps.galb[p];
-            float4 Lcur = ps.L[p];
-            if (mis && lp > 0.0f && se != 0xFFFFFFFFu) {
-                // balance heuristic of the light sample against the
-                // directions this vertex samples (guide mixed with cosine):
-                // lp / (lp + p_mix) in place of the shade kernel's
-                // lp / (lp + cos / pi), applied to a contribution the shadow
-                // kernel found unoccluded
-                const float4* sr = ps.shadow + 3ull * se;
-                if (sr[0].w < 0.0f) {
-                    const float4 lb = sr[1];
-                    const f3 ld = mk3(lb.x, lb.y, lb.z);
-                    const float bp = fmaxf(dot(ff, ld), 0.0f) / kPi;
-                    const float pl = gv.mix * mixture_pdf(m, ld) + (1.0f - gv.mix) * bp;
-                    Lcur = add4(Lcur, mul4s(sr[2], (lp + bp) / (lp + pl) - 1.0f));
-                    ps.L[p] = Lcur;
-                }
-            }
+            const float4 Lcur = ps.L[p];
+            nee = guided_nee(sv, m, gv.mix, light_mix, mis, hp, ff, wl, tp, spec, rng);
+            shadow = nee.shadow;
             f3 w;
             if (rand01(rng) < gv.mix) {
-                const float xl = rand01(rng);
-                int k = 0;
-                float c = m.pi[0];
-#pragma unroll
-                for (int j = 1; j < kLobes; ++j)
-                    if (xl >= c) {
-                        k = j;
-                        c += m.pi[j];
-                    }
-                f3 mu = m.mu[0];
-                float kap = m.kappa[0];
-#pragma unroll
-                for (int j = 1; j < kLobes; ++j)
-                    if (k == j) {
-                        mu = m.mu[j];
-                        kap = m.kappa[j];
-                    }
-                const float x1 = rand01(rng), x2 = rand01(rng);
-                w = vmf_sample(mu, kap, x1, x2);
+                w = mixture_sample(m, rng);
             } else {
                 f3 sd = ff + random_unit_vector(rng);
                 if (length(sd) < 0.001f) sd = ff;
@@ -355,7 +431,8 @@ __global__ __launch_bounds__(64 * kScatterBlockWaves) void k_guide_scatter(PathS
             }
             const float cw = dot(ff, w);
             if (cw > 0.0f) {
-                const float pm = gv.mix * mixture_pdf(m, w) + (1.0f - gv.mix) * (cw / kPi);
+                const float pg = mixture_pdf(m, w);
+                const float pm = gv.mix * pg + (1.0f - gv.mix) * (cw / kPi);
                 tp = mul4(mul4s(spec, (cw / kPi) / pm), tp);
                 if (bounce < kRecBounces && gv.rec.in0) {
                     // kept only if a training step of its frame reads it
@@ -370,7 +447,8 @@ __global__ __launch_bounds__(64 * kScatterBlockWaves) void k_guide_scatter(PathS
                         gv.rec.in1[ri] = make_float4(s[4], s[5], s[6], pm);
                         gv.rec.dir[ri] = make_float4(w.x, w.y, w.z, cw);
                         gv.rec.tp[ri] = tp;
-                        gv.rec.Lb[ri] = Lcur;
+                        gv.rec.Lb[ri] = Lcur;  // before this vertex's light sample
+                        gv.rec.nee[ri] = shadow ? nee.contrib : splat4(0.0f);
                         flags |= kFlagRecorded << bounce;
                     }
                 }
@@ -385,14 +463,28 @@ __global__ __launch_bounds__(64 * kScatterBlockWaves) void k_guide_scatter(PathS
                     else tp = div4s(tp, pr);
                 }
                 if (alive) {
-                    // w: the mixture pdf of the direction, for the balance
-                    // weight of emission it reaches (WSRT_FIX_EMISSION_MIS)
+                    // for the balance weight of emission the direction
+                    // reaches (WSRT_FIX_EMISSION_MIS): d.w its scattering
+                    // density, o.w the light proposal's share of the light
+                    // technique's density there
                     ps.d[p] = f4(w, mis ? pm : 0.0f);
+                    ps.o[p] = f4(hp, mis ? (1.0f - light_mix) * pg : 0.0f);
                     ps.tp[p] = tp;
                     ps.rng[p] = rng;
                     ps.flags[p] = flags;
                 }
             }
+            if (!alive) {
+                ps.rng[p] = rng;
+                ps.flags[p] = flags;
+            }
+        }
+        const uint32_t se = seg_append(qs, bounce, kQShadow, qs.seg_cap, shadow);
+        if (shadow) {
+            float4* r = ps.shadow + 3ull * se;
+            r[0] = f4(hp, nee.dist - 0.001f);
+            r[1] = make_float4(nee.dir.x, nee.dir.y, nee.dir.z, __uint_as_float(p));
+            r[2] = nee.contrib;
         }
         if (ps.tag) {
             if (alive) ps.tag[p] = (uint8_t)(bounce + 1);
@@ -428,7 +520,11 @@ __global__ __launch_bounds__(256) void k_guide_prepare(PathState ps, GuideView g
         a = gv.rec.in0[ri];
         b = gv.rec.in1[ri];
         d = gv.rec.dir[ri];
-        const float4 T = gv.rec.tp[ri], Lb = gv.rec.Lb[ri], L = ps.L[p];
+        const float4 T = gv.rec.tp[ri], L = ps.L[p];
+        float4 Lb = gv.rec.Lb[ri];
+        // the vertex's own light sample is not incident radiance along the
+        // sampled direction
+        if (ps.flags[p] & (kFlagNeeVisible << bounce)) Lb = add4(Lb, gv.rec.nee[ri]);
         float sum = 0.0f;
         int cnt = 0;
         if (T.x > 0.0f) { sum += (L.x - Lb.x) / T.x; cnt++; }
@@ -736,9 +832,10 @@ __global__ __launch_bounds__(64 * kScatterWaves) void k_guide_infer(GuideView gv
 
 size_t guide_act_words(uint32_t ns) { return (size_t)kActRows * ns / 2; }  // in uint32 words
 
-void launch_guide_scatter(const PathState& ps, const Queues& qs, const GuideView& gv, const FrameArgs& fa,
-                          uint32_t bounce, uint32_t* next_queue, Counters* ctr, hipStream_t s) {
-    hipLaunchKernelGGL(k_guide_scatter, dim3(256), dim3(64 * kScatterBlockWaves), 0, s, ps, qs, gv, fa, bounce,
+void launch_guide_scatter(const SceneView& sv, const PathState& ps, const Queues& qs, const GuideView& gv,
+                          const FrameArgs& fa, uint32_t bounce, uint32_t* next_queue, Counters* ctr,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_guide_scatter, dim3(256), dim3(64 * kScatterBlockWaves), 0, s, sv, ps, qs, gv, fa, bounce,
                        next_queue, ctr);
 }
 

@@ -28,6 +28,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "common.hpp"
+
 namespace wsrt {
 namespace guide {
 
@@ -97,7 +99,7 @@ constexpr float kKappaScale = 8.0f, kLogitScale = 4.0f;
 
 // Training records of the Lambert vertices at bounces 0 .. kRecBounces-1 of
 // each path (structure of arrays, index bounce * capacity + path id).
-constexpr uint32_t kRecBounces = 4;
+constexpr uint32_t kRecBounces = kGuideRecordBounces;
 // Unsigned 32-bit division by a divisor fixed at set-up: q = (hi32(m n) + n)
 // >> l in 64 bits, with l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1
 // (Granlund and Montgomery 1994, round-up multiplier); exact for every n.
@@ -123,6 +125,7 @@ struct Records {
     float4* dir;   // sampled direction, cos to the normal
     float4* tp;    // throughput after the scatter (before roulette)
     float4* Lb;    // radiance accumulated before the scatter
+    float4* nee;   // the vertex's light-sample contribution (added if unoccluded)
     uint32_t npx;     // pixels of a frame
     uint32_t ns;      // records (training samples) per frame slot
     uint32_t stride;  // vertices between consecutive records

@@ -140,8 +140,12 @@ struct GuideTrain {
     float lr;
 };
 size_t guide_act_words(uint32_t ns);
-void launch_guide_scatter(const PathState& ps, const Queues& qs, const GuideView& gv, const FrameArgs& fa,
-                          uint32_t bounce, uint32_t* next_queue, Counters* ctr, hipStream_t s);
+void launch_guide_scatter(const SceneView& sv, const PathState& ps, const Queues& qs, const GuideView& gv,
+                          const FrameArgs& fa, uint32_t bounce, uint32_t* next_queue, Counters* ctr,
+                          hipStream_t s);
+// lights the guided light proposal tests one by one (more: uniform light
+// sampling only)
+constexpr uint32_t kGuideLightsMax = 64;
 // Writes the packed operands of params (after an upload or an initialisation).
 void launch_guide_pack(const GuideView& gv, float* params, hipStream_t s);
 // One step over ns samples (a multiple of 64 and of gt.chunk), the loss
@@ -205,7 +209,7 @@ struct PathLaunch {
     Queues qs;
     uint32_t guided;  // 1: Lambert vertices scatter through k_guide_scatter
     uint32_t extend_refill;  // > 0: k_extend_refill, refilling when this many lanes idle
-    uint32_t guide_mis;      // guided with WSRT_FIX_EMISSION_MIS: NEE weights use the guide's pdf
+    uint32_t guide_mis;      // guided: the shadow kernel flags the light samples it finds visible
     GuideView gv;
 };
 void launch_clear_queues(const Queues& qs, uint32_t depths, hipStream_t s);

@@ -331,6 +331,7 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
     const uint32_t base_frame = *fa.frame;
     const uint32_t rr_after = fa.table[base_frame & fa.mask].rr_after;
     const uint32_t quirks = fa.table[base_frame & fa.mask].quirks;
+    const float light_mix = fa.table[base_frame & fa.mask].guide_light_mix;
     const uint32_t photons_on = fa.table[base_frame & fa.mask].enable_photons;
     const uint32_t total = seg_prefix(qs, bounce, kQShade0, kMatBins, pre);
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -357,7 +358,7 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
             const uint32_t cell_base = photons_on ? slot * fs->hash_size : 0u;
             vo = shade_vertex(sv, grid, fs, cell_base, rr_after, quirks, bounce, guided != 0u,
                               __uint_as_float(hr.x), hr.y & 0x7FFFFFFFu, (hr.y >> 31) != 0u,
-                              __uint_as_float(hr.z), __uint_as_float(hr.w), r);
+                              __uint_as_float(hr.z), __uint_as_float(hr.w), r, light_mix);
             ps.L[p] = r.L;
             if (vo.deferred) {
                 ps.galb[p] = vo.galb;
@@ -393,8 +394,7 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
         const uint32_t ge = seg_append(qs, bounce, kQGuide, qs.seg_cap, deferred);
         if (deferred) {
             ps.guide_q[ge] = p;
-            // normal and the vertex's shadow record (none: all ones)
-            ps.gn[p] = f4(vo.ff, __uint_as_float(shadow ? se : 0xFFFFFFFFu));
+            ps.gn[p] = f4(vo.ff, 0.0f);
         }
     }
 }
@@ -438,12 +438,13 @@ __global__ __launch_bounds__(256) void k_order_active(PathState ps, Queues qs, u
     }
 }
 
-// mark_visible: an unoccluded record gets a negative tmax, which the guided
-// scatter reads to reweight the contribution the kernel added.
+// mark_nee (guided tracer): an unoccluded light sample of bounce b < 4 sets
+// the path's kFlagNeeVisible << b, which keeps its contribution out of the
+// training target of the vertex.
 __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(6) void k_shadow(SceneView sv, PathState ps, Queues qs,
                                                           uint32_t count_traversal,
                                                           Counters* __restrict__ ctr, uint32_t bounce,
-                                                          uint2* __restrict__ gstack, uint32_t mark_visible) {
+                                                          uint2* __restrict__ gstack, uint32_t mark_nee) {
     __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
     __shared__ uint32_t pre[kSegs + 1];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -461,7 +462,7 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(6) void k_shadow
     const bool count = count_traversal != 0u;
     uint32_t my_rays = 0, my_nodes = 0, my_tris = 0, my_max = 0;
     for (uint32_t i = gtid; i < n; i += stride) {
-        float4* r = ps.shadow + 3ull * seg_entry(pre, kSegs, i, qs.seg_cap);
+        const float4* r = ps.shadow + 3ull * seg_entry(pre, kSegs, i, qs.seg_cap);
         const float4 a = r[0], b = r[1];
         const uint32_t n_before = my_nodes;
         Hit h = trace_ray<true>(sv, xyz(a), xyz(b), 0.001f, a.w, st, count, my_nodes, my_tris);
@@ -470,7 +471,7 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(6) void k_shadow
         if (h.tri == kInvalid) {
             const uint32_t p = __float_as_uint(b.w);
             ps.L[p] = add4(ps.L[p], r[2]);
-            if (mark_visible) r[0].w = -1.0f;
+            if (mark_nee && bounce < kGuideRecordBounces) ps.flags[p] |= kFlagNeeVisible << bounce;
         }
     }
     if (count) flush_traversal_counts(ctr, kRkShadow, my_rays, my_nodes, my_tris, my_max);
@@ -500,6 +501,7 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(4) void k_tail(S
     const uint32_t base_frame = *fa.frame;
     const uint32_t rr_after = fa.table[base_frame & fa.mask].rr_after;
     const uint32_t quirks = fa.table[base_frame & fa.mask].quirks;
+    const float light_mix = fa.table[base_frame & fa.mask].guide_light_mix;
     const uint32_t photons_on = fa.table[base_frame & fa.mask].enable_photons;
     const bool count = count_traversal != 0u;
     RayTally tally;

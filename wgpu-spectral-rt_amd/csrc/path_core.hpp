@@ -81,7 +81,7 @@ __device__ __forceinline__ VertexOut shade_vertex(const SceneView& sv, const Gri
                                                   const FrameConst* fs, uint32_t cell_base,
                                                   uint32_t rr_after, uint32_t quirks, uint32_t bounce, bool guided,
                                                   float t, uint32_t tri, bool front, float hu, float hv,
-                                                  PathRegs& r) {
+                                                  PathRegs& r, float light_mix = 1.0f) {
     VertexOut out;
     out.alive = out.shadow = out.deferred = false;
     out.hp = out.sdir = out.ff = mk3(0, 0, 0);
@@ -103,8 +103,13 @@ __device__ __forceinline__ VertexOut shade_vertex(const SceneView& sv, const Gri
                 if ((quirks & WSRT_FIX_EMISSION_MIS) && r.d4.w > 0.0f) {
                     // balance heuristic against the light sample's pdf
                     const float cl = fmaxf(dot(ff, -d), 0.0001f);
+                    // the light technique's density: uniform light sampling
+                    // with probability light_mix, plus the guided light
+                    // proposal's share (1 - light_mix) p_guide, which the
+                    // guided scatter left in o.w (0 for plain paths)
                     const float lp = (t * t) / (world_tri_area(sv, tri) * cl * (float)sv.n_lights);
-                    r.L = add4(r.L, mul4s(mul4(em, r.tp), r.d4.w / (r.d4.w + lp)));
+                    const float lq = light_mix * lp + r.o4.w;
+                    r.L = add4(r.L, mul4s(mul4(em, r.tp), r.d4.w / (r.d4.w + lq)));
                 } else {
                     r.L = add4(r.L, mul4(em, r.tp));
                 }
@@ -125,6 +130,15 @@ __device__ __forceinline__ VertexOut shade_vertex(const SceneView& sv, const Gri
             float4 g = gather_photons(grid, *fs, cell_base, hp, ff, wl);
             r.L = add4(r.L, mul4(mul4(r.tp, div4s(spec, kPi)), g));
         }
+        if (guided) {
+            // next-event estimation and scattering happen later, in the
+            // guided scatter kernel, which has the vertex's mixture
+            out.deferred = true;
+            out.ff = ff;
+            out.galb = spec;
+            r.o4 = f4(hp, 0.0f);
+            return out;
+        }
         LightSample ls = sample_light(sv, hp, wl, r.rng);
         if (ls.pdf > 0.0f) {
             float ct = fmaxf(dot(ff, ls.dir), 0.0f);
@@ -135,15 +149,6 @@ __device__ __forceinline__ VertexOut shade_vertex(const SceneView& sv, const Gri
             out.shadow = true;
             out.sdir = ls.dir;
             out.stmax = ls.dist - 0.001f;
-        }
-        if (guided) {
-            // the scatter comes later; o.w keeps the light sample's pdf (0:
-            // none, or a point light) for the guide-aware NEE weight
-            out.deferred = true;
-            out.ff = ff;
-            out.galb = rgb_to_spectrum4(mk3(col.x, col.y, col.z), wl);
-            r.o4 = f4(hp, out.shadow && !ls.delta ? ls.pdf : 0.0f);
-            return out;
         }
     }
     Scatter sc = sample_bsdf(d, ff, m, front, wl, r.tp, r.rng, quirks);
