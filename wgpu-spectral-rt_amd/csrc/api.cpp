@@ -2348,7 +2348,7 @@ extern "C" int wsrt_autotune(wsrt_device* dev, const wsrt_scene* scene, uint32_t
     // ray costs within a wave differ widely (occluded interiors)
     // and path-ordered queues, which shorten shading where paths scatter
     // widely
-    const uint32_t refills[3] = {0u, 16u, 48u};
+    const uint32_t refills[3] = {0u, 16u, 32u};
     float best3 = best2;
     uint32_t bx = cfg->extend_refill, bo = cfg->ordered_queues;
     for (uint32_t o = 0; o < 2; ++o) {

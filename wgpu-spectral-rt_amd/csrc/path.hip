@@ -156,8 +156,15 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend
 // paths are independent.
 struct DoneBuf {
     uint32_t p[128];
-    uint32_t bin[128];
+    uint32_t tri[128];  // the hit triangle; its material bin is looked up when the group is binned
 };
+
+// Material bin of a hit triangle (one queue per material type).
+__device__ __forceinline__ uint32_t tri_bin(const SceneView& sv, uint32_t tri) {
+    const uint32_t mat = sv.instances[sv.tri_instance[tri]].material;
+    const uint32_t type = (uint32_t)sv.materials[mat].extra[0];
+    return type < (uint32_t)kMatBins ? type : 0u;
+}
 
 __device__ __forceinline__ void bin_paths(const Queues& qs, const PathState& ps, uint32_t bounce, bool have,
                                           uint32_t p, uint32_t bin) {
@@ -235,7 +242,7 @@ __device__ __forceinline__ void extend_refill_loop(const SceneView& sv, const Pa
         }
         iters++;
         bool to_bin = false;
-        uint32_t bin = 0;
+        uint32_t tri = 0;
         if (active && !trav_step<false, kCount>(sv, ts, st)) {
             active = false;
             my_nodes += ts.nvis;
@@ -247,33 +254,32 @@ __device__ __forceinline__ void extend_refill_loop(const SceneView& sv, const Pa
             } else {
                 ps.hit[p] = make_uint4(__float_as_uint(h.t), h.tri | (h.front ? 0x80000000u : 0u),
                                        __float_as_uint(h.u), __float_as_uint(h.v));
-                const uint32_t mat = sv.instances[sv.tri_instance[h.tri]].material;
-                const uint32_t type = (uint32_t)sv.materials[mat].extra[0];
-                bin = type < (uint32_t)kMatBins ? type : 0u;
+                tri = h.tri;
                 to_bin = true;
             }
         }
-        // finished rays wait in the wave's buffer; 64 are binned together
+        // finished rays wait in the wave's buffer; 64 are binned together,
+        // their material lookups made then, once per group
         const uint64_t tb = __ballot(to_bin);
         if (tb) {
             if (to_bin) {
                 const uint32_t slot = ndone + (uint32_t)__popcll(tb & lanemask_lt());
                 db.p[slot] = p;
-                db.bin[slot] = bin;
+                db.tri[slot] = tri;
             }
             ndone += (uint32_t)__popcll(tb);
             if (ndone >= 64u) {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();  // other lanes' entries are read below
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                uint32_t key = db.p[lane], val = db.bin[lane];
+                uint32_t key = db.p[lane], val = db.tri[lane];
                 wave_sort64(key, val);
-                bin_paths(qs, ps, bounce, true, key, val);
+                bin_paths(qs, ps, bounce, true, key, tri_bin(sv, val));
                 ndone -= 64u;
                 if (lane < ndone) {
-                    const uint32_t pp = db.p[64 + lane], bb = db.bin[64 + lane];
+                    const uint32_t pp = db.p[64 + lane], tt = db.tri[64 + lane];
                     db.p[lane] = pp;
-                    db.bin[lane] = bb;
+                    db.tri[lane] = tt;
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
@@ -285,9 +291,9 @@ __device__ __forceinline__ void extend_refill_loop(const SceneView& sv, const Pa
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    uint32_t key = lane < ndone ? db.p[lane] : 0xFFFFFFFFu, val = lane < ndone ? db.bin[lane] : 0u;
+    uint32_t key = lane < ndone ? db.p[lane] : 0xFFFFFFFFu, val = lane < ndone ? db.tri[lane] : 0u;
     wave_sort64(key, val);
-    bin_paths(qs, ps, bounce, lane < ndone, key, val);
+    bin_paths(qs, ps, bounce, lane < ndone, key, lane < ndone ? tri_bin(sv, val) : 0u);
     if (kCount) {
         // lane slots the wave spent in the traversal loop
         if (lane == 0 && iters) atomicAdd(&ctr->extend_wave_nodes, (unsigned long long)iters * 64ull);
