@@ -468,7 +468,7 @@ __global__ __launch_bounds__(64 * kScatterBlockWaves) void k_guide_scatter(Scene
                     // density, o.w the light proposal's share of the light
                     // technique's density there
                     ps.d[p] = f4(w, mis ? pm : 0.0f);
-                    ps.o[p] = f4(hp, mis ? (1.0f - light_mix) * pg : 0.0f);
+                    if (mis) ps.o[p] = f4(hp, (1.0f - light_mix) * pg);
                     ps.tp[p] = tp;
                     ps.rng[p] = rng;
                     ps.flags[p] = flags;
