@@ -134,6 +134,35 @@ __device__ __forceinline__ void trav_init(const SceneView& sv, TravState& s, f3 
     s.nvis = s.ntest = 0;
 }
 
+// Tests triangle `tri` against the lane's ray and keeps the closest hit
+// (ties to the lower id); returns true when an any-hit query has its hit.
+template <bool kAny, bool kCount>
+__device__ __forceinline__ bool test_tri(const SceneView& sv, TravState& s, uint32_t tri) {
+    const float4* tr = sv.tris + 3ull * tri;
+    const float4 a = tr[0], b = tr[1], cc = tr[2];
+    // the three loads complete together (one round trip to memory per
+    // triangle): without this the vertex load is issued only after the
+    // determinant test, a second dependent round trip
+    asm volatile("" ::"v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w), "v"(b.x), "v"(b.y), "v"(b.z), "v"(cc.x), "v"(cc.y),
+                 "v"(cc.z));
+    if (kCount) s.ntest++;
+    float tt, u, v;
+    bool fr;
+    if (!tri_intersect(s.o, s.d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), tt, u, v, fr))
+        return false;
+    if (!(tt > s.tmin)) return false;
+    const uint32_t id = __float_as_uint(a.w);
+    if (tt < s.best.t || (tt == s.best.t && s.best.tri != kInvalid && id < s.best.tri)) {
+        s.best.t = tt;
+        s.best.tri = id;
+        s.best.u = u;
+        s.best.v = v;
+        s.best.front = fr;
+        return kAny;
+    }
+    return false;
+}
+
 // One step of a traversal: pop the next node (skipping culled entries is a
 // step of its own) and visit it: box tests of its 8 children, triangle tests
 // of its leaf children, the hit internal children onto the stack. Returns
@@ -201,32 +230,40 @@ __device__ __forceinline__ bool trav_step(const SceneView& sv, TravState& s, Sta
     // slot classes: bit c of the node's imask byte is set when child c is
     // internal (meta bit 7), stored by the builder
     const uint32_t imask = w0.w >> 24;
-    // leaf children, in slot order
-    uint32_t leaves = hitmask & ~imask;
-    while (leaves) {
-        const int c = __ffs(leaves) - 1;
-        leaves &= leaves - 1;
-        const uint32_t meta = ((c < 4 ? w1.z : w1.w) >> (8 * (c & 3))) & 0xFFu;
-        const uint32_t cnt = meta >> 5, off = meta & 0x1Fu;
-        for (uint32_t k = 0; k < cnt; ++k) {
-            const float4* tr = sv.tris + 3ull * (tri_base + off + k);
-            float4 a = tr[0], b = tr[1], cc = tr[2];
-            if (kCount) s.ntest++;
-            float tt, u, v;
-            bool fr;
-            if (!tri_intersect(s.o, s.d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), tt, u, v,
-                               fr))
-                continue;
-            if (!(tt > s.tmin)) continue;
-            uint32_t id = __float_as_uint(a.w);
-            if (tt < s.best.t || (tt == s.best.t && s.best.tri != kInvalid && id < s.best.tri)) {
-                s.best.t = tt;
-                s.best.tri = id;
-                s.best.u = u;
-                s.best.v = v;
-                s.best.front = fr;
-                if (kAny) return false;
+    // Leaf children: a leaf holds at most 3 triangles, and the builder
+    // stores a node's leaf triangles contiguously from tri_base (at most 24),
+    // so the hit leaves' triangles form one mask over offsets 0..23 and a
+    // lane tests them in one flat loop: a wave then runs as many iterations
+    // as its lane with the most triangles, not, leaf by leaf, the most
+    // triangles any lane has in its next leaf. The closest hit (ties to the
+    // lower id) does not depend on the order of the tests.
+    const uint32_t leaves = hitmask & ~imask;
+    if constexpr (kAny) {
+        // any hit: leaf by leaf in slot order, ending at the first hit
+        uint32_t lv = leaves;
+        while (lv) {
+            const int c = __ffs(lv) - 1;
+            lv &= lv - 1;
+            const uint32_t meta = ((c < 4 ? w1.z : w1.w) >> (8 * (c & 3))) & 0xFFu;
+            const uint32_t cnt = meta >> 5, off = meta & 0x1Fu;
+            for (uint32_t k = 0; k < cnt; ++k) {
+                if (test_tri<kAny, kCount>(sv, s, tri_base + off + k)) return false;
             }
+        }
+    } else {
+        uint32_t tmask = 0;
+        if (leaves) {  // skipped when no lane of the wave has a leaf hit
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const uint32_t meta = byte_of(w1.z, w1.w, c);
+                const uint32_t bits = ((1u << (meta >> 5)) - 1u) << (meta & 0x1Fu);
+                tmask |= bits & (0u - ((leaves >> c) & 1u));
+            }
+        }
+        while (tmask) {
+            const int k = __ffs(tmask) - 1;
+            tmask &= tmask - 1;
+            (void)test_tri<kAny, kCount>(sv, s, tri_base + (uint32_t)k);
         }
     }
     // internal children in the order of the ray's octant over the slots:
@@ -250,16 +287,26 @@ __device__ __forceinline__ bool trav_step(const SceneView& sv, TravState& s, Sta
         // with lower ranks nearer the top
         const int first = (__ffs(pm) - 1) ^ oct;
         const uint32_t rest = pm & (pm - 1u);
+        const uint32_t stacked = push & ~(1u << first);
+        s.next = child_base + (((first < 4 ? w1.z : w1.w) >> (8 * (first & 3))) & 0x7Fu);
+        if (s.sp + __popc(rest) <= kShortStack) {
+            // every entry lands in the lane's LDS part (the common case)
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            if (!((push >> c) & 1u)) continue;
-            const uint32_t meta = byte_of(w1.z, w1.w, c);
-            if (c == first) {
-                s.next = child_base + (meta & 0x7Fu);
-                continue;
+            for (int c = 0; c < 8; ++c) {
+                if (!((stacked >> c) & 1u)) continue;
+                const int pos = s.sp + __popc(rest >> ((c ^ oct) + 1));
+                u32x2 e;
+                e.x = child_base + (byte_of(w1.z, w1.w, c) & 0x7Fu);
+                e.y = __float_as_uint(tnear[c]);
+                st.lds[pos * st.lds_stride] = e;
             }
-            const int pos = s.sp + __popc(rest >> ((c ^ oct) + 1));
-            st.put(pos, make_uint2(child_base + (meta & 0x7Fu), __float_as_uint(tnear[c])));
+        } else {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                if (!((stacked >> c) & 1u)) continue;
+                const int pos = s.sp + __popc(rest >> ((c ^ oct) + 1));
+                st.put(pos, make_uint2(child_base + (byte_of(w1.z, w1.w, c) & 0x7Fu), __float_as_uint(tnear[c])));
+            }
         }
         s.sp += __popc(rest);
         s.have_next = true;
