@@ -556,23 +556,30 @@ int or_scene_build(or_scene* s) {
 typedef struct { float t, u, v; uint32_t tri; int front; } hit_t;
 #define NO_HIT 0xFFFFFFFFu
 
+/* cross and dot with fused multiply-adds, each component rounded once per
+ * fused step (the HIP path's triangle test uses the same forms and order) */
+static inline v3 cross3_fma(v3 a, v3 b) {
+    return V3(fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)), fmaf(a.x, b.y, -(a.y * b.x)));
+}
+static inline float dot3_fma(v3 a, v3 b) { return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z)); }
+
 static inline int tri_intersect(v3 o, v3 d, v3 v0, v3 e1, v3 e2, float* t, float* u, float* v,
                                 int* front) {
     /* Moller-Trumbore; the barycentric tests on the unscaled numerators
      * with det's sign normalised, the division only for a triangle that
      * passes them (the HIP path's order of operations) */
-    v3 pvec = cross3(d, e2);
-    float det = dot3(e1, pvec);
+    v3 pvec = cross3_fma(d, e2);
+    float det = dot3_fma(e1, pvec);
     if (det == 0.0f) return 0;
     float sg = det > 0.0f ? 1.0f : -1.0f, adet = det * sg;
     v3 tvec = sub3(o, v0);
-    float U = dot3(tvec, pvec);
+    float U = dot3_fma(tvec, pvec);
     if (U * sg < 0.0f || U * sg > adet) return 0;
-    v3 qvec = cross3(tvec, e1);
-    float V = dot3(d, qvec);
+    v3 qvec = cross3_fma(tvec, e1);
+    float V = dot3_fma(d, qvec);
     if (V * sg < 0.0f || (U + V) * sg > adet) return 0;
     float inv = 1.0f / det;
-    *t = dot3(e2, qvec) * inv;
+    *t = dot3_fma(e2, qvec) * inv;
     *u = U * inv; *v = V * inv; *front = det > 0.0f;
     return 1;
 }

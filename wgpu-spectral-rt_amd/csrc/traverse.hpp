@@ -35,22 +35,32 @@ __device__ __forceinline__ float safe_inv(float d) {
     return 1.0f / d;
 }
 
+// Cross and dot products of the triangle test with fused multiply-adds:
+// each component is one product plus one fused product (cross) or one
+// product and two fused ones (dot), each rounded once. The oracle's triangle
+// test uses the same fused forms in the same order.
+__device__ __forceinline__ f3 cross_fma(f3 a, f3 b) {
+    return mk3(__fmaf_rn(a.y, b.z, -(a.z * b.y)), __fmaf_rn(a.z, b.x, -(a.x * b.z)),
+               __fmaf_rn(a.x, b.y, -(a.y * b.x)));
+}
+__device__ __forceinline__ float dot_fma(f3 a, f3 b) { return __fmaf_rn(a.x, b.x, __fmaf_rn(a.y, b.y, a.z * b.z)); }
+
 // Moller-Trumbore with the barycentric tests on the unscaled numerators
 // (signs normalised by det's), so only a triangle that passes them divides.
 __device__ __forceinline__ bool tri_intersect(f3 o, f3 d, f3 v0, f3 e1, f3 e2, float& t, float& u,
                                               float& v, bool& front) {
-    f3 pvec = cross(d, e2);
-    float det = dot(e1, pvec);
+    f3 pvec = cross_fma(d, e2);
+    float det = dot_fma(e1, pvec);
     if (det == 0.0f) return false;
     const float sg = det > 0.0f ? 1.0f : -1.0f, adet = det * sg;
     f3 tvec = o - v0;
-    float U = dot(tvec, pvec);
+    float U = dot_fma(tvec, pvec);
     if (U * sg < 0.0f || U * sg > adet) return false;
-    f3 qvec = cross(tvec, e1);
-    float V = dot(d, qvec);
+    f3 qvec = cross_fma(tvec, e1);
+    float V = dot_fma(d, qvec);
     if (V * sg < 0.0f || (U + V) * sg > adet) return false;
     float inv = 1.0f / det;
-    t = dot(e2, qvec) * inv;
+    t = dot_fma(e2, qvec) * inv;
     u = U * inv;
     v = V * inv;
     front = det > 0.0f;
