@@ -237,14 +237,17 @@ def summarize_kernels(trace_dir: str, first_batch: int, n_batches: int, frames_p
         for row in csv.DictReader(f):
             rows.append((int(row["Start_Timestamp"]), int(row["End_Timestamp"]), _short(row["Kernel_Name"])))
     rows.sort()
+    if first_batch < 0:  # the statistics only (a run of many renderers)
+        rows = []
     starts = [i for i, r in enumerate(rows) if r[2] == "k_clear_queues"]
-    lo = starts[first_batch]
-    hi = starts[first_batch + n_batches] if first_batch + n_batches < len(starts) else len(rows)
-    sel = rows[lo:hi]
     per = collections.defaultdict(list)
-    for a, b, k in sel:
-        per[k].append((b - a) / 1000.0)
-    span = (max(b for _, b, _ in sel) - sel[0][0]) / 1000.0
+    if rows:
+        lo = starts[first_batch]
+        hi = starts[first_batch + n_batches] if first_batch + n_batches < len(starts) else len(rows)
+        sel = rows[lo:hi]
+        for a, b, k in sel:
+            per[k].append((b - a) / 1000.0)
+        span = (max(b for _, b, _ in sel) - sel[0][0]) / 1000.0
     with open(out, "w") as f:
         f.write(f"# rocprofv3 kernel statistics, {PROFILE_WORKLOADS[PROFILE_TAG]}\n")
         f.write(f"# command: {command}\n")
@@ -254,16 +257,17 @@ def summarize_kernels(trace_dir: str, first_batch: int, n_batches: int, frames_p
         for row in rd:
             f.write(f"{_short(row['Name'])[:34]:34s} {int(row['Calls']):7d} {float(row['TotalDurationNs']) / 1e3:11.1f} "
                     f"{float(row['AverageNs']) / 1e3:9.2f} {float(row['Percentage']):6.2f}\n")
-        f.write(f"\n# batches {first_batch}..{first_batch + n_batches - 1} of the run (the timed region), "
-                "per kernel:\n")
-        for k, ts in sorted(per.items(), key=lambda kv: -sum(kv[1])):
-            f.write(f"#   {k[:28]:28s} launches {len(ts):4d}  total {sum(ts):10.1f} us  "
-                    f"avg {sum(ts) / len(ts):9.2f} us\n")
-        busy = sum(sum(ts) for ts in per.values())
-        frames = n_batches * frames_per_batch
-        f.write(f"#   span of the batches {span:.1f} us ({span / frames:.2f} us per frame under the tracer)\n")
-        f.write(f"#   kernel time of the batches {busy:.1f} us ({busy / frames:.2f} us per frame, "
-                f"{frames} frames)\n")
+        if rows:
+            f.write(f"\n# batches {first_batch}..{first_batch + n_batches - 1} of the run (the timed region), "
+                    "per kernel:\n")
+            for k, ts in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+                f.write(f"#   {k[:28]:28s} launches {len(ts):4d}  total {sum(ts):10.1f} us  "
+                        f"avg {sum(ts) / len(ts):9.2f} us\n")
+            busy = sum(sum(ts) for ts in per.values())
+            frames = n_batches * frames_per_batch
+            f.write(f"#   span of the batches {span:.1f} us ({span / frames:.2f} us per frame under the tracer)\n")
+            f.write(f"#   kernel time of the batches {busy:.1f} us ({busy / frames:.2f} us per frame, "
+                    f"{frames} frames)\n")
     print(open(out).read())
     return 0
 
