@@ -45,6 +45,11 @@ __global__ void k_set_frame(uint32_t* frame, uint32_t value) {
 }
 
 // Path p of a batch of frames belongs to frame slot p / npx, pixel p % npx.
+// The camera ray, radiance (0), wavelengths and flags are stored for the
+// kernels that read them at bounce 0 (extend, the guide's record selection)
+// or at the end (accumulate); the throughput (1) and the RNG state after the
+// camera's draws are not: the shade of bounce 0 recomputes the whole camera
+// path from its pixel and frame instead of reading it back (camera_regs).
 __global__ __launch_bounds__(256) void k_generate(PathState ps, FrameArgs fa, uint32_t npx, uint32_t nf) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npx * nf) return;
@@ -54,12 +59,25 @@ __global__ __launch_bounds__(256) void k_generate(PathState ps, FrameArgs fa, ui
     camera_path(fc, pix % fc.width, pix / fc.width, r);
     ps.o[p] = r.o4;
     ps.d[p] = r.d4;
-    ps.tp[p] = r.tp;
     ps.L[p] = r.L;
     ps.wl[p] = r.wl;
-    ps.rng[p] = r.rng;
     ps.flags[p] = r.flags;
     if (ps.tag) ps.tag[p] = 0;  // tags of an earlier batch are stale
+}
+
+// The state of camera path p before its first vertex, as k_generate made it.
+__device__ __forceinline__ void camera_regs(const FrameArgs& fa, uint32_t base_frame, uint32_t npx, uint32_t p,
+                                            PathRegs& r) {
+    const uint32_t slot = p / npx, pix = p - slot * npx;
+    const FrameConst& fc = fa.table[(base_frame + slot) & fa.mask];
+    camera_path(fc, pix % fc.width, pix / fc.width, r);
+}
+
+// Radiance of path p after its ray escapes to the environment (radiance
+// env along it): L + tp env, where a camera ray (bounce 0) has L = 0 and
+// tp = 1, so its result is env itself and tp is not read.
+__device__ __forceinline__ float4 env_gain(const PathState& ps, uint32_t p, uint32_t bounce, float4 env) {
+    return bounce == 0 ? env : add4(ps.L[p], mul4(ps.tp[p], env));
 }
 
 // Bounce 0 reads path ids 0 .. n0-1 directly (every path starts); later
@@ -112,7 +130,7 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend
             my_rays++;
             if (h.tri == kInvalid) {
                 have = false;  // miss: the path ends (shader.wgsl:566)
-                if (sv.env) ps.L[p] = add4(ps.L[p], mul4(ps.tp[p], env_radiance(sv, xyz(d), ps.wl[p])));
+                if (sv.env) ps.L[p] = env_gain(ps, p, bounce, env_radiance(sv, xyz(d), ps.wl[p]));
             } else {
                 ps.hit[p] = make_uint4(__float_as_uint(h.t), h.tri | (h.front ? 0x80000000u : 0u),
                                        __float_as_uint(h.u), __float_as_uint(h.v));
@@ -250,7 +268,7 @@ __device__ __forceinline__ void extend_refill_loop(const SceneView& sv, const Pa
             if (kCount) my_max = max(my_max, ts.nvis);
             const Hit& h = ts.best;
             if (h.tri == kInvalid) {  // miss: the path ends (shader.wgsl:566)
-                if (sv.env) ps.L[p] = add4(ps.L[p], mul4(ps.tp[p], env_radiance(sv, ts.d, ps.wl[p])));
+                if (sv.env) ps.L[p] = env_gain(ps, p, bounce, env_radiance(sv, ts.d, ps.wl[p]));
             } else {
                 ps.hit[p] = make_uint4(__float_as_uint(h.t), h.tri | (h.front ? 0x80000000u : 0u),
                                        __float_as_uint(h.u), __float_as_uint(h.v));
@@ -352,13 +370,19 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
             p = ps.shade_q[seg_entry(pre, kMatBins * kSegs, i, qs.seg_cap)];
             const uint4 hr = ps.hit[p];
             PathRegs r;
-            r.o4 = ps.o[p];
-            r.d4 = ps.d[p];
-            r.tp = ps.tp[p];
-            r.L = ps.L[p];
-            r.wl = ps.wl[p];
-            r.rng = ps.rng[p];
-            r.flags = ps.flags[p];
+            if (bounce == 0) {
+                // a camera path: recomputed from its pixel and frame, 88
+                // bytes of path state a lane does not read back
+                camera_regs(fa, base_frame, npx, p, r);
+            } else {
+                r.o4 = ps.o[p];
+                r.d4 = ps.d[p];
+                r.tp = ps.tp[p];
+                r.L = ps.L[p];
+                r.wl = ps.wl[p];
+                r.rng = ps.rng[p];
+                r.flags = ps.flags[p];
+            }
             const uint32_t slot = p / npx;
             const FrameConst* fs = photons_on ? &fa.table[(base_frame + slot) & fa.mask] : nullptr;
             const uint32_t cell_base = photons_on ? slot * fs->hash_size : 0u;
@@ -371,6 +395,7 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState
                 ps.o[p] = r.o4;
                 ps.rng[p] = r.rng;
                 ps.flags[p] = r.flags;
+                if (bounce == 0) ps.tp[p] = r.tp;  // the guided scatter reads it; k_generate does not store it
             }
             if (vo.alive) {
                 ps.o[p] = r.o4;
