@@ -645,8 +645,9 @@ def batch_throughput(n_scenes: int = 64) -> dict:
     for photons in (0, 1):
         # without the photon pass: every sample of every job in one persistent
         # launch; with it (the reference's frame: photon pass, grid, camera
-        # paths): each frame every job's pass of 2^20 photon paths and grid,
-        # then one persistent launch, one captured graph per frame
+        # paths): per window of frames every job's passes of 2^20 photon
+        # paths per frame and their grids, then one persistent launch over
+        # the window's frames, one captured graph per window
         cfg = wsrt.default_config(enable_photons=photons, max_depth=8)
         batch = wsrt.Batch(dev, jobs, cfg)
         tune = batch.autotune()

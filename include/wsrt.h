@@ -504,8 +504,10 @@ int wsrt_autotune(wsrt_device* dev, const wsrt_scene* scene, uint32_t width, uin
  * persistent launch pulling 64-pixel work items from a device queue. Each
  * pixel's samples are traced in frame order with the renderer's per-path
  * code, so a job's accumulation equals a wsrt_renderer's with the same
- * scene, camera, size and frame count (photon pass off: the config must have
- * enable_photons = 0 and guiding = 0). Every render restarts at frame 0. */
+ * scene, camera, size and frame count (guiding must be 0). With
+ * enable_photons, each window of frames first runs every job's photon pass
+ * and grid build for its frames, then one persistent launch traces them.
+ * Every render restarts at frame 0. */
 typedef struct wsrt_batch wsrt_batch;
 typedef struct wsrt_batch_job {
     const wsrt_scene* scene;

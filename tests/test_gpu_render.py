@@ -435,8 +435,9 @@ def test_batch_rejects_guiding(device, cornell_pair):
 
 @pytest.mark.parametrize("graphs", [1, 0])
 def test_batch_with_photons_equals_single_renderers(device, graphs):
-    """With the photon pass on, a batch renders frame by frame (each job's
-    photon pass and grid, then one persistent launch over all pixels): each
+    """With the photon pass on, a batch renders window by window (each job's
+    photon passes and grids for the window's frames, then one persistent
+    launch over all pixels of those frames): each
     job's image equals its own renderer's, photon gather included, bit for
     bit, with and without per-frame graphs and after autotuning."""
     print(f"batch with photons, graphs {graphs}", flush=True)

@@ -204,8 +204,8 @@ class _Materials:
 def test_batch_of_procedural_scenes(device, photons):
     """Scenes 0..7 of the batch workload (1.5 M triangles at most) at their
     own resolutions and sample counts in one persistent launch (with the
-    photon pass: 2^20 photon paths per job and frame, then one persistent
-    launch per frame)."""
+    photon pass: 2^20 photon paths per job and frame, run for a window of
+    frames at once, then one persistent launch per window)."""
     print(f"batch: 8 procedural scenes, photons {photons}", flush=True)
     cfg = wsrt.default_config(enable_photons=photons, max_depth=8)
     jobs, refs, types, tris = [], [], set(), []

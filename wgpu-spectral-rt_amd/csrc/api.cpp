@@ -2397,6 +2397,7 @@ struct wsrt_batch {
     // traced, and the trace buffers the jobs' passes share (one job at a time)
     bool photons = false;
     uint32_t max_spp = 0;
+    uint32_t photon_frames = 1;  // frames per window (batch_alloc_photons)
     std::vector<DevBuf> gcount, goffset, gsorted, gtouched, gnvalid;
     DevBuf ph, ph_stored, g_cell, g_slot, g_temp, qcounts, ctr, pgstack, dframe;
     DevBuf ph_id[2], ph_pos[2], ph_dir[2], ph_rng[2], ph_flags[2];
@@ -2410,6 +2411,16 @@ struct wsrt_batch {
     int wall_khz = 100000;
 };
 
+// Most frames of every job one photon pass and one persistent launch cover
+// in photon mode (the window). Each job's pass and grid build is about 15
+// small launches, so a window of W frames takes 1/W of the per-frame
+// launches: on the 64-scene batch at 2^20 photons per frame, 339 Msamples/s
+// frame by frame, 634 with windows of 4, 725 of 8, 791 of 16. The window
+// holds one grid per frame and job (68 MB per job-frame at 2^20 photons and
+// 2^22 cells), so batch_alloc_photons picks the largest window up to this
+// whose grids fit in a quarter of the free device memory.
+constexpr uint32_t kBatchPhotonFramesMax = 16;
+
 static void batch_drop_graph(wsrt_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -2422,20 +2433,22 @@ static void batch_drop_graph(wsrt_batch* b) {
     b->frame_graph.clear();
 }
 
-// Frame f of every job that renders it: each job's photon pass and grid
-// build for that frame (photon_emit.wgsl, build_grid.wgsl: the renderer's
-// kernels with one frame in flight), then one persistent launch tracing
-// frame f of all jobs' pixels.
-static void batch_enqueue_frame(wsrt_batch* b, uint32_t f, const BatchArgs& base, uint32_t blocks) {
+// Frames [f0, f0 + photon_frames) of every job that renders them: each
+// job's photon pass and grid build for those frames at once (photon_emit.wgsl,
+// build_grid.wgsl: the renderer's kernels with that many frames in flight,
+// one grid per frame), then one persistent launch tracing the window's frames
+// of all jobs' pixels, each pixel's samples in frame order.
+static void batch_enqueue_frames(wsrt_batch* b, uint32_t f0, const BatchArgs& base, uint32_t blocks) {
     hipStream_t s = b->stream;
     const uint32_t np = b->cfg.photons_per_pass;
     for (size_t j = 0; j < b->jobs.size(); ++j) {
         const BatchJob& B = b->hjobs[j];
-        if (f >= B.frames) continue;
+        if (f0 >= B.frames) continue;
+        const uint32_t nf = std::min(b->photon_frames, B.frames - f0);
         const FrameArgs fa{b->table.as<FrameConst>() + B.table_base, b->dframe.as<uint32_t>(), 0xFFFFFFFFu};
-        launch_set_frame(b->dframe.as<uint32_t>(), f, s);
+        launch_set_frame(b->dframe.as<uint32_t>(), f0, s);
         launch_clear_queues(b->qs, b->cfg.photon_max_depth + 1, s);
-        launch_photon_emit(B.sv, fa, np, 1, b->qs, b->phs, 0u, b->ctr.as<Counters>(), s);
+        launch_photon_emit(B.sv, fa, np, nf, b->qs, b->phs, 0u, b->ctr.as<Counters>(), s);
         for (uint32_t d = 0; d < b->cfg.photon_max_depth; ++d)
             launch_photon_bounce(B.sv, d, 0u, b->phs, b->qs, b->ph.as<wsrt_photon>(), b->ctr.as<Counters>(),
                                  b->pgstack.as<uint2>(), s);
@@ -2448,34 +2461,54 @@ static void batch_enqueue_frame(wsrt_batch* b, uint32_t f, const BatchArgs& base
         g.n_valid = b->gnvalid[j].as<uint32_t>();
         g.temp = b->g_temp.as<uint32_t>();
         g.touched = b->gtouched[j].as<uint32_t>();
-        grid_build_stored(b->ph.as<wsrt_photon>(), b->phs.stored, np, 1, b->cfg.hash_size, fa, g, s);
+        grid_build_stored(b->ph.as<wsrt_photon>(), b->phs.stored, np, nf, b->cfg.hash_size, fa, g, s);
     }
     BatchArgs a = base;
     a.one_frame = 1u;
-    a.frame = f;
-    launch_batch(a, blocks, s, f == 0);
+    a.frame = f0;
+    a.frame_span = b->photon_frames;
+    launch_batch(a, blocks, s, f0 == 0);
 }
 
-// The photon pass's buffers: per job, a grid (counts, offsets, photons in
-// cell order, the cells the last build touched); shared, the photon trace
-// state of one pass and the grid build's work memory.
+// The photon pass's buffers: per job, the grids of a window of
+// photon_frames frames (counts, offsets, photons in cell order, the cells the
+// last build touched); shared, the photon trace state of one pass over a
+// window and the grid build's work memory.
 static int batch_alloc_photons(wsrt_batch* b, uint32_t max_stack) {
     const wsrt_config& c = b->cfg;
-    const uint32_t np = c.photons_per_pass ? c.photons_per_pass : 1u;
+    const uint32_t np1 = c.photons_per_pass ? c.photons_per_pass : 1u;
     const size_t nj = b->jobs.size();
+    {
+        // the window: the most frames (up to kBatchPhotonFramesMax, within
+        // 2^30 photons and cells) whose grids take at most a quarter of the
+        // free device memory (or of what a memory cap leaves)
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+        const uint64_t cap = g_mem_limit[b->dev->ordinal].load(), used = g_mem_used[b->dev->ordinal].load();
+        if (cap) free_b = std::min<uint64_t>(free_b, cap > used ? cap - used : 0u);
+        const uint64_t per_frame =
+            (uint64_t)nj * (8ull * c.hash_size + (sizeof(wsrt_photon) + 4ull) * np1) + 160ull * np1;
+        uint32_t w = kBatchPhotonFramesMax;
+        while (w > 1 && ((uint64_t)w * per_frame > free_b / 4 || (uint64_t)np1 * w > (1ull << 30) ||
+                         (uint64_t)c.hash_size * w > (1ull << 30)))
+            w >>= 1;
+        b->photon_frames = w;
+    }
+    const uint32_t np = np1 * b->photon_frames;          // photon paths of one window
+    const uint32_t cells = c.hash_size * b->photon_frames;  // grid cells of one window
     b->gcount.resize(nj);
     b->goffset.resize(nj);
     b->gsorted.resize(nj);
     b->gtouched.resize(nj);
     b->gnvalid.resize(nj);
     for (size_t j = 0; j < nj; ++j) {
-        HIP_OR_FAIL(b->gcount[j].alloc(4ull * c.hash_size));
-        HIP_OR_FAIL(b->goffset[j].alloc(4ull * c.hash_size));
+        HIP_OR_FAIL(b->gcount[j].alloc(4ull * cells));
+        HIP_OR_FAIL(b->goffset[j].alloc(4ull * cells));
         HIP_OR_FAIL(b->gsorted[j].alloc(sizeof(wsrt_photon) * np));
         HIP_OR_FAIL(b->gtouched[j].alloc(4ull * np));
         HIP_OR_FAIL(b->gnvalid[j].alloc(4));
-        HIP_OR_FAIL(hipMemsetAsync(b->gcount[j].p, 0, 4ull * c.hash_size, b->stream));
-        HIP_OR_FAIL(hipMemsetAsync(b->goffset[j].p, 0, 4ull * c.hash_size, b->stream));
+        HIP_OR_FAIL(hipMemsetAsync(b->gcount[j].p, 0, 4ull * cells, b->stream));
+        HIP_OR_FAIL(hipMemsetAsync(b->goffset[j].p, 0, 4ull * cells, b->stream));
         HIP_OR_FAIL(hipMemsetAsync(b->gnvalid[j].p, 0, 4, b->stream));
         BatchJob& B = b->hjobs[j];
         B.grid = GridView{b->gcount[j].as<uint32_t>(), b->goffset[j].as<uint32_t>(),
@@ -2485,7 +2518,7 @@ static int batch_alloc_photons(wsrt_batch* b, uint32_t max_stack) {
     HIP_OR_FAIL(b->ph_stored.alloc(4ull * ((np + 31) / 32)));
     HIP_OR_FAIL(b->g_cell.alloc(4ull * np));
     HIP_OR_FAIL(b->g_slot.alloc(4ull * np));
-    HIP_OR_FAIL(b->g_temp.alloc(4 * grid_temp_words(np, c.hash_size)));
+    HIP_OR_FAIL(b->g_temp.alloc(4 * grid_temp_words(np, cells)));
     const uint64_t pq = (uint64_t)kSegs * seg_capacity(np);
     for (int k = 0; k < 2; ++k) {
         HIP_OR_FAIL(b->ph_id[k].alloc(4 * pq));
@@ -2634,6 +2667,7 @@ extern "C" int wsrt_batch_create(wsrt_device* dev, const wsrt_batch_job* jobs, u
     b->args.max_depth = c.max_depth;
     b->args.one_frame = 0;
     b->args.frame = 0;
+    b->args.frame_span = 1;
     if (b->abort_buf.alloc(16) != hipSuccess || hipMemsetAsync(b->abort_buf.p, 0, 16, b->stream) != hipSuccess)
         return bail(fail(WSRT_E_OOM, "batch abort counter"));
     void* stop_dev = nullptr;
@@ -2674,14 +2708,17 @@ extern "C" int wsrt_batch_render(wsrt_batch* b) {
     for (size_t j = 0; j < b->jobs.size(); ++j)
         HIP_OR_FAIL(hipMemsetAsync(b->accum[j].p, 0, b->accum[j].bytes, b->stream));
     if (b->photons) {
-        // frame by frame: every job's photon pass, then one persistent
-        // launch over all jobs' pixels; each frame one captured graph
-        if (b->frame_graph.empty()) b->frame_graph.assign(b->max_spp, {nullptr, nullptr});
-        for (uint32_t f = 0; f < b->max_spp; ++f) {
-            auto& fg = b->frame_graph[f];
+        // window by window of photon_frames frames: every job's photon
+        // pass and grids, then one persistent launch over all jobs' pixels;
+        // each window one captured graph
+        const uint32_t windows = (b->max_spp + b->photon_frames - 1) / b->photon_frames;
+        if (b->frame_graph.empty()) b->frame_graph.assign(windows, {nullptr, nullptr});
+        for (uint32_t w = 0; w < windows; ++w) {
+            const uint32_t f = w * b->photon_frames;
+            auto& fg = b->frame_graph[w];
             if (!fg.second && b->cfg.frame_graphs &&
                 hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-                batch_enqueue_frame(b, f, b->args, blocks);
+                batch_enqueue_frames(b, f, b->args, blocks);
                 if (hipStreamEndCapture(b->stream, &fg.first) != hipSuccess ||
                     hipGraphInstantiate(&fg.second, fg.first, nullptr, nullptr, 0) != hipSuccess) {
                     if (fg.first) (void)hipGraphDestroy(fg.first);
@@ -2690,7 +2727,7 @@ extern "C" int wsrt_batch_render(wsrt_batch* b) {
                 (void)hipGetLastError();
             }
             if (fg.second) HIP_OR_FAIL(hipGraphLaunch(fg.second, b->stream));
-            else batch_enqueue_frame(b, f, b->args, blocks);
+            else batch_enqueue_frames(b, f, b->args, blocks);
         }
         HIP_OR_FAIL(hipGetLastError());
         return WSRT_OK;
@@ -2758,10 +2795,11 @@ extern "C" int wsrt_batch_autotune(wsrt_batch* b, wsrt_batch_tuning* best, float
             a.time_limit = 0;
             const uint32_t blocks = (uint32_t)b->cus * bpc;
             if (b->photons) {
-                // frame 0's grids, then frame 0 of every job
-                batch_enqueue_frame(b, 0, a, blocks);
+                // the first window's grids, then frame 0 of every job
+                batch_enqueue_frames(b, 0, a, blocks);
                 a.one_frame = 1u;
                 a.frame = 0u;
+                a.frame_span = 1u;
             }
             launch_batch(a, blocks, b->stream, true);  // warm
             (void)hipEventRecord(e0, b->stream);

@@ -8,8 +8,9 @@
 // Each lane traces its pixel's samples one after the other with the same
 // per-path steps as the wavefront renderer (path_core.hpp), accumulating in
 // frame order, so a job's image equals the wavefront renderer's bit for bit.
-// With the photon pass, the launch traces one frame of every job, after the
-// host has run each job's photon pass and grid build for that frame.
+// With the photon pass, the launch traces a window of frames of every job,
+// after the host has run each job's photon pass and grid build for those
+// frames (one grid per frame of the window, cell_base apart).
 #include "kernels.hpp"
 #include "path_core.hpp"
 #include "traverse.hpp"
@@ -80,14 +81,16 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
         const uint32_t pix = y * J.width + x;
         const SceneView& sv = J.sv;
         float4 acc = J.accum[pix];
-        const uint32_t f1 = a.one_frame ? a.frame + 1u : J.frames;
+        const uint32_t f1 = a.one_frame ? min(a.frame + a.frame_span, J.frames) : J.frames;
         for (uint32_t f = a.one_frame ? a.frame : 0u; f < f1; ++f) {
             const FrameConst& fc = a.table[J.table_base + f];
             PathRegs r;
             camera_path(fc, x, y, r);
-            // the photon gather reads the job's grid, built for this frame
+            // the photon gather reads the job's grid built for this frame,
+            // the (f - frame)-th of the window's
             const FrameConst* fs = (fc.enable_photons && J.grid.count) ? &fc : nullptr;
-            trace_bounces(sv, J.grid, fs, 0u, fc.rr_after, fc.quirks, 0u, a.max_depth, r, st, false, tally);
+            const uint32_t cell_base = a.one_frame ? (f - a.frame) * fc.hash_size : 0u;
+            trace_bounces(sv, J.grid, fs, cell_base, fc.rr_after, fc.quirks, 0u, a.max_depth, r, st, false, tally);
             acc = accumulate_sample(acc, fc, r.wl, r.L);
         }
         J.accum[pix] = acc;

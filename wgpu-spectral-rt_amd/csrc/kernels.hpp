@@ -182,8 +182,9 @@ struct BatchArgs {
     uint32_t spill;        // spill entries per lane
     uint32_t max_depth;
     uint32_t tile_mode;    // 0: 64 x 1 pixel items, 1: 8 x 8 pixel items
-    uint32_t one_frame;    // 1: trace frame `frame` only (jobs with more samples than that)
+    uint32_t one_frame;    // 1: trace frames [frame, frame + frame_span) only (those a job has)
     uint32_t frame;
+    uint32_t frame_span;   // frames of the window; the job's grid holds one per frame, in order
     // stopping a render early: a host-written flag in pinned memory (null:
     // none) and a wall-clock budget from the render's start (0: none). A
     // wave that takes an item after either fires drops it and counts it in
