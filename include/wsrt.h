@@ -577,12 +577,17 @@ int wsrt_renderer_get_guide(wsrt_renderer* r, wsrt_guide** out);
  * reference's only state is the accumulation and frame_count, which
  * wsrt_renderer_read/write_accum and get/set_frame_count cover). The state
  * is the parameters, both Adam moments (WSRT_GUIDE_PARAMS floats each) and
- * the Adam step; set_state restores it exactly. With the frames a guiding
- * renderer has trained on (guide_train_frames counts them), a renderer
- * restored at a batch boundary renders the frames that follow bit for bit
- * as the original would have. */
+ * the Adam step, and the running average of the parameters the tracer
+ * samples with (each Adam step moves it by 1 % toward the new parameters);
+ * set_state restores the first three exactly and restarts the average at the
+ * parameters, and set_average then restores the average. With the frames a
+ * guiding renderer has trained on (guide_train_frames counts them), a
+ * renderer restored at a batch boundary renders the frames that follow bit
+ * for bit as the original would have. */
 int wsrt_guide_get_state(wsrt_guide* g, float* params, float* m1, float* m2, uint32_t* step);
 int wsrt_guide_set_state(wsrt_guide* g, const float* params, const float* m1, const float* m2, uint32_t step);
+int wsrt_guide_get_average(wsrt_guide* g, float* out);
+int wsrt_guide_set_average(wsrt_guide* g, const float* in);
 int wsrt_renderer_get_guide_frames_trained(const wsrt_renderer* r, uint64_t* out);
 int wsrt_renderer_set_guide_frames_trained(wsrt_renderer* r, uint64_t frames);
 

@@ -102,6 +102,30 @@ def test_adam_step(device):
     np.testing.assert_allclose(g.params, want2, atol=5e-6)
 
 
+def test_running_average_of_the_parameters(device):
+    """The network the tracer samples with is the running average of the Adam
+    iterates: avg_t = b_t avg_(t-1) + (1 - b_t) p_t with b_t = min(0.99,
+    (1 + t) / (10 + t)), in float32 as written, restarted at the parameters
+    by set_params, and restored by set_average."""
+    g = wsrt.Guide(device, 4096, seed=3)
+    p0 = random_params(7)
+    g.params = p0
+    np.testing.assert_array_equal(g.average, g.params)
+    s, d, w = batch(2048, seed=8)
+    avg = g.params.copy()
+    for t in range(1, 4):
+        g.train_step(s, d, w, lr=1e-2)
+        b = np.float32(min(np.float32(0.99), np.float32(1 + t) / np.float32(10 + t)))
+        avg = b * avg + (np.float32(1) - b) * g.params
+        np.testing.assert_array_equal(g.average, avg)
+    assert not np.array_equal(g.average, g.params)
+    st = g.state()
+    g.params = p0  # restarts the average
+    np.testing.assert_array_equal(g.average, g.params)
+    g.set_state(st)
+    np.testing.assert_array_equal(g.average, st["average"])
+
+
 def target_dirs(s, rng):
     """Directions from a vMF (kappa 20) whose mean depends on the inputs."""
     mu = np.stack([np.cos(6 * s[:, 0]), np.sin(6 * s[:, 0]), 2 * s[:, 3] - 1], 1)
@@ -202,6 +226,6 @@ def test_guided_renderer_resumes_bit_for_bit(device):
     b.render_frames(8)
     np.testing.assert_array_equal(b.read_accum(), want)
     sa, sb = a.guide().state(), b.guide().state()
-    for k in ("params", "m1", "m2"):
+    for k in ("params", "m1", "m2", "average"):
         np.testing.assert_array_equal(sa[k], sb[k])
     assert sa["step"] == sb["step"]

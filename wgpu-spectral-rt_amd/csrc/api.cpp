@@ -930,6 +930,7 @@ struct wsrt_guide {
     bool own_stream = false;
     bool owned_by_renderer = false;
     DevBuf wf, wt, params, m1, m2, step, act, partial, tin0, tin1, tdir, out;
+    DevBuf params_avg, wf_avg;  // running average of the parameters and its forward operands (inference)
     uint32_t ns_max = 0;  // training samples per step the buffers hold
     GuideView gv{};
     GuideTrain gt{};
@@ -987,6 +988,25 @@ std::vector<float> guide_initial_params(uint32_t seed) {
     return w;
 }
 
+// The running average restarts at the parameters (and its operands at
+// theirs), in stream order after the parameters are packed.
+hipError_t guide_reset_average(wsrt_guide* g) {
+    hipError_t e = hipMemcpyAsync(g->params_avg.p, g->params.p, 4ull * guide::kParams, hipMemcpyDeviceToDevice,
+                                  g->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(g->wf_avg.p, g->wf.p, 2ull * guide::kFwdPack, hipMemcpyDeviceToDevice, g->stream);
+    return e;
+}
+
+// The view of the network the tracer samples with: the running average's
+// operands and biases in place of the Adam iterate's.
+GuideView guide_inference_view(const wsrt_guide* g) {
+    GuideView v = g->gv;
+    v.wf = g->wf_avg.as<uint16_t>();
+    v.params = g->params_avg.as<float>();
+    return v;
+}
+
 int guide_setup(wsrt_guide* g, uint32_t max_samples, uint32_t seed, float lr, float mix) {
     using namespace guide;
     uint32_t ns, chunk;
@@ -996,6 +1016,8 @@ int guide_setup(wsrt_guide* g, uint32_t max_samples, uint32_t seed, float lr, fl
     HIP_OR_FAIL(g->wf.alloc(2ull * kFwdPack));
     HIP_OR_FAIL(g->wt.alloc(2ull * kBwdPack));
     HIP_OR_FAIL(g->params.alloc(4ull * kParams));
+    HIP_OR_FAIL(g->params_avg.alloc(4ull * kParams));
+    HIP_OR_FAIL(g->wf_avg.alloc(2ull * kFwdPack));
     HIP_OR_FAIL(g->m1.alloc(4ull * kParams));
     HIP_OR_FAIL(g->m2.alloc(4ull * kParams));
     HIP_OR_FAIL(g->step.alloc(4));
@@ -1026,9 +1048,12 @@ int guide_setup(wsrt_guide* g, uint32_t max_samples, uint32_t seed, float lr, fl
     g->gt.m1 = g->m1.as<float>();
     g->gt.m2 = g->m2.as<float>();
     g->gt.step = g->step.as<uint32_t>();
+    g->gt.avg = g->params_avg.as<float>();
+    g->gt.wf_avg = g->wf_avg.as<uint16_t>();
     g->gt.chunk = chunk;
     g->gt.lr = lr;
     launch_guide_pack(g->gv, g->params.as<float>(), s);
+    HIP_OR_FAIL(guide_reset_average(g));
     HIP_OR_FAIL(hipGetLastError());
     HIP_OR_FAIL(hipStreamSynchronize(s));
     return WSRT_OK;
@@ -1310,7 +1335,7 @@ extern "C" int wsrt_renderer_create(wsrt_device* dev, const wsrt_scene* scene, u
                                    (uint32_t)npx, ns,
                                    r->guide_stride, guide::fast_div((uint32_t)npx),
                                    guide::fast_div(r->guide_stride)};
-        pl.gv = g->gv;
+        pl.gv = guide_inference_view(g);
     }
     pl.ps.guide_q = r->g_q.as<uint32_t>();
     pl.ps.gn = r->g_n.as<float4>();
@@ -2116,6 +2141,34 @@ extern "C" int wsrt_guide_set_state(wsrt_guide* g, const float* params, const fl
     return WSRT_OK;
 }
 
+extern "C" int wsrt_guide_get_average(wsrt_guide* g, float* out) {
+    CHECK_ARG(g && out);
+    HIP_OR_FAIL(hipSetDevice(g->dev->ordinal));
+    HIP_OR_FAIL(hipMemcpyAsync(out, g->params_avg.p, 4ull * guide::kParams, hipMemcpyDeviceToHost, g->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(g->stream));
+    return WSRT_OK;
+}
+
+extern "C" int wsrt_guide_set_average(wsrt_guide* g, const float* in) {
+    CHECK_ARG(g && in);
+    HIP_OR_FAIL(hipSetDevice(g->dev->ordinal));
+    std::vector<float> w(in, in + guide::kParams);
+    for (int r = 0; r < guide::kHid; ++r)
+        for (int c = guide::kFeat; c < guide::kIn; ++c) w[guide::kW1 + r * guide::kIn + c] = 0.0f;
+    for (int r = guide::kOut; r < guide::kOutPad; ++r) {
+        for (int c = 0; c < guide::kHid; ++c) w[guide::kW5 + r * guide::kHid + c] = 0.0f;
+        w[guide::kB5 + r] = 0.0f;
+    }
+    HIP_OR_FAIL(hipMemcpyAsync(g->params_avg.p, w.data(), 4ull * guide::kParams, hipMemcpyHostToDevice, g->stream));
+    GuideView v = g->gv;  // pack the forward operands of the average only
+    v.wf = g->wf_avg.as<uint16_t>();
+    v.wt = nullptr;
+    launch_guide_pack(v, g->params_avg.as<float>(), g->stream);
+    HIP_OR_FAIL(hipGetLastError());
+    HIP_OR_FAIL(hipStreamSynchronize(g->stream));
+    return WSRT_OK;
+}
+
 extern "C" int wsrt_renderer_get_guide_frames_trained(const wsrt_renderer* r, uint64_t* out) {
     CHECK_ARG(r && out);
     *out = r->frames_trained;
@@ -2145,6 +2198,7 @@ extern "C" int wsrt_guide_set_params(wsrt_guide* g, const float* in) {
     HIP_OR_FAIL(hipMemsetAsync(g->m2.p, 0, g->m2.bytes, g->stream));
     HIP_OR_FAIL(hipMemsetAsync(g->step.p, 0, 4, g->stream));
     launch_guide_pack(g->gv, g->params.as<float>(), g->stream);
+    HIP_OR_FAIL(guide_reset_average(g));
     HIP_OR_FAIL(hipGetLastError());
     HIP_OR_FAIL(hipStreamSynchronize(g->stream));
     return WSRT_OK;

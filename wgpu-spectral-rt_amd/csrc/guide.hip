@@ -747,14 +747,24 @@ __device__ __forceinline__ int chain_pos(int q) {
 
 // Sums the partial gradients, applies Adam (update != 0) and writes the
 // packed BF16 operands of every weight.
+// Exponential average of the parameters the tracer samples with (Polyak
+// averaging): the Adam iterate itself moves by lr-sized steps on noisy
+// gradients, and one network taken at a given step can be markedly worse
+// than the ones a few steps either side of it. kAvgBeta 0.99 averages over
+// about 100 Adam steps (6 frames of 16 steps); early steps use
+// (1 + t) / (10 + t) instead, so the average follows the first updates.
+constexpr float kAvgBeta = 0.99f;
+
 __global__ __launch_bounds__(256) void k_guide_adam(const float* __restrict__ partial, uint32_t nchunks,
                                                     float* __restrict__ params, float* __restrict__ m1,
                                                     float* __restrict__ m2, const uint32_t* __restrict__ step,
                                                     float lr, uint32_t update, short* __restrict__ wf,
-                                                    short* __restrict__ wt) {
+                                                    short* __restrict__ wt, float* __restrict__ avg,
+                                                    short* __restrict__ wf_avg) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= kParams) return;
     float w = params[i];
+    float wa = 0.0f;
     if (update && !is_padding(i)) {
         float gsum = 0.0f;
 #pragma unroll 8
@@ -768,6 +778,11 @@ __global__ __launch_bounds__(256) void k_guide_adam(const float* __restrict__ pa
         const float mh = m / (1.0f - powf(b1, t)), vh = v / (1.0f - powf(b2, t));
         w -= lr * mh / (sqrtf(vh) + eps);
         params[i] = w;
+        if (avg) {
+            const float beta = fminf(kAvgBeta, (1.0f + t) / (10.0f + t));
+            wa = beta * avg[i] + (1.0f - beta) * w;
+            avg[i] = wa;
+        }
     }
     int layer, o, c, in_dim;
     if (i < kB1) { layer = 1; in_dim = kIn; o = (i - kW1) / kIn; c = (i - kW1) % kIn; }
@@ -778,6 +793,8 @@ __global__ __launch_bounds__(256) void k_guide_adam(const float* __restrict__ pa
     else return;  // biases are read as fp32
     (void)in_dim;
     const short h = bf16_bits(w);
+    const bool pack_avg = update && avg && wf_avg && !is_padding(i);
+    const short ha = pack_avg ? bf16_bits(wa) : (short)0;
     // forward operand A = W_l: row o, k = c (identity order for layer 1)
     {
         const int kb = c >> 5, q = c & 31;
@@ -785,10 +802,12 @@ __global__ __launch_bounds__(256) void k_guide_adam(const float* __restrict__ pa
         const int KB = layer == 1 ? 6 : 2;
         const int base = layer == 1 ? kF1 : layer == 2 ? kF2 : layer == 3 ? kF3 : layer == 4 ? kF4 : kF5;
         const int lane = 16 * (pphys >> 3) + (o & 15);
-        wf[base + (((o >> 4) * KB + kb) * 64 + lane) * 8 + (pphys & 7)] = h;
+        const int at = base + (((o >> 4) * KB + kb) * 64 + lane) * 8 + (pphys & 7);
+        wf[at] = h;
+        if (pack_avg) wf_avg[at] = ha;
     }
     // backward operand A = W_l^T: row c, k = o (layers 2..5)
-    if (layer >= 2) {
+    if (layer >= 2 && wt) {
         const int kb = o >> 5, q = o & 31;
         const int pphys = layer == 5 ? q : chain_pos(q);
         const int base = layer == 5 ? kT5 : layer == 4 ? kT4 : layer == 3 ? kT3 : kT2;
@@ -842,7 +861,7 @@ void launch_guide_scatter(const SceneView& sv, const PathState& ps, const Queues
 void launch_guide_pack(const GuideView& gv, float* params, hipStream_t s) {
     hipLaunchKernelGGL(k_guide_adam, dim3((kParams + 255) / 256), dim3(256), 0, s, nullptr, 0u, params, nullptr,
                        nullptr, nullptr, 0.0f, 0u, const_cast<short*>(reinterpret_cast<const short*>(gv.wf)),
-                       const_cast<short*>(reinterpret_cast<const short*>(gv.wt)));
+                       const_cast<short*>(reinterpret_cast<const short*>(gv.wt)), nullptr, nullptr);
 }
 
 void launch_guide_train_step(const GuideView& gv, const GuideTrain& gt, const float4* in0, const float4* in1,
@@ -861,7 +880,8 @@ void launch_guide_train_step(const GuideView& gv, const GuideTrain& gt, const fl
         hipLaunchKernelGGL(k_guide_adam, dim3((kParams + 255) / 256), dim3(256), 0, s, gt.partial, nchunks,
                            gt.params, gt.m1, gt.m2, gt.step, gt.lr, 1u,
                            const_cast<short*>(reinterpret_cast<const short*>(gv.wf)),
-                           const_cast<short*>(reinterpret_cast<const short*>(gv.wt)));
+                           const_cast<short*>(reinterpret_cast<const short*>(gv.wt)), gt.avg,
+                           reinterpret_cast<short*>(gt.wf_avg));
 }
 
 void launch_guide_prepare(const PathState& ps, const GuideView& gv, const FrameArgs& fa, uint32_t npx,

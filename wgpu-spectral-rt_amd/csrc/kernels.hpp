@@ -135,6 +135,8 @@ struct GuideTrain {
     float* params;          // fp32 master copy
     float* m1;              // Adam moments
     float* m2;
+    float* avg;             // running average of params (inference), or null
+    uint16_t* wf_avg;       // its packed forward operands
     uint32_t* step;         // Adam step counter (device)
     uint32_t chunk;         // samples per partial sum (divides ns, multiple of 32)
     float lr;

@@ -173,6 +173,8 @@ _SIGS = {
     "wsrt_renderer_get_guide": (C.c_int, [vp, P(vp)]),
     "wsrt_guide_get_state": (C.c_int, [vp, vp, vp, vp, P(u32)]),
     "wsrt_guide_set_state": (C.c_int, [vp, vp, vp, vp, u32]),
+    "wsrt_guide_get_average": (C.c_int, [vp, vp]),
+    "wsrt_guide_set_average": (C.c_int, [vp, vp]),
     "wsrt_renderer_get_guide_frames_trained": (C.c_int, [vp, P(C.c_uint64)]),
     "wsrt_renderer_set_guide_frames_trained": (C.c_int, [vp, C.c_uint64]),
     "wsrt_autotune": (C.c_int, [vp, vp, u32, u32, vp, P(Config), C.c_uint64, P(AutotuneResult)]),
@@ -504,16 +506,29 @@ class Guide:
         _check(load().wsrt_guide_set_params(self.h, _p(w)))
 
     def state(self) -> dict:
-        """Parameters, Adam moments and step (a training checkpoint)."""
-        p, m1, m2 = (np.zeros(GUIDE_PARAMS, np.float32) for _ in range(3))
+        """Parameters, Adam moments and step, and the running average of the
+        parameters the tracer samples with (a training checkpoint)."""
+        p, m1, m2, avg = (np.zeros(GUIDE_PARAMS, np.float32) for _ in range(4))
         step = u32()
         _check(load().wsrt_guide_get_state(self.h, _p(p), _p(m1), _p(m2), C.byref(step)))
-        return {"params": p, "m1": m1, "m2": m2, "step": step.value}
+        _check(load().wsrt_guide_get_average(self.h, _p(avg)))
+        return {"params": p, "m1": m1, "m2": m2, "step": step.value, "average": avg}
 
     def set_state(self, st: dict):
         p, m1, m2 = (np.ascontiguousarray(st[k], np.float32) for k in ("params", "m1", "m2"))
         assert p.size == m1.size == m2.size == GUIDE_PARAMS
         _check(load().wsrt_guide_set_state(self.h, _p(p), _p(m1), _p(m2), int(st["step"])))
+        if "average" in st:  # else the average restarts at the parameters
+            avg = np.ascontiguousarray(st["average"], np.float32)
+            assert avg.size == GUIDE_PARAMS
+            _check(load().wsrt_guide_set_average(self.h, _p(avg)))
+
+    @property
+    def average(self) -> np.ndarray:
+        """The running average of the parameters (the network the tracer samples with)."""
+        out = np.zeros(GUIDE_PARAMS, np.float32)
+        _check(load().wsrt_guide_get_average(self.h, _p(out)))
+        return out
 
     def forward(self, scalars) -> np.ndarray:
         x = np.ascontiguousarray(scalars, np.float32).reshape(-1, 7)
