@@ -288,7 +288,7 @@ def copy_calibration(mib: int) -> int:
     return 0
 
 
-def time_to_relmse(frames: int = 64, warmup: int = 256, boxes: int | None = None, width: int = 1024,
+def time_to_relmse(frames: int = 64, warmup: int = 512, boxes: int | None = None, width: int = 1024,
                    ref_frames: int = 2048, frames_in_flight: int = 64, rate_frames_in_flight=(64, 128),
                    guided_runs: int = 3) -> dict:
     """Guided vs plain tracer on the occluded-light room (2,097,182 triangles

@@ -578,7 +578,7 @@ int wsrt_renderer_get_guide(wsrt_renderer* r, wsrt_guide** out);
  * wsrt_renderer_read/write_accum and get/set_frame_count cover). The state
  * is the parameters, both Adam moments (WSRT_GUIDE_PARAMS floats each) and
  * the Adam step, and the running average of the parameters the tracer
- * samples with (each Adam step moves it by 1 % toward the new parameters);
+ * samples with (each Adam step moves it by 0.5 % toward the new parameters);
  * set_state restores the first three exactly and restarts the average at the
  * parameters, and set_average then restores the average. With the frames a
  * guiding renderer has trained on (guide_train_frames counts them), a

@@ -104,7 +104,7 @@ def test_adam_step(device):
 
 def test_running_average_of_the_parameters(device):
     """The network the tracer samples with is the running average of the Adam
-    iterates: avg_t = b_t avg_(t-1) + (1 - b_t) p_t with b_t = min(0.99,
+    iterates: avg_t = b_t avg_(t-1) + (1 - b_t) p_t with b_t = min(0.995,
     (1 + t) / (10 + t)), in float32 as written, restarted at the parameters
     by set_params, and restored by set_average."""
     g = wsrt.Guide(device, 4096, seed=3)
@@ -115,7 +115,7 @@ def test_running_average_of_the_parameters(device):
     avg = g.params.copy()
     for t in range(1, 4):
         g.train_step(s, d, w, lr=1e-2)
-        b = np.float32(min(np.float32(0.99), np.float32(1 + t) / np.float32(10 + t)))
+        b = np.float32(min(np.float32(0.995), np.float32(1 + t) / np.float32(10 + t)))
         avg = b * avg + (np.float32(1) - b) * g.params
         np.testing.assert_array_equal(g.average, avg)
     assert not np.array_equal(g.average, g.params)

@@ -750,10 +750,12 @@ __device__ __forceinline__ int chain_pos(int q) {
 // Exponential average of the parameters the tracer samples with (Polyak
 // averaging): the Adam iterate itself moves by lr-sized steps on noisy
 // gradients, and one network taken at a given step can be markedly worse
-// than the ones a few steps either side of it. kAvgBeta 0.99 averages over
-// about 100 Adam steps (6 frames of 16 steps); early steps use
+// than the ones a few steps either side of it (on the occluded room, C of
+// 128-171 over 8 trainings with the iterate, 127-136 with the average at
+// 0.99, mean 128.3 at 0.995 and 128.2 at 0.999 over 4). kAvgBeta 0.995
+// averages over about 200 Adam steps (12 frames of 16 steps); early steps use
 // (1 + t) / (10 + t) instead, so the average follows the first updates.
-constexpr float kAvgBeta = 0.99f;
+constexpr float kAvgBeta = 0.995f;
 
 __global__ __launch_bounds__(256) void k_guide_adam(const float* __restrict__ partial, uint32_t nchunks,
                                                     float* __restrict__ params, float* __restrict__ m1,
