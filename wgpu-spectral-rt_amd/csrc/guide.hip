@@ -228,6 +228,61 @@ __device__ __forceinline__ f3 vmf_sample(f3 mu, float kap, float x1, float x2) {
     return normalize(mu * w + t1 * (sw * cosf(ph)) + t2 * (sw * sinf(ph)));
 }
 
+// The same mixture for the tracer's sampling and densities, with the
+// hardware's approximate transcendentals (v_exp_f32, v_log_f32, v_sin_f32,
+// v_cos_f32, v_rcp_f32, v_sqrt_f32, v_rsq_f32; about 1 ulp) in place of the
+// correctly rounded library forms the training kernels keep. A sampled
+// direction and the density its weight uses come from the same Mixture, so
+// the one-sample MIS estimate stays unbiased; only the density's rounding
+// differs from the exact formula's.
+__device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504f); }
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ void mixture_from_fast(const float* o, Mixture& m) {
+    float mx = -3.0e38f;
+#pragma unroll
+    for (int k = 0; k < kLobes; ++k) mx = fmaxf(mx, kLogitScale * o[5 * k + 4]);
+    float z = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kLobes; ++k) {
+        const f3 v = mk3(o[5 * k], o[5 * k + 1], o[5 * k + 2]);
+        const float len = fmaxf(__builtin_amdgcn_sqrtf(dot(v, v)), 1e-6f);
+        m.len[k] = len;
+        m.mu[k] = v * fast_rcp(len);
+        m.s[k] = kKappaScale * o[5 * k + 3];
+        const float kap = fast_exp(fminf(fmaxf(m.s[k], kLogKappaMin), kLogKappaMax));
+        m.kappa[k] = kap;
+        // 1 - e^{-2 kappa} >= 0.095 for kappa >= e^-3 (kLogKappaMin)
+        m.norm[k] = kap * fast_rcp(2.0f * kPi * (1.0f - fast_exp(-2.0f * kap)));
+        m.pi[k] = fast_exp(kLogitScale * o[5 * k + 4] - mx);
+        z += m.pi[k];
+    }
+    const float iz = fast_rcp(z);
+#pragma unroll
+    for (int k = 0; k < kLobes; ++k) m.pi[k] *= iz;
+}
+__device__ __forceinline__ float mixture_pdf_fast(const Mixture& m, f3 w) {
+    float p = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kLobes; ++k) p += m.pi[k] * m.norm[k] * fast_exp(m.kappa[k] * (dot(m.mu[k], w) - 1.0f));
+    return p;
+}
+__device__ __forceinline__ f3 vmf_sample_fast(f3 mu, float kap, float x1, float x2) {
+    const float e = fast_exp(-2.0f * kap);
+    // log(y) = log2(y) ln 2; y >= e^{-2 kappa} > 0
+    float w = 1.0f + __builtin_amdgcn_logf(e + x1 * (1.0f - e)) * 0.693147181f * fast_rcp(kap);
+    w = fminf(fmaxf(w, -1.0f), 1.0f);
+    const float sw = __builtin_amdgcn_sqrtf(fmaxf(0.0f, 1.0f - w * w));
+    const float sgn = copysignf(1.0f, mu.z);
+    const float a = -fast_rcp(sgn + mu.z);
+    const float b = mu.x * mu.y * a;
+    const f3 t1 = mk3(1.0f + sgn * mu.x * mu.x * a, sgn * b, -sgn * mu.x);
+    const f3 t2 = mk3(b, sgn + mu.y * mu.y * a, -mu.y);
+    // v_sin/v_cos take revolutions: x2 in [0, 1) is the azimuth / 2 pi
+    const f3 d = mu * w + t1 * (sw * __builtin_amdgcn_cosf(x2)) + t2 * (sw * __builtin_amdgcn_sinf(x2));
+    return d * __builtin_amdgcn_rsqf(dot(d, d));
+}
+__device__ __forceinline__ f3 mixture_sample_fast(const Mixture& m, uint32_t& rng);
+
 // Network inputs of a hit: position over the scene box, normal, hero wavelength.
 __device__ __forceinline__ void scalars_of(const GuideView& gv, f3 hp, f3 n, float wl, float (&s)[kScalars]) {
     s[0] = fminf(fmaxf((hp.x - gv.bmin[0]) * gv.binv[0], 0.0f), 1.0f);
@@ -273,6 +328,27 @@ __device__ __forceinline__ f3 mixture_sample(const Mixture& m, uint32_t& rng) {
     const float x1 = rand01(rng), x2 = rand01(rng);
     return vmf_sample(mu, kap, x1, x2);
 }
+__device__ __forceinline__ f3 mixture_sample_fast(const Mixture& m, uint32_t& rng) {
+    const float xl = rand01(rng);
+    int k = 0;
+    float c = m.pi[0];
+#pragma unroll
+    for (int j = 1; j < kLobes; ++j)
+        if (xl >= c) {
+            k = j;
+            c += m.pi[j];
+        }
+    f3 mu = m.mu[0];
+    float kap = m.kappa[0];
+#pragma unroll
+    for (int j = 1; j < kLobes; ++j)
+        if (k == j) {
+            mu = m.mu[j];
+            kap = m.kappa[j];
+        }
+    const float x1 = rand01(rng), x2 = rand01(rng);
+    return vmf_sample_fast(mu, kap, x1, x2);
+}
 
 // Next-event estimation at a guided Lambert vertex with a learned light
 // proposal. With probability light_mix the light point comes from the plain
@@ -314,7 +390,7 @@ __device__ __forceinline__ GuidedNee guided_nee(const SceneView& sv, const Mixtu
         rad = ls.radiance;
         delta = ls.delta;
     } else {
-        dir = mixture_sample(m, rng);
+        dir = mixture_sample_fast(m, rng);
         // the nearest light triangle along dir (every light is an area
         // light here: light_mix < 1 only then)
         float best = 3.0e38f;
@@ -351,7 +427,7 @@ __device__ __forceinline__ GuidedNee guided_nee(const SceneView& sv, const Mixtu
     if (delta) {
         w = 1.0f / lp;  // a point light (light_mix is 1 then)
     } else {
-        const float pg = mixture_pdf(m, dir);
+        const float pg = mixture_pdf_fast(m, dir);
         const float q = light_mix * lp + (1.0f - light_mix) * pg;
         w = mis ? 1.0f / (q + guide_mix * pg + (1.0f - guide_mix) * bp) : (lp / (lp + bp)) / q;
     }
@@ -413,7 +489,7 @@ __global__ __launch_bounds__(64 * kScatterBlockWaves) void k_guide_scatter(Scene
         GuidedNee nee;
         if (have) {
             Mixture m;
-            mixture_from(o, m);
+            mixture_from_fast(o, m);
             uint32_t rng = ps.rng[p];
             uint32_t flags = ps.flags[p];
             float4 tp = ps.tp[p];
@@ -423,7 +499,7 @@ __global__ __launch_bounds__(64 * kScatterBlockWaves) void k_guide_scatter(Scene
             shadow = nee.shadow;
             f3 w;
             if (rand01(rng) < gv.mix) {
-                w = mixture_sample(m, rng);
+                w = mixture_sample_fast(m, rng);
             } else {
                 f3 sd = ff + random_unit_vector(rng);
                 if (length(sd) < 0.001f) sd = ff;
@@ -431,7 +507,7 @@ __global__ __launch_bounds__(64 * kScatterBlockWaves) void k_guide_scatter(Scene
             }
             const float cw = dot(ff, w);
             if (cw > 0.0f) {
-                const float pg = mixture_pdf(m, w);
+                const float pg = mixture_pdf_fast(m, w);
                 const float pm = gv.mix * pg + (1.0f - gv.mix) * (cw / kPi);
                 tp = mul4(mul4s(spec, (cw / kPi) / pm), tp);
                 if (bounce < kRecBounces && gv.rec.in0) {
