@@ -105,12 +105,21 @@ __device__ void mlp_forward(const bf16x8* __restrict__ Wf, const float* __restri
 #pragma unroll
                 for (int k = 1; k < kScalars; ++k)
                     if (a == k) xa = x[t][k];
+                // band 4 bq: sin and cos of 2 pi frac(2^(4 bq - 1) x) by the
+                // hardware (arguments in revolutions); the quad's next three
+                // bands by the double-angle formulas, full-rate arithmetic in
+                // place of six quarter-rate transcendentals (the error grows
+                // 2x per doubling, to ~1e-6, far below BF16's 2^-9)
+                const float tt = ldexpf(xa, 4 * bq - 1);
+                const float u = tt - floorf(tt);
+                float sn = __builtin_amdgcn_sinf(u), cs = __builtin_amdgcn_cosf(u);
 #pragma unroll
                 for (int bb = 0; bb < 4; ++bb) {
-                    const float tt = ldexpf(xa, 4 * bq + bb - 1);
-                    const float u = tt - floorf(tt);
-                    B[t][2 * bb] = bf16_bits(__builtin_amdgcn_sinf(u));
-                    B[t][2 * bb + 1] = bf16_bits(__builtin_amdgcn_cosf(u));
+                    B[t][2 * bb] = bf16_bits(sn);
+                    B[t][2 * bb + 1] = bf16_bits(cs);
+                    const float s2 = 2.0f * sn * cs;
+                    cs = 1.0f - 2.0f * sn * sn;
+                    sn = s2;
                 }
             } else {
 #pragma unroll
@@ -329,25 +338,23 @@ __device__ __forceinline__ f3 mixture_sample(const Mixture& m, uint32_t& rng) {
     return vmf_sample(mu, kap, x1, x2);
 }
 __device__ __forceinline__ f3 mixture_sample_fast(const Mixture& m, uint32_t& rng) {
+    // the lobe by walking the CDF, its mean and concentration selected in
+    // the same walk as values (a lobe index used afterwards makes the
+    // compiler put the mixture in scratch memory and load it indexed)
     const float xl = rand01(rng);
-    int k = 0;
     float c = m.pi[0];
+    float mx = m.mu[0].x, my = m.mu[0].y, mz = m.mu[0].z, kap = m.kappa[0];
 #pragma unroll
-    for (int j = 1; j < kLobes; ++j)
-        if (xl >= c) {
-            k = j;
-            c += m.pi[j];
-        }
-    f3 mu = m.mu[0];
-    float kap = m.kappa[0];
-#pragma unroll
-    for (int j = 1; j < kLobes; ++j)
-        if (k == j) {
-            mu = m.mu[j];
-            kap = m.kappa[j];
-        }
+    for (int j = 1; j < kLobes; ++j) {
+        const bool take = xl >= c;
+        c = take ? c + m.pi[j] : c;
+        mx = take ? m.mu[j].x : mx;
+        my = take ? m.mu[j].y : my;
+        mz = take ? m.mu[j].z : mz;
+        kap = take ? m.kappa[j] : kap;
+    }
     const float x1 = rand01(rng), x2 = rand01(rng);
-    return vmf_sample_fast(mu, kap, x1, x2);
+    return vmf_sample_fast(mk3(mx, my, mz), kap, x1, x2);
 }
 
 // Next-event estimation at a guided Lambert vertex with a learned light
