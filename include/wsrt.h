@@ -486,7 +486,7 @@ typedef struct wsrt_autotune_result {
     float stage2_ms_per_frame[8]; /* 0: not run */
     uint32_t best_tail_paths, best_trace_rounds;
     /* third stage, at the second stage's choice: the closest-hit kernel's
-     * lane refill threshold (extend_refill 0, 16 and 32) with and without
+     * lane refill threshold (extend_refill 0, 16, 24 and 32) with and without
      * path-ordered queues (ordered_queues) */
     uint32_t n_stage3;
     uint32_t extend_refill[8];

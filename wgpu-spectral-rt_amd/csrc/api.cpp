@@ -2402,11 +2402,11 @@ extern "C" int wsrt_autotune(wsrt_device* dev, const wsrt_scene* scene, uint32_t
     // ray costs within a wave differ widely (occluded interiors)
     // and path-ordered queues, which shorten shading where paths scatter
     // widely
-    const uint32_t refills[3] = {0u, 16u, 32u};
+    const uint32_t refills[4] = {0u, 16u, 24u, 32u};
     float best3 = best2;
     uint32_t bx = cfg->extend_refill, bo = cfg->ordered_queues;
     for (uint32_t o = 0; o < 2; ++o) {
-        for (uint32_t k = 0; k < 3; ++k) {
+        for (uint32_t k = 0; k < 4; ++k) {
             if (refills[k] == cfg->extend_refill && o == cfg->ordered_queues) continue;  // measured above
             wsrt_config c = *cfg;
             c.extend_refill = refills[k];
