@@ -215,28 +215,6 @@ __device__ __forceinline__ void mixture_from(const float* o, Mixture& m) {
 #pragma unroll
     for (int k = 0; k < kLobes; ++k) m.pi[k] *= iz;
 }
-__device__ __forceinline__ float mixture_pdf(const Mixture& m, f3 w) {
-    float p = 0.0f;
-#pragma unroll
-    for (int k = 0; k < kLobes; ++k) p += m.pi[k] * m.norm[k] * expf(m.kappa[k] * (dot(m.mu[k], w) - 1.0f));
-    return p;
-}
-// A direction from the vMF lobe (mu, kappa): cos theta by inverting the CDF
-// 1 + log(e^{-2k} + x1 (1 - e^{-2k})) / k, azimuth 2 pi x2.
-__device__ __forceinline__ f3 vmf_sample(f3 mu, float kap, float x1, float x2) {
-    const float e = expf(-2.0f * kap);
-    float w = 1.0f + logf(e + x1 * (1.0f - e)) / kap;
-    w = fminf(fmaxf(w, -1.0f), 1.0f);
-    const float sw = sqrtf(fmaxf(0.0f, 1.0f - w * w));
-    const float sgn = copysignf(1.0f, mu.z);
-    const float a = -1.0f / (sgn + mu.z);
-    const float b = mu.x * mu.y * a;
-    const f3 t1 = mk3(1.0f + sgn * mu.x * mu.x * a, sgn * b, -sgn * mu.x);
-    const f3 t2 = mk3(b, sgn + mu.y * mu.y * a, -mu.y);
-    const float ph = 2.0f * kPi * x2;
-    return normalize(mu * w + t1 * (sw * cosf(ph)) + t2 * (sw * sinf(ph)));
-}
-
 // The same mixture for the tracer's sampling and densities, with the
 // hardware's approximate transcendentals (v_exp_f32, v_log_f32, v_sin_f32,
 // v_cos_f32, v_rcp_f32, v_sqrt_f32, v_rsq_f32; about 1 ulp) in place of the
@@ -275,6 +253,9 @@ __device__ __forceinline__ float mixture_pdf_fast(const Mixture& m, f3 w) {
     for (int k = 0; k < kLobes; ++k) p += m.pi[k] * m.norm[k] * fast_exp(m.kappa[k] * (dot(m.mu[k], w) - 1.0f));
     return p;
 }
+// A direction from the vMF lobe (mu, kappa): cos theta by inverting the CDF,
+// 1 + log(e^{-2k} + x1 (1 - e^{-2k})) / k, azimuth 2 pi x2, in the frame of
+// mu's branch-free orthonormal basis.
 __device__ __forceinline__ f3 vmf_sample_fast(f3 mu, float kap, float x1, float x2) {
     const float e = fast_exp(-2.0f * kap);
     // log(y) = log2(y) ln 2; y >= e^{-2 kappa} > 0
@@ -290,7 +271,6 @@ __device__ __forceinline__ f3 vmf_sample_fast(f3 mu, float kap, float x1, float 
     const f3 d = mu * w + t1 * (sw * __builtin_amdgcn_cosf(x2)) + t2 * (sw * __builtin_amdgcn_sinf(x2));
     return d * __builtin_amdgcn_rsqf(dot(d, d));
 }
-__device__ __forceinline__ f3 mixture_sample_fast(const Mixture& m, uint32_t& rng);
 
 // Network inputs of a hit: position over the scene box, normal, hero wavelength.
 __device__ __forceinline__ void scalars_of(const GuideView& gv, f3 hp, f3 n, float wl, float (&s)[kScalars]) {
@@ -316,27 +296,6 @@ __device__ __forceinline__ void load_biases(float* dst, const float* params) {
 // update f cos / p_mix, Russian roulette (shader.wgsl:694-700) and the
 // training record of the path's first Lambert vertex.
 // A direction from the mixture: lobe k with probability pi_k, then its vMF.
-__device__ __forceinline__ f3 mixture_sample(const Mixture& m, uint32_t& rng) {
-    const float xl = rand01(rng);
-    int k = 0;
-    float c = m.pi[0];
-#pragma unroll
-    for (int j = 1; j < kLobes; ++j)
-        if (xl >= c) {
-            k = j;
-            c += m.pi[j];
-        }
-    f3 mu = m.mu[0];
-    float kap = m.kappa[0];
-#pragma unroll
-    for (int j = 1; j < kLobes; ++j)
-        if (k == j) {
-            mu = m.mu[j];
-            kap = m.kappa[j];
-        }
-    const float x1 = rand01(rng), x2 = rand01(rng);
-    return vmf_sample(mu, kap, x1, x2);
-}
 __device__ __forceinline__ f3 mixture_sample_fast(const Mixture& m, uint32_t& rng) {
     // the lobe by walking the CDF, its mean and concentration selected in
     // the same walk as values (a lobe index used afterwards makes the
