@@ -532,7 +532,6 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(4) void k_tail(S
     const uint32_t base_frame = *fa.frame;
     const uint32_t rr_after = fa.table[base_frame & fa.mask].rr_after;
     const uint32_t quirks = fa.table[base_frame & fa.mask].quirks;
-    const float light_mix = fa.table[base_frame & fa.mask].guide_light_mix;
     const uint32_t photons_on = fa.table[base_frame & fa.mask].enable_photons;
     const bool count = count_traversal != 0u;
     RayTally tally;

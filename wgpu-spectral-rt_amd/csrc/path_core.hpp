@@ -160,7 +160,10 @@ __device__ __forceinline__ VertexOut shade_vertex(const SceneView& sv, const Gri
                                    : 0.0f;
         r.d4 = f4(sc.dir, prev_pdf);
         out.alive = true;
-        if (bounce > rr_after) {
+        // Russian roulette (shader.wgsl:696-700); a guided renderer plays
+        // none (see k_guide_scatter): its paths' throughput is low exactly
+        // where the guide sends them toward light
+        if (!guided && bounce > rr_after) {
             float pr = fmaxf(fmaxf(r.tp.x, r.tp.y), fmaxf(r.tp.z, r.tp.w));
             if (quirks & WSRT_FIX_RR_CLAMP) pr = fminf(pr, 1.0f);
             if (rand01(r.rng) > pr) out.alive = false;
