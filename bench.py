@@ -314,7 +314,9 @@ def time_to_relmse(frames: int = 64, warmup: int = 512, boxes: int | None = None
     buffers are one-time costs), and the guided tracer its training frames.
     Every timed window is whole batches. The guided tracer's C is the mean
     over `guided_runs` independently trained networks (training is
-    stochastic: over 8 trainings C ranged 128-171 with a median of 134)."""
+    stochastic). The guided tracer plays no Russian roulette (the plain one
+    keeps the reference's); a third tracer, the plain one without roulette,
+    is measured as a control."""
     import numpy as np
     import scenes
     import wsrt
@@ -374,7 +376,12 @@ def time_to_relmse(frames: int = 64, warmup: int = 512, boxes: int | None = None
         def guided(**kw):
             return scenes.guided_config(guide_train_frames=warmup, quirks=quirks, **kw)
 
-        for name, make in (("plain", plain), ("guided", guided)):
+        def plain_no_roulette(**kw):
+            # a control: the plain tracer without Russian roulette, as the
+            # guided tracer runs (rr_after past the last bounce)
+            return scenes.occluded_config(quirks=quirks, rr_after=255, **kw)
+
+        for name, make in (("plain", plain), ("guided", guided), ("plain_no_roulette", plain_no_roulette)):
             cfg = make(frames_in_flight=S)
             r = renderer(cfg, 0)
             tw = {S: warm(r, name, S)}
@@ -420,6 +427,8 @@ def time_to_relmse(frames: int = 64, warmup: int = 512, boxes: int | None = None
                          "training_frames_before_measuring": warmup if name == "guided" else 0}
         res["speedup_guided_vs_plain"] = round(res["plain"]["time_to_relmse_s"] / res["guided"]["time_to_relmse_s"],
                                                3)
+        res["speedup_guided_vs_plain_no_roulette"] = round(
+            res["plain_no_roulette"]["time_to_relmse_s"] / res["guided"]["time_to_relmse_s"], 3)
         return res
 
     # the reference's estimator (emission reached by a Lambert bounce counted
