@@ -154,7 +154,8 @@ typedef struct wsrt_config {
     float gather_radius_exponent; /* -0.15 in the reference                    */
     float spectral_radius;        /* nm                                        */
     uint32_t gather_cap;          /* photons visited per gather (64)           */
-    uint32_t rr_after;            /* roulette when bounce index > rr_after (3) */
+    uint32_t rr_after;            /* roulette when bounce index > rr_after (3);
+                                   * a guiding renderer plays none          */
     uint32_t enable_photons;      /* 1: photon pass + gather, 0: skip both     */
     uint32_t photon_norm;         /* density divisor (2^20 = MAX_PHOTON)       */
     uint32_t guiding;             /* 0 plain BSDF sampling; 1 learned guiding  */
