@@ -165,7 +165,8 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend
 // taken in that order from a wave-uniform cursor; when at least `idle_min`
 // lanes idle, the idle lanes take the next rays, so a wave no longer waits
 // on its longest ray before starting new ones. Each loop iteration is one
-// traversal step (trav_step) of every lane holding a ray. A finished ray's
+// traversal step of every lane holding a ray, with the step's triangle tests
+// spread over all 64 lanes, idle ones included (trav_step_wave). A finished ray's
 // hit is written at once; its (path, material bin) waits in the wave's LDS
 // buffer until 64 have finished, then the 64 are sorted by path and binned
 // together (one segmented append per bin, as in k_extend), so each bin's
@@ -225,7 +226,7 @@ template <bool kCount>
 __device__ __forceinline__ void extend_refill_loop(const SceneView& sv, const PathState& ps, const Queues& qs,
                                                    Counters* __restrict__ ctr, uint32_t bounce, uint32_t n,
                                                    const uint32_t* __restrict__ queue, const uint32_t* pre,
-                                                   Stack st, DoneBuf& db, uint32_t idle_min) {
+                                                   Stack st, DoneBuf& db, TriShare& sh, uint32_t idle_min) {
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t nw = stride >> 6, wave = gtid >> 6, lane = __lane_id();
@@ -261,7 +262,7 @@ __device__ __forceinline__ void extend_refill_loop(const SceneView& sv, const Pa
         iters++;
         bool to_bin = false;
         uint32_t tri = 0;
-        if (active && !trav_step<false, kCount>(sv, ts, st)) {
+        if (!trav_step_wave<kCount>(sv, ts, st, active, sh) && active) {
             active = false;
             my_nodes += ts.nvis;
             my_tris += ts.ntest;
@@ -328,6 +329,7 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend
     __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
     __shared__ uint32_t pre[kSegs + 1];
     __shared__ DoneBuf done_all[kTraceThreads / 64];
+    __shared__ TriShare share_all[kTraceThreads / 64];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;
     st.lds = lds_slot(lds_stack + threadIdx.x);
@@ -343,7 +345,8 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend
         atomicAdd(&ctr->kernel_rays[kRkExtend], (unsigned long long)n);
     }
     DoneBuf& db = done_all[threadIdx.x >> 6];
-    extend_refill_loop<kCount>(sv, ps, qs, ctr, bounce, n, queue, pre, st, db, idle_min);
+    extend_refill_loop<kCount>(sv, ps, qs, ctr, bounce, n, queue, pre, st, db, share_all[threadIdx.x >> 6],
+                               idle_min);
 }
 
 __global__ __launch_bounds__(kShadeThreads) void k_shade(SceneView sv, PathState ps, GridView grid,

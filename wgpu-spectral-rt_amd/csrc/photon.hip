@@ -341,7 +341,7 @@ __device__ __forceinline__ bool photon_interact(const SceneView& sv, const Photo
 // The rays of entries [begin, end) of a bounce, one per lane at a time.
 __device__ __forceinline__ void photon_bounce_range(const SceneView& sv, uint32_t depth, const Queues& qs,
                                                     const PhotonState& ps, const uint32_t* pre,
-                                                    wsrt_photon* __restrict__ photons, Stack st,
+                                                    wsrt_photon* __restrict__ photons, Stack st, TriShare& sh,
                                                     uint32_t begin, uint32_t end, bool count,
                                                     uint32_t& my_rays, uint32_t& my_stored, uint32_t& my_nodes,
                                                     uint32_t& my_tris) {
@@ -349,11 +349,12 @@ __device__ __forceinline__ void photon_bounce_range(const SceneView& sv, uint32_
     const PhotonQueue& nxt = ps.q[(depth + 1u) & 1u];
     for (uint32_t base = begin; base < end; base += 64u) {
         const uint32_t i = base + __lane_id();
+        const bool have = i < end;
         bool alive = false;
         PhotonRay r{};
-        f3 pos, dir;
+        f3 pos = mk3(0.0f, 0.0f, 0.0f), dir = mk3(0.0f, 0.0f, 1.0f);
         const uint32_t s0 = seg_of_wave_base(pre, base);  // whole wave
-        if (i < end) {
+        if (have) {
             const uint32_t p = seg_entry_from(pre, s0, i, qs.pseg_cap);
             r.id = cur.id[p];
             const float4 pw = cur.pos[p], de = cur.dir[p];
@@ -363,7 +364,11 @@ __device__ __forceinline__ void photon_bounce_range(const SceneView& sv, uint32_
             r.caustic = cur.flags[p] != 0u;
             pos = xyz(pw);
             dir = xyz(de);
-            const Hit h = trace_ray<false>(sv, pos, dir, 0.0001f, 100.0f, st, count, my_nodes, my_tris);
+        }
+        // the wave traverses its rays together, triangle tests shared
+        const Hit h = count ? trace_wave<true>(sv, pos, dir, 0.0001f, 100.0f, have, st, sh, my_nodes, my_tris)
+                            : trace_wave<false>(sv, pos, dir, 0.0001f, 100.0f, have, st, sh, my_nodes, my_tris);
+        if (have) {
             my_rays++;
             alive = photon_interact(sv, ps, photons, r, pos, dir, h, my_stored);
         }
@@ -389,6 +394,7 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon
     __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
     __shared__ uint32_t pre[kSegs + 1];
     __shared__ unsigned long long red[kTraceThreads / 64];
+    __shared__ TriShare share_all[kTraceThreads / 64];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;
     st.lds = lds_slot(lds_stack + threadIdx.x);
@@ -402,7 +408,8 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon
     const uint32_t begin = (uint32_t)min((uint64_t)n, (uint64_t)wave * per_wave * 64u);
     const uint32_t end = (uint32_t)min((uint64_t)n, (uint64_t)(wave + 1u) * per_wave * 64u);
     uint32_t my_rays = 0, my_stored = 0, my_nodes = 0, my_tris = 0;
-    photon_bounce_range(sv, depth, qs, ps, pre, photons, st, begin, end, count_traversal != 0u,
+    photon_bounce_range(sv, depth, qs, ps, pre, photons, st, share_all[threadIdx.x >> 6], begin, end,
+                        count_traversal != 0u,
                         my_rays, my_stored, my_nodes, my_tris);
     if (gtid == 0) atomicAdd(&ctr->kernel_rays[kRkPhotonBounce], (unsigned long long)n);
     if (count_traversal) flush_traversal_counts(ctr, kRkPhotonBounce, my_rays, my_nodes, my_tris);
