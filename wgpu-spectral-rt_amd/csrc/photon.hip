@@ -339,10 +339,11 @@ __device__ __forceinline__ bool photon_interact(const SceneView& sv, const Photo
 }
 
 // The rays of entries [begin, end) of a bounce, one per lane at a time.
+template <bool kCount>
 __device__ __forceinline__ void photon_bounce_range(const SceneView& sv, uint32_t depth, const Queues& qs,
                                                     const PhotonState& ps, const uint32_t* pre,
                                                     wsrt_photon* __restrict__ photons, Stack st, TriShare& sh,
-                                                    uint32_t begin, uint32_t end, bool count,
+                                                    uint32_t begin, uint32_t end,
                                                     uint32_t& my_rays, uint32_t& my_stored, uint32_t& my_nodes,
                                                     uint32_t& my_tris) {
     const PhotonQueue& cur = ps.q[depth & 1u];
@@ -366,8 +367,7 @@ __device__ __forceinline__ void photon_bounce_range(const SceneView& sv, uint32_
             dir = xyz(de);
         }
         // the wave traverses its rays together, triangle tests shared
-        const Hit h = count ? trace_wave<true>(sv, pos, dir, 0.0001f, 100.0f, have, st, sh, my_nodes, my_tris)
-                            : trace_wave<false>(sv, pos, dir, 0.0001f, 100.0f, have, st, sh, my_nodes, my_tris);
+        const Hit h = trace_wave<kCount>(sv, pos, dir, 0.0001f, 100.0f, have, st, sh, my_nodes, my_tris);
         if (have) {
             my_rays++;
             alive = photon_interact(sv, ps, photons, r, pos, dir, h, my_stored);
@@ -387,8 +387,11 @@ __device__ __forceinline__ void photon_bounce_range(const SceneView& sv, uint32_
 // hit, then the surface interaction. Survivors are compacted into the next
 // queue, so the few paths refracting through glass run in full waves. Each
 // wave takes a contiguous range of the queue.
+// kCount: the traversal-counting build, a kernel of its own so that its
+// counters do not take registers from the plain one.
+template <bool kCount>
 __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon_bounce(
-    SceneView sv, uint32_t depth, Queues qs, uint32_t count_traversal, PhotonState ps,
+    SceneView sv, uint32_t depth, Queues qs, PhotonState ps,
     wsrt_photon* __restrict__ photons,
     Counters* __restrict__ ctr, uint2* __restrict__ gstack) {
     __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
@@ -408,11 +411,10 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_photon
     const uint32_t begin = (uint32_t)min((uint64_t)n, (uint64_t)wave * per_wave * 64u);
     const uint32_t end = (uint32_t)min((uint64_t)n, (uint64_t)(wave + 1u) * per_wave * 64u);
     uint32_t my_rays = 0, my_stored = 0, my_nodes = 0, my_tris = 0;
-    photon_bounce_range(sv, depth, qs, ps, pre, photons, st, share_all[threadIdx.x >> 6], begin, end,
-                        count_traversal != 0u,
-                        my_rays, my_stored, my_nodes, my_tris);
+    photon_bounce_range<kCount>(sv, depth, qs, ps, pre, photons, st, share_all[threadIdx.x >> 6], begin, end,
+                                my_rays, my_stored, my_nodes, my_tris);
     if (gtid == 0) atomicAdd(&ctr->kernel_rays[kRkPhotonBounce], (unsigned long long)n);
-    if (count_traversal) flush_traversal_counts(ctr, kRkPhotonBounce, my_rays, my_nodes, my_tris);
+    if (kCount) flush_traversal_counts(ctr, kRkPhotonBounce, my_rays, my_nodes, my_tris);
     // per-block statistics, summed on the host; the emit kernel counted the
     // first rays
     const unsigned long long br = block_sum(depth != 0 ? my_rays : 0u, red);
@@ -574,8 +576,9 @@ void launch_photon_emit(const SceneView& sv, const FrameArgs& fa, uint32_t np, u
 
 void launch_photon_bounce(const SceneView& sv, uint32_t depth, uint32_t count_traversal, const PhotonState& ps,
                           const Queues& qs, wsrt_photon* photons, Counters* ctr, uint2* gstack, hipStream_t s) {
-    hipLaunchKernelGGL(k_photon_bounce, dim3(trace_grid(5, qs.trace_rounds)), dim3(kTraceThreads), 0, s, sv, depth, qs,
-                       count_traversal, ps, photons, ctr, gstack);
+    hipLaunchKernelGGL(count_traversal ? k_photon_bounce<true> : k_photon_bounce<false>,
+                       dim3(trace_grid(5, qs.trace_rounds)), dim3(kTraceThreads), 0, s, sv, depth, qs, ps, photons,
+                       ctr, gstack);
 }
 
 void launch_store_frame(const FrameConst& fc, FrameConst* dst, uint32_t* frame, hipStream_t s) {

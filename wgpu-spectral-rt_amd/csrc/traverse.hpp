@@ -367,6 +367,9 @@ __device__ __forceinline__ bool trav_step(const SceneView& sv, TravState& s, Sta
 //    largest key whose t is below tmax; since t > tmin >= 0, keys order as
 //    (t, id), so the final key is the closest hit with ties to the lower id,
 //    the result of the lane's own tests in any order, bit for bit.
+// Any-hit rays (shadow) keep trav_step: there the lane's leaf-by-leaf loop
+// ends at its first hit, and testing all of a node's triangles instead made
+// the occluded room's shadow kernel 7 % slower.
 // Every lane of the wave calls it, in wave-uniform control flow; lanes with
 // no tests help the others.
 struct TriShare {
@@ -402,9 +405,11 @@ __device__ __forceinline__ void wave_test_tris(const SceneView& sv, TravState& s
     const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);  // the wave's tests
     unsigned long long key0 = 0;
     if (cnt) {
-        key0 = s.best.tri != kInvalid
-                   ? ((unsigned long long)__float_as_uint(s.best.t) << 32) | s.best.tri
-                   : ((unsigned long long)(__float_as_uint(s.best.t) - 1u) << 32) | 0xFFFFFFFFull;
+        // (with no hit and tmax not above 0, NaN included, nothing is below
+        // key 0, as no t > tmin >= 0 is below such a tmax)
+        key0 = s.best.tri != kInvalid ? ((unsigned long long)__float_as_uint(s.best.t) << 32) | s.best.tri
+               : s.best.t > 0.0f      ? ((unsigned long long)(__float_as_uint(s.best.t) - 1u) << 32) | 0xFFFFFFFFull
+                                      : 0ull;
         sh.key[lane] = key0;
     }
     for (uint32_t c0 = 0; c0 < total; c0 += 64u) {
