@@ -65,7 +65,7 @@ def cpu_baseline(seconds: float = 12.0) -> dict:
 
 # the kernels a summary names, by their short symbol (the closest-hit kernel
 # is the lane-refill one where the workload's configuration uses it)
-KERNEL_SYMBOLS = {"extend": ("k_extend", "void k_extend_refill<false>"), "shadow": ("k_shadow",),
+KERNEL_SYMBOLS = {"extend": ("k_extend", "void k_extend<false>", "void k_extend_refill<false>"), "shadow": ("k_shadow",),
                   "photon_trace": ("k_photon_bounce", "void k_photon_bounce<false>")}
 
 

@@ -82,13 +82,15 @@ __device__ __forceinline__ float4 env_gain(const PathState& ps, uint32_t p, uint
 
 // Bounce 0 reads path ids 0 .. n0-1 directly (every path starts); later
 // bounces read the compacted queue of the previous shade.
+// kCount: the traversal-counting build, a kernel of its own.
+template <bool kCount>
 __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend(SceneView sv, PathState ps, Queues qs,
-                                                          uint32_t count_traversal,
                                                           Counters* __restrict__ ctr, uint32_t bounce,
                                                           uint32_t n0, const uint32_t* __restrict__ queue,
                                                           uint2* __restrict__ gstack) {
     __shared__ uint2 lds_stack[kShortStack * kTraceThreads];
     __shared__ uint32_t pre[kSegs + 1];
+    __shared__ TriShare share_all[kTraceThreads / 64];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;
     st.lds = lds_slot(lds_stack + threadIdx.x);
@@ -103,8 +105,9 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend
         atomicAdd(&ctr->extend_rays, (unsigned long long)n);
         atomicAdd(&ctr->kernel_rays[kRkExtend], (unsigned long long)n);
     }
+    TriShare& sh = share_all[threadIdx.x >> 6];
     const uint32_t stride = gridDim.x * blockDim.x;
-    const bool count = count_traversal != 0u;
+    const bool count = kCount;
     uint32_t my_rays = 0, my_nodes = 0, my_tris = 0, my_max = 0;
     // wave-uniform trip count, so the binning ballots see the whole wave.
     // Bounce 0 takes path ids in pixel order; a plain grid stride would give
@@ -120,10 +123,15 @@ __global__ __launch_bounds__(kTraceThreads) WSRT_WAVES_PER_SIMD(5) void k_extend
         bool have = i < n;
         uint32_t p = 0, bin = 0, ray_nodes = 0, ray_tris = 0;
         const uint32_t s0 = bounce == 0 ? 0u : seg_of_wave_base(pre, base);  // whole wave
+        float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f), d = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
         if (have) {
             p = bounce == 0 ? i : queue[seg_entry_from(pre, s0, i, qs.seg_cap)];
-            const float4 o = ps.o[p], d = ps.d[p];
-            Hit h = trace_ray<false>(sv, xyz(o), xyz(d), 0.0001f, 100.0f, st, count, ray_nodes, ray_tris);
+            o = ps.o[p];
+            d = ps.d[p];
+        }
+        // the wave traverses its rays together, triangle tests shared
+        const Hit h = trace_wave<kCount>(sv, xyz(o), xyz(d), 0.0001f, 100.0f, have, st, sh, ray_nodes, ray_tris);
+        if (have) {
             my_nodes += ray_nodes;
             my_tris += ray_tris;
             my_max = max(my_max, ray_nodes);
@@ -634,8 +642,9 @@ void launch_extend(const PathLaunch& pl, uint32_t bounce, uint32_t n0, const uin
                            dim3(trace_grid(5, pl.qs.trace_rounds)), dim3(kTraceThreads), 0, s, pl.sv, pl.ps, pl.qs,
                            pl.ctr, bounce, n0, queue, pl.gstack, pl.extend_refill);
     else
-        hipLaunchKernelGGL(k_extend, dim3(trace_grid(5, pl.qs.trace_rounds)), dim3(kTraceThreads), 0, s, pl.sv,
-                           pl.ps, pl.qs, pl.count_traversal, pl.ctr, bounce, n0, queue, pl.gstack);
+        hipLaunchKernelGGL(pl.count_traversal ? k_extend<true> : k_extend<false>,
+                           dim3(trace_grid(5, pl.qs.trace_rounds)), dim3(kTraceThreads), 0, s, pl.sv, pl.ps, pl.qs,
+                           pl.ctr, bounce, n0, queue, pl.gstack);
 }
 void launch_shade(const PathLaunch& pl, uint32_t npx, uint32_t bounce, uint32_t* next_queue, hipStream_t s) {
     hipLaunchKernelGGL(k_shade, dim3(kShadeBlocks), dim3(kShadeThreads), 0, s, pl.sv, pl.ps, pl.grid,
