@@ -35,6 +35,8 @@ __device__ __forceinline__ bool batch_stopping(const BatchArgs& a, uint32_t take
 __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
     __shared__ uint2 lds_stack[kShortStack * kBatchThreads];
     __shared__ uint32_t first[kMaxBatchJobs + 1];
+    __shared__ TriShare share_all[kBatchThreads / 64];
+    TriShare& sh = share_all[threadIdx.x >> 6];
     for (uint32_t j = threadIdx.x; j <= a.njobs; j += blockDim.x)
         first[j] = j < a.njobs ? a.jobs[j].first_item : a.n_items;
     __syncthreads();
@@ -77,10 +79,11 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
             x = tx * 8u + ((uint32_t)lane & 7u);
             y = ty * 8u + ((uint32_t)lane >> 3);
         }
-        if (x >= J.width || y >= J.height) continue;
-        const uint32_t pix = y * J.width + x;
+        // lanes past the image's edge only help the others' traversals
+        const bool valid = x < J.width && y < J.height;
+        const uint32_t pix = valid ? y * J.width + x : 0u;
         const SceneView& sv = J.sv;
-        float4 acc = J.accum[pix];
+        float4 acc = valid ? J.accum[pix] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         const uint32_t f1 = a.one_frame ? min(a.frame + a.frame_span, J.frames) : J.frames;
         for (uint32_t f = a.one_frame ? a.frame : 0u; f < f1; ++f) {
             const FrameConst& fc = a.table[J.table_base + f];
@@ -90,11 +93,14 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchArgs a) {
             // the (f - frame)-th of the window's
             const FrameConst* fs = (fc.enable_photons && J.grid.count) ? &fc : nullptr;
             const uint32_t cell_base = a.one_frame ? (f - a.frame) * fc.hash_size : 0u;
-            trace_bounces(sv, J.grid, fs, cell_base, fc.rr_after, fc.quirks, 0u, a.max_depth, r, st, false, tally);
+            trace_bounces_wave(sv, J.grid, fs, cell_base, fc.rr_after, fc.quirks, 0u, a.max_depth, r, st, sh, valid,
+                               tally);
             acc = accumulate_sample(acc, fc, r.wl, r.L);
         }
-        J.accum[pix] = acc;
-        reinterpret_cast<uchar4*>(J.rgba)[pix] = tone_map(acc);
+        if (valid) {
+            J.accum[pix] = acc;
+            reinterpret_cast<uchar4*>(J.rgba)[pix] = tone_map(acc);
+        }
     }
 }
 

@@ -207,6 +207,38 @@ __device__ __forceinline__ void trace_bounces(const SceneView& sv, const GridVie
     }
 }
 
+// trace_bounces for a wave whose lanes each carry a path (`live` marks the
+// lanes with one), the closest hits traversed by the whole wave with their
+// triangle tests shared (trace_wave); the shadow rays stay per lane. Every
+// lane of the wave calls it, in wave-uniform control flow; each path's
+// result is trace_bounces', bit for bit. Not counting.
+__device__ __forceinline__ void trace_bounces_wave(const SceneView& sv, const GridView& grid, const FrameConst* fs,
+                                                   uint32_t cell_base, uint32_t rr_after, uint32_t quirks,
+                                                   uint32_t first, uint32_t max_depth, PathRegs& r, Stack st,
+                                                   TriShare& sh, bool live, RayTally& tally) {
+    for (uint32_t b = first; b < max_depth; ++b) {
+        if (!__ballot(live)) break;
+        uint32_t nn = 0, nt = 0;
+        const Hit h = trace_wave<false>(sv, xyz(r.o4), xyz(r.d4), 0.0001f, 100.0f, live, st, sh, nn, nt);
+        if (!live) continue;
+        tally.extend_rays++;
+        if (h.tri == kInvalid) {
+            if (sv.env) r.L = add4(r.L, mul4(r.tp, env_radiance(sv, xyz(r.d4), r.wl)));
+            live = false;
+            continue;
+        }
+        const VertexOut vo =
+            shade_vertex(sv, grid, fs, cell_base, rr_after, quirks, b, false, h.t, h.tri, h.front, h.u, h.v, r);
+        if (vo.shadow) {
+            const Hit s = trace_ray<true>(sv, vo.hp, vo.sdir, 0.001f, vo.stmax, st, false, tally.shadow_nodes,
+                                          tally.shadow_tris);
+            tally.shadow_rays++;
+            if (s.tri == kInvalid) r.L = add4(r.L, vo.contrib);
+        }
+        if (!vo.alive) live = false;
+    }
+}
+
 // One sample into the running accumulation (shader.wgsl:771-808): hero
 // wavelengths to XYZ over the sum of their pdfs, to linear sRGB, added as
 // (rgb, 1); frame 0 restarts the accumulation.
