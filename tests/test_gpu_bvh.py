@@ -73,6 +73,37 @@ def test_closest_and_any_hit_match_brute_force(device, cornell_pair, scene):
     np.testing.assert_array_equal(g.any_hit(rays), o.any_hit(rays, brute=True))
 
 
+@pytest.mark.parametrize("scene", ["cornell", "caustic"])
+def test_closest_hit_any_interval_matches_brute_force(device, cornell_pair, scene):
+    """Closest hits on intervals the render kernels never use: tmin of 0 or
+    0.25, tmax equal to tmin, tmax of 0, negative, infinite or NaN. The
+    wave-shared triangle tests start each ray's search key from tmax, so
+    these must agree with brute force too. (tmin stays >= 0: the padded box
+    test assumes distances ahead of the origin, as the reference's ray
+    queries do.)"""
+    g, o = cornell_pair if scene == "cornell" else caustic_pair(device)
+    lo, hi = (-0.95, 0.95) if scene == "cornell" else (-1.5, 2.0)
+    rays = random_rays(40000, 4, lo, hi)
+    rng = np.random.default_rng(5)
+    n = len(rays)
+    rays[:, 3] = rng.choice(np.array([0.0, 1e-4, 1e-3, 0.25], np.float32), n)
+    tmax = rng.uniform(0.05, 3.0, n).astype(np.float32)
+    pick = rng.integers(0, 8, n)
+    tmax[pick == 0] = 0.0
+    tmax[pick == 1] = -0.5
+    tmax[pick == 2] = np.inf
+    tmax[pick == 3] = np.nan
+    tmax[pick == 4] = rays[pick == 4, 3]  # empty interval (tmin, tmin)
+    rays[:, 7] = tmax
+    gh, gf = g.closest_hit(rays)
+    oh, of = o.closest_hit(rays, brute=True)
+    np.testing.assert_array_equal(gf, of)
+    np.testing.assert_array_equal(gh, oh)
+    hit = (gf & 1) != 0
+    assert hit.mean() > 0.05 and hit[pick == 2].any()
+    assert not hit[pick <= 1].any() and not hit[pick == 3].any() and not hit[pick == 4].any()
+
+
 def test_camera_rays_match_brute_force(device, cornell_pair):
     # rays from the camera position: the reference's first-bounce workload
     g, o = cornell_pair

@@ -361,12 +361,14 @@ __device__ __forceinline__ bool trav_step(const SceneView& sv, TravState& s, Sta
 //    an exclusive prefix of the per-lane counts (one ballot per count bit).
 //  * Per chunk of 64 positions, lanes write (owner lane, offset) entries to
 //    LDS; lane j tests entry j on its owner's ray (fetched by lane shuffles)
-//    and folds a hit into the owner's key, (t bits << 32 | triangle id), with
-//    an LDS atomic minimum; the lane whose key won writes u, v and the facing.
-//  * The owner's initial key is its closest hit so far, or, with none, the
-//    largest key whose t is below tmax; since t > tmin >= 0, keys order as
-//    (t, id), so the final key is the closest hit with ties to the lower id,
-//    the result of the lane's own tests in any order, bit for bit.
+//    and folds a hit into the owner's key, (ordered t bits << 32 | triangle
+//    id), with an LDS atomic minimum; the lane whose key won writes t, u, v
+//    and the facing.
+//  * Keys order as (t, id), -0 and +0 alike (ordered_bits). The owner's
+//    initial key is its closest hit so far, or, with none, the largest key
+//    whose t is below tmax, so the final key is the closest hit with ties to
+//    the lower id: the result of the lane's own tests in any order, bit for
+//    bit.
 // Any-hit rays (shadow) keep trav_step: there the lane's leaf-by-leaf loop
 // ends at its first hit, and testing all of a node's triangles instead made
 // the occluded room's shadow kernel 7 % slower.
@@ -374,10 +376,17 @@ __device__ __forceinline__ bool trav_step(const SceneView& sv, TravState& s, Sta
 // no tests help the others.
 struct TriShare {
     unsigned long long key[64];
-    float u[64], v[64];
+    float t[64], u[64], v[64];
     uint32_t front[64];
     uint16_t item[64];  // one chunk: owner lane | (offset from its tri_base << 6)
 };
+
+// A float's bits mapped so that their unsigned order is the float order
+// (negative values flipped, the others offset), -0 taken as +0.
+__device__ __forceinline__ uint32_t ordered_bits(float x) {
+    const uint32_t b = __float_as_uint(x + 0.0f);
+    return b ^ ((uint32_t)((int32_t)b >> 31) | 0x80000000u);
+}
 
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -405,10 +414,10 @@ __device__ __forceinline__ void wave_test_tris(const SceneView& sv, TravState& s
     const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);  // the wave's tests
     unsigned long long key0 = 0;
     if (cnt) {
-        // (with no hit and tmax not above 0, NaN included, nothing is below
-        // key 0, as no t > tmin >= 0 is below such a tmax)
-        key0 = s.best.tri != kInvalid ? ((unsigned long long)__float_as_uint(s.best.t) << 32) | s.best.tri
-               : s.best.t > 0.0f      ? ((unsigned long long)(__float_as_uint(s.best.t) - 1u) << 32) | 0xFFFFFFFFull
+        // (with no hit and a NaN tmax nothing is below key 0, as no t is
+        // below a NaN)
+        key0 = s.best.tri != kInvalid ? ((unsigned long long)ordered_bits(s.best.t) << 32) | s.best.tri
+               : s.best.t == s.best.t ? ((unsigned long long)(ordered_bits(s.best.t) - 1u) << 32) | 0xFFFFFFFFull
                                       : 0ull;
         sh.key[lane] = key0;
     }
@@ -436,12 +445,13 @@ __device__ __forceinline__ void wave_test_tris(const SceneView& sv, TravState& s
                          "v"(cc.y), "v"(cc.z));  // one round trip for the three loads, as in test_tri
             if (tri_intersect(o, d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(cc.x, cc.y, cc.z), tt, u, v, fr) &&
                 tt > tmin) {
-                key = ((unsigned long long)__float_as_uint(tt) << 32) | __float_as_uint(a.w);
+                key = ((unsigned long long)ordered_bits(tt) << 32) | __float_as_uint(a.w);
                 atomicMin(&sh.key[ow], key);
             }
         }
         wave_sync_lds();
         if (key != ~0ull && sh.key[ow] == key) {
+            sh.t[ow] = tt;
             sh.u[ow] = u;
             sh.v[ow] = v;
             sh.front[ow] = fr ? 1u : 0u;
@@ -451,7 +461,7 @@ __device__ __forceinline__ void wave_test_tris(const SceneView& sv, TravState& s
     if (cnt) {
         const unsigned long long k = sh.key[lane];
         if (k < key0) {
-            s.best.t = __uint_as_float((uint32_t)(k >> 32));
+            s.best.t = sh.t[lane];
             s.best.tri = (uint32_t)k;
             s.best.u = sh.u[lane];
             s.best.v = sh.v[lane];
