@@ -442,7 +442,8 @@ int wsrt_photon_grid_build_noalloc(wsrt_device* dev, const wsrt_photon* photons,
  * hits = n x 4 words: t (float bits, +inf on miss), triangle id (0xFFFFFFFF on
  * miss), u (float), v (float) ; flags[n]: bit0 hit, bit1 front face.
  * The triangle id indexes the scene's world triangles (instance order, then
- * mesh order). */
+ * mesh order). tmin must be >= 0: the traversal's padded box test assumes
+ * distances ahead of the origin, and hits behind it (t < 0) can be missed. */
 int wsrt_debug_closest_hit(const wsrt_scene* s, const float* rays, uint32_t n, uint32_t* hits,
                            uint32_t* flags);
 /* Any-hit query (terminate on first hit): occluded[n] = 0/1. */
