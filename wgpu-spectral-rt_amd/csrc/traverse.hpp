@@ -358,7 +358,7 @@ __device__ __forceinline__ bool trav_step(const SceneView& sv, TravState& s, Sta
 // otherwise runs 3.4 (caustic scene) to 4.5 (occluded room) test rounds per
 // step where one would hold them all.
 //  * Every lane's tests get consecutive positions in a wave-wide list, from
-//    an exclusive prefix of the per-lane counts (one ballot per count bit).
+//    an exclusive prefix sum of the per-lane counts (a DPP scan).
 //  * Per chunk of 64 positions, lanes write (owner lane, offset) entries to
 //    LDS; lane j tests entry j on its owner's ray (fetched by lane shuffles)
 //    and folds a hit into the owner's key, (ordered t bits << 32 | triangle
